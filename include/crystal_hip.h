@@ -1,0 +1,139 @@
+/*
+ * crystal_hip.h -- C ABI of the MI355X-native JIT query backend.
+ *
+ * Drop-in boundary for the reference's JIT path.  Every entry point returns
+ * 0 on success and a negative code on failure (no C++ exception crosses this
+ * ABI); cq_last_error() returns the message of the calling thread's last
+ * failure.  A C++ ICompiler / IExecutor adapter re-throws, so QueryEngine's
+ * contract (exceptions out of executeQuery) is unchanged.
+ *
+ * Interfaces replaced (reference file:line):
+ *   cq_compile / cq_module_release
+ *       ICompiler::compile(source, query_hash) -> lib path
+ *       (include/core/execution.h:291-295; AdaptiveCppCompiler,
+ *       src/core/execution.cpp:114-151)
+ *   cq_launch / cq_module_kernel_count
+ *       IExecutor::execute(lib_path, ctx) + extern "C" execute_query(ctx)
+ *       (include/core/execution.h:297-301; DynamicLibraryExecutor,
+ *       src/core/execution.cpp:161-192; generated ABI, visitor.cpp:4804)
+ *   cq_db_create / cq_db_destroy / cq_db_generate_ssb / cq_db_create_table /
+ *   cq_db_load_column / cq_db_set_primary_key / cq_db_finalize
+ *       DatabaseInstance ctor, initCatalog, loadData
+ *       (src/db/database_instance.cpp:21-174)
+ *   cq_db_execute + cq_result_*
+ *       DatabaseInstance::executeQuery -> QueryResult
+ *       (src/db/database_instance.cpp:180-439; include/core/execution.h:56-96)
+ *   cq_db_explain / cq_db_generate_code
+ *       DatabaseInstance::generateQueryCode (database_instance.cpp:441-445)
+ *   cq_db_column_stats
+ *       CatalogCacheManager::getStatsForColumn (include/core/catalog_cache.h:76-123)
+ *   cq_batch_execute
+ *       no reference counterpart: the batched shared-scan form of many
+ *       queries over resident columns (callers: serving / batch schedulers)
+ *   cq_server_*
+ *       no reference counterpart: the continuous-batching serving loop
+ *       (callers: request front ends)
+ */
+#ifndef CRYSTAL_HIP_API_H
+#define CRYSTAL_HIP_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CQ_OK 0
+#define CQ_ERR_INVALID (-1)
+#define CQ_ERR_COMPILE (-2)
+#define CQ_ERR_RUNTIME (-3)
+#define CQ_ERR_UNSUPPORTED (-4)
+#define CQ_ERR_MEMORY (-5)
+
+typedef struct cq_module cq_module;
+typedef struct cq_db cq_db;
+typedef struct cq_result cq_result;
+typedef struct cq_server cq_server;
+struct cq_query_params; /* practice_amd/csrc/device/query_params.h */
+
+const char* cq_last_error(void);
+const char* cq_version(void);
+
+/* ---- compile / launch seam ---------------------------------------- */
+/* hipRTC-compile HIP source for `arch` ("gfx950"); the device primitive
+ * library crystal_hip.h is available as an include. */
+int cq_compile(const char* source, size_t length, const char* arch, cq_module** out);
+/* Launch kernel `name` of a module with a parameter block on a HIP stream
+ * (NULL = default stream).  grid/block in work-groups / work-items. */
+int cq_launch(cq_module* module, const char* name, const struct cq_query_params* params, unsigned grid, unsigned block,
+              void* stream);
+size_t cq_module_code_size(const cq_module* module);
+void cq_module_release(cq_module* module);
+
+/* ---- database ------------------------------------------------------ */
+/* device < 0 creates a host-only database: catalog, planning, code
+ * generation and hipRTC compilation work; execution does not. */
+int cq_db_create(int device, cq_db** out);
+void cq_db_destroy(cq_db* db);
+int cq_db_generate_ssb(cq_db* db, int sf, long long fact_rows, unsigned long long seed, const char* nullable);
+int cq_db_create_table(cq_db* db, const char* table, long long rows, int is_fact);
+int cq_db_load_column(cq_db* db, const char* table, const char* column, const int32_t* data, const uint64_t* validity);
+int cq_db_set_primary_key(cq_db* db, const char* table, const char* column);
+int cq_db_finalize(cq_db* db);
+int cq_db_column_stats(cq_db* db, const char* column, long long* min_value, long long* max_value,
+                       unsigned long long* cardinality, int* nullable, int* unique);
+int cq_db_fetch_column(cq_db* db, const char* column, int32_t* out, uint64_t* validity_out);
+long long cq_db_table_rows(cq_db* db, const char* table);
+/* option names: "kernel_timing", "memory_guard", "lds_budget", "block_threads",
+ * "vectors_per_thread", "lds_tables", "lds_aggregation", "compact_tables",
+ * "memory_guard_reserve_bytes" */
+int cq_db_set_option(cq_db* db, const char* name, long long value);
+long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","compiles" */
+int cq_db_clear_cache(cq_db* db);
+
+int cq_db_execute(cq_db* db, const char* sql, cq_result** out);
+/* scan only fact rows [row_begin, row_end) */
+int cq_db_execute_range(cq_db* db, const char* sql, long long row_begin, long long row_end, cq_result** out);
+/* Copies a NUL-terminated string into buf (truncating); *needed gets the full size. */
+int cq_db_explain(cq_db* db, const char* sql, char* buf, size_t len, size_t* needed);
+int cq_db_generate_code(cq_db* db, const char* sql, char* buf, size_t len, size_t* needed);
+/* JSON description of the query program (layout, signature, LDS / table placement) */
+int cq_db_describe(cq_db* db, const char* sql, char* buf, size_t len, size_t* needed);
+int cq_db_compile(cq_db* db, const char* sql);
+
+/* ---- results ------------------------------------------------------- */
+long long cq_result_rows(const cq_result* r);
+int cq_result_cols(const cq_result* r);
+int cq_result_type(const cq_result* r, int col); /* 0 int64, 1 uint64, 2 float64 */
+int cq_result_nullable(const cq_result* r, int col);
+int cq_result_ordered(const cq_result* r);
+const unsigned long long* cq_result_column(const cq_result* r, int col);
+const unsigned char* cq_result_validity(const cq_result* r, int col);
+/* "gpu_execute_ms","scan_kernel_ms","compile_ms","codegen_ms","parse_ms",
+ * "host_fetch_ms","engine_ms","cache_hit" */
+double cq_result_timing(const cq_result* r, const char* name);
+void cq_result_free(cq_result* r);
+
+/* ---- batched shared scan ------------------------------------------- */
+/* Executes n queries over the resident columns in one shared pass of the
+ * fact table (queries the shared-scan kernel cannot express run on the
+ * per-query JIT path); outs[i] receives query i's result.  options: JSON
+ * text or NULL ({"graph":1,"autotune":1,"block_threads":..}). */
+int cq_batch_execute(cq_db* db, const char* const* sqls, int n, const char* options, cq_result** outs);
+/* JSON of the last batch (launch configuration, timings, per-query path). */
+int cq_batch_last_report(cq_db* db, char* buf, size_t len, size_t* needed);
+
+/* ---- serving -------------------------------------------------------- */
+int cq_server_start(cq_db* db, const char* options, cq_server** out);
+/* Enqueue a request; returns a ticket >= 0. */
+long long cq_server_submit(cq_server* s, const char* sql);
+/* Wait for a ticket (timeout_ms < 0: forever); *out owns the result. */
+int cq_server_wait(cq_server* s, long long ticket, int timeout_ms, cq_result** out);
+int cq_server_stats(cq_server* s, char* buf, size_t len, size_t* needed);
+void cq_server_stop(cq_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

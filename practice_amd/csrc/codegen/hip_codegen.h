@@ -1,0 +1,121 @@
+// HIP code generator for the JIT query path.
+//
+// Replaces the reference JITOperatorVisitor (src/core/visitor.cpp,
+// include/core/visitor.h:200-356), which walks the optimized plan in
+// produce/consume order and emits SYCL text.  This visitor walks the same plan
+// types and emits one hipRTC module per query shape:
+//   cq_build_<j>   one dimension hash-table build per join
+//                  (emitPHTBuildKernel, visitor.cpp:4206-4290)
+//   cq_scan        the fused fact kernel: 128-bit column loads, filters,
+//                  probes, perfect-hash GROUP BY with LDS-privatized u64
+//                  aggregates (produceTableScan + consume*, 1657-1751,
+//                  1823-1893, 4556-4731, 2987-3196)
+//   cq_dense_count present && HAVING flags per group
+//                  (emitAggregateDenseColumnarMaterialization, 2589-2697)
+// Predicate constants are kernel arguments, so the emitted source is the plan
+// signature: queries that differ only in constants share one compiled module.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../plan/operators.h"
+
+namespace db {
+
+struct CodegenOptions {
+    int block_threads = 0;            // 0: chosen from the LDS footprint
+    int vectors_per_thread = 2;       // 128-bit loads per column per lane per tile
+    uint64_t lds_budget = 96 * 1024;  // bytes of LDS a scan block may use
+    bool allow_lds_tables = true;     // stage compact hash tables into LDS
+    bool allow_lds_aggregation = true;
+    bool allow_compact_tables = true;
+};
+
+struct JoinProgram {
+    std::string dim_table, pk_col, fk_col, payload_col;
+    int64_t dim_rows = 0;
+    int pk_dslot = -1, payload_dslot = -1;
+    bool pair = false;        // PHT_2 (carries a payload) vs PHT_1
+    int64_t kmin = 0, len = 0;
+    bool compact = false;     // compact tagged layout vs reference int layout
+    int eb = 0, tb = 0, jlo = 0;
+    int64_t pmin = 0;
+    bool lds = false;
+    uint64_t ht_words = 0;    // u32 words of the table
+};
+
+struct OutputColumn {
+    LogicalType type = LogicalType::UInt64;
+    bool nullable = false;
+    std::string name;
+};
+
+struct QueryProgram {
+    std::string source;
+    std::string signature;    // hex hash of `source`
+    std::string plan_sexpr;
+
+    std::string fact_table;
+    int64_t fact_rows = 0;
+    std::vector<std::string> fact_cols;  // scan slot -> column
+    std::vector<bool> fact_nullable;
+    std::vector<std::string> dim_cols;   // build slot -> column
+    std::vector<bool> dim_nullable;
+    std::vector<JoinProgram> joins;
+    std::vector<int64_t> lits;
+    std::vector<double> flits;
+
+    // aggregate layout (operators.h storage/visible tuples)
+    int ngroups = 0, naggs = 0, visible_ts = 0, storage_ts = 0;
+    int64_t groups = 1;
+    std::vector<std::string> agg_funcs;
+    std::vector<int> hidden_slot;        // per aggregate: AVG count slot or -1
+    std::vector<int> min_slots;          // storage slots initialised to ~0
+    bool grouped = false;
+    bool fast_sparse = false;            // aggregateFastSparseOutputEligible
+    bool lds_aggregation = false;
+    bool has_avg = false;
+    bool has_having = false;
+    std::vector<OutputColumn> out;
+
+    bool has_sort_node = false;
+    std::vector<SortKeyDef> sort_keys;
+    bool has_limit = false;
+    uint64_t limit = 0;
+
+    int block_threads = 256;
+    int vectors_per_thread = 2;
+    uint64_t scan_lds_bytes = 0;
+
+    // algorithmic bytes of one scan (fact columns + bitmaps + dimension columns)
+    uint64_t algorithmic_bytes = 0;
+};
+
+// Visitor entry points as in the reference: AggregateNode / SortLimitNode
+// roots start generation (visitor.cpp:1589-1621); projections are rejected.
+class HipOperatorVisitor : public OperatorVisitor {
+public:
+    HipOperatorVisitor(const Catalog& catalog, CodegenOptions options = {});
+    void visit(const TableScanNode& node) override;
+    void visit(const FilterNode& node) override;
+    void visit(const HashJoinNode& node) override;
+    void visit(const AggregateNode& node) override;
+    void visit(const ProjectionNode& node) override;
+    void visit(const SortLimitNode& node) override;
+    QueryProgram takeProgram();
+
+private:
+    const Catalog& catalog_;
+    CodegenOptions options_;
+    QueryProgram program_;
+    bool done_ = false;
+};
+
+QueryProgram generateQueryProgram(const OperatorNode* root, const Catalog& catalog, const CodegenOptions& options = {});
+
+// 64-bit FNV-1a of a string, hex encoded (module cache key).
+std::string signatureOf(const std::string& text);
+
+}  // namespace db

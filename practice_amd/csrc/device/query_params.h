@@ -1,0 +1,44 @@
+// Kernel argument block shared by the host runtime and every generated or
+// ahead-of-time kernel.  Plain C layout: the host fills it and passes it by
+// value (kernarg segment, < 4 KB), the device reads it.
+//
+// Replaces the reference's ExecutionContext lookups inside the generated
+// execute_query() (ctx->getBuffer<int>("d_<col>"), getNullBitmap, getResultPointer,
+// getScratchIntBuffer; include/core/execution.h:163-212): the code generator
+// assigns each column a slot and the executor binds device pointers to slots.
+#ifndef CQ_QUERY_PARAMS_H
+#define CQ_QUERY_PARAMS_H
+
+#define CQ_MAX_FACT_COLS 16
+#define CQ_MAX_DIM_COLS 24
+#define CQ_MAX_JOINS 8
+#define CQ_MAX_LITS 48
+#define CQ_MAX_FLITS 8
+
+struct cq_query_params {
+    // fact (probe-side) columns by slot, int32
+    const int* col[CQ_MAX_FACT_COLS];
+    // LSB-first validity bitmaps of the fact columns viewed as 32-bit words
+    // (bit r of the u64 bitmap is bit r%32 of word r/32); null = no bitmap
+    const unsigned int* valid[CQ_MAX_FACT_COLS];
+    // dimension (build-side) columns by slot
+    const int* dcol[CQ_MAX_DIM_COLS];
+    const unsigned int* dvalid[CQ_MAX_DIM_COLS];
+    // hash tables, one per join (layout chosen by the code generator)
+    unsigned int* ht[CQ_MAX_JOINS];
+    long long dim_rows[CQ_MAX_JOINS];
+    // aggregate state: u64 slots [group * storage_tuple + slot] and a bitmap
+    // with one validity bit per slot (reference d_result / d_result_validity)
+    unsigned long long* result;
+    unsigned long long* result_valid;
+    // predicate constants (runtime parameters, so one compiled module serves
+    // every query of the same plan signature)
+    long long lit[CQ_MAX_LITS];
+    double flit[CQ_MAX_FLITS];
+    // fact rows [row_begin, row_end) scanned by this launch; row_begin % 4 == 0
+    long long row_begin;
+    long long row_end;
+    long long num_tiles;
+};
+
+#endif

@@ -1,0 +1,61 @@
+// Host launchers of the ahead-of-time kernels (aot_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+enum {
+    CQ_GEN_KEY = 0, CQ_GEN_UNIFORM, CQ_GEN_DATE_FK, CQ_GEN_PCAT, CQ_GEN_PMFGR, CQ_GEN_PBRAND, CQ_GEN_NATION,
+    CQ_GEN_REGION, CQ_GEN_CITY, CQ_GEN_DKEY, CQ_GEN_DYEAR, CQ_GEN_DMONTH, CQ_GEN_DYM, CQ_GEN_DDOW, CQ_GEN_DDOM,
+    CQ_GEN_DDOY, CQ_GEN_DWEEK
+};
+
+// One generated column: value kind + stream seeds (see practice_amd/csrc/runtime/generator.cpp)
+struct cq_gen_recipe {
+    int kind;
+    int null_bp;            // NULL rate in basis points, 0 = not nullable
+    unsigned long long s0, s1, vseed;
+    long long lo, hi;
+};
+
+enum { CQ_AGG_SUM = 0, CQ_AGG_COUNT = 1, CQ_AGG_MIN = 2, CQ_AGG_MAX = 3, CQ_AGG_AVG = 4 };
+enum { CQ_TYPE_INT64 = 0, CQ_TYPE_UINT64 = 1, CQ_TYPE_FLOAT64 = 2 };
+
+struct cq_finalize_desc {
+    int ngroups, naggs, sts, vts;
+    int kind[32];
+    int hidden[32];
+};
+
+struct cq_sort_desc {
+    int nkeys;
+    int col[16];
+    int type[16];
+    int nullable[16];
+    int desc[16];
+};
+
+hipError_t cq_launch_generate(int* out, unsigned long long* valid, const cq_gen_recipe* rc, long long start, long long count,
+                              hipStream_t s);
+hipError_t cq_launch_minmax(const int* col, long long n, unsigned long long* mm, hipStream_t s);
+hipError_t cq_launch_distinct(const int* col, long long n, long long mn, unsigned int* bits, long long words,
+                              unsigned long long* out, hipStream_t s);
+hipError_t cq_launch_fill_u64(unsigned long long* p, unsigned long long v, long long n, hipStream_t s);
+hipError_t cq_launch_fill_u32(unsigned int* p, unsigned int v, long long n, hipStream_t s);
+hipError_t cq_launch_init_min(unsigned long long* res, long long groups, int sts, unsigned int mask, hipStream_t s);
+hipError_t cq_launch_finalize(const unsigned long long* res, const unsigned long long* rv, long long groups,
+                              const cq_finalize_desc* d, unsigned long long* vres, unsigned long long* vvalid, hipStream_t s);
+hipError_t cq_launch_sparse_count(const unsigned long long* res, long long groups, int vts, unsigned long long* flags,
+                                  hipStream_t s);
+long long cq_scan_block_count(long long n);
+hipError_t cq_launch_exclusive_scan(const unsigned long long* in, unsigned long long* out, long long n,
+                                    unsigned long long* block_sums, unsigned long long* grand, hipStream_t s);
+hipError_t cq_launch_write_rows(const unsigned long long* res, const unsigned long long* rv, long long groups, int vts,
+                                const unsigned long long* flags, const unsigned long long* offsets, unsigned int nullable_mask,
+                                int use_validity, unsigned long long* out, unsigned long long* out_valid, long long cap_rows,
+                                long long valid_words, hipStream_t s);
+long long cq_sort_padded(long long rows);
+hipError_t cq_launch_sort(unsigned long long* idx, long long rows, const cq_sort_desc* d, const unsigned long long* cols,
+                          const unsigned long long* valid, long long cap_rows, long long valid_words, hipStream_t s);
+hipError_t cq_launch_gather(const unsigned long long* cols, const unsigned long long* valid, const unsigned long long* idx,
+                            long long rows, int vts, long long cap_rows, long long valid_words, unsigned long long* out,
+                            unsigned long long* out_valid, hipStream_t s);

@@ -1,0 +1,514 @@
+// Query engine runtime implementation.
+#include "execution.h"
+
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+#include "../frontend/sql.h"
+#include "../kernels/aot_kernels.h"
+
+extern const char* const kCrystalHipHeaderSource;
+extern const char* const kQueryParamsHeaderSource;
+
+namespace db {
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ DeviceBuffer
+
+DeviceBuffer::~DeviceBuffer() { release(); }
+
+void DeviceBuffer::release() {
+    if (ptr_) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(ptr_);
+    }
+    ptr_ = nullptr;
+    bytes_ = 0;
+}
+
+void DeviceBuffer::ensure(size_t bytes) {
+    if (bytes <= bytes_ && ptr_) return;
+    size_t want = bytes;
+    if (bytes < (8ull << 20)) want = std::max(bytes, bytes_ + bytes_ / 2);
+    if (want == 0) want = 256;
+    release();
+    hipCheck(hipMalloc(&ptr_, want), "hipMalloc scratch");
+    bytes_ = want;
+}
+
+// ------------------------------------------------------------------ CompiledModule
+
+CompiledModule::CompiledModule(std::string signature, std::vector<char> code)
+    : signature_(std::move(signature)), code_(std::move(code)) {}
+
+CompiledModule::~CompiledModule() {
+    if (module_) (void)hipModuleUnload(module_);
+}
+
+hipFunction_t CompiledModule::function(const std::string& name) {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (!module_) hipCheck(hipModuleLoadData(&module_, code_.data()), "hipModuleLoadData");
+    auto it = functions_.find(name);
+    if (it != functions_.end()) return it->second;
+    hipFunction_t f = nullptr;
+    hipCheck(hipModuleGetFunction(&f, module_, name.c_str()), ("hipModuleGetFunction " + name).c_str());
+    functions_[name] = f;
+    return f;
+}
+
+// ------------------------------------------------------------------ HipRtcCompiler
+
+HipRtcCompiler::HipRtcCompiler(std::string arch, std::vector<std::string> extra)
+    : arch_(std::move(arch)), options_(std::move(extra)) {}
+
+ModuleHandle HipRtcCompiler::compile(const std::string& source, const std::string& query_hash) {
+    hiprtcProgram prog = nullptr;
+    const char* headers[] = {kCrystalHipHeaderSource, kQueryParamsHeaderSource};
+    const char* names[] = {"crystal_hip.h", "query_params.h"};
+    const std::string file = "q_" + query_hash + ".hip";
+    if (hiprtcCreateProgram(&prog, source.c_str(), file.c_str(), 2, headers, names) != HIPRTC_SUCCESS)
+        throw std::runtime_error("hiprtcCreateProgram failed");
+    std::vector<std::string> opts = {"--offload-arch=" + arch_, "-O3", "-std=c++17", "-fwrapv",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    for (const auto& o : options_) opts.push_back(o);
+    std::vector<const char*> copts;
+    for (const auto& o : opts) copts.push_back(o.c_str());
+    const hiprtcResult rc = hiprtcCompileProgram(prog, static_cast<int>(copts.size()), copts.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    std::string log(log_size, '\0');
+    if (log_size) hiprtcGetProgramLog(prog, &log[0]);
+    if (rc != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        throw std::runtime_error("hipRTC compilation failed for " + file + ": " + log);
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    std::vector<char> code(code_size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    ++compiles_;
+    return std::make_shared<CompiledModule>(query_hash, std::move(code));
+}
+
+// ------------------------------------------------------------------ PlanSignatureCache
+
+ModuleHandle PlanSignatureCache::get(const std::string& key) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = index_.find(key);
+    if (it == index_.end()) {
+        ++misses_;
+        return nullptr;
+    }
+    lru_.splice(lru_.begin(), lru_, it->second);
+    ++hits_;
+    return it->second->second;
+}
+
+void PlanSignatureCache::put(const std::string& key, ModuleHandle module) {
+    std::lock_guard<std::mutex> lock(mu_);
+    auto it = index_.find(key);
+    if (it != index_.end()) {
+        it->second->second = std::move(module);
+        lru_.splice(lru_.begin(), lru_, it->second);
+        return;
+    }
+    lru_.emplace_front(key, std::move(module));
+    index_[key] = lru_.begin();
+    while (lru_.size() > capacity_) {
+        index_.erase(lru_.back().first);
+        lru_.pop_back();
+    }
+}
+
+size_t PlanSignatureCache::size() const {
+    std::lock_guard<std::mutex> lock(mu_);
+    return lru_.size();
+}
+
+void PlanSignatureCache::clear() {
+    std::lock_guard<std::mutex> lock(mu_);
+    lru_.clear();
+    index_.clear();
+}
+
+// ------------------------------------------------------------------ memory guard
+
+size_t estimateQueryTemporaryBytes(const QueryProgram& p) {
+    size_t bytes = 0;
+    for (const auto& j : p.joins) bytes += j.ht_words * 4;
+    const size_t slots = static_cast<size_t>(p.groups) * static_cast<size_t>(p.storage_ts);
+    bytes += slots * 8 + ((slots + 63) / 64) * 8;                              // aggregate state
+    const size_t vslots = static_cast<size_t>(p.groups) * static_cast<size_t>(p.visible_ts);
+    if (p.has_avg) bytes += vslots * 8 + ((vslots + 63) / 64) * 8;            // visible copy
+    const size_t rows = static_cast<size_t>(p.groups);
+    const size_t words = (rows + 63) / 64;
+    bytes += 2 * (static_cast<size_t>(p.visible_ts) * (rows * 8 + words * 8));  // output + sort buffers
+    bytes += 3 * rows * 8;                                                      // flags, offsets, indices
+    return bytes;
+}
+
+void preflightDeviceMemoryOrThrow(const ExecutionContext& ctx, const QueryProgram& p) {
+    if (!ctx.config.memory_guard_enabled) return;
+    size_t free_b = 0, total = 0;
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess || total == 0) return;
+    const size_t temp = estimateQueryTemporaryBytes(p);
+    const size_t existing = ctx.scratchBytes();
+    const size_t required = ctx.loaded_device_bytes + std::max(existing, temp);
+    const size_t reserve = std::max(static_cast<size_t>(static_cast<double>(total) * ctx.config.memory_guard_reserve_fraction),
+                                    ctx.config.memory_guard_reserve_bytes);
+    const size_t budget = total > reserve ? total - reserve : total / 2;
+    if (required > budget) {
+        std::ostringstream m;
+        m << "Insufficient GPU memory for query before JIT execution. Estimated resident requirement: " << required
+          << " bytes (loaded columns: " << ctx.loaded_device_bytes << ", query temporaries: " << temp
+          << "). Device memory: " << total << ", safety budget: " << budget << ".";
+        throw std::runtime_error(m.str());
+    }
+}
+
+// ------------------------------------------------------------------ HipModuleExecutor
+
+namespace {
+
+int deviceCUs(int device) {
+    static int cached[64] = {0};
+    if (device >= 0 && device < 64 && cached[device]) return cached[device];
+    hipDeviceProp_t prop;
+    hipCheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    if (device >= 0 && device < 64) cached[device] = prop.multiProcessorCount;
+    return prop.multiProcessorCount;
+}
+
+void launch(hipFunction_t f, unsigned grid, unsigned block, hipStream_t s, void** args, const char* what) {
+    hipCheck(hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, s, args, nullptr), what);
+}
+
+int aggKind(const std::string& f) {
+    if (f == "COUNT") return CQ_AGG_COUNT;
+    if (f == "MIN") return CQ_AGG_MIN;
+    if (f == "MAX") return CQ_AGG_MAX;
+    if (f == "AVG") return CQ_AGG_AVG;
+    return CQ_AGG_SUM;
+}
+
+int typeCode(LogicalType t) {
+    if (t == LogicalType::Int64) return CQ_TYPE_INT64;
+    if (t == LogicalType::Float64) return CQ_TYPE_FLOAT64;
+    return CQ_TYPE_UINT64;
+}
+
+}  // namespace
+
+void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ctx) {
+    const QueryProgram& p = ctx->program;
+    hipStream_t s = ctx->stream;
+    hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!ctx->columns) throw std::runtime_error("ExecutionContext has no column registry");
+
+    cq_query_params P;
+    std::memset(&P, 0, sizeof P);
+    auto column = [&](const std::string& name) -> const DeviceColumn& {
+        auto it = ctx->columns->find(name);
+        if (it == ctx->columns->end()) throw std::runtime_error("column not loaded on the device: " + name);
+        return it->second;
+    };
+    for (size_t i = 0; i < p.fact_cols.size(); ++i) {
+        const DeviceColumn& c = column(p.fact_cols[i]);
+        P.col[i] = c.data;
+        P.valid[i] = reinterpret_cast<const unsigned int*>(c.validity);
+        if (p.fact_nullable[i] && !c.validity) throw std::runtime_error("nullable column without bitmap: " + p.fact_cols[i]);
+    }
+    for (size_t i = 0; i < p.dim_cols.size(); ++i) {
+        const DeviceColumn& c = column(p.dim_cols[i]);
+        P.dcol[i] = c.data;
+        P.dvalid[i] = reinterpret_cast<const unsigned int*>(c.validity);
+    }
+    for (size_t i = 0; i < p.lits.size(); ++i) P.lit[i] = p.lits[i];
+    for (size_t i = 0; i < p.flits.size(); ++i) P.flit[i] = p.flits[i];
+
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ks = nullptr, ke = nullptr;
+    hipCheck(hipEventCreate(&ev0), "hipEventCreate");
+    hipCheck(hipEventCreate(&ev1), "hipEventCreate");
+    if (ctx->config.kernel_timing) {
+        hipCheck(hipEventCreate(&ks), "hipEventCreate");
+        hipCheck(hipEventCreate(&ke), "hipEventCreate");
+    }
+    hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
+
+    // hash tables
+    for (size_t j = 0; j < p.joins.size(); ++j) {
+        const JoinProgram& jp = p.joins[j];
+        DeviceBuffer& b = ctx->scratch("ht" + std::to_string(j));
+        b.ensure(std::max<size_t>(jp.ht_words * 4, 16));
+        P.ht[j] = static_cast<unsigned int*>(b.data());
+        P.dim_rows[j] = jp.dim_rows;
+        hipCheck(cq_launch_fill_u32(P.ht[j], 0u, static_cast<long long>(jp.ht_words), s), "fill hash table");
+    }
+    // aggregate state
+    const int64_t G = p.groups;
+    const size_t slots = static_cast<size_t>(G) * static_cast<size_t>(p.storage_ts);
+    const size_t words = (slots + 63) / 64;
+    DeviceBuffer& rb = ctx->scratch("result");
+    DeviceBuffer& vb = ctx->scratch("result_valid");
+    rb.ensure(slots * 8);
+    vb.ensure(words * 8);
+    P.result = static_cast<unsigned long long*>(rb.data());
+    P.result_valid = static_cast<unsigned long long*>(vb.data());
+    hipCheck(cq_launch_fill_u64(P.result, 0ull, static_cast<long long>(slots), s), "fill result");
+    hipCheck(cq_launch_fill_u64(P.result_valid, 0ull, static_cast<long long>(words), s), "fill result validity");
+    unsigned int min_mask = 0;
+    for (int slot : p.min_slots) min_mask |= 1u << slot;
+    hipCheck(cq_launch_init_min(P.result, G, p.storage_ts, min_mask, s), "init MIN slots");
+
+    // builds
+    for (size_t j = 0; j < p.joins.size(); ++j) {
+        hipFunction_t f = module->function("cq_build_" + std::to_string(j));
+        const long long rows = p.joins[j].dim_rows;
+        unsigned grid = static_cast<unsigned>(std::min<long long>(std::max<long long>((rows + 255) / 256, 1), 8192));
+        void* args[] = {&P};
+        launch(f, grid, 256, s, args, "launch build");
+    }
+
+    // scan
+    const int64_t row_end = ctx->row_end < 0 ? p.fact_rows : ctx->row_end;
+    P.row_begin = ctx->row_begin;
+    P.row_end = row_end;
+    const int64_t tile = static_cast<int64_t>(p.block_threads) * 4 * p.vectors_per_thread;
+    P.num_tiles = row_end > P.row_begin ? (row_end - P.row_begin + tile - 1) / tile : 0;
+    if (P.row_begin % 4 != 0) throw std::runtime_error("scan range must start at a multiple of 4 rows");
+    {
+        hipFunction_t f = module->function("cq_scan");
+        int per_cu = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, p.block_threads, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        const int64_t max_blocks = static_cast<int64_t>(per_cu) * deviceCUs(ctx->device);
+        const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(P.num_tiles, max_blocks)));
+        void* args[] = {&P};
+        if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");
+        if (P.num_tiles > 0) launch(f, grid, static_cast<unsigned>(p.block_threads), s, args, "launch scan");
+        if (ke) hipCheck(hipEventRecord(ke, s), "hipEventRecord");
+    }
+
+    // AVG: storage tuple -> visible tuple
+    unsigned long long* res = P.result;
+    unsigned long long* rv = P.result_valid;
+    const int vts = p.visible_ts;
+    if (p.has_avg) {
+        const size_t vslots = static_cast<size_t>(G) * static_cast<size_t>(vts);
+        DeviceBuffer& a = ctx->scratch("vresult");
+        DeviceBuffer& b = ctx->scratch("vresult_valid");
+        a.ensure(vslots * 8);
+        b.ensure(((vslots + 63) / 64) * 8);
+        hipCheck(cq_launch_fill_u64(static_cast<unsigned long long*>(a.data()), 0ull, static_cast<long long>(vslots), s), "fill");
+        hipCheck(cq_launch_fill_u64(static_cast<unsigned long long*>(b.data()), 0ull,
+                                    static_cast<long long>((vslots + 63) / 64), s), "fill");
+        cq_finalize_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.ngroups = p.ngroups;
+        d.naggs = p.naggs;
+        d.sts = p.storage_ts;
+        d.vts = vts;
+        for (int i = 0; i < p.naggs && i < 32; ++i) {
+            d.kind[i] = aggKind(p.agg_funcs[static_cast<size_t>(i)]);
+            d.hidden[i] = p.hidden_slot[static_cast<size_t>(i)];
+        }
+        hipCheck(cq_launch_finalize(res, rv, G, &d, static_cast<unsigned long long*>(a.data()),
+                                    static_cast<unsigned long long*>(b.data()), s), "finalize AVG");
+        res = static_cast<unsigned long long*>(a.data());
+        rv = static_cast<unsigned long long*>(b.data());
+    }
+
+    QueryResult& out = ctx->result;
+    out.columns.clear();
+    for (const auto& oc : p.out) out.columns.push_back(ResultColumnDesc{oc.type, oc.nullable, oc.name});
+    out.data.assign(static_cast<size_t>(vts), {});
+    out.validity.assign(static_cast<size_t>(vts), {});
+
+    const long long cap_rows = std::max<long long>(G, 1);
+    const long long vwords = (cap_rows + 63) / 64;
+    DeviceBuffer& ob = ctx->scratch("out");
+    DeviceBuffer& ovb = ctx->scratch("out_valid");
+    DeviceBuffer& flags = ctx->scratch("flags");
+    DeviceBuffer& offsets = ctx->scratch("offsets");
+    DeviceBuffer& bsums = ctx->scratch("block_sums");
+    ob.ensure(static_cast<size_t>(vts) * cap_rows * 8);
+    ovb.ensure(static_cast<size_t>(vts) * vwords * 8);
+    flags.ensure(cap_rows * 8);
+    offsets.ensure(cap_rows * 8 + 8);
+    bsums.ensure(static_cast<size_t>(cq_scan_block_count(cap_rows) + 1) * 8);
+    auto* o = static_cast<unsigned long long*>(ob.data());
+    auto* ov = static_cast<unsigned long long*>(ovb.data());
+    auto* fl = static_cast<unsigned long long*>(flags.data());
+    auto* off = static_cast<unsigned long long*>(offsets.data());
+    auto* grand = off + cap_rows;
+    hipCheck(cq_launch_fill_u64(ov, 0ull, static_cast<long long>(vts) * vwords, s), "fill out validity");
+
+    long long rows = 0;
+    bool single_cell = false;
+    if (p.fast_sparse && vts <= 1) {
+        // scalar sparse output: one row, the raw slot (SUM over no rows = 0)
+        single_cell = true;
+        rows = 1;
+        out.ordered = true;
+        out.dense_result = false;
+    } else {
+        if (p.fast_sparse) {
+            hipCheck(cq_launch_sparse_count(res, G, vts, fl, s), "sparse count");
+            out.ordered = false;
+            out.dense_result = false;
+        } else {
+            hipFunction_t f = module->function("cq_dense_count");
+            cq_query_params Q = P;
+            Q.result = res;
+            Q.result_valid = rv;
+            long long g = G;
+            void* args[] = {&Q, &fl, &g};
+            launch(f, static_cast<unsigned>((G + 255) / 256), 256, s, args, "launch dense count");
+            out.ordered = true;
+            out.dense_result = true;
+        }
+        hipCheck(cq_launch_exclusive_scan(fl, off, G, static_cast<unsigned long long*>(bsums.data()), grand, s), "scan");
+        unsigned int nmask = 0;
+        for (int c = 0; c < vts; ++c)
+            if (p.out[static_cast<size_t>(c)].nullable) nmask |= 1u << c;
+        hipCheck(cq_launch_write_rows(res, rv, G, vts, fl, off, nmask, p.fast_sparse ? 0 : 1, o, ov, cap_rows, vwords, s),
+                 "write rows");
+        unsigned long long total = 0;
+        hipCheck(hipMemcpyAsync(&total, grand, 8, hipMemcpyDeviceToHost, s), "copy row count");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        rows = static_cast<long long>(total);
+        if (p.has_sort_node && !p.sort_keys.empty() && rows > 1) {
+            DeviceBuffer& ib = ctx->scratch("sort_idx");
+            DeviceBuffer& o2 = ctx->scratch("out2");
+            DeviceBuffer& ov2 = ctx->scratch("out2_valid");
+            ib.ensure(static_cast<size_t>(cq_sort_padded(rows)) * 8);
+            o2.ensure(static_cast<size_t>(vts) * cap_rows * 8);
+            ov2.ensure(static_cast<size_t>(vts) * vwords * 8);
+            cq_sort_desc d;
+            std::memset(&d, 0, sizeof d);
+            d.nkeys = static_cast<int>(std::min<size_t>(p.sort_keys.size(), 16));
+            for (int k = 0; k < d.nkeys; ++k) {
+                const int c = static_cast<int>(p.sort_keys[static_cast<size_t>(k)].column_index);
+                d.col[k] = c;
+                d.type[k] = typeCode(p.out[static_cast<size_t>(c)].type);
+                d.nullable[k] = p.out[static_cast<size_t>(c)].nullable ? 1 : 0;
+                d.desc[k] = p.sort_keys[static_cast<size_t>(k)].descending ? 1 : 0;
+            }
+            auto* idx = static_cast<unsigned long long*>(ib.data());
+            hipCheck(cq_launch_sort(idx, rows, &d, o, ov, cap_rows, vwords, s), "sort");
+            auto* o2p = static_cast<unsigned long long*>(o2.data());
+            auto* ov2p = static_cast<unsigned long long*>(ov2.data());
+            hipCheck(cq_launch_fill_u64(ov2p, 0ull, static_cast<long long>(vts) * vwords, s), "fill");
+            hipCheck(cq_launch_gather(o, ov, idx, rows, vts, cap_rows, vwords, o2p, ov2p, s), "gather");
+            o = o2p;
+            ov = ov2p;
+        }
+        if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
+    }
+    hipCheck(hipEventRecord(ev1, s), "hipEventRecord");
+
+    // fetch
+    const auto f0 = std::chrono::steady_clock::now();
+    out.row_count = static_cast<size_t>(rows);
+    if (single_cell) {
+        out.data[0].resize(1);
+        hipCheck(hipMemcpyAsync(out.data[0].data(), res, 8, hipMemcpyDeviceToHost, s), "fetch");
+        out.validity[0].assign(1, 1);
+    } else {
+        std::vector<uint64_t> vbits(static_cast<size_t>(vts) * vwords);
+        for (int c = 0; c < vts; ++c) {
+            out.data[static_cast<size_t>(c)].resize(static_cast<size_t>(rows));
+            if (rows)
+                hipCheck(hipMemcpyAsync(out.data[static_cast<size_t>(c)].data(), o + static_cast<size_t>(c) * cap_rows, rows * 8,
+                                        hipMemcpyDeviceToHost, s), "fetch column");
+        }
+        if (!vbits.empty())
+            hipCheck(hipMemcpyAsync(vbits.data(), ov, vbits.size() * 8, hipMemcpyDeviceToHost, s), "fetch validity");
+        hipCheck(hipStreamSynchronize(s), "sync");
+        for (int c = 0; c < vts; ++c) {
+            auto& vv = out.validity[static_cast<size_t>(c)];
+            vv.resize(static_cast<size_t>(rows));
+            for (long long r = 0; r < rows; ++r)
+                vv[static_cast<size_t>(r)] = static_cast<uint8_t>((vbits[static_cast<size_t>(c) * vwords + (r >> 6)] >> (r & 63)) & 1ull);
+        }
+    }
+    hipCheck(hipStreamSynchronize(s), "sync");
+    const auto f1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    ctx->timing.gpu_execute_ms = ms;
+    if (ks && ke) {
+        float kms = 0;
+        (void)hipEventElapsedTime(&kms, ks, ke);
+        ctx->timing.scan_kernel_ms = kms;
+    }
+    ctx->timing.host_fetch_ms = elapsedMs(f0, f1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (ks) (void)hipEventDestroy(ks);
+    if (ke) (void)hipEventDestroy(ke);
+}
+
+// ------------------------------------------------------------------ QueryEngine
+
+QueryEngine::QueryEngine(std::shared_ptr<IQueryCache> cache, std::shared_ptr<ICompiler> compiler,
+                         std::shared_ptr<IExecutor> executor, std::shared_ptr<const Catalog> catalog)
+    : cache_(std::move(cache)), compiler_(std::move(compiler)), executor_(std::move(executor)), catalog_(std::move(catalog)) {}
+
+QueryProgram QueryEngine::prepare(const std::string& sql, const CodegenOptions& options) const {
+    auto tree = buildOptimizedTree(sql, catalog_.get());
+    return generateQueryProgram(tree.get(), *catalog_, options);
+}
+
+ModuleHandle QueryEngine::moduleFor(const QueryProgram& program, QueryTimingStats* timing) {
+    ModuleHandle m = cache_->get(program.signature);
+    if (m) {
+        if (timing) timing->cache_hit = true;
+        return m;
+    }
+    std::lock_guard<std::mutex> lock(compile_mu_);
+    m = cache_->get(program.signature);
+    if (m) {
+        if (timing) timing->cache_hit = true;
+        return m;
+    }
+    const auto c0 = std::chrono::steady_clock::now();
+    m = compiler_->compile(program.source, program.signature);
+    if (timing) timing->compile_ms += elapsedMs(c0, std::chrono::steady_clock::now());
+    cache_->put(program.signature, m);
+    return m;
+}
+
+void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t row_begin = ctx->row_begin, row_end = ctx->row_end;
+    ctx->resetForQuery();
+    ctx->row_begin = row_begin;
+    ctx->row_end = row_end;
+    auto tree = buildOptimizedTree(sql, catalog_.get());
+    const auto t1 = std::chrono::steady_clock::now();
+    ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
+    const auto t2 = std::chrono::steady_clock::now();
+    QueryTimingStats timing;
+    timing.parse_ms = elapsedMs(t0, t1);
+    timing.codegen_ms = elapsedMs(t1, t2);
+    preflightDeviceMemoryOrThrow(*ctx, ctx->program);
+    ModuleHandle m = moduleFor(ctx->program, &timing);
+    executor_->execute(m, ctx);
+    timing.gpu_execute_ms = ctx->timing.gpu_execute_ms;
+    timing.scan_kernel_ms = ctx->timing.scan_kernel_ms;
+    timing.host_fetch_ms = ctx->timing.host_fetch_ms;
+    timing.engine_ms = elapsedMs(t0, std::chrono::steady_clock::now());
+    ctx->timing = timing;
+    ctx->result.timing = timing;
+}
+
+}  // namespace db

@@ -1,0 +1,240 @@
+// Query engine runtime: the compile / execute seam of the JIT path.
+//
+// Reference interfaces replaced (include/core/execution.h:284-346):
+//   ICompiler::compile(source, hash) -> path of an acpp-built .so
+//       -> HipRtcCompiler: hiprtcCompileProgram for gfx950, code object kept
+//          in process and loaded with hipModuleLoadData (returns a handle)
+//   IExecutor::execute(path, ctx) -> dlopen + execute_query(ctx)
+//       -> HipModuleExecutor: binds columns / scratch to cq_query_params and
+//          launches build, scan and materialization kernels on ctx's stream
+//   IQueryCache keyed by std::hash(SQL text)
+//       -> PlanSignatureCache keyed by the generated source (constants are
+//          kernel arguments), thread-safe, LRU-bounded
+//   ExecutionContext (execution.h:131-282) -> column registry, grow-only
+//          named scratch (DynamicDeviceBuffer, device_buffer.h:12-118), the
+//          query program and the host result
+//   QueryEngine::executeQuery (execution.cpp:652-725) incl. the memory guard
+//          preflightDeviceMemoryOrThrow / estimateJitTemporaryBytes (298-404)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../codegen/hip_codegen.h"
+#include "../device/query_params.h"
+#include "../plan/memory.h"
+
+namespace db {
+
+void hipCheck(hipError_t e, const char* what);
+
+// Grow-only device allocation.  Grows by 1.5x below 1 Mi elements and exactly
+// above (device_buffer.h:59-75); frees after synchronizing its stream.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    void ensure(size_t bytes);
+    void release();
+    void* data() const { return ptr_; }
+    size_t capacity() const { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+struct QueryTimingStats {
+    double parse_ms = 0, codegen_ms = 0, compile_ms = 0, library_load_ms = 0;
+    double gpu_execute_ms = 0;   // device work of the query (events on the stream)
+    double scan_kernel_ms = 0;   // cq_scan alone, when kernel timing is enabled
+    double host_fetch_ms = 0, engine_ms = 0;
+    bool cache_hit = false;
+};
+
+struct ResultColumnDesc {
+    LogicalType type = LogicalType::UInt64;
+    bool nullable = false;
+    std::string name;
+};
+
+// Host result: columnar 64-bit cells (int64 / uint64 / double bits) + per-row
+// validity (1 = valid).  `ordered` is false for the sparse output path whose
+// row order the reference leaves unspecified (database_instance.cpp:299).
+struct QueryResult {
+    std::vector<ResultColumnDesc> columns;
+    size_t row_count = 0;
+    bool ordered = true;
+    bool dense_result = true;
+    std::vector<std::vector<uint64_t>> data;      // [column][row]
+    std::vector<std::vector<uint8_t>> validity;   // [column][row]
+    QueryTimingStats timing;
+};
+
+struct EngineConfig {
+    bool memory_guard_enabled = true;
+    size_t memory_guard_reserve_bytes = 512ull << 20;
+    double memory_guard_reserve_fraction = 0.10;
+    bool kernel_timing = false;          // record cq_scan duration with events
+    CodegenOptions codegen;
+    size_t module_cache_capacity = 256;  // compiled modules kept (LRU)
+};
+
+struct DeviceColumn {
+    const int* data = nullptr;
+    const uint64_t* validity = nullptr;  // LSB-first u64 words, null = all valid
+    int64_t rows = 0;
+};
+
+class CompiledModule {
+public:
+    CompiledModule(std::string signature, std::vector<char> code);
+    ~CompiledModule();
+    const std::string& signature() const { return signature_; }
+    size_t codeSize() const { return code_.size(); }
+    const std::vector<char>& code() const { return code_; }
+    // Loads the module on the current device on first use (thread-safe).
+    hipFunction_t function(const std::string& name);
+
+private:
+    std::string signature_;
+    std::vector<char> code_;
+    std::mutex mu_;
+    hipModule_t module_ = nullptr;
+    std::map<std::string, hipFunction_t> functions_;
+};
+using ModuleHandle = std::shared_ptr<CompiledModule>;
+
+struct ExecutionContext {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    EngineConfig config;
+    const std::unordered_map<std::string, DeviceColumn>* columns = nullptr;
+    size_t loaded_device_bytes = 0;
+
+    // per query
+    QueryProgram program;
+    QueryResult result;
+    QueryTimingStats timing;
+    int64_t row_begin = 0, row_end = -1;  // fact rows to scan (-1: all)
+
+    DeviceBuffer& scratch(const std::string& name) { return scratch_[name]; }
+    size_t scratchBytes() const {
+        size_t b = 0;
+        for (const auto& kv : scratch_) b += kv.second.capacity();
+        return b;
+    }
+    void resetForQuery() {
+        result = QueryResult{};
+        timing = QueryTimingStats{};
+        row_begin = 0;
+        row_end = -1;
+    }
+
+private:
+    std::map<std::string, DeviceBuffer> scratch_;
+};
+
+class ICompiler {
+public:
+    virtual ~ICompiler() = default;
+    virtual ModuleHandle compile(const std::string& source_code, const std::string& query_hash) = 0;
+};
+
+class IExecutor {
+public:
+    virtual ~IExecutor() = default;
+    virtual void execute(const ModuleHandle& module, ExecutionContext* ctx) = 0;
+};
+
+class IQueryCache {
+public:
+    virtual ~IQueryCache() = default;
+    virtual ModuleHandle get(const std::string& key) = 0;
+    virtual void put(const std::string& key, ModuleHandle module) = 0;
+    virtual size_t size() const = 0;
+    virtual void clear() = 0;
+};
+
+// hipRTC for --offload-arch=gfx950 with the device primitive library as an
+// in-memory header.  Throws std::runtime_error with the compiler log on failure.
+class HipRtcCompiler : public ICompiler {
+public:
+    explicit HipRtcCompiler(std::string arch = "gfx950", std::vector<std::string> extra_options = {});
+    ModuleHandle compile(const std::string& source_code, const std::string& query_hash) override;
+    size_t compileCount() const { return compiles_; }
+
+private:
+    std::string arch_;
+    std::vector<std::string> options_;
+    size_t compiles_ = 0;
+};
+
+// Thread-safe LRU map from plan signature to compiled module.
+class PlanSignatureCache : public IQueryCache {
+public:
+    explicit PlanSignatureCache(size_t capacity = 256) : capacity_(capacity) {}
+    ModuleHandle get(const std::string& key) override;
+    void put(const std::string& key, ModuleHandle module) override;
+    size_t size() const override;
+    void clear() override;
+    size_t hits() const { return hits_; }
+    size_t misses() const { return misses_; }
+
+private:
+    size_t capacity_;
+    mutable std::mutex mu_;
+    std::list<std::pair<std::string, ModuleHandle>> lru_;
+    std::unordered_map<std::string, std::list<std::pair<std::string, ModuleHandle>>::iterator> index_;
+    size_t hits_ = 0, misses_ = 0;
+};
+
+class HipModuleExecutor : public IExecutor {
+public:
+    void execute(const ModuleHandle& module, ExecutionContext* ctx) override;
+};
+
+// Device bytes a query needs beyond the resident columns (hash tables,
+// aggregate slots, materialization buffers).
+size_t estimateQueryTemporaryBytes(const QueryProgram& program);
+
+class QueryEngine {
+public:
+    QueryEngine(std::shared_ptr<IQueryCache> cache, std::shared_ptr<ICompiler> compiler, std::shared_ptr<IExecutor> executor,
+                std::shared_ptr<const Catalog> catalog);
+    // SQL -> optimized plan -> program -> (cached) module -> execution
+    void executeQuery(const std::string& sql, ExecutionContext* ctx);
+    // plan -> program without executing (codegen text / signature / layout)
+    QueryProgram prepare(const std::string& sql, const CodegenOptions& options) const;
+    // compile or fetch the module for a program
+    ModuleHandle moduleFor(const QueryProgram& program, QueryTimingStats* timing);
+    void setCatalog(std::shared_ptr<const Catalog> catalog) { catalog_ = std::move(catalog); }
+    IQueryCache& cache() { return *cache_; }
+
+private:
+    std::shared_ptr<IQueryCache> cache_;
+    std::shared_ptr<ICompiler> compiler_;
+    std::shared_ptr<IExecutor> executor_;
+    std::shared_ptr<const Catalog> catalog_;
+    std::mutex compile_mu_;
+};
+
+// Throws when the device cannot hold the query (preflightDeviceMemoryOrThrow).
+void preflightDeviceMemoryOrThrow(const ExecutionContext& ctx, const QueryProgram& program);
+
+inline double elapsedMs(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+}  // namespace db

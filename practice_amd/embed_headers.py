@@ -1,0 +1,31 @@
+"""Build step: embed the device primitive headers into the runtime library.
+
+hipRTC compiles every generated module against crystal_hip.h and
+query_params.h; the runtime passes them as in-memory headers, so the built
+library carries their text.  Writes csrc/_gen/device_headers.cpp.
+"""
+import pathlib
+import sys
+
+HERE = pathlib.Path(__file__).resolve().parent
+DELIM = "CQHDR"
+
+
+def main(out_path: str) -> None:
+    dev = HERE / "csrc" / "device"
+    parts = ["// generated by practice_amd/embed_headers.py -- do not edit\n"]
+    for symbol, name in (("kCrystalHipHeaderSource", "crystal_hip.h"), ("kQueryParamsHeaderSource", "query_params.h")):
+        text = (dev / name).read_text()
+        if f"){DELIM}\"" in text:
+            raise SystemExit(f"delimiter collision in {name}")
+        parts.append(f"extern const char* const {symbol};\n")
+        parts.append(f"const char* const {symbol} = R\"{DELIM}(\n{text}){DELIM}\";\n")
+    out = pathlib.Path(out_path)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    new = "".join(parts)
+    if not out.exists() or out.read_text() != new:
+        out.write_text(new)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1])
