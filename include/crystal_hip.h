@@ -81,6 +81,13 @@ int cq_db_create_table(cq_db* db, const char* table, long long rows, int is_fact
 int cq_db_load_column(cq_db* db, const char* table, const char* column, const int32_t* data, const uint64_t* validity);
 int cq_db_set_primary_key(cq_db* db, const char* table, const char* column);
 int cq_db_finalize(cq_db* db);
+/* Catalog-only column and explicit statistics (planning without data). */
+int cq_db_declare_column(cq_db* db, const char* table, const char* column, int nullable);
+int cq_db_set_column_stats(cq_db* db, const char* column, long long min_value, long long max_value,
+                           unsigned long long cardinality);
+/* AggregateNode::calculateResultSize of the optimized plan
+ * (include/core/operators.h:331-369); -1 on error. */
+long long cq_db_result_size(cq_db* db, const char* sql);
 int cq_db_column_stats(cq_db* db, const char* column, long long* min_value, long long* max_value,
                        unsigned long long* cardinality, int* nullable, int* unique);
 int cq_db_fetch_column(cq_db* db, const char* column, int32_t* out, uint64_t* validity_out);

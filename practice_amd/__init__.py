@@ -51,6 +51,9 @@ def _load() -> ctypes.CDLL:
         "cq_db_load_column": (c_int, [c_void_p, c_char_p, c_char_p, c_void_p, c_void_p]),
         "cq_db_set_primary_key": (c_int, [c_void_p, c_char_p, c_char_p]),
         "cq_db_finalize": (c_int, [c_void_p]),
+        "cq_db_declare_column": (c_int, [c_void_p, c_char_p, c_char_p, c_int]),
+        "cq_db_set_column_stats": (c_int, [c_void_p, c_char_p, c_ll, c_ll, c_ull]),
+        "cq_db_result_size": (c_ll, [c_void_p, c_char_p]),
         "cq_db_column_stats": (c_int, [c_void_p, c_char_p, P(c_ll), P(c_ll), P(c_ull), P(c_int), P(c_int)]),
         "cq_db_fetch_column": (c_int, [c_void_p, c_char_p, c_void_p, c_void_p]),
         "cq_db_table_rows": (c_ll, [c_void_p, c_char_p]),
@@ -205,6 +208,18 @@ class Database:
 
     def finalize(self) -> None:
         _check(_lib.cq_db_finalize(self._h))
+
+    def declare_column(self, table: str, column: str, nullable: bool = False) -> None:
+        _check(_lib.cq_db_declare_column(self._h, table.encode(), column.encode(), int(nullable)))
+
+    def set_column_stats(self, column: str, min_value: int, max_value: int, cardinality: int) -> None:
+        _check(_lib.cq_db_set_column_stats(self._h, column.encode(), min_value, max_value, cardinality))
+
+    def result_size(self, sql: str) -> int:
+        n = _lib.cq_db_result_size(self._h, sql.encode())
+        if n < 0:
+            raise CrystalError(-1, _lib.cq_last_error().decode())
+        return n
 
     def column_stats(self, column: str) -> dict:
         mn, mx, card = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_ulonglong()

@@ -166,6 +166,29 @@ int cq_db_finalize(cq_db* d) {
     });
 }
 
+int cq_db_declare_column(cq_db* d, const char* table, const char* column, int nullable) {
+    return guarded([&] {
+        d->inst->declareColumn(table, column, nullable != 0);
+        return CQ_OK;
+    });
+}
+
+int cq_db_set_column_stats(cq_db* d, const char* column, long long mn, long long mx, unsigned long long card) {
+    return guarded([&] {
+        d->inst->setColumnStats(column, mn, mx, card);
+        return CQ_OK;
+    });
+}
+
+long long cq_db_result_size(cq_db* d, const char* sql) {
+    try {
+        return static_cast<long long>(d->inst->resultSize(sql));
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
 int cq_db_column_stats(cq_db* d, const char* column, long long* mn, long long* mx, unsigned long long* card, int* nullable,
                        int* unique) {
     return guarded([&] {

@@ -241,6 +241,38 @@ void DatabaseInstance::setPrimaryKey(const std::string& table, const std::string
     catalog_->getTableMetadataMutable(table).setColumnPrimaryKey(column, true);
 }
 
+void DatabaseInstance::declareColumn(const std::string& table, const std::string& column, bool nullable) {
+    std::lock_guard<std::mutex> lock(mu_);
+    ensureTable(table);
+    auto& meta = catalog_->getTableMetadataMutable(table);
+    std::vector<std::string> names = meta.getColumnNames();
+    names.push_back(column);
+    TableMetadata updated(meta.getName(), names, meta.getSize(), meta.isFactTable());
+    for (const auto& n : meta.getColumnNames())
+        if (meta.hasColumnStats(n)) updated.setColumnStats(n, meta.getColumnStats(n));
+    if (nullable) updated.setColumnNullable(column, true);
+    meta = updated;
+}
+
+void DatabaseInstance::setColumnStats(const std::string& column, int64_t mn, int64_t mx, uint64_t card) {
+    std::lock_guard<std::mutex> lock(mu_);
+    const std::string t = catalog_->tableOfColumn(column);
+    ensureTable(t);
+    ColumnStatistics st;
+    st.min_value_ = mn;
+    st.max_value_ = mx;
+    st.cardinality_ = card;
+    catalog_->getTableMetadataMutable(t).setColumnStats(column, st);
+}
+
+uint64_t DatabaseInstance::resultSize(const std::string& sql) const {
+    auto tree = buildOptimizedTree(sql, catalog_.get());
+    const OperatorNode* n = tree.get();
+    while (n && n->getType() != OperatorType::AGGREGATE)
+        n = n->getChildren().empty() ? nullptr : n->getChildren()[0].get();
+    return n ? static_cast<const AggregateNode*>(n)->calculateResultSize(*catalog_) : 0;
+}
+
 void DatabaseInstance::generateSSB(const GenSpec& spec) {
     if (device_ < 0) throw std::runtime_error("generateSSB needs a GPU device");
     for (const auto& t : genTables()) createTable(t, genTableRows(spec, t), t == "LINEORDER");

@@ -44,6 +44,12 @@ public:
     void createTable(const std::string& name, int64_t rows, bool is_fact);
     void loadColumn(const std::string& table, const std::string& column, const int32_t* data, const uint64_t* validity);
     void setPrimaryKey(const std::string& table, const std::string& column);
+    // Catalog-only column (no data): planning and code generation against
+    // declared statistics, as the reference's catalog cache provides them.
+    void declareColumn(const std::string& table, const std::string& column, bool nullable);
+    void setColumnStats(const std::string& column, int64_t min_value, int64_t max_value, uint64_t cardinality);
+    // AggregateNode::calculateResultSize of the optimized plan (0 without an aggregate)
+    uint64_t resultSize(const std::string& sql) const;
     // Generates every table / column of the synthetic schema in device memory.
     void generateSSB(const GenSpec& spec);
     // Statistics for every column (min, max, distinct count, nullability).

@@ -1,0 +1,88 @@
+"""The C ABI library: loads, exports every declared entry point, reports errors
+as codes + messages (no exception crosses the boundary).  CPU only."""
+import ctypes
+import pathlib
+import re
+
+import pytest
+
+import practice_amd as pa
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "crystal_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cq_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(pa.LIB_PATH))
+    names = declared_symbols()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_no_error():
+    assert b"gfx950" in pa.lib().cq_version()
+
+
+def test_cq_compile_reports_compiler_log():
+    with pytest.raises(pa.CrystalError) as e:
+        pa.compile_source('extern "C" __global__ void k(int* p) { p[threadIdx.x] = undefined_symbol; }')
+    assert e.value.code == -2 and "undefined_symbol" in str(e.value)
+
+
+def test_cq_compile_with_device_library():
+    src = ('#include "crystal_hip.h"\n'
+           'extern "C" __global__ void k(const cq_query_params P) {\n'
+           '  int pay = 0; if (chip::ht_probe_compact(P.ht[0], P.col[0][threadIdx.x], 0, 16, 8, 0, 0, 0, pay))\n'
+           '    atomicAdd(P.result, (unsigned long long)pay); }\n')
+    assert pa.compile_source(src) > 0
+
+
+def test_host_only_database_cannot_execute():
+    db = pa.Database(-1)
+    db.create_table("T", 4, True)
+    db.load_column("T", "t_x", [1, 2, 3, 4])
+    db.finalize()
+    with pytest.raises(pa.CrystalError, match="GPU"):
+        db.execute("select sum(t_x) from t")
+    with pytest.raises(pa.CrystalError, match="GPU"):
+        db.generate_ssb(sf=1)
+
+
+def test_errors_are_codes_not_exceptions():
+    lib = pa.lib()
+    h = ctypes.c_void_p()
+    assert lib.cq_db_create(-1, ctypes.byref(h)) == 0
+    assert lib.cq_db_create_table(h, b"T", 4, 1) == 0
+    rc = lib.cq_db_create_table(h, b"T", 4, 1)
+    assert rc == -3 and b"already exists" in lib.cq_last_error()
+    assert lib.cq_db_set_option(h, b"no_such_option", 1) == -1
+    assert lib.cq_db_table_rows(h, b"MISSING") == -1
+    lib.cq_db_destroy(h)
+
+
+def test_unknown_column_and_unsupported_plan_codes():
+    db = pa.Database(-1)
+    db.create_table("T", 4, True)
+    db.load_column("T", "t_x", [1, 2, 3, 4])
+    db.finalize()
+    with pytest.raises(pa.CrystalError) as e:
+        db.generate_code("select t_x from t")
+    assert e.value.code == -4
+    with pytest.raises(pa.CrystalError):
+        db.generate_code("select sum(t_y) from t")
+
+
+def test_stats_of_loaded_columns():
+    import numpy as np
+    db = pa.Database(-1)
+    db.create_table("T", 6, True)
+    db.load_column("T", "t_x", np.array([5, -2, 5, 9, 9, 0], np.int32),
+                   np.array([0b101111], np.uint64))
+    db.finalize()
+    assert db.column_stats("t_x") == {"min": -2, "max": 9, "card": 4, "nullable": True, "unique": False}
