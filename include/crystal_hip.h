@@ -123,13 +123,26 @@ double cq_result_timing(const cq_result* r, const char* name);
 void cq_result_free(cq_result* r);
 
 /* ---- batched shared scan ------------------------------------------- */
-/* Executes n queries over the resident columns in one shared pass of the
- * fact table (queries the shared-scan kernel cannot express run on the
- * per-query JIT path); outs[i] receives query i's result.  options: JSON
- * text or NULL ({"graph":1,"autotune":1,"block_threads":..}). */
+/* Executes n queries over the resident columns; outs[i] receives query i's
+ * result, equal to what cq_db_execute returns for it alone.  Queries are
+ * grouped by plan signature and every group of up to 16 signatures over
+ * one fact table shares ONE persistent scan launch that reads each fact
+ * column once; the device work of a batch is captured in a HIP graph and
+ * replayed for batches of the same shape.  No reference counterpart: the
+ * reference runs QueryEngine::executeQuery per query
+ * (src/core/execution.cpp:652-725).
+ * options: NULL or "key=value,..." with bt (64..1024, multiple of 64),
+ * vpt (1..4 128-bit loads per column per lane), graph (0/1), autotune (0/1:
+ * time every bt x vpt once per signature set and keep the fastest),
+ * max_templates (<= 16), max_queries (per scan). */
 int cq_batch_execute(cq_db* db, const char* const* sqls, int n, const char* options, cq_result** outs);
-/* JSON of the last batch (launch configuration, timings, per-query path). */
+/* JSON of the last batch (per shared scan: queries, templates, columns,
+ * launch shape, graph replay, scan / device milliseconds, fact bytes). */
 int cq_batch_last_report(cq_db* db, char* buf, size_t len, size_t* needed);
+/* Generated HIP source of a batch (host-only databases too): part < 0 the
+ * shared-scan module of all n queries, part = k query k's template module. */
+int cq_db_generate_batch_code(cq_db* db, const char* const* sqls, int n, const char* options, int part, char* buf, size_t len,
+                              size_t* needed);
 
 /* ---- serving -------------------------------------------------------- */
 int cq_server_start(cq_db* db, const char* options, cq_server** out);

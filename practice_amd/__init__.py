@@ -77,6 +77,7 @@ def _load() -> ctypes.CDLL:
         "cq_result_free": (None, [c_void_p]),
         "cq_batch_execute": (c_int, [c_void_p, P(c_char_p), c_int, c_char_p, P(c_void_p)]),
         "cq_batch_last_report": (c_int, [c_void_p, c_char_p, c_sz, P(c_sz)]),
+        "cq_db_generate_batch_code": (c_int, [c_void_p, P(c_char_p), c_int, c_char_p, c_int, c_char_p, c_sz, P(c_sz)]),
         "cq_server_start": (c_int, [c_void_p, c_char_p, P(c_void_p)]),
         "cq_server_submit": (c_ll, [c_void_p, c_char_p]),
         "cq_server_wait": (c_int, [c_void_p, c_ll, c_int, P(c_void_p)]),
@@ -271,14 +272,26 @@ class Database:
         _check(_lib.cq_db_compile(self._h, sql.encode()))
 
     def batch_execute(self, sqls: Sequence[str], options: Optional[dict] = None) -> List[Result]:
+        """Shared-scan execution of many queries (options: bt, vpt, graph,
+        autotune, max_templates, max_queries)."""
         arr = (ctypes.c_char_p * len(sqls))(*[s.encode() for s in sqls])
         outs = (ctypes.c_void_p * len(sqls))()
-        opts = json.dumps(options).encode() if options else None
-        _check(_lib.cq_batch_execute(self._h, arr, len(sqls), opts, outs))
+        _check(_lib.cq_batch_execute(self._h, arr, len(sqls), _options(options), outs))
         return [_fetch_result(ctypes.c_void_p(outs[i])) for i in range(len(sqls))]
+
+    def generate_batch_code(self, sqls: Sequence[str], options: Optional[dict] = None, part: int = -1) -> str:
+        arr = (ctypes.c_char_p * len(sqls))(*[s.encode() for s in sqls])
+        return _string_out(_lib.cq_db_generate_batch_code, self._h, arr, len(sqls), _options(options), part)
 
     def batch_report(self) -> dict:
         return json.loads(_string_out(_lib.cq_batch_last_report, self._h) or "{}")
+
+
+def _options(options: Optional[dict]) -> Optional[bytes]:
+    """{"bt": 256, "graph": True} -> b"bt=256,graph=1" (None stays None)."""
+    if not options:
+        return None
+    return ",".join(f"{k}={int(v)}" for k, v in options.items()).encode()
 
 
 class Server:
@@ -286,7 +299,7 @@ class Server:
 
     def __init__(self, db: Database, options: Optional[dict] = None):
         h = ctypes.c_void_p()
-        _check(_lib.cq_server_start(db._h, (json.dumps(options) if options else "").encode(), ctypes.byref(h)))
+        _check(_lib.cq_server_start(db._h, _options(options), ctypes.byref(h)))
         self._h = h
         self._db = db
 

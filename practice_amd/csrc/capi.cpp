@@ -374,6 +374,16 @@ int cq_batch_last_report(cq_db* d, char* buf, size_t len, size_t* needed) {
     return guarded([&] { return copyOut(d->last_batch_report, buf, len, needed); });
 }
 
+int cq_db_generate_batch_code(cq_db* d, const char* const* sqls, int n, const char* options, int part, char* buf, size_t len,
+                              size_t* needed) {
+    return guarded([&] {
+        if (!d || (!sqls && n > 0)) return fail(CQ_ERR_INVALID, "cq_db_generate_batch_code: null argument");
+        std::vector<std::string> q;
+        for (int i = 0; i < n; ++i) q.emplace_back(sqls[i]);
+        return copyOut(db::generateBatchCode(*d->inst, q, db::parseBatchOptions(options ? options : ""), part), buf, len, needed);
+    });
+}
+
 int cq_server_start(cq_db* d, const char* options, cq_server** out) {
     return guarded([&] {
         auto s = std::make_unique<cq_server>();

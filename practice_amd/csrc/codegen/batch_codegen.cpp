@@ -1,0 +1,90 @@
+// Source of the batched shared-scan kernel.
+//
+// Shared scan: the fact columns touched by any query of the batch are
+// streamed from HBM once per tile (128-bit nontemporal loads), then every
+// template's pipeline body runs over those registers for each of its
+// queries.  A template is a plan signature; its queries differ only in their
+// parameter blocks (constants, hash tables, result slots), so the module
+// depends on the set of templates, not on the batch size or constants.
+#include <map>
+#include <sstream>
+
+#include "hip_codegen.h"
+
+namespace db {
+
+std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt,
+                                    std::vector<std::string>& union_cols, std::vector<bool>& union_nullable) {
+    if (templates.empty()) throw std::runtime_error("batch without templates");
+    if (templates.size() > 16) throw std::runtime_error("too many query templates in one batch (max 16)");
+    union_cols.clear();
+    union_nullable.clear();
+    std::map<std::string, int> slot;
+    for (const QueryProgram* t : templates) {
+        if (t->batch_body.empty()) throw std::runtime_error("template generated without batch body");
+        for (size_t s = 0; s < t->fact_cols.size(); ++s) {
+            if (slot.count(t->fact_cols[s])) continue;
+            slot[t->fact_cols[s]] = static_cast<int>(union_cols.size());
+            union_cols.push_back(t->fact_cols[s]);
+            union_nullable.push_back(t->fact_nullable[s]);
+        }
+    }
+    if (union_cols.size() > 24) throw std::runtime_error("batch touches more than 24 fact columns");
+    const int ipt = 4 * vpt;
+    std::ostringstream src;
+    src << "// batched shared scan over " << templates.size() << " templates\n"
+        << "#include \"crystal_hip.h\"\n\n"
+        << "#define BT " << bt << "\n#define VPT " << vpt << "\n#define IPT " << ipt << "\n#define TILE (BT * IPT)\n\n"
+        << "extern \"C\" __global__ __launch_bounds__(BT) void cq_batch_scan(const cq_batch_params B) {\n"
+        << "    using namespace chip;\n"
+        << "    const int tid = threadIdx.x;\n"
+        << "    for (long long tile = blockIdx.x; tile < B.num_tiles; tile += gridDim.x) {\n"
+        << "        const long long base = B.row_begin + tile * (long long)TILE;\n";
+    const size_t nu = union_cols.size();
+    for (size_t u = 0; u < nu; ++u) {
+        src << "        int u" << u << "[IPT];\n";
+        if (union_nullable[u]) src << "        u32 uv" << u << " = 0u;\n";
+    }
+    src << "        u32 tile_fl = 0u;\n"
+        << "        if (base + TILE <= B.row_end) {\n"
+        << "#pragma unroll\n            for (int v = 0; v < VPT; ++v) {\n"
+        << "                const long long r = base + (long long)v * (BT * 4) + tid * 4;\n";
+    for (size_t u = 0; u < nu; ++u) {
+        src << "                { const v4i x = __builtin_nontemporal_load((const v4i*)(B.col[" << u << "] + r)); u" << u
+            << "[4 * v] = x.x; u" << u << "[4 * v + 1] = x.y; u" << u << "[4 * v + 2] = x.z; u" << u << "[4 * v + 3] = x.w; }\n";
+        if (union_nullable[u])
+            src << "                uv" << u << " |= ((B.valid[" << u << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
+    }
+    src << "            }\n"
+        << "            tile_fl = " << (ipt == 32 ? "0xffffffffu" : "((1u << IPT) - 1u)") << ";\n"
+        << "        } else {\n"
+        << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+        << "                const long long r = base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
+        << "                const bool in = r < B.row_end;\n"
+        << "                if (in) tile_fl |= 1u << i;\n";
+    for (size_t u = 0; u < nu; ++u) {
+        src << "                u" << u << "[i] = in ? B.col[" << u << "][r] : 0;\n";
+        if (union_nullable[u]) src << "                if (in) uv" << u << " |= chip::bit32(B.valid[" << u << "], r) << i;\n";
+    }
+    src << "            }\n        }\n";
+    for (size_t t = 0; t < templates.size(); ++t) {
+        const QueryProgram* p = templates[t];
+        src << "        // template " << t << ": " << p->signature << "\n"
+            << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
+            << "            const cq_query_params& P = B.qp[q];\n";
+        for (size_t s = 0; s < p->fact_cols.size(); ++s) {
+            const int u = slot[p->fact_cols[s]];
+            src << "            int (&c" << s << ")[IPT] = u" << u << ";\n";
+            if (p->fact_nullable[s]) src << "            const u32 v" << s << " = uv" << u << ";\n";
+        }
+        src << "            u32 fl = tile_fl;\n"
+            << "            do {\n"
+            << p->batch_body
+            << "            } while (0);\n"
+            << "        }\n";
+    }
+    src << "    }\n}\n";
+    return src.str();
+}
+
+}  // namespace db

@@ -799,6 +799,14 @@ private:
     }
 
     // ---------------------------------------------------------------- emission
+    // Plain modules take the parameter block by value (kernarg segment);
+    // modules of the batched path read it from the device array of blocks so
+    // a captured graph stays valid when only constants change.
+    std::string paramDecl() const {
+        return opt_.batch_body ? "const cq_query_params* __restrict__ PP" : "const cq_query_params P";
+    }
+    std::string paramAlias() const { return opt_.batch_body ? "    const cq_query_params& P = *PP;\n" : ""; }
+
     std::string groupHash() {
         const auto* a = agg_;
         std::string h;
@@ -821,7 +829,8 @@ private:
         std::string cond = bp ? buildCond(bp) : "";
         jp.pk_dslot = dimSlot(jp.pk_col);
         if (jp.pair) jp.payload_dslot = dimSlot(jp.payload_col);
-        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_build_" << j << "(const cq_query_params P) {\n"
+        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_build_" << j << "(" << paramDecl() << ") {\n"
+            << paramAlias()
             << "    const long long n = P.dim_rows[" << j << "];\n"
             << "    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < n; r += (long long)gridDim.x * 256) {\n";
         if (!cond.empty()) src << "        if (!" << cond << ") continue;\n";
@@ -849,7 +858,8 @@ private:
         const int nas = sts - ng;  // aggregate storage slots
         const bool grouped = prog_.grouped;
         // LDS aggregation when the domain decodes from the group index
-        bool lds_agg = grouped && opt_.allow_lds_aggregation && sts <= 31;
+        const bool batch = opt_.batch_body;
+        bool lds_agg = grouped && opt_.allow_lds_aggregation && sts <= 31 && !batch;
         for (size_t g = 0; g < gdense_.size(); ++g) lds_agg = lds_agg && gdense_[g];
         const uint64_t agg_lds = grouped ? static_cast<uint64_t>(prog_.groups) * (static_cast<uint64_t>(nas) * 8 + 4) : 0;
         if (agg_lds > opt_.lds_budget) lds_agg = false;
@@ -862,7 +872,7 @@ private:
         for (int j : order) {
             auto& jp = prog_.joins[static_cast<size_t>(j)];
             const uint64_t bytes = jp.ht_words * 4;
-            if (opt_.allow_lds_tables && jp.compact && lds_used + bytes <= opt_.lds_budget) {
+            if (!batch && opt_.allow_lds_tables && jp.compact && lds_used + bytes <= opt_.lds_budget) {
                 jp.lds = true;
                 lds_used += bytes;
             }
@@ -923,8 +933,22 @@ private:
         if (!grouped) emitScalarAggregate(aggcode, prologue, epilogue);
         else if (lds_agg) emitLdsAggregate(aggcode, prologue, epilogue);
         else emitGlobalAggregate(aggcode);
+        if (batch) {
+            std::ostringstream tile;
+            if (!grouped) emitScalarAggregateTile(tile);
+            else tile << aggcode.str();
+            prog_.batch_body = body.str() + tile.str();
+        }
+        for (int i = 0; i < prog_.naggs && !grouped; ++i)
+            if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
 
-        // ---- kernel
+        // ---- kernel (the batched path runs the body inside cq_batch_scan)
+        if (batch) {
+            emitDenseCount(src);
+            prog_.source = src.str();
+            setAlgorithmicBytes();
+            return;
+        }
         src << "#define BT " << bt << "\n#define VPT " << vpt << "\n#define IPT " << ipt << "\n#define TILE (BT * IPT)\n\n";
         src << "extern \"C\" __global__ __launch_bounds__(BT) void cq_scan(const cq_query_params P) {\n";
         src << "    using namespace chip;\n";
@@ -976,8 +1000,12 @@ private:
         emitDenseCount(src);
         prog_.source = src.str();
         (void)a;
+        setAlgorithmicBytes();
+    }
 
-        // algorithmic bytes of one execution
+    // algorithmic bytes of one execution: fact columns + bitmaps + dimension columns
+    void setAlgorithmicBytes() {
+        const size_t nf = prog_.fact_cols.size();
         const uint64_t N = static_cast<uint64_t>(prog_.fact_rows);
         uint64_t bytes = 0;
         for (size_t s = 0; s < nf; ++s) {
@@ -1039,6 +1067,47 @@ private:
             if (f == "COUNT")
                 epi << "    if (blockIdx.x == 0 && tid == 0 && P.row_end > P.row_begin) chip::set_bit64(P.result_valid, " << i << ");\n";
         }
+    }
+
+    // scalar aggregates reduced per tile (batched shared scan)
+    void emitScalarAggregateTile(std::ostringstream& code) {
+        const int na = prog_.naggs;
+        code << "        {\n";
+        for (int i = 0; i < na; ++i) {
+            code << "            u64 acc" << i << " = " << (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN" ? "~0ull" : "0ull")
+                 << "; u32 seen" << i << " = 0u;";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " u64 cnt" << i << " = 0ull;";
+            code << "\n";
+        }
+        code << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+             << "                if (!((fl >> i) & 1u)) continue;\n";
+        for (int i = 0; i < na; ++i) {
+            const AggRow r = aggRow(i);
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            code << "                if (" << r.valid << ") { const u64 x = " << r.value << "; ";
+            if (f == "MIN") code << "acc" << i << " = x < acc" << i << " ? x : acc" << i << ";";
+            else if (f == "MAX") code << "acc" << i << " = x > acc" << i << " ? x : acc" << i << ";";
+            else code << "acc" << i << " += x;";
+            code << " seen" << i << " = 1u;";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " cnt" << i << " += 1ull;";
+            code << " }\n";
+        }
+        code << "            }\n            const int lane = tid & 63;\n";
+        for (int i = 0; i < na; ++i) {
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
+            const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
+            const char* nz = f == "MIN" ? "w != ~0ull" : "w != 0ull";
+            code << "            { const u64 w = " << red << "(acc" << i << "); const bool any = __ballot(seen" << i
+                 << " != 0u) != 0ull;\n              if (lane == 0 && " << nz << ") " << atom << "(&P.result[" << i << "], w);\n";
+            if (f != "COUNT" && prog_.hidden_slot[static_cast<size_t>(i)] < 0)
+                code << "              if (lane == 0 && any) chip::set_bit64(P.result_valid, " << i << ");\n";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
+                code << "              { const u64 c = wave_sum(cnt" << i << "); if (lane == 0 && c) atomicAdd(&P.result["
+                     << prog_.hidden_slot[static_cast<size_t>(i)] << "], c); }\n";
+            code << "            }\n";
+        }
+        code << "        }\n";
     }
 
     // per-row bit mask of storage slots whose validity bit the row sets
@@ -1166,7 +1235,7 @@ private:
     // HAVING over result slots (aggregateTupleExprValue/Valid, visitor.cpp:2277-2450)
     std::string slotValue(int slot) {
         const LogicalType t = prog_.out[static_cast<size_t>(slot)].type;
-        const std::string raw = "P.result[base + " + std::to_string(slot) + "]";
+        const std::string raw = "R[base + " + std::to_string(slot) + "]";
         if (t == LogicalType::Float64) return "chip::bits_to_double(" + raw + ")";
         if (t == LogicalType::Int64) return "(long long)(" + raw + ")";
         return "(u64)(" + raw + ")";
@@ -1218,7 +1287,7 @@ private:
     }
     std::string hvalid(const ExprNode* e) {
         int slot = -1;
-        if (isResultSlotRef(e, slot)) return "chip::bit64(P.result_valid, base + " + std::to_string(slot) + ")";
+        if (isResultSlotRef(e, slot)) return "chip::bit64(RV, base + " + std::to_string(slot) + ")";
         switch (e->getType()) {
             case ExprType::LITERAL_NULL: return "0";
             case ExprType::STAR: case ExprType::LITERAL_INT: case ExprType::LITERAL_FLOAT:
@@ -1252,8 +1321,10 @@ private:
 
     void emitDenseCount(std::ostringstream& src) {
         const int vts = prog_.visible_ts;
-        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count(const cq_query_params P, "
+        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count(" << paramDecl()
+            << ", const unsigned long long* __restrict__ R, const unsigned long long* __restrict__ RV, "
                "unsigned long long* counts, long long G) {\n"
+            << paramAlias()
             << "    using namespace chip;\n"
             << "    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;\n"
             << "    if (g >= G) return;\n"
@@ -1263,7 +1334,7 @@ private:
         } else {
             src << "    bool present = false;\n"
                 << "    for (int c = 0; c < " << vts << " && !present; ++c)\n"
-                << "        present = chip::bit64(P.result_valid, base + c) != 0u || P.result[base + c] != 0ull;\n";
+                << "        present = chip::bit64(RV, base + c) != 0u || R[base + c] != 0ull;\n";
         }
         std::string having = "1";
         if (agg_->having_predicate) {

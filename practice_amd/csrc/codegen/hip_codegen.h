@@ -31,6 +31,9 @@ struct CodegenOptions {
     bool allow_lds_tables = true;     // stage compact hash tables into LDS
     bool allow_lds_aggregation = true;
     bool allow_compact_tables = true;
+    // Also emit the per-tile pipeline body for the batched shared scan
+    // (global tables, per-tile scalar reductions, global GROUP BY atomics).
+    bool batch_body = false;
 };
 
 struct JoinProgram {
@@ -91,7 +94,22 @@ struct QueryProgram {
 
     // algorithmic bytes of one scan (fact columns + bitmaps + dimension columns)
     uint64_t algorithmic_bytes = 0;
+
+    // batched shared scan: pipeline + aggregation code of one tile, reading
+    // c<slot>[i] / v<slot> registers, the flag mask `fl` and the query's
+    // parameter block `P`; `continue` leaves the query's body
+    std::string batch_body;
+    std::vector<int> count_slots_scalar;  // scalar COUNT slots (validity set whenever rows are scanned)
 };
+
+// One hipRTC module with `cq_batch_scan`: the union of the templates' fact
+// columns is loaded once per tile and each template's body runs for each of
+// its queries (a runtime loop over parameter blocks).  Templates must have
+// been generated with CodegenOptions::batch_body.  `union_cols` receives the
+// union slot order and `union_nullable` which of them carry validity.
+std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int block_threads,
+                                    int vectors_per_thread, std::vector<std::string>& union_cols,
+                                    std::vector<bool>& union_nullable);
 
 // Visitor entry points as in the reference: AggregateNode / SortLimitNode
 // roots start generation (visitor.cpp:1589-1621); projections are rejected.

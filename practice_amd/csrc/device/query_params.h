@@ -41,4 +41,21 @@ struct cq_query_params {
     long long num_tiles;
 };
 
+#define CQ_MAX_BATCH_COLS 24
+#define CQ_MAX_TEMPLATES 16
+
+// Kernel arguments of the batched shared scan: the union of fact columns is
+// loaded once per tile; template t runs its pipeline for queries
+// [tq_begin[t], tq_end[t]) of the parameter-block array `qp` (device memory).
+struct cq_batch_params {
+    const int* col[CQ_MAX_BATCH_COLS];
+    const unsigned int* valid[CQ_MAX_BATCH_COLS];
+    const struct cq_query_params* qp;
+    int tq_begin[CQ_MAX_TEMPLATES];
+    int tq_end[CQ_MAX_TEMPLATES];
+    long long row_begin;
+    long long row_end;
+    long long num_tiles;
+};
+
 #endif

@@ -71,11 +71,15 @@ public:
     void fetchColumn(const std::string& column, int32_t* out, uint64_t* validity_out) const;
     size_t loadedDeviceBytes() const { return loaded_bytes_; }
     QueryEngine& engine() { return *engine_; }
+    const QueryEngine& engine() const { return *engine_; }
     EngineConfig& config() { return ctx_->config; }
+    const EngineConfig& config() const { return ctx_->config; }
     PlanSignatureCache& moduleCache() { return *cache_; }
     HipRtcCompiler& compiler() { return *compiler_; }
     ExecutionContext& context() { return *ctx_; }
     hipStream_t stream() const { return ctx_->stream; }
+    // Serializes use of the execution context (queries, batches, generation).
+    std::mutex& contextMutex() const { return mu_; }
 
 private:
     int device_;

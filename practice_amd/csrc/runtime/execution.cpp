@@ -207,13 +207,14 @@ int typeCode(LogicalType t) {
 
 }  // namespace
 
-void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ctx) {
-    const QueryProgram& p = ctx->program;
-    hipStream_t s = ctx->stream;
+void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, ExecutionContext* ctx,
+                             const std::string& prefix, QueryRun& run) {
     hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
     if (!ctx->columns) throw std::runtime_error("ExecutionContext has no column registry");
-
-    cq_query_params P;
+    run.module = module;
+    run.program = &p;
+    run.prefix = prefix;
+    cq_query_params& P = run.P;
     std::memset(&P, 0, sizeof P);
     auto column = [&](const std::string& name) -> const DeviceColumn& {
         auto it = ctx->columns->find(name);
@@ -234,82 +235,122 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
     for (size_t i = 0; i < p.lits.size(); ++i) P.lit[i] = p.lits[i];
     for (size_t i = 0; i < p.flits.size(); ++i) P.flit[i] = p.flits[i];
 
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ks = nullptr, ke = nullptr;
-    hipCheck(hipEventCreate(&ev0), "hipEventCreate");
-    hipCheck(hipEventCreate(&ev1), "hipEventCreate");
-    if (ctx->config.kernel_timing) {
-        hipCheck(hipEventCreate(&ks), "hipEventCreate");
-        hipCheck(hipEventCreate(&ke), "hipEventCreate");
-    }
-    hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
-
-    // hash tables
+    run.builds.clear();
     for (size_t j = 0; j < p.joins.size(); ++j) {
-        const JoinProgram& jp = p.joins[j];
-        DeviceBuffer& b = ctx->scratch("ht" + std::to_string(j));
-        b.ensure(std::max<size_t>(jp.ht_words * 4, 16));
+        DeviceBuffer& b = ctx->scratch(prefix + "ht" + std::to_string(j));
+        b.ensure(std::max<size_t>(p.joins[j].ht_words * 4, 16));
         P.ht[j] = static_cast<unsigned int*>(b.data());
-        P.dim_rows[j] = jp.dim_rows;
-        hipCheck(cq_launch_fill_u32(P.ht[j], 0u, static_cast<long long>(jp.ht_words), s), "fill hash table");
+        P.dim_rows[j] = p.joins[j].dim_rows;
+        run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
     }
+    run.dense_count = p.fast_sparse ? nullptr : module->function("cq_dense_count");
+
     // aggregate state
     const int64_t G = p.groups;
+    run.G = G;
     const size_t slots = static_cast<size_t>(G) * static_cast<size_t>(p.storage_ts);
-    const size_t words = (slots + 63) / 64;
-    DeviceBuffer& rb = ctx->scratch("result");
-    DeviceBuffer& vb = ctx->scratch("result_valid");
+    run.slots = slots;
+    DeviceBuffer& rb = ctx->scratch(prefix + "result");
+    DeviceBuffer& vb = ctx->scratch(prefix + "result_valid");
     rb.ensure(slots * 8);
-    vb.ensure(words * 8);
+    vb.ensure(((slots + 63) / 64) * 8);
     P.result = static_cast<unsigned long long*>(rb.data());
     P.result_valid = static_cast<unsigned long long*>(vb.data());
-    hipCheck(cq_launch_fill_u64(P.result, 0ull, static_cast<long long>(slots), s), "fill result");
-    hipCheck(cq_launch_fill_u64(P.result_valid, 0ull, static_cast<long long>(words), s), "fill result validity");
+    run.res = P.result;
+    run.rv = P.result_valid;
+    if (p.has_avg) {
+        const size_t vslots = static_cast<size_t>(G) * static_cast<size_t>(p.visible_ts);
+        DeviceBuffer& a = ctx->scratch(prefix + "vresult");
+        DeviceBuffer& b = ctx->scratch(prefix + "vresult_valid");
+        a.ensure(vslots * 8);
+        b.ensure(((vslots + 63) / 64) * 8);
+        run.res = static_cast<unsigned long long*>(a.data());
+        run.rv = static_cast<unsigned long long*>(b.data());
+    }
+
+    // materialization buffers
+    const int vts = p.visible_ts;
+    run.single_cell = p.fast_sparse && vts <= 1;
+    run.cap_rows = std::max<long long>(G, 1);
+    run.vwords = (run.cap_rows + 63) / 64;
+    DeviceBuffer& ob = ctx->scratch(prefix + "out");
+    DeviceBuffer& ovb = ctx->scratch(prefix + "out_valid");
+    DeviceBuffer& flags = ctx->scratch(prefix + "flags");
+    DeviceBuffer& offsets = ctx->scratch(prefix + "offsets");
+    DeviceBuffer& bsums = ctx->scratch(prefix + "block_sums");
+    ob.ensure(static_cast<size_t>(vts) * run.cap_rows * 8);
+    ovb.ensure(static_cast<size_t>(vts) * run.vwords * 8);
+    flags.ensure(run.cap_rows * 8);
+    offsets.ensure(run.cap_rows * 8 + 8);
+    bsums.ensure(static_cast<size_t>(cq_scan_block_count(run.cap_rows) + 1) * 8);
+    run.o = static_cast<unsigned long long*>(ob.data());
+    run.ov = static_cast<unsigned long long*>(ovb.data());
+    run.fl = static_cast<unsigned long long*>(flags.data());
+    run.off = static_cast<unsigned long long*>(offsets.data());
+    run.bsums = static_cast<unsigned long long*>(bsums.data());
+
+    const int64_t row_end = ctx->row_end < 0 ? p.fact_rows : std::min<int64_t>(ctx->row_end, p.fact_rows);
+    if (ctx->row_begin < 0 || ctx->row_begin % 4 != 0) throw std::runtime_error("scan range must start at a multiple of 4 rows");
+    P.row_begin = ctx->row_begin;
+    P.row_end = std::max<int64_t>(row_end, ctx->row_begin);
+    const int64_t tile = static_cast<int64_t>(p.block_threads) * 4 * p.vectors_per_thread;
+    P.num_tiles = (P.row_end - P.row_begin + tile - 1) / tile;
+}
+
+void HipModuleExecutor::setup(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params) {
+    const QueryProgram& p = *run.program;
+    cq_query_params& P = run.P;
+    hipStream_t s = ctx->stream;
+    for (size_t j = 0; j < p.joins.size(); ++j)
+        hipCheck(cq_launch_fill_u32(P.ht[j], 0u, static_cast<long long>(p.joins[j].ht_words), s), "fill hash table");
+    const size_t words = (run.slots + 63) / 64;
+    hipCheck(cq_launch_fill_u64(P.result, 0ull, static_cast<long long>(run.slots), s), "fill result");
+    // scalar COUNT of a non-empty scan is valid even when no row qualifies
+    unsigned long long count_bits = 0;
+    if (!p.grouped && P.row_end > P.row_begin)
+        for (int slot : p.count_slots_scalar)
+            if (slot < 64) count_bits |= 1ull << slot;
+    if (words > 0) hipCheck(cq_launch_fill_u64(P.result_valid, count_bits, 1, s), "fill result validity");
+    if (words > 1) hipCheck(cq_launch_fill_u64(P.result_valid + 1, 0ull, static_cast<long long>(words - 1), s), "fill");
     unsigned int min_mask = 0;
     for (int slot : p.min_slots) min_mask |= 1u << slot;
-    hipCheck(cq_launch_init_min(P.result, G, p.storage_ts, min_mask, s), "init MIN slots");
-
-    // builds
+    if (min_mask) hipCheck(cq_launch_init_min(P.result, run.G, p.storage_ts, min_mask, s), "init MIN slots");
     for (size_t j = 0; j < p.joins.size(); ++j) {
-        hipFunction_t f = module->function("cq_build_" + std::to_string(j));
         const long long rows = p.joins[j].dim_rows;
-        unsigned grid = static_cast<unsigned>(std::min<long long>(std::max<long long>((rows + 255) / 256, 1), 8192));
-        void* args[] = {&P};
-        launch(f, grid, 256, s, args, "launch build");
+        const unsigned grid = static_cast<unsigned>(std::min<long long>(std::max<long long>((rows + 255) / 256, 1), 8192));
+        const cq_query_params* dp = device_params;
+        void* by_value[] = {&P};
+        void* by_pointer[] = {&dp};
+        launch(run.builds[j], grid, 256, s, device_params ? by_pointer : by_value, "launch build");
     }
+}
 
-    // scan
-    const int64_t row_end = ctx->row_end < 0 ? p.fact_rows : ctx->row_end;
-    P.row_begin = ctx->row_begin;
-    P.row_end = row_end;
-    const int64_t tile = static_cast<int64_t>(p.block_threads) * 4 * p.vectors_per_thread;
-    P.num_tiles = row_end > P.row_begin ? (row_end - P.row_begin + tile - 1) / tile : 0;
-    if (P.row_begin % 4 != 0) throw std::runtime_error("scan range must start at a multiple of 4 rows");
-    {
-        hipFunction_t f = module->function("cq_scan");
-        int per_cu = 0;
-        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, p.block_threads, 0) != hipSuccess || per_cu < 1)
-            per_cu = 1;
-        const int64_t max_blocks = static_cast<int64_t>(per_cu) * deviceCUs(ctx->device);
-        const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(P.num_tiles, max_blocks)));
-        void* args[] = {&P};
-        if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");
-        if (P.num_tiles > 0) launch(f, grid, static_cast<unsigned>(p.block_threads), s, args, "launch scan");
-        if (ke) hipCheck(hipEventRecord(ke, s), "hipEventRecord");
-    }
+unsigned scanGrid(hipFunction_t f, int block_threads, int device, long long num_tiles) {
+    int per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block_threads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const int64_t max_blocks = static_cast<int64_t>(per_cu) * deviceCUs(device);
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(num_tiles, max_blocks)));
+}
 
-    // AVG: storage tuple -> visible tuple
-    unsigned long long* res = P.result;
-    unsigned long long* rv = P.result_valid;
+void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
+    const QueryProgram& p = *run.program;
+    hipFunction_t f = run.module->function("cq_scan");
+    const unsigned grid = scanGrid(f, p.block_threads, ctx->device, run.P.num_tiles);
+    void* args[] = {&run.P};
+    if (run.P.num_tiles > 0) launch(f, grid, static_cast<unsigned>(p.block_threads), ctx->stream, args, "launch scan");
+}
+
+void HipModuleExecutor::shape(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params,
+                              unsigned long long* row_count) {
+    const QueryProgram& p = *run.program;
+    cq_query_params& P = run.P;
+    hipStream_t s = ctx->stream;
+    const int64_t G = run.G;
     const int vts = p.visible_ts;
     if (p.has_avg) {
         const size_t vslots = static_cast<size_t>(G) * static_cast<size_t>(vts);
-        DeviceBuffer& a = ctx->scratch("vresult");
-        DeviceBuffer& b = ctx->scratch("vresult_valid");
-        a.ensure(vslots * 8);
-        b.ensure(((vslots + 63) / 64) * 8);
-        hipCheck(cq_launch_fill_u64(static_cast<unsigned long long*>(a.data()), 0ull, static_cast<long long>(vslots), s), "fill");
-        hipCheck(cq_launch_fill_u64(static_cast<unsigned long long*>(b.data()), 0ull,
-                                    static_cast<long long>((vslots + 63) / 64), s), "fill");
+        hipCheck(cq_launch_fill_u64(run.res, 0ull, static_cast<long long>(vslots), s), "fill");
+        hipCheck(cq_launch_fill_u64(run.rv, 0ull, static_cast<long long>((vslots + 63) / 64), s), "fill");
         cq_finalize_desc d;
         std::memset(&d, 0, sizeof d);
         d.ngroups = p.ngroups;
@@ -320,127 +361,128 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
             d.kind[i] = aggKind(p.agg_funcs[static_cast<size_t>(i)]);
             d.hidden[i] = p.hidden_slot[static_cast<size_t>(i)];
         }
-        hipCheck(cq_launch_finalize(res, rv, G, &d, static_cast<unsigned long long*>(a.data()),
-                                    static_cast<unsigned long long*>(b.data()), s), "finalize AVG");
-        res = static_cast<unsigned long long*>(a.data());
-        rv = static_cast<unsigned long long*>(b.data());
+        hipCheck(cq_launch_finalize(P.result, P.result_valid, G, &d, run.res, run.rv, s), "finalize AVG");
     }
+    if (run.single_cell) return;
+    hipCheck(cq_launch_fill_u64(run.ov, 0ull, static_cast<long long>(vts) * run.vwords, s), "fill out validity");
+    if (p.fast_sparse) {
+        hipCheck(cq_launch_sparse_count(run.res, G, vts, run.fl, s), "sparse count");
+    } else {
+        long long g = G;
+        const unsigned long long* R = run.res;
+        const unsigned long long* RV = run.rv;
+        unsigned long long* counts = run.fl;
+        const cq_query_params* dp = device_params;
+        void* by_value[] = {&P, &R, &RV, &counts, &g};
+        void* by_pointer[] = {&dp, &R, &RV, &counts, &g};
+        launch(run.dense_count, static_cast<unsigned>((G + 255) / 256), 256, s, device_params ? by_pointer : by_value,
+               "launch dense count");
+    }
+    hipCheck(cq_launch_exclusive_scan(run.fl, run.off, G, run.bsums, row_count ? row_count : run.off + run.cap_rows, s), "scan");
+    unsigned int nmask = 0;
+    for (int c = 0; c < vts; ++c)
+        if (p.out[static_cast<size_t>(c)].nullable) nmask |= 1u << c;
+    hipCheck(cq_launch_write_rows(run.res, run.rv, G, vts, run.fl, run.off, nmask, p.fast_sparse ? 0 : 1, run.o, run.ov,
+                                  run.cap_rows, run.vwords, s),
+             "write rows");
+}
 
-    QueryResult& out = ctx->result;
+void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found) {
+    const QueryProgram& p = *run.program;
+    hipStream_t s = ctx->stream;
+    const int vts = p.visible_ts;
+    QueryResult& out = run.result;
     out.columns.clear();
     for (const auto& oc : p.out) out.columns.push_back(ResultColumnDesc{oc.type, oc.nullable, oc.name});
     out.data.assign(static_cast<size_t>(vts), {});
     out.validity.assign(static_cast<size_t>(vts), {});
-
-    const long long cap_rows = std::max<long long>(G, 1);
-    const long long vwords = (cap_rows + 63) / 64;
-    DeviceBuffer& ob = ctx->scratch("out");
-    DeviceBuffer& ovb = ctx->scratch("out_valid");
-    DeviceBuffer& flags = ctx->scratch("flags");
-    DeviceBuffer& offsets = ctx->scratch("offsets");
-    DeviceBuffer& bsums = ctx->scratch("block_sums");
-    ob.ensure(static_cast<size_t>(vts) * cap_rows * 8);
-    ovb.ensure(static_cast<size_t>(vts) * vwords * 8);
-    flags.ensure(cap_rows * 8);
-    offsets.ensure(cap_rows * 8 + 8);
-    bsums.ensure(static_cast<size_t>(cq_scan_block_count(cap_rows) + 1) * 8);
-    auto* o = static_cast<unsigned long long*>(ob.data());
-    auto* ov = static_cast<unsigned long long*>(ovb.data());
-    auto* fl = static_cast<unsigned long long*>(flags.data());
-    auto* off = static_cast<unsigned long long*>(offsets.data());
-    auto* grand = off + cap_rows;
-    hipCheck(cq_launch_fill_u64(ov, 0ull, static_cast<long long>(vts) * vwords, s), "fill out validity");
-
-    long long rows = 0;
-    bool single_cell = false;
-    if (p.fast_sparse && vts <= 1) {
+    out.ordered = !p.fast_sparse || run.single_cell;
+    out.dense_result = !p.fast_sparse;
+    if (run.single_cell) {
         // scalar sparse output: one row, the raw slot (SUM over no rows = 0)
-        single_cell = true;
-        rows = 1;
-        out.ordered = true;
-        out.dense_result = false;
-    } else {
-        if (p.fast_sparse) {
-            hipCheck(cq_launch_sparse_count(res, G, vts, fl, s), "sparse count");
-            out.ordered = false;
-            out.dense_result = false;
-        } else {
-            hipFunction_t f = module->function("cq_dense_count");
-            cq_query_params Q = P;
-            Q.result = res;
-            Q.result_valid = rv;
-            long long g = G;
-            void* args[] = {&Q, &fl, &g};
-            launch(f, static_cast<unsigned>((G + 255) / 256), 256, s, args, "launch dense count");
-            out.ordered = true;
-            out.dense_result = true;
-        }
-        hipCheck(cq_launch_exclusive_scan(fl, off, G, static_cast<unsigned long long*>(bsums.data()), grand, s), "scan");
-        unsigned int nmask = 0;
-        for (int c = 0; c < vts; ++c)
-            if (p.out[static_cast<size_t>(c)].nullable) nmask |= 1u << c;
-        hipCheck(cq_launch_write_rows(res, rv, G, vts, fl, off, nmask, p.fast_sparse ? 0 : 1, o, ov, cap_rows, vwords, s),
-                 "write rows");
-        unsigned long long total = 0;
-        hipCheck(hipMemcpyAsync(&total, grand, 8, hipMemcpyDeviceToHost, s), "copy row count");
-        hipCheck(hipStreamSynchronize(s), "sync");
-        rows = static_cast<long long>(total);
-        if (p.has_sort_node && !p.sort_keys.empty() && rows > 1) {
-            DeviceBuffer& ib = ctx->scratch("sort_idx");
-            DeviceBuffer& o2 = ctx->scratch("out2");
-            DeviceBuffer& ov2 = ctx->scratch("out2_valid");
-            ib.ensure(static_cast<size_t>(cq_sort_padded(rows)) * 8);
-            o2.ensure(static_cast<size_t>(vts) * cap_rows * 8);
-            ov2.ensure(static_cast<size_t>(vts) * vwords * 8);
-            cq_sort_desc d;
-            std::memset(&d, 0, sizeof d);
-            d.nkeys = static_cast<int>(std::min<size_t>(p.sort_keys.size(), 16));
-            for (int k = 0; k < d.nkeys; ++k) {
-                const int c = static_cast<int>(p.sort_keys[static_cast<size_t>(k)].column_index);
-                d.col[k] = c;
-                d.type[k] = typeCode(p.out[static_cast<size_t>(c)].type);
-                d.nullable[k] = p.out[static_cast<size_t>(c)].nullable ? 1 : 0;
-                d.desc[k] = p.sort_keys[static_cast<size_t>(k)].descending ? 1 : 0;
-            }
-            auto* idx = static_cast<unsigned long long*>(ib.data());
-            hipCheck(cq_launch_sort(idx, rows, &d, o, ov, cap_rows, vwords, s), "sort");
-            auto* o2p = static_cast<unsigned long long*>(o2.data());
-            auto* ov2p = static_cast<unsigned long long*>(ov2.data());
-            hipCheck(cq_launch_fill_u64(ov2p, 0ull, static_cast<long long>(vts) * vwords, s), "fill");
-            hipCheck(cq_launch_gather(o, ov, idx, rows, vts, cap_rows, vwords, o2p, ov2p, s), "gather");
-            o = o2p;
-            ov = ov2p;
-        }
-        if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
-    }
-    hipCheck(hipEventRecord(ev1, s), "hipEventRecord");
-
-    // fetch
-    const auto f0 = std::chrono::steady_clock::now();
-    out.row_count = static_cast<size_t>(rows);
-    if (single_cell) {
+        out.row_count = 1;
         out.data[0].resize(1);
-        hipCheck(hipMemcpyAsync(out.data[0].data(), res, 8, hipMemcpyDeviceToHost, s), "fetch");
+        hipCheck(hipMemcpyAsync(out.data[0].data(), run.res, 8, hipMemcpyDeviceToHost, s), "fetch");
         out.validity[0].assign(1, 1);
-    } else {
-        std::vector<uint64_t> vbits(static_cast<size_t>(vts) * vwords);
-        for (int c = 0; c < vts; ++c) {
-            out.data[static_cast<size_t>(c)].resize(static_cast<size_t>(rows));
-            if (rows)
-                hipCheck(hipMemcpyAsync(out.data[static_cast<size_t>(c)].data(), o + static_cast<size_t>(c) * cap_rows, rows * 8,
-                                        hipMemcpyDeviceToHost, s), "fetch column");
-        }
-        if (!vbits.empty())
-            hipCheck(hipMemcpyAsync(vbits.data(), ov, vbits.size() * 8, hipMemcpyDeviceToHost, s), "fetch validity");
         hipCheck(hipStreamSynchronize(s), "sync");
-        for (int c = 0; c < vts; ++c) {
-            auto& vv = out.validity[static_cast<size_t>(c)];
-            vv.resize(static_cast<size_t>(rows));
-            for (long long r = 0; r < rows; ++r)
-                vv[static_cast<size_t>(r)] = static_cast<uint8_t>((vbits[static_cast<size_t>(c) * vwords + (r >> 6)] >> (r & 63)) & 1ull);
-        }
+        return;
     }
+    long long rows = static_cast<long long>(rows_found);
+    unsigned long long* o = run.o;
+    unsigned long long* ov = run.ov;
+    const long long cap_rows = run.cap_rows, vwords = run.vwords;
+    if (p.has_sort_node && !p.sort_keys.empty() && rows > 1) {
+        DeviceBuffer& ib = ctx->scratch(run.prefix + "sort_idx");
+        DeviceBuffer& o2 = ctx->scratch(run.prefix + "out2");
+        DeviceBuffer& ov2 = ctx->scratch(run.prefix + "out2_valid");
+        ib.ensure(static_cast<size_t>(cq_sort_padded(rows)) * 8);
+        o2.ensure(static_cast<size_t>(vts) * cap_rows * 8);
+        ov2.ensure(static_cast<size_t>(vts) * vwords * 8);
+        cq_sort_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.nkeys = static_cast<int>(std::min<size_t>(p.sort_keys.size(), 16));
+        for (int k = 0; k < d.nkeys; ++k) {
+            const int c = static_cast<int>(p.sort_keys[static_cast<size_t>(k)].column_index);
+            d.col[k] = c;
+            d.type[k] = typeCode(p.out[static_cast<size_t>(c)].type);
+            d.nullable[k] = p.out[static_cast<size_t>(c)].nullable ? 1 : 0;
+            d.desc[k] = p.sort_keys[static_cast<size_t>(k)].descending ? 1 : 0;
+        }
+        auto* idx = static_cast<unsigned long long*>(ib.data());
+        hipCheck(cq_launch_sort(idx, rows, &d, o, ov, cap_rows, vwords, s), "sort");
+        auto* o2p = static_cast<unsigned long long*>(o2.data());
+        auto* ov2p = static_cast<unsigned long long*>(ov2.data());
+        hipCheck(cq_launch_fill_u64(ov2p, 0ull, static_cast<long long>(vts) * vwords, s), "fill");
+        hipCheck(cq_launch_gather(o, ov, idx, rows, vts, cap_rows, vwords, o2p, ov2p, s), "gather");
+        o = o2p;
+        ov = ov2p;
+    }
+    if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
+    out.row_count = static_cast<size_t>(rows);
+    std::vector<uint64_t> vbits(static_cast<size_t>(vts) * vwords);
+    for (int c = 0; c < vts; ++c) {
+        out.data[static_cast<size_t>(c)].resize(static_cast<size_t>(rows));
+        if (rows)
+            hipCheck(hipMemcpyAsync(out.data[static_cast<size_t>(c)].data(), o + static_cast<size_t>(c) * cap_rows, rows * 8,
+                                    hipMemcpyDeviceToHost, s), "fetch column");
+    }
+    if (!vbits.empty()) hipCheck(hipMemcpyAsync(vbits.data(), ov, vbits.size() * 8, hipMemcpyDeviceToHost, s), "fetch validity");
     hipCheck(hipStreamSynchronize(s), "sync");
+    for (int c = 0; c < vts; ++c) {
+        auto& vv = out.validity[static_cast<size_t>(c)];
+        vv.resize(static_cast<size_t>(rows));
+        for (long long r = 0; r < rows; ++r)
+            vv[static_cast<size_t>(r)] = static_cast<uint8_t>((vbits[static_cast<size_t>(c) * vwords + (r >> 6)] >> (r & 63)) & 1ull);
+    }
+}
+
+void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ctx) {
+    hipStream_t s = ctx->stream;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ks = nullptr, ke = nullptr;
+    hipCheck(hipEventCreate(&ev0), "hipEventCreate");
+    hipCheck(hipEventCreate(&ev1), "hipEventCreate");
+    if (ctx->config.kernel_timing) {
+        hipCheck(hipEventCreate(&ks), "hipEventCreate");
+        hipCheck(hipEventCreate(&ke), "hipEventCreate");
+    }
+    QueryRun run;
+    bind(module, ctx->program, ctx, "", run);
+    hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
+    setup(run, ctx, nullptr);
+    if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");
+    scan(run, ctx);
+    if (ke) hipCheck(hipEventRecord(ke, s), "hipEventRecord");
+    shape(run, ctx, nullptr, nullptr);
+    unsigned long long rows = 0;
+    if (!run.single_cell) {
+        hipCheck(hipMemcpyAsync(&rows, run.off + run.cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
+        hipCheck(hipStreamSynchronize(s), "sync");
+    }
+    const auto f0 = std::chrono::steady_clock::now();
+    fetch(run, ctx, rows);
+    hipCheck(hipEventRecord(ev1, s), "hipEventRecord");
+    hipCheck(hipEventSynchronize(ev1), "sync");
+    ctx->result = std::move(run.result);
     const auto f1 = std::chrono::steady_clock::now();
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev0, ev1);
@@ -469,21 +511,25 @@ QueryProgram QueryEngine::prepare(const std::string& sql, const CodegenOptions& 
 }
 
 ModuleHandle QueryEngine::moduleFor(const QueryProgram& program, QueryTimingStats* timing) {
-    ModuleHandle m = cache_->get(program.signature);
+    return moduleForSource(program.source, program.signature, timing);
+}
+
+ModuleHandle QueryEngine::moduleForSource(const std::string& source, const std::string& signature, QueryTimingStats* timing) {
+    ModuleHandle m = cache_->get(signature);
     if (m) {
         if (timing) timing->cache_hit = true;
         return m;
     }
     std::lock_guard<std::mutex> lock(compile_mu_);
-    m = cache_->get(program.signature);
+    m = cache_->get(signature);
     if (m) {
         if (timing) timing->cache_hit = true;
         return m;
     }
     const auto c0 = std::chrono::steady_clock::now();
-    m = compiler_->compile(program.source, program.signature);
+    m = compiler_->compile(source, signature);
     if (timing) timing->compile_ms += elapsedMs(c0, std::chrono::steady_clock::now());
-    cache_->put(program.signature, m);
+    cache_->put(signature, m);
     return m;
 }
 

@@ -200,10 +200,48 @@ private:
     size_t hits_ = 0, misses_ = 0;
 };
 
+// Device state of one query in flight: its parameter block (columns, tables,
+// aggregate slots, constants), module, bound scratch and host result.
+// Scratch buffers are named under `prefix`, so several queries can be in
+// flight in one context (the batched shared scan binds one run per query).
+struct QueryRun {
+    ModuleHandle module;
+    const QueryProgram* program = nullptr;
+    std::string prefix;
+    cq_query_params P;
+    std::vector<hipFunction_t> builds;
+    hipFunction_t dense_count = nullptr;
+    int64_t G = 1;
+    size_t slots = 0;
+    unsigned long long* res = nullptr;  // visible tuples (after AVG finalization)
+    unsigned long long* rv = nullptr;
+    bool single_cell = false;
+    long long cap_rows = 1, vwords = 1;
+    unsigned long long *o = nullptr, *ov = nullptr, *fl = nullptr, *off = nullptr, *bsums = nullptr;
+    QueryResult result;
+};
+
+// Executes a module in phases.  bind() does host work only (parameter block,
+// scratch allocation, module load), so setup() / scan / shape() enqueue
+// kernels and nothing else and can be captured into a HIP graph.
 class HipModuleExecutor : public IExecutor {
 public:
     void execute(const ModuleHandle& module, ExecutionContext* ctx) override;
+    void bind(const ModuleHandle& module, const QueryProgram& program, ExecutionContext* ctx, const std::string& prefix,
+              QueryRun& run);
+    // zero tables / aggregate slots, build the dimension tables.  With
+    // `device_params` the generated kernels read the block from device memory.
+    void setup(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params);
+    void scan(QueryRun& run, ExecutionContext* ctx);
+    // AVG finalization, present/HAVING flags, compaction; the number of
+    // result rows lands in device memory at `row_count` (default: after the offsets)
+    void shape(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params, unsigned long long* row_count);
+    // sort / limit and copy the result to the host (synchronizes)
+    void fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows);
 };
+
+// Persistent grid for a scan kernel: occupancy x CUs, at most one block per tile.
+unsigned scanGrid(hipFunction_t f, int block_threads, int device, long long num_tiles);
 
 // Device bytes a query needs beyond the resident columns (hash tables,
 // aggregate slots, materialization buffers).
@@ -219,6 +257,8 @@ public:
     QueryProgram prepare(const std::string& sql, const CodegenOptions& options) const;
     // compile or fetch the module for a program
     ModuleHandle moduleFor(const QueryProgram& program, QueryTimingStats* timing);
+    ModuleHandle moduleForSource(const std::string& source, const std::string& signature, QueryTimingStats* timing);
+    HipModuleExecutor* hipExecutor() { return dynamic_cast<HipModuleExecutor*>(executor_.get()); }
     void setCatalog(std::shared_ptr<const Catalog> catalog) { catalog_ = std::move(catalog); }
     IQueryCache& cache() { return *cache_; }
 

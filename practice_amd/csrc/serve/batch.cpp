@@ -1,22 +1,405 @@
-// Batched execution (per-query JIT path for every query).
+// Batched execution over one shared fact scan per group of templates.
 #include "batch.h"
 
+#include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <sstream>
+#include <stdexcept>
 
 namespace db {
 
+BatchOptions parseBatchOptions(const std::string& text) {
+    BatchOptions o;
+    std::stringstream ss(text);
+    std::string item;
+    while (std::getline(ss, item, ',')) {
+        if (item.empty()) continue;
+        const size_t eq = item.find('=');
+        if (eq == std::string::npos) throw std::runtime_error("batch option without value: " + item);
+        const std::string k = item.substr(0, eq);
+        const long v = std::stol(item.substr(eq + 1));
+        if (k == "bt") o.block_threads = static_cast<int>(v);
+        else if (k == "vpt") o.vectors_per_thread = static_cast<int>(v);
+        else if (k == "graph") o.graph = v != 0;
+        else if (k == "autotune") o.autotune = v != 0;
+        else if (k == "max_templates") o.max_templates = static_cast<int>(v);
+        else if (k == "max_queries") o.max_queries = static_cast<int>(v);
+        else throw std::runtime_error("unknown batch option: " + k);
+    }
+    if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
+        throw std::runtime_error("batch block size must be a multiple of 64 in [64, 1024]");
+    if (o.vectors_per_thread < 1 || o.vectors_per_thread > 4) throw std::runtime_error("batch vpt must be in [1, 4]");
+    o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
+    o.max_queries = std::max(1, o.max_queries);
+    return o;
+}
+
+std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std::string>& sqls, const BatchOptions& opt,
+                              int part) {
+    CodegenOptions co = db.config().codegen;
+    co.batch_body = true;
+    std::vector<QueryProgram> programs;
+    for (const auto& q : sqls) programs.push_back(db.engine().prepare(q, co));
+    if (part >= 0) {
+        if (static_cast<size_t>(part) >= programs.size()) throw std::runtime_error("batch part out of range");
+        return programs[static_cast<size_t>(part)].source;
+    }
+    std::vector<const QueryProgram*> templates;
+    for (const auto& p : programs) {
+        bool seen = false;
+        for (const auto* t : templates) seen = seen || t->signature == p.signature;
+        if (!seen) templates.push_back(&p);
+    }
+    std::vector<std::string> cols;
+    std::vector<bool> nullable;
+    return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable);
+}
+
+BatchExecutor::BatchExecutor(DatabaseInstance& db) : db_(db) {}
+
+BatchExecutor::~BatchExecutor() {
+    if (db_.hostOnly()) return;
+    (void)hipSetDevice(db_.device());
+    for (auto& kv : graphs_) dropGraph(kv.second);
+    for (auto& e : ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (pinned_) (void)hipHostFree(pinned_);
+}
+
+void BatchExecutor::dropGraph(Graph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+    g.key.clear();
+}
+
+void* BatchExecutor::pinned(size_t bytes) {
+    if (bytes <= pinned_bytes_) return pinned_;
+    if (pinned_) hipCheck(hipHostFree(pinned_), "hipHostFree");
+    pinned_ = nullptr;
+    pinned_bytes_ = 0;
+    const size_t want = std::max<size_t>(bytes, 64 * 1024);
+    hipCheck(hipHostMalloc(&pinned_, want, hipHostMallocDefault), "hipHostMalloc");
+    pinned_bytes_ = want;
+    return pinned_;
+}
+
 std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& sqls, const std::string& options) {
-    (void)options;
-    std::vector<QueryResult> out;
-    out.reserve(sqls.size());
+    return execute(sqls, parseBatchOptions(options));
+}
+
+std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& sqls, const BatchOptions& opt) {
+    if (db_.hostOnly()) throw std::runtime_error("batch execution needs a GPU device (host-only database)");
+    std::lock_guard<std::mutex> lock(db_.contextMutex());
+    hipCheck(hipSetDevice(db_.device()), "hipSetDevice");
+    for (auto& e : ev_)
+        if (!e) hipCheck(hipEventCreate(&e), "hipEventCreate");
     const auto t0 = std::chrono::steady_clock::now();
-    for (const auto& q : sqls) out.push_back(db_.executeQuery(q));
-    const double ms = elapsedMs(t0, std::chrono::steady_clock::now());
+
+    // plans -> programs carrying the per-tile pipeline body
+    CodegenOptions co = db_.config().codegen;
+    co.batch_body = true;
+    co.block_threads = opt.block_threads;
+    co.vectors_per_thread = opt.vectors_per_thread;
+    std::vector<QueryProgram> programs(sqls.size());
+    for (size_t i = 0; i < sqls.size(); ++i) {
+        try {
+            programs[i] = db_.engine().prepare(sqls[i], co);
+        } catch (const std::exception& e) {
+            throw std::runtime_error("batch query " + std::to_string(i) + ": " + e.what());
+        }
+    }
+    const double plan_ms = elapsedMs(t0, std::chrono::steady_clock::now());
+
+    // queries of one template stay together; templates are packed into
+    // shared scans of the same fact table within the template / column /
+    // query limits of one launch
+    std::vector<std::string> order;  // (fact table, signature) keys in first-seen order
+    std::map<std::string, std::vector<int>> members;
+    for (size_t i = 0; i < programs.size(); ++i) {
+        const std::string key = programs[i].fact_table + "|" + programs[i].signature;
+        if (!members.count(key)) order.push_back(key);
+        members[key].push_back(static_cast<int>(i));
+    }
+    std::vector<std::vector<int>> scans;
+    std::vector<int> cur;
+    std::vector<std::string> cur_cols;
+    std::string cur_table;
+    int cur_templates = 0;
+    for (const auto& key : order) {
+        const auto& qs = members[key];
+        const QueryProgram& t = programs[static_cast<size_t>(qs.front())];
+        for (size_t start = 0; start < qs.size(); start += static_cast<size_t>(opt.max_queries)) {
+            const size_t stop = std::min(qs.size(), start + static_cast<size_t>(opt.max_queries));
+            std::vector<std::string> cols = cur_cols;
+            for (const auto& c : t.fact_cols)
+                if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
+            const bool fits = !cur.empty() && cur_table == t.fact_table && cur_templates + 1 <= opt.max_templates &&
+                              cols.size() <= CQ_MAX_BATCH_COLS &&
+                              cur.size() + (stop - start) <= static_cast<size_t>(opt.max_queries);
+            if (!fits && !cur.empty()) {
+                scans.push_back(cur);
+                cur.clear();
+                cur_templates = 0;
+                cols.assign(t.fact_cols.begin(), t.fact_cols.end());
+            }
+            cur_table = t.fact_table;
+            cur_cols = cols;
+            ++cur_templates;
+            for (size_t k = start; k < stop; ++k) cur.push_back(qs[k]);
+        }
+    }
+    if (!cur.empty()) scans.push_back(cur);
+
+    std::vector<QueryResult> results(sqls.size());
     std::ostringstream r;
-    r << "{\"queries\":" << sqls.size() << ",\"shared_scan\":0,\"jit\":" << sqls.size() << ",\"wall_ms\":" << ms << "}";
+    r << "{\"queries\":" << sqls.size() << ",\"shared_scans\":" << scans.size() << ",\"plan_ms\":" << plan_ms << ",\"scans\":[";
+    for (size_t s = 0; s < scans.size(); ++s) {
+        BatchScanReport rep;
+        runScan(scans[s], programs, opt, static_cast<int>(s), results, rep);
+        r << (s ? "," : "") << "{\"queries\":" << rep.queries << ",\"templates\":" << rep.templates
+          << ",\"union_columns\":" << rep.union_columns << ",\"block_threads\":" << rep.block_threads
+          << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
+          << ",\"scan_ms\":" << rep.scan_ms << ",\"device_ms\":" << rep.device_ms << ",\"fact_bytes\":" << rep.fact_bytes << "}";
+    }
+    r << "],\"wall_ms\":" << elapsedMs(t0, std::chrono::steady_clock::now()) << "}";
     report_ = r.str();
-    return out;
+    return results;
+}
+
+namespace {
+
+std::string batchSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt, std::vector<std::string>& cols,
+                        std::vector<bool>& nullable) {
+    return generateBatchScanSource(templates, bt, vpt, cols, nullable);
+}
+
+}  // namespace
+
+void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt,
+                            int scan_index, std::vector<QueryResult>& results, BatchScanReport& rep) {
+    ExecutionContext* ctx = &db_.context();
+    QueryEngine& engine = db_.engine();
+    HipModuleExecutor* ex = engine.hipExecutor();
+    if (!ex) throw std::runtime_error("batch execution needs the HIP module executor");
+    hipStream_t s = ctx->stream;
+
+    // templates in first-seen order, queries grouped by template
+    std::vector<const QueryProgram*> templates;
+    std::vector<std::vector<int>> per_template;
+    for (int q : members) {
+        const QueryProgram& p = programs[static_cast<size_t>(q)];
+        size_t t = 0;
+        while (t < templates.size() && templates[t]->signature != p.signature) ++t;
+        if (t == templates.size()) {
+            templates.push_back(&p);
+            per_template.emplace_back();
+        }
+        per_template[t].push_back(q);
+    }
+    std::vector<int> qorder;
+    for (const auto& v : per_template) qorder.insert(qorder.end(), v.begin(), v.end());
+    const int n = static_cast<int>(qorder.size());
+    std::string set_key;
+    for (const auto* t : templates) set_key += t->signature + ";";
+
+    // per-template modules (dimension builds + result flags)
+    std::vector<ModuleHandle> tmodules;
+    for (const auto* t : templates) tmodules.push_back(engine.moduleFor(*t, nullptr));
+
+    // bind every query into its own scratch namespace
+    ctx->row_begin = 0;
+    ctx->row_end = -1;
+    std::vector<QueryRun> runs(static_cast<size_t>(n));
+    const std::string pre = "s" + std::to_string(scan_index) + "/";
+    for (int k = 0; k < n; ++k) {
+        const int q = qorder[static_cast<size_t>(k)];
+        size_t t = 0;
+        while (templates[t]->signature != programs[static_cast<size_t>(q)].signature) ++t;
+        ex->bind(tmodules[t], programs[static_cast<size_t>(q)], ctx, pre + "q" + std::to_string(k) + "/", runs[static_cast<size_t>(k)]);
+    }
+    DeviceBuffer& qpb = ctx->scratch(pre + "qp");
+    DeviceBuffer& totb = ctx->scratch(pre + "totals");
+    qpb.ensure(static_cast<size_t>(n) * sizeof(cq_query_params));
+    totb.ensure(static_cast<size_t>(n) * 8);
+    auto* dqp = static_cast<cq_query_params*>(qpb.data());
+    auto* dtot = static_cast<unsigned long long*>(totb.data());
+
+    // parameter blocks -> device (pinned staging, stream ordered)
+    const size_t qp_bytes = static_cast<size_t>(n) * sizeof(cq_query_params);
+    char* stage = static_cast<char*>(pinned(qp_bytes + static_cast<size_t>(n) * 8));
+    for (int k = 0; k < n; ++k) std::memcpy(stage + static_cast<size_t>(k) * sizeof(cq_query_params), &runs[static_cast<size_t>(k)].P,
+                                            sizeof(cq_query_params));
+    hipCheck(hipMemcpyAsync(dqp, stage, qp_bytes, hipMemcpyHostToDevice, s), "upload parameter blocks");
+
+    // launch shape: tuned per template set when asked
+    Tuned cfg{opt.block_threads, opt.vectors_per_thread};
+    auto tit = tuned_.find(set_key);
+    if (tit != tuned_.end()) cfg = tit->second;
+    else if (opt.autotune) {
+        cfg = autotune(templates, per_template, runs, dqp, ctx);
+        tuned_[set_key] = cfg;
+    }
+    std::vector<std::string> ucols;
+    std::vector<bool> unull;
+    const std::string src = batchSource(templates, cfg.block_threads, cfg.vectors_per_thread, ucols, unull);
+    ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
+    hipFunction_t fn = bm->function("cq_batch_scan");
+
+    cq_batch_params B;
+    std::memset(&B, 0, sizeof B);
+    const QueryProgram& first = *templates.front();
+    for (size_t u = 0; u < ucols.size(); ++u) {
+        auto it = ctx->columns->find(ucols[u]);
+        if (it == ctx->columns->end()) throw std::runtime_error("column not loaded on the device: " + ucols[u]);
+        B.col[u] = it->second.data;
+        B.valid[u] = reinterpret_cast<const unsigned int*>(it->second.validity);
+        if (unull[u] && !it->second.validity) throw std::runtime_error("nullable column without bitmap: " + ucols[u]);
+    }
+    B.qp = dqp;
+    int at = 0;
+    for (size_t t = 0; t < templates.size(); ++t) {
+        B.tq_begin[t] = at;
+        at += static_cast<int>(per_template[t].size());
+        B.tq_end[t] = at;
+    }
+    B.row_begin = 0;
+    B.row_end = first.fact_rows;
+    const long long tile = static_cast<long long>(cfg.block_threads) * 4 * cfg.vectors_per_thread;
+    B.num_tiles = (B.row_end - B.row_begin + tile - 1) / tile;
+    const unsigned grid = scanGrid(fn, cfg.block_threads, ctx->device, B.num_tiles);
+
+    auto enqueue = [&](bool timed) {
+        for (int k = 0; k < n; ++k) ex->setup(runs[static_cast<size_t>(k)], ctx, dqp + k);
+        if (timed) hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
+        void* args[] = {&B};
+        if (B.num_tiles > 0)
+            hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(cfg.block_threads), 1, 1, 0, s, args, nullptr),
+                     "launch batch scan");
+        if (timed) hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
+        for (int k = 0; k < n; ++k) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
+    };
+
+    hipCheck(hipEventRecord(ev_[0], s), "hipEventRecord");
+    bool replayed = false;
+    if (opt.graph) {
+        // everything a replay depends on besides the uploaded parameter blocks
+        std::ostringstream key;
+        key << bm->signature() << "|" << grid << "|" << B.num_tiles << "|" << dqp << "|" << dtot;
+        for (size_t u = 0; u < ucols.size(); ++u) key << "|" << B.col[u] << "," << B.valid[u];
+        for (int k = 0; k < n; ++k) {
+            const QueryRun& rr = runs[static_cast<size_t>(k)];
+            key << "|" << rr.program->signature << ":" << rr.P.result << "," << rr.P.result_valid << "," << rr.res << "," << rr.rv
+                << "," << rr.o << "," << rr.ov << "," << rr.fl << "," << rr.off << "," << rr.bsums << "," << rr.P.row_end;
+            for (size_t j = 0; j < rr.program->joins.size(); ++j) key << "," << rr.P.ht[j] << "/" << rr.P.dim_rows[j];
+        }
+        Graph& g = graphs_[pre];
+        if (g.exec && g.key == key.str()) {
+            replayed = true;
+        } else {
+            dropGraph(g);
+            hipCheck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                enqueue(false);
+            } catch (...) {
+                hipGraph_t broken = nullptr;
+                (void)hipStreamEndCapture(s, &broken);
+                if (broken) (void)hipGraphDestroy(broken);
+                throw;
+            }
+            hipCheck(hipStreamEndCapture(s, &g.graph), "hipStreamEndCapture");
+            hipCheck(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            g.key = key.str();
+        }
+        hipCheck(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
+    } else {
+        enqueue(true);
+    }
+    hipCheck(hipEventRecord(ev_[1], s), "hipEventRecord");
+    auto* htot = reinterpret_cast<unsigned long long*>(stage + qp_bytes);
+    hipCheck(hipMemcpyAsync(htot, dtot, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost, s), "copy row counts");
+    hipCheck(hipStreamSynchronize(s), "batch sync");
+    float dms = 0, sms = 0;
+    (void)hipEventElapsedTime(&dms, ev_[0], ev_[1]);
+    if (!opt.graph) (void)hipEventElapsedTime(&sms, ev_[2], ev_[3]);
+
+    std::vector<unsigned long long> rows(htot, htot + n);
+    for (int k = 0; k < n; ++k) {
+        QueryRun& rr = runs[static_cast<size_t>(k)];
+        ex->fetch(rr, ctx, rows[static_cast<size_t>(k)]);
+        rr.result.timing.gpu_execute_ms = dms;
+        results[static_cast<size_t>(qorder[static_cast<size_t>(k)])] = std::move(rr.result);
+    }
+
+    rep.queries = n;
+    rep.templates = static_cast<int>(templates.size());
+    rep.union_columns = static_cast<int>(ucols.size());
+    rep.block_threads = cfg.block_threads;
+    rep.vectors_per_thread = cfg.vectors_per_thread;
+    rep.graph_replayed = replayed;
+    rep.scan_ms = sms;
+    rep.device_ms = dms;
+    const uint64_t N = static_cast<uint64_t>(first.fact_rows);
+    for (size_t u = 0; u < ucols.size(); ++u) rep.fact_bytes += 4 * N + (unull[u] ? 8 * ((N + 63) / 64) : 0);
+}
+
+BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgram*>& templates,
+                                             const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
+                                             const cq_query_params* dqp, ExecutionContext* ctx) {
+    QueryEngine& engine = db_.engine();
+    HipModuleExecutor* ex = engine.hipExecutor();
+    hipStream_t s = ctx->stream;
+    Tuned best;
+    float best_ms = -1;
+    const int bts[] = {64, 128, 256, 512, 1024};
+    const int vpts[] = {1, 2, 4};
+    for (int bt : bts) {
+        for (int vpt : vpts) {
+            std::vector<std::string> ucols;
+            std::vector<bool> unull;
+            const std::string src = batchSource(templates, bt, vpt, ucols, unull);
+            ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
+            hipFunction_t fn = bm->function("cq_batch_scan");
+            cq_batch_params B;
+            std::memset(&B, 0, sizeof B);
+            for (size_t u = 0; u < ucols.size(); ++u) {
+                const DeviceColumn& c = ctx->columns->at(ucols[u]);
+                B.col[u] = c.data;
+                B.valid[u] = reinterpret_cast<const unsigned int*>(c.validity);
+            }
+            B.qp = dqp;
+            int at = 0;
+            for (size_t t = 0; t < templates.size(); ++t) {
+                B.tq_begin[t] = at;
+                at += static_cast<int>(per_template[t].size());
+                B.tq_end[t] = at;
+            }
+            B.row_end = templates.front()->fact_rows;
+            const long long tile = static_cast<long long>(bt) * 4 * vpt;
+            B.num_tiles = (B.row_end + tile - 1) / tile;
+            if (B.num_tiles == 0) return best;
+            const unsigned grid = scanGrid(fn, bt, ctx->device, B.num_tiles);
+            float ms = 0;
+            for (int rep = 0; rep < 2; ++rep) {  // first run warms the module and caches
+                for (size_t k = 0; k < runs.size(); ++k) ex->setup(runs[k], ctx, dqp + k);
+                hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
+                void* args[] = {&B};
+                hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(bt), 1, 1, 0, s, args, nullptr),
+                         "launch batch scan");
+                hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
+                hipCheck(hipEventSynchronize(ev_[3]), "sync");
+                (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+            }
+            if (best_ms < 0 || ms < best_ms) {
+                best_ms = ms;
+                best.block_threads = bt;
+                best.vectors_per_thread = vpt;
+            }
+        }
+    }
+    return best;
 }
 
 }  // namespace db

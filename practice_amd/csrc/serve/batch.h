@@ -1,6 +1,21 @@
-// Batched execution of many queries over the resident columns.
+// Batched execution: many queries, one shared fact scan.
+//
+// The reference runs every query as its own JIT kernel sequence
+// (QueryEngine::executeQuery, src/core/execution.cpp:652-725), so N queries
+// stream the fact table N times.  Here a batch is split into shared scans:
+// each one groups its queries by plan signature ("templates"), loads the
+// union of their fact columns once per tile in one persistent launch
+// (cq_batch_scan) and runs every template's pipeline for each of its queries
+// over those registers.  Dimension builds and result shaping stay per query;
+// all device work of a batch between the parameter upload and the row-count
+// readback is captured once into a HIP graph and replayed for batches of the
+// same shape (only the device-resident parameter blocks change).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -8,16 +23,70 @@
 
 namespace db {
 
+struct BatchOptions {
+    int block_threads = 256;      // cq_batch_scan block size (64..1024)
+    int vectors_per_thread = 2;   // 128-bit loads per column per lane per tile (1..4)
+    bool graph = true;            // capture / replay the device phases as a HIP graph
+    bool autotune = false;        // time block / load-width variants, keep the fastest
+    int max_templates = 16;       // templates per shared scan
+    int max_queries = 256;        // queries per shared scan
+};
+
+// "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates, max_queries)
+BatchOptions parseBatchOptions(const std::string& text);
+
+struct BatchScanReport {
+    int queries = 0, templates = 0, union_columns = 0;
+    int block_threads = 0, vectors_per_thread = 0;
+    bool graph_replayed = false;
+    double scan_ms = 0;            // cq_batch_scan alone (events)
+    double device_ms = 0;          // builds + scan + shaping (events)
+    uint64_t fact_bytes = 0;       // algorithmic fact bytes read by the shared scan
+};
+
+// Generated sources of a batch, host-only databases included: part < 0 is
+// the shared-scan module of all the queries (one scan), part = k the
+// template module (dimension builds + result flags) of query k.
+std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std::string>& sqls, const BatchOptions& options,
+                              int part);
+
 class BatchExecutor {
 public:
-    explicit BatchExecutor(DatabaseInstance& db) : db_(db) {}
-    // Executes every query; returns results in input order.
+    explicit BatchExecutor(DatabaseInstance& db);
+    ~BatchExecutor();
+    BatchExecutor(const BatchExecutor&) = delete;
+    BatchExecutor& operator=(const BatchExecutor&) = delete;
+
+    // Executes every query; results come back in input order and are the
+    // same as DatabaseInstance::executeQuery's for each query alone.
     std::vector<QueryResult> execute(const std::vector<std::string>& sqls, const std::string& options);
+    std::vector<QueryResult> execute(const std::vector<std::string>& sqls, const BatchOptions& options);
+    // JSON: per shared scan counts, timings, tuning choice, graph use.
     const std::string& lastReport() const { return report_; }
 
 private:
+    struct Graph {
+        std::string key;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    struct Tuned {
+        int block_threads = 256, vectors_per_thread = 2;
+    };
     DatabaseInstance& db_;
     std::string report_;
+    std::map<std::string, Graph> graphs_;   // keyed by scan slot
+    std::map<std::string, Tuned> tuned_;    // keyed by template-set signature
+    void* pinned_ = nullptr;                // staging for parameter blocks / row counts
+    size_t pinned_bytes_ = 0;
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+
+    void* pinned(size_t bytes);
+    void runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt, int scan_index,
+                 std::vector<QueryResult>& results, BatchScanReport& rep);
+    Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
+                   std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
+    void dropGraph(Graph& g);
 };
 
 }  // namespace db

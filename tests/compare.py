@@ -17,3 +17,13 @@ def assert_same(gpu, ora, ordered=None):
     assert len(g) == len(o), (len(g), len(o))
     for i, (a, b) in enumerate(zip(g, o)):
         assert a == b, f"row {i}: gpu {a} != oracle {b}"
+
+
+def assert_results_equal(a, b, what=""):
+    """Two product Results (e.g. batched vs. alone) hold the same rows, bit for bit."""
+    assert a.types == b.types, (what, a.types, b.types)
+    assert a.nullable == b.nullable, (what, a.nullable, b.nullable)
+    ordered = a.ordered and b.ordered
+    x = normalize(a.tuples(), ordered)
+    y = normalize(b.tuples(), ordered)
+    assert x == y, f"{what}: {len(x)} rows vs {len(y)} rows differ"
