@@ -44,6 +44,8 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
     p.add_argument("--extra", action="store_true", help="also time the other plain configurations")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                   help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -92,12 +94,32 @@ def cpu_baseline(workload, sql_text, sample_rows, seed):
                       f"dimensions), CPU oracle restatement with OpenMP, median of 3 runs: {t * 1000:.0f} ms"}
 
 
+def copy_calibration(nbytes, steps):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE counters give
+    the factor between counter readings and bytes moved."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.int32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1000 / steps
+    print(json.dumps({"copy_bytes": n * 4, "steps": steps, "ms_per_copy": round(ms, 4),
+                      "GBps_read_plus_write": round(2 * n * 4 / ms / 1e6, 1)}), flush=True)
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU scaling is not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, max(1, args.steps))
+        return
     import practice_amd as pa
     from tests import queries as Q
 

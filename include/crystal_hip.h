@@ -94,7 +94,8 @@ int cq_db_fetch_column(cq_db* db, const char* column, int32_t* out, uint64_t* va
 long long cq_db_table_rows(cq_db* db, const char* table);
 /* option names: "kernel_timing", "memory_guard", "lds_budget", "block_threads",
  * "vectors_per_thread", "lds_tables", "lds_aggregation", "compact_tables",
- * "memory_guard_reserve_bytes" */
+ * "memory_guard_reserve_bytes", "prefetch" (load the next tile while the
+ * current one is probed) */
 int cq_db_set_option(cq_db* db, const char* name, long long value);
 long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","compiles" */
 int cq_db_clear_cache(cq_db* db);

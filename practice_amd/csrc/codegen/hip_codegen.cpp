@@ -228,7 +228,9 @@ public:
         prog_.plan_sexpr = sx.str();
         analyze(root);
         emit();
-        prog_.signature = signatureOf(prog_.source);
+        // a batched template is identified by its per-tile body as well:
+        // different pipelines can share identical build / flag kernels
+        prog_.signature = signatureOf(prog_.batch_body.empty() ? prog_.source : prog_.source + prog_.batch_body);
         return std::move(prog_);
     }
 
@@ -961,36 +963,40 @@ private:
         }
         src << prologue.str();
         src << "    __syncthreads();\n";
-        src << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n";
-        src << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
         const size_t nf = prog_.fact_cols.size();
-        for (size_t s = 0; s < nf; ++s) {
-            src << "        int c" << s << "[IPT];\n";
-            if (prog_.fact_nullable[s]) src << "        u32 v" << s << " = 0u;\n";
+        if (!opt_.prefetch) {
+            src << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n";
+            src << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
+            for (size_t s = 0; s < nf; ++s) {
+                src << "        int c" << s << "[IPT];\n";
+                if (prog_.fact_nullable[s]) src << "        u32 v" << s << " = 0u;\n";
+            }
+            src << "        u32 fl = 0u;\n";
+            emitTileLoad(src, "        ", "", "base", "fl");
+        } else {
+            // software pipeline: the next tile's columns are in flight while
+            // this tile's probes and aggregation run
+            for (size_t s = 0; s < nf; ++s) {
+                src << "    int nc" << s << "[IPT];\n";
+                if (prog_.fact_nullable[s]) src << "    u32 nv" << s << " = 0u;\n";
+            }
+            src << "    u32 nfl = 0u;\n"
+                << "    if ((long long)blockIdx.x < P.num_tiles) {\n"
+                << "        const long long nbase = P.row_begin + (long long)blockIdx.x * (long long)TILE;\n";
+            emitTileLoad(src, "        ", "n", "nbase", "nfl");
+            src << "    }\n";
+            src << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n";
+            for (size_t s = 0; s < nf; ++s) {
+                src << "        int c" << s << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) c" << s << "[i] = nc" << s
+                    << "[i];\n";
+                if (prog_.fact_nullable[s]) src << "        const u32 v" << s << " = nv" << s << "; nv" << s << " = 0u;\n";
+            }
+            src << "        u32 fl = nfl; nfl = 0u;\n"
+                << "        if (tile + gridDim.x < P.num_tiles) {\n"
+                << "            const long long nbase = P.row_begin + (tile + gridDim.x) * (long long)TILE;\n";
+            emitTileLoad(src, "            ", "n", "nbase", "nfl");
+            src << "        }\n";
         }
-        src << "        u32 fl = 0u;\n";
-        src << "        if (base + TILE <= P.row_end) {\n"
-            << "#pragma unroll\n            for (int v = 0; v < VPT; ++v) {\n"
-            << "                const long long r = base + (long long)v * (BT * 4) + tid * 4;\n";
-        for (size_t s = 0; s < nf; ++s) {
-            src << "                { const v4i x = __builtin_nontemporal_load((const v4i*)(P.col[" << s << "] + r));"
-                << " c" << s << "[4 * v] = x.x; c" << s << "[4 * v + 1] = x.y; c" << s << "[4 * v + 2] = x.z; c" << s
-                << "[4 * v + 3] = x.w; }\n";
-            if (prog_.fact_nullable[s])
-                src << "                v" << s << " |= ((P.valid[" << s << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
-        }
-        src << "            }\n"
-            << "            fl = " << (ipt == 32 ? "0xffffffffu" : "((1u << IPT) - 1u)") << ";\n"
-            << "        } else {\n"
-            << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
-            << "                const long long r = base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
-            << "                const bool in = r < P.row_end;\n"
-            << "                if (in) fl |= 1u << i;\n";
-        for (size_t s = 0; s < nf; ++s) {
-            src << "                c" << s << "[i] = in ? P.col[" << s << "][r] : 0;\n";
-            if (prog_.fact_nullable[s]) src << "                if (in) v" << s << " |= chip::bit32(P.valid[" << s << "], r) << i;\n";
-        }
-        src << "            }\n        }\n";
         src << body.str();
         src << aggcode.str();
         src << "    }\n";
@@ -1001,6 +1007,38 @@ private:
         prog_.source = src.str();
         (void)a;
         setAlgorithmicBytes();
+    }
+
+    // Loads one tile of every fact column into <pre>c<s>[IPT] / <pre>v<s>
+    // (validity bits) and sets the row-present mask `flv` (full tiles: 128-bit
+    // nontemporal loads; the ragged last tile: guarded scalar loads).
+    void emitTileLoad(std::ostringstream& src, const std::string& ind, const std::string& pre, const std::string& base,
+                      const std::string& flv) {
+        const size_t nf = prog_.fact_cols.size();
+        const int ipt = 4 * prog_.vectors_per_thread;
+        src << ind << "if (" << base << " + TILE <= P.row_end) {\n"
+            << "#pragma unroll\n" << ind << "    for (int v = 0; v < VPT; ++v) {\n"
+            << ind << "        const long long r = " << base << " + (long long)v * (BT * 4) + tid * 4;\n";
+        for (size_t s = 0; s < nf; ++s) {
+            const std::string c = pre + "c" + std::to_string(s);
+            src << ind << "        { const v4i x = __builtin_nontemporal_load((const v4i*)(P.col[" << s << "] + r)); " << c
+                << "[4 * v] = x.x; " << c << "[4 * v + 1] = x.y; " << c << "[4 * v + 2] = x.z; " << c << "[4 * v + 3] = x.w; }\n";
+            if (prog_.fact_nullable[s])
+                src << ind << "        " << pre << "v" << s << " |= ((P.valid[" << s << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
+        }
+        src << ind << "    }\n"
+            << ind << "    " << flv << " = " << (ipt == 32 ? "0xffffffffu" : "((1u << IPT) - 1u)") << ";\n"
+            << ind << "} else {\n"
+            << "#pragma unroll\n" << ind << "    for (int i = 0; i < IPT; ++i) {\n"
+            << ind << "        const long long r = " << base << " + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
+            << ind << "        const bool in = r < P.row_end;\n"
+            << ind << "        if (in) " << flv << " |= 1u << i;\n";
+        for (size_t s = 0; s < nf; ++s) {
+            src << ind << "        " << pre << "c" << s << "[i] = in ? P.col[" << s << "][r] : 0;\n";
+            if (prog_.fact_nullable[s])
+                src << ind << "        if (in) " << pre << "v" << s << " |= chip::bit32(P.valid[" << s << "], r) << i;\n";
+        }
+        src << ind << "    }\n" << ind << "}\n";
     }
 
     // algorithmic bytes of one execution: fact columns + bitmaps + dimension columns

@@ -31,6 +31,7 @@ struct CodegenOptions {
     bool allow_lds_tables = true;     // stage compact hash tables into LDS
     bool allow_lds_aggregation = true;
     bool allow_compact_tables = true;
+    bool prefetch = false;            // load tile t+stride while tile t is processed
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;

@@ -54,6 +54,22 @@ def test_shared_scan_loads_each_column_once(hdb):
     pa.compile_source(src)
 
 
+def test_templates_with_identical_builds_stay_distinct(hdb):
+    # no joins, no HAVING: both template modules hold the same kernels, but
+    # their per-tile pipelines differ and must not be merged
+    qs = ["select count(*) from lineorder where lo_revenue < 5000000",
+          "select count(*) from lineorder where lo_revenue is null",
+          "select count(*) from lineorder where lo_revenue < 7000000"]
+    assert hdb.generate_batch_code(qs, part=0) == hdb.generate_batch_code(qs, part=1)
+    src = hdb.generate_batch_code(qs)
+    assert src.count("for (int q = B.tq_begin[") == 2
+
+
+def test_too_many_templates_for_one_scan(hdb):
+    with pytest.raises(pa.CrystalError, match="max 16"):
+        hdb.generate_batch_code(mixed_queries())
+
+
 def test_template_modules_compile(hdb):
     qs = mixed_queries()
     for k in range(len(qs)):
