@@ -146,12 +146,25 @@ int cq_db_generate_batch_code(cq_db* db, const char* const* sqls, int n, const c
                               size_t* needed);
 
 /* ---- serving -------------------------------------------------------- */
+/* Continuous-batching server over the database's resident columns (no
+ * reference counterpart).  Lanes (own stream each) take every queued
+ * request they can admit, up to max_batch, and run them as one shared scan;
+ * admission reserves each request's device footprint against the budget;
+ * cache misses compile on a background thread.  options: NULL or
+ * "key=value,..." with lanes (1..8, default 2), max_batch (64),
+ * max_templates (16), budget_mb (default: a quarter of free memory),
+ * window_us (0), async_compile (1), bt, vpt, graph. */
 int cq_server_start(cq_db* db, const char* options, cq_server** out);
-/* Enqueue a request; returns a ticket >= 0. */
+/* Enqueue a request; returns a ticket >= 0 (negative on error). */
 long long cq_server_submit(cq_server* s, const char* sql);
 /* Wait for a ticket (timeout_ms < 0: forever); *out owns the result. */
 int cq_server_wait(cq_server* s, long long ticket, int timeout_ms, cq_result** out);
+/* JSON: completed, p50_ms, p99_ms, max_ms, qps, batches, mean_batch,
+ * admission_waits, compiles, rejected, lanes, budget_bytes, in_flight_bytes */
 int cq_server_stats(cq_server* s, char* buf, size_t len, size_t* needed);
+/* Restart the latency / throughput window. */
+int cq_server_reset_stats(cq_server* s);
+/* Stops the lanes; requests still queued fail with "server stopped". */
 void cq_server_stop(cq_server* s);
 
 #ifdef __cplusplus

@@ -82,6 +82,7 @@ def _load() -> ctypes.CDLL:
         "cq_server_submit": (c_ll, [c_void_p, c_char_p]),
         "cq_server_wait": (c_int, [c_void_p, c_ll, c_int, P(c_void_p)]),
         "cq_server_stats": (c_int, [c_void_p, c_char_p, c_sz, P(c_sz)]),
+        "cq_server_reset_stats": (c_int, [c_void_p]),
         "cq_server_stop": (None, [c_void_p]),
     }
     for name, (res, args) in sig.items():
@@ -295,7 +296,10 @@ def _options(options: Optional[dict]) -> Optional[bytes]:
 
 
 class Server:
-    """Continuous-batching query server over a resident Database."""
+    """Continuous-batching query server over a resident Database.
+
+    options: lanes, max_batch, max_templates, budget_mb, window_us,
+    async_compile, bt, vpt, graph."""
 
     def __init__(self, db: Database, options: Optional[dict] = None):
         h = ctypes.c_void_p()
@@ -316,6 +320,9 @@ class Server:
 
     def stats(self) -> dict:
         return json.loads(_string_out(_lib.cq_server_stats, self._h))
+
+    def reset_stats(self) -> None:
+        _check(_lib.cq_server_reset_stats(self._h))
 
     def stop(self) -> None:
         if self._h:

@@ -420,6 +420,14 @@ int cq_server_stats(cq_server* s, char* buf, size_t len, size_t* needed) {
     return guarded([&] { return copyOut(s->server->statsJson(), buf, len, needed); });
 }
 
+int cq_server_reset_stats(cq_server* s) {
+    return guarded([&] {
+        if (!s) return fail(CQ_ERR_INVALID, "cq_server_reset_stats: null server");
+        s->server->resetStats();
+        return CQ_OK;
+    });
+}
+
 void cq_server_stop(cq_server* s) { delete s; }
 
 }  // extern "C"

@@ -161,7 +161,21 @@ DatabaseInstance::DatabaseInstance(int device, EngineConfig config) : device_(de
         if (device_ >= n) throw std::runtime_error("HIP device " + std::to_string(device_) + " not present");
         hipCheck(hipSetDevice(device_), "hipSetDevice");
         hipCheck(hipStreamCreate(&ctx_->stream), "hipStreamCreate");
+        ctx_->owns_stream = true;
     }
+}
+
+std::unique_ptr<ExecutionContext> DatabaseInstance::newContext() {
+    if (device_ < 0) throw std::runtime_error("execution contexts need a GPU device (host-only database)");
+    auto c = std::make_unique<ExecutionContext>();
+    c->config = ctx_->config;
+    c->device = device_;
+    c->columns = &columns_;
+    c->loaded_device_bytes = loaded_bytes_;
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    hipCheck(hipStreamCreate(&c->stream), "hipStreamCreate");
+    c->owns_stream = true;
+    return c;
 }
 
 DatabaseInstance::~DatabaseInstance() {

@@ -77,6 +77,10 @@ public:
     PlanSignatureCache& moduleCache() { return *cache_; }
     HipRtcCompiler& compiler() { return *compiler_; }
     ExecutionContext& context() { return *ctx_; }
+    // Another context on its own stream over the same resident columns,
+    // catalog and module cache (serving lanes).  Columns must not be loaded
+    // or dropped while such contexts run queries.
+    std::unique_ptr<ExecutionContext> newContext();
     hipStream_t stream() const { return ctx_->stream; }
     // Serializes use of the execution context (queries, batches, generation).
     std::mutex& contextMutex() const { return mu_; }

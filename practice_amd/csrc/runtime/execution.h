@@ -117,8 +117,16 @@ private:
 using ModuleHandle = std::shared_ptr<CompiledModule>;
 
 struct ExecutionContext {
+    ExecutionContext() = default;
+    ExecutionContext(const ExecutionContext&) = delete;
+    ExecutionContext& operator=(const ExecutionContext&) = delete;
+    ~ExecutionContext() {
+        scratch_.clear();
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
+    }
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = false;
     EngineConfig config;
     const std::unordered_map<std::string, DeviceColumn>* columns = nullptr;
     size_t loaded_device_bytes = 0;

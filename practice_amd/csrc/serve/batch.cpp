@@ -56,7 +56,19 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
     return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable);
 }
 
-BatchExecutor::BatchExecutor(DatabaseInstance& db) : db_(db) {}
+BatchExecutor::BatchExecutor(DatabaseInstance& db)
+    : db_(db), ctx_(db.hostOnly() ? nullptr : &db.context()), ctx_mu_(&db.contextMutex()) {}
+
+BatchExecutor::BatchExecutor(DatabaseInstance& db, ExecutionContext* ctx, std::mutex* ctx_mutex)
+    : db_(db), ctx_(ctx), ctx_mu_(ctx_mutex) {}
+
+CodegenOptions BatchExecutor::codegenOptions(const BatchOptions& opt) const {
+    CodegenOptions co = db_.config().codegen;
+    co.batch_body = true;
+    co.block_threads = opt.block_threads;
+    co.vectors_per_thread = opt.vectors_per_thread;
+    return co;
+}
 
 BatchExecutor::~BatchExecutor() {
     if (db_.hostOnly()) return;
@@ -92,17 +104,9 @@ std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& 
 
 std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& sqls, const BatchOptions& opt) {
     if (db_.hostOnly()) throw std::runtime_error("batch execution needs a GPU device (host-only database)");
-    std::lock_guard<std::mutex> lock(db_.contextMutex());
-    hipCheck(hipSetDevice(db_.device()), "hipSetDevice");
-    for (auto& e : ev_)
-        if (!e) hipCheck(hipEventCreate(&e), "hipEventCreate");
-    const auto t0 = std::chrono::steady_clock::now();
-
     // plans -> programs carrying the per-tile pipeline body
-    CodegenOptions co = db_.config().codegen;
-    co.batch_body = true;
-    co.block_threads = opt.block_threads;
-    co.vectors_per_thread = opt.vectors_per_thread;
+    const auto t0 = std::chrono::steady_clock::now();
+    const CodegenOptions co = codegenOptions(opt);
     std::vector<QueryProgram> programs(sqls.size());
     for (size_t i = 0; i < sqls.size(); ++i) {
         try {
@@ -111,7 +115,22 @@ std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& 
             throw std::runtime_error("batch query " + std::to_string(i) + ": " + e.what());
         }
     }
-    const double plan_ms = elapsedMs(t0, std::chrono::steady_clock::now());
+    plan_ms_ = elapsedMs(t0, std::chrono::steady_clock::now());
+    return executePrograms(programs, opt, nullptr);
+}
+
+std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram>& programs, const BatchOptions& opt,
+                                                        const std::vector<const QueryProgram*>* module_templates) {
+    if (db_.hostOnly() || !ctx_) throw std::runtime_error("batch execution needs a GPU device (host-only database)");
+    std::lock_guard<std::mutex> lock(*ctx_mu_);
+    hipCheck(hipSetDevice(db_.device()), "hipSetDevice");
+    for (auto& e : ev_)
+        if (!e) hipCheck(hipEventCreate(&e), "hipEventCreate");
+    const auto t0 = std::chrono::steady_clock::now();
+    const double plan_ms = plan_ms_;
+    plan_ms_ = 0;
+    for (const auto& p : programs)
+        if (p.batch_body.empty()) throw std::runtime_error("batch program generated without its per-tile body");
 
     // queries of one template stay together; templates are packed into
     // shared scans of the same fact table within the template / column /
@@ -124,6 +143,12 @@ std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& 
         members[key].push_back(static_cast<int>(i));
     }
     std::vector<std::vector<int>> scans;
+    if (module_templates) {
+        // the caller sized the batch for one launch of its module
+        scans.emplace_back();
+        for (size_t i = 0; i < programs.size(); ++i) scans.back().push_back(static_cast<int>(i));
+        order.clear();
+    }
     std::vector<int> cur;
     std::vector<std::string> cur_cols;
     std::string cur_table;
@@ -153,12 +178,12 @@ std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& 
     }
     if (!cur.empty()) scans.push_back(cur);
 
-    std::vector<QueryResult> results(sqls.size());
+    std::vector<QueryResult> results(programs.size());
     std::ostringstream r;
-    r << "{\"queries\":" << sqls.size() << ",\"shared_scans\":" << scans.size() << ",\"plan_ms\":" << plan_ms << ",\"scans\":[";
+    r << "{\"queries\":" << programs.size() << ",\"shared_scans\":" << scans.size() << ",\"plan_ms\":" << plan_ms << ",\"scans\":[";
     for (size_t s = 0; s < scans.size(); ++s) {
         BatchScanReport rep;
-        runScan(scans[s], programs, opt, static_cast<int>(s), results, rep);
+        runScan(scans[s], programs, opt, static_cast<int>(s), results, rep, module_templates);
         r << (s ? "," : "") << "{\"queries\":" << rep.queries << ",\"templates\":" << rep.templates
           << ",\"union_columns\":" << rep.union_columns << ",\"block_threads\":" << rep.block_threads
           << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
@@ -179,26 +204,32 @@ std::string batchSource(const std::vector<const QueryProgram*>& templates, int b
 }  // namespace
 
 void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt,
-                            int scan_index, std::vector<QueryResult>& results, BatchScanReport& rep) {
-    ExecutionContext* ctx = &db_.context();
+                            int scan_index, std::vector<QueryResult>& results, BatchScanReport& rep,
+                            const std::vector<const QueryProgram*>* module_templates) {
+    ExecutionContext* ctx = ctx_;
     QueryEngine& engine = db_.engine();
     HipModuleExecutor* ex = engine.hipExecutor();
     if (!ex) throw std::runtime_error("batch execution needs the HIP module executor");
     hipStream_t s = ctx->stream;
 
-    // templates in first-seen order, queries grouped by template
+    // module templates (given, or the members' in first-seen order) and
+    // the members grouped by template
     std::vector<const QueryProgram*> templates;
-    std::vector<std::vector<int>> per_template;
+    if (module_templates) templates = *module_templates;
+    std::vector<std::vector<int>> per_template(templates.size());
     for (int q : members) {
         const QueryProgram& p = programs[static_cast<size_t>(q)];
         size_t t = 0;
         while (t < templates.size() && templates[t]->signature != p.signature) ++t;
         if (t == templates.size()) {
+            if (module_templates) throw std::runtime_error("batch query template missing from the shared-scan module");
             templates.push_back(&p);
             per_template.emplace_back();
         }
         per_template[t].push_back(q);
     }
+    if (templates.empty()) return;
+    if (templates.size() > static_cast<size_t>(CQ_MAX_TEMPLATES)) throw std::runtime_error("too many templates in one scan");
     std::vector<int> qorder;
     for (const auto& v : per_template) qorder.insert(qorder.end(), v.begin(), v.end());
     const int n = static_cast<int>(qorder.size());
@@ -207,7 +238,8 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
 
     // per-template modules (dimension builds + result flags)
     std::vector<ModuleHandle> tmodules;
-    for (const auto* t : templates) tmodules.push_back(engine.moduleFor(*t, nullptr));
+    for (size_t t = 0; t < templates.size(); ++t)
+        tmodules.push_back(per_template[t].empty() ? nullptr : engine.moduleFor(*templates[t], nullptr));
 
     // bind every query into its own scratch namespace
     ctx->row_begin = 0;

@@ -16,6 +16,7 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,7 +53,10 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
 
 class BatchExecutor {
 public:
+    // Runs on the database's own context (serialized with its queries).
     explicit BatchExecutor(DatabaseInstance& db);
+    // Runs on `ctx` (its stream and scratch), serialized by `ctx_mutex`.
+    BatchExecutor(DatabaseInstance& db, ExecutionContext* ctx, std::mutex* ctx_mutex);
     ~BatchExecutor();
     BatchExecutor(const BatchExecutor&) = delete;
     BatchExecutor& operator=(const BatchExecutor&) = delete;
@@ -61,6 +65,13 @@ public:
     // same as DatabaseInstance::executeQuery's for each query alone.
     std::vector<QueryResult> execute(const std::vector<std::string>& sqls, const std::string& options);
     std::vector<QueryResult> execute(const std::vector<std::string>& sqls, const BatchOptions& options);
+    // Prepared programs (CodegenOptions::batch_body); `module_templates`, when
+    // given, lists the templates the shared-scan module is generated for (a
+    // superset of the programs' templates: ones without queries are skipped).
+    std::vector<QueryResult> executePrograms(std::vector<QueryProgram>& programs, const BatchOptions& options,
+                                             const std::vector<const QueryProgram*>* module_templates = nullptr);
+    // Codegen options a batch prepares its programs with.
+    CodegenOptions codegenOptions(const BatchOptions& options) const;
     // JSON: per shared scan counts, timings, tuning choice, graph use.
     const std::string& lastReport() const { return report_; }
 
@@ -74,16 +85,20 @@ private:
         int block_threads = 256, vectors_per_thread = 2;
     };
     DatabaseInstance& db_;
+    ExecutionContext* ctx_;
+    std::mutex* ctx_mu_;
     std::string report_;
     std::map<std::string, Graph> graphs_;   // keyed by scan slot
     std::map<std::string, Tuned> tuned_;    // keyed by template-set signature
+    double plan_ms_ = 0;                    // SQL -> programs time of the current call
     void* pinned_ = nullptr;                // staging for parameter blocks / row counts
     size_t pinned_bytes_ = 0;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
 
     void* pinned(size_t bytes);
     void runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt, int scan_index,
-                 std::vector<QueryResult>& results, BatchScanReport& rep);
+                 std::vector<QueryResult>& results, BatchScanReport& rep,
+                 const std::vector<const QueryProgram*>* module_templates);
     Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
                    std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
     void dropGraph(Graph& g);

@@ -1,14 +1,65 @@
-// Query server (serial worker).
+// Query server: lanes of shared-scan batches, footprint admission, async compile.
 #include "server.h"
 
 #include <algorithm>
 #include <sstream>
+#include <stdexcept>
 
 namespace db {
 
-QueryServer::QueryServer(DatabaseInstance& db, const std::string& options) : db_(db) {
-    (void)options;
-    worker_ = std::thread([this] { loop(); });
+ServerOptions parseServerOptions(const std::string& text) {
+    ServerOptions o;
+    std::string batch_opts;
+    std::stringstream ss(text);
+    std::string item;
+    while (std::getline(ss, item, ',')) {
+        if (item.empty()) continue;
+        const size_t eq = item.find('=');
+        if (eq == std::string::npos) throw std::runtime_error("server option without value: " + item);
+        const std::string k = item.substr(0, eq);
+        const long long v = std::stoll(item.substr(eq + 1));
+        if (k == "lanes") o.lanes = static_cast<int>(v);
+        else if (k == "max_batch") o.max_batch = static_cast<int>(v);
+        else if (k == "max_templates") o.max_templates = static_cast<int>(v);
+        else if (k == "budget_mb") o.budget_bytes = static_cast<size_t>(v) << 20;
+        else if (k == "window_us") o.window_us = static_cast<int>(v);
+        else if (k == "async_compile") o.async_compile = v != 0;
+        else if (k == "bt" || k == "vpt" || k == "graph") batch_opts += (batch_opts.empty() ? "" : ",") + item;
+        else throw std::runtime_error("unknown server option: " + k);
+    }
+    o.batch = parseBatchOptions(batch_opts);
+    if (o.lanes < 1 || o.lanes > 8) throw std::runtime_error("server lanes must be in [1, 8]");
+    if (o.max_batch < 1) throw std::runtime_error("server max_batch must be positive");
+    o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
+    return o;
+}
+
+QueryServer::QueryServer(DatabaseInstance& db, const std::string& options) : QueryServer(db, parseServerOptions(options)) {}
+
+QueryServer::QueryServer(DatabaseInstance& db, const ServerOptions& options) : db_(db), opt_(options) {
+    if (db_.hostOnly()) throw std::runtime_error("the query server needs a GPU device (host-only database)");
+    if (opt_.budget_bytes == 0) {
+        size_t free_b = 0, total = 0;
+        hipCheck(hipSetDevice(db_.device()), "hipSetDevice");
+        hipCheck(hipMemGetInfo(&free_b, &total), "hipMemGetInfo");
+        opt_.budget_bytes = free_b / 4;
+    }
+    stats_start_ = std::chrono::steady_clock::now();
+    start();
+}
+
+void QueryServer::start() {
+    for (int i = 0; i < opt_.lanes; ++i) {
+        auto lane = std::make_unique<Lane>();
+        lane->ctx = db_.newContext();
+        lane->exec = std::make_unique<BatchExecutor>(db_, lane->ctx.get(), &lane->mu);
+        lanes_.push_back(std::move(lane));
+    }
+    for (auto& l : lanes_) {
+        Lane* lp = l.get();
+        l->thread = std::thread([this, lp] { laneLoop(lp); });
+    }
+    if (opt_.async_compile) compiler_ = std::thread([this] { compileLoop(); });
 }
 
 QueryServer::~QueryServer() {
@@ -17,41 +68,238 @@ QueryServer::~QueryServer() {
         stop_ = true;
     }
     cv_.notify_all();
-    if (worker_.joinable()) worker_.join();
+    compile_cv_.notify_all();
+    for (auto& l : lanes_)
+        if (l->thread.joinable()) l->thread.join();
+    if (compiler_.joinable()) compiler_.join();
+    // requests still queued when the server stops fail
+    std::lock_guard<std::mutex> lock(mu_);
+    for (auto* q : {&queue_, &compile_q_}) {
+        for (auto& r : *q) {
+            Done d;
+            d.rc = -3;
+            d.err = "server stopped";
+            done_[r.ticket] = std::move(d);
+        }
+        q->clear();
+    }
+    done_cv_.notify_all();
 }
 
 long long QueryServer::submit(const std::string& sql) {
     std::lock_guard<std::mutex> lock(mu_);
-    const long long t = next_++;
-    queue_.push_back({t, Request{sql, std::chrono::steady_clock::now()}});
+    if (stop_) throw std::runtime_error("server stopped");
+    Request r;
+    r.ticket = next_++;
+    r.sql = sql;
+    r.arrival = std::chrono::steady_clock::now();
+    queue_.push_back(std::move(r));
     cv_.notify_one();
-    return t;
+    return next_ - 1;
 }
 
-void QueryServer::loop() {
+void QueryServer::finish(Request& r, int rc, const std::string& err, QueryResult* result) {
+    Done d;
+    d.rc = rc;
+    d.err = err;
+    if (result) d.result = std::move(*result);
+    d.latency_ms = elapsedMs(r.arrival, std::chrono::steady_clock::now());
+    std::lock_guard<std::mutex> lock(mu_);
+    if (rc == 0) latencies_.push_back(d.latency_ms);
+    else ++rejected_;
+    done_[r.ticket] = std::move(d);
+    done_cv_.notify_all();
+}
+
+bool QueryServer::prepare(Request& r) {
+    try {
+        BatchOptions bo = opt_.batch;
+        CodegenOptions co = db_.config().codegen;
+        co.batch_body = true;
+        co.block_threads = bo.block_threads;
+        co.vectors_per_thread = bo.vectors_per_thread;
+        r.program = std::make_shared<QueryProgram>(db_.engine().prepare(r.sql, co));
+        r.footprint = estimateQueryTemporaryBytes(*r.program);
+        if (r.footprint > opt_.budget_bytes) {
+            finish(r, -5, "query needs " + std::to_string(r.footprint) + " bytes of device memory, over the serving budget of " +
+                              std::to_string(opt_.budget_bytes),
+                   nullptr);
+            return false;
+        }
+        return true;
+    } catch (const std::exception& e) {
+        finish(r, -4, e.what(), nullptr);
+        return false;
+    }
+}
+
+bool QueryServer::inScanSet(const std::string& signature) const {
+    if (!scan_set_ready_) return false;
+    for (const auto& t : scan_set_)
+        if (t->signature == signature) return true;
+    return false;
+}
+
+void QueryServer::laneLoop(Lane* lane) {
     for (;;) {
-        std::pair<long long, Request> item;
+        std::vector<Request> taken;
         {
             std::unique_lock<std::mutex> lock(mu_);
             cv_.wait(lock, [&] { return stop_ || !queue_.empty(); });
-            if (stop_ && queue_.empty()) return;
-            item = std::move(queue_.front());
-            queue_.pop_front();
+            if (stop_) return;
+            if (opt_.window_us > 0 && static_cast<int>(queue_.size()) < opt_.max_batch) {
+                lock.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(opt_.window_us));
+                lock.lock();
+                if (stop_) return;
+            }
+            while (!queue_.empty() && static_cast<int>(taken.size()) < opt_.max_batch) {
+                taken.push_back(std::move(queue_.front()));
+                queue_.pop_front();
+            }
         }
-        Done d;
-        try {
-            d.result = db_.executeQuery(item.second.sql);
-        } catch (const std::exception& e) {
-            d.rc = -3;
-            d.err = e.what();
-        }
-        d.latency_ms = elapsedMs(item.second.arrival, std::chrono::steady_clock::now());
+        // plan + codegen outside the lock
+        std::vector<Request> ready;
+        for (auto& r : taken)
+            if (r.program || prepare(r)) ready.push_back(std::move(r));
+        if (ready.empty()) continue;
+
+        // runnable = template compiled into the current scan set; admission
+        // reserves each request's footprint against the budget
+        std::vector<Request> batch, later, cold;
+        std::vector<std::shared_ptr<QueryProgram>> set;
         {
             std::lock_guard<std::mutex> lock(mu_);
-            latencies_.push_back(d.latency_ms);
-            done_[item.first] = std::move(d);
+            set = scan_set_;
+            for (auto& r : ready) {
+                if (!inScanSet(r.program->signature)) {
+                    cold.push_back(std::move(r));
+                } else if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
+                    in_flight_bytes_ += r.footprint;
+                    batch.push_back(std::move(r));
+                } else {
+                    later.push_back(std::move(r));
+                }
+            }
+            if (!later.empty()) {
+                ++admission_waits_;
+                for (auto it = later.rbegin(); it != later.rend(); ++it) queue_.push_front(std::move(*it));
+            }
+            if (!cold.empty() && opt_.async_compile) {
+                for (auto& r : cold) compile_q_.push_back(std::move(r));
+                cold.clear();
+                compile_cv_.notify_one();
+            }
         }
-        done_cv_.notify_all();
+        if (!cold.empty()) {
+            // synchronous compilation on this lane, then back to the queue
+            for (auto& r : cold) {
+                std::lock_guard<std::mutex> lock(mu_);
+                compile_q_.push_back(std::move(r));
+            }
+            compileLoop();
+            continue;
+        }
+        if (batch.empty()) {
+            // nothing admissible right now: wait for memory to free up
+            std::unique_lock<std::mutex> lock(mu_);
+            const size_t before = in_flight_bytes_;
+            cv_.wait_for(lock, std::chrono::milliseconds(5), [&] { return stop_ || in_flight_bytes_ < before; });
+            continue;
+        }
+
+        std::vector<QueryProgram> programs;
+        programs.reserve(batch.size());
+        for (auto& r : batch) programs.push_back(*r.program);
+        std::vector<const QueryProgram*> templates;
+        for (const auto& t : set) templates.push_back(t.get());
+        std::vector<QueryResult> results;
+        std::string err;
+        try {
+            results = lane->exec->executePrograms(programs, opt_.batch, &templates);
+        } catch (const std::exception& e) {
+            err = e.what();
+        }
+        size_t released = 0;
+        for (size_t i = 0; i < batch.size(); ++i) {
+            released += batch[i].footprint;
+            if (err.empty()) finish(batch[i], 0, "", &results[i]);
+            else finish(batch[i], -3, err, nullptr);
+        }
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            in_flight_bytes_ -= released;
+            ++batches_;
+            batched_queries_ += static_cast<long long>(batch.size());
+        }
+        cv_.notify_all();
+    }
+}
+
+void QueryServer::compileLoop() {
+    const bool inline_call = !opt_.async_compile;
+    for (;;) {
+        std::vector<Request> work;
+        {
+            std::unique_lock<std::mutex> lock(mu_);
+            if (!inline_call) compile_cv_.wait(lock, [&] { return stop_ || !compile_q_.empty(); });
+            if (stop_) return;
+            while (!compile_q_.empty()) {
+                work.push_back(std::move(compile_q_.front()));
+                compile_q_.pop_front();
+            }
+            if (work.empty()) return;
+        }
+        // template modules, then the shared-scan module of the grown set
+        std::vector<std::shared_ptr<QueryProgram>> set;
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            set = scan_set_;
+        }
+        std::vector<Request> ok;
+        for (auto& r : work) {
+            try {
+                db_.engine().moduleFor(*r.program, nullptr);
+                ++compiles_;
+                bool present = false;
+                for (const auto& t : set) present = present || t->signature == r.program->signature;
+                if (!present) {
+                    if (!set.empty() && set.front()->fact_table != r.program->fact_table)
+                        throw std::runtime_error("a server scans one fact table (" + set.front()->fact_table + ")");
+                    if (static_cast<int>(set.size()) >= opt_.max_templates) set.erase(set.begin());
+                    set.push_back(r.program);
+                }
+                ok.push_back(std::move(r));
+            } catch (const std::exception& e) {
+                finish(r, -2, e.what(), nullptr);
+            }
+        }
+        try {
+            if (!set.empty()) {
+                std::vector<const QueryProgram*> tp;
+                for (const auto& t : set) tp.push_back(t.get());
+                std::vector<std::string> cols;
+                std::vector<bool> nullable;
+                const std::string src =
+                    generateBatchScanSource(tp, opt_.batch.block_threads, opt_.batch.vectors_per_thread, cols, nullable);
+                db_.engine().moduleForSource(src, signatureOf(src), nullptr);
+            }
+        } catch (const std::exception& e) {
+            for (auto& r : ok) finish(r, -2, e.what(), nullptr);
+            ok.clear();
+            set.clear();
+        }
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            if (!set.empty()) {
+                scan_set_ = set;
+                scan_set_ready_ = true;
+            }
+            // requests whose template fell out of the set go round again
+            for (auto it = ok.rbegin(); it != ok.rend(); ++it) queue_.push_front(std::move(*it));
+        }
+        cv_.notify_all();
+        if (inline_call) return;
     }
 }
 
@@ -70,16 +318,35 @@ int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::s
         return d.rc;
     }
     out = std::move(d.result);
+    out.timing.engine_ms = d.latency_ms;
     return 0;
+}
+
+void QueryServer::resetStats() {
+    std::lock_guard<std::mutex> lock(mu_);
+    latencies_.clear();
+    batches_ = batched_queries_ = admission_waits_ = compiles_ = rejected_ = 0;
+    stats_start_ = std::chrono::steady_clock::now();
 }
 
 std::string QueryServer::statsJson() {
     std::lock_guard<std::mutex> lock(mu_);
     std::vector<double> l = latencies_;
     std::sort(l.begin(), l.end());
-    auto pct = [&](double q) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, static_cast<size_t>(q * l.size()))]; };
+    auto pct = [&](double q) {
+        if (l.empty()) return 0.0;
+        const size_t i = static_cast<size_t>(q * static_cast<double>(l.size() - 1) + 0.5);
+        return l[std::min(l.size() - 1, i)];
+    };
+    const double secs = elapsedMs(stats_start_, std::chrono::steady_clock::now()) / 1000.0;
     std::ostringstream o;
-    o << "{\"completed\":" << l.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p99_ms\":" << pct(0.99) << "}";
+    o << "{\"completed\":" << l.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p99_ms\":" << pct(0.99)
+      << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
+      << ",\"batches\":" << batches_ << ",\"mean_batch\":"
+      << (batches_ ? static_cast<double>(batched_queries_) / static_cast<double>(batches_) : 0.0)
+      << ",\"admission_waits\":" << admission_waits_ << ",\"compiles\":" << compiles_ << ",\"rejected\":" << rejected_
+      << ",\"lanes\":" << opt_.lanes << ",\"budget_bytes\":" << opt_.budget_bytes << ",\"in_flight_bytes\":" << in_flight_bytes_
+      << ",\"scan_set\":" << scan_set_.size() << "}";
     return o.str();
 }
 

@@ -1,30 +1,74 @@
-// Query server: request queue in front of the database.
+// Query server: continuous batching over resident columns.
+//
+// No reference counterpart (the reference executes one query per call,
+// src/core/execution.cpp:652-725).  Requests queue on submit().  Each lane
+// (own stream, scratch and shared-scan executor) repeatedly takes every
+// queued request it can admit, up to max_batch, and runs them as one shared
+// scan; while one lane's batch is on the GPU the next lane admits the
+// requests that arrived meanwhile, so batches overlap on the device.
+//
+// Admission: a request's device footprint (hash tables, aggregate slots,
+// materialization buffers; estimateQueryTemporaryBytes) is reserved against
+// the HBM budget for as long as it is in flight; requests that do not fit
+// wait, requests larger than the whole budget fail.
+//
+// Kernel cache: requests whose plan signature has no compiled module (or
+// is not yet part of the compiled shared-scan set) go to a compile thread
+// and rejoin the queue when their modules are ready, so one ad-hoc query's
+// hipRTC compilation never stalls the others.
 #pragma once
 
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
+#include <vector>
 
-#include "../runtime/database.h"
+#include "batch.h"
 
 namespace db {
+
+struct ServerOptions {
+    int lanes = 2;                 // concurrent batches on the GPU (own stream each)
+    int max_batch = 64;            // requests per shared scan
+    int max_templates = 16;        // plan signatures per shared-scan module
+    size_t budget_bytes = 0;       // HBM for query temporaries (0: quarter of free memory)
+    int window_us = 0;             // wait for more arrivals before taking a batch
+    bool async_compile = true;     // compile cache misses off the lanes
+    BatchOptions batch;
+};
+
+// "key=value,..." with lanes, max_batch, max_templates, budget_mb, window_us,
+// async_compile, bt, vpt, graph
+ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
 public:
     QueryServer(DatabaseInstance& db, const std::string& options);
+    QueryServer(DatabaseInstance& db, const ServerOptions& options);
     ~QueryServer();
+    QueryServer(const QueryServer&) = delete;
+    QueryServer& operator=(const QueryServer&) = delete;
+
     long long submit(const std::string& sql);
     // 0 on success, negative error code otherwise (err receives the message)
     int wait(long long ticket, int timeout_ms, QueryResult& out, std::string& err);
+    // completed, p50 / p99 / max latency, qps since start, batches, admission
+    // waits, compiles, rejected requests
     std::string statsJson();
+    void resetStats();
 
 private:
     struct Request {
+        long long ticket = 0;
         std::string sql;
         std::chrono::steady_clock::time_point arrival;
+        std::shared_ptr<QueryProgram> program;   // prepared (batch body)
+        size_t footprint = 0;
+        bool compiled = false;                   // template module + scan set ready
     };
     struct Done {
         int rc = 0;
@@ -32,16 +76,42 @@ private:
         QueryResult result;
         double latency_ms = 0;
     };
+    struct Lane {
+        std::unique_ptr<ExecutionContext> ctx;
+        std::mutex mu;
+        std::unique_ptr<BatchExecutor> exec;
+        std::thread thread;
+    };
+
     DatabaseInstance& db_;
+    ServerOptions opt_;
     std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
-    std::deque<std::pair<long long, Request>> queue_;
+    std::condition_variable cv_, done_cv_, compile_cv_;
+    std::deque<Request> queue_;        // runnable or not yet prepared
+    std::deque<Request> compile_q_;    // waiting for modules
     std::map<long long, Done> done_;
     long long next_ = 0;
     bool stop_ = false;
-    std::thread worker_;
+    size_t in_flight_bytes_ = 0;
+
+    // templates of the compiled shared-scan set (signature order = kernel order)
+    std::vector<std::shared_ptr<QueryProgram>> scan_set_;
+    bool scan_set_ready_ = false;
+
+    std::vector<std::unique_ptr<Lane>> lanes_;
+    std::thread compiler_;
+
+    // statistics
     std::vector<double> latencies_;
-    void loop();
+    std::chrono::steady_clock::time_point stats_start_;
+    long long batches_ = 0, batched_queries_ = 0, admission_waits_ = 0, compiles_ = 0, rejected_ = 0;
+
+    void start();
+    void laneLoop(Lane* lane);
+    void compileLoop();
+    void finish(Request& r, int rc, const std::string& err, QueryResult* result);
+    bool prepare(Request& r);
+    bool inScanSet(const std::string& signature) const;
 };
 
 }  // namespace db
