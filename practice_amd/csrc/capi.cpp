@@ -295,7 +295,9 @@ int cq_db_describe(cq_db* d, const char* sql, char* buf, size_t len, size_t* nee
             const auto& jp = p.joins[j];
             o << (j ? "," : "") << "{\"dim\":\"" << jp.dim_table << "\",\"pk\":\"" << jp.pk_col << "\",\"fk\":\"" << jp.fk_col
               << "\",\"payload\":\"" << jp.payload_col << "\",\"kmin\":" << jp.kmin << ",\"len\":" << jp.len
-              << ",\"compact\":" << (jp.compact ? 1 : 0) << ",\"entry_bits\":" << jp.eb << ",\"tag_bits\":" << jp.tb
+              << ",\"layout\":\"" << (jp.dense ? "key-indexed" : (jp.compact ? "compact" : "reference"))
+              << "\",\"compact\":" << (jp.compact ? 1 : 0) << ",\"entry_bits\":" << (jp.dense ? jp.db : jp.eb)
+              << ",\"tag_bits\":" << (jp.dense ? 0 : jp.tb) << ",\"absent_code\":" << (jp.dense ? jp.doff : 1)
               << ",\"lds\":" << (jp.lds ? 1 : 0) << ",\"bytes\":" << jp.ht_words * 4 << "}";
         }
         o << "],\"outputs\":[";

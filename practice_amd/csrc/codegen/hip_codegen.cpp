@@ -242,6 +242,9 @@ private:
     const SortLimitNode* sort_ = nullptr;
     std::vector<Op> ops_;
     std::vector<const ExprNode*> build_pred_;
+    bool build_filtered_ = false;  // join being analysed has a build-side filter
+    int flag_w_ = 0;                // LDS flag bits per group (0: LDS aggregation impossible)
+    std::vector<int> flag_dyn_, flag_static_;
     std::map<std::string, int> fact_slot_;
     std::map<std::string, int> dim_slot_;
     int visible_joins_ = 0;
@@ -353,6 +356,7 @@ private:
         jp.dim_rows = static_cast<int64_t>(m.getSize());
         bpred = build_side->getType() == OperatorType::FILTER ? static_cast<const FilterNode*>(build_side)->predicate.get() : nullptr;
         if (bpred && !buildFilterSupported(bpred)) unsupported("Unsupported build-side filter predicate for block codegen");
+        build_filtered_ = bpred != nullptr;
         const std::string& t = jp.dim_table;
         if (t == "PART" || t == "SUPPLIER" || t == "CUSTOMER") {
             jp.len = jp.dim_rows;
@@ -453,6 +457,60 @@ private:
         }
         if (jp.compact) jp.ht_words = (static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32;
         else jp.ht_words = 2 * static_cast<uint64_t>(jp.len);
+        // key-indexed candidate: same key conditions as the compact layout
+        jp.dense_ok = false;
+        if (opt_.allow_dense_tables && jp.compact) {
+            const auto* ks2 = stats(jp.pk_col);
+            jp.dmin = ks2->min_value_;
+            jp.dspan = ks2->max_value_ - ks2->min_value_ + 1;
+            const auto& tm = meta(jp.dim_table);
+            const bool all_present = !build_filtered_ && static_cast<uint64_t>(jp.dspan) == tm.getSize() &&
+                                     ks2->cardinality_ == tm.getSize();
+            jp.doff = all_present ? 0 : 1;
+            uint64_t range = 0;
+            if (jp.pair) {
+                const auto* ps = stats(jp.payload_col);
+                range = static_cast<uint64_t>(ps->max_value_ - ps->min_value_);
+            }
+            jp.dbits = jp.pair ? bitsFor(range + static_cast<uint64_t>(jp.doff)) : (all_present ? 0 : 1);
+            jp.dense_ok = jp.dbits <= 31 && jp.dspan > 0 && jp.dspan <= (int64_t(1) << 31);
+        }
+    }
+
+    static int pow2Width(int bits) {
+        int w = bits <= 0 ? 0 : 1;
+        while (w && w < bits) w <<= 1;
+        return w;
+    }
+    static uint64_t denseWords(const JoinProgram& jp, int width) {
+        if (width == 0) return 0;
+        return (static_cast<uint64_t>(jp.dspan) * static_cast<uint64_t>(width) + 31) / 32 + 1;  // + padding word
+    }
+    // Final table layout once LDS placement is known: key-indexed when it is
+    // no larger than the compact table (exact widths in LDS, power-of-two
+    // widths in HBM so a probe is one load).
+    void finalizeLayout(JoinProgram& jp, bool lds) {
+        jp.lds = lds;
+        jp.dense = false;
+        if (jp.dense_ok) {
+            const int w = lds ? jp.dbits : pow2Width(jp.dbits);
+            const uint64_t dw = denseWords(jp, w);
+            const uint64_t cw = (static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32;
+            if (w <= 31 && dw <= cw) {
+                jp.dense = true;
+                jp.db = w;
+                jp.ht_words = dw;
+                return;
+            }
+        }
+        if (jp.compact) jp.ht_words = (static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32;
+    }
+    int64_t densePbase(const JoinProgram& jp) const { return (jp.pair ? jp.pmin : 0) - jp.doff; }
+    // LDS bytes the table needs when placed there (0: not placeable)
+    uint64_t ldsBytes(const JoinProgram& jp) const {
+        if (jp.dense_ok) return denseWords(jp, jp.dbits) * 4;
+        if (jp.compact) return ((static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32) * 4;
+        return 0;
     }
 
     void analyzeAggregate() {
@@ -559,7 +617,11 @@ private:
             if (gexpr->getType() == ExprType::COLUMN_REF) {
                 const auto& col = static_cast<const ColumnRefExpr*>(gexpr.get())->column_name;
                 const auto* s = stats(col);
-                nul = nullable(col);
+                // dimension columns arrive in payload registers, which carry
+                // no validity: their NULL code can never occur
+                bool payload = false;
+                for (const auto& jp : prog_.joins) payload = payload || jp.payload_col == col;
+                nul = nullable(col) && !payload;
                 if (s) {
                     mn = s->min_value_;
                     card = s->cardinality_;
@@ -840,7 +902,10 @@ private:
             src << "        if (!chip::bit32(P.dvalid[" << jp.pk_dslot << "], r)) continue;\n";
         src << "        const int k = P.dcol[" << jp.pk_dslot << "][r];\n";
         src << "        const int pay = " << (jp.pair ? "P.dcol[" + std::to_string(jp.payload_dslot) + "][r]" : "0") << ";\n";
-        if (jp.compact)
+        if (jp.dense)
+            src << "        chip::ht_insert_dense(P.ht[" << j << "], k, " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
+                << densePbase(jp) << ", pay);\n";
+        else if (jp.compact)
             src << "        chip::ht_insert_compact(P.ht[" << j << "], k, " << jp.kmin << ", " << jp.len << ", " << jp.eb << ", "
                 << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", pay);\n";
         else
@@ -855,29 +920,47 @@ private:
         src << "#include \"crystal_hip.h\"\n\n";
         for (int j = 0; j < static_cast<int>(prog_.joins.size()); ++j) emitBuild(src, j);
 
-        const int ng = prog_.ngroups;
-        const int sts = prog_.storage_ts;
-        const int nas = sts - ng;  // aggregate storage slots
         const bool grouped = prog_.grouped;
         // LDS aggregation when the domain decodes from the group index
         const bool batch = opt_.batch_body;
-        bool lds_agg = grouped && opt_.allow_lds_aggregation && sts <= 31 && !batch;
+        computeFlagLayout();
+        bool lds_agg = grouped && opt_.allow_lds_aggregation && !batch && flag_w_ > 0 && prog_.storage_ts <= 64;
         for (size_t g = 0; g < gdense_.size(); ++g) lds_agg = lds_agg && gdense_[g];
-        const uint64_t agg_lds = grouped ? static_cast<uint64_t>(prog_.groups) * (static_cast<uint64_t>(nas) * 8 + 4) : 0;
+        const uint64_t agg_lds = grouped ? aggLdsBytes() : 0;
         if (agg_lds > opt_.lds_budget) lds_agg = false;
         prog_.lds_aggregation = lds_agg;
         uint64_t lds_used = lds_agg ? agg_lds : 0;
-        // LDS placement of compact tables, smallest first
-        std::vector<int> order;
-        for (int j = 0; j < static_cast<int>(prog_.joins.size()); ++j) order.push_back(j);
-        std::sort(order.begin(), order.end(), [&](int x, int y) { return prog_.joins[x].ht_words < prog_.joins[y].ht_words; });
-        for (int j : order) {
-            auto& jp = prog_.joins[static_cast<size_t>(j)];
-            const uint64_t bytes = jp.ht_words * 4;
-            if (!batch && opt_.allow_lds_tables && jp.compact && lds_used + bytes <= opt_.lds_budget) {
-                jp.lds = true;
-                lds_used += bytes;
+        // LDS placement of the dimension tables: the subset that avoids the
+        // most HBM-side gathers within the budget (tables beyond the 32 KiB
+        // L1 weigh more: their probes go to L2), then the final layouts
+        const int nj = static_cast<int>(prog_.joins.size());
+        int best_mask = 0;
+        if (!batch && opt_.allow_lds_tables) {
+            int64_t best_w = 0;
+            uint64_t best_b = 0;
+            for (int m = 1; m < (1 << nj); ++m) {
+                uint64_t b = 0;
+                int64_t w = 0;
+                bool ok = true;
+                for (int j = 0; j < nj && ok; ++j) {
+                    if (!((m >> j) & 1)) continue;
+                    const uint64_t lb = ldsBytes(prog_.joins[static_cast<size_t>(j)]);
+                    if (lb == 0) ok = false;
+                    b += lb;
+                    w += lb > 32 * 1024 ? 8 : 1;
+                }
+                if (!ok || lds_used + b > opt_.lds_budget) continue;
+                if (w > best_w || (w == best_w && b < best_b)) {
+                    best_w = w;
+                    best_b = b;
+                    best_mask = m;
+                }
             }
+        }
+        for (int j = 0; j < nj; ++j) {
+            auto& jp = prog_.joins[static_cast<size_t>(j)];
+            finalizeLayout(jp, (best_mask >> j) & 1);
+            if (jp.lds) lds_used += jp.ht_words * 4;
         }
         prog_.scan_lds_bytes = lds_used;
         int bt = opt_.block_threads;
@@ -916,7 +999,11 @@ private:
                      << "            if (!((fl >> i) & 1u)) continue;\n";
                 if (prog_.fact_nullable[static_cast<size_t>(fs)])
                     body << "            if (!((v" << fs << " >> i) & 1u)) { fl &= ~(1u << i); continue; }\n";
-                if (jp.compact)
+                if (jp.dense)
+                    body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ">("
+                         << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
+                         << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j << "[i])) fl &= ~(1u << i);\n";
+                else if (jp.compact)
                     body << "            if (!chip::ht_probe_compact(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
                          << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j
                          << "[i])) fl &= ~(1u << i);\n";
@@ -1163,11 +1250,64 @@ private:
         for (int i = 0; i < prog_.naggs; ++i) rows.push_back(aggRow(i));
     }
 
+    // LDS flag bits per group: bit 0 = group hit, then one bit per storage
+    // slot whose validity depends on the row ("dynamic"); slots valid on
+    // every hit ("static") are derived from the hit bit at the merge.
+    void computeFlagLayout() {
+        flag_dyn_.clear();
+        flag_static_.clear();
+        flag_w_ = 0;
+        if (!prog_.grouped) return;
+        const auto* a = agg_;
+        // aggregation runs after every probe: all payload registers visible
+        const int saved_visible = visible_joins_;
+        visible_joins_ = static_cast<int>(prog_.joins.size());
+        if (!prog_.fast_sparse) {
+            for (int g = 0; g < prog_.ngroups; ++g) {
+                const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
+                if (litTrue(v)) flag_static_.push_back(g);
+                else if (!litFalse(v)) flag_dyn_.push_back(g);
+            }
+            for (int i = 0; i < prog_.naggs; ++i) {
+                const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+                const int slot = prog_.ngroups + i;
+                if (f == "COUNT") flag_static_.push_back(slot);
+                else if (prog_.hidden_slot[static_cast<size_t>(i)] < 0) {
+                    const std::string v = aggRow(i).valid;
+                    if (litTrue(v)) flag_static_.push_back(slot);
+                    else if (!litFalse(v)) flag_dyn_.push_back(slot);
+                }
+            }
+        }
+        visible_joins_ = saved_visible;
+        const int need = 1 + static_cast<int>(flag_dyn_.size());
+        if (need > 32) return;  // no LDS aggregation
+        flag_w_ = 1;
+        while (flag_w_ < need) flag_w_ <<= 1;
+    }
+    uint64_t aggLdsBytes() const {
+        const uint64_t G = static_cast<uint64_t>(prog_.groups);
+        const uint64_t nas = static_cast<uint64_t>(prog_.storage_ts - prog_.ngroups);
+        const uint64_t w = static_cast<uint64_t>(std::max(flag_w_, 1));
+        return G * nas * 8 + ((G * w + 31) / 32) * 4;
+    }
+    int dynBit(int slot) const {
+        for (size_t k = 0; k < flag_dyn_.size(); ++k)
+            if (flag_dyn_[k] == slot) return 1 + static_cast<int>(k);
+        return -1;
+    }
+
     void emitLdsAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
         const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
         const int64_t G = prog_.groups;
+        const int W = flag_w_;
+        const int per = 32 / W;
+        int lp = 0;
+        while ((1 << lp) < per) ++lp;
+        const int64_t FW = (G * W + 31) / 32;
+        const std::string mask = W == 32 ? "0xffffffffu" : std::to_string((1u << W) - 1u) + "u";
         pro << "    __shared__ u64 s_agg[" << G * nas << "];\n"
-            << "    __shared__ u32 s_flag[" << G << "];\n";
+            << "    __shared__ u32 s_flag[" << FW << "];  // " << W << " flag bits per group\n";
         // MIN slots start at ~0
         std::string init = "0ull";
         if (!prog_.min_slots.empty()) {
@@ -1176,35 +1316,43 @@ private:
             init = "((" + cond + ") ? ~0ull : 0ull)";
         }
         pro << "    for (int k = tid; k < " << G * nas << "; k += BT) s_agg[k] = " << init << ";\n"
-            << "    for (int k = tid; k < " << G << "; k += BT) s_flag[k] = 0u;\n";
+            << "    for (int k = tid; k < " << FW << "; k += BT) s_flag[k] = 0u;\n";
         code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
              << "            if (!((fl >> i) & 1u)) continue;\n"
              << "            const int h = " << groupHash() << ";\n"
-             << "            if ((unsigned)h >= " << G << "u) continue;\n";
-        std::vector<AggRow> rows;
-        rowSlotBits(code, "            ", rows);
+             << "            if ((unsigned)h >= " << G << "u) continue;\n"
+             << "            u32 want = 1u;\n";
+        const auto* a = agg_;
+        for (int g : flag_dyn_) {
+            if (g >= ng) continue;
+            code << "            if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") want |= 1u << " << dynBit(g)
+                 << ";\n";
+        }
         for (int i = 0; i < prog_.naggs; ++i) {
+            const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             const int slot = ng + i;
             const std::string ref = "s_agg[h * " + std::to_string(nas) + " + " + std::to_string(slot - ng) + "]";
-            code << "            if (" << rows[static_cast<size_t>(i)].valid << ") {\n";
-            if (f == "MIN") code << "                atomicMin(&" << ref << ", " << rows[static_cast<size_t>(i)].value << ");\n";
-            else if (f == "MAX") code << "                atomicMax(&" << ref << ", " << rows[static_cast<size_t>(i)].value << ");\n";
-            else code << "                atomicAdd(&" << ref << ", " << rows[static_cast<size_t>(i)].value << ");\n";
+            code << "            if (" << r.valid << ") {\n";
+            if (f == "MIN") code << "                atomicMin(&" << ref << ", " << r.value << ");\n";
+            else if (f == "MAX") code << "                atomicMax(&" << ref << ", " << r.value << ");\n";
+            else code << "                atomicAdd(&" << ref << ", " << r.value << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
             if (hs >= 0) code << "                atomicAdd(&s_agg[h * " << nas << " + " << (hs - ng) << "], 1ull);\n";
-            else if (f != "COUNT" && !prog_.fast_sparse) code << "                want |= 1u << " << slot << ";\n";
+            const int db = dynBit(slot);
+            if (db > 0) code << "                want |= 1u << " << db << ";\n";
             code << "            }\n";
-            if (f == "COUNT" && !prog_.fast_sparse) code << "            want |= 1u << " << slot << ";\n";
         }
-        code << "            if ((s_flag[h] & want) != want) atomicOr(&s_flag[h], want);\n"
+        code << "            { u32* fw = &s_flag[h >> " << lp << "]; const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
+             << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n"
              << "        }\n";
         // merge into the global slots once per block
         epi << "    __syncthreads();\n"
             << "    for (int g = tid; g < " << G << "; g += BT) {\n"
-            << "        const u32 f = s_flag[g];\n"
-            << "        if (!(f & 0x80000000u)) continue;\n"
+            << "        const u32 f = (s_flag[g >> " << lp << "] >> ((u32)(g & " << (per - 1) << ") * " << W << "u)) & " << mask
+            << ";\n"
+            << "        if (!(f & 1u)) continue;\n"
             << "        const long long base = (long long)g * " << sts << ";\n"
             << "        int rem = g;\n";
         for (int g = ng - 1; g >= 0; --g) {
@@ -1227,8 +1375,16 @@ private:
             else if (f == "MAX") epi << "        { const u64 x = " << v << "; if (x) atomicMax(&P.result[base + " << s << "], x); }\n";
             else epi << "        { const u64 x = " << v << "; if (x) atomicAdd(&P.result[base + " << s << "], x); }\n";
         }
-        if (!prog_.fast_sparse)
-            epi << "        for (int c = 0; c < " << sts << "; ++c) if ((f >> c) & 1u) chip::set_bit64(P.result_valid, base + c);\n";
+        if (!prog_.fast_sparse && (!flag_static_.empty() || !flag_dyn_.empty())) {
+            // the group's validity bits are contiguous: one or two atomicOr
+            unsigned long long smask = 0;
+            for (int s : flag_static_) smask |= 1ull << s;
+            epi << "        u64 vm = " << smask << "ull;\n";
+            for (int s : flag_dyn_) epi << "        vm |= (u64)((f >> " << dynBit(s) << ") & 1u) << " << s << ";\n";
+            epi << "        { const long long b0 = base; const u32 o = (u32)(b0 & 63);\n"
+                << "          atomicOr(&P.result_valid[b0 >> 6], vm << o);\n"
+                << "          if (o + " << sts << "u > 64u) atomicOr(&P.result_valid[(b0 >> 6) + 1], vm >> (64u - o)); }\n";
+        }
         epi << "    }\n";
     }
 

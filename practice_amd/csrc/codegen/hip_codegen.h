@@ -27,10 +27,11 @@ namespace db {
 struct CodegenOptions {
     int block_threads = 0;            // 0: chosen from the LDS footprint
     int vectors_per_thread = 2;       // 128-bit loads per column per lane per tile
-    uint64_t lds_budget = 96 * 1024;  // bytes of LDS a scan block may use
+    uint64_t lds_budget = 160 * 1024;  // bytes of LDS a scan block may use (gfx950: 160 KiB per CU)
     bool allow_lds_tables = true;     // stage compact hash tables into LDS
     bool allow_lds_aggregation = true;
     bool allow_compact_tables = true;
+    bool allow_dense_tables = true;   // key-indexed tables where the keys allow
     bool prefetch = false;            // load tile t+stride while tile t is processed
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
@@ -46,6 +47,13 @@ struct JoinProgram {
     bool compact = false;     // compact tagged layout vs reference int layout
     int eb = 0, tb = 0, jlo = 0;
     int64_t pmin = 0;
+    // key-indexed layout (chip::ht_probe_dense): candidate and choice
+    bool dense_ok = false;    // keys unique, non-null, within the reference slot range
+    bool dense = false;       // chosen
+    int64_t dmin = 0, dspan = 0;
+    int dbits = 0;            // exact entry width (0: all keys present, no payload)
+    int doff = 1;             // 1: code 0 marks an absent key; 0: every key present
+    int db = 0;               // chosen width (exact in LDS, power of two in HBM)
     bool lds = false;
     uint64_t ht_words = 0;    // u32 words of the table
 };

@@ -12,8 +12,9 @@
 //   include/crystal/join.h   HASH(k,len,min) = (k-min) % len, CAS 0->key
 //                            -> same slot function; either the reference
 //                            interleaved int layout or a compact tagged layout
-//                            (see ht_probe_compact) that keeps the exact
-//                            `ht[slot] == key` semantics in 1..32 bits per slot
+//                            (see ht_probe_compact) or a key-indexed layout
+//                            (ht_probe_dense) that keep the exact
+//                            `ht[slot] == key` semantics in a few bits per key
 //   include/core/inline_math.h  db::safe_* promotion rules, u64 atomics
 //                            ->  safe_* below, u64 atomics / wave reductions
 #ifndef CRYSTAL_HIP_H
@@ -112,6 +113,46 @@ __device__ __forceinline__ void ht_insert_compact(u32* w, int k, int kmin, int l
     const u32 e = 1u | (tag << 1) | ((u32)(payload - pmin) << (1 + tb));
     const u64 bit = (u64)(u32)s * (u32)eb;
     atomicOr(&w[bit >> 5], e << (bit & 31));
+}
+
+// Key-indexed layout.  Entry x = fk - dmin of `db` bits holds
+// payload - pbase; with `off` set, code 0 marks a key that was not inserted
+// (filtered out on the build side, or no such row) and pbase = pmin - 1;
+// without it every key of the span is present and pbase = pmin.  Entries may
+// straddle two words (tables carry one padding word).  Same validity
+// condition as the compact layout (unique keys spanning at most `len`
+// consecutive values at or above the reference kmin: then ht[slot] == fk holds
+// exactly for the inserted keys), and dmin..dmin+dspan-1 covers every key.
+// db == 0: every key present, no payload: the probe is the range test alone.
+__device__ __forceinline__ u32 dense_entry(const u32* w, u32 x, int db) {
+    const u32 b = x * (u32)db;
+    const u32 lo = w[b >> 5] >> (b & 31);
+    const u32 mask = (1u << db) - 1u;
+    if ((b & 31) + (u32)db <= 32u) return lo & mask;
+    return (lo | (w[(b >> 5) + 1] << (32u - (b & 31)))) & mask;
+}
+// db a power of two: an entry never straddles (one load per probe)
+__device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) {
+    const u32 b = x * (u32)db;
+    return (w[b >> 5] >> (b & 31)) & ((1u << db) - 1u);
+}
+template <bool ALIGNED>
+__device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, int dspan, int db, int pbase, bool off,
+                                               int& payload) {
+    const u32 x = (u32)fk - (u32)dmin;
+    if (x >= (u32)dspan) return false;
+    if (db == 0) return true;
+    const u32 e = ALIGNED ? dense_entry_aligned(w, x, db) : dense_entry(w, x, db);
+    payload = pbase + (int)e;
+    return !off || e != 0u;
+}
+__device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload) {
+    const u32 x = (u32)k - (u32)dmin;
+    if (x >= (u32)dspan || db == 0) return;
+    const u32 c = (u32)(payload - pbase);
+    const u32 b = x * (u32)db;
+    atomicOr(&w[b >> 5], c << (b & 31));
+    if ((b & 31) + (u32)db > 32u) atomicOr(&w[(b >> 5) + 1], c >> (32u - (b & 31)));
 }
 
 // Reference layout: int[2*len]; PHT_1 keys in ht[s], PHT_2 [key, payload] pairs.

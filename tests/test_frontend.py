@@ -156,9 +156,20 @@ def test_constants_are_kernel_arguments(small_db):
 
 def test_layout_decisions_for_configs(small_db):
     c3 = small_db.describe(Q.C3)
-    assert c3["lds_aggregation"] == 1 and c3["groups"] == 25 * 26 * 7
+    # nation keys arrive in payload registers (no validity): no NULL code in the domain
+    assert c3["lds_aggregation"] == 1 and c3["groups"] == 25 * 25 * 7
     joins = {j["dim"]: j for j in c3["joins"]}
-    assert joins["DDATE"]["compact"] == 1 and joins["DDATE"]["entry_bits"] == 4
     assert joins["SUPPLIER"]["payload"] == "s_nation" and joins["CUSTOMER"]["payload"] == "c_nation"
+    # every supplier / customer key present: 5-bit nation codes indexed by key, no absent code
+    for d in ("SUPPLIER", "CUSTOMER"):
+        assert joins[d]["layout"] == "key-indexed" and joins[d]["entry_bits"] == 5 and joins[d]["absent_code"] == 0
+    # 2556 dates over a 61131-key span: 3-bit year codes with code 0 = absent
+    assert joins["DDATE"]["layout"] == "key-indexed" and joins["DDATE"]["entry_bits"] == 3
+    assert joins["DDATE"]["absent_code"] == 1
+    assert c3["scan_lds_bytes"] <= 160 * 1024
     c1 = small_db.describe(Q.C1)
     assert c1["fast_sparse"] == 1 and c1["joins"][0]["payload"] == "" and c1["joins"][0]["entry_bits"] == 1
+    c2 = small_db.describe(Q.C2)
+    joins = {j["dim"]: j for j in c2["joins"]}
+    # s_region filter: a 1-bit membership table for the 2000 suppliers
+    assert joins["SUPPLIER"]["payload"] == "" and joins["SUPPLIER"]["entry_bits"] == 1 and joins["SUPPLIER"]["lds"] == 1
