@@ -44,6 +44,12 @@ def parse_args():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
     p.add_argument("--extra", action="store_true", help="also time the other plain configurations")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched shared-scan measurement")
+    p.add_argument("--no-served", action="store_true", help="skip the serving measurement")
+    p.add_argument("--batch-queries", type=int, default=64)
+    p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
+    p.add_argument("--serve-seconds", type=float, default=5.0)
+    p.add_argument("--serve-adhoc", type=float, default=0.05, help="fraction of requests with a new plan shape")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     return p.parse_args()
@@ -92,6 +98,111 @@ def cpu_baseline(workload, sql_text, sample_rows, seed):
     return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{workload.upper()} query over a {sample_rows:,}-row fact sample (same generator, scale-{sf} "
                       f"dimensions), CPU oracle restatement with OpenMP, median of 3 runs: {t * 1000:.0f} ms"}
+
+
+def same_rows(a, b):
+    key = lambda t: tuple((x is None, x) for x in t)
+    x, y = a.tuples(), b.tuples()
+    if not (a.ordered and b.ordered):
+        x, y = sorted(x, key=key), sorted(y, key=key)
+    return a.types == b.types and x == y
+
+
+def bench_batched(pa, Q, args):
+    """64 template queries (1-4 dimensions, random constants) over resident
+    scale-20 columns: one shared-scan batch vs. the same queries run one at a
+    time through the plain path."""
+    db = pa.Database(0)
+    db.generate_ssb(sf=20, fact_rows=120_000_000, seed=args.seed)
+    db.finalize()
+    n = args.batch_queries
+    mixes = [Q.template_mix(n, args.seed + k) for k in range(args.steps + 1)]
+    # warm-up: template modules, shared-scan module (autotuned once), plain modules
+    db.batch_execute(mixes[0], {"autotune": 1})
+    tuned = db.batch_report()["scans"][0]
+    for q in mixes[0]:
+        db.execute(q)
+    # serial plain
+    t0 = time.perf_counter()
+    serial = [db.execute(q) for q in mixes[1]]
+    serial_s = time.perf_counter() - t0
+    # batched, fresh constants every step
+    t0 = time.perf_counter()
+    out = None
+    for k in range(1, args.steps + 1):
+        res = db.batch_execute(mixes[k])
+        if k == 1:
+            out = res
+    batched_s = (time.perf_counter() - t0) / args.steps
+    rep = db.batch_report()
+    exact = all(same_rows(a, b) for a, b in zip(out, serial))
+    db.close()
+    return {"workload": f"{n} queries, 4 templates (1-4 dimension joins), random constants, scale 20 (120,000,000 fact rows)",
+            "qps": round(n / batched_s, 1), "serial_qps": round(n / serial_s, 1),
+            "speedup": round(serial_s / batched_s, 2), "batch_ms": round(batched_s * 1000, 3),
+            "serial_ms": round(serial_s * 1000, 3), "shared_scans": rep["shared_scans"],
+            "block_threads": tuned["block_threads"], "vectors_per_thread": tuned["vectors_per_thread"],
+            "graph_replayed": rep["scans"][0]["graph_replayed"], "bit_exact_vs_serial": exact}
+
+
+def bench_served(pa, Q, args):
+    """Open-loop Poisson arrivals against the continuous-batching server over
+    resident scale-100 columns (600,000,000 fact rows); a fraction of the
+    requests have a plan shape never seen before (hipRTC compilation)."""
+    import random
+    import threading
+    db = pa.Database(0)
+    db.generate_ssb(sf=100, fact_rows=600_000_000, seed=args.seed, nullable=WORKLOADS["c3"][2])
+    db.finalize()
+    rng = random.Random(args.seed)
+    # serial plain reference on the same templates
+    warm = [Q.template_query(k, rng) for k in (1, 2, 3, 4)]
+    for q in warm:
+        db.execute(q)
+    ser = Q.template_mix(32, args.seed + 7)
+    t0 = time.perf_counter()
+    for q in ser:
+        db.execute(q)
+    serial_qps = len(ser) / (time.perf_counter() - t0)
+    srv = pa.Server(db, {"lanes": 2, "max_batch": 64})
+    for t in [srv.submit(q) for q in warm]:
+        srv.wait(t)
+    srv.reset_stats()
+    n = int(args.serve_rate * args.serve_seconds)
+    reqs = []
+    for _ in range(n):
+        reqs.append(Q.adhoc_query(rng) if rng.random() < args.serve_adhoc else Q.template_query(rng.randint(1, 4), rng))
+    tickets = []
+
+    def producer():
+        t = time.perf_counter()
+        for q in reqs:
+            t += rng.expovariate(args.serve_rate)
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append(srv.submit(q))
+
+    th = threading.Thread(target=producer)
+    t0 = time.perf_counter()
+    th.start()
+    th.join()
+    errors = 0
+    for t in tickets:
+        try:
+            srv.wait(t, 600_000)
+        except pa.CrystalError:
+            errors += 1
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.stop()
+    db.close()
+    return {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, 4 templates + "
+                        f"{args.serve_adhoc:.0%} new plan shapes, scale 100 (600,000,000 fact rows)",
+            "requests": n, "qps": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
+            "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2), "serial_qps": round(serial_qps, 1),
+            "speedup": round(st["completed"] / wall / serial_qps, 2), "mean_batch": round(st["mean_batch"], 1),
+            "compiles": st["compiles"], "errors": errors}
 
 
 def copy_calibration(nbytes, steps):
@@ -203,6 +314,10 @@ def main():
             d2.close()
         line["other_configs"] = extra
     db.close()
+    if not args.no_batched:
+        line["batched"] = bench_batched(pa, Q, args)
+    if not args.no_served:
+        line["served"] = bench_served(pa, Q, args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, sql, min(args.cpu_sample_rows, rows), args.seed)
     print(json.dumps(line), flush=True)

@@ -474,6 +474,20 @@ private:
             }
             jp.dbits = jp.pair ? bitsFor(range + static_cast<uint64_t>(jp.doff)) : (all_present ? 0 : 1);
             jp.dense_ok = jp.dbits <= 31 && jp.dspan > 0 && jp.dspan <= (int64_t(1) << 31);
+            // key 0 against the zero-filled reference: which key owns its slot
+            jp.zmode = 0;
+            jp.zx = 0;
+            if (jp.kmin <= 0) {
+                const int64_t r = (-jp.kmin) % jp.len;
+                const int64_t first = jp.kmin + r;  // smallest key >= kmin in key 0's slot
+                int64_t t = jp.dmin > first ? (jp.dmin - first + jp.len - 1) / jp.len : 0;
+                const int64_t k0 = first + t * jp.len;
+                if (k0 > jp.dmin + jp.dspan - 1) jp.zmode = 1;
+                else {
+                    jp.zmode = k0 == 0 ? 3 : 2;
+                    jp.zx = static_cast<int>(k0 - jp.dmin);
+                }
+            }
         }
     }
 
@@ -918,7 +932,6 @@ private:
         std::ostringstream src;
         src << "// generated by crystal-hip for plan " << "\n";
         src << "#include \"crystal_hip.h\"\n\n";
-        for (int j = 0; j < static_cast<int>(prog_.joins.size()); ++j) emitBuild(src, j);
 
         const bool grouped = prog_.grouped;
         // LDS aggregation when the domain decodes from the group index
@@ -962,6 +975,8 @@ private:
             finalizeLayout(jp, (best_mask >> j) & 1);
             if (jp.lds) lds_used += jp.ht_words * 4;
         }
+        // builds follow the final layouts (the probes below use the same)
+        for (int j = 0; j < nj; ++j) emitBuild(src, j);
         prog_.scan_lds_bytes = lds_used;
         int bt = opt_.block_threads;
         if (bt <= 0) bt = lds_used > 40 * 1024 ? 1024 : (lds_used > 0 ? 512 : 256);
@@ -1002,7 +1017,8 @@ private:
                 if (jp.dense)
                     body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ">("
                          << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
-                         << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j << "[i])) fl &= ~(1u << i);\n";
+                         << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p"
+                         << j << "[i])) fl &= ~(1u << i);\n";
                 else if (jp.compact)
                     body << "            if (!chip::ht_probe_compact(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
                          << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j

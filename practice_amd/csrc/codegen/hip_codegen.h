@@ -54,6 +54,7 @@ struct JoinProgram {
     int dbits = 0;            // exact entry width (0: all keys present, no payload)
     int doff = 1;             // 1: code 0 marks an absent key; 0: every key present
     int db = 0;               // chosen width (exact in LDS, power of two in HBM)
+    int zmode = 0, zx = 0;    // probe key 0 vs. the reference's empty-slot match (chip::ht_probe_dense)
     bool lds = false;
     uint64_t ht_words = 0;    // u32 words of the table
 };

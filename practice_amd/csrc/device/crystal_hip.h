@@ -21,6 +21,7 @@
 #define CRYSTAL_HIP_H
 
 #include "query_params.h"
+#include "ht_layouts.h"
 
 namespace chip {
 
@@ -75,108 +76,7 @@ __device__ __forceinline__ u32 bit64(const u64* w, i64 idx) { return (u32)((w[id
 __device__ __forceinline__ void set_bit64(u64* w, i64 idx) { atomicOr(&w[idx >> 6], 1ull << (idx & 63)); }
 
 // ------------------------------------------------------------------ hash tables
-// Reference slot: (k - kmin) % len in int arithmetic.  A negative k - kmin
-// (an out-of-bounds access in the reference) never matches here.
-__device__ __forceinline__ int ht_slot(int k, int kmin, int len, int& x) {
-    x = (int)((u32)k - (u32)kmin);
-    return x < len ? x : x % len;
-}
-
-// Compact tagged layout.  Entry of `eb` bits per slot:
-//   bit 0            occupied
-//   bits 1..tb       tag t: the key stored in slot s is kmin + s + (jlo+t)*len
-//   bits tb+1..      payload - pmin
-// Valid when the keys inserted are unique and span at most `len` consecutive
-// values starting at or above kmin (then no two keys share a slot).  An empty
-// slot reads as key 0 / payload 0, exactly like the zero-filled reference table.
-__device__ __forceinline__ bool ht_probe_compact(const u32* w, int fk, int kmin, int len, int eb, int tb, int jlo,
-                                                 int pmin, int& payload) {
-    int x;
-    const int s = ht_slot(fk, kmin, len, x);
-    if (x < 0) return false;
-    const u64 bit = (u64)(u32)s * (u32)eb;
-    const u32 mask = eb >= 32 ? 0xffffffffu : ((1u << eb) - 1u);
-    const u32 e = (w[bit >> 5] >> (bit & 31)) & mask;
-    const bool occ = (e & 1u) != 0;
-    const int tag = tb ? (int)((e >> 1) & ((1u << tb) - 1u)) : 0;
-    const int key = occ ? (int)((u32)kmin + (u32)s + (u32)((jlo + tag) * len)) : 0;
-    payload = occ ? pmin + (int)(e >> (1 + tb)) : 0;
-    return key == fk;
-}
-
-__device__ __forceinline__ void ht_insert_compact(u32* w, int k, int kmin, int len, int eb, int tb, int jlo, int pmin,
-                                                  int payload) {
-    int x;
-    const int s = ht_slot(k, kmin, len, x);
-    if (x < 0) return;
-    const u32 tag = (u32)(x / len - jlo);
-    const u32 e = 1u | (tag << 1) | ((u32)(payload - pmin) << (1 + tb));
-    const u64 bit = (u64)(u32)s * (u32)eb;
-    atomicOr(&w[bit >> 5], e << (bit & 31));
-}
-
-// Key-indexed layout.  Entry x = fk - dmin of `db` bits holds
-// payload - pbase; with `off` set, code 0 marks a key that was not inserted
-// (filtered out on the build side, or no such row) and pbase = pmin - 1;
-// without it every key of the span is present and pbase = pmin.  Entries may
-// straddle two words (tables carry one padding word).  Same validity
-// condition as the compact layout (unique keys spanning at most `len`
-// consecutive values at or above the reference kmin: then ht[slot] == fk holds
-// exactly for the inserted keys), and dmin..dmin+dspan-1 covers every key.
-// db == 0: every key present, no payload: the probe is the range test alone.
-__device__ __forceinline__ u32 dense_entry(const u32* w, u32 x, int db) {
-    const u32 b = x * (u32)db;
-    const u32 lo = w[b >> 5] >> (b & 31);
-    const u32 mask = (1u << db) - 1u;
-    if ((b & 31) + (u32)db <= 32u) return lo & mask;
-    return (lo | (w[(b >> 5) + 1] << (32u - (b & 31)))) & mask;
-}
-// db a power of two: an entry never straddles (one load per probe)
-__device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) {
-    const u32 b = x * (u32)db;
-    return (w[b >> 5] >> (b & 31)) & ((1u << db) - 1u);
-}
-template <bool ALIGNED>
-__device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, int dspan, int db, int pbase, bool off,
-                                               int& payload) {
-    const u32 x = (u32)fk - (u32)dmin;
-    if (x >= (u32)dspan) return false;
-    if (db == 0) return true;
-    const u32 e = ALIGNED ? dense_entry_aligned(w, x, db) : dense_entry(w, x, db);
-    payload = pbase + (int)e;
-    return !off || e != 0u;
-}
-__device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload) {
-    const u32 x = (u32)k - (u32)dmin;
-    if (x >= (u32)dspan || db == 0) return;
-    const u32 c = (u32)(payload - pbase);
-    const u32 b = x * (u32)db;
-    atomicOr(&w[b >> 5], c << (b & 31));
-    if ((b & 31) + (u32)db > 32u) atomicOr(&w[(b >> 5) + 1], c >> (32u - (b & 31)));
-}
-
-// Reference layout: int[2*len]; PHT_1 keys in ht[s], PHT_2 [key, payload] pairs.
-__device__ __forceinline__ bool ht_probe_ref(const int* ht, int fk, int kmin, int len, bool pair, int& payload) {
-    int x;
-    const int s = ht_slot(fk, kmin, len, x);
-    if (x < 0) return false;
-    if (!pair) return ht[s] == fk;
-    if (ht[2 * s] != fk) return false;
-    payload = ht[2 * s + 1];
-    return true;
-}
-
-__device__ __forceinline__ void ht_insert_ref(int* ht, int k, int kmin, int len, bool pair, int payload) {
-    int x;
-    const int s = ht_slot(k, kmin, len, x);
-    if (x < 0) return;
-    if (!pair) {
-        atomicCAS(&ht[s], 0, k);
-        return;
-    }
-    atomicCAS(&ht[2 * s], 0, k);
-    ht[2 * s + 1] = payload;  // unconditional, as BlockBuildSelectivePHT_2 (join.h:203-205)
-}
+// Layouts and their probe / insert primitives: ht_layouts.h
 
 // ------------------------------------------------------------------ wave reductions
 __device__ __forceinline__ u64 wave_sum(u64 v) {

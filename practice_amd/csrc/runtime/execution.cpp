@@ -13,6 +13,7 @@
 
 extern const char* const kCrystalHipHeaderSource;
 extern const char* const kQueryParamsHeaderSource;
+extern const char* const kHtLayoutsHeaderSource;
 
 namespace db {
 
@@ -70,10 +71,10 @@ HipRtcCompiler::HipRtcCompiler(std::string arch, std::vector<std::string> extra)
 
 ModuleHandle HipRtcCompiler::compile(const std::string& source, const std::string& query_hash) {
     hiprtcProgram prog = nullptr;
-    const char* headers[] = {kCrystalHipHeaderSource, kQueryParamsHeaderSource};
-    const char* names[] = {"crystal_hip.h", "query_params.h"};
+    const char* headers[] = {kCrystalHipHeaderSource, kQueryParamsHeaderSource, kHtLayoutsHeaderSource};
+    const char* names[] = {"crystal_hip.h", "query_params.h", "ht_layouts.h"};
     const std::string file = "q_" + query_hash + ".hip";
-    if (hiprtcCreateProgram(&prog, source.c_str(), file.c_str(), 2, headers, names) != HIPRTC_SUCCESS)
+    if (hiprtcCreateProgram(&prog, source.c_str(), file.c_str(), 3, headers, names) != HIPRTC_SUCCESS)
         throw std::runtime_error("hiprtcCreateProgram failed");
     std::vector<std::string> opts = {"--offload-arch=" + arch_, "-O3", "-std=c++17", "-fwrapv",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
@@ -520,16 +521,30 @@ ModuleHandle QueryEngine::moduleForSource(const std::string& source, const std::
         if (timing) timing->cache_hit = true;
         return m;
     }
-    std::lock_guard<std::mutex> lock(compile_mu_);
-    m = cache_->get(signature);
-    if (m) {
-        if (timing) timing->cache_hit = true;
-        return m;
+    {
+        std::unique_lock<std::mutex> lock(compile_mu_);
+        compile_cv_.wait(lock, [&] { return compiling_.count(signature) == 0; });
+        m = cache_->get(signature);
+        if (m) {
+            if (timing) timing->cache_hit = true;
+            return m;
+        }
+        compiling_.insert(signature);
     }
     const auto c0 = std::chrono::steady_clock::now();
-    m = compiler_->compile(source, signature);
+    try {
+        m = compiler_->compile(source, signature);
+    } catch (...) {
+        std::lock_guard<std::mutex> lock(compile_mu_);
+        compiling_.erase(signature);
+        compile_cv_.notify_all();
+        throw;
+    }
     if (timing) timing->compile_ms += elapsedMs(c0, std::chrono::steady_clock::now());
+    std::lock_guard<std::mutex> lock(compile_mu_);
     cache_->put(signature, m);
+    compiling_.erase(signature);
+    compile_cv_.notify_all();
     return m;
 }
 

@@ -19,12 +19,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdint>
 #include <list>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -181,12 +184,12 @@ class HipRtcCompiler : public ICompiler {
 public:
     explicit HipRtcCompiler(std::string arch = "gfx950", std::vector<std::string> extra_options = {});
     ModuleHandle compile(const std::string& source_code, const std::string& query_hash) override;
-    size_t compileCount() const { return compiles_; }
+    size_t compileCount() const { return compiles_.load(); }
 
 private:
     std::string arch_;
     std::vector<std::string> options_;
-    size_t compiles_ = 0;
+    std::atomic<size_t> compiles_{0};
 };
 
 // Thread-safe LRU map from plan signature to compiled module.
@@ -275,7 +278,11 @@ private:
     std::shared_ptr<ICompiler> compiler_;
     std::shared_ptr<IExecutor> executor_;
     std::shared_ptr<const Catalog> catalog_;
+    // one compilation per signature at a time; different signatures compile
+    // concurrently (hipRTC programs are independent)
     std::mutex compile_mu_;
+    std::condition_variable compile_cv_;
+    std::set<std::string> compiling_;
 };
 
 // Throws when the device cannot hold the query (preflightDeviceMemoryOrThrow).

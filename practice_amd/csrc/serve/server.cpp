@@ -21,6 +21,8 @@ ServerOptions parseServerOptions(const std::string& text) {
         if (k == "lanes") o.lanes = static_cast<int>(v);
         else if (k == "max_batch") o.max_batch = static_cast<int>(v);
         else if (k == "max_templates") o.max_templates = static_cast<int>(v);
+        else if (k == "promote_after") o.promote_after = static_cast<int>(v);
+        else if (k == "compile_threads") o.compile_threads = static_cast<int>(v);
         else if (k == "budget_mb") o.budget_bytes = static_cast<size_t>(v) << 20;
         else if (k == "window_us") o.window_us = static_cast<int>(v);
         else if (k == "async_compile") o.async_compile = v != 0;
@@ -31,6 +33,8 @@ ServerOptions parseServerOptions(const std::string& text) {
     if (o.lanes < 1 || o.lanes > 8) throw std::runtime_error("server lanes must be in [1, 8]");
     if (o.max_batch < 1) throw std::runtime_error("server max_batch must be positive");
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
+    o.promote_after = std::max(1, o.promote_after);
+    o.compile_threads = std::max(1, std::min(o.compile_threads, 16));
     return o;
 }
 
@@ -59,7 +63,8 @@ void QueryServer::start() {
         Lane* lp = l.get();
         l->thread = std::thread([this, lp] { laneLoop(lp); });
     }
-    if (opt_.async_compile) compiler_ = std::thread([this] { compileLoop(); });
+    if (opt_.async_compile)
+        for (int i = 0; i < opt_.compile_threads; ++i) compilers_.emplace_back([this] { compileLoop(); });
 }
 
 QueryServer::~QueryServer() {
@@ -71,10 +76,11 @@ QueryServer::~QueryServer() {
     compile_cv_.notify_all();
     for (auto& l : lanes_)
         if (l->thread.joinable()) l->thread.join();
-    if (compiler_.joinable()) compiler_.join();
+    for (auto& t : compilers_)
+        if (t.joinable()) t.join();
     // requests still queued when the server stops fail
     std::lock_guard<std::mutex> lock(mu_);
-    for (auto* q : {&queue_, &compile_q_}) {
+    for (auto* q : {&queue_, &compile_q_, &plain_ready_}) {
         for (auto& r : *q) {
             Done d;
             d.rc = -3;
@@ -134,7 +140,6 @@ bool QueryServer::prepare(Request& r) {
 }
 
 bool QueryServer::inScanSet(const std::string& signature) const {
-    if (!scan_set_ready_) return false;
     for (const auto& t : scan_set_)
         if (t->signature == signature) return true;
     return false;
@@ -142,10 +147,10 @@ bool QueryServer::inScanSet(const std::string& signature) const {
 
 void QueryServer::laneLoop(Lane* lane) {
     for (;;) {
-        std::vector<Request> taken;
+        std::vector<Request> taken, plain_taken;
         {
             std::unique_lock<std::mutex> lock(mu_);
-            cv_.wait(lock, [&] { return stop_ || !queue_.empty(); });
+            cv_.wait(lock, [&] { return stop_ || !queue_.empty() || !plain_ready_.empty(); });
             if (stop_) return;
             if (opt_.window_us > 0 && static_cast<int>(queue_.size()) < opt_.max_batch) {
                 lock.unlock();
@@ -157,23 +162,34 @@ void QueryServer::laneLoop(Lane* lane) {
                 taken.push_back(std::move(queue_.front()));
                 queue_.pop_front();
             }
+            while (!plain_ready_.empty() && static_cast<int>(plain_taken.size()) < opt_.max_batch) {
+                plain_taken.push_back(std::move(plain_ready_.front()));
+                plain_ready_.pop_front();
+            }
         }
         // plan + codegen outside the lock
         std::vector<Request> ready;
         for (auto& r : taken)
             if (r.program || prepare(r)) ready.push_back(std::move(r));
-        if (ready.empty()) continue;
 
-        // runnable = template compiled into the current scan set; admission
-        // reserves each request's footprint against the budget
-        std::vector<Request> batch, later, cold;
+        // hot signatures join the shared scan, cold ones go to compilation;
+        // admission reserves each request's footprint against the budget
+        std::vector<Request> batch, plain, later, later_plain;
         std::vector<std::shared_ptr<QueryProgram>> set;
+        bool compile_work = false;
         {
             std::lock_guard<std::mutex> lock(mu_);
             set = scan_set_;
             for (auto& r : ready) {
-                if (!inScanSet(r.program->signature)) {
-                    cold.push_back(std::move(r));
+                const std::string& sig = r.program->signature;
+                if (!inScanSet(sig)) {
+                    const int n = ++seen_[sig];
+                    if (n >= opt_.promote_after && !promoting_.count(sig)) {
+                        promoting_.insert(sig);
+                        promote_q_.push_back(r.program);
+                    }
+                    compile_q_.push_back(std::move(r));
+                    compile_work = true;
                 } else if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
                     in_flight_bytes_ += r.footprint;
                     batch.push_back(std::move(r));
@@ -181,126 +197,169 @@ void QueryServer::laneLoop(Lane* lane) {
                     later.push_back(std::move(r));
                 }
             }
-            if (!later.empty()) {
-                ++admission_waits_;
-                for (auto it = later.rbegin(); it != later.rend(); ++it) queue_.push_front(std::move(*it));
+            for (auto& r : plain_taken) {
+                if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
+                    in_flight_bytes_ += r.footprint;
+                    plain.push_back(std::move(r));
+                } else {
+                    later_plain.push_back(std::move(r));
+                }
             }
-            if (!cold.empty() && opt_.async_compile) {
-                for (auto& r : cold) compile_q_.push_back(std::move(r));
-                cold.clear();
-                compile_cv_.notify_one();
-            }
+            if (!later.empty() || !later_plain.empty()) ++admission_waits_;
+            for (auto it = later.rbegin(); it != later.rend(); ++it) queue_.push_front(std::move(*it));
+            for (auto it = later_plain.rbegin(); it != later_plain.rend(); ++it) plain_ready_.push_front(std::move(*it));
         }
-        if (!cold.empty()) {
-            // synchronous compilation on this lane, then back to the queue
-            for (auto& r : cold) {
-                std::lock_guard<std::mutex> lock(mu_);
-                compile_q_.push_back(std::move(r));
-            }
-            compileLoop();
-            continue;
+        if (compile_work) {
+            if (opt_.async_compile) compile_cv_.notify_all();
+            else
+                while (compileOne()) {
+                }
         }
-        if (batch.empty()) {
-            // nothing admissible right now: wait for memory to free up
-            std::unique_lock<std::mutex> lock(mu_);
-            const size_t before = in_flight_bytes_;
-            cv_.wait_for(lock, std::chrono::milliseconds(5), [&] { return stop_ || in_flight_bytes_ < before; });
+        if (batch.empty() && plain.empty()) {
+            if (!later.empty() || !later_plain.empty()) {
+                // nothing admissible right now: wait for memory to free up
+                std::unique_lock<std::mutex> lock(mu_);
+                const size_t before = in_flight_bytes_;
+                cv_.wait_for(lock, std::chrono::milliseconds(5), [&] { return stop_ || in_flight_bytes_ < before; });
+            }
             continue;
         }
 
-        std::vector<QueryProgram> programs;
-        programs.reserve(batch.size());
-        for (auto& r : batch) programs.push_back(*r.program);
-        std::vector<const QueryProgram*> templates;
-        for (const auto& t : set) templates.push_back(t.get());
-        std::vector<QueryResult> results;
-        std::string err;
-        try {
-            results = lane->exec->executePrograms(programs, opt_.batch, &templates);
-        } catch (const std::exception& e) {
-            err = e.what();
-        }
         size_t released = 0;
-        for (size_t i = 0; i < batch.size(); ++i) {
-            released += batch[i].footprint;
-            if (err.empty()) finish(batch[i], 0, "", &results[i]);
-            else finish(batch[i], -3, err, nullptr);
+        if (!batch.empty()) {
+            std::vector<QueryProgram> programs;
+            programs.reserve(batch.size());
+            for (auto& r : batch) programs.push_back(*r.program);
+            std::vector<const QueryProgram*> templates;
+            for (const auto& t : set) templates.push_back(t.get());
+            std::vector<QueryResult> results;
+            std::string err;
+            try {
+                results = lane->exec->executePrograms(programs, opt_.batch, &templates);
+            } catch (const std::exception& e) {
+                err = e.what();
+            }
+            for (size_t i = 0; i < batch.size(); ++i) {
+                released += batch[i].footprint;
+                if (err.empty()) finish(batch[i], 0, "", &results[i]);
+                else finish(batch[i], -3, err, nullptr);
+            }
+        }
+        for (auto& r : plain) {
+            // cold signature: its own scan on the plain path (module compiled)
+            released += r.footprint;
+            try {
+                std::lock_guard<std::mutex> lock(lane->mu);
+                ExecutionContext* ctx = lane->ctx.get();
+                ctx->row_begin = 0;
+                ctx->row_end = -1;
+                db_.engine().executeQuery(r.sql, ctx);
+                QueryResult res = std::move(ctx->result);
+                finish(r, 0, "", &res);
+            } catch (const std::exception& e) {
+                finish(r, -3, e.what(), nullptr);
+            }
         }
         {
             std::lock_guard<std::mutex> lock(mu_);
             in_flight_bytes_ -= released;
-            ++batches_;
-            batched_queries_ += static_cast<long long>(batch.size());
+            if (!batch.empty()) {
+                ++batches_;
+                batched_queries_ += static_cast<long long>(batch.size());
+            }
+            plain_queries_ += static_cast<long long>(plain.size());
         }
         cv_.notify_all();
     }
 }
 
 void QueryServer::compileLoop() {
-    const bool inline_call = !opt_.async_compile;
     for (;;) {
-        std::vector<Request> work;
         {
             std::unique_lock<std::mutex> lock(mu_);
-            if (!inline_call) compile_cv_.wait(lock, [&] { return stop_ || !compile_q_.empty(); });
+            compile_cv_.wait(lock, [&] { return stop_ || !compile_q_.empty() || !promote_q_.empty(); });
             if (stop_) return;
-            while (!compile_q_.empty()) {
-                work.push_back(std::move(compile_q_.front()));
-                compile_q_.pop_front();
-            }
-            if (work.empty()) return;
         }
-        // template modules, then the shared-scan module of the grown set
-        std::vector<std::shared_ptr<QueryProgram>> set;
-        {
-            std::lock_guard<std::mutex> lock(mu_);
-            set = scan_set_;
+        compileOne();
+    }
+}
+
+bool QueryServer::compileOne() {
+    Request r;
+    std::shared_ptr<QueryProgram> promote;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (!compile_q_.empty()) {  // latency first: a request is waiting for it
+            r = std::move(compile_q_.front());
+            compile_q_.pop_front();
+        } else if (!promote_q_.empty()) {
+            promote = promote_q_.front();
+            promote_q_.pop_front();
+        } else {
+            return false;
         }
-        std::vector<Request> ok;
-        for (auto& r : work) {
-            try {
-                db_.engine().moduleFor(*r.program, nullptr);
-                ++compiles_;
-                bool present = false;
-                for (const auto& t : set) present = present || t->signature == r.program->signature;
-                if (!present) {
-                    if (!set.empty() && set.front()->fact_table != r.program->fact_table)
-                        throw std::runtime_error("a server scans one fact table (" + set.front()->fact_table + ")");
-                    if (static_cast<int>(set.size()) >= opt_.max_templates) set.erase(set.begin());
-                    set.push_back(r.program);
-                }
-                ok.push_back(std::move(r));
-            } catch (const std::exception& e) {
-                finish(r, -2, e.what(), nullptr);
-            }
-        }
+    }
+    if (!promote) {
+        // plain module of a cold request, then it waits for a lane
         try {
-            if (!set.empty()) {
-                std::vector<const QueryProgram*> tp;
-                for (const auto& t : set) tp.push_back(t.get());
-                std::vector<std::string> cols;
-                std::vector<bool> nullable;
-                const std::string src =
-                    generateBatchScanSource(tp, opt_.batch.block_threads, opt_.batch.vectors_per_thread, cols, nullable);
-                db_.engine().moduleForSource(src, signatureOf(src), nullptr);
-            }
-        } catch (const std::exception& e) {
-            for (auto& r : ok) finish(r, -2, e.what(), nullptr);
-            ok.clear();
-            set.clear();
-        }
-        {
+            QueryProgram p = db_.engine().prepare(r.sql, db_.config().codegen);
+            QueryTimingStats t;
+            db_.engine().moduleFor(p, &t);
             std::lock_guard<std::mutex> lock(mu_);
-            if (!set.empty()) {
-                scan_set_ = set;
-                scan_set_ready_ = true;
-            }
-            // requests whose template fell out of the set go round again
-            for (auto it = ok.rbegin(); it != ok.rend(); ++it) queue_.push_front(std::move(*it));
+            if (!t.cache_hit) ++compiles_;
+            plain_ready_.push_back(std::move(r));
+        } catch (const std::exception& e) {
+            finish(r, -2, e.what(), nullptr);
+            return true;
         }
         cv_.notify_all();
-        if (inline_call) return;
+        return true;
     }
+    // promotion: template module, then the shared-scan module of the grown set
+    std::vector<std::shared_ptr<QueryProgram>> set;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        set = scan_set_;
+    }
+    try {
+        if (!set.empty() && set.front()->fact_table != promote->fact_table)
+            throw std::runtime_error("a server's shared scan covers one fact table (" + set.front()->fact_table + ")");
+        db_.engine().moduleFor(*promote, nullptr);
+        bool present = false;
+        for (const auto& t : set) present = present || t->signature == promote->signature;
+        if (!present) {
+            if (static_cast<int>(set.size()) >= opt_.max_templates) set.erase(set.begin());
+            set.push_back(promote);
+        }
+        std::vector<const QueryProgram*> tp;
+        for (const auto& t : set) tp.push_back(t.get());
+        std::vector<std::string> cols;
+        std::vector<bool> nullable;
+        const std::string src = generateBatchScanSource(tp, opt_.batch.block_threads, opt_.batch.vectors_per_thread, cols, nullable);
+        db_.engine().moduleForSource(src, signatureOf(src), nullptr);
+        std::lock_guard<std::mutex> lock(mu_);
+        // keep templates promoted meanwhile by another thread
+        for (const auto& t : scan_set_) {
+            bool have = false;
+            for (const auto& u : set) have = have || u->signature == t->signature;
+            if (!have) {
+                set.clear();
+                break;
+            }
+        }
+        if (!set.empty()) {
+            scan_set_ = set;
+            ++promotions_;
+            promoting_.erase(promote->signature);
+        } else {
+            promote_q_.push_back(promote);  // raced with another promotion: redo on the newer set
+        }
+    } catch (const std::exception&) {
+        std::lock_guard<std::mutex> lock(mu_);
+        promoting_.erase(promote->signature);  // stays on the plain tier
+    }
+    cv_.notify_all();
+    return true;
 }
 
 int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::string& err) {
@@ -325,7 +384,7 @@ int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::s
 void QueryServer::resetStats() {
     std::lock_guard<std::mutex> lock(mu_);
     latencies_.clear();
-    batches_ = batched_queries_ = admission_waits_ = compiles_ = rejected_ = 0;
+    batches_ = batched_queries_ = plain_queries_ = admission_waits_ = compiles_ = promotions_ = rejected_ = 0;
     stats_start_ = std::chrono::steady_clock::now();
 }
 
@@ -344,7 +403,8 @@ std::string QueryServer::statsJson() {
       << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
       << ",\"batches\":" << batches_ << ",\"mean_batch\":"
       << (batches_ ? static_cast<double>(batched_queries_) / static_cast<double>(batches_) : 0.0)
-      << ",\"admission_waits\":" << admission_waits_ << ",\"compiles\":" << compiles_ << ",\"rejected\":" << rejected_
+      << ",\"plain_queries\":" << plain_queries_ << ",\"admission_waits\":" << admission_waits_
+      << ",\"compiles\":" << compiles_ << ",\"promotions\":" << promotions_ << ",\"rejected\":" << rejected_
       << ",\"lanes\":" << opt_.lanes << ",\"budget_bytes\":" << opt_.budget_bytes << ",\"in_flight_bytes\":" << in_flight_bytes_
       << ",\"scan_set\":" << scan_set_.size() << "}";
     return o.str();

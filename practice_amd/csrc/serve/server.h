@@ -3,19 +3,23 @@
 // No reference counterpart (the reference executes one query per call,
 // src/core/execution.cpp:652-725).  Requests queue on submit().  Each lane
 // (own stream, scratch and shared-scan executor) repeatedly takes every
-// queued request it can admit, up to max_batch, and runs them as one shared
-// scan; while one lane's batch is on the GPU the next lane admits the
-// requests that arrived meanwhile, so batches overlap on the device.
+// queued request it can admit, up to max_batch, and runs them together;
+// while one lane's work is on the GPU the next lane admits the requests that
+// arrived meanwhile, so the lanes' work overlaps on the device.
+//
+// Two execution tiers per plan signature:
+//   hot    signatures in the compiled shared-scan set run as one batch
+//          (cq_batch_scan: one pass over the fact columns for all of them)
+//   cold   a signature seen for the first time runs alone on the plain
+//          one-query path as soon as its module is compiled; once seen
+//          `promote_after` times it is compiled into the shared-scan set
+// Compilation (hipRTC) happens on background threads, so a new plan shape
+// delays only its own requests.
 //
 // Admission: a request's device footprint (hash tables, aggregate slots,
 // materialization buffers; estimateQueryTemporaryBytes) is reserved against
-// the HBM budget for as long as it is in flight; requests that do not fit
-// wait, requests larger than the whole budget fail.
-//
-// Kernel cache: requests whose plan signature has no compiled module (or
-// is not yet part of the compiled shared-scan set) go to a compile thread
-// and rejoin the queue when their modules are ready, so one ad-hoc query's
-// hipRTC compilation never stalls the others.
+// the HBM budget while it is in flight; requests that do not fit wait,
+// requests larger than the whole budget fail.
 #pragma once
 
 #include <condition_variable>
@@ -23,6 +27,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -35,14 +40,16 @@ struct ServerOptions {
     int lanes = 2;                 // concurrent batches on the GPU (own stream each)
     int max_batch = 64;            // requests per shared scan
     int max_templates = 16;        // plan signatures per shared-scan module
+    int promote_after = 2;         // requests of a signature before it joins the shared scan
+    int compile_threads = 2;       // background hipRTC threads
     size_t budget_bytes = 0;       // HBM for query temporaries (0: quarter of free memory)
     int window_us = 0;             // wait for more arrivals before taking a batch
-    bool async_compile = true;     // compile cache misses off the lanes
+    bool async_compile = true;     // compile off the lanes
     BatchOptions batch;
 };
 
-// "key=value,..." with lanes, max_batch, max_templates, budget_mb, window_us,
-// async_compile, bt, vpt, graph
+// "key=value,..." with lanes, max_batch, max_templates, promote_after,
+// compile_threads, budget_mb, window_us, async_compile, bt, vpt, graph
 ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
@@ -56,8 +63,6 @@ public:
     long long submit(const std::string& sql);
     // 0 on success, negative error code otherwise (err receives the message)
     int wait(long long ticket, int timeout_ms, QueryResult& out, std::string& err);
-    // completed, p50 / p99 / max latency, qps since start, batches, admission
-    // waits, compiles, rejected requests
     std::string statsJson();
     void resetStats();
 
@@ -66,9 +71,8 @@ private:
         long long ticket = 0;
         std::string sql;
         std::chrono::steady_clock::time_point arrival;
-        std::shared_ptr<QueryProgram> program;   // prepared (batch body)
+        std::shared_ptr<QueryProgram> program;   // prepared with the per-tile body
         size_t footprint = 0;
-        bool compiled = false;                   // template module + scan set ready
     };
     struct Done {
         int rc = 0;
@@ -87,28 +91,33 @@ private:
     ServerOptions opt_;
     std::mutex mu_;
     std::condition_variable cv_, done_cv_, compile_cv_;
-    std::deque<Request> queue_;        // runnable or not yet prepared
-    std::deque<Request> compile_q_;    // waiting for modules
+    std::deque<Request> queue_;         // new requests and hot-tier requests
+    std::deque<Request> plain_ready_;   // cold requests whose plain module is compiled
+    std::deque<Request> compile_q_;     // cold requests waiting for their plain module
+    std::deque<std::shared_ptr<QueryProgram>> promote_q_;  // signatures to add to the shared scan
+    std::map<std::string, int> seen_;   // requests per signature
+    std::set<std::string> promoting_;
     std::map<long long, Done> done_;
     long long next_ = 0;
     bool stop_ = false;
     size_t in_flight_bytes_ = 0;
 
-    // templates of the compiled shared-scan set (signature order = kernel order)
+    // templates of the compiled shared-scan set (order = kernel order)
     std::vector<std::shared_ptr<QueryProgram>> scan_set_;
-    bool scan_set_ready_ = false;
 
     std::vector<std::unique_ptr<Lane>> lanes_;
-    std::thread compiler_;
+    std::vector<std::thread> compilers_;
 
     // statistics
     std::vector<double> latencies_;
     std::chrono::steady_clock::time_point stats_start_;
-    long long batches_ = 0, batched_queries_ = 0, admission_waits_ = 0, compiles_ = 0, rejected_ = 0;
+    long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, admission_waits_ = 0, compiles_ = 0,
+              promotions_ = 0, rejected_ = 0;
 
     void start();
     void laneLoop(Lane* lane);
     void compileLoop();
+    bool compileOne();  // one queued compilation job; false when there is none
     void finish(Request& r, int rc, const std::string& err, QueryResult* result);
     bool prepare(Request& r);
     bool inScanSet(const std::string& signature) const;

@@ -1,7 +1,7 @@
 """Build step: embed the device primitive headers into the runtime library.
 
-hipRTC compiles every generated module against crystal_hip.h and
-query_params.h; the runtime passes them as in-memory headers, so the built
+hipRTC compiles every generated module against crystal_hip.h,
+query_params.h and ht_layouts.h; the runtime passes them as in-memory headers, so the built
 library carries their text.  Writes csrc/_gen/device_headers.cpp.
 """
 import pathlib
@@ -14,7 +14,8 @@ DELIM = "CQHDR"
 def main(out_path: str) -> None:
     dev = HERE / "csrc" / "device"
     parts = ["// generated by practice_amd/embed_headers.py -- do not edit\n"]
-    for symbol, name in (("kCrystalHipHeaderSource", "crystal_hip.h"), ("kQueryParamsHeaderSource", "query_params.h")):
+    for symbol, name in (("kCrystalHipHeaderSource", "crystal_hip.h"), ("kQueryParamsHeaderSource", "query_params.h"),
+                         ("kHtLayoutsHeaderSource", "ht_layouts.h")):
         text = (dev / name).read_text()
         if f"){DELIM}\"" in text:
             raise SystemExit(f"delimiter collision in {name}")

@@ -1,0 +1,155 @@
+// Host build of the device hash-table primitives (practice_amd/csrc/device/
+// crystal_hip.h) checked against the reference perfect-hash semantics:
+//   slot = (k - kmin) % len, CAS 0 -> key, payload stored beside the key
+//   (include/crystal/join.h), probe matches iff ht[slot] == fk.
+// For random unique key sets inside the layouts' validity condition, random
+// build filters and payload ranges, every probe key in and around the key
+// range must give the same (match, payload) for the compact and key-indexed
+// layouts as for the reference table, and no insert may write outside the
+// table's words.  Exit status 0 on success; prints the first mismatch.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#define __device__
+#define __forceinline__ inline
+static inline unsigned int atomicOr(unsigned int* p, unsigned int v) {
+    const unsigned int o = *p;
+    *p = o | v;
+    return o;
+}
+static inline int atomicCAS(int* p, int cmp, int v) {
+    const int o = *p;
+    if (o == cmp) *p = v;
+    return o;
+}
+static inline unsigned long long atomicOr(unsigned long long* p, unsigned long long v) {
+    const unsigned long long o = *p;
+    *p = o | v;
+    return o;
+}
+#include "device/ht_layouts.h"
+
+using namespace chip;
+
+static int bits_for(uint64_t r) {
+    int b = 0;
+    while (b < 64 && (r >> b) != 0) ++b;
+    return b;
+}
+
+int main(int argc, char** argv) {
+    const int cases = argc > 1 ? atoi(argv[1]) : 400;
+    std::mt19937_64 rng(12345);
+    long long zseen[4] = {0, 0, 0, 0}, zero_matches = 0;
+    for (int c = 0; c < cases; ++c) {
+        // reference table parameters
+        const int len = 1 + (int)(rng() % 3000);
+        const int kmin = c % 7 == 0 ? -(int)(rng() % 50) : (int)(rng() % 2000) - (c % 3 == 0 ? 0 : 1000);
+        const bool pair = rng() % 4 != 0;
+        // unique keys within `len` consecutive values at or above kmin
+        const int span = 1 + (int)(rng() % len);
+        const int lo = c % 7 == 0 ? kmin + (int)(rng() % (1 - kmin + 1)) : kmin + (int)(rng() % (2 * len + 1));
+        std::vector<int> keys;
+        for (int v = lo; v < lo + span; ++v)
+            if (rng() % 4 != 0 || c % 5 == 0) keys.push_back(v);
+        if (keys.empty()) keys.push_back(lo);
+        const int pmin = (int)(rng() % 100) - 50, prange = c % 4 == 1 ? 0 : (int)(rng() % 40);
+        std::vector<int> pay(keys.size());
+        for (auto& p : pay) p = pmin + (int)(rng() % (prange + 1));
+        std::vector<char> inserted(keys.size());
+        const bool filtered = rng() % 2 == 0;
+        for (size_t i = 0; i < keys.size(); ++i) inserted[i] = !filtered || rng() % 3 != 0;
+
+        // reference: int[2*len], slot (k - kmin) % len
+        std::vector<int> ref(2 * (size_t)len, 0);
+        for (size_t i = 0; i < keys.size(); ++i) {
+            if (!inserted[i]) continue;
+            const int s = (keys[i] - kmin) % len;
+            atomicCAS(&ref[2 * s], 0, keys[i]);
+            ref[2 * s + 1] = pay[i];
+        }
+        // compact layout (tags: which multiple of len above kmin)
+        const int kmn = keys.front(), kmx = keys.back();
+        const int jlo = (kmn - kmin) / len, jhi = (kmx - kmin) / len;
+        const int tb = jhi > jlo ? 1 : 0;
+        const int code_bits = pair ? bits_for((uint64_t)prange) : 0;
+        int eb = 1;
+        while (eb < 1 + tb + code_bits) eb <<= 1;
+        const size_t cwords = ((size_t)len * eb + 31) / 32;
+        std::vector<unsigned int> cbuf(cwords + 16, 0xdeadbeefu);
+        for (size_t i = 0; i < cwords; ++i) cbuf[8 + i] = 0;
+        for (size_t i = 0; i < keys.size(); ++i)
+            if (inserted[i]) ht_insert_compact(&cbuf[8], keys[i], kmin, len, eb, tb, jlo, pair ? pmin : 0, pair ? pay[i] : 0);
+        // key-indexed layout, exact and power-of-two widths
+        const bool all_present = !filtered && (int)keys.size() == kmx - kmn + 1;
+        const int doff = all_present ? 0 : 1;
+        const int dbits = pair ? bits_for((uint64_t)prange + doff) : (all_present ? 0 : 1);
+        const int dspan = kmx - kmn + 1;
+        // key 0's reference slot: which key (if any) shares it
+        int zmode = 0, zx = 0;
+        if (kmin <= 0) {
+            const long long r = (-(long long)kmin) % len, first = kmin + r;
+            const long long t = kmn > first ? (kmn - first + len - 1) / len : 0;
+            const long long k0 = first + t * len;
+            if (k0 > kmx) zmode = 1;
+            else zmode = k0 == 0 ? 3 : 2, zx = (int)(k0 - kmn);
+        }
+        ++zseen[zmode];
+        for (int aligned = 0; aligned < 2; ++aligned) {
+            int db = dbits;
+            if (aligned && db > 0) {
+                int w = 1;
+                while (w < db) w <<= 1;
+                db = w;
+            }
+            const size_t dwords = db ? ((size_t)dspan * db + 31) / 32 + 1 : 0;
+            std::vector<unsigned int> dbuf(dwords + 16, 0xdeadbeefu);
+            for (size_t i = 0; i < dwords; ++i) dbuf[8 + i] = 0;
+            const int pbase = (pair ? pmin : 0) - doff;
+            for (size_t i = 0; i < keys.size(); ++i)
+                if (inserted[i]) ht_insert_dense(&dbuf[8], keys[i], kmn, dspan, db, pbase, pair ? pay[i] : 0);
+            for (int g = 0; g < 8; ++g)
+                if (dbuf[g] != 0xdeadbeefu || dbuf[8 + dwords + g] != 0xdeadbeefu) {
+                    printf("case %d: key-indexed insert wrote outside the table\n", c);
+                    return 1;
+                }
+            for (int fk = kmin - len - 3; fk <= kmin + 3 * len + 3; ++fk) {
+                // reference probe (a negative slot never matches)
+                const int x = fk - kmin;
+                bool want = false;
+                int wp = 0;
+                if (x >= 0) {
+                    const int s = x % len;
+                    want = ref[2 * s] == fk;
+                    wp = pair ? ref[2 * s + 1] : 0;
+                }
+                if (fk == 0 && want) ++zero_matches;
+                int cp = 0, dp = 0;
+                const bool cm = ht_probe_compact(&cbuf[8], fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, cp);
+                if (cm != want || (want && pair && cp != wp)) {
+                    printf("case %d: compact fk=%d got (%d,%d) want (%d,%d)\n", c, fk, cm, cp, want, wp);
+                    return 1;
+                }
+                const bool dm = aligned ? ht_probe_dense<true>(&dbuf[8], fk, kmn, dspan, db, pbase, doff != 0, zmode, zx, dp)
+                                        : ht_probe_dense<false>(&dbuf[8], fk, kmn, dspan, db, pbase, doff != 0, zmode, zx, dp);
+                if (dm != want || (want && pair && dp != wp)) {
+                    printf("case %d: key-indexed(%s) fk=%d got (%d,%d) want (%d,%d) zmode=%d\n", c, aligned ? "pow2" : "exact", fk,
+                           dm, dp, want, wp, zmode);
+                    return 1;
+                }
+            }
+        }
+        for (int g = 0; g < 8; ++g)
+            if (cbuf[g] != 0xdeadbeefu || cbuf[8 + cwords + g] != 0xdeadbeefu) {
+                printf("case %d: compact insert wrote outside the table\n", c);
+                return 1;
+            }
+    }
+    printf("ok %d cases; key-0 modes %lld %lld %lld %lld; key-0 matches %lld\n", cases, zseen[0], zseen[1], zseen[2],
+           zseen[3], zero_matches);
+    return 0;
+}
