@@ -1061,14 +1061,16 @@ static int vvalid(const rowctx* rc, const expr* e) {
     }
 }
 
-/* translateMathExprPush: the leftmost leaf is cast to unsigned long long,
- * right operands keep their C type, operators are plain C operators. */
+/* translateMathExprPush: the leftmost leaf is cast to unsigned long long
+ * when it is a column, *, IS [NOT] NULL or a predicate; literals keep their C
+ * type (visitor.cpp:1961-1968 print them uncast), right operands keep theirs,
+ * operators are plain C operators. */
 static cval mexpr(const rowctx* rc, const expr* e, int cast) {
     cval r;
     switch (e->k) {
         case E_COL: r = mk_i32(col_value(rc, e)); break;
-        case E_INT: r = int_literal(e->ival); break;
-        case E_FLOAT: r = float_literal(e->fval); break;
+        case E_INT: return int_literal(e->ival);
+        case E_FLOAT: return float_literal(e->fval);
         case E_STAR: r = mk_i32(1); break;
         case E_ADD: case E_SUB: case E_MUL: case E_DIV:
             return arith(e->k, mexpr(rc, e->a, cast), mexpr(rc, e->b, 0));
@@ -1078,7 +1080,7 @@ static cval mexpr(const rowctx* rc, const expr* e, int cast) {
         case E_AND: case E_OR: case E_NOT: case E_CASE:
             r = vexpr(rc, e);
             break;
-        default: r = mk_i32(0); break;
+        default: return mk_i32(0); /* NULL literal: "0", uncast */
     }
     return cast ? conv(r, T_U64) : r;
 }
@@ -1222,7 +1224,7 @@ static void aggregate_row(const rowctx* rc, aggstate* s) {
             const char* fn = a->aggfn[i];
             int valid = a->aggs[i]->k == E_STAR ? 1 : vvalid(rc, a->aggs[i]);
             if (!valid) continue;
-            uint64_t val = !strcmp(fn, "COUNT") ? 1ULL : mexpr(rc, a->aggs[i], 1).u;
+            uint64_t val = !strcmp(fn, "COUNT") ? 1ULL : conv(mexpr(rc, a->aggs[i], 1), T_U64).u;
             if (!strcmp(fn, "MIN")) { if (val < s->res[i]) s->res[i] = val; }
             else if (!strcmp(fn, "MAX")) { if (val > s->res[i]) s->res[i] = val; }
             else s->res[i] += val;
@@ -1245,7 +1247,7 @@ static void aggregate_row(const rowctx* rc, aggstate* s) {
         const int slot = a->ngroups + i;
         int valid = a->aggs[i]->k == E_STAR ? 1 : vvalid(rc, a->aggs[i]);
         if (valid) {
-            uint64_t val = !strcmp(fn, "COUNT") ? 1ULL : mexpr(rc, a->aggs[i], 1).u;
+            uint64_t val = !strcmp(fn, "COUNT") ? 1ULL : conv(mexpr(rc, a->aggs[i], 1), T_U64).u;
             uint64_t* ref = &s->res[out + slot];
             if (!strcmp(fn, "MIN")) { if (val < *ref) *ref = val; }
             else if (!strcmp(fn, "MAX")) { if (val > *ref) *ref = val; }

@@ -105,11 +105,17 @@ def _check(rc: int) -> None:
 
 
 def _string_out(fn, *args) -> str:
+    """Two-call string protocol; the text may grow between the calls (live
+    statistics), so the buffer is re-sized until it holds the whole of it."""
     need = ctypes.c_size_t(0)
     _check(fn(*args, None, 0, ctypes.byref(need)))
-    buf = ctypes.create_string_buffer(need.value)
-    _check(fn(*args, buf, need.value, ctypes.byref(need)))
-    return buf.value.decode()
+    size = need.value + 256
+    while True:
+        buf = ctypes.create_string_buffer(size)
+        _check(fn(*args, buf, size, ctypes.byref(need)))
+        if need.value <= size:
+            return buf.value.decode()
+        size = need.value + 256
 
 
 @dataclass

@@ -118,6 +118,13 @@ EXTRA = [
     "order by d_year, c_nation",
     "select p_brand1, sum(lo_revenue) from lineorder, part where lo_partkey = p_partkey and p_brand1 >= 100 "
     "and p_brand1 <= 107 group by p_brand1 order by p_brand1",
+    # leftmost literal: C int arithmetic; leftmost column: unsigned long long
+    "select sum((0 - lo_discount) / 3) from lineorder",
+    "select sum((lo_discount - 20) / 3) from lineorder",
+    "select lo_tax, sum((0 - lo_quantity) / 7), count(*) from lineorder group by lo_tax order by lo_tax",
+    # key-indexed tables with build filters and nullable payloads in one plan
+    "select s_nation, c_nation, count(*) from lineorder, supplier, customer where lo_suppkey = s_suppkey "
+    "and lo_custkey = c_custkey and s_region = 2 and c_region = 3 group by s_nation, c_nation",
 ]
 
 

@@ -168,6 +168,19 @@ def test_c1_closed_form(small_ssb):
     assert odb.execute(db.explain(Q.C1)).tuples() == [(want,)]
 
 
+def test_aggregate_literal_stays_int_cast_column_goes_u64(small_ssb):
+    # translateMathExprPush casts only a leftmost COLUMN (or *, predicate) to
+    # unsigned long long; a leftmost literal keeps C int arithmetic
+    # (visitor.cpp:1954-1968), which changes what division does to negatives
+    data, odb, db = small_ssb
+    d = data["LINEORDER"]["lo_discount"][0].astype(np.int64)
+    m64 = (1 << 64) - 1
+    lit_first = sum(int(-x / 3) if x else 0 for x in d.tolist()) & m64   # int: truncates toward zero
+    col_first = sum((((x - 20) & m64) // 3) for x in d.tolist()) & m64    # u64 wrap, then divide
+    assert odb.execute(db.explain("select sum((0 - lo_discount) / 3) from lineorder")).tuples() == [(lit_first,)]
+    assert odb.execute(db.explain("select sum((lo_discount - 20) / 3) from lineorder")).tuples() == [(col_first,)]
+
+
 def test_c2_closed_form(small_ssb):
     data, odb, db = small_ssb
     lo, p, s, d = data["LINEORDER"], data["PART"], data["SUPPLIER"], data["DDATE"]
