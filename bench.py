@@ -142,7 +142,9 @@ def bench_batched(pa, Q, args):
             "speedup": round(serial_s / batched_s, 2), "batch_ms": round(batched_s * 1000, 3),
             "serial_ms": round(serial_s * 1000, 3), "shared_scans": rep["shared_scans"],
             "block_threads": tuned["block_threads"], "vectors_per_thread": tuned["vectors_per_thread"],
-            "graph_replayed": rep["scans"][0]["graph_replayed"], "bit_exact_vs_serial": exact}
+            "graph": rep["scans"][0]["graph"], "scan_ms_last": rep["scans"][0]["scan_ms"],
+            "device_ms_last": rep["scans"][0]["device_ms"], "host_ms_last": rep["scans"][0]["host_ms"],
+            "plan_ms_last": rep["plan_ms"], "bit_exact_vs_serial": exact}
 
 
 def bench_served(pa, Q, args):
@@ -202,7 +204,8 @@ def bench_served(pa, Q, args):
             "requests": n, "qps": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
             "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2), "serial_qps": round(serial_qps, 1),
             "speedup": round(st["completed"] / wall / serial_qps, 2), "mean_batch": round(st["mean_batch"], 1),
-            "compiles": st["compiles"], "errors": errors}
+            "compiles": st["compiles"], "plain_queries": st["plain_queries"], "batches": st["batches"],
+            "promotions": st["promotions"], "errors": errors}
 
 
 def copy_calibration(nbytes, steps):

@@ -247,7 +247,7 @@ private:
     std::vector<int> flag_dyn_, flag_static_;
     std::map<std::string, int> fact_slot_;
     std::map<std::string, int> dim_slot_;
-    int visible_joins_ = 0;
+    unsigned visible_mask_ = 0;  // joins whose payload registers exist at this point of the pipeline
     // GROUP BY hash parameters
     std::vector<int64_t> gmin_;
     std::vector<uint64_t> gcard_;
@@ -713,8 +713,8 @@ private:
         return s;
     }
     int payloadJoin(const std::string& col) const {
-        for (int j = 0; j < visible_joins_; ++j)
-            if (prog_.joins[static_cast<size_t>(j)].payload_col == col) return j;
+        for (int j = 0; j < static_cast<int>(prog_.joins.size()); ++j)
+            if (((visible_mask_ >> j) & 1u) && prog_.joins[static_cast<size_t>(j)].payload_col == col) return j;
         return -1;
     }
     std::string colValue(const std::string& col) {
@@ -901,6 +901,61 @@ private:
         return "(int)((" + h + ") % " + std::to_string(prog_.groups) + ")";
     }
 
+    // Fraction of dimension rows a build filter keeps, from its structure
+    // and catalog cardinalities only (never from the constants, so queries
+    // that differ in constants keep one plan signature).
+    double buildSelectivity(const ExprNode* e) const {
+        if (!e) return 1.0;
+        const ExprType t = e->getType();
+        if (t == ExprType::OP_AND || t == ExprType::OP_OR) {
+            const auto* b = static_cast<const BinaryExpr*>(e);
+            const double l = buildSelectivity(b->left.get()), r = buildSelectivity(b->right.get());
+            return t == ExprType::OP_AND ? l * r : std::min(1.0, l + r);
+        }
+        const auto* c = static_cast<const BinaryExpr*>(e);
+        const ExprNode* col = c->left->getType() == ExprType::COLUMN_REF ? c->left.get() : c->right.get();
+        const auto* st = stats(static_cast<const ColumnRefExpr*>(col)->column_name);
+        const double card = st && st->cardinality_ > 0 ? static_cast<double>(st->cardinality_) : 10.0;
+        switch (c->op_type) {
+            case ExprType::OP_EQ: return 1.0 / card;
+            case ExprType::OP_NEQ: return 1.0 - 1.0 / card;
+            default: return 0.5;
+        }
+    }
+    // Probe cost relative to an LDS probe: tables beyond the 32 KiB L1 go to L2.
+    double probeCost(const JoinProgram& jp) const { return jp.lds ? 1.0 : (jp.ht_words * 4 <= 32 * 1024 ? 2.0 : 8.0); }
+
+    // Execution order of the pipeline: every filter and probe only removes
+    // rows, so the order does not change the result.  Filters on fact
+    // columns first, then probes by rows removed per unit cost (the classic
+    // predicate-ordering rank), then filters that read payload registers.
+    std::vector<Op> pipelineOrder() const {
+        std::vector<Op> first, joins, last;
+        for (const Op& op : ops_) {
+            if (op.kind == 1) {
+                joins.push_back(op);
+                continue;
+            }
+            std::vector<std::string> cols;
+            collectColumns(op.pred, cols);
+            bool fact_only = true;
+            for (const auto& c : cols) fact_only = fact_only && tableOf(c) == prog_.fact_table;
+            (fact_only ? first : last).push_back(op);
+        }
+        std::stable_sort(joins.begin(), joins.end(), [&](const Op& a, const Op& b) {
+            const JoinProgram& ja = prog_.joins[static_cast<size_t>(a.join)];
+            const JoinProgram& jb = prog_.joins[static_cast<size_t>(b.join)];
+            const double ra = (1.0 - buildSelectivity(build_pred_[static_cast<size_t>(a.join)])) / probeCost(ja);
+            const double rb = (1.0 - buildSelectivity(build_pred_[static_cast<size_t>(b.join)])) / probeCost(jb);
+            if (ra != rb) return ra > rb;
+            return probeCost(ja) < probeCost(jb);
+        });
+        std::vector<Op> out = first;
+        out.insert(out.end(), joins.begin(), joins.end());
+        out.insert(out.end(), last.begin(), last.end());
+        return out;
+    }
+
     void emitBuild(std::ostringstream& src, int j) {
         JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
         const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
@@ -987,8 +1042,8 @@ private:
 
         // ---- pipeline body first: it decides which fact columns are loaded
         std::ostringstream body;
-        visible_joins_ = 0;
-        for (const Op& op : ops_) {
+        visible_mask_ = 0;
+        for (const Op& op : pipelineOrder()) {
             if (op.kind == 0) {
                 if (op.fast) {
                     const std::string c = fastCond(op.pred);
@@ -1028,7 +1083,7 @@ private:
                          << jp.len << ", " << (jp.pair ? "true" : "false") << ", p" << j << "[i])) fl &= ~(1u << i);\n";
                 body << "        }\n";
                 body << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
-                ++visible_joins_;
+                visible_mask_ |= 1u << op.join;
             }
         }
 
@@ -1276,8 +1331,8 @@ private:
         if (!prog_.grouped) return;
         const auto* a = agg_;
         // aggregation runs after every probe: all payload registers visible
-        const int saved_visible = visible_joins_;
-        visible_joins_ = static_cast<int>(prog_.joins.size());
+        const unsigned saved_visible = visible_mask_;
+        visible_mask_ = (1u << prog_.joins.size()) - 1u;
         if (!prog_.fast_sparse) {
             for (int g = 0; g < prog_.ngroups; ++g) {
                 const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
@@ -1295,7 +1350,7 @@ private:
                 }
             }
         }
-        visible_joins_ = saved_visible;
+        visible_mask_ = saved_visible;
         const int need = 1 + static_cast<int>(flag_dyn_.size());
         if (need > 32) return;  // no LDS aggregation
         flag_w_ = 1;

@@ -208,8 +208,16 @@ int typeCode(LogicalType t) {
 
 }  // namespace
 
+void* ScratchArena::take(size_t bytes) {
+    const size_t off = (used_ + 255) & ~static_cast<size_t>(255);
+    used_ = off + std::max<size_t>(bytes, 8);
+    if (!base_) return nullptr;  // measuring pass
+    if (used_ > capacity_) throw std::runtime_error("scratch arena overflow");
+    return static_cast<char*>(base_) + off;
+}
+
 void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, ExecutionContext* ctx,
-                             const std::string& prefix, QueryRun& run) {
+                             const std::string& prefix, QueryRun& run, ScratchArena* arena) {
     hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
     if (!ctx->columns) throw std::runtime_error("ExecutionContext has no column registry");
     run.module = module;
@@ -236,11 +244,18 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
     for (size_t i = 0; i < p.lits.size(); ++i) P.lit[i] = p.lits[i];
     for (size_t i = 0; i < p.flits.size(); ++i) P.flit[i] = p.flits[i];
 
+    // device scratch: the batch's arena when given, else named buffers of ctx
+    auto take = [&](const std::string& name, size_t bytes) -> void* {
+        if (arena) return arena->take(bytes);
+        DeviceBuffer& b = ctx->scratch(prefix + name);
+        b.ensure(std::max<size_t>(bytes, 8));
+        return b.data();
+    };
+    auto take64 = [&](const std::string& name, size_t bytes) { return static_cast<unsigned long long*>(take(name, bytes)); };
+
     run.builds.clear();
     for (size_t j = 0; j < p.joins.size(); ++j) {
-        DeviceBuffer& b = ctx->scratch(prefix + "ht" + std::to_string(j));
-        b.ensure(std::max<size_t>(p.joins[j].ht_words * 4, 16));
-        P.ht[j] = static_cast<unsigned int*>(b.data());
+        P.ht[j] = static_cast<unsigned int*>(take("ht" + std::to_string(j), std::max<size_t>(p.joins[j].ht_words * 4, 16)));
         P.dim_rows[j] = p.joins[j].dim_rows;
         run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
     }
@@ -251,22 +266,14 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
     run.G = G;
     const size_t slots = static_cast<size_t>(G) * static_cast<size_t>(p.storage_ts);
     run.slots = slots;
-    DeviceBuffer& rb = ctx->scratch(prefix + "result");
-    DeviceBuffer& vb = ctx->scratch(prefix + "result_valid");
-    rb.ensure(slots * 8);
-    vb.ensure(((slots + 63) / 64) * 8);
-    P.result = static_cast<unsigned long long*>(rb.data());
-    P.result_valid = static_cast<unsigned long long*>(vb.data());
+    P.result = take64("result", slots * 8);
+    P.result_valid = take64("result_valid", ((slots + 63) / 64) * 8);
     run.res = P.result;
     run.rv = P.result_valid;
     if (p.has_avg) {
         const size_t vslots = static_cast<size_t>(G) * static_cast<size_t>(p.visible_ts);
-        DeviceBuffer& a = ctx->scratch(prefix + "vresult");
-        DeviceBuffer& b = ctx->scratch(prefix + "vresult_valid");
-        a.ensure(vslots * 8);
-        b.ensure(((vslots + 63) / 64) * 8);
-        run.res = static_cast<unsigned long long*>(a.data());
-        run.rv = static_cast<unsigned long long*>(b.data());
+        run.res = take64("vresult", vslots * 8);
+        run.rv = take64("vresult_valid", ((vslots + 63) / 64) * 8);
     }
 
     // materialization buffers
@@ -274,21 +281,17 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
     run.single_cell = p.fast_sparse && vts <= 1;
     run.cap_rows = std::max<long long>(G, 1);
     run.vwords = (run.cap_rows + 63) / 64;
-    DeviceBuffer& ob = ctx->scratch(prefix + "out");
-    DeviceBuffer& ovb = ctx->scratch(prefix + "out_valid");
-    DeviceBuffer& flags = ctx->scratch(prefix + "flags");
-    DeviceBuffer& offsets = ctx->scratch(prefix + "offsets");
-    DeviceBuffer& bsums = ctx->scratch(prefix + "block_sums");
-    ob.ensure(static_cast<size_t>(vts) * run.cap_rows * 8);
-    ovb.ensure(static_cast<size_t>(vts) * run.vwords * 8);
-    flags.ensure(run.cap_rows * 8);
-    offsets.ensure(run.cap_rows * 8 + 8);
-    bsums.ensure(static_cast<size_t>(cq_scan_block_count(run.cap_rows) + 1) * 8);
-    run.o = static_cast<unsigned long long*>(ob.data());
-    run.ov = static_cast<unsigned long long*>(ovb.data());
-    run.fl = static_cast<unsigned long long*>(flags.data());
-    run.off = static_cast<unsigned long long*>(offsets.data());
-    run.bsums = static_cast<unsigned long long*>(bsums.data());
+    run.o = take64("out", static_cast<size_t>(vts) * run.cap_rows * 8);
+    run.ov = take64("out_valid", static_cast<size_t>(vts) * run.vwords * 8);
+    run.fl = take64("flags", run.cap_rows * 8);
+    run.off = take64("offsets", run.cap_rows * 8 + 8);
+    run.bsums = take64("block_sums", static_cast<size_t>(cq_scan_block_count(run.cap_rows) + 1) * 8);
+    run.sidx = run.o2 = run.ov2 = nullptr;
+    if (p.has_sort_node && !p.sort_keys.empty()) {
+        run.sidx = take64("sort_idx", static_cast<size_t>(cq_sort_padded(run.cap_rows)) * 8);
+        run.o2 = take64("out2", static_cast<size_t>(vts) * run.cap_rows * 8);
+        run.ov2 = take64("out2_valid", static_cast<size_t>(vts) * run.vwords * 8);
+    }
 
     const int64_t row_end = ctx->row_end < 0 ? p.fact_rows : std::min<int64_t>(ctx->row_end, p.fact_rows);
     if (ctx->row_begin < 0 || ctx->row_begin % 4 != 0) throw std::runtime_error("scan range must start at a multiple of 4 rows");
@@ -388,24 +391,21 @@ void HipModuleExecutor::shape(QueryRun& run, ExecutionContext* ctx, const cq_que
              "write rows");
 }
 
-void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found) {
+size_t HipModuleExecutor::fetchBytes(const QueryRun& run, unsigned long long rows) const {
+    const QueryProgram& p = *run.program;
+    if (run.single_cell) return 8;
+    const size_t r = static_cast<size_t>(std::min<unsigned long long>(rows, static_cast<unsigned long long>(run.cap_rows)));
+    return static_cast<size_t>(p.visible_ts) * (r * 8 + static_cast<size_t>(run.vwords) * 8);
+}
+
+void HipModuleExecutor::fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found, void* host) {
     const QueryProgram& p = *run.program;
     hipStream_t s = ctx->stream;
     const int vts = p.visible_ts;
-    QueryResult& out = run.result;
-    out.columns.clear();
-    for (const auto& oc : p.out) out.columns.push_back(ResultColumnDesc{oc.type, oc.nullable, oc.name});
-    out.data.assign(static_cast<size_t>(vts), {});
-    out.validity.assign(static_cast<size_t>(vts), {});
-    out.ordered = !p.fast_sparse || run.single_cell;
-    out.dense_result = !p.fast_sparse;
+    run.fetch_host = static_cast<char*>(host);
     if (run.single_cell) {
-        // scalar sparse output: one row, the raw slot (SUM over no rows = 0)
-        out.row_count = 1;
-        out.data[0].resize(1);
-        hipCheck(hipMemcpyAsync(out.data[0].data(), run.res, 8, hipMemcpyDeviceToHost, s), "fetch");
-        out.validity[0].assign(1, 1);
-        hipCheck(hipStreamSynchronize(s), "sync");
+        run.rows = 1;
+        hipCheck(hipMemcpyAsync(host, run.res, 8, hipMemcpyDeviceToHost, s), "fetch");
         return;
     }
     long long rows = static_cast<long long>(rows_found);
@@ -413,12 +413,17 @@ void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned lon
     unsigned long long* ov = run.ov;
     const long long cap_rows = run.cap_rows, vwords = run.vwords;
     if (p.has_sort_node && !p.sort_keys.empty() && rows > 1) {
-        DeviceBuffer& ib = ctx->scratch(run.prefix + "sort_idx");
-        DeviceBuffer& o2 = ctx->scratch(run.prefix + "out2");
-        DeviceBuffer& ov2 = ctx->scratch(run.prefix + "out2_valid");
-        ib.ensure(static_cast<size_t>(cq_sort_padded(rows)) * 8);
-        o2.ensure(static_cast<size_t>(vts) * cap_rows * 8);
-        ov2.ensure(static_cast<size_t>(vts) * vwords * 8);
+        if (!run.sidx) {
+            DeviceBuffer& ib = ctx->scratch(run.prefix + "sort_idx");
+            DeviceBuffer& o2 = ctx->scratch(run.prefix + "out2");
+            DeviceBuffer& ov2 = ctx->scratch(run.prefix + "out2_valid");
+            ib.ensure(static_cast<size_t>(cq_sort_padded(cap_rows)) * 8);
+            o2.ensure(static_cast<size_t>(vts) * cap_rows * 8);
+            ov2.ensure(static_cast<size_t>(vts) * vwords * 8);
+            run.sidx = static_cast<unsigned long long*>(ib.data());
+            run.o2 = static_cast<unsigned long long*>(o2.data());
+            run.ov2 = static_cast<unsigned long long*>(ov2.data());
+        }
         cq_sort_desc d;
         std::memset(&d, 0, sizeof d);
         d.nkeys = static_cast<int>(std::min<size_t>(p.sort_keys.size(), 16));
@@ -429,32 +434,60 @@ void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned lon
             d.nullable[k] = p.out[static_cast<size_t>(c)].nullable ? 1 : 0;
             d.desc[k] = p.sort_keys[static_cast<size_t>(k)].descending ? 1 : 0;
         }
-        auto* idx = static_cast<unsigned long long*>(ib.data());
-        hipCheck(cq_launch_sort(idx, rows, &d, o, ov, cap_rows, vwords, s), "sort");
-        auto* o2p = static_cast<unsigned long long*>(o2.data());
-        auto* ov2p = static_cast<unsigned long long*>(ov2.data());
-        hipCheck(cq_launch_fill_u64(ov2p, 0ull, static_cast<long long>(vts) * vwords, s), "fill");
-        hipCheck(cq_launch_gather(o, ov, idx, rows, vts, cap_rows, vwords, o2p, ov2p, s), "gather");
-        o = o2p;
-        ov = ov2p;
+        hipCheck(cq_launch_sort(run.sidx, rows, &d, o, ov, cap_rows, vwords, s), "sort");
+        hipCheck(cq_launch_fill_u64(run.ov2, 0ull, static_cast<long long>(vts) * vwords, s), "fill");
+        hipCheck(cq_launch_gather(o, ov, run.sidx, rows, vts, cap_rows, vwords, run.o2, run.ov2, s), "gather");
+        o = run.o2;
+        ov = run.ov2;
     }
     if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
-    out.row_count = static_cast<size_t>(rows);
-    std::vector<uint64_t> vbits(static_cast<size_t>(vts) * vwords);
-    for (int c = 0; c < vts; ++c) {
-        out.data[static_cast<size_t>(c)].resize(static_cast<size_t>(rows));
-        if (rows)
-            hipCheck(hipMemcpyAsync(out.data[static_cast<size_t>(c)].data(), o + static_cast<size_t>(c) * cap_rows, rows * 8,
-                                    hipMemcpyDeviceToHost, s), "fetch column");
+    run.rows = rows;
+    // host layout: [column][rows] values, then [column][vwords] validity words
+    char* h = static_cast<char*>(host);
+    for (int c = 0; c < vts && rows; ++c)
+        hipCheck(hipMemcpyAsync(h + static_cast<size_t>(c) * rows * 8, o + static_cast<size_t>(c) * cap_rows, rows * 8,
+                                hipMemcpyDeviceToHost, s), "fetch column");
+    if (vts) hipCheck(hipMemcpyAsync(h + static_cast<size_t>(vts) * rows * 8, ov, static_cast<size_t>(vts) * vwords * 8,
+                                     hipMemcpyDeviceToHost, s), "fetch validity");
+}
+
+void HipModuleExecutor::fetchFinish(QueryRun& run) {
+    const QueryProgram& p = *run.program;
+    const int vts = p.visible_ts;
+    QueryResult& out = run.result;
+    out.columns.clear();
+    for (const auto& oc : p.out) out.columns.push_back(ResultColumnDesc{oc.type, oc.nullable, oc.name});
+    out.data.assign(static_cast<size_t>(vts), {});
+    out.validity.assign(static_cast<size_t>(vts), {});
+    out.ordered = !p.fast_sparse || run.single_cell;
+    out.dense_result = !p.fast_sparse;
+    const char* h = run.fetch_host;
+    if (run.single_cell) {
+        out.row_count = 1;
+        out.data[0].resize(1);
+        std::memcpy(out.data[0].data(), h, 8);
+        out.validity[0].assign(1, 1);
+        return;
     }
-    if (!vbits.empty()) hipCheck(hipMemcpyAsync(vbits.data(), ov, vbits.size() * 8, hipMemcpyDeviceToHost, s), "fetch validity");
-    hipCheck(hipStreamSynchronize(s), "sync");
+    const long long rows = run.rows;
+    out.row_count = static_cast<size_t>(rows);
+    const auto* vbits = reinterpret_cast<const uint64_t*>(h + static_cast<size_t>(vts) * rows * 8);
     for (int c = 0; c < vts; ++c) {
+        auto& dv = out.data[static_cast<size_t>(c)];
+        dv.resize(static_cast<size_t>(rows));
+        if (rows) std::memcpy(dv.data(), h + static_cast<size_t>(c) * rows * 8, static_cast<size_t>(rows) * 8);
         auto& vv = out.validity[static_cast<size_t>(c)];
         vv.resize(static_cast<size_t>(rows));
-        for (long long r = 0; r < rows; ++r)
-            vv[static_cast<size_t>(r)] = static_cast<uint8_t>((vbits[static_cast<size_t>(c) * vwords + (r >> 6)] >> (r & 63)) & 1ull);
+        const uint64_t* w = vbits + static_cast<size_t>(c) * run.vwords;
+        for (long long r = 0; r < rows; ++r) vv[static_cast<size_t>(r)] = static_cast<uint8_t>((w[r >> 6] >> (r & 63)) & 1ull);
     }
+}
+
+void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found) {
+    std::vector<char> host(fetchBytes(run, rows_found) + 8);
+    fetchLaunch(run, ctx, rows_found, host.data());
+    hipCheck(hipStreamSynchronize(ctx->stream), "sync");
+    fetchFinish(run);
 }
 
 void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ctx) {
@@ -467,7 +500,7 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
         hipCheck(hipEventCreate(&ke), "hipEventCreate");
     }
     QueryRun run;
-    bind(module, ctx->program, ctx, "", run);
+    bind(module, ctx->program, ctx, "", run, nullptr);
     hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
     setup(run, ctx, nullptr);
     if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");

@@ -229,7 +229,25 @@ struct QueryRun {
     bool single_cell = false;
     long long cap_rows = 1, vwords = 1;
     unsigned long long *o = nullptr, *ov = nullptr, *fl = nullptr, *off = nullptr, *bsums = nullptr;
+    unsigned long long *sidx = nullptr, *o2 = nullptr, *ov2 = nullptr;  // ORDER BY buffers
+    long long rows = 0;              // result rows after LIMIT (fetchLaunch)
+    char* fetch_host = nullptr;      // host staging of the result (fetchLaunch)
     QueryResult result;
+};
+
+// Sub-allocates the device scratch of many runs from one block (256-byte
+// aligned).  With no base it only measures: bind() every run once to size
+// the block, allocate it, then bind() again with the base set.
+class ScratchArena {
+public:
+    void measure() { base_ = nullptr, capacity_ = 0, used_ = 0; }
+    void assign(void* base, size_t capacity) { base_ = base, capacity_ = capacity, used_ = 0; }
+    void* take(size_t bytes);
+    size_t used() const { return used_; }
+
+private:
+    void* base_ = nullptr;
+    size_t capacity_ = 0, used_ = 0;
 };
 
 // Executes a module in phases.  bind() does host work only (parameter block,
@@ -239,7 +257,7 @@ class HipModuleExecutor : public IExecutor {
 public:
     void execute(const ModuleHandle& module, ExecutionContext* ctx) override;
     void bind(const ModuleHandle& module, const QueryProgram& program, ExecutionContext* ctx, const std::string& prefix,
-              QueryRun& run);
+              QueryRun& run, ScratchArena* arena);
     // zero tables / aggregate slots, build the dimension tables.  With
     // `device_params` the generated kernels read the block from device memory.
     void setup(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params);
@@ -249,6 +267,13 @@ public:
     void shape(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params, unsigned long long* row_count);
     // sort / limit and copy the result to the host (synchronizes)
     void fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows);
+    // the same in two steps, so many runs share one synchronization:
+    // fetchLaunch enqueues sort / gather and the copies into `host`
+    // (fetchBytes(run, rows) bytes, pinned for asynchronous copies),
+    // fetchFinish decodes once the stream has reached that point
+    size_t fetchBytes(const QueryRun& run, unsigned long long rows) const;
+    void fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows, void* host);
+    void fetchFinish(QueryRun& run);
 };
 
 // Persistent grid for a scan kernel: occupancy x CUs, at most one block per tile.

@@ -187,7 +187,10 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         r << (s ? "," : "") << "{\"queries\":" << rep.queries << ",\"templates\":" << rep.templates
           << ",\"union_columns\":" << rep.union_columns << ",\"block_threads\":" << rep.block_threads
           << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
-          << ",\"scan_ms\":" << rep.scan_ms << ",\"device_ms\":" << rep.device_ms << ",\"fact_bytes\":" << rep.fact_bytes << "}";
+          << ",\"graph\":\"" << rep.graph_mode << "\""
+          << ",\"scan_ms\":" << rep.scan_ms << ",\"device_ms\":" << rep.device_ms << ",\"fact_bytes\":" << rep.fact_bytes
+          << ",\"host_ms\":{\"modules\":" << rep.modules_ms << ",\"bind\":" << rep.bind_ms << ",\"enqueue\":" << rep.enqueue_ms
+          << ",\"wait\":" << rep.wait_ms << ",\"fetch\":" << rep.fetch_ms << "},\"arena_bytes\":" << rep.arena_bytes << "}";
     }
     r << "],\"wall_ms\":" << elapsedMs(t0, std::chrono::steady_clock::now()) << "}";
     report_ = r.str();
@@ -237,30 +240,44 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     for (const auto* t : templates) set_key += t->signature + ";";
 
     // per-template modules (dimension builds + result flags)
+    const auto h0 = std::chrono::steady_clock::now();
     std::vector<ModuleHandle> tmodules;
     for (size_t t = 0; t < templates.size(); ++t)
         tmodules.push_back(per_template[t].empty() ? nullptr : engine.moduleFor(*templates[t], nullptr));
 
-    // bind every query into its own scratch namespace
+    // bind every query: a measuring pass sizes one arena for all of their
+    // scratch, the second pass hands out the addresses
+    const auto h1 = std::chrono::steady_clock::now();
     ctx->row_begin = 0;
     ctx->row_end = -1;
     std::vector<QueryRun> runs(static_cast<size_t>(n));
-    const std::string pre = "s" + std::to_string(scan_index) + "/";
+    std::vector<size_t> tix(static_cast<size_t>(n));
     for (int k = 0; k < n; ++k) {
         const int q = qorder[static_cast<size_t>(k)];
         size_t t = 0;
         while (templates[t]->signature != programs[static_cast<size_t>(q)].signature) ++t;
-        ex->bind(tmodules[t], programs[static_cast<size_t>(q)], ctx, pre + "q" + std::to_string(k) + "/", runs[static_cast<size_t>(k)]);
+        tix[static_cast<size_t>(k)] = t;
     }
-    DeviceBuffer& qpb = ctx->scratch(pre + "qp");
-    DeviceBuffer& totb = ctx->scratch(pre + "totals");
-    qpb.ensure(static_cast<size_t>(n) * sizeof(cq_query_params));
-    totb.ensure(static_cast<size_t>(n) * 8);
-    auto* dqp = static_cast<cq_query_params*>(qpb.data());
-    auto* dtot = static_cast<unsigned long long*>(totb.data());
+    const std::string pre = "s" + std::to_string(scan_index) + "/";
+    const size_t qp_bytes = static_cast<size_t>(n) * sizeof(cq_query_params);
+    ScratchArena arena;
+    cq_query_params* dqp = nullptr;
+    unsigned long long* dtot = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 0) arena.measure();
+        else {
+            DeviceBuffer& ab = ctx->scratch(pre + "arena");
+            ab.ensure(arena.used());
+            arena.assign(ab.data(), ab.capacity());
+        }
+        for (int k = 0; k < n; ++k)
+            ex->bind(tmodules[tix[static_cast<size_t>(k)]], programs[static_cast<size_t>(qorder[static_cast<size_t>(k)])], ctx, pre,
+                     runs[static_cast<size_t>(k)], &arena);
+        dqp = static_cast<cq_query_params*>(arena.take(qp_bytes));
+        dtot = static_cast<unsigned long long*>(arena.take(static_cast<size_t>(n) * 8));
+    }
 
     // parameter blocks -> device (pinned staging, stream ordered)
-    const size_t qp_bytes = static_cast<size_t>(n) * sizeof(cq_query_params);
     char* stage = static_cast<char*>(pinned(qp_bytes + static_cast<size_t>(n) * 8));
     for (int k = 0; k < n; ++k) std::memcpy(stage + static_cast<size_t>(k) * sizeof(cq_query_params), &runs[static_cast<size_t>(k)].P,
                                             sizeof(cq_query_params));
@@ -314,8 +331,12 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
         for (int k = 0; k < n; ++k) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
     };
 
+    // A graph pays off for a batch shape that comes back: the first time a
+    // shape is seen it runs eagerly, the second time it is captured, later
+    // it is replayed (only the uploaded parameter blocks differ).
+    const auto h2 = std::chrono::steady_clock::now();
     hipCheck(hipEventRecord(ev_[0], s), "hipEventRecord");
-    bool replayed = false;
+    bool replayed = false, timed = true;
     if (opt.graph) {
         // everything a replay depends on besides the uploaded parameter blocks
         std::ostringstream key;
@@ -323,15 +344,21 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
         for (size_t u = 0; u < ucols.size(); ++u) key << "|" << B.col[u] << "," << B.valid[u];
         for (int k = 0; k < n; ++k) {
             const QueryRun& rr = runs[static_cast<size_t>(k)];
-            key << "|" << rr.program->signature << ":" << rr.P.result << "," << rr.P.result_valid << "," << rr.res << "," << rr.rv
-                << "," << rr.o << "," << rr.ov << "," << rr.fl << "," << rr.off << "," << rr.bsums << "," << rr.P.row_end;
-            for (size_t j = 0; j < rr.program->joins.size(); ++j) key << "," << rr.P.ht[j] << "/" << rr.P.dim_rows[j];
+            key << "|" << rr.program->signature << ":" << rr.P.result << "," << rr.P.row_end;
         }
-        Graph& g = graphs_[pre];
-        if (g.exec && g.key == key.str()) {
+        const std::string ks = key.str();
+        auto git = graphs_.find(ks);
+        if (git != graphs_.end()) {
+            hipCheck(hipGraphLaunch(git->second.exec, s), "hipGraphLaunch");
             replayed = true;
-        } else {
-            dropGraph(g);
+            timed = false;
+            rep.graph_mode = "replayed";
+        } else if (++graph_seen_[ks] >= 2) {
+            if (graphs_.size() >= 8) {  // keep a few shapes
+                dropGraph(graphs_.begin()->second);
+                graphs_.erase(graphs_.begin());
+            }
+            Graph& g = graphs_[ks];
             hipCheck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
                 enqueue(false);
@@ -339,31 +366,49 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
                 hipGraph_t broken = nullptr;
                 (void)hipStreamEndCapture(s, &broken);
                 if (broken) (void)hipGraphDestroy(broken);
+                graphs_.erase(ks);
                 throw;
             }
             hipCheck(hipStreamEndCapture(s, &g.graph), "hipStreamEndCapture");
             hipCheck(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-            g.key = key.str();
+            g.key = ks;
+            hipCheck(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
+            timed = false;
+            rep.graph_mode = "captured";
+        } else {
+            enqueue(true);
         }
-        hipCheck(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
+        if (graph_seen_.size() > 1024) graph_seen_.clear();
     } else {
         enqueue(true);
     }
     hipCheck(hipEventRecord(ev_[1], s), "hipEventRecord");
     auto* htot = reinterpret_cast<unsigned long long*>(stage + qp_bytes);
     hipCheck(hipMemcpyAsync(htot, dtot, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost, s), "copy row counts");
+    const auto h3 = std::chrono::steady_clock::now();
     hipCheck(hipStreamSynchronize(s), "batch sync");
+    const auto h4 = std::chrono::steady_clock::now();
     float dms = 0, sms = 0;
     (void)hipEventElapsedTime(&dms, ev_[0], ev_[1]);
-    if (!opt.graph) (void)hipEventElapsedTime(&sms, ev_[2], ev_[3]);
+    if (timed) (void)hipEventElapsedTime(&sms, ev_[2], ev_[3]);
 
+    // results: sort / gather and copies for every query, one synchronization
     std::vector<unsigned long long> rows(htot, htot + n);
+    std::vector<size_t> off(static_cast<size_t>(n) + 1, 0);
+    for (int k = 0; k < n; ++k)
+        off[static_cast<size_t>(k) + 1] = off[static_cast<size_t>(k)] +
+                                          ((ex->fetchBytes(runs[static_cast<size_t>(k)], rows[static_cast<size_t>(k)]) + 255) & ~size_t(255));
+    char* host = static_cast<char*>(pinned(off.back() + 256));
+    for (int k = 0; k < n; ++k)
+        ex->fetchLaunch(runs[static_cast<size_t>(k)], ctx, rows[static_cast<size_t>(k)], host + off[static_cast<size_t>(k)]);
+    hipCheck(hipStreamSynchronize(s), "fetch sync");
     for (int k = 0; k < n; ++k) {
         QueryRun& rr = runs[static_cast<size_t>(k)];
-        ex->fetch(rr, ctx, rows[static_cast<size_t>(k)]);
+        ex->fetchFinish(rr);
         rr.result.timing.gpu_execute_ms = dms;
         results[static_cast<size_t>(qorder[static_cast<size_t>(k)])] = std::move(rr.result);
     }
+    const auto h5 = std::chrono::steady_clock::now();
 
     rep.queries = n;
     rep.templates = static_cast<int>(templates.size());
@@ -373,6 +418,12 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     rep.graph_replayed = replayed;
     rep.scan_ms = sms;
     rep.device_ms = dms;
+    rep.modules_ms = elapsedMs(h0, h1);
+    rep.bind_ms = elapsedMs(h1, h2);
+    rep.enqueue_ms = elapsedMs(h2, h3);
+    rep.wait_ms = elapsedMs(h3, h4);
+    rep.fetch_ms = elapsedMs(h4, h5);
+    rep.arena_bytes = arena.used();
     const uint64_t N = static_cast<uint64_t>(first.fact_rows);
     for (size_t u = 0; u < ucols.size(); ++u) rep.fact_bytes += 4 * N + (unull[u] ? 8 * ((N + 63) / 64) : 0);
 }

@@ -40,9 +40,13 @@ struct BatchScanReport {
     int queries = 0, templates = 0, union_columns = 0;
     int block_threads = 0, vectors_per_thread = 0;
     bool graph_replayed = false;
+    const char* graph_mode = "eager";  // eager / captured / replayed
     double scan_ms = 0;            // cq_batch_scan alone (events)
     double device_ms = 0;          // builds + scan + shaping (events)
     uint64_t fact_bytes = 0;       // algorithmic fact bytes read by the shared scan
+    // host time per phase
+    double modules_ms = 0, bind_ms = 0, enqueue_ms = 0, wait_ms = 0, fetch_ms = 0;
+    size_t arena_bytes = 0;        // device scratch of the batch (one block)
 };
 
 // Generated sources of a batch, host-only databases included: part < 0 is
@@ -88,7 +92,8 @@ private:
     ExecutionContext* ctx_;
     std::mutex* ctx_mu_;
     std::string report_;
-    std::map<std::string, Graph> graphs_;   // keyed by scan slot
+    std::map<std::string, Graph> graphs_;   // keyed by batch shape
+    std::map<std::string, int> graph_seen_; // shapes seen (captured on the second)
     std::map<std::string, Tuned> tuned_;    // keyed by template-set signature
     double plan_ms_ = 0;                    // SQL -> programs time of the current call
     void* pinned_ = nullptr;                // staging for parameter blocks / row counts

@@ -117,7 +117,11 @@ def test_graph_replay_with_new_constants():
     spec, data, odb, gdb = dataset("sf20")
     first = [Q.c2(c, r) for c, r in [(1, 0), (2, 1), (3, 2), (4, 3)]] + [Q.C1]
     second = [Q.c2(c, r) for c, r in [(20, 4), (21, 3), (22, 2), (23, 1)]] + [Q.C1.replace("1993", "1995")]
+    # a batch shape runs eagerly the first time, is captured the second time
     gdb.batch_execute(first, {"graph": 1})
+    assert gdb.batch_report()["scans"][0]["graph"] == "eager"
+    gdb.batch_execute(first, {"graph": 1})
+    assert gdb.batch_report()["scans"][0]["graph"] == "captured"
     for qs in (second, first, second):
         res = gdb.batch_execute(qs, {"graph": 1})
         assert gdb.batch_report()["scans"][0]["graph_replayed"] == 1
