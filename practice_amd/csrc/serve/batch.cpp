@@ -233,6 +233,20 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     }
     if (templates.empty()) return;
     if (templates.size() > static_cast<size_t>(CQ_MAX_TEMPLATES)) throw std::runtime_error("too many templates in one scan");
+    if (!module_templates) {
+        // canonical order: the module depends on the set of templates only
+        std::vector<size_t> ord(templates.size());
+        for (size_t t = 0; t < ord.size(); ++t) ord[t] = t;
+        std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return templates[a]->signature < templates[b]->signature; });
+        std::vector<const QueryProgram*> t2;
+        std::vector<std::vector<int>> p2;
+        for (size_t t : ord) {
+            t2.push_back(templates[t]);
+            p2.push_back(std::move(per_template[t]));
+        }
+        templates.swap(t2);
+        per_template.swap(p2);
+    }
     std::vector<int> qorder;
     for (const auto& v : per_template) qorder.insert(qorder.end(), v.begin(), v.end());
     const int n = static_cast<int>(qorder.size());
