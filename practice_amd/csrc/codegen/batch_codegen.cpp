@@ -37,8 +37,25 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         << "#define BT " << bt << "\n#define VPT " << vpt << "\n#define IPT " << ipt << "\n#define TILE (BT * IPT)\n\n"
         << "extern \"C\" __global__ __launch_bounds__(BT) void cq_batch_scan(const cq_batch_params B) {\n"
         << "    using namespace chip;\n"
-        << "    const int tid = threadIdx.x;\n"
-        << "    for (long long tile = blockIdx.x; tile < B.num_tiles; tile += gridDim.x) {\n"
+        << "    const int tid = threadIdx.x;\n";
+    // per-query LDS aggregation regions (dynamic shared memory)
+    bool any_lds = false;
+    for (const QueryProgram* t : templates) any_lds = any_lds || t->batch_lds_stride > 0;
+    if (any_lds) {
+        src << "    extern __shared__ u64 cq_lds[];\n";
+        for (size_t t = 0; t < templates.size(); ++t) {
+            const QueryProgram* p = templates[t];
+            if (p->batch_lds_stride == 0) continue;
+            src << "    for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
+                << "        u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
+                << p->batch_lds_stride << "u);\n"
+                << "        u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n"
+                << "        (void)s_flag;\n"
+                << p->batch_lds_init << "    }\n";
+        }
+        src << "    __syncthreads();\n";
+    }
+    src << "    for (long long tile = blockIdx.x; tile < B.num_tiles; tile += gridDim.x) {\n"
         << "        const long long base = B.row_begin + tile * (long long)TILE;\n";
     const size_t nu = union_cols.size();
     for (size_t u = 0; u < nu; ++u) {
@@ -70,20 +87,43 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     for (size_t t = 0; t < templates.size(); ++t) {
         const QueryProgram* p = templates[t];
         src << "        // template " << t << ": " << p->signature << "\n"
-            << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
-            << "            const cq_query_params& P = B.qp[q];\n";
+            << "        if (B.tq_begin[" << t << "] < B.tq_end[" << t << "]) {\n";
         for (size_t s = 0; s < p->fact_cols.size(); ++s) {
             const int u = slot[p->fact_cols[s]];
-            src << "            int (&c" << s << ")[IPT] = u" << u << ";\n";
-            if (p->fact_nullable[s]) src << "            const u32 v" << s << " = uv" << u << ";\n";
+            src << "        int (&c" << s << ")[IPT] = u" << u << ";\n";
+            if (p->fact_nullable[s]) src << "        const u32 v" << s << " = uv" << u << ";\n";
         }
+        if (!p->batch_shared.empty())
+            src << "        const cq_query_params& P0 = B.qp[B.tq_begin[" << t << "]];\n" << p->batch_shared;
+        src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
+            << "            const cq_query_params& P = B.qp[q];\n";
+        if (p->batch_lds_stride > 0)
+            src << "            u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
+                << p->batch_lds_stride << "u);\n"
+                << "            u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n";
         src << "            u32 fl = tile_fl;\n"
             << "            do {\n"
             << p->batch_body
             << "            } while (0);\n"
+            << "        }\n"
             << "        }\n";
     }
-    src << "    }\n}\n";
+    src << "    }\n";
+    if (any_lds) {
+        // merge every query's LDS partials into its global slots
+        src << "    __syncthreads();\n";
+        for (size_t t = 0; t < templates.size(); ++t) {
+            const QueryProgram* p = templates[t];
+            if (p->batch_lds_stride == 0) continue;
+            src << "    for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
+                << "        const cq_query_params& P = B.qp[q];\n"
+                << "        u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
+                << p->batch_lds_stride << "u);\n"
+                << "        u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n"
+                << p->batch_lds_flush << "    }\n";
+        }
+    }
+    src << "}\n";
     return src.str();
 }
 

@@ -858,11 +858,15 @@ private:
 
     // build-side predicate on raw dimension values
     std::string buildCond(const ExprNode* e) {
+        return buildCondWith(e, [&](const std::string& name) { return "P.dcol[" + std::to_string(dimSlot(name)) + "][r]"; });
+    }
+    // the same predicate over another source of the raw dimension values
+    std::string buildCondWith(const ExprNode* e, const std::function<std::string(const std::string&)>& value) {
         const ExprType t = e->getType();
         if (t == ExprType::OP_AND || t == ExprType::OP_OR) {
             const auto* b = static_cast<const BinaryExpr*>(e);
-            const std::string l = buildCond(b->left.get());
-            return "(" + l + (t == ExprType::OP_AND ? " && " : " || ") + buildCond(b->right.get()) + ")";
+            const std::string l = buildCondWith(b->left.get(), value);
+            return "(" + l + (t == ExprType::OP_AND ? " && " : " || ") + buildCondWith(b->right.get(), value) + ")";
         }
         const auto* c = static_cast<const BinaryExpr*>(e);
         const bool col_left = c->left->getType() == ExprType::COLUMN_REF;
@@ -873,7 +877,7 @@ private:
         std::string litcode;
         if (l->getType() == ExprType::LITERAL_INT) litcode = litInt32(static_cast<const LiteralIntExpr*>(l)->value);
         else litcode = "((int)" + flit(static_cast<const LiteralFloatExpr*>(l)->value) + ")";
-        return "(P.dcol[" + std::to_string(dimSlot(name)) + "][r] " + cmpOp(op) + " " + litcode + ")";
+        return "(" + value(name) + " " + cmpOp(op) + " " + litcode + ")";
     }
 
     // ---------------------------------------------------------------- emission
@@ -992,6 +996,10 @@ private:
         // LDS aggregation when the domain decodes from the group index
         const bool batch = opt_.batch_body;
         computeFlagLayout();
+        if (batch && sharedEligible()) {
+            emitBatchShared(src);
+            return;
+        }
         bool lds_agg = grouped && opt_.allow_lds_aggregation && !batch && flag_w_ > 0 && prog_.storage_ts <= 64;
         for (size_t g = 0; g < gdense_.size(); ++g) lds_agg = lds_agg && gdense_[g];
         const uint64_t agg_lds = grouped ? aggLdsBytes() : 0;
@@ -1102,6 +1110,7 @@ private:
         for (int i = 0; i < prog_.naggs && !grouped; ++i)
             if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
 
+        if (prog_.keys_kernel) emitKeysKernel(src);
         // ---- kernel (the batched path runs the body inside cq_batch_scan)
         if (batch) {
             emitDenseCount(src);
@@ -1197,6 +1206,205 @@ private:
                 src << ind << "        if (in) " << pre << "v" << s << " |= chip::bit32(P.valid[" << s << "], r) << i;\n";
         }
         src << ind << "    }\n" << ind << "}\n";
+    }
+
+    // ------------------------------------------------ batched templates
+    // Shared-probe form: possible when every join has the key-indexed
+    // layout and its attributes (payload + build-filter columns) pack into
+    // one 32-bit entry.
+    bool sharedEligible() {
+        if (!opt_.batch_shared_tables) return false;
+        for (size_t j = 0; j < prog_.joins.size(); ++j) {
+            JoinProgram& jp = prog_.joins[j];
+            if (!jp.dense_ok) return false;
+            jp.s_cols.clear();
+            jp.s_min.clear();
+            jp.s_bits.clear();
+            jp.s_off.clear();
+            jp.s_payload = -1;
+            std::vector<std::string> cols;
+            if (jp.pair) cols.push_back(jp.payload_col);
+            if (build_pred_[j]) collectColumns(build_pred_[j], cols);
+            const auto& tm = meta(jp.dim_table);
+            const auto* ks = stats(jp.pk_col);
+            jp.s_present = (static_cast<uint64_t>(jp.dspan) == tm.getSize() && ks->cardinality_ == tm.getSize()) ? 0 : 1;
+            int off = jp.s_present;
+            for (const auto& c : cols) {
+                if (std::find(jp.s_cols.begin(), jp.s_cols.end(), c) != jp.s_cols.end()) continue;
+                const auto* cs = stats(c);
+                if (!cs || tableOf(c) != jp.dim_table) return false;
+                const int b = bitsFor(static_cast<uint64_t>(cs->max_value_ - cs->min_value_));
+                if (c == jp.payload_col) jp.s_payload = static_cast<int>(jp.s_cols.size());
+                jp.s_cols.push_back(c);
+                jp.s_min.push_back(cs->min_value_);
+                jp.s_bits.push_back(b);
+                jp.s_off.push_back(off);
+                off += b;
+            }
+            if (off > 32) return false;
+        }
+        return true;
+    }
+    // attribute `a` of join j from entry expression `e`, as the raw int value
+    std::string sharedAttr(const JoinProgram& jp, size_t a, const std::string& e) const {
+        const int b = jp.s_bits[a];
+        const std::string code = b == 0 ? "0" : "(int)((" + e + " >> " + std::to_string(jp.s_off[a]) + ") & " +
+                                                    std::to_string(b == 32 ? 0xffffffffu : ((1u << b) - 1u)) + "u)";
+        return "((int)" + std::to_string(jp.s_min[a]) + " + " + code + ")";
+    }
+    std::string sharedAttrOf(const JoinProgram& jp, const std::string& col, const std::string& e) const {
+        for (size_t a = 0; a < jp.s_cols.size(); ++a)
+            if (jp.s_cols[a] == col) return sharedAttr(jp, a, e);
+        throw std::runtime_error("internal: attribute " + col + " not packed");
+    }
+
+    void emitBatchShared(std::ostringstream& src) {
+        prog_.shared_tables = true;
+        const int nj = static_cast<int>(prog_.joins.size());
+        for (int j = 0; j < nj; ++j) {
+            JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            int bits = jp.s_present;
+            for (int b : jp.s_bits) bits += b;
+            jp.shared = true;
+            jp.dense = true;
+            jp.lds = false;
+            jp.db = pow2Width(bits);
+            jp.doff = jp.s_present;
+            jp.ht_words = denseWords(jp, jp.db);
+        }
+        prog_.block_threads = opt_.block_threads > 0 ? opt_.block_threads : 256;
+        prog_.vectors_per_thread = std::max(1, std::min(8, opt_.vectors_per_thread));
+
+        // template builds: every dimension row with its packed attributes
+        for (int j = 0; j < nj; ++j) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            const int pk = dimSlot(jp.pk_col);
+            src << "extern \"C\" __global__ __launch_bounds__(256) void cq_build_" << j << "(" << paramDecl() << ") {\n"
+                << paramAlias() << "    const long long n = P.dim_rows[" << j << "];\n";
+            if (jp.db == 0) {
+                src << "    (void)n;\n}\n\n";
+                continue;
+            }
+            src << "    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < n; r += (long long)gridDim.x * 256) {\n";
+            if (nullable(jp.pk_col)) src << "        if (!chip::bit32(P.dvalid[" << pk << "], r)) continue;\n";
+            src << "        const unsigned int x = (unsigned int)P.dcol[" << pk << "][r] - (unsigned int)" << jp.dmin << ";\n"
+                << "        if (x >= " << jp.dspan << "u) continue;\n"
+                << "        unsigned int e = " << jp.s_present << "u;\n";
+            for (size_t a = 0; a < jp.s_cols.size(); ++a)
+                if (jp.s_bits[a] > 0)
+                    src << "        e |= ((unsigned int)(P.dcol[" << dimSlot(jp.s_cols[a]) << "][r] - (int)" << jp.s_min[a] << ")) << "
+                        << jp.s_off[a] << ";\n";
+            src << "        const unsigned int b = x * " << jp.db << "u;\n"
+                << "        atomicOr(&P.ht[" << j << "][b >> 5], e << (b & 31));\n"
+                << "    }\n}\n\n";
+        }
+
+        // template-level probes, once per tile for all of its queries
+        std::ostringstream sh;
+        for (int j = 0; j < nj; ++j) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            const int fs = factSlot(jp.fk_col);
+            const std::string J = std::to_string(j);
+            sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
+               << "        u32 e" << J << "[IPT]; u32 m" << J << " = 0u, z" << J << " = 0u;\n"
+               << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+               << "            e" << J << "[i] = 0u;\n"
+               << "            if (!((tile_fl >> i) & 1u)) continue;\n";
+            if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
+            sh << "            const int fk = c" << fs << "[i];\n";
+            auto entry = [&](const std::string& x) {
+                return jp.db == 0 ? std::string("0u") : "chip::dense_entry_aligned(P0.ht[" + J + "], " + x + ", " + std::to_string(jp.db) + ")";
+            };
+            if (jp.zmode != 0) {
+                std::string ze = "0u";
+                if (jp.zmode == 2) ze = entry(std::to_string(jp.zx) + "u");
+                if (jp.zmode == 3) ze = entry("(u32)(0 - " + std::to_string(jp.dmin) + ")");
+                sh << "            if (fk == 0) { z" << J << " |= 1u << i; e" << J << "[i] = " << ze << "; continue; }\n";
+            }
+            sh << "            const u32 x = (u32)fk - (u32)" << jp.dmin << ";\n"
+               << "            if (x >= " << jp.dspan << "u) continue;\n"
+               << "            const u32 e = " << entry("x") << ";\n";
+            if (jp.s_present) sh << "            if (!(e & 1u)) continue;\n";
+            sh << "            e" << J << "[i] = e; m" << J << " |= 1u << i;\n"
+               << "        }\n";
+        }
+        prog_.batch_shared = sh.str();
+
+        // per-query pipeline: fact filters, this query's build filters on
+        // the attributes, payload-dependent filters, aggregation
+        std::ostringstream body;
+        visible_mask_ = 0;
+        std::vector<Op> order = pipelineOrder();
+        std::vector<Op> fact_f, late_f;
+        for (const Op& op : order) {
+            if (op.kind == 1) continue;
+            std::vector<std::string> cols;
+            collectColumns(op.pred, cols);
+            bool fact_only = true;
+            for (const auto& c : cols) fact_only = fact_only && tableOf(c) == prog_.fact_table;
+            (fact_only ? fact_f : late_f).push_back(op);
+        }
+        auto emitFilter = [&](const Op& op) {
+            const std::string c = op.fast ? fastCond(op.pred) : andTerms({vvalid(op.pred), vexpr(op.pred)});
+            body << "        // filter\n#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                 << "            if (!(" << c << ")) fl &= ~(1u << i);\n"
+                 << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+        };
+        for (const Op& op : fact_f) emitFilter(op);
+        for (int j = 0; j < nj; ++j) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            const std::string J = std::to_string(j);
+            const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
+            const std::string cond = bp ? buildCondWith(bp, [&](const std::string& col) { return sharedAttrOf(jp, col, "e"); }) : "1";
+            const std::string ins = jp.s_present ? "((e & 1u) && " + cond + ")" : cond;
+            const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
+            body << "        // join " << jp.dim_table << ": shared probe, this query's build filter\n"
+                 << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            p" << J << "[i] = 0;\n"
+                 << "            if (!((fl >> i) & 1u)) continue;\n"
+                 << "            const u32 e = e" << J << "[i];\n";
+            if (jp.zmode != 0) {
+                // probe key 0 against the zero-filled reference table
+                body << "            if ((z" << J << " >> i) & 1u) {\n";
+                if (jp.zmode == 2) body << "                if (" << ins << ") fl &= ~(1u << i);\n";
+                if (jp.zmode == 3) body << "                if (" << ins << ") p" << J << "[i] = " << pay << ";\n";
+                body << "                continue;\n            }\n";
+            }
+            body << "            if (!((m" << J << " >> i) & 1u)) { fl &= ~(1u << i); continue; }\n";
+            if (bp) body << "            if (!(" << cond << ")) { fl &= ~(1u << i); continue; }\n";
+            if (jp.s_payload >= 0) body << "            p" << J << "[i] = " << pay << ";\n";
+            body << "        }\n"
+                 << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+            visible_mask_ |= 1u << j;
+        }
+        for (const Op& op : late_f) emitFilter(op);
+
+        // aggregation: scalars per tile, small group domains in the query's
+        // LDS region, the rest with global atomics
+        std::ostringstream aggcode, pro, epi;
+        if (!prog_.grouped) {
+            emitScalarAggregateTile(aggcode);
+        } else {
+            bool lds = opt_.allow_lds_aggregation && flag_w_ > 0 && prog_.storage_ts <= 64;
+            for (bool d : gdense_) lds = lds && d;
+            const uint64_t bytes = (aggLdsBytes() + 7) & ~uint64_t(7);
+            if (lds && bytes <= 4096) {
+                emitLdsAggregate(aggcode, pro, epi, true);
+                prog_.lds_aggregation = true;
+                prog_.batch_lds_stride = bytes;
+                prog_.batch_lds_init = pro.str();
+                prog_.batch_lds_flush = epi.str();
+            } else {
+                emitGlobalAggregate(aggcode);
+            }
+        }
+        for (int i = 0; i < prog_.naggs && !prog_.grouped; ++i)
+            if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
+        prog_.batch_body = body.str() + aggcode.str();
+        if (prog_.keys_kernel) emitKeysKernel(src);
+        emitDenseCount(src);
+        prog_.source = src.str();
+        setAlgorithmicBytes();
     }
 
     // algorithmic bytes of one execution: fact columns + bitmaps + dimension columns
@@ -1368,7 +1576,9 @@ private:
         return -1;
     }
 
-    void emitLdsAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
+    // batch_form: no __shared__ declarations and no block barrier; the
+    // caller provides s_agg / s_flag pointers into its per-query LDS region
+    void emitLdsAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi, bool batch_form = false) {
         const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
         const int64_t G = prog_.groups;
         const int W = flag_w_;
@@ -1377,8 +1587,9 @@ private:
         while ((1 << lp) < per) ++lp;
         const int64_t FW = (G * W + 31) / 32;
         const std::string mask = W == 32 ? "0xffffffffu" : std::to_string((1u << W) - 1u) + "u";
-        pro << "    __shared__ u64 s_agg[" << G * nas << "];\n"
-            << "    __shared__ u32 s_flag[" << FW << "];  // " << W << " flag bits per group\n";
+        if (!batch_form)
+            pro << "    __shared__ u64 s_agg[" << G * nas << "];\n"
+                << "    __shared__ u32 s_flag[" << FW << "];  // " << W << " flag bits per group\n";
         // MIN slots start at ~0
         std::string init = "0ull";
         if (!prog_.min_slots.empty()) {
@@ -1419,8 +1630,8 @@ private:
              << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n"
              << "        }\n";
         // merge into the global slots once per block
-        epi << "    __syncthreads();\n"
-            << "    for (int g = tid; g < " << G << "; g += BT) {\n"
+        if (!batch_form) epi << "    __syncthreads();\n";
+        epi << "    for (int g = tid; g < " << G << "; g += BT) {\n"
             << "        const u32 f = (s_flag[g >> " << lp << "] >> ((u32)(g & " << (per - 1) << ") * " << W << "u)) & " << mask
             << ";\n"
             << "        if (!(f & 1u)) continue;\n"
@@ -1459,10 +1670,22 @@ private:
         epi << "    }\n";
     }
 
+    // Global aggregation can leave the group keys to cq_keys when they
+    // decode from the group index and some slot is valid on every hit (that
+    // slot's validity bit then marks the group as hit).
+    bool keysDecodable() const {
+        if (!prog_.grouped || prog_.fast_sparse || flag_w_ == 0 || prog_.storage_ts > 64 || flag_static_.empty()) return false;
+        for (bool d : gdense_)
+            if (!d) return false;
+        return true;
+    }
+
     void emitGlobalAggregate(std::ostringstream& code) {
         const auto* a = agg_;
         const int ng = prog_.ngroups, sts = prog_.storage_ts;
         const int64_t G = prog_.groups;
+        const bool decode = keysDecodable();
+        prog_.keys_kernel = decode;
         code << "        // GROUP BY: perfect hash, global u64 atomics per row\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
              << "            if (!((fl >> i) & 1u)) continue;\n"
@@ -1472,8 +1695,17 @@ private:
         auto setbit = [&](const std::string& idx) {
             return "{ const long long b = " + idx + "; unsigned long long* w = &P.result_valid[b >> 6]; const u64 m = 1ull << (b & 63); if (!(*w & m)) atomicOr(w, m); }";
         };
+        auto isStatic = [&](int slot) {
+            return std::find(flag_static_.begin(), flag_static_.end(), slot) != flag_static_.end();
+        };
         for (int g = 0; g < ng; ++g) {
             const ExprNode* e = a->group_by_exprs[static_cast<size_t>(g)].get();
+            if (decode) {
+                if (isStatic(g)) continue;  // written by cq_keys
+                const std::string v = vvalid(e);
+                if (!litFalse(v)) code << "            if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
+                continue;
+            }
             code << "            P.result[out + " << g << "] = (u64)(" << vexpr(e) << ");\n";
             if (!prog_.fast_sparse) {
                 const std::string v = vvalid(e);
@@ -1490,11 +1722,41 @@ private:
             code << "                " << atom << "(&P.result[out + " << slot << "], " << r.value << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
             if (hs >= 0) code << "                atomicAdd(&P.result[out + " << hs << "], 1ull);\n";
-            else if (f != "COUNT" && !prog_.fast_sparse) code << "                " << setbit("out + " + std::to_string(slot)) << "\n";
+            else if (f != "COUNT" && !prog_.fast_sparse && !(decode && isStatic(slot)))
+                code << "                " << setbit("out + " + std::to_string(slot)) << "\n";
             code << "            }\n";
-            if (f == "COUNT" && !prog_.fast_sparse) code << "            " << setbit("out + " + std::to_string(slot)) << "\n";
+            if (f == "COUNT" && !prog_.fast_sparse && !decode) code << "            " << setbit("out + " + std::to_string(slot)) << "\n";
         }
+        if (decode) code << "            " << setbit("out + " + std::to_string(flag_static_.front())) << "  // hit\n";
         code << "        }\n";
+    }
+
+    void emitKeysKernel(std::ostringstream& src) {
+        const int ng = prog_.ngroups, sts = prog_.storage_ts;
+        const int64_t G = prog_.groups;
+        unsigned long long smask = 0;
+        for (int s : flag_static_) smask |= 1ull << s;
+        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_keys(" << paramDecl() << ") {\n"
+            << paramAlias() << "    using namespace chip;\n"
+            << "    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;\n"
+            << "    if (g >= " << G << "ll) return;\n"
+            << "    const long long base = g * " << sts << ";\n"
+            << "    if (!chip::bit64(P.result_valid, base + " << flag_static_.front() << ")) return;\n"
+            << "    long long rem = g;\n";
+        for (int g = ng - 1; g >= 0; --g)
+            src << "    const long long d" << g << " = rem % " << gcard_[static_cast<size_t>(g)] << "; rem /= " << gcard_[static_cast<size_t>(g)]
+                << ";\n";
+        for (int g = 0; g < ng; ++g) {
+            const std::string mn = std::to_string(gmin_[static_cast<size_t>(g)]);
+            std::string val;
+            if (gnullable_[static_cast<size_t>(g)]) val = "(d" + std::to_string(g) + " == 0 ? 0ll : (long long)" + mn + " + d" + std::to_string(g) + " - 1)";
+            else val = "((long long)" + mn + " + d" + std::to_string(g) + ")";
+            src << "    P.result[base + " << g << "] = (u64)" << val << ";\n";
+        }
+        src << "    const u64 vm = " << smask << "ull; const u32 o = (u32)(base & 63);\n"
+            << "    atomicOr(&P.result_valid[base >> 6], vm << o);\n"
+            << "    if (o + " << sts << "u > 64u) atomicOr(&P.result_valid[(base >> 6) + 1], vm >> (64u - o));\n"
+            << "}\n\n";
     }
 
     // HAVING over result slots (aggregateTupleExprValue/Valid, visitor.cpp:2277-2450)

@@ -36,6 +36,8 @@ struct CodegenOptions {
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;
+    // batched templates share one probe per dimension (packed attributes)
+    bool batch_shared_tables = true;
 };
 
 struct JoinProgram {
@@ -55,6 +57,15 @@ struct JoinProgram {
     int doff = 1;             // 1: code 0 marks an absent key; 0: every key present
     int db = 0;               // chosen width (exact in LDS, power of two in HBM)
     int zmode = 0, zx = 0;    // probe key 0 vs. the reference's empty-slot match (chip::ht_probe_dense)
+    // batched templates: one table per template, every dimension row
+    // inserted with its packed attributes (payload + build-filter columns);
+    // each query applies its own build filter to the attributes
+    bool shared = false;
+    std::vector<std::string> s_cols;
+    std::vector<int64_t> s_min;
+    std::vector<int> s_bits, s_off;
+    int s_present = 0;        // 1: bit 0 marks an inserted key
+    int s_payload = -1;       // attribute index of the payload column
     bool lds = false;
     uint64_t ht_words = 0;    // u32 words of the table
 };
@@ -110,6 +121,14 @@ struct QueryProgram {
     // parameter block `P`; `continue` leaves the query's body
     std::string batch_body;
     std::vector<int> count_slots_scalar;  // scalar COUNT slots (validity set whenever rows are scanned)
+    // shared-probe form: template-level code run once per tile before the
+    // per-query bodies (probes of the template's shared tables), and
+    // per-query LDS aggregation (init / merge code, bytes per query)
+    bool shared_tables = false;
+    std::string batch_shared;
+    std::string batch_lds_init, batch_lds_flush;
+    uint64_t batch_lds_stride = 0;
+    bool keys_kernel = false;             // cq_keys writes group keys after the scan
 };
 
 // One hipRTC module with `cq_batch_scan`: the union of the templates' fact

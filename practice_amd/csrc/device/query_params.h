@@ -53,6 +53,9 @@ struct cq_batch_params {
     const struct cq_query_params* qp;
     int tq_begin[CQ_MAX_TEMPLATES];
     int tq_end[CQ_MAX_TEMPLATES];
+    // byte offset of each template's per-query LDS aggregation regions in
+    // the dynamic shared memory of the launch
+    unsigned int lds_off[CQ_MAX_TEMPLATES];
     long long row_begin;
     long long row_end;
     long long num_tiles;

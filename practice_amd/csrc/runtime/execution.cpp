@@ -255,11 +255,14 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
 
     run.builds.clear();
     for (size_t j = 0; j < p.joins.size(); ++j) {
-        P.ht[j] = static_cast<unsigned int*>(take("ht" + std::to_string(j), std::max<size_t>(p.joins[j].ht_words * 4, 16)));
+        // shared (template) tables are owned and built by the batch
+        P.ht[j] = p.shared_tables ? nullptr
+                                  : static_cast<unsigned int*>(take("ht" + std::to_string(j), std::max<size_t>(p.joins[j].ht_words * 4, 16)));
         P.dim_rows[j] = p.joins[j].dim_rows;
         run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
     }
     run.dense_count = p.fast_sparse ? nullptr : module->function("cq_dense_count");
+    run.keys = p.keys_kernel ? module->function("cq_keys") : nullptr;
 
     // aggregate state
     const int64_t G = p.groups;
@@ -305,8 +308,9 @@ void HipModuleExecutor::setup(QueryRun& run, ExecutionContext* ctx, const cq_que
     const QueryProgram& p = *run.program;
     cq_query_params& P = run.P;
     hipStream_t s = ctx->stream;
-    for (size_t j = 0; j < p.joins.size(); ++j)
-        hipCheck(cq_launch_fill_u32(P.ht[j], 0u, static_cast<long long>(p.joins[j].ht_words), s), "fill hash table");
+    if (!p.shared_tables)
+        for (size_t j = 0; j < p.joins.size(); ++j)
+            hipCheck(cq_launch_fill_u32(P.ht[j], 0u, static_cast<long long>(p.joins[j].ht_words), s), "fill hash table");
     const size_t words = (run.slots + 63) / 64;
     hipCheck(cq_launch_fill_u64(P.result, 0ull, static_cast<long long>(run.slots), s), "fill result");
     // scalar COUNT of a non-empty scan is valid even when no row qualifies
@@ -319,13 +323,18 @@ void HipModuleExecutor::setup(QueryRun& run, ExecutionContext* ctx, const cq_que
     unsigned int min_mask = 0;
     for (int slot : p.min_slots) min_mask |= 1u << slot;
     if (min_mask) hipCheck(cq_launch_init_min(P.result, run.G, p.storage_ts, min_mask, s), "init MIN slots");
+    if (!p.shared_tables) buildTables(run, ctx, device_params);
+}
+
+void HipModuleExecutor::buildTables(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params) {
+    const QueryProgram& p = *run.program;
     for (size_t j = 0; j < p.joins.size(); ++j) {
         const long long rows = p.joins[j].dim_rows;
         const unsigned grid = static_cast<unsigned>(std::min<long long>(std::max<long long>((rows + 255) / 256, 1), 8192));
         const cq_query_params* dp = device_params;
-        void* by_value[] = {&P};
+        void* by_value[] = {&run.P};
         void* by_pointer[] = {&dp};
-        launch(run.builds[j], grid, 256, s, device_params ? by_pointer : by_value, "launch build");
+        launch(run.builds[j], grid, 256, ctx->stream, device_params ? by_pointer : by_value, "launch build");
     }
 }
 
@@ -351,6 +360,12 @@ void HipModuleExecutor::shape(QueryRun& run, ExecutionContext* ctx, const cq_que
     hipStream_t s = ctx->stream;
     const int64_t G = run.G;
     const int vts = p.visible_ts;
+    if (run.keys) {
+        const cq_query_params* dp = device_params;
+        void* by_value[] = {&P};
+        void* by_pointer[] = {&dp};
+        launch(run.keys, static_cast<unsigned>((G + 255) / 256), 256, s, device_params ? by_pointer : by_value, "launch keys");
+    }
     if (p.has_avg) {
         const size_t vslots = static_cast<size_t>(G) * static_cast<size_t>(vts);
         hipCheck(cq_launch_fill_u64(run.res, 0ull, static_cast<long long>(vslots), s), "fill");

@@ -222,6 +222,7 @@ struct QueryRun {
     cq_query_params P;
     std::vector<hipFunction_t> builds;
     hipFunction_t dense_count = nullptr;
+    hipFunction_t keys = nullptr;       // cq_keys (group keys decoded after the scan)
     int64_t G = 1;
     size_t slots = 0;
     unsigned long long* res = nullptr;  // visible tuples (after AVG finalization)
@@ -261,6 +262,9 @@ public:
     // zero tables / aggregate slots, build the dimension tables.  With
     // `device_params` the generated kernels read the block from device memory.
     void setup(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params);
+    // the build kernels of run's tables (setup() runs them unless the
+    // program's tables are shared by a batch template)
+    void buildTables(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params);
     void scan(QueryRun& run, ExecutionContext* ctx);
     // AVG finalization, present/HAVING flags, compaction; the number of
     // result rows lands in device memory at `row_count` (default: after the offsets)

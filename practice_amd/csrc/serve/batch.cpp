@@ -1,6 +1,8 @@
 // Batched execution over one shared fact scan per group of templates.
 #include "batch.h"
 
+#include "../kernels/aot_kernels.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -8,6 +10,10 @@
 #include <stdexcept>
 
 namespace db {
+
+namespace {
+constexpr unsigned kBatchLdsLimit = 64 * 1024;  // dynamic LDS of one shared-scan block
+}  // namespace
 
 BatchOptions parseBatchOptions(const std::string& text) {
     BatchOptions o;
@@ -144,35 +150,55 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
     }
     std::vector<std::vector<int>> scans;
     if (module_templates) {
-        // the caller sized the batch for one launch of its module
-        scans.emplace_back();
-        for (size_t i = 0; i < programs.size(); ++i) scans.back().push_back(static_cast<int>(i));
+        // one launch of the caller's module per chunk that fits the LDS regions
+        std::vector<int> cur;
+        uint64_t lds = 0;
+        for (size_t i = 0; i < programs.size(); ++i) {
+            const uint64_t need = programs[i].batch_lds_stride;
+            if (!cur.empty() && lds + need > kBatchLdsLimit) {
+                scans.push_back(cur);
+                cur.clear();
+                lds = 0;
+            }
+            cur.push_back(static_cast<int>(i));
+            lds += need;
+        }
+        if (!cur.empty()) scans.push_back(cur);
         order.clear();
     }
     std::vector<int> cur;
     std::vector<std::string> cur_cols;
     std::string cur_table;
     int cur_templates = 0;
+    uint64_t cur_lds = 0;
     for (const auto& key : order) {
         const auto& qs = members[key];
         const QueryProgram& t = programs[static_cast<size_t>(qs.front())];
-        for (size_t start = 0; start < qs.size(); start += static_cast<size_t>(opt.max_queries)) {
-            const size_t stop = std::min(qs.size(), start + static_cast<size_t>(opt.max_queries));
+        // queries of one template per launch: bounded by max_queries and by
+        // their LDS aggregation regions
+        size_t chunk = static_cast<size_t>(opt.max_queries);
+        if (t.batch_lds_stride > 0) chunk = std::min<size_t>(chunk, std::max<uint64_t>(1, kBatchLdsLimit / t.batch_lds_stride));
+        for (size_t start = 0; start < qs.size(); start += chunk) {
+            const size_t stop = std::min(qs.size(), start + chunk);
+            const uint64_t lds_need = t.batch_lds_stride * (stop - start);
             std::vector<std::string> cols = cur_cols;
             for (const auto& c : t.fact_cols)
                 if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
             const bool fits = !cur.empty() && cur_table == t.fact_table && cur_templates + 1 <= opt.max_templates &&
                               cols.size() <= CQ_MAX_BATCH_COLS &&
-                              cur.size() + (stop - start) <= static_cast<size_t>(opt.max_queries);
+                              cur.size() + (stop - start) <= static_cast<size_t>(opt.max_queries) &&
+                              cur_lds + lds_need <= kBatchLdsLimit;
             if (!fits && !cur.empty()) {
                 scans.push_back(cur);
                 cur.clear();
                 cur_templates = 0;
+                cur_lds = 0;
                 cols.assign(t.fact_cols.begin(), t.fact_cols.end());
             }
             cur_table = t.fact_table;
             cur_cols = cols;
             ++cur_templates;
+            cur_lds += lds_need;
             for (size_t k = start; k < stop; ++k) cur.push_back(qs[k]);
         }
     }
@@ -205,6 +231,42 @@ std::string batchSource(const std::vector<const QueryProgram*>& templates, int b
 }
 
 }  // namespace
+
+namespace {
+
+// persistent grid for a launch with dynamic shared memory
+unsigned scanGridShared(hipFunction_t f, int block_threads, int device, long long num_tiles, unsigned lds_bytes) {
+    int per_cu = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block_threads, lds_bytes) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    hipDeviceProp_t prop;
+    hipCheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    const long long max_blocks = static_cast<long long>(per_cu) * prop.multiProcessorCount;
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(num_tiles, max_blocks)));
+}
+
+}  // namespace
+
+// Zero and build every table of the batch: a template's shared tables once
+// (from its first query's parameter block), per-query tables in setup().
+void BatchExecutor::prepareTables(const std::vector<const QueryProgram*>& templates,
+                                  const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
+                                  const cq_query_params* dqp, ExecutionContext* ctx) {
+    HipModuleExecutor* ex = db_.engine().hipExecutor();
+    int first = 0;
+    for (size_t t = 0; t < templates.size(); ++t) {
+        const int cnt = static_cast<int>(per_template[t].size());
+        if (cnt > 0 && templates[t]->shared_tables) {
+            QueryRun& r0 = runs[static_cast<size_t>(first)];
+            for (size_t j = 0; j < templates[t]->joins.size(); ++j)
+                hipCheck(cq_launch_fill_u32(r0.P.ht[j], 0u, static_cast<long long>(templates[t]->joins[j].ht_words), ctx->stream),
+                         "fill shared table");
+            ex->buildTables(r0, ctx, dqp + first);
+        }
+        first += cnt;
+    }
+    for (size_t k = 0; k < runs.size(); ++k) ex->setup(runs[k], ctx, dqp + k);
+}
 
 void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt,
                             int scan_index, std::vector<QueryResult>& results, BatchScanReport& rep,
@@ -287,6 +349,18 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
         for (int k = 0; k < n; ++k)
             ex->bind(tmodules[tix[static_cast<size_t>(k)]], programs[static_cast<size_t>(qorder[static_cast<size_t>(k)])], ctx, pre,
                      runs[static_cast<size_t>(k)], &arena);
+        // one table per join for each template with shared tables
+        int first = 0;
+        for (size_t t = 0; t < templates.size(); ++t) {
+            const int cnt = static_cast<int>(per_template[t].size());
+            if (cnt > 0 && templates[t]->shared_tables) {
+                for (size_t j = 0; j < templates[t]->joins.size(); ++j) {
+                    auto* tab = static_cast<unsigned int*>(arena.take(std::max<size_t>(templates[t]->joins[j].ht_words * 4, 16)));
+                    for (int k = first; k < first + cnt; ++k) runs[static_cast<size_t>(k)].P.ht[j] = tab;
+                }
+            }
+            first += cnt;
+        }
         dqp = static_cast<cq_query_params*>(arena.take(qp_bytes));
         dtot = static_cast<unsigned long long*>(arena.take(static_cast<size_t>(n) * 8));
     }
@@ -332,14 +406,20 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     B.row_end = first.fact_rows;
     const long long tile = static_cast<long long>(cfg.block_threads) * 4 * cfg.vectors_per_thread;
     B.num_tiles = (B.row_end - B.row_begin + tile - 1) / tile;
-    const unsigned grid = scanGrid(fn, cfg.block_threads, ctx->device, B.num_tiles);
+    unsigned lds_bytes = 0;
+    for (size_t t = 0; t < templates.size(); ++t) {
+        B.lds_off[t] = lds_bytes;
+        lds_bytes += static_cast<unsigned>(templates[t]->batch_lds_stride * per_template[t].size());
+    }
+    if (lds_bytes > kBatchLdsLimit) throw std::runtime_error("batch LDS aggregation regions exceed the limit");
+    const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes);
 
     auto enqueue = [&](bool timed) {
-        for (int k = 0; k < n; ++k) ex->setup(runs[static_cast<size_t>(k)], ctx, dqp + k);
+        prepareTables(templates, per_template, runs, dqp, ctx);
         if (timed) hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
         void* args[] = {&B};
         if (B.num_tiles > 0)
-            hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(cfg.block_threads), 1, 1, 0, s, args, nullptr),
+            hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(cfg.block_threads), 1, 1, lds_bytes, s, args, nullptr),
                      "launch batch scan");
         if (timed) hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
         for (int k = 0; k < n; ++k) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
@@ -446,7 +526,6 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
                                              const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
                                              const cq_query_params* dqp, ExecutionContext* ctx) {
     QueryEngine& engine = db_.engine();
-    HipModuleExecutor* ex = engine.hipExecutor();
     hipStream_t s = ctx->stream;
     Tuned best;
     float best_ms = -1;
@@ -477,13 +556,18 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
             const long long tile = static_cast<long long>(bt) * 4 * vpt;
             B.num_tiles = (B.row_end + tile - 1) / tile;
             if (B.num_tiles == 0) return best;
-            const unsigned grid = scanGrid(fn, bt, ctx->device, B.num_tiles);
+            unsigned lds_bytes = 0;
+            for (size_t t = 0; t < templates.size(); ++t) {
+                B.lds_off[t] = lds_bytes;
+                lds_bytes += static_cast<unsigned>(templates[t]->batch_lds_stride * per_template[t].size());
+            }
+            const unsigned grid = scanGridShared(fn, bt, ctx->device, B.num_tiles, lds_bytes);
             float ms = 0;
             for (int rep = 0; rep < 2; ++rep) {  // first run warms the module and caches
-                for (size_t k = 0; k < runs.size(); ++k) ex->setup(runs[k], ctx, dqp + k);
+                prepareTables(templates, per_template, runs, dqp, ctx);
                 hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
                 void* args[] = {&B};
-                hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(bt), 1, 1, 0, s, args, nullptr),
+                hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(bt), 1, 1, lds_bytes, s, args, nullptr),
                          "launch batch scan");
                 hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
                 hipCheck(hipEventSynchronize(ev_[3]), "sync");

@@ -104,6 +104,8 @@ private:
     void runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt, int scan_index,
                  std::vector<QueryResult>& results, BatchScanReport& rep,
                  const std::vector<const QueryProgram*>* module_templates);
+    void prepareTables(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
+                       std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
     Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
                    std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
     void dropGraph(Graph& g);

@@ -137,3 +137,43 @@ def test_autotuned_batch_matches():
     scan = gdb.batch_report()["scans"][0]
     assert scan["block_threads"] in (64, 128, 256, 512, 1024) and scan["vectors_per_thread"] in (1, 2, 4)
     _check_batch(gdb, odb, qs, res)
+
+
+def _key_zero_tables():
+    """SUPPLIER keyed 1..16 (reference slot (k - 0) % 16: key 16 shares slot 0
+    with a probe key 0) and a fact table whose supplier keys include 0 and
+    keys beyond the dimension."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    n = 4096
+    sup = {"s_suppkey": (np.arange(1, 17, dtype=np.int32), None),
+           "s_region": (np.array([i % 3 for i in range(16)], np.int32), None),
+           "s_nation": (np.array([i % 5 for i in range(16)], np.int32), None)}
+    lo = {"lo_suppkey": (rng.integers(0, 19, n).astype(np.int32), None),
+          "lo_revenue": (rng.integers(1, 1000, n).astype(np.int32), None)}
+    return {"LINEORDER": (n, True, lo), "SUPPLIER": (16, False, sup)}
+
+
+KEY_ZERO_QUERIES = [
+    # supplier 16 has region 0: filtered out for region 1 (key 0 then matches
+    # the empty slot) and kept for region 0 (key 0 does not match)
+    "select count(*), sum(lo_revenue) from lineorder, supplier where lo_suppkey = s_suppkey and s_region = 1",
+    "select count(*), sum(lo_revenue) from lineorder, supplier where lo_suppkey = s_suppkey and s_region = 0",
+    "select s_nation, count(*) from lineorder, supplier where lo_suppkey = s_suppkey and s_region = 2 group by s_nation",
+    "select s_nation, sum(lo_revenue) from lineorder, supplier where lo_suppkey = s_suppkey group by s_nation",
+]
+
+
+@pytest.mark.gpu
+def test_probe_key_zero_against_reference_empty_slot():
+    from tests.fixtures import build_pair
+    db, odb = build_pair(_key_zero_tables(), pks=[("SUPPLIER", "s_suppkey")], device=0)
+    try:
+        res = db.batch_execute(KEY_ZERO_QUERIES, {"graph": 0})
+        for sql, r in zip(KEY_ZERO_QUERIES, res):
+            print(f"[key0] sql={sql}", flush=True)
+            o = odb.execute(db.explain(sql))
+            assert_same(db.execute(sql), o)
+            assert_same(r, o)
+    finally:
+        db.close()
