@@ -20,6 +20,7 @@
 // and privatizes GROUP BY aggregates in LDS (u64 ds atomics) before one global
 // merge per block.
 #include "hip_codegen.h"
+#include "vm_expr.h"
 
 #include <algorithm>
 #include <cstring>
@@ -861,12 +862,15 @@ private:
         return buildCondWith(e, [&](const std::string& name) { return "P.dcol[" + std::to_string(dimSlot(name)) + "][r]"; });
     }
     // the same predicate over another source of the raw dimension values
-    std::string buildCondWith(const ExprNode* e, const std::function<std::string(const std::string&)>& value) {
+    // bitwise: 0/1 terms combined with & and | (no short-circuit branches)
+    std::string buildCondWith(const ExprNode* e, const std::function<std::string(const std::string&)>& value,
+                              bool bitwise = false) {
         const ExprType t = e->getType();
         if (t == ExprType::OP_AND || t == ExprType::OP_OR) {
             const auto* b = static_cast<const BinaryExpr*>(e);
-            const std::string l = buildCondWith(b->left.get(), value);
-            return "(" + l + (t == ExprType::OP_AND ? " && " : " || ") + buildCondWith(b->right.get(), value) + ")";
+            const std::string l = buildCondWith(b->left.get(), value, bitwise);
+            const char* op = bitwise ? (t == ExprType::OP_AND ? " & " : " | ") : (t == ExprType::OP_AND ? " && " : " || ");
+            return "(" + l + op + buildCondWith(b->right.get(), value, bitwise) + ")";
         }
         const auto* c = static_cast<const BinaryExpr*>(e);
         const bool col_left = c->left->getType() == ExprType::COLUMN_REF;
@@ -877,6 +881,7 @@ private:
         std::string litcode;
         if (l->getType() == ExprType::LITERAL_INT) litcode = litInt32(static_cast<const LiteralIntExpr*>(l)->value);
         else litcode = "((int)" + flit(static_cast<const LiteralFloatExpr*>(l)->value) + ")";
+        if (bitwise) return "((int)(" + value(name) + " " + cmpOp(op) + " " + litcode + "))";
         return "(" + value(name) + " " + cmpOp(op) + " " + litcode + ")";
     }
 
@@ -996,6 +1001,10 @@ private:
         // LDS aggregation when the domain decodes from the group index
         const bool batch = opt_.batch_body;
         computeFlagLayout();
+        if (opt_.vm) {
+            emitVm();
+            return;
+        }
         if (batch && sharedEligible()) {
             emitBatchShared(src);
             return;
@@ -1208,6 +1217,134 @@ private:
         src << ind << "    }\n" << ind << "}\n";
     }
 
+    // ------------------------------------------------ interpreted tier
+    // The same expressions the HIP source would contain, compiled to
+    // register code for the ahead-of-time interpreter (no hipRTC).
+    void emitVm() {
+        auto vp = std::make_shared<cq_vm_program>();
+        std::memset(vp.get(), 0, sizeof(cq_vm_program));
+        std::vector<cq_vm_instr> code;
+        vm::Builder b(code);
+        const int nj = static_cast<int>(prog_.joins.size());
+        for (int j = 0; j < nj; ++j) finalizeLayout(prog_.joins[static_cast<size_t>(j)], false);
+        prog_.block_threads = 128;
+        prog_.vectors_per_thread = 1;
+        vp->njoins = nj;
+        // dimension tables: layout and build-filter programs
+        for (int j = 0; j < nj; ++j) {
+            JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            cq_vm_join& J = vp->joins[j];
+            J.layout = jp.dense ? CQ_L_DENSE : (jp.compact ? CQ_L_COMPACT : CQ_L_REF);
+            J.fk_slot = factSlot(jp.fk_col);
+            J.pk_dslot = jp.pk_dslot = dimSlot(jp.pk_col);
+            J.payload_dslot = jp.pair ? (jp.payload_dslot = dimSlot(jp.payload_col)) : -1;
+            J.pk_nullable = nullable(jp.pk_col) ? 1 : 0;
+            J.fk_nullable = prog_.fact_nullable[static_cast<size_t>(J.fk_slot)] ? 1 : 0;
+            J.kmin = static_cast<int>(jp.kmin);
+            J.len = static_cast<int>(jp.len);
+            J.eb = jp.eb, J.tb = jp.tb, J.jlo = jp.jlo, J.pmin = static_cast<int>(jp.pmin);
+            J.dmin = static_cast<int>(jp.dmin), J.dspan = static_cast<int>(jp.dspan), J.db = jp.db;
+            J.pbase = static_cast<int>(densePbase(jp)), J.doff = jp.doff, J.zmode = jp.zmode, J.zx = jp.zx;
+            J.build_begin = static_cast<int>(code.size());
+            if (const ExprNode* bp = build_pred_[static_cast<size_t>(j)]) {
+                vm::Val c = b.compile(buildCond(bp));
+                b.emit(CQ_OP_FILTER, c.type, 0, 0, c.reg, 0, 0, 0);
+                b.release(c);
+            }
+            J.build_end = static_cast<int>(code.size());
+        }
+        // fact pipeline in the plan's execution order
+        vp->scan_begin = static_cast<int>(code.size());
+        std::vector<int> pay_reg(static_cast<size_t>(nj));
+        for (int j = 0; j < nj; ++j) {
+            pay_reg[static_cast<size_t>(j)] = b.allocPermanent();
+            b.setPayloadReg(j, pay_reg[static_cast<size_t>(j)]);
+        }
+        visible_mask_ = 0;
+        for (const Op& op : pipelineOrder()) {
+            if (op.kind == 0) {
+                vm::Val c = b.compile(op.fast ? fastCond(op.pred) : andTerms({vvalid(op.pred), vexpr(op.pred)}));
+                b.emit(CQ_OP_FILTER, c.type, 0, 0, c.reg, 0, 0, 0);
+                b.release(c);
+            } else {
+                const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
+                const int fs = factSlot(jp.fk_col);
+                if (prog_.fact_nullable[static_cast<size_t>(fs)]) {
+                    vm::Val v = b.compile("((v" + std::to_string(fs) + " >> i) & 1u)");
+                    b.emit(CQ_OP_FILTER, v.type, 0, 0, v.reg, 0, 0, 0);
+                    b.release(v);
+                }
+                vm::Val k = b.compile("c" + std::to_string(fs) + "[i]");
+                b.emit(CQ_OP_PROBE, CQ_T_I32, 0, pay_reg[static_cast<size_t>(op.join)], k.reg, 0, 0, op.join);
+                b.release(k);
+                visible_mask_ |= 1u << op.join;
+            }
+            b.emit(CQ_OP_EXIT, 0, 0, 0, 0, 0, 0, 0);
+        }
+        // aggregation inputs (left in registers for the interpreter's epilogue)
+        const auto* a = agg_;
+        vp->naggs = prog_.naggs;
+        vp->ngroups = prog_.ngroups;
+        vp->storage_ts = prog_.storage_ts;
+        vp->visible_ts = prog_.visible_ts;
+        vp->fast_sparse = prog_.fast_sparse ? 1 : 0;
+        vp->groups = prog_.groups;
+        if (prog_.naggs > CQ_VM_MAX_AGGS || prog_.ngroups > CQ_VM_MAX_AGGS || prog_.storage_ts > 31)
+            unsupported("interpreted tier: too many aggregate slots");
+        if (prog_.grouped) {
+            vm::Val h = b.compile(groupHash());
+            vp->group_reg = h.reg;
+            for (int g = 0; g < prog_.ngroups; ++g) {
+                const ExprNode* e = a->group_by_exprs[static_cast<size_t>(g)].get();
+                vp->key_val[g] = b.toType(b.compile(vexpr(e)), CQ_T_U64).reg;
+                const std::string v = vvalid(e);
+                vp->key_valid[g] = litTrue(v) ? -1 : b.compile(v).reg;
+                vp->key_min[g] = gmin_[static_cast<size_t>(g)];
+                vp->key_card[g] = static_cast<long long>(gcard_[static_cast<size_t>(g)]);
+                vp->key_nullable[g] = gnullable_[static_cast<size_t>(g)] ? 1 : 0;
+            }
+            bool dec = true;
+            for (bool d : gdense_) dec = dec && d;
+            vp->decodable = dec ? 1 : 0;
+        }
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            vp->agg_fn[i] = f == "MIN" ? CQ_VM_MIN : (f == "MAX" ? CQ_VM_MAX : (f == "COUNT" ? CQ_VM_COUNT : CQ_VM_SUM));
+            vp->agg_hidden[i] = prog_.hidden_slot[static_cast<size_t>(i)];
+            const AggRow r = aggRow(i);
+            vp->agg_val[i] = f == "COUNT" ? -1 : b.compile(r.value).reg;
+            vp->agg_valid[i] = litTrue(r.valid) ? -1 : b.compile(r.valid).reg;
+        }
+        vp->scan_end = static_cast<int>(code.size());
+        // HAVING over the visible result slots of a group
+        vp->having_reg = -1;
+        if (a->having_predicate) {
+            b.resetColumnCache();
+            vp->having_begin = static_cast<int>(code.size());
+            vm::Val hv = b.compile(havingExpr());
+            vp->having_reg = hv.reg;
+            vp->having_type = hv.type;
+            vp->having_end = static_cast<int>(code.size());
+        }
+        for (int i = 0; i < prog_.naggs && !prog_.grouped; ++i)
+            if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
+        vp->ncode = static_cast<int>(code.size());
+        std::copy(code.begin(), code.end(), vp->code);
+        vp->nregs = b.maxRegs();
+        // group aggregates in LDS when the domain decodes and fits
+        const uint64_t nas = static_cast<uint64_t>(prog_.storage_ts - prog_.ngroups);
+        const uint64_t lds = static_cast<uint64_t>(prog_.groups) * (nas * 8 + 4);
+        prog_.vm_lds_bytes = (prog_.grouped && vp->decodable && lds <= 64 * 1024) ? lds : 0;
+        prog_.vm = vp;
+        std::ostringstream src;
+        src << "// interpreted: " << code.size() << " instructions, " << b.maxRegs() << " registers\n";
+        for (size_t k = 0; k < code.size(); ++k)
+            src << "// " << k << ": op " << int(code[k].op) << " t" << int(code[k].t) << "/" << int(code[k].t2) << " d" << int(code[k].d)
+                << " a" << int(code[k].a) << " b" << int(code[k].b) << " c" << int(code[k].c) << " imm " << code[k].imm << "\n";
+        prog_.source = src.str();
+        setAlgorithmicBytes();
+    }
+
     // ------------------------------------------------ batched templates
     // Shared-probe form: possible when every join has the key-indexed
     // layout and its attributes (payload + build-filter columns) pack into
@@ -1355,25 +1492,25 @@ private:
             const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
             const std::string J = std::to_string(j);
             const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
-            const std::string cond = bp ? buildCondWith(bp, [&](const std::string& col) { return sharedAttrOf(jp, col, "e"); }) : "1";
-            const std::string ins = jp.s_present ? "((e & 1u) && " + cond + ")" : cond;
+            // straight-line 0/1 arithmetic: the build filter of this query on
+            // the packed attributes, no short-circuit branches
+            const std::string cond =
+                bp ? buildCondWith(bp, [&](const std::string& col) { return sharedAttrOf(jp, col, "e"); }, true) : "1";
+            const std::string ins = jp.s_present ? "((int)(e & 1u) & " + cond + ")" : cond;
             const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
             body << "        // join " << jp.dim_table << ": shared probe, this query's build filter\n"
                  << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                 << "            p" << J << "[i] = 0;\n"
-                 << "            if (!((fl >> i) & 1u)) continue;\n"
-                 << "            const u32 e = e" << J << "[i];\n";
-            if (jp.zmode != 0) {
-                // probe key 0 against the zero-filled reference table
-                body << "            if ((z" << J << " >> i) & 1u) {\n";
-                if (jp.zmode == 2) body << "                if (" << ins << ") fl &= ~(1u << i);\n";
-                if (jp.zmode == 3) body << "                if (" << ins << ") p" << J << "[i] = " << pay << ";\n";
-                body << "                continue;\n            }\n";
-            }
-            body << "            if (!((m" << J << " >> i) & 1u)) { fl &= ~(1u << i); continue; }\n";
-            if (bp) body << "            if (!(" << cond << ")) { fl &= ~(1u << i); continue; }\n";
-            if (jp.s_payload >= 0) body << "            p" << J << "[i] = " << pay << ";\n";
-            body << "        }\n"
+                 << "            const u32 e = e" << J << "[i];\n"
+                 << "            int keep = (int)((m" << J << " >> i) & 1u) & " << cond << ";\n"
+                 << "            int pv = keep ? " << pay << " : 0;\n";
+            if (jp.zmode == 1) body << "            if ((z" << J << " >> i) & 1u) { keep = 1; pv = 0; }\n";
+            if (jp.zmode == 2) body << "            if ((z" << J << " >> i) & 1u) { keep = " << ins << " ^ 1; pv = 0; }\n";
+            if (jp.zmode == 3)
+                body << "            if ((z" << J << " >> i) & 1u) { const int in0 = " << ins << "; keep = 1; pv = in0 ? " << pay
+                     << " : 0; }\n";
+            body << "            if (!keep) fl &= ~(1u << i);\n"
+                 << "            p" << J << "[i] = ((fl >> i) & 1u) ? pv : 0;\n"
+                 << "        }\n"
                  << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
             visible_mask_ |= 1u << j;
         }
@@ -1846,6 +1983,16 @@ private:
         }
     }
 
+    std::string havingExpr() {
+        std::string having = "1";
+        if (agg_->having_predicate) {
+            const std::string v = hvalid(agg_->having_predicate.get());
+            const std::string x = hvalue(agg_->having_predicate.get());
+            having = litTrue(v) ? "(" + x + ")" : (litFalse(v) ? "0" : "((" + v + ") && (" + x + "))");
+        }
+        return having;
+    }
+
     void emitDenseCount(std::ostringstream& src) {
         const int vts = prog_.visible_ts;
         src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count(" << paramDecl()
@@ -1863,12 +2010,7 @@ private:
                 << "    for (int c = 0; c < " << vts << " && !present; ++c)\n"
                 << "        present = chip::bit64(RV, base + c) != 0u || R[base + c] != 0ull;\n";
         }
-        std::string having = "1";
-        if (agg_->having_predicate) {
-            const std::string v = hvalid(agg_->having_predicate.get());
-            const std::string x = hvalue(agg_->having_predicate.get());
-            having = litTrue(v) ? "(" + x + ")" : (litFalse(v) ? "0" : "((" + v + ") && (" + x + "))");
-        }
+        const std::string having = havingExpr();
         src << "    counts[g] = (present && (" << having << ")) ? 1ull : 0ull;\n"
             << "}\n";
     }

@@ -17,9 +17,11 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "../device/vm_program.h"
 #include "../plan/operators.h"
 
 namespace db {
@@ -38,6 +40,9 @@ struct CodegenOptions {
     bool batch_body = false;
     // batched templates share one probe per dimension (packed attributes)
     bool batch_shared_tables = true;
+    // interpreted tier: register code for the ahead-of-time interpreter
+    // instead of HIP source
+    bool vm = false;
 };
 
 struct JoinProgram {
@@ -129,6 +134,10 @@ struct QueryProgram {
     std::string batch_lds_init, batch_lds_flush;
     uint64_t batch_lds_stride = 0;
     bool keys_kernel = false;             // cq_keys writes group keys after the scan
+    // interpreted tier (CodegenOptions::vm): the program and the dynamic LDS
+    // of its group aggregates (0: global atomics)
+    std::shared_ptr<cq_vm_program> vm;
+    uint64_t vm_lds_bytes = 0;
 };
 
 // One hipRTC module with `cq_batch_scan`: the union of the templates' fact

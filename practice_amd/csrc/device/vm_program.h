@@ -1,0 +1,95 @@
+// Bytecode of the interpreted tier: a plan the code generator accepts,
+// compiled to register code that ahead-of-time kernels execute (no hipRTC).
+// Shared by the host compiler (codegen/hip_codegen.cpp, VM mode) and the
+// device interpreter (kernels/vm_kernels.hip).  Plain C layout.
+//
+// Values live in 64-bit registers and carry the C type the generated HIP
+// source would give them (the interpreter follows the same conversions):
+// CQ_T_I32 (int, also bool), CQ_T_U32, CQ_T_I64, CQ_T_U64 (two's
+// complement / raw bits) and CQ_T_F64 (IEEE bits).
+#ifndef CQ_VM_PROGRAM_H
+#define CQ_VM_PROGRAM_H
+
+#define CQ_VM_MAX_CODE 512
+#define CQ_VM_MAX_REGS 32
+#define CQ_VM_MAX_AGGS 32
+
+enum cq_vm_type { CQ_T_I32 = 0, CQ_T_U32 = 1, CQ_T_I64 = 2, CQ_T_U64 = 3, CQ_T_F64 = 4 };
+
+enum cq_vm_op {
+    CQ_OP_END = 0,
+    CQ_OP_COL,      // d = fact column a (int32) of the row
+    CQ_OP_VALID,    // d = validity bit of fact column a
+    CQ_OP_DCOL,     // d = dimension column a (int32) of the row (build programs)
+    CQ_OP_DVALID,   // d = validity bit of dimension column a
+    CQ_OP_PAY,      // d = payload register of join a (0 before / without a payload)
+    CQ_OP_LIT,      // d = P.lit[imm] as type t (CQ_T_I32 truncates like (int))
+    CQ_OP_FLIT,     // d = P.flit[imm]
+    CQ_OP_IMM,      // d = 64-bit immediate (imm | imm2 << 32) of type t
+    CQ_OP_SLOT,     // d = result slot imm of the group (HAVING programs) as type t
+    CQ_OP_SLOTV,    // d = validity bit of result slot imm
+    CQ_OP_CONV,     // d = (t)a where a has type t2
+    CQ_OP_ADD, CQ_OP_SUB, CQ_OP_MUL, CQ_OP_DIV, CQ_OP_MOD,  // d = a op b in type t
+    CQ_OP_EQ, CQ_OP_NE, CQ_OP_LT, CQ_OP_LE, CQ_OP_GT, CQ_OP_GE,  // d = (a op b) in type t, as int 0/1
+    CQ_OP_NOT,      // d = !a            (a of type t)
+    CQ_OP_AND,      // d = a && b        (a of type t, b of type t2)
+    CQ_OP_OR,       // d = a || b
+    CQ_OP_SEL,      // d = c ? a : b     (c of type t2; a, b already of the result type)
+    CQ_OP_FILTER,   // row survives only if a != 0 (type t)
+    CQ_OP_PROBE,    // probe join imm with key register a; payload into the join's payload register
+    CQ_OP_EXIT      // the row leaves the pipeline if no row of the wave is alive (early exit)
+};
+
+struct cq_vm_instr {
+    unsigned char op, t, t2, d;
+    unsigned char a, b, c, pad;
+    int imm;
+    unsigned int imm2;
+};
+
+// dimension table of one join, in the layout the code generator chose
+enum cq_vm_layout { CQ_L_REF = 0, CQ_L_COMPACT = 1, CQ_L_DENSE = 2 };
+struct cq_vm_join {
+    int layout;
+    int fk_slot;          // fact column of the probe key
+    int pk_dslot;         // dimension key column
+    int payload_dslot;    // -1 without payload
+    int pk_nullable, fk_nullable;
+    int kmin, len;        // reference slot function
+    int eb, tb, jlo, pmin;             // compact
+    int dmin, dspan, db, pbase, doff, zmode, zx;  // key-indexed
+    int build_begin, build_end;        // build-filter program (code range; empty: every row)
+};
+
+enum cq_vm_agg_fn { CQ_VM_SUM = 0, CQ_VM_MIN = 1, CQ_VM_MAX = 2, CQ_VM_COUNT = 3 };
+
+struct cq_vm_program {
+    int ncode;
+    struct cq_vm_instr code[CQ_VM_MAX_CODE];
+    int nregs;
+    int njoins;
+    struct cq_vm_join joins[8];
+    int scan_begin, scan_end;          // per-row pipeline
+    // aggregation: per aggregate the value / validity registers after the
+    // pipeline (value already converted to u64), function, hidden count slot
+    int naggs, ngroups, storage_ts, visible_ts;
+    int agg_fn[CQ_VM_MAX_AGGS];
+    int agg_val[CQ_VM_MAX_AGGS], agg_valid[CQ_VM_MAX_AGGS];  // registers (-1: constant 1 / always valid)
+    int agg_hidden[CQ_VM_MAX_AGGS];                          // AVG count slot or -1
+    int agg_static[CQ_VM_MAX_AGGS];                          // validity set on every hit (COUNT)
+    // GROUP BY: group index register, key value / validity registers
+    int group_reg;
+    long long groups;
+    int key_val[CQ_VM_MAX_AGGS], key_valid[CQ_VM_MAX_AGGS];
+    // decode of a dense domain (keys rebuilt from the group index at merge)
+    int decodable;
+    long long key_min[CQ_VM_MAX_AGGS];
+    long long key_card[CQ_VM_MAX_AGGS];
+    int key_nullable[CQ_VM_MAX_AGGS];
+    int fast_sparse;
+    // HAVING (one "row" per group)
+    int having_begin, having_end, having_reg;  // having_reg < 0: no HAVING
+    int having_type;
+};
+
+#endif

@@ -1,0 +1,16 @@
+// Host launchers of the interpreted tier (vm_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../device/query_params.h"
+#include "../device/vm_program.h"
+
+// V: the program in device memory
+hipError_t cq_vm_launch_build(const cq_query_params* P, const cq_vm_program* V, int join, long long dim_rows, hipStream_t s);
+// lds_group_bytes: dynamic LDS for the group aggregates (0: global atomics)
+hipError_t cq_vm_launch_scan(const cq_query_params* P, const cq_vm_program* V, unsigned lds_group_bytes, unsigned grid,
+                             hipStream_t s);
+hipError_t cq_vm_launch_dense_count(const cq_query_params* P, const cq_vm_program* V, const unsigned long long* R,
+                                    const unsigned long long* RV, unsigned long long* counts, long long G, hipStream_t s);
+int cq_vm_block_threads();

@@ -1,0 +1,404 @@
+// Interpreted tier: ahead-of-time kernels that execute the register code of
+// device/vm_program.h, so a plan runs without waiting for hipRTC.
+//
+//   cq_vm_build        one dimension table: build-filter program per row,
+//                      insert in the layout the code generator chose
+//   cq_vm_scan         the fact pipeline (filters, probes) and aggregation,
+//                      one row per thread per tile; the register file lives
+//                      in LDS ([register][thread], conflict-free), group
+//                      aggregates in LDS when the domain decodes and fits
+//   cq_vm_dense_count  present && HAVING per group slot
+//
+// Every instruction is uniform across the block (the program is read with
+// scalar loads), so interpretation costs one switch per instruction per wave.
+#include "vm_kernels.h"
+
+#include <algorithm>
+
+#include "../device/crystal_hip.h"
+
+namespace {
+
+using chip::i64;
+using chip::u32;
+using chip::u64;
+
+constexpr int VM_BT = 128;
+
+__device__ __forceinline__ bool truthy(u64 v, int t) {
+    return t == CQ_T_F64 ? __builtin_bit_cast(double, v) != 0.0 : v != 0ull;
+}
+__device__ __forceinline__ u64 canon(i64 v, int t) {  // keep the C value of type t in 64 bits
+    switch (t) {
+        case CQ_T_I32: return (u64)(i64)(int)(u32)(u64)v;
+        case CQ_T_U32: return (u64)(u32)(u64)v;
+        default: return (u64)v;
+    }
+}
+__device__ __forceinline__ double as_f64(u64 v, int t) {
+    switch (t) {
+        case CQ_T_I32: return (double)(int)(i64)v;
+        case CQ_T_U32: return (double)(u32)v;
+        case CQ_T_I64: return (double)(i64)v;
+        case CQ_T_U64: return (double)v;
+        default: return __builtin_bit_cast(double, v);
+    }
+}
+__device__ __forceinline__ u64 conv(u64 v, int to, int from) {
+    if (from == CQ_T_F64) {
+        const double d = __builtin_bit_cast(double, v);
+        switch (to) {
+            case CQ_T_I32: return (u64)(i64)(int)d;
+            case CQ_T_U32: return (u64)(u32)d;
+            case CQ_T_I64: return (u64)(i64)d;
+            case CQ_T_U64: return (u64)d;
+            default: return v;
+        }
+    }
+    if (to == CQ_T_F64) return __builtin_bit_cast(u64, as_f64(v, from));
+    return canon((i64)v, to);  // integer registers already hold the value modulo 2^64
+}
+__device__ __forceinline__ u64 arith(int op, u64 a, u64 b, int t) {
+    if (t == CQ_T_F64) {
+        const double x = __builtin_bit_cast(double, a), y = __builtin_bit_cast(double, b);
+        double r = 0;
+        switch (op) {
+            case CQ_OP_ADD: r = x + y; break;
+            case CQ_OP_SUB: r = x - y; break;
+            case CQ_OP_MUL: r = x * y; break;
+            default: r = x / y; break;
+        }
+        return __builtin_bit_cast(u64, r);
+    }
+    switch (op) {
+        case CQ_OP_ADD: return canon((i64)(a + b), t);
+        case CQ_OP_SUB: return canon((i64)(a - b), t);
+        case CQ_OP_MUL: return canon((i64)(a * b), t);
+        case CQ_OP_DIV:
+            switch (t) {
+                case CQ_T_I32: return canon(chip::div_((int)(i64)a, (int)(i64)b), t);
+                case CQ_T_U32: return canon((i64)chip::div_((u64)(u32)a, (u64)(u32)b), t);
+                case CQ_T_I64: return (u64)chip::div_((i64)a, (i64)b);
+                default: return chip::div_(a, b);
+            }
+        default:  // MOD (x % 0 and x % -1 are 0 here)
+            switch (t) {
+                case CQ_T_I32: { const int x = (int)(i64)a, y = (int)(i64)b; return canon((y == 0 || y == -1) ? 0 : x % y, t); }
+                case CQ_T_U32: { const u32 x = (u32)a, y = (u32)b; return y ? (u64)(x % y) : 0ull; }
+                case CQ_T_I64: { const i64 x = (i64)a, y = (i64)b; return (y == 0 || y == -1) ? 0ull : (u64)(x % y); }
+                default: return b ? a % b : 0ull;
+            }
+    }
+}
+__device__ __forceinline__ int compare(int op, u64 a, u64 b, int t) {
+    int lt, eq;
+    if (t == CQ_T_F64) {
+        const double x = __builtin_bit_cast(double, a), y = __builtin_bit_cast(double, b);
+        switch (op) {
+            case CQ_OP_EQ: return x == y;
+            case CQ_OP_NE: return x != y;
+            case CQ_OP_LT: return x < y;
+            case CQ_OP_LE: return x <= y;
+            case CQ_OP_GT: return x > y;
+            default: return x >= y;
+        }
+    }
+    if (t == CQ_T_I32 || t == CQ_T_I64) lt = (i64)a < (i64)b;
+    else lt = a < b;
+    eq = a == b;
+    switch (op) {
+        case CQ_OP_EQ: return eq;
+        case CQ_OP_NE: return !eq;
+        case CQ_OP_LT: return lt;
+        case CQ_OP_LE: return lt || eq;
+        case CQ_OP_GT: return !lt && !eq;
+        default: return !lt;
+    }
+}
+
+// Row context of one interpreted program.
+struct RowCtx {
+    const cq_query_params* P;
+    long long row;                 // fact row (scan) or dimension row (build)
+    const u64* R;                  // result slots of the group (HAVING)
+    const u64* RV;
+    long long slot_base;
+    bool in;                       // row exists (loads are guarded)
+};
+
+__device__ __forceinline__ bool probe(const cq_vm_join& J, const u32* ht, int fk, int& payload) {
+    switch (J.layout) {
+        case CQ_L_COMPACT: return chip::ht_probe_compact(ht, fk, J.kmin, J.len, J.eb, J.tb, J.jlo, J.pmin, payload);
+        case CQ_L_DENSE:
+            if ((J.db & (J.db - 1)) == 0)
+                return chip::ht_probe_dense<true>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload);
+            return chip::ht_probe_dense<false>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload);
+        default: return chip::ht_probe_ref((const int*)ht, fk, J.kmin, J.len, J.payload_dslot >= 0, payload);
+    }
+}
+
+// Runs code[begin, end) for this thread's row; registers are RF[r * VM_BT].
+// Returns whether the row survived the filters and probes.
+__device__ bool run(const cq_vm_program* __restrict__ V, int begin, int end, u64* RF, const RowCtx& c, bool alive) {
+    for (int pc = begin; pc < end; ++pc) {
+        const cq_vm_instr in = V->code[pc];
+        u64* d = RF + (int)in.d * VM_BT;
+        const u64 a = RF[(int)in.a * VM_BT], b = RF[(int)in.b * VM_BT];
+        switch (in.op) {
+            case CQ_OP_COL: *d = c.in ? (u64)(i64)c.P->col[in.imm][c.row] : 0ull; break;
+            case CQ_OP_VALID: *d = (c.in && c.P->valid[in.imm]) ? (u64)chip::bit32(c.P->valid[in.imm], c.row) : (c.in ? 1ull : 0ull); break;
+            case CQ_OP_DCOL: *d = c.in ? (u64)(i64)c.P->dcol[in.imm][c.row] : 0ull; break;
+            case CQ_OP_DVALID: *d = (c.in && c.P->dvalid[in.imm]) ? (u64)chip::bit32(c.P->dvalid[in.imm], c.row) : (c.in ? 1ull : 0ull); break;
+            case CQ_OP_LIT: *d = (u64)c.P->lit[in.imm]; break;
+            case CQ_OP_FLIT: *d = __builtin_bit_cast(u64, c.P->flit[in.imm]); break;
+            case CQ_OP_IMM: *d = canon((i64)(((u64)in.imm2 << 32) | (u64)(u32)in.imm), in.t); break;
+            case CQ_OP_SLOT: *d = c.in ? c.R[c.slot_base + in.imm] : 0ull; break;
+            case CQ_OP_SLOTV: *d = c.in ? (u64)chip::bit64(c.RV, c.slot_base + in.imm) : 0ull; break;
+            case CQ_OP_CONV: *d = conv(a, in.t, in.t2); break;
+            case CQ_OP_ADD: case CQ_OP_SUB: case CQ_OP_MUL: case CQ_OP_DIV: case CQ_OP_MOD: *d = arith(in.op, a, b, in.t); break;
+            case CQ_OP_EQ: case CQ_OP_NE: case CQ_OP_LT: case CQ_OP_LE: case CQ_OP_GT: case CQ_OP_GE:
+                *d = (u64)compare(in.op, a, b, in.t);
+                break;
+            case CQ_OP_NOT: *d = truthy(a, in.t) ? 0ull : 1ull; break;
+            case CQ_OP_AND: *d = (truthy(a, in.t) && truthy(b, in.t2)) ? 1ull : 0ull; break;
+            case CQ_OP_OR: *d = (truthy(a, in.t) || truthy(b, in.t2)) ? 1ull : 0ull; break;
+            case CQ_OP_SEL: *d = truthy(RF[(int)in.c * VM_BT], in.t2) ? a : b; break;
+            case CQ_OP_FILTER: alive = alive && truthy(a, in.t); break;
+            case CQ_OP_PROBE: {
+                const cq_vm_join& J = V->joins[in.imm];
+                int pay = 0;
+                if (alive) alive = probe(J, c.P->ht[in.imm], (int)(i64)a, pay);
+                *d = (u64)(i64)pay;
+                break;
+            }
+            case CQ_OP_EXIT:
+                if (__ballot(alive) == 0ull) return false;
+                break;
+            default: break;
+        }
+    }
+    return alive;
+}
+
+__global__ __launch_bounds__(VM_BT) void k_vm_build(const cq_query_params P, const cq_vm_program* __restrict__ V, int j) {
+    __shared__ u64 RF_all[CQ_VM_MAX_REGS * VM_BT];
+    const cq_vm_join& J = V->joins[j];
+    const long long n = P.dim_rows[j];
+    const int tid = threadIdx.x;
+    u64* RF = RF_all + tid;
+    for (long long base = (long long)blockIdx.x * VM_BT; base < n; base += (long long)gridDim.x * VM_BT) {
+        RowCtx c{&P, base + tid, nullptr, nullptr, 0, base + tid < n};
+        bool alive = run(V, J.build_begin, J.build_end, RF, c, c.in);
+        if (!alive) continue;
+        if (J.pk_nullable && !chip::bit32(P.dvalid[J.pk_dslot], c.row)) continue;
+        const int k = P.dcol[J.pk_dslot][c.row];
+        const int pay = J.payload_dslot >= 0 ? P.dcol[J.payload_dslot][c.row] : 0;
+        switch (J.layout) {
+            case CQ_L_COMPACT: chip::ht_insert_compact(P.ht[j], k, J.kmin, J.len, J.eb, J.tb, J.jlo, J.pmin, pay); break;
+            case CQ_L_DENSE: chip::ht_insert_dense(P.ht[j], k, J.dmin, J.dspan, J.db, J.pbase, pay); break;
+            default: chip::ht_insert_ref((int*)P.ht[j], k, J.kmin, J.len, J.payload_dslot >= 0, pay); break;
+        }
+    }
+}
+
+__device__ __forceinline__ u64 wave_reduce(u64 v, int fn) {
+    return fn == CQ_VM_MIN ? chip::wave_min(v) : (fn == CQ_VM_MAX ? chip::wave_max(v) : chip::wave_sum(v));
+}
+__device__ __forceinline__ void atomic_fn(u64* p, u64 v, int fn) {
+    if (fn == CQ_VM_MIN) atomicMin(p, v);
+    else if (fn == CQ_VM_MAX) atomicMax(p, v);
+    else atomicAdd(p, v);
+}
+__device__ __forceinline__ void set_bit_once(u64* w, long long b) {
+    u64* x = &w[b >> 6];
+    const u64 m = 1ull << (b & 63);
+    if (!(*x & m)) atomicOr(x, m);
+}
+
+// lds_groups: aggregate slots of every group in LDS (dynamic shared memory,
+// G * nas u64 + G u32 flag words: bit 31 hit, bit s validity of slot s)
+__global__ __launch_bounds__(VM_BT) void k_vm_scan(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
+    __shared__ u64 RF_all[CQ_VM_MAX_REGS * VM_BT];
+    __shared__ u64 s_acc[CQ_VM_MAX_AGGS * 2];
+    __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
+    extern __shared__ u64 s_dyn[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    u64* RF = RF_all + tid;
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    u64* s_agg = s_dyn;
+    u32* s_flag = (u32*)(s_dyn + (lds_groups ? G * nas : 0));
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += VM_BT) {
+            bool is_min = false;
+            for (int i = 0; i < V->naggs; ++i) is_min = is_min || (i == k && V->agg_fn[i] == CQ_VM_MIN);
+            s_acc[k] = is_min ? ~0ull : 0ull;
+            s_seen[k] = 0u;
+        }
+    } else if (lds_groups) {
+        for (long long k = tid; k < G * nas; k += VM_BT) {
+            const int s = (int)(k % nas);
+            bool is_min = false;
+            for (int i = 0; i < V->naggs; ++i) is_min = is_min || (ng + i == ng + s && V->agg_fn[i] == CQ_VM_MIN);
+            s_agg[k] = is_min ? ~0ull : 0ull;
+        }
+        for (long long k = tid; k < G; k += VM_BT) s_flag[k] = 0u;
+    }
+    __syncthreads();
+    const long long nrows = P.row_end - P.row_begin;
+    for (long long base = (long long)blockIdx.x * VM_BT; base < nrows; base += (long long)gridDim.x * VM_BT) {
+        RowCtx c{&P, P.row_begin + base + tid, nullptr, nullptr, 0, base + tid < nrows};
+        const bool alive = run(V, V->scan_begin, V->scan_end, RF, c, c.in);
+        if (ng == 0) {
+            for (int i = 0; i < V->naggs; ++i) {
+                const int fn = V->agg_fn[i];
+                const bool v = alive && (V->agg_valid[i] < 0 || truthy(RF[V->agg_valid[i] * VM_BT], CQ_T_U32));
+                const u64 x = fn == CQ_VM_COUNT ? 1ull : (V->agg_val[i] >= 0 ? RF[V->agg_val[i] * VM_BT] : 1ull);
+                const u64 id = fn == CQ_VM_MIN ? ~0ull : 0ull;
+                const u64 w = wave_reduce(v ? x : id, fn);
+                const bool any = __ballot(v) != 0ull;
+                if (lane == 0 && any) {
+                    atomic_fn(&s_acc[i], w, fn);
+                    s_seen[i] = 1u;
+                }
+                if (V->agg_hidden[i] >= 0) {
+                    const u64 cnt = wave_reduce(v ? 1ull : 0ull, CQ_VM_SUM);
+                    if (lane == 0 && cnt) atomicAdd(&s_acc[V->agg_hidden[i]], cnt);
+                }
+            }
+            continue;
+        }
+        if (!alive) continue;
+        const int h = (int)(i64)RF[V->group_reg * VM_BT];
+        if ((unsigned)h >= (unsigned long long)G) continue;
+        if (lds_groups) {
+            u32 want = 0x80000000u;
+            for (int g = 0; g < ng; ++g)
+                if (!V->fast_sparse && (V->key_valid[g] < 0 || truthy(RF[V->key_valid[g] * VM_BT], CQ_T_U32))) want |= 1u << g;
+            for (int i = 0; i < V->naggs; ++i) {
+                const int fn = V->agg_fn[i];
+                const bool v = V->agg_valid[i] < 0 || truthy(RF[V->agg_valid[i] * VM_BT], CQ_T_U32);
+                if (v) {
+                    const u64 x = fn == CQ_VM_COUNT ? 1ull : RF[V->agg_val[i] * VM_BT];
+                    atomic_fn(&s_agg[(long long)h * nas + i], x, fn);
+                    if (V->agg_hidden[i] >= 0) atomicAdd(&s_agg[(long long)h * nas + (V->agg_hidden[i] - ng)], 1ull);
+                    else if (!V->fast_sparse) want |= 1u << (ng + i);
+                }
+                if (fn == CQ_VM_COUNT && !V->fast_sparse) want |= 1u << (ng + i);
+            }
+            if ((s_flag[h] & want) != want) atomicOr(&s_flag[h], want);
+        } else {
+            const long long out = (long long)h * sts;
+            for (int g = 0; g < ng; ++g) {
+                P.result[out + g] = RF[V->key_val[g] * VM_BT];
+                if (!V->fast_sparse && (V->key_valid[g] < 0 || truthy(RF[V->key_valid[g] * VM_BT], CQ_T_U32)))
+                    set_bit_once(P.result_valid, out + g);
+            }
+            for (int i = 0; i < V->naggs; ++i) {
+                const int fn = V->agg_fn[i];
+                const int slot = ng + i;
+                const bool v = V->agg_valid[i] < 0 || truthy(RF[V->agg_valid[i] * VM_BT], CQ_T_U32);
+                if (v) {
+                    const u64 x = fn == CQ_VM_COUNT ? 1ull : RF[V->agg_val[i] * VM_BT];
+                    atomic_fn(&P.result[out + slot], x, fn);
+                    if (V->agg_hidden[i] >= 0) atomicAdd(&P.result[out + V->agg_hidden[i]], 1ull);
+                    else if (fn != CQ_VM_COUNT && !V->fast_sparse) set_bit_once(P.result_valid, out + slot);
+                }
+                if (fn == CQ_VM_COUNT && !V->fast_sparse) set_bit_once(P.result_valid, out + slot);
+            }
+        }
+    }
+    __syncthreads();
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += VM_BT) {
+            int fn = CQ_VM_SUM;
+            bool hidden = false, scalar_valid_bit = false;
+            for (int i = 0; i < V->naggs; ++i) {
+                if (i == k) {
+                    fn = V->agg_fn[i];
+                    scalar_valid_bit = fn != CQ_VM_COUNT && V->agg_hidden[i] < 0;
+                }
+                hidden = hidden || V->agg_hidden[i] == k;
+            }
+            const u64 x = s_acc[k];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
+            else if (x) atomicAdd(&P.result[k], x);
+            if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
+        }
+        return;
+    }
+    if (!lds_groups) return;
+    for (long long g = tid; g < G; g += VM_BT) {
+        const u32 f = s_flag[g];
+        if (!(f & 0x80000000u)) continue;
+        const long long base = g * sts;
+        long long rem = g;
+        long long d[CQ_VM_MAX_AGGS];
+        for (int k = ng - 1; k >= 0; --k) {
+            d[k] = rem % V->key_card[k];
+            rem /= V->key_card[k];
+        }
+        for (int k = 0; k < ng; ++k) {
+            const long long v = V->key_nullable[k] ? (d[k] == 0 ? 0ll : V->key_min[k] + d[k] - 1) : V->key_min[k] + d[k];
+            P.result[base + k] = (u64)v;
+        }
+        for (int s = 0; s < nas; ++s) {
+            int fn = CQ_VM_SUM;
+            for (int i = 0; i < V->naggs; ++i)
+                if (i == s) fn = V->agg_fn[i];
+            const u64 x = s_agg[g * nas + s];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[base + ng + s], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[base + ng + s], x); }
+            else if (x) atomicAdd(&P.result[base + ng + s], x);
+        }
+        if (!V->fast_sparse)
+            for (int s = 0; s < sts; ++s)
+                if ((f >> s) & 1u) chip::set_bit64(P.result_valid, base + s);
+    }
+}
+
+__global__ __launch_bounds__(VM_BT) void k_vm_dense_count(const cq_query_params P, const cq_vm_program* __restrict__ V,
+                                                          const u64* __restrict__ R, const u64* __restrict__ RV, u64* counts,
+                                                          long long G) {
+    __shared__ u64 RF_all[CQ_VM_MAX_REGS * VM_BT];
+    const int tid = threadIdx.x;
+    u64* RF = RF_all + tid;
+    const long long g = (long long)blockIdx.x * VM_BT + tid;
+    const int vts = V->visible_ts;
+    const long long base = g * vts;
+    RowCtx c{&P, g, R, RV, base, g < G};
+    bool present = V->ngroups == 0;
+    if (c.in)
+        for (int s = 0; s < vts && !present; ++s) present = chip::bit64(RV, base + s) != 0u || R[base + s] != 0ull;
+    bool keep = present;
+    if (V->having_reg >= 0) {
+        run(V, V->having_begin, V->having_end, RF, c, c.in);
+        keep = keep && truthy(RF[V->having_reg * VM_BT], V->having_type);
+    }
+    if (c.in) counts[g] = keep ? 1ull : 0ull;
+}
+
+}  // namespace
+
+hipError_t cq_vm_launch_build(const cq_query_params* P, const cq_vm_program* V, int join, long long dim_rows, hipStream_t s) {
+    const unsigned grid = (unsigned)std::min<long long>(std::max<long long>((dim_rows + VM_BT - 1) / VM_BT, 1), 8192);
+    hipLaunchKernelGGL(k_vm_build, dim3(grid), dim3(VM_BT), 0, s, *P, V, join);
+    return hipGetLastError();
+}
+
+hipError_t cq_vm_launch_scan(const cq_query_params* P, const cq_vm_program* V, unsigned lds_group_bytes, unsigned grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_vm_scan, dim3(grid), dim3(VM_BT), lds_group_bytes, s, *P, V, lds_group_bytes ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t cq_vm_launch_dense_count(const cq_query_params* P, const cq_vm_program* V, const unsigned long long* R,
+                                    const unsigned long long* RV, unsigned long long* counts, long long G, hipStream_t s) {
+    const unsigned grid = (unsigned)std::max<long long>((G + VM_BT - 1) / VM_BT, 1);
+    hipLaunchKernelGGL(k_vm_dense_count, dim3(grid), dim3(VM_BT), 0, s, *P, V, R, RV, counts, G);
+    return hipGetLastError();
+}
+
+int cq_vm_block_threads() { return VM_BT; }

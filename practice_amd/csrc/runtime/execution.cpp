@@ -10,6 +10,7 @@
 
 #include "../frontend/sql.h"
 #include "../kernels/aot_kernels.h"
+#include "../kernels/vm_kernels.h"
 
 extern const char* const kCrystalHipHeaderSource;
 extern const char* const kQueryParamsHeaderSource;
@@ -259,10 +260,11 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
         P.ht[j] = p.shared_tables ? nullptr
                                   : static_cast<unsigned int*>(take("ht" + std::to_string(j), std::max<size_t>(p.joins[j].ht_words * 4, 16)));
         P.dim_rows[j] = p.joins[j].dim_rows;
-        run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
+        if (!p.vm) run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
     }
-    run.dense_count = p.fast_sparse ? nullptr : module->function("cq_dense_count");
+    run.dense_count = (p.fast_sparse || p.vm) ? nullptr : module->function("cq_dense_count");
     run.keys = p.keys_kernel ? module->function("cq_keys") : nullptr;
+    run.vm = p.vm ? static_cast<cq_vm_program*>(take("vm", sizeof(cq_vm_program))) : nullptr;
 
     // aggregate state
     const int64_t G = p.groups;
@@ -323,6 +325,13 @@ void HipModuleExecutor::setup(QueryRun& run, ExecutionContext* ctx, const cq_que
     unsigned int min_mask = 0;
     for (int slot : p.min_slots) min_mask |= 1u << slot;
     if (min_mask) hipCheck(cq_launch_init_min(P.result, run.G, p.storage_ts, min_mask, s), "init MIN slots");
+    if (p.vm) {
+        // the interpreted tier: program upload, then the generic build kernel
+        hipCheck(hipMemcpyAsync(run.vm, p.vm.get(), sizeof(cq_vm_program), hipMemcpyHostToDevice, s), "upload program");
+        for (size_t j = 0; j < p.joins.size(); ++j)
+            hipCheck(cq_vm_launch_build(&P, run.vm, static_cast<int>(j), p.joins[j].dim_rows, s), "interpreted build");
+        return;
+    }
     if (!p.shared_tables) buildTables(run, ctx, device_params);
 }
 
@@ -347,6 +356,14 @@ unsigned scanGrid(hipFunction_t f, int block_threads, int device, long long num_
 
 void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
     const QueryProgram& p = *run.program;
+    if (p.vm) {
+        const long long rows = run.P.row_end - run.P.row_begin;
+        if (rows <= 0) return;
+        const long long bt = cq_vm_block_threads();
+        const unsigned grid = static_cast<unsigned>(std::min<long long>((rows + bt - 1) / bt, static_cast<long long>(deviceCUs(ctx->device)) * 8));
+        hipCheck(cq_vm_launch_scan(&run.P, run.vm, static_cast<unsigned>(p.vm_lds_bytes), grid, ctx->stream), "interpreted scan");
+        return;
+    }
     hipFunction_t f = run.module->function("cq_scan");
     const unsigned grid = scanGrid(f, p.block_threads, ctx->device, run.P.num_tiles);
     void* args[] = {&run.P};
@@ -394,8 +411,10 @@ void HipModuleExecutor::shape(QueryRun& run, ExecutionContext* ctx, const cq_que
         const cq_query_params* dp = device_params;
         void* by_value[] = {&P, &R, &RV, &counts, &g};
         void* by_pointer[] = {&dp, &R, &RV, &counts, &g};
-        launch(run.dense_count, static_cast<unsigned>((G + 255) / 256), 256, s, device_params ? by_pointer : by_value,
-               "launch dense count");
+        if (p.vm) hipCheck(cq_vm_launch_dense_count(&P, run.vm, R, RV, counts, g, s), "interpreted dense count");
+        else
+            launch(run.dense_count, static_cast<unsigned>((G + 255) / 256), 256, s, device_params ? by_pointer : by_value,
+                   "launch dense count");
     }
     hipCheck(cq_launch_exclusive_scan(run.fl, run.off, G, run.bsums, row_count ? row_count : run.off + run.cap_rows, s), "scan");
     unsigned int nmask = 0;
@@ -610,7 +629,7 @@ void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
     timing.parse_ms = elapsedMs(t0, t1);
     timing.codegen_ms = elapsedMs(t1, t2);
     preflightDeviceMemoryOrThrow(*ctx, ctx->program);
-    ModuleHandle m = moduleFor(ctx->program, &timing);
+    ModuleHandle m = ctx->program.vm ? nullptr : moduleFor(ctx->program, &timing);
     executor_->execute(m, ctx);
     timing.gpu_execute_ms = ctx->timing.gpu_execute_ms;
     timing.scan_kernel_ms = ctx->timing.scan_kernel_ms;

@@ -223,6 +223,7 @@ struct QueryRun {
     std::vector<hipFunction_t> builds;
     hipFunction_t dense_count = nullptr;
     hipFunction_t keys = nullptr;       // cq_keys (group keys decoded after the scan)
+    cq_vm_program* vm = nullptr;        // interpreted tier: the program in device memory
     int64_t G = 1;
     size_t slots = 0;
     unsigned long long* res = nullptr;  // visible tuples (after AVG finalization)

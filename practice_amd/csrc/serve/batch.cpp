@@ -31,6 +31,8 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "autotune") o.autotune = v != 0;
         else if (k == "max_templates") o.max_templates = static_cast<int>(v);
         else if (k == "max_queries") o.max_queries = static_cast<int>(v);
+        else if (k == "shared") o.shared_tables = v != 0;
+        else if (k == "lds_agg") o.lds_aggregation = v != 0;
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
@@ -73,6 +75,8 @@ CodegenOptions BatchExecutor::codegenOptions(const BatchOptions& opt) const {
     co.batch_body = true;
     co.block_threads = opt.block_threads;
     co.vectors_per_thread = opt.vectors_per_thread;
+    co.batch_shared_tables = opt.shared_tables;
+    co.allow_lds_aggregation = co.allow_lds_aggregation && opt.lds_aggregation;
     return co;
 }
 

@@ -31,6 +31,8 @@ struct BatchOptions {
     bool autotune = false;        // time block / load-width variants, keep the fastest
     int max_templates = 16;       // templates per shared scan
     int max_queries = 256;        // queries per shared scan
+    bool shared_tables = true;    // one probe per template and dimension (packed attributes)
+    bool lds_aggregation = true;  // per-query LDS aggregation for small group domains
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates, max_queries)

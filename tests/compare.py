@@ -26,4 +26,6 @@ def assert_results_equal(a, b, what=""):
     ordered = a.ordered and b.ordered
     x = normalize(a.tuples(), ordered)
     y = normalize(b.tuples(), ordered)
-    assert x == y, f"{what}: {len(x)} rows vs {len(y)} rows differ"
+    if x != y:
+        diff = [(i, u, v) for i, (u, v) in enumerate(zip(x, y)) if u != v][:5]
+        raise AssertionError(f"{what}: {len(x)} rows vs {len(y)} rows; first differences (row, got, want): {diff}")
