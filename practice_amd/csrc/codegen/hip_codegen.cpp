@@ -1531,6 +1531,12 @@ private:
                 prog_.batch_lds_stride = bytes;
                 prog_.batch_lds_init = pro.str();
                 prog_.batch_lds_flush = epi.str();
+            } else if (opt_.allow_lds_aggregation && hashAggEligible() && hashAggBytes() <= 4096) {
+                emitLdsHashAggregate(aggcode, pro, epi);
+                prog_.lds_aggregation = true;
+                prog_.batch_lds_stride = hashAggBytes();
+                prog_.batch_lds_init = pro.str();
+                prog_.batch_lds_flush = epi.str();
             } else {
                 emitGlobalAggregate(aggcode);
             }
@@ -1818,17 +1824,24 @@ private:
     }
 
     void emitGlobalAggregate(std::ostringstream& code) {
-        const auto* a = agg_;
-        const int ng = prog_.ngroups, sts = prog_.storage_ts;
+        const int sts = prog_.storage_ts;
         const int64_t G = prog_.groups;
-        const bool decode = keysDecodable();
-        prog_.keys_kernel = decode;
+        prog_.keys_kernel = keysDecodable();
         code << "        // GROUP BY: perfect hash, global u64 atomics per row\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
              << "            if (!((fl >> i) & 1u)) continue;\n"
              << "            const int h = " << groupHash() << ";\n"
              << "            if ((unsigned)h >= " << G << "u) continue;\n"
              << "            const long long out = (long long)h * " << sts << ";\n";
+        globalRow(code, "            ");
+        code << "        }\n";
+    }
+
+    // per-row global aggregation statements (h / out in scope)
+    void globalRow(std::ostringstream& code, const std::string& ind) {
+        const auto* a = agg_;
+        const int ng = prog_.ngroups;
+        const bool decode = keysDecodable();
         auto setbit = [&](const std::string& idx) {
             return "{ const long long b = " + idx + "; unsigned long long* w = &P.result_valid[b >> 6]; const u64 m = 1ull << (b & 63); if (!(*w & m)) atomicOr(w, m); }";
         };
@@ -1840,32 +1853,149 @@ private:
             if (decode) {
                 if (isStatic(g)) continue;  // written by cq_keys
                 const std::string v = vvalid(e);
-                if (!litFalse(v)) code << "            if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
+                if (!litFalse(v)) code << ind << "if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
                 continue;
             }
-            code << "            P.result[out + " << g << "] = (u64)(" << vexpr(e) << ");\n";
+            code << ind << "P.result[out + " << g << "] = (u64)(" << vexpr(e) << ");\n";
             if (!prog_.fast_sparse) {
                 const std::string v = vvalid(e);
-                if (litTrue(v)) code << "            " << setbit("out + " + std::to_string(g)) << "\n";
-                else if (!litFalse(v)) code << "            if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
+                if (litTrue(v)) code << ind << setbit("out + " + std::to_string(g)) << "\n";
+                else if (!litFalse(v)) code << ind << "if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
             }
         }
         for (int i = 0; i < prog_.naggs; ++i) {
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             const AggRow r = aggRow(i);
             const int slot = ng + i;
-            code << "            if (" << r.valid << ") {\n";
+            code << ind << "if (" << r.valid << ") {\n";
             const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
-            code << "                " << atom << "(&P.result[out + " << slot << "], " << r.value << ");\n";
+            code << ind << "    " << atom << "(&P.result[out + " << slot << "], " << r.value << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
-            if (hs >= 0) code << "                atomicAdd(&P.result[out + " << hs << "], 1ull);\n";
+            if (hs >= 0) code << ind << "    atomicAdd(&P.result[out + " << hs << "], 1ull);\n";
             else if (f != "COUNT" && !prog_.fast_sparse && !(decode && isStatic(slot)))
-                code << "                " << setbit("out + " + std::to_string(slot)) << "\n";
-            code << "            }\n";
-            if (f == "COUNT" && !prog_.fast_sparse && !decode) code << "            " << setbit("out + " + std::to_string(slot)) << "\n";
+                code << ind << "    " << setbit("out + " + std::to_string(slot)) << "\n";
+            code << ind << "}\n";
+            if (f == "COUNT" && !prog_.fast_sparse && !decode) code << ind << setbit("out + " + std::to_string(slot)) << "\n";
         }
-        if (decode) code << "            " << setbit("out + " + std::to_string(flag_static_.front())) << "  // hit\n";
-        code << "        }\n";
+        if (decode) code << ind << setbit("out + " + std::to_string(flag_static_.front())) << "  // hit\n";
+    }
+
+    // Batched queries whose dense domain does not fit their LDS region: an
+    // open-addressing table of kHashSlots groups per query in LDS (group
+    // index + 1 as key, linear probing up to 8 slots); rows of groups that
+    // find no slot take the global path.  Keys come back from the group
+    // index: written by the merge (sparse output) or by cq_keys.
+    static constexpr int kHashSlots = 128;
+    bool hashAggEligible() {
+        if (!prog_.grouped || prog_.storage_ts > 64) return false;
+        for (bool d : gdense_)
+            if (!d) return false;
+        return prog_.fast_sparse || keysDecodable();
+    }
+    uint64_t hashAggBytes() const {
+        const uint64_t nas = static_cast<uint64_t>(prog_.storage_ts - prog_.ngroups);
+        return static_cast<uint64_t>(kHashSlots) * (8 + 8 * nas);
+    }
+    void emitLdsHashAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
+        const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
+        const int64_t G = prog_.groups;
+        const int S = kHashSlots;
+        prog_.keys_kernel = !prog_.fast_sparse;
+        // region: u32 keys[S], u32 flags[S], u64 vals[S * nas]
+        std::string init = "0ull";
+        if (!prog_.min_slots.empty()) {
+            std::string cond;
+            for (int s : prog_.min_slots) cond += (cond.empty() ? "" : " || ") + std::string("k % ") + std::to_string(nas) + " == " + std::to_string(s - ng);
+            init = "((" + cond + ") ? ~0ull : 0ull)";
+        }
+        pro << "    for (int k = tid; k < " << 2 * S << "; k += BT) ((u32*)s_agg)[k] = 0u;\n"
+            << "    for (int k = tid; k < " << S * nas << "; k += BT) s_agg[" << S / 1 << " + k] = " << init << ";\n";
+        code << "        // GROUP BY: LDS hash table of this query, global atomics on overflow\n#pragma unroll\n"
+             << "        for (int i = 0; i < IPT; ++i) {\n"
+             << "            if (!((fl >> i) & 1u)) continue;\n"
+             << "            const int h = " << groupHash() << ";\n"
+             << "            if ((unsigned)h >= " << G << "u) continue;\n"
+             << "            u32* s_hk = (u32*)s_agg;\n"
+             << "            u32* s_hf = s_hk + " << S << ";\n"
+             << "            u64* s_hv = s_agg + " << S << ";\n"
+             << "            u32 slot = ((u32)h * 2654435761u) >> " << (32 - 7) << ";\n"
+             << "            int found = -1;\n"
+             << "            for (int t = 0; t < 8; ++t) {\n"
+             << "                u32 cur = s_hk[slot];\n"
+             << "                if (cur == 0u) { const u32 old = atomicCAS(&s_hk[slot], 0u, (u32)h + 1u); cur = old == 0u ? (u32)h + 1u : old; }\n"
+             << "                if (cur == (u32)h + 1u) { found = (int)slot; break; }\n"
+             << "                slot = (slot + 1u) & " << (S - 1) << "u;\n"
+             << "            }\n"
+             << "            if (found >= 0) {\n"
+             << "                u32 want = 1u;\n";
+        const auto* a = agg_;
+        for (int g : flag_dyn_) {
+            if (g >= ng || prog_.fast_sparse) continue;
+            code << "                if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") want |= 1u << " << dynBit(g)
+                 << ";\n";
+        }
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const AggRow r = aggRow(i);
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const std::string ref = "s_hv[found * " + std::to_string(nas) + " + " + std::to_string(i) + "]";
+            code << "                if (" << r.valid << ") {\n";
+            if (f == "MIN") code << "                    atomicMin(&" << ref << ", " << r.value << ");\n";
+            else if (f == "MAX") code << "                    atomicMax(&" << ref << ", " << r.value << ");\n";
+            else code << "                    atomicAdd(&" << ref << ", " << r.value << ");\n";
+            const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
+            if (hs >= 0) code << "                    atomicAdd(&s_hv[found * " << nas << " + " << (hs - ng) << "], 1ull);\n";
+            const int db = prog_.fast_sparse ? -1 : dynBit(ng + i);
+            if (db > 0) code << "                    want |= 1u << " << db << ";\n";
+            code << "                }\n";
+        }
+        code << "                if ((s_hf[found] & want) != want) atomicOr(&s_hf[found], want);\n"
+             << "            } else {\n"
+             << "                const long long out = (long long)h * " << sts << ";\n";
+        globalRow(code, "                ");
+        code << "            }\n"
+             << "        }\n";
+        // merge: aggregates (and sparse keys) of every used slot
+        epi << "    for (int slot = tid; slot < " << S << "; slot += BT) {\n"
+            << "        const u32* s_hk = (const u32*)s_agg;\n"
+            << "        const u32 key = s_hk[slot];\n"
+            << "        if (key == 0u) continue;\n"
+            << "        const long long g = (long long)key - 1;\n"
+            << "        const u32 f = s_hk[" << S << " + slot];\n"
+            << "        const long long base = g * " << sts << ";\n";
+        if (prog_.fast_sparse) {
+            epi << "        long long rem = g;\n";
+            for (int g = ng - 1; g >= 0; --g)
+                epi << "        const long long d" << g << " = rem % " << gcard_[static_cast<size_t>(g)] << "; rem /= "
+                    << gcard_[static_cast<size_t>(g)] << ";\n";
+            for (int g = 0; g < ng; ++g) {
+                const std::string mn = std::to_string(gmin_[static_cast<size_t>(g)]);
+                std::string val;
+                if (gnullable_[static_cast<size_t>(g)]) val = "(d" + std::to_string(g) + " == 0 ? 0ll : (long long)" + mn + " + d" + std::to_string(g) + " - 1)";
+                else val = "((long long)" + mn + " + d" + std::to_string(g) + ")";
+                epi << "        P.result[base + " << g << "] = (u64)" << val << ";\n";
+            }
+        }
+        for (int s2 = ng; s2 < sts; ++s2) {
+            std::string f = "SUM";
+            for (int i = 0; i < prog_.naggs; ++i)
+                if (ng + i == s2) f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const std::string v = "s_agg[" + std::to_string(S) + " + slot * " + std::to_string(nas) + " + " + std::to_string(s2 - ng) + "]";
+            if (f == "MIN") epi << "        { const u64 x = " << v << "; if (x != ~0ull) atomicMin(&P.result[base + " << s2 << "], x); }\n";
+            else if (f == "MAX") epi << "        { const u64 x = " << v << "; if (x) atomicMax(&P.result[base + " << s2 << "], x); }\n";
+            else epi << "        { const u64 x = " << v << "; if (x) atomicAdd(&P.result[base + " << s2 << "], x); }\n";
+        }
+        if (!prog_.fast_sparse) {
+            unsigned long long smask = 0;
+            for (int sl : flag_static_) smask |= 1ull << sl;
+            epi << "        u64 vm = " << smask << "ull;\n";
+            for (int sl : flag_dyn_) epi << "        vm |= (u64)((f >> " << dynBit(sl) << ") & 1u) << " << sl << ";\n";
+            epi << "        { const u32 o = (u32)(base & 63);\n"
+                << "          atomicOr(&P.result_valid[base >> 6], vm << o);\n"
+                << "          if (o + " << sts << "u > 64u) atomicOr(&P.result_valid[(base >> 6) + 1], vm >> (64u - o)); }\n";
+        } else {
+            epi << "        (void)f;\n";
+        }
+        epi << "    }\n";
     }
 
     void emitKeysKernel(std::ostringstream& src) {

@@ -26,6 +26,7 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "budget_mb") o.budget_bytes = static_cast<size_t>(v) << 20;
         else if (k == "window_us") o.window_us = static_cast<int>(v);
         else if (k == "async_compile") o.async_compile = v != 0;
+        else if (k == "interpret") o.interpret_cold = v != 0;
         else if (k == "bt" || k == "vpt" || k == "graph") batch_opts += (batch_opts.empty() ? "" : ",") + item;
         else throw std::runtime_error("unknown server option: " + k);
     }
@@ -174,7 +175,7 @@ void QueryServer::laneLoop(Lane* lane) {
 
         // hot signatures join the shared scan, cold ones go to compilation;
         // admission reserves each request's footprint against the budget
-        std::vector<Request> batch, plain, later, later_plain;
+        std::vector<Request> batch, plain, later, later_plain, interp;
         std::vector<std::shared_ptr<QueryProgram>> set;
         bool compile_work = false;
         {
@@ -187,9 +188,17 @@ void QueryServer::laneLoop(Lane* lane) {
                     if (n >= opt_.promote_after && !promoting_.count(sig)) {
                         promoting_.insert(sig);
                         promote_q_.push_back(r.program);
+                        compile_work = true;
                     }
-                    compile_q_.push_back(std::move(r));
-                    compile_work = true;
+                    if (!opt_.interpret_cold) {
+                        compile_q_.push_back(std::move(r));
+                        compile_work = true;
+                    } else if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
+                        in_flight_bytes_ += r.footprint;
+                        interp.push_back(std::move(r));
+                    } else {
+                        later.push_back(std::move(r));
+                    }
                 } else if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
                     in_flight_bytes_ += r.footprint;
                     batch.push_back(std::move(r));
@@ -215,7 +224,7 @@ void QueryServer::laneLoop(Lane* lane) {
                 while (compileOne()) {
                 }
         }
-        if (batch.empty() && plain.empty()) {
+        if (batch.empty() && plain.empty() && interp.empty()) {
             if (!later.empty() || !later_plain.empty()) {
                 // nothing admissible right now: wait for memory to free up
                 std::unique_lock<std::mutex> lock(mu_);
@@ -245,6 +254,24 @@ void QueryServer::laneLoop(Lane* lane) {
                 else finish(batch[i], -3, err, nullptr);
             }
         }
+        for (auto& r : interp) {
+            // cold signature: its own scan on the interpreted tier, no compile
+            released += r.footprint;
+            try {
+                std::lock_guard<std::mutex> lock(lane->mu);
+                ExecutionContext* ctx = lane->ctx.get();
+                ctx->row_begin = 0;
+                ctx->row_end = -1;
+                ctx->config.codegen.vm = true;
+                db_.engine().executeQuery(r.sql, ctx);
+                ctx->config.codegen.vm = false;
+                QueryResult res = std::move(ctx->result);
+                finish(r, 0, "", &res);
+            } catch (const std::exception& e) {
+                lane->ctx->config.codegen.vm = false;
+                finish(r, -3, e.what(), nullptr);
+            }
+        }
         for (auto& r : plain) {
             // cold signature: its own scan on the plain path (module compiled)
             released += r.footprint;
@@ -268,6 +295,7 @@ void QueryServer::laneLoop(Lane* lane) {
                 batched_queries_ += static_cast<long long>(batch.size());
             }
             plain_queries_ += static_cast<long long>(plain.size());
+            interpreted_queries_ += static_cast<long long>(interp.size());
         }
         cv_.notify_all();
     }
@@ -384,7 +412,7 @@ int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::s
 void QueryServer::resetStats() {
     std::lock_guard<std::mutex> lock(mu_);
     latencies_.clear();
-    batches_ = batched_queries_ = plain_queries_ = admission_waits_ = compiles_ = promotions_ = rejected_ = 0;
+    batches_ = batched_queries_ = plain_queries_ = interpreted_queries_ = admission_waits_ = compiles_ = promotions_ = rejected_ = 0;
     stats_start_ = std::chrono::steady_clock::now();
 }
 
@@ -403,7 +431,8 @@ std::string QueryServer::statsJson() {
       << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
       << ",\"batches\":" << batches_ << ",\"mean_batch\":"
       << (batches_ ? static_cast<double>(batched_queries_) / static_cast<double>(batches_) : 0.0)
-      << ",\"plain_queries\":" << plain_queries_ << ",\"admission_waits\":" << admission_waits_
+      << ",\"plain_queries\":" << plain_queries_ << ",\"interpreted_queries\":" << interpreted_queries_
+      << ",\"admission_waits\":" << admission_waits_
       << ",\"compiles\":" << compiles_ << ",\"promotions\":" << promotions_ << ",\"rejected\":" << rejected_
       << ",\"lanes\":" << opt_.lanes << ",\"budget_bytes\":" << opt_.budget_bytes << ",\"in_flight_bytes\":" << in_flight_bytes_
       << ",\"scan_set\":" << scan_set_.size() << "}";

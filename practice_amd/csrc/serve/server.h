@@ -7,14 +7,14 @@
 // while one lane's work is on the GPU the next lane admits the requests that
 // arrived meanwhile, so the lanes' work overlaps on the device.
 //
-// Two execution tiers per plan signature:
+// Execution tiers per plan signature:
 //   hot    signatures in the compiled shared-scan set run as one batch
 //          (cq_batch_scan: one pass over the fact columns for all of them)
-//   cold   a signature seen for the first time runs alone on the plain
-//          one-query path as soon as its module is compiled; once seen
+//   cold   a signature not in the set runs at once on the ahead-of-time
+//          interpreter (no compilation on the request's path); once seen
 //          `promote_after` times it is compiled into the shared-scan set
-// Compilation (hipRTC) happens on background threads, so a new plan shape
-// delays only its own requests.
+//          on a background thread.  With interpret_cold off, cold requests
+//          wait for their plain module instead.
 //
 // Admission: a request's device footprint (hash tables, aggregate slots,
 // materialization buffers; estimateQueryTemporaryBytes) is reserved against
@@ -45,11 +45,13 @@ struct ServerOptions {
     size_t budget_bytes = 0;       // HBM for query temporaries (0: quarter of free memory)
     int window_us = 0;             // wait for more arrivals before taking a batch
     bool async_compile = true;     // compile off the lanes
+    bool interpret_cold = true;    // run new signatures at once on the interpreted tier
     BatchOptions batch;
 };
 
 // "key=value,..." with lanes, max_batch, max_templates, promote_after,
-// compile_threads, budget_mb, window_us, async_compile, bt, vpt, graph
+// compile_threads, budget_mb, window_us, async_compile, interpret, bt, vpt,
+// graph
 ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
@@ -111,8 +113,8 @@ private:
     // statistics
     std::vector<double> latencies_;
     std::chrono::steady_clock::time_point stats_start_;
-    long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, admission_waits_ = 0, compiles_ = 0,
-              promotions_ = 0, rejected_ = 0;
+    long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, interpreted_queries_ = 0, admission_waits_ = 0,
+              compiles_ = 0, promotions_ = 0, rejected_ = 0;
 
     void start();
     void laneLoop(Lane* lane);

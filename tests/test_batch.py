@@ -49,8 +49,8 @@ def test_shared_scan_loads_each_column_once(hdb):
     # union of fact columns: C1 {orderdate, discount, quantity, extendedprice},
     # C2 {orderdate, partkey, suppkey, revenue}, C3 {custkey, suppkey, orderdate, revenue}
     assert src.count("__builtin_nontemporal_load") == 8
-    # three templates (the two C2 queries share one), one runtime loop each
-    assert src.count("for (int q = B.tq_begin[") == 3
+    # three templates (the two C2 queries share one), one block each per tile
+    assert src.count("        // template ") == 3
     pa.compile_source(src)
 
 
@@ -62,7 +62,7 @@ def test_templates_with_identical_builds_stay_distinct(hdb):
           "select count(*) from lineorder where lo_revenue < 7000000"]
     assert hdb.generate_batch_code(qs, part=0) == hdb.generate_batch_code(qs, part=1)
     src = hdb.generate_batch_code(qs)
-    assert src.count("for (int q = B.tq_begin[") == 2
+    assert src.count("        // template ") == 2
 
 
 def test_too_many_templates_for_one_scan(hdb):
