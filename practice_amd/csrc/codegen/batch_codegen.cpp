@@ -13,6 +13,22 @@
 
 namespace db {
 
+namespace {
+
+// pointers to query q's LDS aggregation region of template t (stride and
+// hash size set by the launch)
+std::string ldsRegion(const QueryProgram* p, size_t t, const std::string& ind) {
+    std::ostringstream o;
+    o << ind << "u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t
+      << "]) * B.lds_stride[" << t << "]);\n"
+      << ind << "u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n"
+      << ind << "const unsigned cq_hbits = B.hbits[" << t << "];\n"
+      << ind << "(void)s_flag; (void)cq_hbits;\n";
+    return o.str();
+}
+
+}  // namespace
+
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt,
                                     std::vector<std::string>& union_cols, std::vector<bool>& union_nullable) {
     if (templates.empty()) throw std::runtime_error("batch without templates");
@@ -47,10 +63,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
             const QueryProgram* p = templates[t];
             if (p->batch_lds_stride == 0) continue;
             src << "    for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
-                << "        u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
-                << p->batch_lds_stride << "u);\n"
-                << "        u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n"
-                << "        (void)s_flag;\n"
+                << ldsRegion(p, t, "        ")
                 << p->batch_lds_init << "    }\n";
         }
         src << "    __syncthreads();\n";
@@ -97,10 +110,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
             src << "        const cq_query_params& P0 = B.qp[B.tq_begin[" << t << "]];\n" << p->batch_shared;
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
             << "            const cq_query_params& P = B.qp[q];\n";
-        if (p->batch_lds_stride > 0)
-            src << "            u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
-                << p->batch_lds_stride << "u);\n"
-                << "            u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n";
+        if (p->batch_lds_stride > 0) src << ldsRegion(p, t, "            ");
         src << "            u32 fl = tile_fl;\n"
             << "            do {\n"
             << p->batch_body
@@ -117,9 +127,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
             if (p->batch_lds_stride == 0) continue;
             src << "    for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
                 << "        const cq_query_params& P = B.qp[q];\n"
-                << "        u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[" << t << "] + (unsigned)(q - B.tq_begin[" << t << "]) * "
-                << p->batch_lds_stride << "u);\n"
-                << "        u32* s_flag = (u32*)(s_agg + " << p->groups * (p->storage_ts - p->ngroups) << ");\n"
+                << ldsRegion(p, t, "        ")
                 << p->batch_lds_flush << "    }\n";
         }
     }

@@ -27,6 +27,7 @@
 #include <functional>
 #include <map>
 #include <set>
+#include <regex>
 #include <sstream>
 
 namespace db {
@@ -843,18 +844,20 @@ private:
     }
 
     // fast path: AND chain of (int)col op (int)literal, raw values
-    std::string fastCond(const ExprNode* e) {
+    // (bitwise: 0/1 terms combined with &, no short-circuit branches)
+    std::string fastCond(const ExprNode* e, bool bitwise = false) {
         if (e->getType() == ExprType::OP_AND) {
             const auto* b = static_cast<const BinaryExpr*>(e);
-            const std::string l = fastCond(b->left.get());
-            return l + " && " + fastCond(b->right.get());
+            const std::string l = fastCond(b->left.get(), bitwise);
+            return l + (bitwise ? " & " : " && ") + fastCond(b->right.get(), bitwise);
         }
         const auto* c = static_cast<const BinaryExpr*>(e);
         const auto& col = static_cast<const ColumnRefExpr*>(c->left.get())->column_name;
         if (tableOf(col) != prog_.fact_table)
             unsupported("fast-path filter on dimension column " + col + " above its join");
         const std::string v = "c" + std::to_string(factSlot(col)) + "[i]";
-        return "(" + v + " " + cmpOp(c->op_type) + " " + litInt32(static_cast<const LiteralIntExpr*>(c->right.get())->value) + ")";
+        const std::string t = v + " " + cmpOp(c->op_type) + " " + litInt32(static_cast<const LiteralIntExpr*>(c->right.get())->value);
+        return bitwise ? "((int)(" + t + "))" : "(" + t + ")";
     }
 
     // build-side predicate on raw dimension values
@@ -1482,10 +1485,10 @@ private:
             (fact_only ? fact_f : late_f).push_back(op);
         }
         auto emitFilter = [&](const Op& op) {
-            const std::string c = op.fast ? fastCond(op.pred) : andTerms({vvalid(op.pred), vexpr(op.pred)});
-            body << "        // filter\n#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                 << "            if (!(" << c << ")) fl &= ~(1u << i);\n"
-                 << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+            body << "        // filter\n#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n";
+            if (op.fast) body << "            fl &= ~((u32)((" << fastCond(op.pred, true) << ") ^ 1) << i);\n";
+            else body << "            if (!(" << andTerms({vvalid(op.pred), vexpr(op.pred)}) << ")) fl &= ~(1u << i);\n";
+            body << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
         };
         for (const Op& op : fact_f) emitFilter(op);
         for (int j = 0; j < nj; ++j) {
@@ -1508,7 +1511,7 @@ private:
             if (jp.zmode == 3)
                 body << "            if ((z" << J << " >> i) & 1u) { const int in0 = " << ins << "; keep = 1; pv = in0 ? " << pay
                      << " : 0; }\n";
-            body << "            if (!keep) fl &= ~(1u << i);\n"
+            body << "            fl &= ~((u32)(keep ^ 1) << i);\n"
                  << "            p" << J << "[i] = ((fl >> i) & 1u) ? pv : 0;\n"
                  << "        }\n"
                  << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
@@ -1519,7 +1522,13 @@ private:
         // aggregation: scalars per tile, small group domains in the query's
         // LDS region, the rest with global atomics
         std::ostringstream aggcode, pro, epi;
-        if (!prog_.grouped) {
+        if (!prog_.grouped && opt_.allow_lds_aggregation && prog_.naggs <= 32) {
+            emitScalarAggregateLds(aggcode, pro, epi);
+            prog_.lds_aggregation = true;
+            prog_.batch_lds_stride = 8 * (2 * static_cast<uint64_t>(prog_.naggs) + 1);
+            prog_.batch_lds_init = pro.str();
+            prog_.batch_lds_flush = epi.str();
+        } else if (!prog_.grouped) {
             emitScalarAggregateTile(aggcode);
         } else {
             bool lds = opt_.allow_lds_aggregation && flag_w_ > 0 && prog_.storage_ts <= 64;
@@ -1535,6 +1544,7 @@ private:
                 emitLdsHashAggregate(aggcode, pro, epi);
                 prog_.lds_aggregation = true;
                 prog_.batch_lds_stride = hashAggBytes();
+                prog_.batch_hash_slot_bytes = hashSlotBytes();
                 prog_.batch_lds_init = pro.str();
                 prog_.batch_lds_flush = epi.str();
             } else {
@@ -1543,11 +1553,46 @@ private:
         }
         for (int i = 0; i < prog_.naggs && !prog_.grouped; ++i)
             if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
-        prog_.batch_body = body.str() + aggcode.str();
+        prog_.batch_body = hoistParams(body.str() + aggcode.str());
         if (prog_.keys_kernel) emitKeysKernel(src);
         emitDenseCount(src);
         prog_.source = src.str();
         setAlgorithmicBytes();
+    }
+
+    // The per-query body reads its constants from the query's parameter block
+    // in device memory: load each one once at the top of the body (scalar
+    // loads issued together) instead of at every use.
+    static std::string hoistParams(const std::string& code) {
+        static const std::regex use(R"(P\.(lit|flit|ht)\[(\d+)\]|P\.result_valid\b|P\.result\b)");
+        std::map<std::string, std::string> decl;  // replacement -> declaration
+        std::string out;
+        auto it = std::sregex_iterator(code.begin(), code.end(), use);
+        size_t last = 0;
+        for (; it != std::sregex_iterator(); ++it) {
+            const std::smatch& m = *it;
+            std::string name, d;
+            if (m[1].matched) {
+                const std::string k = m[1].str();
+                name = "cq_" + k + m[2].str();
+                const char* type = k == "flit" ? "const double " : (k == "ht" ? "unsigned int* const " : "const long long ");
+                d = type + name + " = " + m.str() + ";";
+            } else if (m.str() == "P.result_valid") {
+                name = "cq_result_valid";
+                d = "unsigned long long* const cq_result_valid = P.result_valid;";
+            } else {
+                name = "cq_result";
+                d = "unsigned long long* const cq_result = P.result;";
+            }
+            decl[name] = d;
+            out += code.substr(last, static_cast<size_t>(m.position()) - last) + name;
+            last = static_cast<size_t>(m.position() + m.length());
+        }
+        out += code.substr(last);
+        std::string head;
+        for (const auto& kv : decl) head += "        " + kv.second + "\n";
+        if (!head.empty()) head += "        (void)0;\n";
+        return head + out;
     }
 
     // algorithmic bytes of one execution: fact columns + bitmaps + dimension columns
@@ -1655,6 +1700,71 @@ private:
             code << "            }\n";
         }
         code << "        }\n";
+    }
+
+    // scalar aggregates of a batched query: per-wave partials combined in the
+    // query's LDS region (acc[na], cnt[na], seen bits), one global atomic per
+    // aggregate and block at the merge
+    void emitScalarAggregateLds(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
+        const int na = prog_.naggs;
+        std::string mins;
+        for (int i = 0; i < na; ++i)
+            if (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN") mins += (mins.empty() ? "" : " || ") + std::string("k == ") + std::to_string(i);
+        pro << "    for (int k = tid; k < " << 2 * na + 1 << "; k += BT) s_agg[k] = " << (mins.empty() ? "0ull" : "(" + mins + ") ? ~0ull : 0ull")
+            << ";\n";
+        code << "        {\n";
+        for (int i = 0; i < na; ++i) {
+            code << "            u64 acc" << i << " = " << (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN" ? "~0ull" : "0ull")
+                 << "; u32 seen" << i << " = 0u;";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " u64 cnt" << i << " = 0ull;";
+            code << "\n";
+        }
+        // predicated (every expression is total: safe division, wrapping
+        // arithmetic), so the loop has no branches
+        code << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+             << "                const u32 row = (fl >> i) & 1u;\n";
+        for (int i = 0; i < na; ++i) {
+            const AggRow r = aggRow(i);
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            code << "                { const u32 on = row & (u32)(" << r.valid << "); const u64 x = " << r.value << "; ";
+            if (f == "MIN") code << "acc" << i << " = (on && x < acc" << i << ") ? x : acc" << i << ";";
+            else if (f == "MAX") code << "acc" << i << " = (on && x > acc" << i << ") ? x : acc" << i << ";";
+            else code << "acc" << i << " += on ? x : 0ull;";
+            code << " seen" << i << " |= on;";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " cnt" << i << " += (u64)on;";
+            code << " }\n";
+        }
+        code << "            }\n            const int lane = tid & 63;\n";
+        for (int i = 0; i < na; ++i) {
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
+            const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
+            const char* nz = f == "MIN" ? "w != ~0ull" : "w != 0ull";
+            code << "            if (__ballot(seen" << i << " != 0u) != 0ull) {\n"
+                 << "              const u64 w = " << red << "(acc" << i << ");\n"
+                 << "              if (lane == 0 && " << nz << ") " << atom << "(&s_agg[" << i << "], w);\n"
+                 << "              if (lane == 0) atomicOr((u32*)(s_agg + " << 2 * na << "), " << (1u << i) << "u);\n";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
+                code << "              { const u64 c = wave_sum(cnt" << i << "); if (lane == 0 && c) atomicAdd(&s_agg[" << na + i
+                     << "], c); }\n";
+            code << "            }\n";
+        }
+        code << "        }\n";
+        epi << "    if (tid == 0) {\n"
+            << "        const u32 seen = *(const u32*)(s_agg + " << 2 * na << ");\n"
+            << "        (void)seen;\n";
+        for (int i = 0; i < na; ++i) {
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
+            const char* nz = f == "MIN" ? "x != ~0ull" : "x != 0ull";
+            epi << "        { const u64 x = s_agg[" << i << "]; if (" << nz << ") " << atom << "(&P.result[" << i << "], x); }\n";
+            if (f != "COUNT" && prog_.hidden_slot[static_cast<size_t>(i)] < 0)
+                epi << "        if ((seen >> " << i << ") & 1u) chip::set_bit64(P.result_valid, " << i << ");\n";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
+                epi << "        { const u64 c = s_agg[" << na + i << "]; if (c) atomicAdd(&P.result["
+                    << prog_.hidden_slot[static_cast<size_t>(i)] << "], c); }\n";
+        }
+        epi << "    }\n";
     }
 
     // per-row bit mask of storage slots whose validity bit the row sets
@@ -1881,25 +1991,27 @@ private:
     }
 
     // Batched queries whose dense domain does not fit their LDS region: an
-    // open-addressing table of kHashSlots groups per query in LDS (group
-    // index + 1 as key, linear probing up to 8 slots); rows of groups that
-    // find no slot take the global path.  Keys come back from the group
-    // index: written by the merge (sparse output) or by cq_keys.
-    static constexpr int kHashSlots = 128;
+    // open-addressing table per query in LDS (group index + 1 as key, linear
+    // probing up to 8 slots); rows of groups that find no slot take the
+    // global path.  The table has 1 << cq_hbits slots, chosen per launch from
+    // the LDS left over by the other templates (at least kHashMinBits).  Keys
+    // come back from the group index: written by the merge (sparse output)
+    // or by cq_keys.
+    static constexpr int kHashMinBits = 7;
     bool hashAggEligible() {
         if (!prog_.grouped || prog_.storage_ts > 64) return false;
         for (bool d : gdense_)
             if (!d) return false;
         return prog_.fast_sparse || keysDecodable();
     }
-    uint64_t hashAggBytes() const {
+    uint64_t hashSlotBytes() const {
         const uint64_t nas = static_cast<uint64_t>(prog_.storage_ts - prog_.ngroups);
-        return static_cast<uint64_t>(kHashSlots) * (8 + 8 * nas);
+        return 8 + 8 * nas;
     }
+    uint64_t hashAggBytes() const { return (uint64_t(1) << kHashMinBits) * hashSlotBytes(); }
     void emitLdsHashAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
         const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
         const int64_t G = prog_.groups;
-        const int S = kHashSlots;
         prog_.keys_kernel = !prog_.fast_sparse;
         // region: u32 keys[S], u32 flags[S], u64 vals[S * nas]
         std::string init = "0ull";
@@ -1908,23 +2020,23 @@ private:
             for (int s : prog_.min_slots) cond += (cond.empty() ? "" : " || ") + std::string("k % ") + std::to_string(nas) + " == " + std::to_string(s - ng);
             init = "((" + cond + ") ? ~0ull : 0ull)";
         }
-        pro << "    for (int k = tid; k < " << 2 * S << "; k += BT) ((u32*)s_agg)[k] = 0u;\n"
-            << "    for (int k = tid; k < " << S * nas << "; k += BT) s_agg[" << S / 1 << " + k] = " << init << ";\n";
+        pro << "    for (int k = tid; k < (2 << cq_hbits); k += BT) ((u32*)s_agg)[k] = 0u;\n"
+            << "    for (int k = tid; k < (" << nas << " << cq_hbits); k += BT) s_agg[(1 << cq_hbits) + k] = " << init << ";\n";
         code << "        // GROUP BY: LDS hash table of this query, global atomics on overflow\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
              << "            if (!((fl >> i) & 1u)) continue;\n"
              << "            const int h = " << groupHash() << ";\n"
              << "            if ((unsigned)h >= " << G << "u) continue;\n"
              << "            u32* s_hk = (u32*)s_agg;\n"
-             << "            u32* s_hf = s_hk + " << S << ";\n"
-             << "            u64* s_hv = s_agg + " << S << ";\n"
-             << "            u32 slot = ((u32)h * 2654435761u) >> " << (32 - 7) << ";\n"
+             << "            u32* s_hf = s_hk + (1u << cq_hbits);\n"
+             << "            u64* s_hv = s_agg + (1u << cq_hbits);\n"
+             << "            u32 slot = ((u32)h * 2654435761u) >> (32u - cq_hbits);\n"
              << "            int found = -1;\n"
              << "            for (int t = 0; t < 8; ++t) {\n"
              << "                u32 cur = s_hk[slot];\n"
              << "                if (cur == 0u) { const u32 old = atomicCAS(&s_hk[slot], 0u, (u32)h + 1u); cur = old == 0u ? (u32)h + 1u : old; }\n"
              << "                if (cur == (u32)h + 1u) { found = (int)slot; break; }\n"
-             << "                slot = (slot + 1u) & " << (S - 1) << "u;\n"
+             << "                slot = (slot + 1u) & ((1u << cq_hbits) - 1u);\n"
              << "            }\n"
              << "            if (found >= 0) {\n"
              << "                u32 want = 1u;\n";
@@ -1955,12 +2067,12 @@ private:
         code << "            }\n"
              << "        }\n";
         // merge: aggregates (and sparse keys) of every used slot
-        epi << "    for (int slot = tid; slot < " << S << "; slot += BT) {\n"
+        epi << "    for (int slot = tid; slot < (1 << cq_hbits); slot += BT) {\n"
             << "        const u32* s_hk = (const u32*)s_agg;\n"
             << "        const u32 key = s_hk[slot];\n"
             << "        if (key == 0u) continue;\n"
             << "        const long long g = (long long)key - 1;\n"
-            << "        const u32 f = s_hk[" << S << " + slot];\n"
+            << "        const u32 f = s_hk[(1 << cq_hbits) + slot];\n"
             << "        const long long base = g * " << sts << ";\n";
         if (prog_.fast_sparse) {
             epi << "        long long rem = g;\n";
@@ -1979,7 +2091,7 @@ private:
             std::string f = "SUM";
             for (int i = 0; i < prog_.naggs; ++i)
                 if (ng + i == s2) f = prog_.agg_funcs[static_cast<size_t>(i)];
-            const std::string v = "s_agg[" + std::to_string(S) + " + slot * " + std::to_string(nas) + " + " + std::to_string(s2 - ng) + "]";
+            const std::string v = "s_agg[(1 << cq_hbits) + slot * " + std::to_string(nas) + " + " + std::to_string(s2 - ng) + "]";
             if (f == "MIN") epi << "        { const u64 x = " << v << "; if (x != ~0ull) atomicMin(&P.result[base + " << s2 << "], x); }\n";
             else if (f == "MAX") epi << "        { const u64 x = " << v << "; if (x) atomicMax(&P.result[base + " << s2 << "], x); }\n";
             else epi << "        { const u64 x = " << v << "; if (x) atomicAdd(&P.result[base + " << s2 << "], x); }\n";

@@ -131,8 +131,13 @@ struct QueryProgram {
     // per-query LDS aggregation (init / merge code, bytes per query)
     bool shared_tables = false;
     std::string batch_shared;
+    // (scope of the batch kernel: cq_hbits = log2 slots of the query's LDS
+    // hash table).  batch_hash_slot_bytes > 0 marks a hash region whose size
+    // the launch chooses (stride = slot bytes << hbits, at least
+    // batch_lds_stride).
     std::string batch_lds_init, batch_lds_flush;
     uint64_t batch_lds_stride = 0;
+    uint64_t batch_hash_slot_bytes = 0;
     bool keys_kernel = false;             // cq_keys writes group keys after the scan
     // interpreted tier (CodegenOptions::vm): the program and the dynamic LDS
     // of its group aggregates (0: global atomics)

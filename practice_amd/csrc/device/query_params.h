@@ -54,8 +54,12 @@ struct cq_batch_params {
     int tq_begin[CQ_MAX_TEMPLATES];
     int tq_end[CQ_MAX_TEMPLATES];
     // byte offset of each template's per-query LDS aggregation regions in
-    // the dynamic shared memory of the launch
+    // the dynamic shared memory of the launch, the bytes of one query's
+    // region, and log2 of the slots of a per-query LDS hash table (templates
+    // whose region size is chosen at launch)
     unsigned int lds_off[CQ_MAX_TEMPLATES];
+    unsigned int lds_stride[CQ_MAX_TEMPLATES];
+    unsigned int hbits[CQ_MAX_TEMPLATES];
     long long row_begin;
     long long row_end;
     long long num_tiles;

@@ -12,7 +12,8 @@
 namespace db {
 
 namespace {
-constexpr unsigned kBatchLdsLimit = 64 * 1024;  // dynamic LDS of one shared-scan block
+constexpr int kHashMinBits = 7, kHashMaxBits = 10;  // per-query LDS hash tables: 128..1024 slots
+unsigned ldsLimit(const BatchOptions& opt) { return static_cast<unsigned>(opt.lds_kb) * 1024u; }
 }  // namespace
 
 BatchOptions parseBatchOptions(const std::string& text) {
@@ -33,11 +34,13 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "max_queries") o.max_queries = static_cast<int>(v);
         else if (k == "shared") o.shared_tables = v != 0;
         else if (k == "lds_agg") o.lds_aggregation = v != 0;
+        else if (k == "lds_kb") o.lds_kb = static_cast<int>(v);
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
         throw std::runtime_error("batch block size must be a multiple of 64 in [64, 1024]");
     if (o.vectors_per_thread < 1 || o.vectors_per_thread > 4) throw std::runtime_error("batch vpt must be in [1, 4]");
+    if (o.lds_kb < 16 || o.lds_kb > 160) throw std::runtime_error("batch lds_kb must be in [16, 160]");
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
     o.max_queries = std::max(1, o.max_queries);
     return o;
@@ -159,7 +162,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         uint64_t lds = 0;
         for (size_t i = 0; i < programs.size(); ++i) {
             const uint64_t need = programs[i].batch_lds_stride;
-            if (!cur.empty() && lds + need > kBatchLdsLimit) {
+            if (!cur.empty() && lds + need > ldsLimit(opt)) {
                 scans.push_back(cur);
                 cur.clear();
                 lds = 0;
@@ -181,7 +184,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         // queries of one template per launch: bounded by max_queries and by
         // their LDS aggregation regions
         size_t chunk = static_cast<size_t>(opt.max_queries);
-        if (t.batch_lds_stride > 0) chunk = std::min<size_t>(chunk, std::max<uint64_t>(1, kBatchLdsLimit / t.batch_lds_stride));
+        if (t.batch_lds_stride > 0) chunk = std::min<size_t>(chunk, std::max<uint64_t>(1, ldsLimit(opt) / t.batch_lds_stride));
         for (size_t start = 0; start < qs.size(); start += chunk) {
             const size_t stop = std::min(qs.size(), start + chunk);
             const uint64_t lds_need = t.batch_lds_stride * (stop - start);
@@ -191,7 +194,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
             const bool fits = !cur.empty() && cur_table == t.fact_table && cur_templates + 1 <= opt.max_templates &&
                               cols.size() <= CQ_MAX_BATCH_COLS &&
                               cur.size() + (stop - start) <= static_cast<size_t>(opt.max_queries) &&
-                              cur_lds + lds_need <= kBatchLdsLimit;
+                              cur_lds + lds_need <= ldsLimit(opt);
             if (!fits && !cur.empty()) {
                 scans.push_back(cur);
                 cur.clear();
@@ -237,6 +240,42 @@ std::string batchSource(const std::vector<const QueryProgram*>& templates, int b
 }  // namespace
 
 namespace {
+
+// Per-query LDS regions of one launch: offset and stride per template, and
+// for hash-aggregating templates the table size, grown one bit at a time for
+// all of them while the launch stays within the limit.  Returns the bytes.
+unsigned layoutLds(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
+                   unsigned limit, cq_batch_params& B) {
+    const size_t nt = templates.size();
+    std::vector<int> bits(nt, kHashMinBits);
+    auto stride = [&](size_t t) -> uint64_t {
+        const QueryProgram* p = templates[t];
+        return p->batch_hash_slot_bytes > 0 ? p->batch_hash_slot_bytes << bits[t] : p->batch_lds_stride;
+    };
+    auto total = [&]() {
+        uint64_t b = 0;
+        for (size_t t = 0; t < nt; ++t) b += stride(t) * per_template[t].size();
+        return b;
+    };
+    if (total() > limit) throw std::runtime_error("batch LDS aggregation regions exceed the limit");
+    for (int b = kHashMinBits + 1; b <= kHashMaxBits; ++b) {
+        for (size_t t = 0; t < nt; ++t) {
+            // only tables that reached b - 1 grow again
+            if (templates[t]->batch_hash_slot_bytes == 0 || per_template[t].empty() || bits[t] != b - 1) continue;
+            bits[t] = b;
+            if (total() > limit) bits[t] = b - 1;
+        }
+    }
+    unsigned off = 0;
+    for (size_t t = 0; t < nt; ++t) {
+        B.lds_off[t] = off;
+        B.lds_stride[t] = static_cast<unsigned>(stride(t));
+        B.hbits[t] = static_cast<unsigned>(bits[t]);
+        off += static_cast<unsigned>(stride(t) * per_template[t].size());
+    }
+    if (off > limit) throw std::logic_error("batch LDS layout exceeds its limit");
+    return off;
+}
 
 // persistent grid for a launch with dynamic shared memory
 unsigned scanGridShared(hipFunction_t f, int block_threads, int device, long long num_tiles, unsigned lds_bytes) {
@@ -380,7 +419,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     auto tit = tuned_.find(set_key);
     if (tit != tuned_.end()) cfg = tit->second;
     else if (opt.autotune) {
-        cfg = autotune(templates, per_template, runs, dqp, ctx);
+        cfg = autotune(templates, per_template, runs, dqp, ldsLimit(opt), ctx);
         tuned_[set_key] = cfg;
     }
     std::vector<std::string> ucols;
@@ -410,21 +449,28 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     B.row_end = first.fact_rows;
     const long long tile = static_cast<long long>(cfg.block_threads) * 4 * cfg.vectors_per_thread;
     B.num_tiles = (B.row_end - B.row_begin + tile - 1) / tile;
-    unsigned lds_bytes = 0;
-    for (size_t t = 0; t < templates.size(); ++t) {
-        B.lds_off[t] = lds_bytes;
-        lds_bytes += static_cast<unsigned>(templates[t]->batch_lds_stride * per_template[t].size());
-    }
-    if (lds_bytes > kBatchLdsLimit) throw std::runtime_error("batch LDS aggregation regions exceed the limit");
+    const unsigned lds_bytes = layoutLds(templates, per_template, ldsLimit(opt), B);
     const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes);
 
     auto enqueue = [&](bool timed) {
         prepareTables(templates, per_template, runs, dqp, ctx);
         if (timed) hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
         void* args[] = {&B};
-        if (B.num_tiles > 0)
-            hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(cfg.block_threads), 1, 1, lds_bytes, s, args, nullptr),
-                     "launch batch scan");
+        if (B.num_tiles > 0) {
+            const hipError_t e =
+                hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(cfg.block_threads), 1, 1, lds_bytes, s, args, nullptr);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();  // a failed launch must not fail the next kernel launch
+                int max_threads = 0, regs = 0, max_dyn = 0;
+                (void)hipFuncGetAttribute(&max_threads, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, fn);
+                (void)hipFuncGetAttribute(&regs, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn);
+                (void)hipFuncGetAttribute(&max_dyn, HIP_FUNC_ATTRIBUTE_MAX_DYNAMIC_SHARED_SIZE_BYTES, fn);
+                throw std::runtime_error("launch batch scan (block " + std::to_string(cfg.block_threads) + ", grid " + std::to_string(grid) +
+                                         ", LDS " + std::to_string(lds_bytes) + " B; kernel: max block " + std::to_string(max_threads) +
+                                         ", registers " + std::to_string(regs) + ", max dynamic LDS " + std::to_string(max_dyn) +
+                                         "): " + hipGetErrorString(e));
+            }
+        }
         if (timed) hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
         for (int k = 0; k < n; ++k) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
     };
@@ -528,7 +574,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
 
 BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgram*>& templates,
                                              const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
-                                             const cq_query_params* dqp, ExecutionContext* ctx) {
+                                             const cq_query_params* dqp, unsigned lds_limit, ExecutionContext* ctx) {
     QueryEngine& engine = db_.engine();
     hipStream_t s = ctx->stream;
     Tuned best;
@@ -560,11 +606,7 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
             const long long tile = static_cast<long long>(bt) * 4 * vpt;
             B.num_tiles = (B.row_end + tile - 1) / tile;
             if (B.num_tiles == 0) return best;
-            unsigned lds_bytes = 0;
-            for (size_t t = 0; t < templates.size(); ++t) {
-                B.lds_off[t] = lds_bytes;
-                lds_bytes += static_cast<unsigned>(templates[t]->batch_lds_stride * per_template[t].size());
-            }
+            const unsigned lds_bytes = layoutLds(templates, per_template, lds_limit, B);
             const unsigned grid = scanGridShared(fn, bt, ctx->device, B.num_tiles, lds_bytes);
             float ms = 0;
             for (int rep = 0; rep < 2; ++rep) {  // first run warms the module and caches

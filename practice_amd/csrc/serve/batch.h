@@ -33,9 +33,11 @@ struct BatchOptions {
     int max_queries = 256;        // queries per shared scan
     bool shared_tables = true;    // one probe per template and dimension (packed attributes)
     bool lds_aggregation = true;  // per-query LDS aggregation for small group domains
+    int lds_kb = 64;              // dynamic LDS of one shared-scan block (KiB, 16..160)
 };
 
-// "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates, max_queries)
+// "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
+// max_queries, shared, lds_agg, lds_kb)
 BatchOptions parseBatchOptions(const std::string& text);
 
 struct BatchScanReport {
@@ -109,7 +111,7 @@ private:
     void prepareTables(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
                        std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
     Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
-                   std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
+                   std::vector<QueryRun>& runs, const cq_query_params* dqp, unsigned lds_limit, ExecutionContext* ctx);
     void dropGraph(Graph& g);
 };
 

@@ -83,7 +83,8 @@ def test_launch_shapes_compile(hdb, opts):
     pa.compile_source(hdb.generate_batch_code([Q.C3, Q.C1], opts))
 
 
-@pytest.mark.parametrize("opts,msg", [({"bt": 100}, "multiple of 64"), ({"vpt": 8}, "vpt"), ({"bogus": 1}, "unknown")])
+@pytest.mark.parametrize("opts,msg", [({"bt": 100}, "multiple of 64"), ({"vpt": 8}, "vpt"), ({"bogus": 1}, "unknown"),
+                                      ({"lds_kb": 200}, "lds_kb")])
 def test_bad_options_rejected(hdb, opts, msg):
     with pytest.raises(pa.CrystalError, match=msg):
         hdb.generate_batch_code([Q.C1], opts)
@@ -108,6 +109,21 @@ def test_batch_equals_alone_and_oracle(graph):
     res = gdb.batch_execute(qs, {"graph": graph})
     rep = gdb.batch_report()
     assert rep["queries"] == len(qs) and rep["shared_scans"] >= 2
+    _check_batch(gdb, odb, qs, res)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lds_kb", [16, 160])
+def test_lds_region_sizes(lds_kb):
+    """16 KiB: hash tables stay at 128 slots and C2-shaped queries (up to 280
+    groups) overflow to global atomics; 160 KiB: tables grow to 1024 slots."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(11)
+    qs = [Q.template_query(k, rng) for k in (3, 3, 3, 3, 3, 3, 4, 4, 4, 1, 1, 2, 2)]
+    print(f"[batch] lds_kb={lds_kb}", flush=True)
+    res = gdb.batch_execute(qs, {"graph": 0, "lds_kb": lds_kb})
     _check_batch(gdb, odb, qs, res)
 
 
