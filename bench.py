@@ -47,8 +47,9 @@ def parse_args():
     p.add_argument("--no-batched", action="store_true", help="skip the batched shared-scan measurement")
     p.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     p.add_argument("--batch-queries", type=int, default=64)
-    p.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
-    p.add_argument("--serve-seconds", type=float, default=5.0)
+    p.add_argument("--serve-rates", default="100,500,2000", help="Poisson arrival rates (req/s), comma separated")
+    p.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each arrival rate")
+    p.add_argument("--serve-burst", type=int, default=1024, help="requests of the closed-loop burst")
     p.add_argument("--serve-adhoc", type=float, default=0.05, help="fraction of requests with a new plan shape")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
@@ -136,6 +137,8 @@ def bench_batched(pa, Q, args):
     batched_s = (time.perf_counter() - t0) / args.steps
     rep = db.batch_report()
     exact = all(same_rows(a, b) for a, b in zip(out, serial))
+    # bytes of one shared pass: every union column of every launch, read once
+    union_bytes = sum(4 * 120_000_000 * sc["union_columns"] for sc in rep["scans"])
     db.close()
     return {"workload": f"{n} queries, 4 templates (1-4 dimension joins), random constants, scale 20 (120,000,000 fact rows)",
             "qps": round(n / batched_s, 1), "serial_qps": round(n / serial_s, 1),
@@ -144,20 +147,24 @@ def bench_batched(pa, Q, args):
             "block_threads": tuned["block_threads"], "vectors_per_thread": tuned["vectors_per_thread"],
             "graph": rep["scans"][0]["graph"], "scan_ms_last": rep["scans"][0]["scan_ms"],
             "device_ms_last": rep["scans"][0]["device_ms"], "host_ms_last": rep["scans"][0]["host_ms"],
-            "plan_ms_last": rep["plan_ms"], "bit_exact_vs_serial": exact}
+            "plan_ms_last": rep["plan_ms"], "union_GBps": round(union_bytes / batched_s / 1e9, 1),
+            "bit_exact_vs_serial": exact}
 
 
 def bench_served(pa, Q, args):
     """Open-loop Poisson arrivals against the continuous-batching server over
-    resident scale-100 columns (600,000,000 fact rows); a fraction of the
-    requests have a plan shape never seen before (hipRTC compilation)."""
+    resident scale-100 columns (600,000,000 fact rows) at each rate of
+    --serve-rates; a fraction of the requests have a plan shape never seen
+    before (run at once on the interpreted tier, compiled in the background).
+    Then a closed-loop burst (every request submitted at once) gives the
+    saturated throughput.  Serial: the same templates one at a time through
+    the plain path."""
     import random
     import threading
     db = pa.Database(0)
     db.generate_ssb(sf=100, fact_rows=600_000_000, seed=args.seed, nullable=WORKLOADS["c3"][2])
     db.finalize()
     rng = random.Random(args.seed)
-    # serial plain reference on the same templates
     warm = [Q.template_query(k, rng) for k in (1, 2, 3, 4)]
     for q in warm:
         db.execute(q)
@@ -169,43 +176,65 @@ def bench_served(pa, Q, args):
     srv = pa.Server(db, {"lanes": 2, "max_batch": 64})
     for t in [srv.submit(q) for q in warm]:
         srv.wait(t)
+
+    def draw(n):
+        return [Q.adhoc_query(rng) if rng.random() < args.serve_adhoc else Q.template_query(rng.randint(1, 4), rng)
+                for _ in range(n)]
+
+    def drain(tickets):
+        errors = 0
+        for t in tickets:
+            try:
+                srv.wait(t, 600_000)
+            except pa.CrystalError:
+                errors += 1
+        return errors
+
+    rates = []
+    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+        srv.reset_stats()
+        reqs = draw(max(1, int(rate * args.serve_seconds)))
+        tickets = []
+
+        def producer():
+            t = time.perf_counter()
+            for q in reqs:
+                t += rng.expovariate(rate)
+                d = t - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                tickets.append(srv.submit(q))
+
+        th = threading.Thread(target=producer)
+        t0 = time.perf_counter()
+        th.start()
+        th.join()
+        errors = drain(tickets)
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+        rates.append({"rate": rate, "requests": len(reqs), "qps": round(st["completed"] / wall, 1),
+                      "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2),
+                      "mean_batch": round(st["mean_batch"], 1), "interpreted": st["interpreted_queries"],
+                      "promotions": st["promotions"], "errors": errors})
+        print(f"[served] rate {rate:g}: {rates[-1]}", file=sys.stderr, flush=True)
+    # closed loop: every request queued at once
     srv.reset_stats()
-    n = int(args.serve_rate * args.serve_seconds)
-    reqs = []
-    for _ in range(n):
-        reqs.append(Q.adhoc_query(rng) if rng.random() < args.serve_adhoc else Q.template_query(rng.randint(1, 4), rng))
-    tickets = []
-
-    def producer():
-        t = time.perf_counter()
-        for q in reqs:
-            t += rng.expovariate(args.serve_rate)
-            d = t - time.perf_counter()
-            if d > 0:
-                time.sleep(d)
-            tickets.append(srv.submit(q))
-
-    th = threading.Thread(target=producer)
+    reqs = draw(args.serve_burst)
     t0 = time.perf_counter()
-    th.start()
-    th.join()
-    errors = 0
-    for t in tickets:
-        try:
-            srv.wait(t, 600_000)
-        except pa.CrystalError:
-            errors += 1
+    tickets = [srv.submit(q) for q in reqs]
+    errors = drain(tickets)
     wall = time.perf_counter() - t0
     st = srv.stats()
+    burst = {"requests": len(reqs), "qps": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
+             "p99_ms": round(st["p99_ms"], 2), "mean_batch": round(st["mean_batch"], 1), "errors": errors}
+    print(f"[served] burst: {burst}", file=sys.stderr, flush=True)
     srv.stop()
     db.close()
-    return {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, 4 templates + "
-                        f"{args.serve_adhoc:.0%} new plan shapes, scale 100 (600,000,000 fact rows)",
-            "requests": n, "qps": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
-            "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2), "serial_qps": round(serial_qps, 1),
-            "speedup": round(st["completed"] / wall / serial_qps, 2), "mean_batch": round(st["mean_batch"], 1),
-            "compiles": st["compiles"], "plain_queries": st["plain_queries"], "batches": st["batches"],
-            "promotions": st["promotions"], "errors": errors}
+    return {"workload": f"Poisson arrivals at {args.serve_rates} req/s for {args.serve_seconds:g} s each, then a burst of "
+                        f"{args.serve_burst}; 4 templates + {args.serve_adhoc:.0%} new plan shapes, scale 100 "
+                        f"(600,000,000 fact rows)",
+            "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
+            "saturated_speedup": round(burst["qps"] / serial_qps, 2)}
 
 
 def copy_calibration(nbytes, steps):

@@ -133,10 +133,13 @@ void cq_result_free(cq_result* r);
  * replayed for batches of the same shape.  No reference counterpart: the
  * reference runs QueryEngine::executeQuery per query
  * (src/core/execution.cpp:652-725).
- * options: NULL or "key=value,..." with bt (64..1024, multiple of 64),
- * vpt (1..4 128-bit loads per column per lane), graph (0/1), autotune (0/1:
- * time every bt x vpt once per signature set and keep the fastest),
- * max_templates (<= 16), max_queries (per scan). */
+ * options: NULL or "key=value,..." with bt (64..1024, multiple of 64,
+ * default 1024), vpt (1..4 128-bit loads per column per lane, default 2),
+ * graph (0/1), autotune (0/1: time every bt x vpt once per signature set and
+ * keep the fastest), max_templates (<= 16), max_queries (per scan), shared
+ * (0/1: one probe per template and dimension), lds_agg (0/1: per-query LDS
+ * aggregation), lds_kb (16..160, default 160: LDS per scan block for the
+ * per-query aggregation regions). */
 int cq_batch_execute(cq_db* db, const char* const* sqls, int n, const char* options, cq_result** outs);
 /* JSON of the last batch (per shared scan: queries, templates, columns,
  * launch shape, graph replay, scan / device milliseconds, fact bytes). */
@@ -153,8 +156,11 @@ int cq_db_generate_batch_code(cq_db* db, const char* const* sqls, int n, const c
  * admission reserves each request's device footprint against the budget;
  * cache misses compile on a background thread.  options: NULL or
  * "key=value,..." with lanes (1..8, default 2), max_batch (64),
- * max_templates (16), budget_mb (default: a quarter of free memory),
- * window_us (0), async_compile (1), bt, vpt, graph. */
+ * max_templates (16), promote_after (2: requests of a new signature before
+ * it is compiled into the shared scan), compile_threads (2), budget_mb
+ * (default: a quarter of free memory), window_us (0), async_compile (1),
+ * interpret (1: new signatures run at once on the interpreted tier), bt,
+ * vpt, graph, lds_kb (as for cq_batch_execute). */
 int cq_server_start(cq_db* db, const char* options, cq_server** out);
 /* Enqueue a request; returns a ticket >= 0 (negative on error). */
 long long cq_server_submit(cq_server* s, const char* sql);

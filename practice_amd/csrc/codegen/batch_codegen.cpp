@@ -111,8 +111,11 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
             << "            const cq_query_params& P = B.qp[q];\n";
         if (p->batch_lds_stride > 0) src << ldsRegion(p, t, "            ");
-        src << "            u32 fl = tile_fl;\n"
-            << "            do {\n"
+        if (p->batch_bool_flags)
+            src << "            bool fl[IPT];\n#pragma unroll\n            for (int i = 0; i < IPT; ++i) fl[i] = (tile_fl >> i) & 1u;\n";
+        else
+            src << "            u32 fl = tile_fl;\n";
+        src << "            do {\n"
             << p->batch_body
             << "            } while (0);\n"
             << "        }\n"

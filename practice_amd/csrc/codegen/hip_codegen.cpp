@@ -843,6 +843,18 @@ private:
         }
     }
 
+    // Row flags of the items of a lane: a u32 bit mask `fl` (plain kernels)
+    // or, in shared batched templates, `bool fl[IPT]` (wave-wide lane masks
+    // in scalar registers, so flag logic runs on the scalar unit)
+    bool bool_flags_ = false;
+    std::string flGet() const { return bool_flags_ ? "(u32)fl[i]" : "((fl >> i) & 1u)"; }
+    std::string flClear() const { return bool_flags_ ? "fl[i] = false;" : "fl &= ~(1u << i);"; }
+    std::string flAny() const { return bool_flags_ ? "chip::any_of(fl)" : "fl != 0u"; }
+    // keep the row only if the 0/1 int expression `c` is 1
+    std::string flKeep(const std::string& c) const {
+        return bool_flags_ ? "fl[i] = fl[i] & ((" + c + ") != 0);" : "fl &= ~((u32)((" + c + ") ^ 1) << i);";
+    }
+
     // fast path: AND chain of (int)col op (int)literal, raw values
     // (bitwise: 0/1 terms combined with &, no short-circuit branches)
     std::string fastCond(const ExprNode* e, bool bitwise = false) {
@@ -1069,14 +1081,14 @@ private:
                     const std::string c = fastCond(op.pred);
                     body << "        // filter (fast path, raw values)\n"
                          << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                         << "            if (!(" << c << ")) fl &= ~(1u << i);\n";
+                         << "            if (!(" << c << ")) " << flClear() << "\n";
                 } else {
                     const std::string c = andTerms({vvalid(op.pred), vexpr(op.pred)});
                     body << "        // filter (three-valued logic)\n"
                          << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                         << "            if (!(" << c << ")) fl &= ~(1u << i);\n";
+                         << "            if (!(" << c << ")) " << flClear() << "\n";
                 }
-                body << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+                body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
             } else {
                 const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
                 const int fs = factSlot(jp.fk_col);
@@ -1086,23 +1098,23 @@ private:
                 body << "        int p" << j << "[IPT];\n";
                 body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
                      << "            p" << j << "[i] = 0;\n"
-                     << "            if (!((fl >> i) & 1u)) continue;\n";
+                     << "            if (!" << flGet() << ") continue;\n";
                 if (prog_.fact_nullable[static_cast<size_t>(fs)])
-                    body << "            if (!((v" << fs << " >> i) & 1u)) { fl &= ~(1u << i); continue; }\n";
+                    body << "            if (!((v" << fs << " >> i) & 1u)) { " << flClear() << " continue; }\n";
                 if (jp.dense)
                     body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ">("
                          << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
                          << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p"
-                         << j << "[i])) fl &= ~(1u << i);\n";
+                         << j << "[i])) " << flClear() << "\n";
                 else if (jp.compact)
                     body << "            if (!chip::ht_probe_compact(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
                          << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j
-                         << "[i])) fl &= ~(1u << i);\n";
+                         << "[i])) " << flClear() << "\n";
                 else
                     body << "            if (!chip::ht_probe_ref((const int*)" << table << ", c" << fs << "[i], " << jp.kmin << ", "
-                         << jp.len << ", " << (jp.pair ? "true" : "false") << ", p" << j << "[i])) fl &= ~(1u << i);\n";
+                         << jp.len << ", " << (jp.pair ? "true" : "false") << ", p" << j << "[i])) " << flClear() << "\n";
                 body << "        }\n";
-                body << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+                body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
                 visible_mask_ |= 1u << op.join;
             }
         }
@@ -1400,6 +1412,8 @@ private:
 
     void emitBatchShared(std::ostringstream& src) {
         prog_.shared_tables = true;
+        prog_.batch_bool_flags = true;
+        bool_flags_ = true;
         const int nj = static_cast<int>(prog_.joins.size());
         for (int j = 0; j < nj; ++j) {
             JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
@@ -1445,10 +1459,12 @@ private:
             const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
             const int fs = factSlot(jp.fk_col);
             const std::string J = std::to_string(j);
+            // e: entry, m: the key is in the table, z: probe key 0 (the
+            // reference's empty-slot corner); m and z are lane masks
             sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
-               << "        u32 e" << J << "[IPT]; u32 m" << J << " = 0u, z" << J << " = 0u;\n"
+               << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n"
                << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-               << "            e" << J << "[i] = 0u;\n"
+               << "            e" << J << "[i] = 0u; m" << J << "[i] = false; z" << J << "[i] = false;\n"
                << "            if (!((tile_fl >> i) & 1u)) continue;\n";
             if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
             sh << "            const int fk = c" << fs << "[i];\n";
@@ -1459,13 +1475,13 @@ private:
                 std::string ze = "0u";
                 if (jp.zmode == 2) ze = entry(std::to_string(jp.zx) + "u");
                 if (jp.zmode == 3) ze = entry("(u32)(0 - " + std::to_string(jp.dmin) + ")");
-                sh << "            if (fk == 0) { z" << J << " |= 1u << i; e" << J << "[i] = " << ze << "; continue; }\n";
+                sh << "            if (fk == 0) { z" << J << "[i] = true; e" << J << "[i] = " << ze << "; continue; }\n";
             }
             sh << "            const u32 x = (u32)fk - (u32)" << jp.dmin << ";\n"
                << "            if (x >= " << jp.dspan << "u) continue;\n"
                << "            const u32 e = " << entry("x") << ";\n";
             if (jp.s_present) sh << "            if (!(e & 1u)) continue;\n";
-            sh << "            e" << J << "[i] = e; m" << J << " |= 1u << i;\n"
+            sh << "            e" << J << "[i] = e; m" << J << "[i] = true;\n"
                << "        }\n";
         }
         prog_.batch_shared = sh.str();
@@ -1486,9 +1502,9 @@ private:
         }
         auto emitFilter = [&](const Op& op) {
             body << "        // filter\n#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n";
-            if (op.fast) body << "            fl &= ~((u32)((" << fastCond(op.pred, true) << ") ^ 1) << i);\n";
-            else body << "            if (!(" << andTerms({vvalid(op.pred), vexpr(op.pred)}) << ")) fl &= ~(1u << i);\n";
-            body << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+            if (op.fast) body << "            " << flKeep(fastCond(op.pred, true)) << "\n";
+            else body << "            if (!(" << andTerms({vvalid(op.pred), vexpr(op.pred)}) << ")) " << flClear() << "\n";
+            body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
         };
         for (const Op& op : fact_f) emitFilter(op);
         for (int j = 0; j < nj; ++j) {
@@ -1501,20 +1517,26 @@ private:
                 bp ? buildCondWith(bp, [&](const std::string& col) { return sharedAttrOf(jp, col, "e"); }, true) : "1";
             const std::string ins = jp.s_present ? "((int)(e & 1u) & " + cond + ")" : cond;
             const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
+            // The payload register of a row only matters while the row is
+            // alive, so it is the entry's payload (0 for probe key 0 unless
+            // the empty-slot match takes the build row's): independent of
+            // the query except in zmode 3, hoisted out of the query loop.
             body << "        // join " << jp.dim_table << ": shared probe, this query's build filter\n"
                  << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
                  << "            const u32 e = e" << J << "[i];\n"
-                 << "            int keep = (int)((m" << J << " >> i) & 1u) & " << cond << ";\n"
-                 << "            int pv = keep ? " << pay << " : 0;\n";
-            if (jp.zmode == 1) body << "            if ((z" << J << " >> i) & 1u) { keep = 1; pv = 0; }\n";
-            if (jp.zmode == 2) body << "            if ((z" << J << " >> i) & 1u) { keep = " << ins << " ^ 1; pv = 0; }\n";
+                 << "            bool keep = m" << J << "[i] & ((" << cond << ") != 0);\n";
+            if (jp.zmode == 1) body << "            keep = keep | z" << J << "[i];\n";
+            if (jp.zmode == 2) body << "            keep = z" << J << "[i] ? ((" << ins << ") == 0) : keep;\n";
+            if (jp.zmode == 3) body << "            keep = keep | z" << J << "[i];\n";
+            body << "            fl[i] = fl[i] & keep;\n";
             if (jp.zmode == 3)
-                body << "            if ((z" << J << " >> i) & 1u) { const int in0 = " << ins << "; keep = 1; pv = in0 ? " << pay
-                     << " : 0; }\n";
-            body << "            fl &= ~((u32)(keep ^ 1) << i);\n"
-                 << "            p" << J << "[i] = ((fl >> i) & 1u) ? pv : 0;\n"
-                 << "        }\n"
-                 << "        if (__ballot(fl != 0u) == 0ull) continue;\n";
+                body << "            p" << J << "[i] = z" << J << "[i] ? ((" << ins << ") ? " << pay << " : 0) : " << pay << ";\n";
+            else if (jp.zmode != 0)
+                body << "            p" << J << "[i] = z" << J << "[i] ? 0 : " << pay << ";\n";
+            else
+                body << "            p" << J << "[i] = " << pay << ";\n";
+            body << "        }\n"
+                 << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
             visible_mask_ |= 1u << j;
         }
         for (const Op& op : late_f) emitFilter(op);
@@ -1525,7 +1547,7 @@ private:
         if (!prog_.grouped && opt_.allow_lds_aggregation && prog_.naggs <= 32) {
             emitScalarAggregateLds(aggcode, pro, epi);
             prog_.lds_aggregation = true;
-            prog_.batch_lds_stride = 8 * (2 * static_cast<uint64_t>(prog_.naggs) + 1);
+            prog_.batch_lds_stride = 8 * scalarLdsWords();
             prog_.batch_lds_init = pro.str();
             prog_.batch_lds_flush = epi.str();
         } else if (!prog_.grouped) {
@@ -1554,6 +1576,7 @@ private:
         for (int i = 0; i < prog_.naggs && !prog_.grouped; ++i)
             if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
         prog_.batch_body = hoistParams(body.str() + aggcode.str());
+        bool_flags_ = false;
         if (prog_.keys_kernel) emitKeysKernel(src);
         emitDenseCount(src);
         prog_.source = src.str();
@@ -1628,7 +1651,7 @@ private:
             if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) pro << "    u64 cnt" << i << " = 0ull;\n";
         }
         code << "        // scalar aggregates\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!((fl >> i) & 1u)) continue;\n";
+             << "            if (!" << flGet() << ") continue;\n";
         for (int i = 0; i < na; ++i) {
             const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
@@ -1672,7 +1695,7 @@ private:
             code << "\n";
         }
         code << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
-             << "                if (!((fl >> i) & 1u)) continue;\n";
+             << "                if (!" << flGet() << ") continue;\n";
         for (int i = 0; i < na; ++i) {
             const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
@@ -1702,27 +1725,40 @@ private:
         code << "        }\n";
     }
 
-    // scalar aggregates of a batched query: per-wave partials combined in the
-    // query's LDS region (acc[na], cnt[na], seen bits), one global atomic per
-    // aggregate and block at the merge
+    // scalar aggregates of a batched query: per-lane partials in the query's
+    // LDS region (one u64 per aggregate and wave lane, plus the AVG counts
+    // and a word of "row seen" bits), added with one LDS atomic per wave and
+    // tile (no cross-lane reduction per tile); the merge reduces the 64 lane
+    // slots and issues one global atomic per aggregate and block
+    uint64_t scalarLdsWords() const {
+        int nh = 0;
+        for (int i = 0; i < prog_.naggs; ++i) nh += prog_.hidden_slot[static_cast<size_t>(i)] >= 0;
+        return 64 * static_cast<uint64_t>(prog_.naggs + nh) + 1;
+    }
     void emitScalarAggregateLds(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
         const int na = prog_.naggs;
+        std::vector<int> cnt_base(static_cast<size_t>(na), -1);
+        int nh = 0;
+        for (int i = 0; i < na; ++i)
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) cnt_base[static_cast<size_t>(i)] = 64 * (na + nh++);
+        const uint64_t words = scalarLdsWords();
+        const uint64_t seen_word = words - 1;
         std::string mins;
         for (int i = 0; i < na; ++i)
-            if (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN") mins += (mins.empty() ? "" : " || ") + std::string("k == ") + std::to_string(i);
-        pro << "    for (int k = tid; k < " << 2 * na + 1 << "; k += BT) s_agg[k] = " << (mins.empty() ? "0ull" : "(" + mins + ") ? ~0ull : 0ull")
-            << ";\n";
+            if (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN") mins += (mins.empty() ? "" : " || ") + std::string("(k >> 6) == ") + std::to_string(i);
+        pro << "    for (int k = tid; k < " << words << "; k += BT) s_agg[k] = "
+            << (mins.empty() ? "0ull" : "(k < " + std::to_string(64 * na) + " && (" + mins + ")) ? ~0ull : 0ull") << ";\n";
         code << "        {\n";
         for (int i = 0; i < na; ++i) {
             code << "            u64 acc" << i << " = " << (prog_.agg_funcs[static_cast<size_t>(i)] == "MIN" ? "~0ull" : "0ull")
                  << "; u32 seen" << i << " = 0u;";
-            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " u64 cnt" << i << " = 0ull;";
+            if (cnt_base[static_cast<size_t>(i)] >= 0) code << " u64 cnt" << i << " = 0ull;";
             code << "\n";
         }
         // predicated (every expression is total: safe division, wrapping
         // arithmetic), so the loop has no branches
         code << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
-             << "                const u32 row = (fl >> i) & 1u;\n";
+             << "                const u32 row = " << flGet() << ";\n";
         for (int i = 0; i < na; ++i) {
             const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
@@ -1731,37 +1767,36 @@ private:
             else if (f == "MAX") code << "acc" << i << " = (on && x > acc" << i << ") ? x : acc" << i << ";";
             else code << "acc" << i << " += on ? x : 0ull;";
             code << " seen" << i << " |= on;";
-            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) code << " cnt" << i << " += (u64)on;";
+            if (cnt_base[static_cast<size_t>(i)] >= 0) code << " cnt" << i << " += (u64)on;";
             code << " }\n";
         }
         code << "            }\n            const int lane = tid & 63;\n";
         for (int i = 0; i < na; ++i) {
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
-            const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
             const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
-            const char* nz = f == "MIN" ? "w != ~0ull" : "w != 0ull";
             code << "            if (__ballot(seen" << i << " != 0u) != 0ull) {\n"
-                 << "              const u64 w = " << red << "(acc" << i << ");\n"
-                 << "              if (lane == 0 && " << nz << ") " << atom << "(&s_agg[" << i << "], w);\n"
-                 << "              if (lane == 0) atomicOr((u32*)(s_agg + " << 2 * na << "), " << (1u << i) << "u);\n";
-            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
-                code << "              { const u64 c = wave_sum(cnt" << i << "); if (lane == 0 && c) atomicAdd(&s_agg[" << na + i
-                     << "], c); }\n";
+                 << "              " << atom << "(&s_agg[" << 64 * i << " + lane], acc" << i << ");\n";
+            if (f != "COUNT" && cnt_base[static_cast<size_t>(i)] < 0)
+                code << "              if (lane == 0) atomicOr((u32*)(s_agg + " << seen_word << "), " << (1u << i) << "u);\n";
+            if (cnt_base[static_cast<size_t>(i)] >= 0)
+                code << "              atomicAdd(&s_agg[" << cnt_base[static_cast<size_t>(i)] << " + lane], cnt" << i << ");\n";
             code << "            }\n";
         }
         code << "        }\n";
-        epi << "    if (tid == 0) {\n"
-            << "        const u32 seen = *(const u32*)(s_agg + " << 2 * na << ");\n"
+        epi << "    if (tid < 64) {\n"
+            << "        const u32 seen = *(const u32*)(s_agg + " << seen_word << ");\n"
             << "        (void)seen;\n";
         for (int i = 0; i < na; ++i) {
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
             const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
             const char* nz = f == "MIN" ? "x != ~0ull" : "x != 0ull";
-            epi << "        { const u64 x = s_agg[" << i << "]; if (" << nz << ") " << atom << "(&P.result[" << i << "], x); }\n";
-            if (f != "COUNT" && prog_.hidden_slot[static_cast<size_t>(i)] < 0)
-                epi << "        if ((seen >> " << i << ") & 1u) chip::set_bit64(P.result_valid, " << i << ");\n";
-            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
-                epi << "        { const u64 c = s_agg[" << na + i << "]; if (c) atomicAdd(&P.result["
+            epi << "        { const u64 x = " << red << "(s_agg[" << 64 * i << " + tid]); if (tid == 0 && " << nz << ") " << atom
+                << "(&P.result[" << i << "], x); }\n";
+            if (f != "COUNT" && cnt_base[static_cast<size_t>(i)] < 0)
+                epi << "        if (tid == 0 && ((seen >> " << i << ") & 1u)) chip::set_bit64(P.result_valid, " << i << ");\n";
+            if (cnt_base[static_cast<size_t>(i)] >= 0)
+                epi << "        { const u64 c = wave_sum(s_agg[" << cnt_base[static_cast<size_t>(i)] << " + tid]); if (tid == 0 && c) atomicAdd(&P.result["
                     << prog_.hidden_slot[static_cast<size_t>(i)] << "], c); }\n";
         }
         epi << "    }\n";
@@ -1854,7 +1889,7 @@ private:
             << "    for (int k = tid; k < " << FW << "; k += BT) s_flag[k] = 0u;\n";
         code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!((fl >> i) & 1u)) continue;\n"
+             << "            if (!" << flGet() << ") continue;\n"
              << "            const int h = " << groupHash() << ";\n"
              << "            if ((unsigned)h >= " << G << "u) continue;\n"
              << "            u32 want = 1u;\n";
@@ -1939,7 +1974,7 @@ private:
         prog_.keys_kernel = keysDecodable();
         code << "        // GROUP BY: perfect hash, global u64 atomics per row\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!((fl >> i) & 1u)) continue;\n"
+             << "            if (!" << flGet() << ") continue;\n"
              << "            const int h = " << groupHash() << ";\n"
              << "            if ((unsigned)h >= " << G << "u) continue;\n"
              << "            const long long out = (long long)h * " << sts << ";\n";
@@ -2024,7 +2059,7 @@ private:
             << "    for (int k = tid; k < (" << nas << " << cq_hbits); k += BT) s_agg[(1 << cq_hbits) + k] = " << init << ";\n";
         code << "        // GROUP BY: LDS hash table of this query, global atomics on overflow\n#pragma unroll\n"
              << "        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!((fl >> i) & 1u)) continue;\n"
+             << "            if (!" << flGet() << ") continue;\n"
              << "            const int h = " << groupHash() << ";\n"
              << "            if ((unsigned)h >= " << G << "u) continue;\n"
              << "            u32* s_hk = (u32*)s_agg;\n"

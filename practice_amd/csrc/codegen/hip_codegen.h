@@ -130,6 +130,7 @@ struct QueryProgram {
     // per-query bodies (probes of the template's shared tables), and
     // per-query LDS aggregation (init / merge code, bytes per query)
     bool shared_tables = false;
+    bool batch_bool_flags = false;  // the body keeps row flags in `bool fl[IPT]` (else `u32 fl`)
     std::string batch_shared;
     // (scope of the batch kernel: cq_hbits = log2 slots of the query's LDS
     // hash table).  batch_hash_slot_bytes > 0 marks a hash region whose size

@@ -78,6 +78,15 @@ __device__ __forceinline__ void set_bit64(u64* w, i64 idx) { atomicOr(&w[idx >> 
 // ------------------------------------------------------------------ hash tables
 // Layouts and their probe / insert primitives: ht_layouts.h
 
+// any item of a lane still alive
+template <int N>
+__device__ __forceinline__ bool any_of(const bool (&f)[N]) {
+    bool a = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) a = a | f[i];
+    return a;
+}
+
 // ------------------------------------------------------------------ wave reductions
 __device__ __forceinline__ u64 wave_sum(u64 v) {
 #pragma unroll

@@ -25,7 +25,7 @@
 namespace db {
 
 struct BatchOptions {
-    int block_threads = 256;      // cq_batch_scan block size (64..1024)
+    int block_threads = 1024;     // cq_batch_scan block size (64..1024)
     int vectors_per_thread = 2;   // 128-bit loads per column per lane per tile (1..4)
     bool graph = true;            // capture / replay the device phases as a HIP graph
     bool autotune = false;        // time block / load-width variants, keep the fastest
@@ -33,7 +33,7 @@ struct BatchOptions {
     int max_queries = 256;        // queries per shared scan
     bool shared_tables = true;    // one probe per template and dimension (packed attributes)
     bool lds_aggregation = true;  // per-query LDS aggregation for small group domains
-    int lds_kb = 64;              // dynamic LDS of one shared-scan block (KiB, 16..160)
+    int lds_kb = 160;             // dynamic LDS of one shared-scan block (KiB, 16..160)
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
@@ -90,7 +90,7 @@ private:
         hipGraphExec_t exec = nullptr;
     };
     struct Tuned {
-        int block_threads = 256, vectors_per_thread = 2;
+        int block_threads = 1024, vectors_per_thread = 2;
     };
     DatabaseInstance& db_;
     ExecutionContext* ctx_;

@@ -27,7 +27,7 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "window_us") o.window_us = static_cast<int>(v);
         else if (k == "async_compile") o.async_compile = v != 0;
         else if (k == "interpret") o.interpret_cold = v != 0;
-        else if (k == "bt" || k == "vpt" || k == "graph") batch_opts += (batch_opts.empty() ? "" : ",") + item;
+        else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb") batch_opts += (batch_opts.empty() ? "" : ",") + item;
         else throw std::runtime_error("unknown server option: " + k);
     }
     o.batch = parseBatchOptions(batch_opts);
