@@ -134,7 +134,7 @@ void cq_result_free(cq_result* r);
  * reference runs QueryEngine::executeQuery per query
  * (src/core/execution.cpp:652-725).
  * options: NULL or "key=value,..." with bt (64..1024, multiple of 64,
- * default 1024), vpt (1..4 128-bit loads per column per lane, default 2),
+ * default 1024), vpt (1..4 128-bit loads per column per lane, default 1),
  * graph (0/1), autotune (0/1: time every bt x vpt once per signature set and
  * keep the fastest), max_templates (<= 16), max_queries (per scan), shared
  * (0/1: one probe per template and dimension), lds_agg (0/1: per-query LDS
@@ -159,8 +159,11 @@ int cq_db_generate_batch_code(cq_db* db, const char* const* sqls, int n, const c
  * max_templates (16), promote_after (2: requests of a new signature before
  * it is compiled into the shared scan), compile_threads (2), budget_mb
  * (default: a quarter of free memory), window_us (0), async_compile (1),
- * interpret (1: new signatures run at once on the interpreted tier), bt,
- * vpt, graph, lds_kb (as for cq_batch_execute). */
+ * interpret (1: new signatures run at once on the interpreted tier),
+ * cold_lanes (1: lanes for not-yet-compiled signatures), per_template (1:
+ * one shared-scan launch per hot signature in a batch; 0: one module for the
+ * whole hot set), max_hot (64), bt, vpt, graph, lds_kb (as for
+ * cq_batch_execute). */
 int cq_server_start(cq_db* db, const char* options, cq_server** out);
 /* Enqueue a request; returns a ticket >= 0 (negative on error). */
 long long cq_server_submit(cq_server* s, const char* sql);

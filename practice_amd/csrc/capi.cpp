@@ -236,6 +236,7 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "compact_tables") c.codegen.allow_compact_tables = v != 0;
         else if (n == "prefetch") c.codegen.prefetch = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
+        else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else return fail(CQ_ERR_INVALID, "unknown option " + n);
         return CQ_OK;
     });

@@ -68,44 +68,55 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         }
         src << "    __syncthreads();\n";
     }
+    // the last template (kernel order) that reads each union column: its
+    // load is nontemporal, earlier templates' loads keep the tile in cache
+    std::map<std::string, size_t> last_use;
+    for (size_t t = 0; t < templates.size(); ++t)
+        for (const auto& c : templates[t]->fact_cols) last_use[c] = t;
     src << "    for (long long tile = blockIdx.x; tile < B.num_tiles; tile += gridDim.x) {\n"
-        << "        const long long base = B.row_begin + tile * (long long)TILE;\n";
-    const size_t nu = union_cols.size();
-    for (size_t u = 0; u < nu; ++u) {
-        src << "        int u" << u << "[IPT];\n";
-        if (union_nullable[u]) src << "        u32 uv" << u << " = 0u;\n";
-    }
-    src << "        u32 tile_fl = 0u;\n"
-        << "        if (base + TILE <= B.row_end) {\n"
-        << "#pragma unroll\n            for (int v = 0; v < VPT; ++v) {\n"
-        << "                const long long r = base + (long long)v * (BT * 4) + tid * 4;\n";
-    for (size_t u = 0; u < nu; ++u) {
-        src << "                { const v4i x = __builtin_nontemporal_load((const v4i*)(B.col[" << u << "] + r)); u" << u
-            << "[4 * v] = x.x; u" << u << "[4 * v + 1] = x.y; u" << u << "[4 * v + 2] = x.z; u" << u << "[4 * v + 3] = x.w; }\n";
-        if (union_nullable[u])
-            src << "                uv" << u << " |= ((B.valid[" << u << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
-    }
-    src << "            }\n"
-        << "            tile_fl = " << (ipt == 32 ? "0xffffffffu" : "((1u << IPT) - 1u)") << ";\n"
-        << "        } else {\n"
-        << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
-        << "                const long long r = base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
-        << "                const bool in = r < B.row_end;\n"
-        << "                if (in) tile_fl |= 1u << i;\n";
-    for (size_t u = 0; u < nu; ++u) {
-        src << "                u" << u << "[i] = in ? B.col[" << u << "][r] : 0;\n";
-        if (union_nullable[u]) src << "                if (in) uv" << u << " |= chip::bit32(B.valid[" << u << "], r) << i;\n";
-    }
-    src << "            }\n        }\n";
+        << "        const long long base = B.row_begin + tile * (long long)TILE;\n"
+        << "        const bool full = base + TILE <= B.row_end;\n"
+        << "        u32 tile_fl = " << (ipt == 32 ? "0xffffffffu" : "((1u << IPT) - 1u)") << ";\n"
+        << "        if (!full) {\n"
+        << "            tile_fl = 0u;\n"
+        << "#pragma unroll\n            for (int i = 0; i < IPT; ++i)\n"
+        << "                if (base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3) < B.row_end) tile_fl |= 1u << i;\n"
+        << "        }\n";
     for (size_t t = 0; t < templates.size(); ++t) {
         const QueryProgram* p = templates[t];
         src << "        // template " << t << ": " << p->signature << "\n"
             << "        if (B.tq_begin[" << t << "] < B.tq_end[" << t << "]) {\n";
-        for (size_t s = 0; s < p->fact_cols.size(); ++s) {
-            const int u = slot[p->fact_cols[s]];
-            src << "        int (&c" << s << ")[IPT] = u" << u << ";\n";
-            if (p->fact_nullable[s]) src << "        const u32 v" << s << " = uv" << u << ";\n";
+        // this template's columns, loaded in its own scope: registers of
+        // different templates do not overlap
+        const size_t nf = p->fact_cols.size();
+        for (size_t s2 = 0; s2 < nf; ++s2) {
+            src << "        int c" << s2 << "[IPT];\n";
+            if (p->fact_nullable[s2]) src << "        u32 v" << s2 << " = 0u;\n";
         }
+        src << "        if (full) {\n"
+            << "#pragma unroll\n            for (int v = 0; v < VPT; ++v) {\n"
+            << "                const long long r = base + (long long)v * (BT * 4) + tid * 4;\n";
+        for (size_t s2 = 0; s2 < nf; ++s2) {
+            const int u = slot[p->fact_cols[s2]];
+            const bool nt = last_use[p->fact_cols[s2]] == t;
+            src << "                { const v4i x = " << (nt ? "__builtin_nontemporal_load(" : "*(") << "(const v4i*)(B.col[" << u
+                << "] + r)); c" << s2 << "[4 * v] = x.x; c" << s2 << "[4 * v + 1] = x.y; c" << s2 << "[4 * v + 2] = x.z; c" << s2
+                << "[4 * v + 3] = x.w; }\n";
+            if (p->fact_nullable[s2])
+                src << "                v" << s2 << " |= ((B.valid[" << u << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
+        }
+        src << "            }\n"
+            << "        } else {\n"
+            << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+            << "                const long long r = base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
+            << "                const bool in = r < B.row_end;\n";
+        for (size_t s2 = 0; s2 < nf; ++s2) {
+            const int u = slot[p->fact_cols[s2]];
+            src << "                c" << s2 << "[i] = in ? B.col[" << u << "][r] : 0;\n";
+            if (p->fact_nullable[s2]) src << "                if (in) v" << s2 << " |= chip::bit32(B.valid[" << u << "], r) << i;\n";
+        }
+        src << "            }\n"
+            << "        }\n";
         if (!p->batch_shared.empty())
             src << "        const cq_query_params& P0 = B.qp[B.tq_begin[" << t << "]];\n" << p->batch_shared;
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"

@@ -59,3 +59,13 @@ hipError_t cq_launch_sort(unsigned long long* idx, long long rows, const cq_sort
 hipError_t cq_launch_gather(const unsigned long long* cols, const unsigned long long* valid, const unsigned long long* idx,
                             long long rows, int vts, long long cap_rows, long long valid_words, unsigned long long* out,
                             unsigned long long* out_valid, hipStream_t s);
+
+// Copy of many small device ranges into one contiguous buffer (so that one
+// device-to-host copy fetches the results of a whole batch): segment k
+// copies `words` u64 from `src` to dst + dst_word.
+struct cq_copy_seg {
+    const unsigned long long* src;
+    unsigned long long dst_word;
+    unsigned long long words;
+};
+hipError_t cq_launch_copy_segments(const cq_copy_seg* segs, int n, unsigned long long* dst, hipStream_t s);

@@ -372,6 +372,13 @@ __global__ __launch_bounds__(1024) void k_bitonic_local(u64* idx, i64 padded, cq
     for (i64 i = threadIdx.x; i < padded; i += 1024) idx[i] = s[i];
 }
 
+// one block per segment (segments are a few KB: result columns of a query)
+__global__ __launch_bounds__(256) void k_copy_segments(const cq_copy_seg* __restrict__ segs, u64* __restrict__ dst) {
+    const cq_copy_seg sg = segs[blockIdx.x];
+    u64* out = dst + sg.dst_word;
+    for (u64 k = threadIdx.x; k < sg.words; k += 256) out[k] = sg.src[k];
+}
+
 __global__ __launch_bounds__(256) void k_gather(const u64* __restrict__ cols, const u64* __restrict__ valid,
                                                 const u64* __restrict__ idx, i64 rows, int vts, i64 cap_rows,
                                                 i64 valid_words, u64* __restrict__ out, u64* __restrict__ out_valid) {
@@ -485,6 +492,12 @@ long long cq_sort_padded(long long rows) {
     long long padded = 1;
     while (padded < rows) padded <<= 1;
     return padded;
+}
+
+hipError_t cq_launch_copy_segments(const cq_copy_seg* segs, int n, unsigned long long* dst, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_segments, dim3(n), dim3(256), 0, s, segs, dst);
+    return hipGetLastError();
 }
 
 hipError_t cq_launch_gather(const unsigned long long* cols, const unsigned long long* valid, const unsigned long long* idx,

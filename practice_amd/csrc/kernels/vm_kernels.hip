@@ -201,6 +201,19 @@ __global__ __launch_bounds__(VM_BT) void k_vm_build(const cq_query_params P, con
     }
 }
 
+// scalar slots accumulated per thread in registers by k_vm_scan
+constexpr int VM_RACC = 4;
+// function of scalar storage slot k: aggregate k's, or SUM for a hidden count
+__device__ __forceinline__ int slot_fn(const cq_vm_program* V, int k) {
+    return k < V->naggs ? V->agg_fn[k] : CQ_VM_SUM;
+}
+__device__ __forceinline__ bool slot_is_min(const cq_vm_program* V, int k) { return k < V->naggs && V->agg_fn[k] == CQ_VM_MIN; }
+__device__ __forceinline__ u64 combine(u64 acc, u64 x, int fn) {
+    if (fn == CQ_VM_MIN) return x < acc ? x : acc;
+    if (fn == CQ_VM_MAX) return x > acc ? x : acc;
+    return acc + x;
+}
+
 __device__ __forceinline__ u64 wave_reduce(u64 v, int fn) {
     return fn == CQ_VM_MIN ? chip::wave_min(v) : (fn == CQ_VM_MAX ? chip::wave_max(v) : chip::wave_sum(v));
 }
@@ -246,6 +259,12 @@ __global__ __launch_bounds__(VM_BT) void k_vm_scan(const cq_query_params P, cons
         for (long long k = tid; k < G; k += VM_BT) s_flag[k] = 0u;
     }
     __syncthreads();
+    // scalar slots 0..VM_RACC-1 accumulate per thread in registers across
+    // rows (one wave reduction at the end); later slots per row and wave
+    u64 racc[VM_RACC];
+    u32 rseen = 0u;
+#pragma unroll
+    for (int k = 0; k < VM_RACC; ++k) racc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
     const long long nrows = P.row_end - P.row_begin;
     for (long long base = (long long)blockIdx.x * VM_BT; base < nrows; base += (long long)gridDim.x * VM_BT) {
         RowCtx c{&P, P.row_begin + base + tid, nullptr, nullptr, 0, base + tid < nrows};
@@ -255,16 +274,28 @@ __global__ __launch_bounds__(VM_BT) void k_vm_scan(const cq_query_params P, cons
                 const int fn = V->agg_fn[i];
                 const bool v = alive && (V->agg_valid[i] < 0 || truthy(RF[V->agg_valid[i] * VM_BT], CQ_T_U32));
                 const u64 x = fn == CQ_VM_COUNT ? 1ull : (V->agg_val[i] >= 0 ? RF[V->agg_val[i] * VM_BT] : 1ull);
-                const u64 id = fn == CQ_VM_MIN ? ~0ull : 0ull;
-                const u64 w = wave_reduce(v ? x : id, fn);
-                const bool any = __ballot(v) != 0ull;
-                if (lane == 0 && any) {
-                    atomic_fn(&s_acc[i], w, fn);
-                    s_seen[i] = 1u;
+                if (i < VM_RACC) {
+#pragma unroll
+                    for (int k = 0; k < VM_RACC; ++k)
+                        if (k == i && v) racc[k] = combine(racc[k], x, fn);
+                    if (v) rseen |= 1u << i;
+                } else {
+                    const u64 id = fn == CQ_VM_MIN ? ~0ull : 0ull;
+                    const u64 w = wave_reduce(v ? x : id, fn);
+                    const bool any = __ballot(v) != 0ull;
+                    if (lane == 0 && any) {
+                        atomic_fn(&s_acc[i], w, fn);
+                        s_seen[i] = 1u;
+                    }
                 }
-                if (V->agg_hidden[i] >= 0) {
+                const int hs = V->agg_hidden[i];
+                if (hs >= 0 && hs < VM_RACC) {
+#pragma unroll
+                    for (int k = 0; k < VM_RACC; ++k)
+                        if (k == hs && v) racc[k] += 1ull;
+                } else if (hs >= 0) {
                     const u64 cnt = wave_reduce(v ? 1ull : 0ull, CQ_VM_SUM);
-                    if (lane == 0 && cnt) atomicAdd(&s_acc[V->agg_hidden[i]], cnt);
+                    if (lane == 0 && cnt) atomicAdd(&s_acc[hs], cnt);
                 }
             }
             continue;
@@ -307,6 +338,18 @@ __global__ __launch_bounds__(VM_BT) void k_vm_scan(const cq_query_params P, cons
                 }
                 if (fn == CQ_VM_COUNT && !V->fast_sparse) set_bit_once(P.result_valid, out + slot);
             }
+        }
+    }
+    if (ng == 0) {
+#pragma unroll
+        for (int k = 0; k < VM_RACC; ++k) {
+            if (k >= sts) break;
+            const int fn = slot_fn(V, k);
+            const u64 w = wave_reduce(racc[k], fn);
+            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
+            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
+            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
+            if (lane == 0 && any) s_seen[k] = 1u;
         }
     }
     __syncthreads();
@@ -380,6 +423,273 @@ __global__ __launch_bounds__(VM_BT) void k_vm_dense_count(const cq_query_params 
     if (c.in) counts[g] = keep ? 1ull : 0ull;
 }
 
+// ---------------------------------------------------------------- 4-row scan
+// The same interpreter with 4 consecutive rows per thread (one 16-byte load
+// per column) and the register file in VGPRs: one array of VM4_REGS
+// registers per row, indexed by the instruction's wave-uniform register
+// numbers (relative VGPR addressing, no LDS round trips).  Decoding and
+// dispatch are paid once per instruction for 4 rows.  Programs with more
+// registers, or with instructions that only build / HAVING programs use,
+// run on k_vm_scan.
+constexpr int VM4_BT = 256;
+constexpr int VM4_REGS = 16;
+
+#define VM4_EACH(stmt)                         \
+    {                                          \
+        {                                      \
+            u64* R = R0;                       \
+            const int k = 0;                   \
+            (void)k;                           \
+            stmt;                              \
+        }                                      \
+        {                                      \
+            u64* R = R1;                       \
+            const int k = 1;                   \
+            (void)k;                           \
+            stmt;                              \
+        }                                      \
+        {                                      \
+            u64* R = R2;                       \
+            const int k = 2;                   \
+            (void)k;                           \
+            stmt;                              \
+        }                                      \
+        {                                      \
+            u64* R = R3;                       \
+            const int k = 3;                   \
+            (void)k;                           \
+            stmt;                              \
+        }                                      \
+    }
+
+// one row's aggregation after the pipeline (k_vm_scan's epilogue per row)
+__device__ __forceinline__ void vm4_aggregate_row(const cq_query_params& P, const cq_vm_program* __restrict__ V, const u64* R,
+                                                  bool alive, int lds_groups, u64* s_agg, u32* s_flag, u64 (&racc)[VM_RACC],
+                                                  u32& rseen) {
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    if (!alive) return;
+    if (ng == 0) {
+        for (int i = 0; i < V->naggs && i < VM_RACC; ++i) {
+            const int fn = V->agg_fn[i];
+            const bool v = V->agg_valid[i] < 0 || truthy(R[V->agg_valid[i]], CQ_T_U32);
+            if (!v) continue;
+            const u64 x = fn == CQ_VM_COUNT ? 1ull : (V->agg_val[i] >= 0 ? R[V->agg_val[i]] : 1ull);
+#pragma unroll
+            for (int k = 0; k < VM_RACC; ++k)
+                if (k == i) racc[k] = combine(racc[k], x, fn);
+            rseen |= 1u << i;
+            const int hs = V->agg_hidden[i];
+#pragma unroll
+            for (int k = 0; k < VM_RACC; ++k)
+                if (k == hs) racc[k] += 1ull;
+        }
+        return;
+    }
+    const int h = (int)(i64)R[V->group_reg];
+    if ((unsigned)h >= (unsigned long long)G) return;
+    if (lds_groups) {
+        u32 want = 0x80000000u;
+        for (int g = 0; g < ng; ++g)
+            if (!V->fast_sparse && (V->key_valid[g] < 0 || truthy(R[V->key_valid[g]], CQ_T_U32))) want |= 1u << g;
+        for (int i = 0; i < V->naggs; ++i) {
+            const int fn = V->agg_fn[i];
+            const bool v = V->agg_valid[i] < 0 || truthy(R[V->agg_valid[i]], CQ_T_U32);
+            if (v) {
+                const u64 x = fn == CQ_VM_COUNT ? 1ull : R[V->agg_val[i]];
+                atomic_fn(&s_agg[(long long)h * nas + i], x, fn);
+                if (V->agg_hidden[i] >= 0) atomicAdd(&s_agg[(long long)h * nas + (V->agg_hidden[i] - ng)], 1ull);
+                else if (!V->fast_sparse) want |= 1u << (ng + i);
+            }
+            if (fn == CQ_VM_COUNT && !V->fast_sparse) want |= 1u << (ng + i);
+        }
+        if ((s_flag[h] & want) != want) atomicOr(&s_flag[h], want);
+        return;
+    }
+    const long long out = (long long)h * sts;
+    for (int g = 0; g < ng; ++g) {
+        P.result[out + g] = R[V->key_val[g]];
+        if (!V->fast_sparse && (V->key_valid[g] < 0 || truthy(R[V->key_valid[g]], CQ_T_U32)))
+            set_bit_once(P.result_valid, out + g);
+    }
+    for (int i = 0; i < V->naggs; ++i) {
+        const int fn = V->agg_fn[i];
+        const int slot = ng + i;
+        const bool v = V->agg_valid[i] < 0 || truthy(R[V->agg_valid[i]], CQ_T_U32);
+        if (v) {
+            const u64 x = fn == CQ_VM_COUNT ? 1ull : R[V->agg_val[i]];
+            atomic_fn(&P.result[out + slot], x, fn);
+            if (V->agg_hidden[i] >= 0) atomicAdd(&P.result[out + V->agg_hidden[i]], 1ull);
+            else if (fn != CQ_VM_COUNT && !V->fast_sparse) set_bit_once(P.result_valid, out + slot);
+        }
+        if (fn == CQ_VM_COUNT && !V->fast_sparse) set_bit_once(P.result_valid, out + slot);
+    }
+}
+
+__global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
+    __shared__ u64 s_acc[CQ_VM_MAX_AGGS * 2];
+    __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
+    extern __shared__ u64 s_dyn[];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    u64* s_agg = s_dyn;
+    u32* s_flag = (u32*)(s_dyn + (lds_groups ? G * nas : 0));
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += VM4_BT) {
+            s_acc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
+            s_seen[k] = 0u;
+        }
+    } else if (lds_groups) {
+        for (long long k = tid; k < G * nas; k += VM4_BT) s_agg[k] = slot_is_min(V, (int)(k % nas)) ? ~0ull : 0ull;
+        for (long long k = tid; k < G; k += VM4_BT) s_flag[k] = 0u;
+    }
+    __syncthreads();
+    u64 racc[VM_RACC];
+    u32 rseen = 0u;
+#pragma unroll
+    for (int k = 0; k < VM_RACC; ++k) racc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
+    u64 R0[VM4_REGS], R1[VM4_REGS], R2[VM4_REGS], R3[VM4_REGS];
+#pragma unroll
+    for (int r = 0; r < VM4_REGS; ++r) R0[r] = R1[r] = R2[r] = R3[r] = 0ull;
+    const long long nrows = P.row_end - P.row_begin;
+    const int code_end = V->scan_end;
+    for (long long base = (long long)blockIdx.x * (VM4_BT * 4); base < nrows; base += (long long)gridDim.x * (VM4_BT * 4)) {
+        const long long off = base + (long long)tid * 4;  // first of this thread's 4 rows
+        const long long row0 = P.row_begin + off;          // multiple of 4
+        const bool full = off + 4 <= nrows;
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) in[k] = off + k < nrows;
+        bool alive[4] = {in[0], in[1], in[2], in[3]};
+        for (int pc = V->scan_begin; pc < code_end; ++pc) {
+            const cq_vm_instr ins = V->code[pc];
+            const int d = ins.d, a = ins.a, b = ins.b, t = ins.t, t2 = ins.t2;
+            switch (ins.op) {
+                case CQ_OP_COL: {
+                    const int* col = P.col[ins.imm];
+                    if (full) {
+                        const chip::v4i x = *(const chip::v4i*)(col + row0);
+                        R0[d] = (u64)(i64)x.x;
+                        R1[d] = (u64)(i64)x.y;
+                        R2[d] = (u64)(i64)x.z;
+                        R3[d] = (u64)(i64)x.w;
+                    } else {
+                        VM4_EACH(R[d] = in[k] ? (u64)(i64)col[row0 + k] : 0ull)
+                    }
+                    break;
+                }
+                case CQ_OP_VALID: {
+                    const u32* vb = P.valid[ins.imm];
+                    // 4 rows from a multiple of 4: bits of one 32-bit word
+                    const u32 w = (vb && off < nrows) ? vb[row0 >> 5] >> (row0 & 31) : 0xfu;
+                    VM4_EACH(R[d] = in[k] ? (u64)((w >> k) & 1u) : 0ull)
+                    break;
+                }
+                case CQ_OP_LIT: VM4_EACH(R[d] = (u64)P.lit[ins.imm]) break;
+                case CQ_OP_FLIT: VM4_EACH(R[d] = __builtin_bit_cast(u64, P.flit[ins.imm])) break;
+                case CQ_OP_IMM: {
+                    const u64 v = canon((i64)(((u64)ins.imm2 << 32) | (u64)(u32)ins.imm), t);
+                    VM4_EACH(R[d] = v)
+                    break;
+                }
+                case CQ_OP_CONV: VM4_EACH(R[d] = conv(R[a], t, t2)) break;
+                case CQ_OP_ADD: case CQ_OP_SUB: case CQ_OP_MUL: case CQ_OP_DIV: case CQ_OP_MOD: {
+                    const int op = ins.op;
+                    VM4_EACH(R[d] = arith(op, R[a], R[b], t))
+                    break;
+                }
+                case CQ_OP_EQ: case CQ_OP_NE: case CQ_OP_LT: case CQ_OP_LE: case CQ_OP_GT: case CQ_OP_GE: {
+                    const int op = ins.op;
+                    VM4_EACH(R[d] = (u64)compare(op, R[a], R[b], t))
+                    break;
+                }
+                case CQ_OP_NOT: VM4_EACH(R[d] = truthy(R[a], t) ? 0ull : 1ull) break;
+                case CQ_OP_AND: VM4_EACH(R[d] = (truthy(R[a], t) && truthy(R[b], t2)) ? 1ull : 0ull) break;
+                case CQ_OP_OR: VM4_EACH(R[d] = (truthy(R[a], t) || truthy(R[b], t2)) ? 1ull : 0ull) break;
+                case CQ_OP_SEL: {
+                    const int c = ins.c;
+                    VM4_EACH(R[d] = truthy(R[c], t2) ? R[a] : R[b])
+                    break;
+                }
+                case CQ_OP_FILTER: VM4_EACH(alive[k] = alive[k] && truthy(R[a], t)) break;
+                case CQ_OP_PROBE: {
+                    const cq_vm_join& J = V->joins[ins.imm];
+                    const u32* ht = P.ht[ins.imm];
+                    VM4_EACH({
+                        int pay = 0;
+                        if (alive[k]) alive[k] = probe(J, ht, (int)(i64)R[a], pay);
+                        R[d] = (u64)(i64)pay;
+                    })
+                    break;
+                }
+                case CQ_OP_EXIT:
+                    if (__ballot(alive[0] || alive[1] || alive[2] || alive[3]) == 0ull) pc = code_end;
+                    break;
+                default: break;  // build / HAVING instructions never reach this kernel
+            }
+        }
+        vm4_aggregate_row(P, V, R0, alive[0], lds_groups, s_agg, s_flag, racc, rseen);
+        vm4_aggregate_row(P, V, R1, alive[1], lds_groups, s_agg, s_flag, racc, rseen);
+        vm4_aggregate_row(P, V, R2, alive[2], lds_groups, s_agg, s_flag, racc, rseen);
+        vm4_aggregate_row(P, V, R3, alive[3], lds_groups, s_agg, s_flag, racc, rseen);
+    }
+    if (ng == 0) {
+#pragma unroll
+        for (int k = 0; k < VM_RACC; ++k) {
+            if (k >= sts) break;
+            const int fn = slot_fn(V, k);
+            const u64 w = wave_reduce(racc[k], fn);
+            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
+            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
+            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
+            if (lane == 0 && any) s_seen[k] = 1u;
+        }
+    }
+    __syncthreads();
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += VM4_BT) {
+            const int fn = slot_fn(V, k);
+            bool hidden = false;
+            for (int i = 0; i < V->naggs; ++i) hidden = hidden || V->agg_hidden[i] == k;
+            const bool scalar_valid_bit = k < V->naggs && fn != CQ_VM_COUNT && V->agg_hidden[k] < 0;
+            const u64 x = s_acc[k];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
+            else if (x) atomicAdd(&P.result[k], x);
+            if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
+        }
+        return;
+    }
+    if (!lds_groups) return;
+    for (long long g = tid; g < G; g += VM4_BT) {
+        const u32 f = s_flag[g];
+        if (!(f & 0x80000000u)) continue;
+        const long long base = g * sts;
+        long long rem = g;
+        long long dg[CQ_VM_MAX_AGGS];
+        for (int k = ng - 1; k >= 0; --k) {
+            dg[k] = rem % V->key_card[k];
+            rem /= V->key_card[k];
+        }
+        for (int k = 0; k < ng; ++k) {
+            const long long v = V->key_nullable[k] ? (dg[k] == 0 ? 0ll : V->key_min[k] + dg[k] - 1) : V->key_min[k] + dg[k];
+            P.result[base + k] = (u64)v;
+        }
+        for (int s2 = 0; s2 < nas; ++s2) {
+            const int fn = s2 < V->naggs ? V->agg_fn[s2] : CQ_VM_SUM;
+            const u64 x = s_agg[g * nas + s2];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[base + ng + s2], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[base + ng + s2], x); }
+            else if (x) atomicAdd(&P.result[base + ng + s2], x);
+        }
+        if (!V->fast_sparse)
+            for (int s2 = 0; s2 < sts; ++s2)
+                if ((f >> s2) & 1u) chip::set_bit64(P.result_valid, base + s2);
+    }
+}
+
 }  // namespace
 
 hipError_t cq_vm_launch_build(const cq_query_params* P, const cq_vm_program* V, int join, long long dim_rows, hipStream_t s) {
@@ -402,3 +712,29 @@ hipError_t cq_vm_launch_dense_count(const cq_query_params* P, const cq_vm_progra
 }
 
 int cq_vm_block_threads() { return VM_BT; }
+
+bool cq_vm_scan4_supported(const cq_vm_program* V) {
+    if (V->nregs > VM4_REGS || V->scan_begin < 0 || V->scan_end > V->ncode) return false;
+    // scalar aggregates: every slot in the per-thread registers
+    if (V->ngroups == 0 && V->storage_ts > VM_RACC) return false;
+    for (int pc = V->scan_begin; pc < V->scan_end; ++pc) {
+        const int op = V->code[pc].op;
+        if (op == CQ_OP_DCOL || op == CQ_OP_DVALID || op == CQ_OP_SLOT || op == CQ_OP_SLOTV) return false;
+    }
+    return true;
+}
+
+hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, unsigned lds_group_bytes, int device,
+                              hipStream_t s) {
+    const long long rows = P->row_end - P->row_begin;
+    if (rows <= 0) return hipSuccess;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_vm_scan4), VM4_BT,
+                                                     lds_group_bytes) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
+    const long long tiles = (rows + VM4_BT * 4 - 1) / (VM4_BT * 4);
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, (long long)per_cu * cus));
+    hipLaunchKernelGGL(k_vm_scan4, dim3(grid), dim3(VM4_BT), lds_group_bytes, s, *P, V, lds_group_bytes ? 1 : 0);
+    return hipGetLastError();
+}

@@ -359,6 +359,11 @@ void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
     if (p.vm) {
         const long long rows = run.P.row_end - run.P.row_begin;
         if (rows <= 0) return;
+        if (!ctx->config.vm_row_kernel && cq_vm_scan4_supported(p.vm.get()) && run.P.row_begin % 4 == 0) {
+            hipCheck(cq_vm_launch_scan4(&run.P, run.vm, static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
+                     "interpreted scan");
+            return;
+        }
         const long long bt = cq_vm_block_threads();
         const unsigned grid = static_cast<unsigned>(std::min<long long>((rows + bt - 1) / bt, static_cast<long long>(deviceCUs(ctx->device)) * 8));
         hipCheck(cq_vm_launch_scan(&run.P, run.vm, static_cast<unsigned>(p.vm_lds_bytes), grid, ctx->stream), "interpreted scan");
@@ -432,14 +437,15 @@ size_t HipModuleExecutor::fetchBytes(const QueryRun& run, unsigned long long row
     return static_cast<size_t>(p.visible_ts) * (r * 8 + static_cast<size_t>(run.vwords) * 8);
 }
 
-void HipModuleExecutor::fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found, void* host) {
+void HipModuleExecutor::fetchSegments(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found, char* host,
+                                      unsigned long long host_word, std::vector<cq_copy_seg>& segs) {
     const QueryProgram& p = *run.program;
     hipStream_t s = ctx->stream;
     const int vts = p.visible_ts;
-    run.fetch_host = static_cast<char*>(host);
+    run.fetch_host = host;
     if (run.single_cell) {
         run.rows = 1;
-        hipCheck(hipMemcpyAsync(host, run.res, 8, hipMemcpyDeviceToHost, s), "fetch");
+        segs.push_back(cq_copy_seg{run.res, host_word, 1});
         return;
     }
     long long rows = static_cast<long long>(rows_found);
@@ -477,12 +483,20 @@ void HipModuleExecutor::fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsign
     if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
     run.rows = rows;
     // host layout: [column][rows] values, then [column][vwords] validity words
-    char* h = static_cast<char*>(host);
     for (int c = 0; c < vts && rows; ++c)
-        hipCheck(hipMemcpyAsync(h + static_cast<size_t>(c) * rows * 8, o + static_cast<size_t>(c) * cap_rows, rows * 8,
-                                hipMemcpyDeviceToHost, s), "fetch column");
-    if (vts) hipCheck(hipMemcpyAsync(h + static_cast<size_t>(vts) * rows * 8, ov, static_cast<size_t>(vts) * vwords * 8,
-                                     hipMemcpyDeviceToHost, s), "fetch validity");
+        segs.push_back(cq_copy_seg{o + static_cast<size_t>(c) * cap_rows, host_word + static_cast<unsigned long long>(c) * rows,
+                                   static_cast<unsigned long long>(rows)});
+    if (vts)
+        segs.push_back(cq_copy_seg{ov, host_word + static_cast<unsigned long long>(vts) * rows,
+                                   static_cast<unsigned long long>(vts) * vwords});
+}
+
+void HipModuleExecutor::fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found, void* host) {
+    std::vector<cq_copy_seg> segs;
+    fetchSegments(run, ctx, rows_found, static_cast<char*>(host), 0, segs);
+    char* h = static_cast<char*>(host);
+    for (const auto& sg : segs)
+        hipCheck(hipMemcpyAsync(h + sg.dst_word * 8, sg.src, sg.words * 8, hipMemcpyDeviceToHost, ctx->stream), "fetch");
 }
 
 void HipModuleExecutor::fetchFinish(QueryRun& run) {

@@ -34,6 +34,7 @@
 
 #include "../codegen/hip_codegen.h"
 #include "../device/query_params.h"
+#include "../kernels/aot_kernels.h"
 #include "../plan/memory.h"
 
 namespace db {
@@ -90,6 +91,7 @@ struct EngineConfig {
     size_t memory_guard_reserve_bytes = 512ull << 20;
     double memory_guard_reserve_fraction = 0.10;
     bool kernel_timing = false;          // record cq_scan duration with events
+    bool vm_row_kernel = false;          // interpreted tier: always the one-row-per-thread scan
     CodegenOptions codegen;
     size_t module_cache_capacity = 256;  // compiled modules kept (LRU)
 };
@@ -278,6 +280,10 @@ public:
     // fetchFinish decodes once the stream has reached that point
     size_t fetchBytes(const QueryRun& run, unsigned long long rows) const;
     void fetchLaunch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows, void* host);
+    // the same, as device ranges to copy (u64 words; destination words from
+    // host_word on, in the host layout fetchFinish decodes at `host`)
+    void fetchSegments(QueryRun& run, ExecutionContext* ctx, unsigned long long rows, char* host, unsigned long long host_word,
+                       std::vector<cq_copy_seg>& segs);
     void fetchFinish(QueryRun& run);
 };
 

@@ -542,9 +542,27 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     for (int k = 0; k < n; ++k)
         off[static_cast<size_t>(k) + 1] = off[static_cast<size_t>(k)] +
                                           ((ex->fetchBytes(runs[static_cast<size_t>(k)], rows[static_cast<size_t>(k)]) + 255) & ~size_t(255));
-    char* host = static_cast<char*>(pinned(off.back() + 256));
+    // every result range gathered on the device into one buffer, then one
+    // device-to-host copy (instead of a copy per column and query)
+    const size_t fetch_bytes = off.back();
+    std::vector<cq_copy_seg> segs;
     for (int k = 0; k < n; ++k)
-        ex->fetchLaunch(runs[static_cast<size_t>(k)], ctx, rows[static_cast<size_t>(k)], host + off[static_cast<size_t>(k)]);
+        ex->fetchSegments(runs[static_cast<size_t>(k)], ctx, rows[static_cast<size_t>(k)], nullptr, off[static_cast<size_t>(k)] / 8, segs);
+    const size_t seg_bytes = segs.size() * sizeof(cq_copy_seg);
+    char* host = static_cast<char*>(pinned(fetch_bytes + seg_bytes + 256));
+    char* hseg = host + ((fetch_bytes + 255) & ~size_t(255));
+    if (!segs.empty()) std::memcpy(hseg, segs.data(), seg_bytes);
+    DeviceBuffer& dfetch = ctx->scratch("batch_fetch");
+    dfetch.ensure(((fetch_bytes + 255) & ~size_t(255)) + seg_bytes + 8);
+    char* dbase = static_cast<char*>(dfetch.data());
+    cq_copy_seg* dseg = reinterpret_cast<cq_copy_seg*>(dbase + ((fetch_bytes + 255) & ~size_t(255)));
+    if (!segs.empty()) {
+        hipCheck(hipMemcpyAsync(dseg, hseg, seg_bytes, hipMemcpyHostToDevice, s), "upload fetch segments");
+        hipCheck(cq_launch_copy_segments(dseg, static_cast<int>(segs.size()), reinterpret_cast<unsigned long long*>(dbase), s),
+                 "gather results");
+        hipCheck(hipMemcpyAsync(host, dbase, fetch_bytes, hipMemcpyDeviceToHost, s), "fetch results");
+    }
+    for (int k = 0; k < n; ++k) runs[static_cast<size_t>(k)].fetch_host = host + off[static_cast<size_t>(k)];
     hipCheck(hipStreamSynchronize(s), "fetch sync");
     for (int k = 0; k < n; ++k) {
         QueryRun& rr = runs[static_cast<size_t>(k)];
@@ -596,7 +614,7 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
                 B.valid[u] = reinterpret_cast<const unsigned int*>(c.validity);
             }
             B.qp = dqp;
-            int at = 0;
+                    int at = 0;
             for (size_t t = 0; t < templates.size(); ++t) {
                 B.tq_begin[t] = at;
                 at += static_cast<int>(per_template[t].size());

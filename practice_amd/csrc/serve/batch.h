@@ -26,7 +26,7 @@ namespace db {
 
 struct BatchOptions {
     int block_threads = 1024;     // cq_batch_scan block size (64..1024)
-    int vectors_per_thread = 2;   // 128-bit loads per column per lane per tile (1..4)
+    int vectors_per_thread = 1;   // 128-bit loads per column per lane per tile (1..4)
     bool graph = true;            // capture / replay the device phases as a HIP graph
     bool autotune = false;        // time block / load-width variants, keep the fastest
     int max_templates = 16;       // templates per shared scan
@@ -90,7 +90,7 @@ private:
         hipGraphExec_t exec = nullptr;
     };
     struct Tuned {
-        int block_threads = 1024, vectors_per_thread = 2;
+        int block_threads = 1024, vectors_per_thread = 1;
     };
     DatabaseInstance& db_;
     ExecutionContext* ctx_;

@@ -27,6 +27,9 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "window_us") o.window_us = static_cast<int>(v);
         else if (k == "async_compile") o.async_compile = v != 0;
         else if (k == "interpret") o.interpret_cold = v != 0;
+        else if (k == "cold_lanes") o.cold_lanes = static_cast<int>(v);
+        else if (k == "per_template") o.per_template = v != 0;
+        else if (k == "max_hot") o.max_hot = static_cast<int>(v);
         else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb") batch_opts += (batch_opts.empty() ? "" : ",") + item;
         else throw std::runtime_error("unknown server option: " + k);
     }
@@ -36,6 +39,8 @@ ServerOptions parseServerOptions(const std::string& text) {
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
     o.promote_after = std::max(1, o.promote_after);
     o.compile_threads = std::max(1, std::min(o.compile_threads, 16));
+    if (o.cold_lanes < 1 || o.cold_lanes > 8) throw std::runtime_error("server cold_lanes must be in [1, 8]");
+    o.max_hot = std::max(1, o.max_hot);
     return o;
 }
 
@@ -60,9 +65,18 @@ void QueryServer::start() {
         lane->exec = std::make_unique<BatchExecutor>(db_, lane->ctx.get(), &lane->mu);
         lanes_.push_back(std::move(lane));
     }
+    for (int i = 0; i < opt_.cold_lanes; ++i) {
+        auto lane = std::make_unique<Lane>();
+        lane->ctx = db_.newContext();
+        cold_lanes_.push_back(std::move(lane));
+    }
     for (auto& l : lanes_) {
         Lane* lp = l.get();
         l->thread = std::thread([this, lp] { laneLoop(lp); });
+    }
+    for (auto& l : cold_lanes_) {
+        Lane* lp = l.get();
+        l->thread = std::thread([this, lp] { coldLoop(lp); });
     }
     if (opt_.async_compile)
         for (int i = 0; i < opt_.compile_threads; ++i) compilers_.emplace_back([this] { compileLoop(); });
@@ -75,13 +89,16 @@ QueryServer::~QueryServer() {
     }
     cv_.notify_all();
     compile_cv_.notify_all();
+    cold_cv_.notify_all();
     for (auto& l : lanes_)
+        if (l->thread.joinable()) l->thread.join();
+    for (auto& l : cold_lanes_)
         if (l->thread.joinable()) l->thread.join();
     for (auto& t : compilers_)
         if (t.joinable()) t.join();
     // requests still queued when the server stops fail
     std::lock_guard<std::mutex> lock(mu_);
-    for (auto* q : {&queue_, &compile_q_, &plain_ready_}) {
+    for (auto* q : {&queue_, &compile_q_, &plain_ready_, &cold_q_}) {
         for (auto& r : *q) {
             Done d;
             d.rc = -3;
@@ -94,15 +111,30 @@ QueryServer::~QueryServer() {
 }
 
 long long QueryServer::submit(const std::string& sql) {
-    std::lock_guard<std::mutex> lock(mu_);
-    if (stop_) throw std::runtime_error("server stopped");
     Request r;
-    r.ticket = next_++;
     r.sql = sql;
     r.arrival = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        if (stop_) throw std::runtime_error("server stopped");
+        r.ticket = next_++;
+    }
+    // plan and code generation on the caller's thread, off the lanes; a
+    // request that fails here is already finished with its error
+    const long long ticket = r.ticket;
+    if (!prepare(r)) return ticket;
+    std::lock_guard<std::mutex> lock(mu_);
+    if (stop_) {  // stopped while this request was being planned
+        Done d;
+        d.rc = -3;
+        d.err = "server stopped";
+        done_[ticket] = std::move(d);
+        done_cv_.notify_all();
+        return ticket;
+    }
     queue_.push_back(std::move(r));
     cv_.notify_one();
-    return next_ - 1;
+    return ticket;
 }
 
 void QueryServer::finish(Request& r, int rc, const std::string& err, QueryResult* result) {
@@ -244,7 +276,14 @@ void QueryServer::laneLoop(Lane* lane) {
             std::vector<QueryResult> results;
             std::string err;
             try {
-                results = lane->exec->executePrograms(programs, opt_.batch, &templates);
+                if (opt_.per_template) {
+                    // one launch per signature, each on its own (precompiled) module
+                    BatchOptions bo = opt_.batch;
+                    bo.max_templates = 1;
+                    results = lane->exec->executePrograms(programs, bo, nullptr);
+                } else {
+                    results = lane->exec->executePrograms(programs, opt_.batch, &templates);
+                }
             } catch (const std::exception& e) {
                 err = e.what();
             }
@@ -254,48 +293,59 @@ void QueryServer::laneLoop(Lane* lane) {
                 else finish(batch[i], -3, err, nullptr);
             }
         }
-        for (auto& r : interp) {
-            // cold signature: its own scan on the interpreted tier, no compile
-            released += r.footprint;
-            try {
-                std::lock_guard<std::mutex> lock(lane->mu);
-                ExecutionContext* ctx = lane->ctx.get();
-                ctx->row_begin = 0;
-                ctx->row_end = -1;
-                ctx->config.codegen.vm = true;
-                db_.engine().executeQuery(r.sql, ctx);
-                ctx->config.codegen.vm = false;
-                QueryResult res = std::move(ctx->result);
-                finish(r, 0, "", &res);
-            } catch (const std::exception& e) {
-                lane->ctx->config.codegen.vm = false;
-                finish(r, -3, e.what(), nullptr);
-            }
-        }
-        for (auto& r : plain) {
-            // cold signature: its own scan on the plain path (module compiled)
-            released += r.footprint;
-            try {
-                std::lock_guard<std::mutex> lock(lane->mu);
-                ExecutionContext* ctx = lane->ctx.get();
-                ctx->row_begin = 0;
-                ctx->row_end = -1;
-                db_.engine().executeQuery(r.sql, ctx);
-                QueryResult res = std::move(ctx->result);
-                finish(r, 0, "", &res);
-            } catch (const std::exception& e) {
-                finish(r, -3, e.what(), nullptr);
-            }
-        }
         {
+            // cold requests (footprint already reserved) run on the cold
+            // lanes, concurrently with this lane's batches
             std::lock_guard<std::mutex> lock(mu_);
+            for (auto& r : interp) {
+                r.interpreted = true;
+                cold_q_.push_back(std::move(r));
+            }
+            for (auto& r : plain) {
+                r.interpreted = false;
+                cold_q_.push_back(std::move(r));
+            }
             in_flight_bytes_ -= released;
             if (!batch.empty()) {
                 ++batches_;
                 batched_queries_ += static_cast<long long>(batch.size());
             }
-            plain_queries_ += static_cast<long long>(plain.size());
-            interpreted_queries_ += static_cast<long long>(interp.size());
+        }
+        if (!interp.empty() || !plain.empty()) cold_cv_.notify_all();
+        cv_.notify_all();
+    }
+}
+
+void QueryServer::coldLoop(Lane* lane) {
+    for (;;) {
+        Request r;
+        {
+            std::unique_lock<std::mutex> lock(mu_);
+            cold_cv_.wait(lock, [&] { return stop_ || !cold_q_.empty(); });
+            if (stop_) return;
+            r = std::move(cold_q_.front());
+            cold_q_.pop_front();
+        }
+        // a cold signature: its own scan on the interpreted tier (no
+        // compilation) or on its plain module once that is compiled
+        ExecutionContext* ctx = lane->ctx.get();
+        try {
+            ctx->row_begin = 0;
+            ctx->row_end = -1;
+            ctx->config.codegen.vm = r.interpreted;
+            db_.engine().executeQuery(r.sql, ctx);
+            ctx->config.codegen.vm = false;
+            QueryResult res = std::move(ctx->result);
+            finish(r, 0, "", &res);
+        } catch (const std::exception& e) {
+            ctx->config.codegen.vm = false;
+            finish(r, -3, e.what(), nullptr);
+        }
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            in_flight_bytes_ -= r.footprint;
+            if (r.interpreted) ++interpreted_queries_;
+            else ++plain_queries_;
         }
         cv_.notify_all();
     }
@@ -339,6 +389,30 @@ bool QueryServer::compileOne() {
         } catch (const std::exception& e) {
             finish(r, -2, e.what(), nullptr);
             return true;
+        }
+        cv_.notify_all();
+        return true;
+    }
+    if (opt_.per_template) {
+        // promotion: the template's module and its own shared-scan module
+        try {
+            db_.engine().moduleFor(*promote, nullptr);
+            std::vector<const QueryProgram*> tp{promote.get()};
+            std::vector<std::string> cols;
+            std::vector<bool> nullable;
+            const std::string src =
+                generateBatchScanSource(tp, opt_.batch.block_threads, opt_.batch.vectors_per_thread, cols, nullable);
+            db_.engine().moduleForSource(src, signatureOf(src), nullptr);
+            std::lock_guard<std::mutex> lock(mu_);
+            if (!inScanSet(promote->signature)) {
+                if (static_cast<int>(scan_set_.size()) >= opt_.max_hot) scan_set_.erase(scan_set_.begin());
+                scan_set_.push_back(promote);
+                ++promotions_;
+            }
+            promoting_.erase(promote->signature);
+        } catch (const std::exception&) {
+            std::lock_guard<std::mutex> lock(mu_);
+            promoting_.erase(promote->signature);  // stays on the cold tier
         }
         cv_.notify_all();
         return true;

@@ -8,13 +8,16 @@
 // arrived meanwhile, so the lanes' work overlaps on the device.
 //
 // Execution tiers per plan signature:
-//   hot    signatures in the compiled shared-scan set run as one batch
-//          (cq_batch_scan: one pass over the fact columns for all of them)
-//   cold   a signature not in the set runs at once on the ahead-of-time
-//          interpreter (no compilation on the request's path); once seen
-//          `promote_after` times it is compiled into the shared-scan set
-//          on a background thread.  With interpret_cold off, cold requests
-//          wait for their plain module instead.
+//   hot    promoted signatures: the requests of one signature in a batch
+//          share one cq_batch_scan launch (one pass over its fact columns
+//          for all of them; per_template) or all hot signatures share one
+//          launch of a module holding the whole hot set
+//   cold   a signature not yet hot runs at once on the ahead-of-time
+//          interpreter on a cold lane (no compilation on the request's
+//          path, no wait behind hot batches); once seen `promote_after`
+//          times its modules are compiled on a background thread and it
+//          turns hot.  With interpret_cold off, cold requests wait for their
+//          plain module instead.
 //
 // Admission: a request's device footprint (hash tables, aggregate slots,
 // materialization buffers; estimateQueryTemporaryBytes) is reserved against
@@ -46,12 +49,18 @@ struct ServerOptions {
     int window_us = 0;             // wait for more arrivals before taking a batch
     bool async_compile = true;     // compile off the lanes
     bool interpret_cold = true;    // run new signatures at once on the interpreted tier
+    int cold_lanes = 1;            // lanes (own stream each) running cold requests one at a time
+    // hot signatures each get their own shared-scan module and launch (a
+    // batch runs one launch per signature present); off: one module holds
+    // the whole hot set (up to max_templates) and a batch is one launch
+    bool per_template = true;
+    int max_hot = 64;              // hot signatures kept (least recently promoted dropped)
     BatchOptions batch;
 };
 
 // "key=value,..." with lanes, max_batch, max_templates, promote_after,
-// compile_threads, budget_mb, window_us, async_compile, interpret, bt, vpt,
-// graph
+// compile_threads, budget_mb, window_us, async_compile, interpret,
+// cold_lanes, per_template, max_hot, bt, vpt, graph, lds_kb
 ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
@@ -75,6 +84,7 @@ private:
         std::chrono::steady_clock::time_point arrival;
         std::shared_ptr<QueryProgram> program;   // prepared with the per-tile body
         size_t footprint = 0;
+        bool interpreted = false;                 // cold request: interpreted tier (else its plain module)
     };
     struct Done {
         int rc = 0;
@@ -97,6 +107,8 @@ private:
     std::deque<Request> plain_ready_;   // cold requests whose plain module is compiled
     std::deque<Request> compile_q_;     // cold requests waiting for their plain module
     std::deque<std::shared_ptr<QueryProgram>> promote_q_;  // signatures to add to the shared scan
+    std::deque<Request> cold_q_;        // admitted cold requests for the cold lanes
+    std::condition_variable cold_cv_;
     std::map<std::string, int> seen_;   // requests per signature
     std::set<std::string> promoting_;
     std::map<long long, Done> done_;
@@ -108,6 +120,7 @@ private:
     std::vector<std::shared_ptr<QueryProgram>> scan_set_;
 
     std::vector<std::unique_ptr<Lane>> lanes_;
+    std::vector<std::unique_ptr<Lane>> cold_lanes_;
     std::vector<std::thread> compilers_;
 
     // statistics
@@ -118,6 +131,7 @@ private:
 
     void start();
     void laneLoop(Lane* lane);
+    void coldLoop(Lane* lane);
     void compileLoop();
     bool compileOne();  // one queued compilation job; false when there is none
     void finish(Request& r, int rc, const std::string& err, QueryResult* result);

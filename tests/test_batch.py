@@ -47,8 +47,12 @@ def test_shared_scan_loads_each_column_once(hdb):
     qs = [Q.C1, Q.C2, Q.c2(3, 4), Q.C3]
     src = hdb.generate_batch_code(qs)
     # union of fact columns: C1 {orderdate, discount, quantity, extendedprice},
-    # C2 {orderdate, partkey, suppkey, revenue}, C3 {custkey, suppkey, orderdate, revenue}
+    # C2 {orderdate, partkey, suppkey, revenue}, C3 {custkey, suppkey, orderdate, revenue}:
+    # each of the 8 is streamed from HBM once per tile (nontemporal load by the
+    # last template that reads it); the 4 earlier reads of shared columns are
+    # cached loads of the same tile
     assert src.count("__builtin_nontemporal_load") == 8
+    assert src.count("*((const v4i*)") == 4
     # three templates (the two C2 queries share one), one block each per tile
     assert src.count("        // template ") == 3
     pa.compile_source(src)
