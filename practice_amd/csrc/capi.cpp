@@ -237,6 +237,8 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "prefetch") c.codegen.prefetch = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
+        else if (n == "host_sort_rows") c.host_sort_rows = v;
+        else if (n == "prepared_cache") c.prepared_cache = v != 0;
         else return fail(CQ_ERR_INVALID, "unknown option " + n);
         return CQ_OK;
     });
@@ -302,7 +304,9 @@ int cq_db_describe(cq_db* d, const char* sql, char* buf, size_t len, size_t* nee
               << ",\"tag_bits\":" << (jp.dense ? 0 : jp.tb) << ",\"absent_code\":" << (jp.dense ? jp.doff : 1)
               << ",\"lds\":" << (jp.lds ? 1 : 0) << ",\"bytes\":" << jp.ht_words * 4 << "}";
         }
-        o << "],\"outputs\":[";
+        o << "],\"lits\":[";
+        for (size_t k = 0; k < p.lits.size(); ++k) o << (k ? "," : "") << p.lits[k];
+        o << "],\"has_limit\":" << (p.has_limit ? 1 : 0) << ",\"limit\":" << p.limit << ",\"outputs\":[";
         for (size_t c = 0; c < p.out.size(); ++c)
             o << (c ? "," : "") << "{\"type\":" << static_cast<int>(p.out[c].type) << ",\"nullable\":" << (p.out[c].nullable ? 1 : 0)
               << "}";

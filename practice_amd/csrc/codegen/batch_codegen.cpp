@@ -6,6 +6,7 @@
 // queries.  A template is a plan signature; its queries differ only in their
 // parameter blocks (constants, hash tables, result slots), so the module
 // depends on the set of templates, not on the batch size or constants.
+#include <algorithm>
 #include <map>
 #include <sstream>
 
@@ -30,7 +31,8 @@ std::string ldsRegion(const QueryProgram* p, size_t t, const std::string& ind) {
 }  // namespace
 
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt,
-                                    std::vector<std::string>& union_cols, std::vector<bool>& union_nullable) {
+                                    std::vector<std::string>& union_cols, std::vector<bool>& union_nullable,
+                                    unsigned* table_lds_bytes) {
     if (templates.empty()) throw std::runtime_error("batch without templates");
     if (templates.size() > 16) throw std::runtime_error("too many query templates in one batch (max 16)");
     union_cols.clear();
@@ -68,6 +70,54 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         }
         src << "    __syncthreads();\n";
     }
+    // Small shared dimension tables live in LDS for the whole launch (one copy
+    // per layout key: templates with equal keys share it), staged from the
+    // first template with queries that owns one; larger ones stay in HBM.
+    struct LdsTable {
+        std::string key;
+        uint64_t words = 0;
+        std::vector<std::pair<size_t, size_t>> users;  // (template, join)
+    };
+    std::vector<LdsTable> cands;
+    for (size_t t = 0; t < templates.size(); ++t) {
+        const QueryProgram* p = templates[t];
+        if (!p->shared_tables) continue;
+        for (size_t j = 0; j < p->joins.size(); ++j) {
+            const JoinProgram& jp = p->joins[j];
+            if (!jp.shared || jp.db == 0 || jp.s_key.empty() || 4 * jp.ht_words > kBatchTableLdsBytes) continue;
+            size_t k = 0;
+            while (k < cands.size() && cands[k].key != jp.s_key) ++k;
+            if (k == cands.size()) cands.push_back(LdsTable{jp.s_key, jp.ht_words, {}});
+            cands[k].users.emplace_back(t, j);
+        }
+    }
+    // most templates served first, then the smallest
+    std::stable_sort(cands.begin(), cands.end(), [](const LdsTable& x, const LdsTable& y) {
+        return x.users.size() != y.users.size() ? x.users.size() > y.users.size() : x.words < y.words;
+    });
+    std::vector<LdsTable> tabs;
+    uint64_t tab_bytes = 0;
+    std::map<std::pair<size_t, size_t>, size_t> tab_of;
+    for (auto& c : cands) {
+        if (tab_bytes + 4 * c.words > kBatchTableLdsBytes) continue;
+        tab_bytes += 4 * c.words;
+        for (const auto& u : c.users) tab_of[u] = tabs.size();
+        tabs.push_back(std::move(c));
+    }
+    for (size_t k = 0; k < tabs.size(); ++k) src << "    __shared__ u32 cq_tab" << k << "[" << tabs[k].words << "];\n";
+    for (size_t k = 0; k < tabs.size(); ++k) {
+        src << "    {\n        const u32* src = nullptr;\n";
+        for (size_t u = 0; u < tabs[k].users.size(); ++u) {
+            const size_t t = tabs[k].users[u].first, j = tabs[k].users[u].second;
+            src << "        " << (u ? "else " : "") << "if (B.tq_begin[" << t << "] < B.tq_end[" << t << "]) src = B.qp[B.tq_begin[" << t
+                << "]].ht[" << j << "];\n";
+        }
+        src << "        if (src)\n            for (int w = tid; w < " << tabs[k].words << "; w += BT) cq_tab" << k << "[w] = src[w];\n"
+            << "    }\n";
+    }
+    if (!tabs.empty()) src << "    __syncthreads();\n";
+    if (table_lds_bytes) *table_lds_bytes = static_cast<unsigned>(tab_bytes);
+
     // the last template (kernel order) that reads each union column: its
     // load is nontemporal, earlier templates' loads keep the tile in cache
     std::map<std::string, size_t> last_use;
@@ -117,8 +167,15 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         }
         src << "            }\n"
             << "        }\n";
-        if (!p->batch_shared.empty())
-            src << "        const cq_query_params& P0 = B.qp[B.tq_begin[" << t << "]];\n" << p->batch_shared;
+        if (!p->batch_shared.empty()) {
+            src << "        const cq_query_params& P0 = B.qp[B.tq_begin[" << t << "]];\n";
+            for (size_t j = 0; j < p->joins.size(); ++j) {
+                const auto it = tab_of.find({t, j});
+                src << "        const u32* cq_sht" << j << " = " << (it != tab_of.end() ? "cq_tab" + std::to_string(it->second) : "P0.ht[" + std::to_string(j) + "]")
+                    << ";\n";
+            }
+            src << p->batch_shared;
+        }
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
             << "            const cq_query_params& P = B.qp[q];\n";
         if (p->batch_lds_stride > 0) src << ldsRegion(p, t, "            ");

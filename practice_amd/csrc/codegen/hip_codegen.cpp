@@ -1425,6 +1425,11 @@ private:
             jp.db = pow2Width(bits);
             jp.doff = jp.s_present;
             jp.ht_words = denseWords(jp, jp.db);
+            // tables of equal key: equal contents (every row, packed attributes)
+            std::ostringstream k;
+            k << jp.dim_table << "|" << jp.pk_col << "|" << jp.dmin << "|" << jp.dspan << "|" << jp.db << "|" << jp.s_present;
+            for (size_t a = 0; a < jp.s_cols.size(); ++a) k << "|" << jp.s_cols[a] << ":" << jp.s_min[a] << ":" << jp.s_bits[a] << ":" << jp.s_off[a];
+            jp.s_key = k.str();
         }
         prog_.block_threads = opt_.block_threads > 0 ? opt_.block_threads : 256;
         prog_.vectors_per_thread = std::max(1, std::min(8, opt_.vectors_per_thread));
@@ -1469,7 +1474,7 @@ private:
             if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
             sh << "            const int fk = c" << fs << "[i];\n";
             auto entry = [&](const std::string& x) {
-                return jp.db == 0 ? std::string("0u") : "chip::dense_entry_aligned(P0.ht[" + J + "], " + x + ", " + std::to_string(jp.db) + ")";
+                return jp.db == 0 ? std::string("0u") : "chip::dense_entry_aligned(cq_sht" + J + ", " + x + ", " + std::to_string(jp.db) + ")";
             };
             if (jp.zmode != 0) {
                 std::string ze = "0u";
@@ -1887,27 +1892,40 @@ private:
         }
         pro << "    for (int k = tid; k < " << G * nas << "; k += BT) s_agg[k] = " << init << ";\n"
             << "    for (int k = tid; k < " << FW << "; k += BT) s_flag[k] = 0u;\n";
-        code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n#pragma unroll\n"
-             << "        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!" << flGet() << ") continue;\n"
-             << "            const int h = " << groupHash() << ";\n"
-             << "            if ((unsigned)h >= " << G << "u) continue;\n"
-             << "            u32 want = 1u;\n";
         const auto* a = agg_;
+        std::function<std::string(int)> keyValid = [&](int g) { return vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()); };
+        std::function<std::string(int)> aggValid = [&](int i) { return aggRow(i).valid; };
+        std::function<std::string(int)> aggValue = [&](int i) { return aggRow(i).value; };
+        if (batch_form) {
+            // batched: live rows are sparse, one queued row per lane per round
+            code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n";
+            const RowQueue rq = emitRowQueue(code);
+            keyValid = rq.keyValid;
+            aggValid = rq.aggValid;
+            aggValue = rq.aggValue;
+            code << "            {\n"
+                 << "            u32 want = 1u;\n";
+        } else {
+            code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n#pragma unroll\n"
+                 << "        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            if (!" << flGet() << ") continue;\n"
+                 << "            const int h = " << groupHash() << ";\n"
+                 << "            if ((unsigned)h >= " << G << "u) continue;\n"
+                 << "            u32 want = 1u;\n";
+        }
         for (int g : flag_dyn_) {
             if (g >= ng) continue;
-            code << "            if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") want |= 1u << " << dynBit(g)
-                 << ";\n";
+            code << "            if (" << keyValid(g) << ") want |= 1u << " << dynBit(g) << ";\n";
         }
         for (int i = 0; i < prog_.naggs; ++i) {
-            const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             const int slot = ng + i;
             const std::string ref = "s_agg[h * " + std::to_string(nas) + " + " + std::to_string(slot - ng) + "]";
-            code << "            if (" << r.valid << ") {\n";
-            if (f == "MIN") code << "                atomicMin(&" << ref << ", " << r.value << ");\n";
-            else if (f == "MAX") code << "                atomicMax(&" << ref << ", " << r.value << ");\n";
-            else code << "                atomicAdd(&" << ref << ", " << r.value << ");\n";
+            const std::string value = aggValue(i);
+            code << "            if (" << aggValid(i) << ") {\n";
+            if (f == "MIN") code << "                atomicMin(&" << ref << ", " << value << ");\n";
+            else if (f == "MAX") code << "                atomicMax(&" << ref << ", " << value << ");\n";
+            else code << "                atomicAdd(&" << ref << ", " << value << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
             if (hs >= 0) code << "                atomicAdd(&s_agg[h * " << nas << " + " << (hs - ng) << "], 1ull);\n";
             const int db = dynBit(slot);
@@ -1917,6 +1935,7 @@ private:
         code << "            { u32* fw = &s_flag[h >> " << lp << "]; const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
              << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n"
              << "        }\n";
+        if (batch_form) code << "        }\n";  // round loop
         // merge into the global slots once per block
         if (!batch_form) epi << "    __syncthreads();\n";
         epi << "    for (int g = tid; g < " << G << "; g += BT) {\n"
@@ -1985,6 +2004,16 @@ private:
     // per-row global aggregation statements (h / out in scope)
     void globalRow(std::ostringstream& code, const std::string& ind) {
         const auto* a = agg_;
+        globalRowWith(
+            code, ind, [&](int g) { return vexpr(a->group_by_exprs[static_cast<size_t>(g)].get()); },
+            [&](int g) { return vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()); },
+            [&](int i) { return aggRow(i).valid; }, [&](int i) { return aggRow(i).value; });
+    }
+    // the same with the row's group key values / validity and aggregate
+    // validity / values given as expressions (e.g. picked from item arrays)
+    void globalRowWith(std::ostringstream& code, const std::string& ind, const std::function<std::string(int)>& keyValue,
+                       const std::function<std::string(int)>& keyValid, const std::function<std::string(int)>& aggValid,
+                       const std::function<std::string(int)>& aggValue) {
         const int ng = prog_.ngroups;
         const bool decode = keysDecodable();
         auto setbit = [&](const std::string& idx) {
@@ -1994,27 +2023,25 @@ private:
             return std::find(flag_static_.begin(), flag_static_.end(), slot) != flag_static_.end();
         };
         for (int g = 0; g < ng; ++g) {
-            const ExprNode* e = a->group_by_exprs[static_cast<size_t>(g)].get();
             if (decode) {
                 if (isStatic(g)) continue;  // written by cq_keys
-                const std::string v = vvalid(e);
+                const std::string v = keyValid(g);
                 if (!litFalse(v)) code << ind << "if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
                 continue;
             }
-            code << ind << "P.result[out + " << g << "] = (u64)(" << vexpr(e) << ");\n";
+            code << ind << "P.result[out + " << g << "] = (u64)(" << keyValue(g) << ");\n";
             if (!prog_.fast_sparse) {
-                const std::string v = vvalid(e);
+                const std::string v = keyValid(g);
                 if (litTrue(v)) code << ind << setbit("out + " + std::to_string(g)) << "\n";
                 else if (!litFalse(v)) code << ind << "if (" << v << ") " << setbit("out + " + std::to_string(g)) << "\n";
             }
         }
         for (int i = 0; i < prog_.naggs; ++i) {
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
-            const AggRow r = aggRow(i);
             const int slot = ng + i;
-            code << ind << "if (" << r.valid << ") {\n";
+            code << ind << "if (" << aggValid(i) << ") {\n";
             const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
-            code << ind << "    " << atom << "(&P.result[out + " << slot << "], " << r.value << ");\n";
+            code << ind << "    " << atom << "(&P.result[out + " << slot << "], " << aggValue(i) << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
             if (hs >= 0) code << ind << "    atomicAdd(&P.result[out + " << hs << "], 1ull);\n";
             else if (f != "COUNT" && !prog_.fast_sparse && !(decode && isStatic(slot)))
@@ -2023,6 +2050,73 @@ private:
             if (f == "COUNT" && !prog_.fast_sparse && !decode) code << ind << setbit("out + " + std::to_string(slot)) << "\n";
         }
         if (decode) code << ind << setbit("out + " + std::to_string(flag_static_.front())) << "  // hit\n";
+    }
+
+    // Rows of one query are sparse after the joins: instead of running a
+    // (branchy) per-row aggregate update once per item slot with a few lanes
+    // active, every lane queues its live items and each round handles one
+    // item per lane (rounds = the most live items any lane holds, usually
+    // one).  Emits the queue and opens the round loop with the row's group
+    // index in `h` and accessors for its key / aggregate values; the caller
+    // closes the loop with "        }\n".
+    struct RowQueue {
+        std::function<std::string(int)> keyValue, keyValid, aggValid, aggValue;
+    };
+    RowQueue emitRowQueue(std::ostringstream& code) {
+        const auto* a = agg_;
+        const int ng = prog_.ngroups;
+        const int64_t G = prog_.groups;
+        const bool decode = keysDecodable();
+        std::vector<int> key_vals, key_valids;  // group columns whose value / validity the row provides
+        for (int g = 0; g < ng; ++g) {
+            const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
+            if (!litTrue(v) && !litFalse(v)) key_valids.push_back(g);
+            if (!decode) key_vals.push_back(g);
+        }
+        code << "        u32 pend = 0u;\n        int gh[IPT];\n";
+        for (int g : key_vals) code << "        u64 gkv" << g << "[IPT];\n";
+        for (int g : key_valids) code << "        u32 gkok" << g << " = 0u;\n";
+        for (int i = 0; i < prog_.naggs; ++i) code << "        u64 gav" << i << "[IPT]; u32 gaok" << i << " = 0u;\n";
+        code << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+             << "            const int h = " << groupHash() << ";\n"
+             << "            gh[i] = h;\n"
+             << "            if (" << flGet() << " && (unsigned)h < " << G << "u) pend |= 1u << i;\n";
+        for (int g : key_vals) code << "            gkv" << g << "[i] = (u64)(" << vexpr(a->group_by_exprs[static_cast<size_t>(g)].get()) << ");\n";
+        for (int g : key_valids)
+            code << "            if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") gkok" << g << " |= 1u << i;\n";
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const AggRow r = aggRow(i);
+            code << "            gav" << i << "[i] = " << r.value << ";\n";
+            if (litTrue(r.valid)) code << "            gaok" << i << " |= 1u << i;\n";
+            else if (!litFalse(r.valid)) code << "            if (" << r.valid << ") gaok" << i << " |= 1u << i;\n";
+        }
+        code << "        }\n"
+             << "        while (__ballot(pend != 0u) != 0ull) {\n"
+             << "            if (pend == 0u) continue;\n"
+             << "            const int it = __builtin_ctz(pend);\n"
+             << "            pend &= pend - 1u;\n"
+             << "            int h = 0;\n";
+        for (int g : key_vals) code << "            u64 kv" << g << " = 0ull;\n";
+        for (int i = 0; i < prog_.naggs; ++i) code << "            u64 av" << i << " = 0ull;\n";
+        code << "#pragma unroll\n            for (int j = 0; j < IPT; ++j)\n                if (j == it) {\n"
+             << "                    h = gh[j];\n";
+        for (int g : key_vals) code << "                    kv" << g << " = gkv" << g << "[j];\n";
+        for (int i = 0; i < prog_.naggs; ++i) code << "                    av" << i << " = gav" << i << "[j];\n";
+        code << "                }\n";
+        RowQueue q;
+        q.keyValue = [](int g) { return "kv" + std::to_string(g); };
+        q.aggValue = [](int i) { return "av" + std::to_string(i); };
+        q.keyValid = [this, a](int g) -> std::string {
+            const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
+            if (litTrue(v) || litFalse(v)) return v;
+            return "((gkok" + std::to_string(g) + " >> it) & 1u)";
+        };
+        q.aggValid = [this](int i) -> std::string {
+            const std::string v = aggRow(i).valid;
+            if (litTrue(v) || litFalse(v)) return v;
+            return "((gaok" + std::to_string(i) + " >> it) & 1u)";
+        };
+        return q;
     }
 
     // Batched queries whose dense domain does not fit their LDS region: an
@@ -2046,7 +2140,6 @@ private:
     uint64_t hashAggBytes() const { return (uint64_t(1) << kHashMinBits) * hashSlotBytes(); }
     void emitLdsHashAggregate(std::ostringstream& code, std::ostringstream& pro, std::ostringstream& epi) {
         const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
-        const int64_t G = prog_.groups;
         prog_.keys_kernel = !prog_.fast_sparse;
         // region: u32 keys[S], u32 flags[S], u64 vals[S * nas]
         std::string init = "0ull";
@@ -2057,12 +2150,11 @@ private:
         }
         pro << "    for (int k = tid; k < (2 << cq_hbits); k += BT) ((u32*)s_agg)[k] = 0u;\n"
             << "    for (int k = tid; k < (" << nas << " << cq_hbits); k += BT) s_agg[(1 << cq_hbits) + k] = " << init << ";\n";
-        code << "        // GROUP BY: LDS hash table of this query, global atomics on overflow\n#pragma unroll\n"
-             << "        for (int i = 0; i < IPT; ++i) {\n"
-             << "            if (!" << flGet() << ") continue;\n"
-             << "            const int h = " << groupHash() << ";\n"
-             << "            if ((unsigned)h >= " << G << "u) continue;\n"
-             << "            u32* s_hk = (u32*)s_agg;\n"
+        code << "        // GROUP BY: LDS hash table of this query, global atomics on overflow\n";
+        const RowQueue rq = emitRowQueue(code);
+        const auto& keyValidAt = rq.keyValid;
+        const auto& aggValidAt = rq.aggValid;
+        code << "            u32* s_hk = (u32*)s_agg;\n"
              << "            u32* s_hf = s_hk + (1u << cq_hbits);\n"
              << "            u64* s_hv = s_agg + (1u << cq_hbits);\n"
              << "            u32 slot = ((u32)h * 2654435761u) >> (32u - cq_hbits);\n"
@@ -2075,20 +2167,17 @@ private:
              << "            }\n"
              << "            if (found >= 0) {\n"
              << "                u32 want = 1u;\n";
-        const auto* a = agg_;
         for (int g : flag_dyn_) {
             if (g >= ng || prog_.fast_sparse) continue;
-            code << "                if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") want |= 1u << " << dynBit(g)
-                 << ";\n";
+            code << "                if (" << keyValidAt(g) << ") want |= 1u << " << dynBit(g) << ";\n";
         }
         for (int i = 0; i < prog_.naggs; ++i) {
-            const AggRow r = aggRow(i);
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             const std::string ref = "s_hv[found * " + std::to_string(nas) + " + " + std::to_string(i) + "]";
-            code << "                if (" << r.valid << ") {\n";
-            if (f == "MIN") code << "                    atomicMin(&" << ref << ", " << r.value << ");\n";
-            else if (f == "MAX") code << "                    atomicMax(&" << ref << ", " << r.value << ");\n";
-            else code << "                    atomicAdd(&" << ref << ", " << r.value << ");\n";
+            code << "                if (" << aggValidAt(i) << ") {\n";
+            if (f == "MIN") code << "                    atomicMin(&" << ref << ", av" << i << ");\n";
+            else if (f == "MAX") code << "                    atomicMax(&" << ref << ", av" << i << ");\n";
+            else code << "                    atomicAdd(&" << ref << ", av" << i << ");\n";
             const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
             if (hs >= 0) code << "                    atomicAdd(&s_hv[found * " << nas << " + " << (hs - ng) << "], 1ull);\n";
             const int db = prog_.fast_sparse ? -1 : dynBit(ng + i);
@@ -2098,7 +2187,7 @@ private:
         code << "                if ((s_hf[found] & want) != want) atomicOr(&s_hf[found], want);\n"
              << "            } else {\n"
              << "                const long long out = (long long)h * " << sts << ";\n";
-        globalRow(code, "                ");
+        globalRowWith(code, "                ", rq.keyValue, keyValidAt, aggValidAt, rq.aggValue);
         code << "            }\n"
              << "        }\n";
         // merge: aggregates (and sparse keys) of every used slot

@@ -71,6 +71,9 @@ struct JoinProgram {
     std::vector<int> s_bits, s_off;
     int s_present = 0;        // 1: bit 0 marks an inserted key
     int s_payload = -1;       // attribute index of the payload column
+    std::string s_key;        // layout key: tables with equal keys hold equal entries
+    // (the shared probe reads the table through `cq_sht<j>`, which the batch
+    // kernel points at LDS or at the table in HBM)
     bool lds = false;
     uint64_t ht_words = 0;    // u32 words of the table
 };
@@ -151,9 +154,12 @@ struct QueryProgram {
 // its queries (a runtime loop over parameter blocks).  Templates must have
 // been generated with CodegenOptions::batch_body.  `union_cols` receives the
 // union slot order and `union_nullable` which of them carry validity.
+// `table_lds_bytes` (optional) receives the static LDS of the shared
+// dimension tables staged into LDS (at most kBatchTableLdsBytes).
+constexpr unsigned kBatchTableLdsBytes = 32 * 1024;
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int block_threads,
                                     int vectors_per_thread, std::vector<std::string>& union_cols,
-                                    std::vector<bool>& union_nullable);
+                                    std::vector<bool>& union_nullable, unsigned* table_lds_bytes = nullptr);
 
 // Visitor entry points as in the reference: AggregateNode / SortLimitNode
 // roots start generation (visitor.cpp:1589-1621); projections are rejected.

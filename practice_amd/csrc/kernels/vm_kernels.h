@@ -14,9 +14,10 @@ hipError_t cq_vm_launch_scan(const cq_query_params* P, const cq_vm_program* V, u
 hipError_t cq_vm_launch_dense_count(const cq_query_params* P, const cq_vm_program* V, const unsigned long long* R,
                                     const unsigned long long* RV, unsigned long long* counts, long long G, hipStream_t s);
 int cq_vm_block_threads();
-// 4 rows per thread, registers in VGPRs: programs with at most 16 registers
-// whose scan uses no build / HAVING instructions (V: host copy)
+// 4 rows per thread: programs with at most 16 registers whose scan uses no
+// build / HAVING instructions (V: host copy)
 bool cq_vm_scan4_supported(const cq_vm_program* V);
-// row_begin must be a multiple of 4; the grid fills the device
-hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, unsigned lds_group_bytes, int device,
+// row_begin must be a multiple of 4; nregs: the program's register count;
+// the grid fills the device
+hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,
                               hipStream_t s);

@@ -425,45 +425,57 @@ __global__ __launch_bounds__(VM_BT) void k_vm_dense_count(const cq_query_params 
 
 // ---------------------------------------------------------------- 4-row scan
 // The same interpreter with 4 consecutive rows per thread (one 16-byte load
-// per column) and the register file in VGPRs: one array of VM4_REGS
-// registers per row, indexed by the instruction's wave-uniform register
-// numbers (relative VGPR addressing, no LDS round trips).  Decoding and
-// dispatch are paid once per instruction for 4 rows.  Programs with more
-// registers, or with instructions that only build / HAVING programs use,
-// run on k_vm_scan.
-constexpr int VM4_BT = 256;
+// per column): decoding and dispatch (switches on the wave-uniform opcode
+// and types, resolved to straight-line code per case) are paid once per
+// instruction for 4 rows, and the 4 register accesses of an operand are
+// independent LDS operations.  The register file is dynamic LDS sized to
+// the program ([register][row][thread] u64, conflict-free), so occupancy
+// follows the program's register count.  Programs with more than VM4_REGS
+// registers, more scalar slots than VM_RACC, or instructions that only
+// build / HAVING programs use run on k_vm_scan.
+constexpr int VM4_BT = 128;
 constexpr int VM4_REGS = 16;
 
-#define VM4_EACH(stmt)                         \
-    {                                          \
-        {                                      \
-            u64* R = R0;                       \
-            const int k = 0;                   \
-            (void)k;                           \
-            stmt;                              \
-        }                                      \
-        {                                      \
-            u64* R = R1;                       \
-            const int k = 1;                   \
-            (void)k;                           \
-            stmt;                              \
-        }                                      \
-        {                                      \
-            u64* R = R2;                       \
-            const int k = 2;                   \
-            (void)k;                           \
-            stmt;                              \
-        }                                      \
-        {                                      \
-            u64* R = R3;                       \
-            const int k = 3;                   \
-            (void)k;                           \
-            stmt;                              \
-        }                                      \
+// register r of row k of this thread (RFt = RF + tid)
+struct VmReg {
+    u64* base;
+    __device__ __forceinline__ u64& operator[](int r) const { return base[r * (4 * VM4_BT)]; }
+};
+
+#define VM4_EACH(stmt)                                   \
+    {                                                    \
+        _Pragma("unroll") for (int k = 0; k < 4; ++k) {  \
+            const VmReg R{RFt + k * VM4_BT};             \
+            stmt;                                        \
+        }                                                \
     }
 
+// dispatch on a wave-uniform type to code with that type as a constant
+#define VM4_TYPES(tv, M)                      \
+    switch (tv) {                             \
+        case CQ_T_I32: M(CQ_T_I32) break;     \
+        case CQ_T_U32: M(CQ_T_U32) break;     \
+        case CQ_T_I64: M(CQ_T_I64) break;     \
+        case CQ_T_U64: M(CQ_T_U64) break;     \
+        default: M(CQ_T_F64) break;           \
+    }
+#define VM4_ARITH_ADD(T) VM4_EACH(R[d] = arith(CQ_OP_ADD, R[a], R[b], T))
+#define VM4_ARITH_SUB(T) VM4_EACH(R[d] = arith(CQ_OP_SUB, R[a], R[b], T))
+#define VM4_ARITH_MUL(T) VM4_EACH(R[d] = arith(CQ_OP_MUL, R[a], R[b], T))
+#define VM4_ARITH_DIV(T) VM4_EACH(R[d] = arith(CQ_OP_DIV, R[a], R[b], T))
+#define VM4_ARITH_MOD(T) VM4_EACH(R[d] = arith(CQ_OP_MOD, R[a], R[b], T))
+#define VM4_CMP_EQ(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_EQ, R[a], R[b], T))
+#define VM4_CMP_NE(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_NE, R[a], R[b], T))
+#define VM4_CMP_LT(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_LT, R[a], R[b], T))
+#define VM4_CMP_LE(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_LE, R[a], R[b], T))
+#define VM4_CMP_GT(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_GT, R[a], R[b], T))
+#define VM4_CMP_GE(T) VM4_EACH(R[d] = (u64)compare(CQ_OP_GE, R[a], R[b], T))
+#define VM4_CONV_FROM_F64(T) VM4_EACH(R[d] = conv(R[a], T, CQ_T_F64))
+#define VM4_CONV_TO_F64(T) VM4_EACH(R[d] = conv(R[a], CQ_T_F64, T))
+
 // one row's aggregation after the pipeline (k_vm_scan's epilogue per row)
-__device__ __forceinline__ void vm4_aggregate_row(const cq_query_params& P, const cq_vm_program* __restrict__ V, const u64* R,
+template <typename Regs>
+__device__ __forceinline__ void vm4_aggregate_row(const cq_query_params& P, const cq_vm_program* __restrict__ V, const Regs& R,
                                                   bool alive, int lds_groups, u64* s_agg, u32* s_flag, u64 (&racc)[VM_RACC],
                                                   u32& rseen) {
     const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
@@ -526,6 +538,8 @@ __device__ __forceinline__ void vm4_aggregate_row(const cq_query_params& P, cons
     }
 }
 
+// dynamic LDS: register file (nregs x 4 x VM4_BT u64), then the group
+// aggregates when lds_groups (G * nas u64 + G u32 flag words)
 __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
     __shared__ u64 s_acc[CQ_VM_MAX_AGGS * 2];
     __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
@@ -534,8 +548,9 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
     const int lane = tid & 63;
     const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
     const long long G = V->groups;
-    u64* s_agg = s_dyn;
-    u32* s_flag = (u32*)(s_dyn + (lds_groups ? G * nas : 0));
+    u64* RFt = s_dyn + tid;
+    u64* s_agg = s_dyn + (long long)V->nregs * 4 * VM4_BT;
+    u32* s_flag = (u32*)(s_agg + (lds_groups ? G * nas : 0));
     if (ng == 0) {
         for (int k = tid; k < sts; k += VM4_BT) {
             s_acc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
@@ -550,9 +565,7 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
     u32 rseen = 0u;
 #pragma unroll
     for (int k = 0; k < VM_RACC; ++k) racc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
-    u64 R0[VM4_REGS], R1[VM4_REGS], R2[VM4_REGS], R3[VM4_REGS];
-#pragma unroll
-    for (int r = 0; r < VM4_REGS; ++r) R0[r] = R1[r] = R2[r] = R3[r] = 0ull;
+    for (int r = 0; r < V->nregs; ++r) VM4_EACH(R[r] = 0ull)
     const long long nrows = P.row_end - P.row_begin;
     const int code_end = V->scan_end;
     for (long long base = (long long)blockIdx.x * (VM4_BT * 4); base < nrows; base += (long long)gridDim.x * (VM4_BT * 4)) {
@@ -571,10 +584,10 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                     const int* col = P.col[ins.imm];
                     if (full) {
                         const chip::v4i x = *(const chip::v4i*)(col + row0);
-                        R0[d] = (u64)(i64)x.x;
-                        R1[d] = (u64)(i64)x.y;
-                        R2[d] = (u64)(i64)x.z;
-                        R3[d] = (u64)(i64)x.w;
+                        RFt[(d * 4 + 0) * VM4_BT] = (u64)(i64)x.x;
+                        RFt[(d * 4 + 1) * VM4_BT] = (u64)(i64)x.y;
+                        RFt[(d * 4 + 2) * VM4_BT] = (u64)(i64)x.z;
+                        RFt[(d * 4 + 3) * VM4_BT] = (u64)(i64)x.w;
                     } else {
                         VM4_EACH(R[d] = in[k] ? (u64)(i64)col[row0 + k] : 0ull)
                     }
@@ -594,34 +607,92 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                     VM4_EACH(R[d] = v)
                     break;
                 }
-                case CQ_OP_CONV: VM4_EACH(R[d] = conv(R[a], t, t2)) break;
-                case CQ_OP_ADD: case CQ_OP_SUB: case CQ_OP_MUL: case CQ_OP_DIV: case CQ_OP_MOD: {
-                    const int op = ins.op;
-                    VM4_EACH(R[d] = arith(op, R[a], R[b], t))
+                // every op / type case below is straight-line code for its
+                // constant op and types (the helpers fold), so no lane
+                // evaluates another case's arithmetic
+                case CQ_OP_CONV:
+                    if (t2 == CQ_T_F64) {
+                        VM4_TYPES(t, VM4_CONV_FROM_F64)
+                    } else if (t == CQ_T_F64) {
+                        VM4_TYPES(t2, VM4_CONV_TO_F64)
+                    } else if (t == CQ_T_I32) {
+                        VM4_EACH(R[d] = canon((i64)R[a], CQ_T_I32))
+                    } else if (t == CQ_T_U32) {
+                        VM4_EACH(R[d] = canon((i64)R[a], CQ_T_U32))
+                    } else {
+                        VM4_EACH(R[d] = R[a])
+                    }
                     break;
-                }
-                case CQ_OP_EQ: case CQ_OP_NE: case CQ_OP_LT: case CQ_OP_LE: case CQ_OP_GT: case CQ_OP_GE: {
-                    const int op = ins.op;
-                    VM4_EACH(R[d] = (u64)compare(op, R[a], R[b], t))
+                case CQ_OP_ADD: VM4_TYPES(t, VM4_ARITH_ADD) break;
+                case CQ_OP_SUB: VM4_TYPES(t, VM4_ARITH_SUB) break;
+                case CQ_OP_MUL: VM4_TYPES(t, VM4_ARITH_MUL) break;
+                case CQ_OP_DIV: VM4_TYPES(t, VM4_ARITH_DIV) break;
+                case CQ_OP_MOD: VM4_TYPES(t, VM4_ARITH_MOD) break;
+                case CQ_OP_EQ: VM4_TYPES(t, VM4_CMP_EQ) break;
+                case CQ_OP_NE: VM4_TYPES(t, VM4_CMP_NE) break;
+                case CQ_OP_LT: VM4_TYPES(t, VM4_CMP_LT) break;
+                case CQ_OP_LE: VM4_TYPES(t, VM4_CMP_LE) break;
+                case CQ_OP_GT: VM4_TYPES(t, VM4_CMP_GT) break;
+                case CQ_OP_GE: VM4_TYPES(t, VM4_CMP_GE) break;
+                case CQ_OP_NOT:
+                    if (t == CQ_T_F64) VM4_EACH(R[d] = truthy(R[a], CQ_T_F64) ? 0ull : 1ull)
+                    else VM4_EACH(R[d] = R[a] != 0ull ? 0ull : 1ull)
                     break;
-                }
-                case CQ_OP_NOT: VM4_EACH(R[d] = truthy(R[a], t) ? 0ull : 1ull) break;
-                case CQ_OP_AND: VM4_EACH(R[d] = (truthy(R[a], t) && truthy(R[b], t2)) ? 1ull : 0ull) break;
-                case CQ_OP_OR: VM4_EACH(R[d] = (truthy(R[a], t) || truthy(R[b], t2)) ? 1ull : 0ull) break;
+                case CQ_OP_AND:
+                    if (t != CQ_T_F64 && t2 != CQ_T_F64) VM4_EACH(R[d] = (R[a] != 0ull && R[b] != 0ull) ? 1ull : 0ull)
+                    else VM4_EACH(R[d] = (truthy(R[a], t) && truthy(R[b], t2)) ? 1ull : 0ull)
+                    break;
+                case CQ_OP_OR:
+                    if (t != CQ_T_F64 && t2 != CQ_T_F64) VM4_EACH(R[d] = (R[a] != 0ull || R[b] != 0ull) ? 1ull : 0ull)
+                    else VM4_EACH(R[d] = (truthy(R[a], t) || truthy(R[b], t2)) ? 1ull : 0ull)
+                    break;
                 case CQ_OP_SEL: {
                     const int c = ins.c;
-                    VM4_EACH(R[d] = truthy(R[c], t2) ? R[a] : R[b])
+                    if (t2 != CQ_T_F64) VM4_EACH(R[d] = R[c] != 0ull ? R[a] : R[b])
+                    else VM4_EACH(R[d] = truthy(R[c], CQ_T_F64) ? R[a] : R[b])
                     break;
                 }
-                case CQ_OP_FILTER: VM4_EACH(alive[k] = alive[k] && truthy(R[a], t)) break;
+                case CQ_OP_FILTER:
+                    if (t != CQ_T_F64) VM4_EACH(alive[k] = alive[k] && R[a] != 0ull)
+                    else VM4_EACH(alive[k] = alive[k] && truthy(R[a], CQ_T_F64))
+                    break;
                 case CQ_OP_PROBE: {
                     const cq_vm_join& J = V->joins[ins.imm];
                     const u32* ht = P.ht[ins.imm];
-                    VM4_EACH({
-                        int pay = 0;
-                        if (alive[k]) alive[k] = probe(J, ht, (int)(i64)R[a], pay);
-                        R[d] = (u64)(i64)pay;
-                    })
+                    switch (J.layout) {
+                        case CQ_L_DENSE:
+                            if ((J.db & (J.db - 1)) == 0)
+                                VM4_EACH({
+                                    int pay = 0;
+                                    if (alive[k])
+                                        alive[k] = chip::ht_probe_dense<true>(ht, (int)(i64)R[a], J.dmin, J.dspan, J.db, J.pbase,
+                                                                              J.doff != 0, J.zmode, J.zx, pay);
+                                    R[d] = (u64)(i64)pay;
+                                })
+                            else
+                                VM4_EACH({
+                                    int pay = 0;
+                                    if (alive[k])
+                                        alive[k] = chip::ht_probe_dense<false>(ht, (int)(i64)R[a], J.dmin, J.dspan, J.db, J.pbase,
+                                                                               J.doff != 0, J.zmode, J.zx, pay);
+                                    R[d] = (u64)(i64)pay;
+                                })
+                            break;
+                        case CQ_L_COMPACT:
+                            VM4_EACH({
+                                int pay = 0;
+                                if (alive[k]) alive[k] = chip::ht_probe_compact(ht, (int)(i64)R[a], J.kmin, J.len, J.eb, J.tb, J.jlo, J.pmin, pay);
+                                R[d] = (u64)(i64)pay;
+                            })
+                            break;
+                        default:
+                            VM4_EACH({
+                                int pay = 0;
+                                if (alive[k]) alive[k] = chip::ht_probe_ref((const int*)ht, (int)(i64)R[a], J.kmin, J.len, J.payload_dslot >= 0, pay);
+                                R[d] = (u64)(i64)pay;
+                            })
+                            break;
+                    }
                     break;
                 }
                 case CQ_OP_EXIT:
@@ -630,10 +701,8 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                 default: break;  // build / HAVING instructions never reach this kernel
             }
         }
-        vm4_aggregate_row(P, V, R0, alive[0], lds_groups, s_agg, s_flag, racc, rseen);
-        vm4_aggregate_row(P, V, R1, alive[1], lds_groups, s_agg, s_flag, racc, rseen);
-        vm4_aggregate_row(P, V, R2, alive[2], lds_groups, s_agg, s_flag, racc, rseen);
-        vm4_aggregate_row(P, V, R3, alive[3], lds_groups, s_agg, s_flag, racc, rseen);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) vm4_aggregate_row(P, V, VmReg{RFt + k * VM4_BT}, alive[k], lds_groups, s_agg, s_flag, racc, rseen);
     }
     if (ng == 0) {
 #pragma unroll
@@ -714,7 +783,7 @@ hipError_t cq_vm_launch_dense_count(const cq_query_params* P, const cq_vm_progra
 int cq_vm_block_threads() { return VM_BT; }
 
 bool cq_vm_scan4_supported(const cq_vm_program* V) {
-    if (V->nregs > VM4_REGS || V->scan_begin < 0 || V->scan_end > V->ncode) return false;
+    if (V->nregs > VM4_REGS || V->nregs < 1 || V->scan_begin < 0 || V->scan_end > V->ncode) return false;
     // scalar aggregates: every slot in the per-thread registers
     if (V->ngroups == 0 && V->storage_ts > VM_RACC) return false;
     for (int pc = V->scan_begin; pc < V->scan_end; ++pc) {
@@ -724,17 +793,19 @@ bool cq_vm_scan4_supported(const cq_vm_program* V) {
     return true;
 }
 
-hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, unsigned lds_group_bytes, int device,
+hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,
                               hipStream_t s) {
     const long long rows = P->row_end - P->row_begin;
     if (rows <= 0) return hipSuccess;
+    const unsigned lds = (unsigned)nregs * 4u * VM4_BT * 8u + lds_group_bytes;
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_vm_scan4), VM4_BT,
-                                                     lds_group_bytes) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_vm_scan4), VM4_BT, lds) !=
+            hipSuccess ||
+        per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
     const long long tiles = (rows + VM4_BT * 4 - 1) / (VM4_BT * 4);
     const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, (long long)per_cu * cus));
-    hipLaunchKernelGGL(k_vm_scan4, dim3(grid), dim3(VM4_BT), lds_group_bytes, s, *P, V, lds_group_bytes ? 1 : 0);
+    hipLaunchKernelGGL(k_vm_scan4, dim3(grid), dim3(VM4_BT), lds, s, *P, V, lds_group_bytes ? 1 : 0);
     return hipGetLastError();
 }

@@ -392,6 +392,7 @@ QueryResult DatabaseInstance::executeQueryRange(const std::string& sql, int64_t 
 }
 
 QueryProgram DatabaseInstance::prepare(const std::string& sql) const {
+    if (ctx_->config.prepared_cache) return engine_->prepareCached(sql, ctx_->config.codegen);
     return engine_->prepare(sql, ctx_->config.codegen);
 }
 

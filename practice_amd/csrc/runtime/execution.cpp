@@ -4,6 +4,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -360,7 +361,7 @@ void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
         const long long rows = run.P.row_end - run.P.row_begin;
         if (rows <= 0) return;
         if (!ctx->config.vm_row_kernel && cq_vm_scan4_supported(p.vm.get()) && run.P.row_begin % 4 == 0) {
-            hipCheck(cq_vm_launch_scan4(&run.P, run.vm, static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
+            hipCheck(cq_vm_launch_scan4(&run.P, run.vm, p.vm->nregs, static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
                      "interpreted scan");
             return;
         }
@@ -452,7 +453,11 @@ void HipModuleExecutor::fetchSegments(QueryRun& run, ExecutionContext* ctx, unsi
     unsigned long long* o = run.o;
     unsigned long long* ov = run.ov;
     const long long cap_rows = run.cap_rows, vwords = run.vwords;
-    if (p.has_sort_node && !p.sort_keys.empty() && rows > 1) {
+    // small results are sorted on the host after the copy (same order as the
+    // device sort: a total order with the group index as last key), which
+    // saves the sort launches of every such query
+    run.host_sort = p.has_sort_node && !p.sort_keys.empty() && rows > 1 && rows <= ctx->config.host_sort_rows;
+    if (p.has_sort_node && !p.sort_keys.empty() && rows > 1 && !run.host_sort) {
         if (!run.sidx) {
             DeviceBuffer& ib = ctx->scratch(run.prefix + "sort_idx");
             DeviceBuffer& o2 = ctx->scratch(run.prefix + "out2");
@@ -480,7 +485,8 @@ void HipModuleExecutor::fetchSegments(QueryRun& run, ExecutionContext* ctx, unsi
         o = run.o2;
         ov = run.ov2;
     }
-    if (p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit) rows = static_cast<long long>(p.limit);
+    if (!run.host_sort && p.has_sort_node && p.has_limit && static_cast<unsigned long long>(rows) > p.limit)
+        rows = static_cast<long long>(p.limit);
     run.rows = rows;
     // host layout: [column][rows] values, then [column][vwords] validity words
     for (int c = 0; c < vts && rows; ++c)
@@ -529,6 +535,68 @@ void HipModuleExecutor::fetchFinish(QueryRun& run) {
         const uint64_t* w = vbits + static_cast<size_t>(c) * run.vwords;
         for (long long r = 0; r < rows; ++r) vv[static_cast<size_t>(r)] = static_cast<uint8_t>((w[r >> 6] >> (r & 63)) & 1ull);
     }
+    if (run.host_sort) sortHost(run);
+}
+
+// ORDER BY [LIMIT] of a fetched result on the host: the device sort's order
+// (k_bitonic_*: per key, a NULL sorts after values ascending, the type's
+// comparison, DESC flips it, the group index breaks ties)
+void HipModuleExecutor::sortHost(QueryRun& run) {
+    const QueryProgram& p = *run.program;
+    QueryResult& out = run.result;
+    const size_t rows = out.row_count;
+    std::vector<size_t> idx(rows);
+    for (size_t i = 0; i < rows; ++i) idx[i] = i;
+    const size_t nk = std::min<size_t>(p.sort_keys.size(), 16);
+    auto before = [&](size_t l, size_t r) {  // row l sorts before row r
+        for (size_t k = 0; k < nk; ++k) {
+            const size_t c = p.sort_keys[k].column_index;
+            const bool desc = p.sort_keys[k].descending;
+            if (p.out[c].nullable) {
+                const uint8_t lv = out.validity[c][l], rv = out.validity[c][r];
+                if (lv != rv) return lv != 0;  // valid first
+                if (!lv) continue;
+            }
+            const uint64_t a = out.data[c][l], b = out.data[c][r];
+            bool lt, gt;
+            switch (typeCode(p.out[c].type)) {
+                case CQ_TYPE_INT64:
+                    lt = static_cast<long long>(a) < static_cast<long long>(b);
+                    gt = static_cast<long long>(b) < static_cast<long long>(a);
+                    break;
+                case CQ_TYPE_FLOAT64: {
+                    double x, y;
+                    std::memcpy(&x, &a, 8);
+                    std::memcpy(&y, &b, 8);
+                    lt = x < y;
+                    gt = y < x;
+                    break;
+                }
+                default:
+                    lt = a < b;
+                    gt = b < a;
+                    break;
+            }
+            if (lt) return !desc;
+            if (gt) return desc;
+        }
+        return l < r;
+    };
+    std::sort(idx.begin(), idx.end(), before);
+    size_t keep = rows;
+    if (p.has_limit && keep > p.limit) keep = static_cast<size_t>(p.limit);
+    for (size_t c = 0; c < out.data.size(); ++c) {
+        std::vector<uint64_t> d(keep);
+        std::vector<uint8_t> v(keep);
+        for (size_t i = 0; i < keep; ++i) {
+            d[i] = out.data[c][idx[i]];
+            v[i] = out.validity[c][idx[i]];
+        }
+        out.data[c].swap(d);
+        out.validity[c].swap(v);
+    }
+    out.row_count = keep;
+    run.rows = static_cast<long long>(keep);
 }
 
 void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found) {
@@ -590,6 +658,117 @@ QueryEngine::QueryEngine(std::shared_ptr<IQueryCache> cache, std::shared_ptr<ICo
 QueryProgram QueryEngine::prepare(const std::string& sql, const CodegenOptions& options) const {
     auto tree = buildOptimizedTree(sql, catalog_.get());
     return generateQueryProgram(tree.get(), *catalog_, options);
+}
+
+namespace {
+
+// Integer literals of a statement (digit runs not attached to an
+// identifier, not part of a decimal number), and the statement with each
+// replaced by '?'.  Empty `values` and no '?' when the text has none.
+bool splitLiterals(const std::string& sql, std::string& shape, std::vector<long long>& values,
+                   std::vector<std::pair<size_t, size_t>>& spans) {
+    shape.clear();
+    values.clear();
+    spans.clear();
+    size_t i = 0;
+    const size_t n = sql.size();
+    bool in_string = false;
+    while (i < n) {
+        const char c = sql[i];
+        if (c == '\'') in_string = !in_string;
+        const bool ident_before = i > 0 && (std::isalnum(static_cast<unsigned char>(sql[i - 1])) || sql[i - 1] == '_' || sql[i - 1] == '.');
+        if (!in_string && std::isdigit(static_cast<unsigned char>(c)) && !ident_before) {
+            size_t j = i;
+            while (j < n && std::isdigit(static_cast<unsigned char>(sql[j]))) ++j;
+            if (j < n && (sql[j] == '.' || std::isalpha(static_cast<unsigned char>(sql[j])) || sql[j] == '_')) return false;
+            if (j - i > 18) return false;
+            values.push_back(std::stoll(sql.substr(i, j - i)));
+            spans.emplace_back(i, j);
+            shape += '?';
+            i = j;
+            continue;
+        }
+        shape += c;
+        ++i;
+    }
+    return !in_string;
+}
+
+std::string optionsKey(const CodegenOptions& o) {
+    std::ostringstream k;
+    k << o.block_threads << ',' << o.vectors_per_thread << ',' << o.lds_budget << ',' << o.allow_lds_tables << o.allow_lds_aggregation
+      << o.allow_compact_tables << o.allow_dense_tables << o.prefetch << o.batch_body << o.batch_shared_tables << o.vm;
+    return k.str();
+}
+
+}  // namespace
+
+QueryProgram QueryEngine::prepareCached(const std::string& sql, const CodegenOptions& options) {
+    std::string shape;
+    std::vector<long long> values;
+    std::vector<std::pair<size_t, size_t>> spans;
+    if (!splitLiterals(sql, shape, values, spans) || values.empty()) return prepare(sql, options);
+    const std::string key = optionsKey(options) + "|" + shape;
+    {
+        std::lock_guard<std::mutex> lock(prepared_mu_);
+        auto it = prepared_.find(key);
+        if (it != prepared_.end()) {
+            if (!it->second.cacheable) return prepare(sql, options);
+            QueryProgram p = *it->second.program;
+            for (size_t k = 0; k < p.lits.size(); ++k) {
+                const int li = it->second.slot_literal[k];
+                if (li >= 0) p.lits[k] = values[static_cast<size_t>(li)];
+            }
+            return p;
+        }
+    }
+    QueryProgram p = prepare(sql, options);
+    // plan again with marker constants: distinct values of the same C
+    // literal class (int / long long) as each original
+    PreparedShape entry;
+    try {
+        std::string marked;
+        std::vector<long long> markers;
+        size_t at = 0;
+        for (size_t k = 0; k < spans.size(); ++k) {
+            const long long v = values[k];
+            const long long m = v <= 2147483647ll ? 2000000011ll - 7919ll * static_cast<long long>(k)
+                                                  : 9000000000000000000ll - 7919ll * static_cast<long long>(k);
+            markers.push_back(m);
+            marked += sql.substr(at, spans[k].first - at) + std::to_string(m);
+            at = spans[k].second;
+        }
+        marked += sql.substr(at);
+        const QueryProgram q = prepare(marked, options);
+        bool ok = q.signature == p.signature && q.has_limit == p.has_limit && q.limit == p.limit && q.lits.size() == p.lits.size() &&
+                  q.flits == p.flits && static_cast<bool>(q.vm) == static_cast<bool>(p.vm);
+        if (ok && p.vm) ok = std::memcmp(p.vm.get(), q.vm.get(), sizeof(cq_vm_program)) == 0;
+        std::vector<int> slot(q.lits.size(), -1);
+        for (size_t k = 0; ok && k < q.lits.size(); ++k) {
+            int found = -1;
+            for (size_t m = 0; m < markers.size(); ++m)
+                if (q.lits[k] == markers[m]) found = static_cast<int>(m);
+            if (found < 0) {
+                ok = q.lits[k] == p.lits[k];  // a constant the statement does not vary
+            } else {
+                ok = p.lits[k] == values[static_cast<size_t>(found)];
+                slot[k] = found;
+            }
+        }
+        entry.cacheable = ok;
+        entry.slot_literal = std::move(slot);
+        if (ok) {
+            auto cached = std::make_shared<QueryProgram>(p);
+            cached->plan_sexpr.clear();
+            entry.program = std::move(cached);
+        }
+    } catch (const std::exception&) {
+        entry.cacheable = false;
+    }
+    std::lock_guard<std::mutex> lock(prepared_mu_);
+    if (prepared_.size() >= 4096) prepared_.clear();
+    prepared_[key] = std::move(entry);
+    return p;
 }
 
 ModuleHandle QueryEngine::moduleFor(const QueryProgram& program, QueryTimingStats* timing) {

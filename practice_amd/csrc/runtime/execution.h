@@ -92,6 +92,8 @@ struct EngineConfig {
     double memory_guard_reserve_fraction = 0.10;
     bool kernel_timing = false;          // record cq_scan duration with events
     bool vm_row_kernel = false;          // interpreted tier: always the one-row-per-thread scan
+    long long host_sort_rows = 4096;     // ORDER BY results up to this size are sorted on the host
+    bool prepared_cache = false;         // DatabaseInstance::prepare goes through QueryEngine::prepareCached
     CodegenOptions codegen;
     size_t module_cache_capacity = 256;  // compiled modules kept (LRU)
 };
@@ -235,6 +237,7 @@ struct QueryRun {
     unsigned long long *o = nullptr, *ov = nullptr, *fl = nullptr, *off = nullptr, *bsums = nullptr;
     unsigned long long *sidx = nullptr, *o2 = nullptr, *ov2 = nullptr;  // ORDER BY buffers
     long long rows = 0;              // result rows after LIMIT (fetchLaunch)
+    bool host_sort = false;          // ORDER BY of a small result: sorted (and limited) by fetchFinish
     char* fetch_host = nullptr;      // host staging of the result (fetchLaunch)
     QueryResult result;
 };
@@ -285,6 +288,7 @@ public:
     void fetchSegments(QueryRun& run, ExecutionContext* ctx, unsigned long long rows, char* host, unsigned long long host_word,
                        std::vector<cq_copy_seg>& segs);
     void fetchFinish(QueryRun& run);
+    void sortHost(QueryRun& run);
 };
 
 // Persistent grid for a scan kernel: occupancy x CUs, at most one block per tile.
@@ -302,6 +306,13 @@ public:
     void executeQuery(const std::string& sql, ExecutionContext* ctx);
     // plan -> program without executing (codegen text / signature / layout)
     QueryProgram prepare(const std::string& sql, const CodegenOptions& options) const;
+    // prepare() for statements that repeat with other integer constants:
+    // the SQL with its integer literals taken out is the key of a cached
+    // program whose constants are re-bound (lits only; plan_sexpr is left
+    // empty).  A statement is cached only when planning it with marker
+    // constants shows that every constant reaches the program through
+    // `lits` and nothing else depends on the values.
+    QueryProgram prepareCached(const std::string& sql, const CodegenOptions& options);
     // compile or fetch the module for a program
     ModuleHandle moduleFor(const QueryProgram& program, QueryTimingStats* timing);
     ModuleHandle moduleForSource(const std::string& source, const std::string& signature, QueryTimingStats* timing);
@@ -319,6 +330,15 @@ private:
     std::mutex compile_mu_;
     std::condition_variable compile_cv_;
     std::set<std::string> compiling_;
+    // prepareCached: normalized statement -> program and, per lits slot, the
+    // index of the statement's literal it takes (-1: fixed value)
+    struct PreparedShape {
+        std::shared_ptr<const QueryProgram> program;
+        std::vector<int> slot_literal;
+        bool cacheable = false;
+    };
+    std::mutex prepared_mu_;
+    std::unordered_map<std::string, PreparedShape> prepared_;
 };
 
 // Throws when the device cannot hold the query (preflightDeviceMemoryOrThrow).

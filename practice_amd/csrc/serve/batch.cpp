@@ -13,7 +13,11 @@ namespace db {
 
 namespace {
 constexpr int kHashMinBits = 7, kHashMaxBits = 10;  // per-query LDS hash tables: 128..1024 slots
-unsigned ldsLimit(const BatchOptions& opt) { return static_cast<unsigned>(opt.lds_kb) * 1024u; }
+// dynamic LDS for the per-query regions: what lds_kb allows, minus the room
+// kept for shared dimension tables staged into LDS
+unsigned ldsLimit(const BatchOptions& opt) {
+    return std::min(static_cast<unsigned>(opt.lds_kb) * 1024u, 160u * 1024u - kBatchTableLdsBytes);
+}
 }  // namespace
 
 BatchOptions parseBatchOptions(const std::string& text) {
@@ -123,7 +127,7 @@ std::vector<QueryResult> BatchExecutor::execute(const std::vector<std::string>& 
     std::vector<QueryProgram> programs(sqls.size());
     for (size_t i = 0; i < sqls.size(); ++i) {
         try {
-            programs[i] = db_.engine().prepare(sqls[i], co);
+            programs[i] = db_.engine().prepareCached(sqls[i], co);
         } catch (const std::exception& e) {
             throw std::runtime_error("batch query " + std::to_string(i) + ": " + e.what());
         }

@@ -157,7 +157,7 @@ bool QueryServer::prepare(Request& r) {
         co.batch_body = true;
         co.block_threads = bo.block_threads;
         co.vectors_per_thread = bo.vectors_per_thread;
-        r.program = std::make_shared<QueryProgram>(db_.engine().prepare(r.sql, co));
+        r.program = std::make_shared<QueryProgram>(db_.engine().prepareCached(r.sql, co));
         r.footprint = estimateQueryTemporaryBytes(*r.program);
         if (r.footprint > opt_.budget_bytes) {
             finish(r, -5, "query needs " + std::to_string(r.footprint) + " bytes of device memory, over the serving budget of " +

@@ -134,6 +134,21 @@ def test_extra_query_parity(sql):
     assert_same(g, o)
 
 
+@pytest.mark.parametrize("host_sort_rows", [0, 4096])
+def test_order_by_host_and_device_sort(host_sort_rows):
+    """ORDER BY [LIMIT] results: sorted by the device bitonic kernels (0) or,
+    for small results, on the host after the copy: the same order either way."""
+    spec, data, odb, gdb = dataset("sf1")
+    gdb.set_option("host_sort_rows", host_sort_rows)
+    try:
+        for sql in [q for q in EXTRA if "order by" in q] + [Q.C2]:
+            print(f"[sort] host_sort_rows={host_sort_rows} sql={sql}", flush=True)
+            g = gdb.execute(sql)
+            assert_same(g, odb.execute(gdb.explain(sql)), ordered=True)
+    finally:
+        gdb.set_option("host_sort_rows", 4096)
+
+
 def test_plan_signature_cache_shares_modules():
     spec, data, odb, gdb = dataset("sf20")
     gdb.execute(Q.c2(3, 2))

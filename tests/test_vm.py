@@ -21,8 +21,10 @@ HAVING = [
 
 
 def all_queries():
+    import random
     from tests.test_batch import mixed_queries
-    return mixed_queries() + HAVING
+    rng = random.Random(12)
+    return mixed_queries() + HAVING + [Q.adhoc_query(rng) for _ in range(8)]
 
 
 def test_every_parity_query_has_an_interpreted_program():
@@ -45,19 +47,24 @@ def test_interpreted_programs_keep_constants_as_arguments():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("row_kernel", [0, 1])
 @pytest.mark.parametrize("name", ["sf1", "sf100"])
-def test_interpreted_equals_compiled_and_oracle(name):
+def test_interpreted_equals_compiled_and_oracle(name, row_kernel):
+    """row_kernel 0: the 4-rows-per-thread interpreter where the program
+    fits it (the others fall back); 1: the one-row interpreter for all."""
     from tests.test_gpu_parity import dataset
     spec, data, odb, gdb = dataset(name)
     for q in all_queries():
-        print(f"[vm] dataset={name} sql={q}", flush=True)
+        print(f"[vm] dataset={name} row_kernel={row_kernel} sql={q}", flush=True)
         gdb.set_option("interpret", 0)
         jit = gdb.execute(q)
         gdb.set_option("interpret", 1)
+        gdb.set_option("vm_row_kernel", row_kernel)
         try:
             vm = gdb.execute(q)
         finally:
             gdb.set_option("interpret", 0)
+            gdb.set_option("vm_row_kernel", 0)
         assert_results_equal(vm, jit, q)
         assert_same(vm, odb.execute(gdb.explain(q)))
 
