@@ -16,6 +16,17 @@ namespace db {
 
 namespace {
 
+std::string withTemplate(const std::string& code, size_t t) {
+    std::string out;
+    const std::string tag = "@T@", idx = std::to_string(t);
+    size_t at = 0;
+    for (size_t k = code.find(tag); k != std::string::npos; k = code.find(tag, at)) {
+        out += code.substr(at, k - at) + idx;
+        at = k + tag.size();
+    }
+    return out + code.substr(at);
+}
+
 // pointers to query q's LDS aggregation region of template t (stride and
 // hash size set by the launch)
 std::string ldsRegion(const QueryProgram* p, size_t t, const std::string& ind) {
@@ -78,6 +89,23 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         uint64_t words = 0;
         std::vector<std::pair<size_t, size_t>> users;  // (template, join)
     };
+    // query-mask tables of mask-form templates first (they replace the
+    // per-query loop), within the same static LDS budget
+    uint64_t mask_bytes = 0;
+    std::vector<bool> use_mask(templates.size(), false);
+    for (size_t t = 0; t < templates.size(); ++t) {
+        const QueryProgram* p = templates[t];
+        if (!p->batch_mask) continue;
+        uint64_t b = 0;
+        for (const auto& tb : p->batch_mask_tables) b += 8 * static_cast<uint64_t>(tb.second);
+        if (mask_bytes + b > kBatchTableLdsBytes) continue;
+        mask_bytes += b;
+        use_mask[t] = true;
+        for (const auto& tb : p->batch_mask_tables)
+            src << "    __shared__ u64 " << withTemplate(tb.first, t) << "[" << tb.second << "];\n";
+    }
+    for (size_t t = 0; t < templates.size(); ++t)
+        if (use_mask[t]) src << withTemplate(templates[t]->batch_mask_build, t);
     std::vector<LdsTable> cands;
     for (size_t t = 0; t < templates.size(); ++t) {
         const QueryProgram* p = templates[t];
@@ -99,7 +127,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     uint64_t tab_bytes = 0;
     std::map<std::pair<size_t, size_t>, size_t> tab_of;
     for (auto& c : cands) {
-        if (tab_bytes + 4 * c.words > kBatchTableLdsBytes) continue;
+        if (mask_bytes + tab_bytes + 4 * c.words > kBatchTableLdsBytes) continue;
         tab_bytes += 4 * c.words;
         for (const auto& u : c.users) tab_of[u] = tabs.size();
         tabs.push_back(std::move(c));
@@ -115,8 +143,8 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         src << "        if (src)\n            for (int w = tid; w < " << tabs[k].words << "; w += BT) cq_tab" << k << "[w] = src[w];\n"
             << "    }\n";
     }
-    if (!tabs.empty()) src << "    __syncthreads();\n";
-    if (table_lds_bytes) *table_lds_bytes = static_cast<unsigned>(tab_bytes);
+    if (!tabs.empty() || mask_bytes > 0) src << "    __syncthreads();\n";
+    if (table_lds_bytes) *table_lds_bytes = static_cast<unsigned>(tab_bytes + mask_bytes);
 
     // the last template (kernel order) that reads each union column: its
     // load is nontemporal, earlier templates' loads keep the tile in cache
@@ -175,6 +203,10 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
                     << ";\n";
             }
             src << p->batch_shared;
+        }
+        if (use_mask[t]) {
+            src << withTemplate(p->batch_mask_body, t) << "        }\n";
+            continue;
         }
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"
             << "            const cq_query_params& P = B.qp[q];\n";

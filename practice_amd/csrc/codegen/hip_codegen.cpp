@@ -232,7 +232,8 @@ public:
         emit();
         // a batched template is identified by its per-tile body as well:
         // different pipelines can share identical build / flag kernels
-        prog_.signature = signatureOf(prog_.batch_body.empty() ? prog_.source : prog_.source + prog_.batch_body);
+        prog_.signature = signatureOf(prog_.batch_body.empty() ? prog_.source
+                                                               : prog_.source + prog_.batch_body + prog_.batch_mask_build + prog_.batch_mask_body);
         return std::move(prog_);
     }
 
@@ -1582,10 +1583,358 @@ private:
             if (prog_.agg_funcs[static_cast<size_t>(i)] == "COUNT") prog_.count_slots_scalar.push_back(i);
         prog_.batch_body = hoistParams(body.str() + aggcode.str());
         bool_flags_ = false;
+        {
+            // the mask form is an option on top of the per-query form: if it
+            // cannot be built (e.g. too many constants), the template keeps
+            // the per-query form only
+            const size_t nlits = prog_.lits.size();
+            const unsigned vis = visible_mask_;
+            try {
+                emitMaskForm(fact_f, late_f);
+            } catch (const std::exception&) {
+                prog_.lits.resize(nlits);
+                prog_.batch_mask = false;
+                prog_.batch_mask_tables.clear();
+                prog_.batch_mask_build.clear();
+                prog_.batch_mask_body.clear();
+            }
+            visible_mask_ = vis;
+        }
         if (prog_.keys_kernel) emitKeysKernel(src);
         emitDenseCount(src);
         prog_.source = src.str();
         setAlgorithmicBytes();
+    }
+
+    // ------------------------------------------------ query-mask form
+    // A shared template whose per-query work is only (a) fast-path filters
+    // on fact columns with a small value domain and (b) build filters on a
+    // few packed attribute bits evaluates, per launch, each such predicate
+    // for every value of its domain and every query of the template into
+    // an LDS table of query bit masks.  A row's accepting queries are then
+    // the AND of a few table lookups, and aggregation visits only (row,
+    // query) pairs that pass: the cost per row no longer grows with the
+    // number of queries.  Results are those of the per-query form (the
+    // tables hold the same predicates, evaluated exactly on every value the
+    // column or attribute can take).
+    struct MaskFact {
+        int slot = 0;
+        int64_t min = 0, domain = 0;
+        std::vector<const BinaryExpr*> leaves;
+    };
+    static constexpr int64_t kMaskDomain = 1024;
+    static constexpr int kMaskAttrBits = 10;
+
+    static void andLeaves(const ExprNode* e, std::vector<const BinaryExpr*>& out) {
+        if (e->getType() == ExprType::OP_AND) {
+            const auto* b = static_cast<const BinaryExpr*>(e);
+            andLeaves(b->left.get(), out);
+            andLeaves(b->right.get(), out);
+            return;
+        }
+        out.push_back(static_cast<const BinaryExpr*>(e));
+    }
+
+    void emitMaskForm(const std::vector<Op>& fact_f, const std::vector<Op>& late_f) {
+        prog_.batch_mask = false;
+        if (!opt_.batch_masks || !late_f.empty()) return;
+        // aggregation kinds with a per-row insert for one query
+        const bool scalar_lds = !prog_.grouped && prog_.batch_lds_stride > 0;
+        const bool hash_lds = prog_.grouped && prog_.batch_hash_slot_bytes > 0;
+        const bool dense_lds = prog_.grouped && prog_.batch_lds_stride > 0 && !hash_lds;
+        const bool global = prog_.grouped && prog_.batch_lds_stride == 0;
+        if (!scalar_lds && !hash_lds && !dense_lds && !global) return;
+        // fact filters: fast path on non-nullable columns with small domains
+        std::map<std::string, MaskFact> facts;
+        std::vector<std::string> fact_order;
+        for (const Op& op : fact_f) {
+            if (!op.fast) return;
+            std::vector<const BinaryExpr*> leaves;
+            andLeaves(op.pred, leaves);
+            for (const BinaryExpr* b : leaves) {
+                const auto& col = static_cast<const ColumnRefExpr*>(b->left.get())->column_name;
+                const int slot = factSlot(col);
+                if (prog_.fact_nullable[static_cast<size_t>(slot)]) return;
+                const auto* cs = stats(col);
+                if (!cs || cs->max_value_ < cs->min_value_ || cs->max_value_ - cs->min_value_ + 1 > kMaskDomain) return;
+                if (!facts.count(col)) {
+                    fact_order.push_back(col);
+                    facts[col].slot = slot;
+                    facts[col].min = cs->min_value_;
+                    facts[col].domain = cs->max_value_ - cs->min_value_ + 1;
+                }
+                facts[col].leaves.push_back(b);
+            }
+        }
+        // joins: attribute bits their build filters read
+        const int nj = static_cast<int>(prog_.joins.size());
+        std::vector<int> jlo(static_cast<size_t>(nj), 0), jbits(static_cast<size_t>(nj), -1);
+        for (int j = 0; j < nj; ++j) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            if (jp.zmode == 3) return;
+            const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
+            if (!bp) continue;
+            std::vector<std::string> cols;
+            collectColumns(bp, cols);
+            int lo = 32, hi = 0;
+            for (const auto& c : cols)
+                for (size_t a = 0; a < jp.s_cols.size(); ++a)
+                    if (jp.s_cols[a] == c && jp.s_bits[a] > 0) {
+                        lo = std::min(lo, jp.s_off[a]);
+                        hi = std::max(hi, jp.s_off[a] + jp.s_bits[a]);
+                    }
+            if (hi <= lo) lo = hi = 0;
+            if (hi - lo > kMaskAttrBits) return;
+            jlo[static_cast<size_t>(j)] = lo;
+            jbits[static_cast<size_t>(j)] = hi - lo;
+        }
+        // aggregate inputs and the group index must not depend on a query's constants
+        auto constantFree = [](const std::string& x) { return x.find("P.lit[") == std::string::npos && x.find("P.flit[") == std::string::npos; };
+        const auto* a = agg_;
+        const bool decode = keysDecodable();
+        std::string gh = prog_.grouped ? groupHash() : "0";
+        if (!constantFree(gh)) return;
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const AggRow r = aggRow(i);
+            if (!constantFree(r.value) || !constantFree(r.valid)) return;
+        }
+        for (int g = 0; g < prog_.ngroups; ++g) {
+            const ExprNode* e = a->group_by_exprs[static_cast<size_t>(g)].get();
+            if (!constantFree(vvalid(e)) || (!decode && !constantFree(vexpr(e)))) return;
+        }
+
+        // ---- tables and their build (per launch, every block)
+        std::ostringstream build, body;
+        prog_.batch_mask_tables.clear();
+        build << "    {\n        const int q0 = B.tq_begin[@T@], nq = B.tq_end[@T@] - q0;\n";
+        for (size_t k = 0; k < fact_order.size(); ++k) {
+            const MaskFact& f = facts[fact_order[k]];
+            const std::string name = "cq_m@T@_f" + std::to_string(k);
+            prog_.batch_mask_tables.emplace_back(name, static_cast<int>(f.domain));
+            std::string cond;
+            for (const BinaryExpr* b : f.leaves)
+                cond += (cond.empty() ? "" : " && ") + std::string("(cv ") + cmpOp(b->op_type) + " " +
+                        litInt32(static_cast<const LiteralIntExpr*>(b->right.get())->value) + ")";
+            build << "        for (int v = tid; v < " << f.domain << "; v += BT) {\n"
+                  << "            const int cv = (int)(" << f.min << "ll + v);\n"
+                  << "            u64 m = 0ull;\n"
+                  << "            for (int q = 0; q < nq; ++q) {\n"
+                  << "                const cq_query_params& P = B.qp[q0 + q];\n"
+                  << "                if (" << cond << ") m |= 1ull << q;\n"
+                  << "            }\n"
+                  << "            " << name << "[v] = m;\n"
+                  << "        }\n";
+        }
+        for (int j = 0; j < nj; ++j) {
+            if (jbits[static_cast<size_t>(j)] < 0) continue;
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            const std::string name = "cq_m@T@_j" + std::to_string(j);
+            const int n = 1 << jbits[static_cast<size_t>(j)];
+            prog_.batch_mask_tables.emplace_back(name, n);
+            const std::string cond =
+                buildCondWith(build_pred_[static_cast<size_t>(j)], [&](const std::string& col) { return sharedAttrOf(jp, col, "e"); }, true);
+            build << "        for (int v = tid; v < " << n << "; v += BT) {\n"
+                  << "            const u32 e = (u32)v << " << jlo[static_cast<size_t>(j)] << ";\n"
+                  << "            u64 m = 0ull;\n"
+                  << "            for (int q = 0; q < nq; ++q) {\n"
+                  << "                const cq_query_params& P = B.qp[q0 + q];\n"
+                  << "                if ((" << cond << ") != 0) m |= 1ull << q;\n"
+                  << "            }\n"
+                  << "            " << name << "[v] = m;\n"
+                  << "        }\n";
+        }
+        build << "    }\n";
+
+        // ---- per tile: accepting queries of every item
+        body << "        // query masks: the template's queries that accept each item\n"
+             << "        const int cq_nq = B.tq_end[@T@] - B.tq_begin[@T@];\n"
+             << "        const u64 cq_all = cq_nq >= 64 ? ~0ull : ((1ull << cq_nq) - 1ull);\n"
+             << "        u64 qm[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) qm[i] = ((tile_fl >> i) & 1u) ? cq_all : 0ull;\n";
+        for (size_t k = 0; k < fact_order.size(); ++k) {
+            const MaskFact& f = facts[fact_order[k]];
+            body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            const unsigned v = (unsigned)(c" << f.slot << "[i] - (int)" << f.min << ");\n"
+                 << "            qm[i] &= v < " << f.domain << "u ? cq_m@T@_f" << k << "[v] : 0ull;\n"
+                 << "        }\n";
+        }
+        for (int j = 0; j < nj; ++j) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            const std::string J = std::to_string(j);
+            const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
+            const bool filt = jbits[static_cast<size_t>(j)] >= 0;
+            const std::string idx = "((e >> " + std::to_string(jlo[static_cast<size_t>(j)]) + ") & " +
+                                    std::to_string((1u << std::max(0, jbits[static_cast<size_t>(j)])) - 1u) + "u)";
+            const std::string acc = filt ? "cq_m@T@_j" + J + "[" + idx + "]" : "~0ull";
+            body << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            const u32 e = e" << J << "[i];\n"
+                 << "            u64 jm = m" << J << "[i] ? " << acc << " : 0ull;\n";
+            if (jp.zmode == 1) body << "            if (z" << J << "[i]) jm = ~0ull;\n";
+            if (jp.zmode == 2) {
+                // probe key 0 matches the reference's empty slot: the queries
+                // that did not insert the key sharing that slot
+                const std::string ins = jp.s_present ? "((e & 1u) ? " + acc + " : 0ull)" : acc;
+                body << "            if (z" << J << "[i]) jm = ~" << ins << ";\n";
+            }
+            body << "            qm[i] &= jm;\n";
+            if (jp.zmode != 0) body << "            p" << J << "[i] = z" << J << "[i] ? 0 : " << pay << ";\n";
+            else body << "            p" << J << "[i] = " << pay << ";\n";
+            body << "        }\n";
+        }
+        visible_mask_ = (1u << nj) - 1u;
+
+        // ---- aggregation over (row, query) pairs
+        const int ng = prog_.ngroups, sts = prog_.storage_ts, nas = sts - ng;
+        const int64_t G = prog_.groups;
+        std::vector<int> key_vals, key_valids;
+        for (int g = 0; g < ng; ++g) {
+            const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
+            if (!litTrue(v) && !litFalse(v)) key_valids.push_back(g);
+            if (!decode && (global || hash_lds)) key_vals.push_back(g);
+        }
+        body << "        u32 pend = 0u;\n        int gh[IPT];\n";
+        for (int g : key_vals) body << "        u64 gkv" << g << "[IPT];\n";
+        for (int g : key_valids) body << "        u32 gkok" << g << " = 0u;\n";
+        for (int i = 0; i < prog_.naggs; ++i) body << "        u64 gav" << i << "[IPT]; u32 gaok" << i << " = 0u;\n";
+        body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+             << "            const int h = " << gh << ";\n"
+             << "            gh[i] = h;\n"
+             << "            if (qm[i] != 0ull" << (prog_.grouped ? " && (unsigned)h < " + std::to_string(G) + "u" : "") << ") pend |= 1u << i;\n";
+        for (int g : key_vals) body << "            gkv" << g << "[i] = (u64)(" << vexpr(a->group_by_exprs[static_cast<size_t>(g)].get()) << ");\n";
+        for (int g : key_valids)
+            body << "            if (" << vvalid(a->group_by_exprs[static_cast<size_t>(g)].get()) << ") gkok" << g << " |= 1u << i;\n";
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const AggRow r = aggRow(i);
+            body << "            gav" << i << "[i] = " << r.value << ";\n";
+            if (litTrue(r.valid)) body << "            gaok" << i << " |= 1u << i;\n";
+            else if (!litFalse(r.valid)) body << "            if (" << r.valid << ") gaok" << i << " |= 1u << i;\n";
+        }
+        body << "        }\n"
+             << "        const int lane = tid & 63;\n        (void)lane;\n"
+             << "        while (__ballot(pend != 0u) != 0ull) {\n"
+             << "            if (pend == 0u) continue;\n"
+             << "            const int it = __builtin_ctz(pend);\n"
+             << "            u64 qmi = 0ull;\n            int h = 0;\n";
+        for (int g : key_vals) body << "            u64 kv" << g << " = 0ull;\n";
+        for (int i = 0; i < prog_.naggs; ++i) body << "            u64 av" << i << " = 0ull;\n";
+        body << "#pragma unroll\n            for (int j = 0; j < IPT; ++j)\n                if (j == it) {\n"
+             << "                    qmi = qm[j];\n                    h = gh[j];\n";
+        for (int g : key_vals) body << "                    kv" << g << " = gkv" << g << "[j];\n";
+        for (int i = 0; i < prog_.naggs; ++i) body << "                    av" << i << " = gav" << i << "[j];\n";
+        body << "                }\n"
+             << "            const int qb = __builtin_ctzll(qmi);\n"
+             << "            qmi &= qmi - 1ull;\n"
+             << "#pragma unroll\n            for (int j = 0; j < IPT; ++j)\n                if (j == it) qm[j] = qmi;\n"
+             << "            if (qmi == 0ull) pend &= ~(1u << it);\n"
+             << "            const cq_query_params& P = B.qp[B.tq_begin[@T@] + qb];\n"
+             << "            (void)P;\n";
+        if (prog_.batch_lds_stride > 0)
+            body << "            u64* s_agg = (u64*)((char*)cq_lds + B.lds_off[@T@] + (unsigned)qb * B.lds_stride[@T@]);\n"
+                 << "            u32* s_flag = (u32*)(s_agg + " << prog_.groups * (prog_.storage_ts - prog_.ngroups) << ");\n"
+                 << "            const unsigned cq_hbits = B.hbits[@T@];\n"
+                 << "            (void)s_flag; (void)cq_hbits;\n";
+        auto keyValidAt = [&](int g) -> std::string {
+            const std::string v = vvalid(a->group_by_exprs[static_cast<size_t>(g)].get());
+            if (litTrue(v) || litFalse(v)) return v;
+            return "((gkok" + std::to_string(g) + " >> it) & 1u)";
+        };
+        auto aggValidAt = [&](int i) -> std::string {
+            const std::string v = aggRow(i).valid;
+            if (litTrue(v) || litFalse(v)) return v;
+            return "((gaok" + std::to_string(i) + " >> it) & 1u)";
+        };
+        auto aggValueAt = [](int i) { return "av" + std::to_string(i); };
+        auto keyValueAt = [](int g) { return "kv" + std::to_string(g); };
+        const std::string ind = "            ";
+        if (scalar_lds) {
+            // per-lane partial slots of the query (emitScalarAggregateLds layout)
+            const int na = prog_.naggs;
+            std::vector<int> cnt_base(static_cast<size_t>(na), -1);
+            int nh = 0;
+            for (int i = 0; i < na; ++i)
+                if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) cnt_base[static_cast<size_t>(i)] = 64 * (na + nh++);
+            const uint64_t seen_word = scalarLdsWords() - 1;
+            for (int i = 0; i < na; ++i) {
+                const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+                const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
+                body << ind << "if (" << aggValidAt(i) << ") {\n"
+                     << ind << "    " << atom << "(&s_agg[" << 64 * i << " + lane], av" << i << ");\n";
+                if (f != "COUNT" && cnt_base[static_cast<size_t>(i)] < 0)
+                    body << ind << "    atomicOr((u32*)(s_agg + " << seen_word << "), " << (1u << i) << "u);\n";
+                if (cnt_base[static_cast<size_t>(i)] >= 0)
+                    body << ind << "    atomicAdd(&s_agg[" << cnt_base[static_cast<size_t>(i)] << " + lane], 1ull);\n";
+                body << ind << "}\n";
+            }
+        } else if (dense_lds) {
+            // emitLdsAggregate layout: u64 slots per group, W flag bits per group
+            const int W = flag_w_;
+            const int per = 32 / W;
+            int lp = 0;
+            while ((1 << lp) < per) ++lp;
+            body << ind << "u32 want = 1u;\n";
+            for (int g : flag_dyn_) {
+                if (g >= ng) continue;
+                body << ind << "if (" << keyValidAt(g) << ") want |= 1u << " << dynBit(g) << ";\n";
+            }
+            for (int i = 0; i < prog_.naggs; ++i) {
+                const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+                const int slot = ng + i;
+                const std::string ref = "s_agg[h * " + std::to_string(nas) + " + " + std::to_string(slot - ng) + "]";
+                body << ind << "if (" << aggValidAt(i) << ") {\n";
+                if (f == "MIN") body << ind << "    atomicMin(&" << ref << ", av" << i << ");\n";
+                else if (f == "MAX") body << ind << "    atomicMax(&" << ref << ", av" << i << ");\n";
+                else body << ind << "    atomicAdd(&" << ref << ", av" << i << ");\n";
+                const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
+                if (hs >= 0) body << ind << "    atomicAdd(&s_agg[h * " << nas << " + " << (hs - ng) << "], 1ull);\n";
+                const int db = dynBit(slot);
+                if (db > 0) body << ind << "    want |= 1u << " << db << ";\n";
+                body << ind << "}\n";
+            }
+            body << ind << "{ u32* fw = &s_flag[h >> " << lp << "]; const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
+                 << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n";
+        } else if (hash_lds) {
+            body << ind << "u32* s_hk = (u32*)s_agg;\n"
+                 << ind << "u32* s_hf = s_hk + (1u << cq_hbits);\n"
+                 << ind << "u64* s_hv = s_agg + (1u << cq_hbits);\n"
+                 << ind << "u32 slot = ((u32)h * 2654435761u) >> (32u - cq_hbits);\n"
+                 << ind << "int found = -1;\n"
+                 << ind << "for (int t = 0; t < 8; ++t) {\n"
+                 << ind << "    u32 cur = s_hk[slot];\n"
+                 << ind << "    if (cur == 0u) { const u32 old = atomicCAS(&s_hk[slot], 0u, (u32)h + 1u); cur = old == 0u ? (u32)h + 1u : old; }\n"
+                 << ind << "    if (cur == (u32)h + 1u) { found = (int)slot; break; }\n"
+                 << ind << "    slot = (slot + 1u) & ((1u << cq_hbits) - 1u);\n"
+                 << ind << "}\n"
+                 << ind << "if (found >= 0) {\n"
+                 << ind << "    u32 want = 1u;\n";
+            for (int g : flag_dyn_) {
+                if (g >= ng || prog_.fast_sparse) continue;
+                body << ind << "    if (" << keyValidAt(g) << ") want |= 1u << " << dynBit(g) << ";\n";
+            }
+            for (int i = 0; i < prog_.naggs; ++i) {
+                const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+                const std::string ref = "s_hv[found * " + std::to_string(nas) + " + " + std::to_string(i) + "]";
+                body << ind << "    if (" << aggValidAt(i) << ") {\n";
+                if (f == "MIN") body << ind << "        atomicMin(&" << ref << ", av" << i << ");\n";
+                else if (f == "MAX") body << ind << "        atomicMax(&" << ref << ", av" << i << ");\n";
+                else body << ind << "        atomicAdd(&" << ref << ", av" << i << ");\n";
+                const int hs = prog_.hidden_slot[static_cast<size_t>(i)];
+                if (hs >= 0) body << ind << "        atomicAdd(&s_hv[found * " << nas << " + " << (hs - ng) << "], 1ull);\n";
+                const int db = prog_.fast_sparse ? -1 : dynBit(ng + i);
+                if (db > 0) body << ind << "        want |= 1u << " << db << ";\n";
+                body << ind << "    }\n";
+            }
+            body << ind << "    if ((s_hf[found] & want) != want) atomicOr(&s_hf[found], want);\n"
+                 << ind << "} else {\n"
+                 << ind << "    const long long out = (long long)h * " << sts << ";\n";
+            globalRowWith(body, ind + "    ", keyValueAt, keyValidAt, aggValidAt, aggValueAt);
+            body << ind << "}\n";
+        } else {
+            body << ind << "const long long out = (long long)h * " << sts << ";\n";
+            globalRowWith(body, ind, keyValueAt, keyValidAt, aggValidAt, aggValueAt);
+        }
+        body << "        }\n";
+        prog_.batch_mask_build = build.str();
+        prog_.batch_mask_body = body.str();
+        prog_.batch_mask = true;
     }
 
     // The per-query body reads its constants from the query's parameter block

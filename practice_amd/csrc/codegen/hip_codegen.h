@@ -40,6 +40,9 @@ struct CodegenOptions {
     bool batch_body = false;
     // batched templates share one probe per dimension (packed attributes)
     bool batch_shared_tables = true;
+    // shared templates whose filters reduce to small lookup tables evaluate
+    // all their queries at once through per-row query bit masks
+    bool batch_masks = true;
     // interpreted tier: register code for the ahead-of-time interpreter
     // instead of HIP source
     bool vm = false;
@@ -134,6 +137,15 @@ struct QueryProgram {
     // per-query LDS aggregation (init / merge code, bytes per query)
     bool shared_tables = false;
     bool batch_bool_flags = false;  // the body keeps row flags in `bool fl[IPT]` (else `u32 fl`)
+    // query-mask form (CodegenOptions::batch_masks): per launch, LDS tables
+    // map a filter column's value / a join's filter attributes to the bit
+    // mask of the template's queries that accept it (built by
+    // batch_mask_build), and batch_mask_body handles a tile for all queries
+    // of the template (up to 64).  "@T@" in both stands for the template's
+    // index in the launch.
+    bool batch_mask = false;
+    std::vector<std::pair<std::string, int>> batch_mask_tables;  // LDS u64 tables: name, entries
+    std::string batch_mask_build, batch_mask_body;
     std::string batch_shared;
     // (scope of the batch kernel: cq_hbits = log2 slots of the query's LDS
     // hash table).  batch_hash_slot_bytes > 0 marks a hash region whose size

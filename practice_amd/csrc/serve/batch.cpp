@@ -13,10 +13,12 @@ namespace db {
 
 namespace {
 constexpr int kHashMinBits = 7, kHashMaxBits = 10;  // per-query LDS hash tables: 128..1024 slots
+constexpr int kMaskQueries = 64;                    // queries of a mask-form template per launch
 // dynamic LDS for the per-query regions: what lds_kb allows, minus the room
 // kept for shared dimension tables staged into LDS
 unsigned ldsLimit(const BatchOptions& opt) {
-    return std::min(static_cast<unsigned>(opt.lds_kb) * 1024u, 160u * 1024u - kBatchTableLdsBytes);
+    // 1 KiB more for the kernel's other static LDS
+    return std::min(static_cast<unsigned>(opt.lds_kb) * 1024u, 160u * 1024u - kBatchTableLdsBytes - 1024u);
 }
 }  // namespace
 
@@ -39,6 +41,7 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "shared") o.shared_tables = v != 0;
         else if (k == "lds_agg") o.lds_aggregation = v != 0;
         else if (k == "lds_kb") o.lds_kb = static_cast<int>(v);
+        else if (k == "masks") o.masks = v != 0;
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
@@ -54,6 +57,11 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
                               int part) {
     CodegenOptions co = db.config().codegen;
     co.batch_body = true;
+    co.block_threads = opt.block_threads;
+    co.vectors_per_thread = opt.vectors_per_thread;
+    co.batch_shared_tables = opt.shared_tables;
+    co.batch_masks = opt.masks;
+    co.allow_lds_aggregation = co.allow_lds_aggregation && opt.lds_aggregation;
     std::vector<QueryProgram> programs;
     for (const auto& q : sqls) programs.push_back(db.engine().prepare(q, co));
     if (part >= 0) {
@@ -83,6 +91,7 @@ CodegenOptions BatchExecutor::codegenOptions(const BatchOptions& opt) const {
     co.block_threads = opt.block_threads;
     co.vectors_per_thread = opt.vectors_per_thread;
     co.batch_shared_tables = opt.shared_tables;
+    co.batch_masks = opt.masks;
     co.allow_lds_aggregation = co.allow_lds_aggregation && opt.lds_aggregation;
     return co;
 }
@@ -164,14 +173,18 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         // one launch of the caller's module per chunk that fits the LDS regions
         std::vector<int> cur;
         uint64_t lds = 0;
+        std::map<std::string, int> per_sig;  // queries of each signature in `cur`
         for (size_t i = 0; i < programs.size(); ++i) {
             const uint64_t need = programs[i].batch_lds_stride;
-            if (!cur.empty() && lds + need > ldsLimit(opt)) {
+            const bool mask_full = programs[i].batch_mask && per_sig[programs[i].signature] >= kMaskQueries;
+            if (!cur.empty() && (lds + need > ldsLimit(opt) || mask_full)) {
                 scans.push_back(cur);
                 cur.clear();
+                per_sig.clear();
                 lds = 0;
             }
             cur.push_back(static_cast<int>(i));
+            ++per_sig[programs[i].signature];
             lds += need;
         }
         if (!cur.empty()) scans.push_back(cur);
@@ -189,6 +202,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         // their LDS aggregation regions
         size_t chunk = static_cast<size_t>(opt.max_queries);
         if (t.batch_lds_stride > 0) chunk = std::min<size_t>(chunk, std::max<uint64_t>(1, ldsLimit(opt) / t.batch_lds_stride));
+        if (t.batch_mask) chunk = std::min<size_t>(chunk, kMaskQueries);  // one bit per query in a 64-bit mask
         for (size_t start = 0; start < qs.size(); start += chunk) {
             const size_t stop = std::min(qs.size(), start + chunk);
             const uint64_t lds_need = t.batch_lds_stride * (stop - start);
@@ -340,6 +354,9 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
         }
         per_template[t].push_back(q);
     }
+    for (size_t t = 0; t < templates.size(); ++t)
+        if (templates[t]->batch_mask && per_template[t].size() > static_cast<size_t>(kMaskQueries))
+            throw std::logic_error("mask-form template with more than 64 queries in one launch");
     if (templates.empty()) return;
     if (templates.size() > static_cast<size_t>(CQ_MAX_TEMPLATES)) throw std::runtime_error("too many templates in one scan");
     if (!module_templates) {

@@ -34,10 +34,11 @@ struct BatchOptions {
     bool shared_tables = true;    // one probe per template and dimension (packed attributes)
     bool lds_aggregation = true;  // per-query LDS aggregation for small group domains
     int lds_kb = 160;             // dynamic LDS of one shared-scan block (KiB, 16..160)
+    bool masks = true;            // query-mask form for templates whose filters allow it
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
-// max_queries, shared, lds_agg, lds_kb)
+// max_queries, shared, lds_agg, lds_kb, masks)
 BatchOptions parseBatchOptions(const std::string& text);
 
 struct BatchScanReport {

@@ -82,6 +82,24 @@ def test_template_modules_compile(hdb):
         pa.compile_source(part)
 
 
+def test_mask_form_for_template_queries(hdb):
+    """The four serving templates reduce to query-mask tables (one pass for
+    all queries of a template); masks=0 keeps the per-query loop."""
+    import random
+    rng = random.Random(4)
+    qs = [Q.template_query(k, rng) for k in (1, 2, 3, 4) for _ in range(3)]
+    src = hdb.generate_batch_code(qs)
+    assert src.count("// query masks") == 4
+    assert "for (int q = B.tq_begin[" not in src.split("for (long long tile")[1].split("__syncthreads")[0]
+    plain = hdb.generate_batch_code(qs, {"masks": 0})
+    assert "// query masks" not in plain
+    pa.compile_source(src)
+    # a filter on a column with a large domain keeps the per-query form
+    wide = ["select sum(lo_revenue) from lineorder where lo_extendedprice < 5000000",
+            "select sum(lo_revenue) from lineorder where lo_extendedprice < 6000000"]
+    assert "// query masks" not in hdb.generate_batch_code(wide)
+
+
 @pytest.mark.parametrize("opts", [{"bt": 1024, "vpt": 4}, {"bt": 64, "vpt": 1}])
 def test_launch_shapes_compile(hdb, opts):
     pa.compile_source(hdb.generate_batch_code([Q.C3, Q.C1], opts))
@@ -104,16 +122,36 @@ def _check_batch(gdb, odb, qs, results):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graph", [0, 1])
-def test_batch_equals_alone_and_oracle(graph):
+@pytest.mark.parametrize("graph,masks", [(0, 1), (1, 1), (0, 0)])
+def test_batch_equals_alone_and_oracle(graph, masks):
+    """masks 1: templates whose filters allow it run in the query-mask form
+    (all their queries at once); masks 0: every template per query."""
     from tests.test_gpu_parity import dataset
     spec, data, odb, gdb = dataset("sf1")
     qs = mixed_queries()
-    print(f"[batch] {len(qs)} queries graph={graph}", flush=True)
-    res = gdb.batch_execute(qs, {"graph": graph})
+    print(f"[batch] {len(qs)} queries graph={graph} masks={masks}", flush=True)
+    res = gdb.batch_execute(qs, {"graph": graph, "masks": masks})
     rep = gdb.batch_report()
     assert rep["queries"] == len(qs) and rep["shared_scans"] >= 2
     _check_batch(gdb, odb, qs, res)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("masks", [0, 1])
+def test_templates_with_many_queries(masks):
+    """80 queries of each of the four templates: mask-form templates take at
+    most 64 queries per launch, the rest go to another launch."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(31)
+    qs = [Q.template_query(k, rng) for k in (1, 2, 3, 4) for _ in range(80)]
+    print(f"[batch] {len(qs)} template queries masks={masks}", flush=True)
+    res = gdb.batch_execute(qs, {"graph": 0, "masks": masks})
+    rep = gdb.batch_report()
+    assert rep["queries"] == len(qs)
+    for sql, r in zip(qs, res):
+        assert_same(r, odb.execute(gdb.explain(sql)))
 
 
 @pytest.mark.gpu
