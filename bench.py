@@ -50,6 +50,8 @@ def parse_args():
     p.add_argument("--serve-rates", default="100,500,2000", help="Poisson arrival rates (req/s), comma separated")
     p.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each arrival rate")
     p.add_argument("--serve-burst", type=int, default=1024, help="requests of the closed-loop burst")
+    p.add_argument("--serve-options", default="lanes=2,max_batch=64,cold_lanes=2",
+                   help="server options, key=value,... (cq_server_start)")
     p.add_argument("--serve-adhoc", type=float, default=0.05, help="fraction of requests with a new plan shape")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
@@ -173,7 +175,7 @@ def bench_served(pa, Q, args):
     for q in ser:
         db.execute(q)
     serial_qps = len(ser) / (time.perf_counter() - t0)
-    srv = pa.Server(db, {"lanes": 2, "max_batch": 64, "cold_lanes": 2})
+    srv = pa.Server(db, dict(kv.split("=") for kv in args.serve_options.split(",") if kv))
     # warm the kernel cache: every template seen twice is compiled into the
     # hot set in the background (the ad-hoc shapes below stay cold)
     for _ in range(2):
@@ -239,7 +241,7 @@ def bench_served(pa, Q, args):
     return {"workload": f"Poisson arrivals at {args.serve_rates} req/s for {args.serve_seconds:g} s each, then a burst of "
                         f"{args.serve_burst}; 4 templates + {args.serve_adhoc:.0%} new plan shapes, scale 100 "
                         f"(600,000,000 fact rows)",
-            "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
+            "server_options": args.serve_options, "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
             "saturated_speedup": round(burst["qps"] / serial_qps, 2)}
 
 

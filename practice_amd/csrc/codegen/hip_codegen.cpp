@@ -1279,9 +1279,33 @@ private:
         visible_mask_ = 0;
         for (const Op& op : pipelineOrder()) {
             if (op.kind == 0) {
-                vm::Val c = b.compile(op.fast ? fastCond(op.pred) : andTerms({vvalid(op.pred), vexpr(op.pred)}));
-                b.emit(CQ_OP_FILTER, c.type, 0, 0, c.reg, 0, 0, 0);
-                b.release(c);
+                if (op.fast) {
+                    // one fused compare-and-filter per `col op literal` of the AND chain
+                    std::vector<const BinaryExpr*> leaves;
+                    andLeaves(op.pred, leaves);
+                    for (const BinaryExpr* lf : leaves) {
+                        const auto& col = static_cast<const ColumnRefExpr*>(lf->left.get())->column_name;
+                        if (tableOf(col) != prog_.fact_table)
+                            unsupported("fast-path filter on dimension column " + col + " above its join");
+                        const int k = static_cast<int>(prog_.lits.size());
+                        (void)litInt32(static_cast<const LiteralIntExpr*>(lf->right.get())->value);
+                        int cmp = CQ_OP_EQ;
+                        switch (lf->op_type) {
+                            case ExprType::OP_EQ: cmp = CQ_OP_EQ; break;
+                            case ExprType::OP_NEQ: cmp = CQ_OP_NE; break;
+                            case ExprType::OP_LT: cmp = CQ_OP_LT; break;
+                            case ExprType::OP_LTE: cmp = CQ_OP_LE; break;
+                            case ExprType::OP_GT: cmp = CQ_OP_GT; break;
+                            case ExprType::OP_GTE: cmp = CQ_OP_GE; break;
+                            default: unsupported("fast-path comparison");
+                        }
+                        b.emit(CQ_OP_FCMP, cmp, 0, 0, 0, 0, k, factSlot(col));
+                    }
+                } else {
+                    vm::Val c = b.compile(andTerms({vvalid(op.pred), vexpr(op.pred)}));
+                    b.emit(CQ_OP_FILTER, c.type, 0, 0, c.reg, 0, 0, 0);
+                    b.release(c);
+                }
             } else {
                 const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
                 const int fs = factSlot(jp.fk_col);

@@ -37,7 +37,9 @@ enum cq_vm_op {
     CQ_OP_SEL,      // d = c ? a : b     (c of type t2; a, b already of the result type)
     CQ_OP_FILTER,   // row survives only if a != 0 (type t)
     CQ_OP_PROBE,    // probe join imm with key register a; payload into the join's payload register
-    CQ_OP_EXIT      // the row leaves the pipeline if no row of the wave is alive (early exit)
+    CQ_OP_EXIT,     // the row leaves the pipeline if no row of the wave is alive (early exit)
+    CQ_OP_FCMP      // fast-path filter: row survives only if (int)fact column imm  <t>  (int)P.lit[c]
+                    // (t one of CQ_OP_EQ..CQ_OP_GE; raw values, no validity)
 };
 
 struct cq_vm_instr {

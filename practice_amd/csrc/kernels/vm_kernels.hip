@@ -174,6 +174,12 @@ __device__ bool run(const cq_vm_program* __restrict__ V, int begin, int end, u64
             case CQ_OP_EXIT:
                 if (__ballot(alive) == 0ull) return false;
                 break;
+            case CQ_OP_FCMP:
+                if (alive) {
+                    const int v = c.P->col[in.imm][c.row];
+                    alive = compare(in.t, (u64)(i64)v, (u64)(i64)(int)c.P->lit[in.c], CQ_T_I32) != 0;
+                }
+                break;
             default: break;
         }
     }
@@ -698,6 +704,37 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                 case CQ_OP_EXIT:
                     if (__ballot(alive[0] || alive[1] || alive[2] || alive[3]) == 0ull) pc = code_end;
                     break;
+                case CQ_OP_FCMP: {
+                    const int* col = P.col[ins.imm];
+                    const int l = (int)P.lit[ins.c];
+                    int x[4];
+                    if (full) {
+                        const chip::v4i v = *(const chip::v4i*)(col + row0);
+                        x[0] = v.x;
+                        x[1] = v.y;
+                        x[2] = v.z;
+                        x[3] = v.w;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) x[k] = in[k] ? col[row0 + k] : 0;
+                    }
+                    switch (t) {
+#define VM4_FCMP(OPC, EXPR)                                                   \
+    case OPC:                                                                  \
+        _Pragma("unroll") for (int k = 0; k < 4; ++k) alive[k] = alive[k] && (EXPR); \
+        break;
+                        VM4_FCMP(CQ_OP_EQ, x[k] == l)
+                        VM4_FCMP(CQ_OP_NE, x[k] != l)
+                        VM4_FCMP(CQ_OP_LT, x[k] < l)
+                        VM4_FCMP(CQ_OP_LE, x[k] <= l)
+                        VM4_FCMP(CQ_OP_GT, x[k] > l)
+                        default:
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) alive[k] = alive[k] && (x[k] >= l);
+                            break;
+                    }
+                    break;
+                }
                 default: break;  // build / HAVING instructions never reach this kernel
             }
         }

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <set>
 #include <sstream>
 #include <stdexcept>
 
@@ -193,6 +194,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
     std::vector<int> cur;
     std::vector<std::string> cur_cols;
     std::string cur_table;
+    std::set<std::string> cur_keys;  // templates in `cur` (a template's chunks go to different launches)
     int cur_templates = 0;
     uint64_t cur_lds = 0;
     for (const auto& key : order) {
@@ -209,19 +211,21 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
             std::vector<std::string> cols = cur_cols;
             for (const auto& c : t.fact_cols)
                 if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
-            const bool fits = !cur.empty() && cur_table == t.fact_table && cur_templates + 1 <= opt.max_templates &&
+            const bool fits = !cur.empty() && !cur_keys.count(key) && cur_table == t.fact_table && cur_templates + 1 <= opt.max_templates &&
                               cols.size() <= CQ_MAX_BATCH_COLS &&
                               cur.size() + (stop - start) <= static_cast<size_t>(opt.max_queries) &&
                               cur_lds + lds_need <= ldsLimit(opt);
             if (!fits && !cur.empty()) {
                 scans.push_back(cur);
                 cur.clear();
+                cur_keys.clear();
                 cur_templates = 0;
                 cur_lds = 0;
                 cols.assign(t.fact_cols.begin(), t.fact_cols.end());
             }
             cur_table = t.fact_table;
             cur_cols = cols;
+            cur_keys.insert(key);
             ++cur_templates;
             cur_lds += lds_need;
             for (size_t k = start; k < stop; ++k) cur.push_back(qs[k]);
