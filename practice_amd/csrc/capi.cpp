@@ -235,6 +235,7 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "lds_aggregation") c.codegen.allow_lds_aggregation = v != 0;
         else if (n == "compact_tables") c.codegen.allow_compact_tables = v != 0;
         else if (n == "prefetch") c.codegen.prefetch = v != 0;
+        else if (n == "stream_tables") c.codegen.stream_large_tables = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else if (n == "host_sort_rows") c.host_sort_rows = v;

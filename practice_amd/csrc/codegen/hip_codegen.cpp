@@ -1102,13 +1102,17 @@ private:
                      << "            if (!" << flGet() << ") continue;\n";
                 if (prog_.fact_nullable[static_cast<size_t>(fs)])
                     body << "            if (!((v" << fs << " >> i) & 1u)) { " << flClear() << " continue; }\n";
+                // tables beyond L1 (32 KiB) that stay in HBM are read with
+                // streaming loads: L1 keeps the small ones
+                const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
+                const std::string st = stream ? "true" : "false";
                 if (jp.dense)
-                    body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ">("
+                    body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ", " << st << ">("
                          << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
                          << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p"
                          << j << "[i])) " << flClear() << "\n";
                 else if (jp.compact)
-                    body << "            if (!chip::ht_probe_compact(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
+                    body << "            if (!chip::ht_probe_compact<" << st << ">(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
                          << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j
                          << "[i])) " << flClear() << "\n";
                 else

@@ -35,6 +35,7 @@ struct CodegenOptions {
     bool allow_compact_tables = true;
     bool allow_dense_tables = true;   // key-indexed tables where the keys allow
     bool prefetch = false;            // load tile t+stride while tile t is processed
+    bool stream_large_tables = false; // streaming loads for probes of HBM tables beyond L1 (measured 2x slower on C3: off)
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;

@@ -18,6 +18,17 @@ typedef unsigned long long u64;
 
 // Reference slot: (k - kmin) % len in int arithmetic.  A negative k - kmin
 // (an out-of-bounds access in the reference) never matches here.
+// A table word.  STREAM: a table much larger than L1 that is probed at
+// random; its lines are loaded non-temporally so that they do not push the
+// small tables probed beside it out of L1.
+template <bool STREAM>
+__device__ __forceinline__ u32 ht_word(const u32* w, u32 i) {
+#if defined(__clang__)
+    if (STREAM) return __builtin_nontemporal_load(w + i);
+#endif
+    return w[i];
+}
+
 __device__ __forceinline__ int ht_slot(int k, int kmin, int len, int& x) {
     x = (int)((u32)k - (u32)kmin);
     return x < len ? x : x % len;
@@ -30,6 +41,7 @@ __device__ __forceinline__ int ht_slot(int k, int kmin, int len, int& x) {
 // Valid when the keys inserted are unique and span at most `len` consecutive
 // values starting at or above kmin (then no two keys share a slot).  An empty
 // slot reads as key 0 / payload 0, exactly like the zero-filled reference table.
+template <bool STREAM = false>
 __device__ __forceinline__ bool ht_probe_compact(const u32* w, int fk, int kmin, int len, int eb, int tb, int jlo,
                                                  int pmin, int& payload) {
     int x;
@@ -37,7 +49,7 @@ __device__ __forceinline__ bool ht_probe_compact(const u32* w, int fk, int kmin,
     if (x < 0) return false;
     const u64 bit = (u64)(u32)s * (u32)eb;
     const u32 mask = eb >= 32 ? 0xffffffffu : ((1u << eb) - 1u);
-    const u32 e = (w[bit >> 5] >> (bit & 31)) & mask;
+    const u32 e = (ht_word<STREAM>(w, (u32)(bit >> 5)) >> (bit & 31)) & mask;
     const bool occ = (e & 1u) != 0;
     const int tag = tb ? (int)((e >> 1) & ((1u << tb) - 1u)) : 0;
     const int key = occ ? (int)((u32)kmin + (u32)s + (u32)((jlo + tag) * len)) : 0;
@@ -66,17 +78,19 @@ __device__ __forceinline__ void ht_insert_compact(u32* w, int k, int kmin, int l
 // exactly for the inserted keys), and dmin..dmin+dspan-1 covers every key.
 // db == 0: every key present and a single payload value (pbase) or none:
 // the probe is the range test alone (no table).
+template <bool STREAM = false>
 __device__ __forceinline__ u32 dense_entry(const u32* w, u32 x, int db) {
     const u32 b = x * (u32)db;
-    const u32 lo = w[b >> 5] >> (b & 31);
+    const u32 lo = ht_word<STREAM>(w, b >> 5) >> (b & 31);
     const u32 mask = (1u << db) - 1u;
     if ((b & 31) + (u32)db <= 32u) return lo & mask;
-    return (lo | (w[(b >> 5) + 1] << (32u - (b & 31)))) & mask;
+    return (lo | (ht_word<STREAM>(w, (b >> 5) + 1) << (32u - (b & 31)))) & mask;
 }
 // db a power of two: an entry never straddles (one load per probe)
+template <bool STREAM = false>
 __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) {
     const u32 b = x * (u32)db;
-    return (w[b >> 5] >> (b & 31)) & ((1u << db) - 1u);
+    return (ht_word<STREAM>(w, b >> 5) >> (b & 31)) & ((1u << db) - 1u);
 }
 // Probe key 0 in the zero-filled reference table matches an EMPTY slot
 // (key 0 == empty key, payload 0).  `zmode` reproduces that (fixed per join
@@ -85,7 +99,7 @@ __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) 
 //   1  no key shares key 0's slot: key 0 always matches, payload 0
 //   2  key zx + dmin != 0 shares the slot: key 0 matches iff it is absent
 //   3  key 0 itself owns the slot: key 0 always matches, payload if inserted
-template <bool ALIGNED>
+template <bool ALIGNED, bool STREAM = false>
 __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, int dspan, int db, int pbase, bool off,
                                                int zmode, int zx, int& payload) {
     if (zmode != 0 && fk == 0) {
@@ -93,7 +107,7 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
             payload = 0;
             return true;
         }
-        const u32 e0 = db == 0 ? 0u : (ALIGNED ? dense_entry_aligned(w, (u32)zx, db) : dense_entry(w, (u32)zx, db));
+        const u32 e0 = db == 0 ? 0u : (ALIGNED ? dense_entry_aligned<STREAM>(w, (u32)zx, db) : dense_entry<STREAM>(w, (u32)zx, db));
         const bool in0 = db == 0 || !off || e0 != 0u;
         payload = (zmode == 3 && in0) ? pbase + (int)e0 : 0;
         return zmode == 3 || !in0;
@@ -104,7 +118,7 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
         payload = pbase;
         return true;
     }
-    const u32 e = ALIGNED ? dense_entry_aligned(w, x, db) : dense_entry(w, x, db);
+    const u32 e = ALIGNED ? dense_entry_aligned<STREAM>(w, x, db) : dense_entry<STREAM>(w, x, db);
     payload = pbase + (int)e;
     return !off || e != 0u;
 }

@@ -33,7 +33,7 @@ def _layout_calls(src):
     builds, probes = {}, {}
     for m in re.finditer(r"ht_insert_(\w+)\((?:\(int\*\))?P\.ht\[(\d+)\], k, ([^;]*)\);", src):
         builds[int(m.group(2))] = (m.group(1), m.group(3))
-    for m in re.finditer(r"ht_probe_(compact|dense<\w+>|ref)\((?:\(const int\*\))?(?:s_ht|P\.ht\[)(\d+)\]?, c\d+\[i\], ([^)]*)\)", src):
+    for m in re.finditer(r"ht_probe_(compact(?:<\w+>)?|dense<\w+(?:, \w+)?>|ref)\((?:\(const int\*\))?(?:s_ht|P\.ht\[)(\d+)\]?, c\d+\[i\], ([^)]*)\)", src):
         probes[int(m.group(2))] = (m.group(1).split("<")[0], m.group(3))
     return builds, probes
 
