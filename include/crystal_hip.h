@@ -94,9 +94,12 @@ int cq_db_fetch_column(cq_db* db, const char* column, int32_t* out, uint64_t* va
 long long cq_db_table_rows(cq_db* db, const char* table);
 /* option names: "kernel_timing", "memory_guard", "lds_budget", "block_threads",
  * "vectors_per_thread", "lds_tables", "lds_aggregation", "compact_tables",
+ * "dense_tables" (key-indexed tables), "calendar_tables" (key-indexed tables
+ * of yyyymmdd date keys indexed by year, month and day), "stream_tables",
  * "memory_guard_reserve_bytes", "prefetch" (load the next tile while the
  * current one is probed), "interpret" (1: run plans on the ahead-of-time
- * interpreter instead of hipRTC-compiled kernels; same results, no compile) */
+ * interpreter instead of hipRTC-compiled kernels; same results, no compile),
+ * "vm_row_kernel", "host_sort_rows", "prepared_cache" */
 int cq_db_set_option(cq_db* db, const char* name, long long value);
 long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","compiles" */
 int cq_db_clear_cache(cq_db* db);

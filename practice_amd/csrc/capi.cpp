@@ -234,8 +234,10 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "lds_tables") c.codegen.allow_lds_tables = v != 0;
         else if (n == "lds_aggregation") c.codegen.allow_lds_aggregation = v != 0;
         else if (n == "compact_tables") c.codegen.allow_compact_tables = v != 0;
+        else if (n == "dense_tables") c.codegen.allow_dense_tables = v != 0;
         else if (n == "prefetch") c.codegen.prefetch = v != 0;
         else if (n == "stream_tables") c.codegen.stream_large_tables = v != 0;
+        else if (n == "calendar_tables") c.codegen.calendar_tables = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else if (n == "host_sort_rows") c.host_sort_rows = v;
@@ -303,7 +305,8 @@ int cq_db_describe(cq_db* d, const char* sql, char* buf, size_t len, size_t* nee
               << ",\"layout\":\"" << (jp.dense ? "key-indexed" : (jp.compact ? "compact" : "reference"))
               << "\",\"compact\":" << (jp.compact ? 1 : 0) << ",\"entry_bits\":" << (jp.dense ? jp.db : jp.eb)
               << ",\"tag_bits\":" << (jp.dense ? 0 : jp.tb) << ",\"absent_code\":" << (jp.dense ? jp.doff : 1)
-              << ",\"lds\":" << (jp.lds ? 1 : 0) << ",\"bytes\":" << jp.ht_words * 4 << "}";
+              << ",\"calendar\":" << (jp.dense ? jp.cal : 0) << ",\"lds\":" << (jp.lds ? 1 : 0) << ",\"bytes\":" << jp.ht_words * 4
+              << "}";
         }
         o << "],\"lits\":[";
         for (size_t k = 0; k < p.lits.size(); ++k) o << (k ? "," : "") << p.lits[k];

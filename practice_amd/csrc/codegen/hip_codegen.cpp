@@ -491,6 +491,22 @@ private:
                     jp.zx = static_cast<int>(k0 - jp.dmin);
                 }
             }
+            // calendar keys (yyyymmdd dates): index by (year, month, day),
+            // 372 entries a year instead of 10,000, when that is smaller.
+            // Some of those entries are no date, so code 0 marks absent keys.
+            jp.cal = 0;
+            if (jp.dense_ok && opt_.calendar_tables && ks2->is_calendar_ && jp.zmode == 0) {
+                const int64_t y0 = ks2->min_value_ / 10000, y1 = ks2->max_value_ / 10000;
+                const int64_t span = (y1 - y0 + 1) * 372;
+                if (y0 > 0 && span < jp.dspan) {
+                    jp.cal = static_cast<int>(y0);
+                    jp.dmin = y0 * 10000 + 101;
+                    jp.dspan = span;
+                    jp.doff = 1;
+                    jp.dbits = jp.pair ? bitsFor(range + 1) : 1;
+                    jp.dense_ok = jp.dbits <= 31;
+                }
+            }
         }
     }
 
@@ -998,7 +1014,7 @@ private:
         src << "        const int pay = " << (jp.pair ? "P.dcol[" + std::to_string(jp.payload_dslot) + "][r]" : "0") << ";\n";
         if (jp.dense)
             src << "        chip::ht_insert_dense(P.ht[" << j << "], k, " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
-                << densePbase(jp) << ", pay);\n";
+                << densePbase(jp) << ", pay, " << jp.cal << ");\n";
         else if (jp.compact)
             src << "        chip::ht_insert_compact(P.ht[" << j << "], k, " << jp.kmin << ", " << jp.len << ", " << jp.eb << ", "
                 << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", pay);\n";
@@ -1110,7 +1126,7 @@ private:
                     body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ", " << st << ">("
                          << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
                          << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p"
-                         << j << "[i])) " << flClear() << "\n";
+                         << j << "[i], " << jp.cal << ")) " << flClear() << "\n";
                 else if (jp.compact)
                     body << "            if (!chip::ht_probe_compact<" << st << ">(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
                          << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j
@@ -1264,7 +1280,7 @@ private:
             J.len = static_cast<int>(jp.len);
             J.eb = jp.eb, J.tb = jp.tb, J.jlo = jp.jlo, J.pmin = static_cast<int>(jp.pmin);
             J.dmin = static_cast<int>(jp.dmin), J.dspan = static_cast<int>(jp.dspan), J.db = jp.db;
-            J.pbase = static_cast<int>(densePbase(jp)), J.doff = jp.doff, J.zmode = jp.zmode, J.zx = jp.zx;
+            J.pbase = static_cast<int>(densePbase(jp)), J.doff = jp.doff, J.zmode = jp.zmode, J.zx = jp.zx, J.cal = jp.cal;
             J.build_begin = static_cast<int>(code.size());
             if (const ExprNode* bp = build_pred_[static_cast<size_t>(j)]) {
                 vm::Val c = b.compile(buildCond(bp));
@@ -1456,7 +1472,8 @@ private:
             jp.ht_words = denseWords(jp, jp.db);
             // tables of equal key: equal contents (every row, packed attributes)
             std::ostringstream k;
-            k << jp.dim_table << "|" << jp.pk_col << "|" << jp.dmin << "|" << jp.dspan << "|" << jp.db << "|" << jp.s_present;
+            k << jp.dim_table << "|" << jp.pk_col << "|" << jp.dmin << "|" << jp.dspan << "|" << jp.cal << "|" << jp.db << "|"
+              << jp.s_present;
             for (size_t a = 0; a < jp.s_cols.size(); ++a) k << "|" << jp.s_cols[a] << ":" << jp.s_min[a] << ":" << jp.s_bits[a] << ":" << jp.s_off[a];
             jp.s_key = k.str();
         }
@@ -1475,7 +1492,8 @@ private:
             }
             src << "    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < n; r += (long long)gridDim.x * 256) {\n";
             if (nullable(jp.pk_col)) src << "        if (!chip::bit32(P.dvalid[" << pk << "], r)) continue;\n";
-            src << "        const unsigned int x = (unsigned int)P.dcol[" << pk << "][r] - (unsigned int)" << jp.dmin << ";\n"
+            src << "        const unsigned int x = chip::dense_index(P.dcol[" << pk << "][r], " << jp.dmin << ", " << jp.dspan << ", "
+                << jp.cal << ");\n"
                 << "        if (x >= " << jp.dspan << "u) continue;\n"
                 << "        unsigned int e = " << jp.s_present << "u;\n";
             for (size_t a = 0; a < jp.s_cols.size(); ++a)
@@ -1511,7 +1529,7 @@ private:
                 if (jp.zmode == 3) ze = entry("(u32)(0 - " + std::to_string(jp.dmin) + ")");
                 sh << "            if (fk == 0) { z" << J << "[i] = true; e" << J << "[i] = " << ze << "; continue; }\n";
             }
-            sh << "            const u32 x = (u32)fk - (u32)" << jp.dmin << ";\n"
+            sh << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
                << "            if (x >= " << jp.dspan << "u) continue;\n"
                << "            const u32 e = " << entry("x") << ";\n";
             if (jp.s_present) sh << "            if (!(e & 1u)) continue;\n";

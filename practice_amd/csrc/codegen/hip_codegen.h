@@ -34,6 +34,7 @@ struct CodegenOptions {
     bool allow_lds_aggregation = true;
     bool allow_compact_tables = true;
     bool allow_dense_tables = true;   // key-indexed tables where the keys allow
+    bool calendar_tables = true;      // key-indexed tables of yyyymmdd keys indexed by (year, month, day)
     bool prefetch = false;            // load tile t+stride while tile t is processed
     bool stream_large_tables = false; // streaming loads for probes of HBM tables beyond L1 (measured 2x slower on C3: off)
     // Also emit the per-tile pipeline body for the batched shared scan
@@ -66,6 +67,7 @@ struct JoinProgram {
     int doff = 1;             // 1: code 0 marks an absent key; 0: every key present
     int db = 0;               // chosen width (exact in LDS, power of two in HBM)
     int zmode = 0, zx = 0;    // probe key 0 vs. the reference's empty-slot match (chip::ht_probe_dense)
+    int cal = 0;              // calendar index from this first year (keys yyyymmdd; chip::dense_index), 0: key - dmin
     // batched templates: one table per template, every dimension row
     // inserted with its packed attributes (payload + build-filter columns);
     // each query applies its own build filter to the attributes

@@ -92,6 +92,19 @@ __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) 
     const u32 b = x * (u32)db;
     return (ht_word<STREAM>(w, b >> 5) >> (b & 31)) & ((1u << db) - 1u);
 }
+// Entry index of key k: k - dmin, or for calendar keys (cal = first year,
+// every table key a yyyymmdd date) ((year - cal) * 12 + month - 1) * 31 +
+// day - 1, which packs 7 years of dates into 2,604 entries instead of the
+// 61,131 of the numeric range.  An index >= dspan: no entry (also every key
+// that is not such a date, since no table key is one).
+__device__ __forceinline__ u32 dense_index(int k, int dmin, int dspan, int cal) {
+    if (!cal) return (u32)k - (u32)dmin;
+    if (k < 0) return (u32)dspan;
+    const int y = k / 10000, md = k - y * 10000, m = md / 100, d = md - m * 100;
+    if (m < 1 || m > 12 || d < 1 || d > 31 || y < cal) return (u32)dspan;
+    return (u32)(((y - cal) * 12 + (m - 1)) * 31 + (d - 1));
+}
+
 // Probe key 0 in the zero-filled reference table matches an EMPTY slot
 // (key 0 == empty key, payload 0).  `zmode` reproduces that (fixed per join
 // by the code generator from the reference slot of key 0):
@@ -101,7 +114,7 @@ __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) 
 //   3  key 0 itself owns the slot: key 0 always matches, payload if inserted
 template <bool ALIGNED, bool STREAM = false>
 __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, int dspan, int db, int pbase, bool off,
-                                               int zmode, int zx, int& payload) {
+                                               int zmode, int zx, int& payload, int cal = 0) {
     if (zmode != 0 && fk == 0) {
         if (zmode == 1) {
             payload = 0;
@@ -112,7 +125,7 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
         payload = (zmode == 3 && in0) ? pbase + (int)e0 : 0;
         return zmode == 3 || !in0;
     }
-    const u32 x = (u32)fk - (u32)dmin;
+    const u32 x = dense_index(fk, dmin, dspan, cal);
     if (x >= (u32)dspan) return false;
     if (db == 0) {  // every key present, one payload value (or none)
         payload = pbase;
@@ -122,8 +135,8 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
     payload = pbase + (int)e;
     return !off || e != 0u;
 }
-__device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload) {
-    const u32 x = (u32)k - (u32)dmin;
+__device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload, int cal = 0) {
+    const u32 x = dense_index(k, dmin, dspan, cal);
     if (x >= (u32)dspan || db == 0) return;
     const u32 c = (u32)(payload - pbase);
     const u32 b = x * (u32)db;

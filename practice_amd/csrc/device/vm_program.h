@@ -60,6 +60,7 @@ struct cq_vm_join {
     int kmin, len;        // reference slot function
     int eb, tb, jlo, pmin;             // compact
     int dmin, dspan, db, pbase, doff, zmode, zx;  // key-indexed
+    int cal;                           // calendar index from this first year (0: key - dmin)
     int build_begin, build_end;        // build-filter program (code range; empty: every row)
 };
 

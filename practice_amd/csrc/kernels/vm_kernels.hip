@@ -131,8 +131,8 @@ __device__ __forceinline__ bool probe(const cq_vm_join& J, const u32* ht, int fk
         case CQ_L_COMPACT: return chip::ht_probe_compact(ht, fk, J.kmin, J.len, J.eb, J.tb, J.jlo, J.pmin, payload);
         case CQ_L_DENSE:
             if ((J.db & (J.db - 1)) == 0)
-                return chip::ht_probe_dense<true>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload);
-            return chip::ht_probe_dense<false>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload);
+                return chip::ht_probe_dense<true>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload, J.cal);
+            return chip::ht_probe_dense<false>(ht, fk, J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, payload, J.cal);
         default: return chip::ht_probe_ref((const int*)ht, fk, J.kmin, J.len, J.payload_dslot >= 0, payload);
     }
 }
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(VM_BT) void k_vm_build(const cq_query_params P, con
         const int pay = J.payload_dslot >= 0 ? P.dcol[J.payload_dslot][c.row] : 0;
         switch (J.layout) {
             case CQ_L_COMPACT: chip::ht_insert_compact(P.ht[j], k, J.kmin, J.len, J.eb, J.tb, J.jlo, J.pmin, pay); break;
-            case CQ_L_DENSE: chip::ht_insert_dense(P.ht[j], k, J.dmin, J.dspan, J.db, J.pbase, pay); break;
+            case CQ_L_DENSE: chip::ht_insert_dense(P.ht[j], k, J.dmin, J.dspan, J.db, J.pbase, pay, J.cal); break;
             default: chip::ht_insert_ref((int*)P.ht[j], k, J.kmin, J.len, J.payload_dslot >= 0, pay); break;
         }
     }
@@ -672,7 +672,7 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                                     int pay = 0;
                                     if (alive[k])
                                         alive[k] = chip::ht_probe_dense<true>(ht, (int)(i64)R[a], J.dmin, J.dspan, J.db, J.pbase,
-                                                                              J.doff != 0, J.zmode, J.zx, pay);
+                                                                              J.doff != 0, J.zmode, J.zx, pay, J.cal);
                                     R[d] = (u64)(i64)pay;
                                 })
                             else
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
                                     int pay = 0;
                                     if (alive[k])
                                         alive[k] = chip::ht_probe_dense<false>(ht, (int)(i64)R[a], J.dmin, J.dspan, J.db, J.pbase,
-                                                                               J.doff != 0, J.zmode, J.zx, pay);
+                                                                               J.doff != 0, J.zmode, J.zx, pay, J.cal);
                                     R[d] = (u64)(i64)pay;
                                 })
                             break;

@@ -29,6 +29,9 @@ struct ColumnStatistics {
     bool is_unique_ = false;
     bool is_primary_key_ = false;
     bool is_nullable_ = false;
+    // every value is a calendar date written yyyymmdd (month 1-12, day 1-31):
+    // key-indexed tables may index such keys by (year, month, day)
+    bool is_calendar_ = false;
 };
 
 class TableMetadata {

@@ -11,6 +11,22 @@
 
 namespace db {
 
+namespace {
+
+// all values dates written yyyymmdd: month 1..12, day 1..31, 8 digits
+bool calendarColumn(const int32_t* d, int64_t rows, int64_t mn, int64_t mx) {
+    if (rows <= 0 || mn < 10000101 || mx > 99991231) return false;
+    for (int64_t i = 0; i < rows; ++i) {
+        const int32_t k = d[i];
+        const int32_t m = (k / 100) % 100, day = k % 100;
+        if (m < 1 || m > 12 || day < 1 || day > 31) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+
 // ------------------------------------------------------------------ synthetic schema
 
 namespace {
@@ -338,6 +354,7 @@ ColumnStatistics DatabaseInstance::computeStats(const std::string& column, bool&
         st.min_value_ = mn;
         st.max_value_ = mx;
         st.cardinality_ = card;
+        st.is_calendar_ = calendarColumn(d, dc.rows, mn, mx);
         return st;
     }
     hipStream_t s = ctx_->stream;
@@ -363,6 +380,13 @@ ColumnStatistics DatabaseInstance::computeStats(const std::string& column, bool&
     hipCheck(hipMemcpyAsync(&h[2], m + 2, 8, hipMemcpyDeviceToHost, s), "stats read");
     hipCheck(hipStreamSynchronize(s), "stats sync");
     st.cardinality_ = h[2];
+    // dimension-sized columns of 8-digit values: are they all yyyymmdd dates?
+    if (dc.rows <= (int64_t(1) << 24) && st.min_value_ >= 10000101 && st.max_value_ <= 99991231) {
+        std::vector<int32_t> host(static_cast<size_t>(dc.rows));
+        hipCheck(hipMemcpyAsync(host.data(), dc.data, static_cast<size_t>(dc.rows) * 4, hipMemcpyDeviceToHost, s), "calendar check");
+        hipCheck(hipStreamSynchronize(s), "calendar check");
+        st.is_calendar_ = calendarColumn(host.data(), dc.rows, st.min_value_, st.max_value_);
+    }
     return st;
 }
 

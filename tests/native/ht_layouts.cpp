@@ -149,7 +149,81 @@ int main(int argc, char** argv) {
                 return 1;
             }
     }
-    printf("ok %d cases; key-0 modes %lld %lld %lld %lld; key-0 matches %lld\n", cases, zseen[0], zseen[1], zseen[2],
-           zseen[3], zero_matches);
+    // calendar index: keys yyyymmdd dates (month 1-12, day 1-31) over a few
+    // years, the key-indexed table indexed by (year, month, day); probes of
+    // every value around the key range and of non-dates must match the
+    // reference table
+    long long cal_probes = 0;
+    for (int c = 0; c < 1 + cases / 10; ++c) {
+        const int y0 = 1990 + (int)(rng() % 20), ny = 1 + (int)(rng() % 8);
+        const bool pair = c % 3 != 0;
+        std::vector<int> keys;
+        for (int y = y0; y < y0 + ny; ++y)
+            for (int m = 1; m <= 12; ++m)
+                for (int d = 1; d <= 31; ++d)
+                    if (rng() % 8 != 0) keys.push_back(y * 10000 + m * 100 + d);
+        const int kmn = keys.front(), kmx = keys.back();
+        const int kmin = kmn - (int)(rng() % 100), len = kmx - kmin + 1 + (int)(rng() % 1000);
+        const int pmin = 1990, prange = (int)(rng() % 12);
+        std::vector<int> pay(keys.size());
+        for (auto& p : pay) p = pmin + (int)(rng() % (prange + 1));
+        std::vector<char> inserted(keys.size());
+        for (size_t i = 0; i < keys.size(); ++i) inserted[i] = rng() % 3 != 0;
+        std::vector<int> ref(2 * (size_t)len, 0);
+        for (size_t i = 0; i < keys.size(); ++i)
+            if (inserted[i]) {
+                const int sl = (keys[i] - kmin) % len;
+                atomicCAS(&ref[2 * sl], 0, keys[i]);
+                ref[2 * sl + 1] = pay[i];
+            }
+        const int cal = kmn / 10000, dspan = (kmx / 10000 - cal + 1) * 372, dmin = cal * 10000 + 101;
+        const int dbits = pair ? bits_for((uint64_t)prange + 1) : 1;
+        for (int aligned = 0; aligned < 2; ++aligned) {
+            int db = dbits;
+            if (aligned) {
+                int w = 1;
+                while (w < db) w <<= 1;
+                db = w;
+            }
+            const size_t dwords = ((size_t)dspan * db + 31) / 32 + 1;
+            std::vector<unsigned int> dbuf(dwords + 16, 0xdeadbeefu);
+            for (size_t i = 0; i < dwords; ++i) dbuf[8 + i] = 0;
+            const int pbase = (pair ? pmin : 0) - 1;
+            for (size_t i = 0; i < keys.size(); ++i)
+                if (inserted[i]) ht_insert_dense(&dbuf[8], keys[i], dmin, dspan, db, pbase, pair ? pay[i] : 0, cal);
+            for (int g = 0; g < 8; ++g)
+                if (dbuf[g] != 0xdeadbeefu || dbuf[8 + dwords + g] != 0xdeadbeefu) {
+                    printf("calendar case %d: insert wrote outside the table\n", c);
+                    return 1;
+                }
+            std::vector<int> probes;
+            for (int fk = kmin - 2000; fk <= kmx + 12000; ++fk) probes.push_back(fk);
+            const int odd[] = {0, -1, 1, -19920101, 19920100, 19921332, 19920001, 19921300, 2147483647, -2147483647 - 1,
+                               99991231, 10000101, kmn - 10000, kmx + 10000};
+            for (int v : odd) probes.push_back(v);
+            for (int r = 0; r < 2000; ++r) probes.push_back((int)(uint32_t)rng());
+            for (int fk : probes) {
+                const long long x = (long long)fk - kmin;
+                bool want = false;
+                int wp = 0;
+                if (x >= 0) {
+                    const int sl = (int)(x % len);
+                    want = ref[2 * sl] == fk;
+                    wp = pair ? ref[2 * sl + 1] : 0;
+                }
+                int dp = 0;
+                const bool dm = aligned ? ht_probe_dense<true>(&dbuf[8], fk, dmin, dspan, db, pbase, true, 0, 0, dp, cal)
+                                        : ht_probe_dense<false>(&dbuf[8], fk, dmin, dspan, db, pbase, true, 0, 0, dp, cal);
+                ++cal_probes;
+                if (dm != want || (want && pair && dp != wp)) {
+                    printf("calendar case %d: (%s) fk=%d got (%d,%d) want (%d,%d)\n", c, aligned ? "pow2" : "exact", fk, dm, dp,
+                           want, wp);
+                    return 1;
+                }
+            }
+        }
+    }
+    printf("ok %d cases; key-0 modes %lld %lld %lld %lld; key-0 matches %lld; calendar probes %lld\n", cases, zseen[0],
+           zseen[1], zseen[2], zseen[3], zero_matches, cal_probes);
     return 0;
 }

@@ -154,6 +154,18 @@ def test_constants_are_kernel_arguments(small_db):
     assert small_db.describe(Q.C1)["signature"] != a["signature"]
 
 
+def test_calendar_index_option(small_db):
+    small_db.set_option("calendar_tables", 0)
+    try:
+        dd = {j["dim"]: j for j in small_db.describe(Q.C3)["joins"]}["DDATE"]
+        assert dd["calendar"] == 0 and dd["layout"] == "key-indexed"
+        assert dd["bytes"] == ((61130 * 3 + 31) // 32 + 1) * 4
+    finally:
+        small_db.set_option("calendar_tables", 1)
+    dd = {j["dim"]: j for j in small_db.describe(Q.C3)["joins"]}["DDATE"]
+    assert dd["calendar"] == 1992
+
+
 def test_layout_decisions_for_configs(small_db):
     c3 = small_db.describe(Q.C3)
     # nation keys arrive in payload registers (no validity): no NULL code in the domain
@@ -163,9 +175,11 @@ def test_layout_decisions_for_configs(small_db):
     # every supplier / customer key present: 5-bit nation codes indexed by key, no absent code
     for d in ("SUPPLIER", "CUSTOMER"):
         assert joins[d]["layout"] == "key-indexed" and joins[d]["entry_bits"] == 5 and joins[d]["absent_code"] == 0
-    # 2556 dates over a 61131-key span: 3-bit year codes with code 0 = absent
+    # 2556 dates over a 61130-key span: 3-bit year codes with code 0 = absent,
+    # indexed by (year, month, day): 7 x 372 entries, under 1 KiB
     assert joins["DDATE"]["layout"] == "key-indexed" and joins["DDATE"]["entry_bits"] == 3
-    assert joins["DDATE"]["absent_code"] == 1
+    assert joins["DDATE"]["absent_code"] == 1 and joins["DDATE"]["calendar"] == 1992
+    assert joins["DDATE"]["bytes"] == ((7 * 372 * 3 + 31) // 32 + 1) * 4 and joins["DDATE"]["lds"] == 1
     assert c3["scan_lds_bytes"] <= 160 * 1024
     c1 = small_db.describe(Q.C1)
     assert c1["fast_sparse"] == 1 and c1["joins"][0]["payload"] == "" and c1["joins"][0]["entry_bits"] == 1
