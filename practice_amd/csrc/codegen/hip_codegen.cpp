@@ -529,7 +529,7 @@ private:
             const int w = lds ? jp.dbits : pow2Width(jp.dbits);
             const uint64_t dw = denseWords(jp, w);
             const uint64_t cw = (static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32;
-            if (w <= 31 && dw <= cw) {
+            if (w <= 31 && dw <= cw + 1) {  // + the padding word: at equal widths the key-indexed probe is cheaper
                 jp.dense = true;
                 jp.db = w;
                 jp.ht_words = dw;
@@ -1113,6 +1113,49 @@ private:
                 const std::string table = jp.lds ? "s_ht" + j : "P.ht[" + j + "]";
                 body << "        // probe " << jp.dim_table << " on " << jp.fk_col << "\n";
                 body << "        int p" << j << "[IPT];\n";
+                if (opt_.split_probes && (jp.dense || jp.compact)) {
+                    // locate every row's entry, issue all loads, then decode:
+                    // one table round trip per tile instead of one per row
+                    const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
+                    const std::string st = stream ? "true" : "false";
+                    const std::string c = "c" + std::to_string(fs) + "[i]";
+                    std::string live = flGet();
+                    if (prog_.fact_nullable[static_cast<size_t>(fs)]) live = "(" + live + " && ((v" + std::to_string(fs) + " >> i) & 1u))";
+                    body << "        {\n";
+                    if (jp.dense) {
+                        const std::string al = (jp.db & (jp.db - 1)) == 0 ? "true" : "false";
+                        body << "        u32 q" << j << "[IPT], x" << j << "[IPT], l" << j << "[IPT], h" << j << "[IPT];\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                             << "            q" << j << "[i] = chip::dense_locate(" << c << ", " << live << ", " << jp.dmin << ", "
+                             << jp.dspan << ", " << jp.zmode << ", " << jp.zx << ", " << jp.cal << ", x" << j << "[i]);\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                             << "            chip::dense_words<" << al << ", " << st << ">(" << table << ", x" << j << "[i], " << jp.db
+                             << ", l" << j << "[i], h" << j << "[i]);\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                             << "            p" << j << "[i] = 0;\n"
+                             << "            if (!chip::dense_decode<" << al << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
+                             << "[i], " << jp.db << ", " << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j
+                             << "[i])) " << flClear() << "\n"
+                             << "        }\n";
+                    } else {
+                        body << "        int s" << j << "[IPT]; u32 w" << j << "[IPT];\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                             << "            w" << j << "[i] = chip::compact_locate(" << c << ", " << live << ", " << jp.kmin << ", "
+                             << jp.len << ", " << jp.eb << ", s" << j << "[i]);\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) w" << j << "[i] = chip::ht_word<" << st << ">("
+                             << table << ", w" << j << "[i]);\n"
+                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                             << "            p" << j << "[i] = 0;\n"
+                             << "            if (!chip::compact_decode(w" << j << "[i], s" << j << "[i], " << c << ", " << jp.kmin << ", "
+                             << jp.len << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j << "[i])) "
+                             << flClear() << "\n"
+                             << "        }\n";
+                    }
+                    body << "        }\n"
+                         << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
+                    visible_mask_ |= 1u << op.join;
+                    continue;
+                }
                 body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
                      << "            p" << j << "[i] = 0;\n"
                      << "            if (!" << flGet() << ") continue;\n";
@@ -1513,6 +1556,37 @@ private:
             const std::string J = std::to_string(j);
             // e: entry, m: the key is in the table, z: probe key 0 (the
             // reference's empty-slot corner); m and z are lane masks
+            if (opt_.split_probes) {
+                // entry index of every row first, then all loads, then the
+                // present test (a dead row reads entry 0)
+                sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
+                   << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n"
+                   << "        {\n        u32 x" << J << "[IPT];\n"
+                   << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                   << "            x" << J << "[i] = 0u; m" << J << "[i] = false; z" << J << "[i] = false;\n"
+                   << "            if (!((tile_fl >> i) & 1u)) continue;\n";
+                if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
+                sh << "            const int fk = c" << fs << "[i];\n";
+                if (jp.zmode != 0) {
+                    const std::string zx = jp.zmode == 1 ? "0u" : jp.zmode == 2 ? std::to_string(jp.zx) + "u"
+                                                                             : "(u32)(0 - " + std::to_string(jp.dmin) + ")";
+                    sh << "            if (fk == 0) { z" << J << "[i] = true; x" << J << "[i] = " << zx << "; continue; }\n";
+                }
+                sh << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
+                   << "            if (x >= " << jp.dspan << "u) continue;\n"
+                   << "            x" << J << "[i] = x; m" << J << "[i] = true;\n"
+                   << "        }\n";
+                if (jp.db == 0) sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) e" << J << "[i] = 0u;\n";
+                else
+                    sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) e" << J << "[i] = chip::dense_entry_aligned(cq_sht" << J
+                       << ", x" << J << "[i], " << jp.db << ");\n";
+                sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n";
+                if (jp.s_present) sh << "            if (m" << J << "[i] && !(e" << J << "[i] & 1u)) m" << J << "[i] = false;\n";
+                if (jp.zmode == 1) sh << "            if (z" << J << "[i]) e" << J << "[i] = 0u;\n";
+                sh << "            if (!m" << J << "[i] && !z" << J << "[i]) e" << J << "[i] = 0u;\n"
+                   << "        }\n        }\n";
+                continue;
+            }
             sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
                << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n"
                << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
@@ -2300,6 +2374,22 @@ private:
             aggValue = rq.aggValue;
             code << "            {\n"
                  << "            u32 want = 1u;\n";
+        } else if (opt_.split_probes) {
+            // group slots and flag words of every row first (all flag reads
+            // in flight at once), then the updates
+            code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n"
+                 << "        int gh[IPT]; u32 gw[IPT];\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            gh[i] = -1;\n"
+                 << "            if (!" << flGet() << ") continue;\n"
+                 << "            const int h = " << groupHash() << ";\n"
+                 << "            if ((unsigned)h < " << G << "u) gh[i] = h;\n"
+                 << "        }\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) gw[i] = s_flag[(gh[i] < 0 ? 0 : gh[i]) >> " << lp << "];\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            if (gh[i] < 0) continue;\n"
+                 << "            const int h = gh[i];\n"
+                 << "            u32 want = 1u;\n";
         } else {
             code << "        // GROUP BY: perfect hash, LDS-privatized aggregates\n#pragma unroll\n"
                  << "        for (int i = 0; i < IPT; ++i) {\n"
@@ -2327,9 +2417,14 @@ private:
             if (db > 0) code << "                want |= 1u << " << db << ";\n";
             code << "            }\n";
         }
-        code << "            { u32* fw = &s_flag[h >> " << lp << "]; const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
-             << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n"
-             << "        }\n";
+        if (!batch_form && opt_.split_probes)
+            code << "            { const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
+                 << "u; if (((gw[i] >> sh) & want) != want) atomicOr(&s_flag[h >> " << lp << "], want << sh); }\n"
+                 << "        }\n";
+        else
+            code << "            { u32* fw = &s_flag[h >> " << lp << "]; const u32 sh = (u32)(h & " << (per - 1) << ") * " << W
+                 << "u; if (((*fw >> sh) & want) != want) atomicOr(fw, want << sh); }\n"
+                 << "        }\n";
         if (batch_form) code << "        }\n";  // round loop
         // merge into the global slots once per block
         if (!batch_form) epi << "    __syncthreads();\n";

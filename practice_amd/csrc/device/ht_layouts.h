@@ -16,8 +16,6 @@ namespace chip {
 typedef unsigned int u32;
 typedef unsigned long long u64;
 
-// Reference slot: (k - kmin) % len in int arithmetic.  A negative k - kmin
-// (an out-of-bounds access in the reference) never matches here.
 // A table word.  STREAM: a table much larger than L1 that is probed at
 // random; its lines are loaded non-temporally so that they do not push the
 // small tables probed beside it out of L1.
@@ -29,6 +27,8 @@ __device__ __forceinline__ u32 ht_word(const u32* w, u32 i) {
     return w[i];
 }
 
+// Reference slot: (k - kmin) % len in int arithmetic.  A negative k - kmin
+// (an out-of-bounds access in the reference) never matches here.
 __device__ __forceinline__ int ht_slot(int k, int kmin, int len, int& x) {
     x = (int)((u32)k - (u32)kmin);
     return x < len ? x : x % len;
@@ -50,6 +50,31 @@ __device__ __forceinline__ bool ht_probe_compact(const u32* w, int fk, int kmin,
     const u64 bit = (u64)(u32)s * (u32)eb;
     const u32 mask = eb >= 32 ? 0xffffffffu : ((1u << eb) - 1u);
     const u32 e = (ht_word<STREAM>(w, (u32)(bit >> 5)) >> (bit & 31)) & mask;
+    const bool occ = (e & 1u) != 0;
+    const int tag = tb ? (int)((e >> 1) & ((1u << tb) - 1u)) : 0;
+    const int key = occ ? (int)((u32)kmin + (u32)s + (u32)((jlo + tag) * len)) : 0;
+    payload = occ ? pmin + (int)(e >> (1 + tb)) : 0;
+    return key == fk;
+}
+
+// The same probe in three steps, so that a thread can issue the loads of
+// all its rows before it waits for the first (a wave then has one table
+// round trip per tile instead of one per row):
+//   compact_locate  slot of fk (a dead or unmatched row reads word 0)
+//   ht_word         the entry's word
+//   compact_decode  match and payload, as ht_probe_compact
+__device__ __forceinline__ u32 compact_locate(int fk, bool live, int kmin, int len, int eb, int& s) {
+    int x;
+    s = ht_slot(fk, kmin, len, x);
+    if (!live || x < 0) s = -1;
+    return s < 0 ? 0u : (u32)(((u64)(u32)s * (u32)eb) >> 5);
+}
+__device__ __forceinline__ bool compact_decode(u32 word, int s, int fk, int kmin, int len, int eb, int tb, int jlo, int pmin,
+                                               int& payload) {
+    if (s < 0) return false;
+    const u64 bit = (u64)(u32)s * (u32)eb;
+    const u32 mask = eb >= 32 ? 0xffffffffu : ((1u << eb) - 1u);
+    const u32 e = (word >> (bit & 31)) & mask;
     const bool occ = (e & 1u) != 0;
     const int tag = tb ? (int)((e >> 1) & ((1u << tb) - 1u)) : 0;
     const int key = occ ? (int)((u32)kmin + (u32)s + (u32)((jlo + tag) * len)) : 0;
@@ -135,6 +160,47 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
     payload = pbase + (int)e;
     return !off || e != 0u;
 }
+// The key-indexed probe in three steps (see compact_locate):
+//   dense_locate  entry x to read and what decides the match (kind):
+//                 0 no match, 1 the entry (present code), 2 key 0 matches
+//                 iff entry zx is absent, 3 key 0 matches (payload if
+//                 present), 4 key 0 matches with payload 0
+//   dense_words   the entry's word (and the next one for exact widths)
+//   dense_decode  match and payload, as ht_probe_dense
+__device__ __forceinline__ u32 dense_locate(int fk, bool live, int dmin, int dspan, int zmode, int zx, int cal, u32& x) {
+    x = 0u;
+    if (!live) return 0u;
+    if (zmode != 0 && fk == 0) {
+        if (zmode == 1) return 4u;
+        x = (u32)zx;
+        return (u32)zmode;
+    }
+    const u32 y = dense_index(fk, dmin, dspan, cal);
+    if (y >= (u32)dspan) return 0u;
+    x = y;
+    return 1u;
+}
+template <bool ALIGNED, bool STREAM = false>
+__device__ __forceinline__ void dense_words(const u32* w, u32 x, int db, u32& lo, u32& hi) {
+    const u32 b = x * (u32)db;
+    lo = db == 0 ? 0u : ht_word<STREAM>(w, b >> 5);
+    hi = (ALIGNED || db == 0) ? 0u : ht_word<STREAM>(w, (b >> 5) + 1);  // the padding word keeps this in the table
+}
+template <bool ALIGNED>
+__device__ __forceinline__ bool dense_decode(u32 kind, u32 x, u32 lo, u32 hi, int db, int pbase, bool off, int& payload) {
+    const u32 b = x * (u32)db;
+    const u32 mask = (1u << db) - 1u;
+    const u32 e = db == 0 ? 0u : (ALIGNED ? (lo >> (b & 31)) & mask : (u32)((((u64)hi << 32) | lo) >> (b & 31)) & mask);
+    const bool in = db == 0 || !off || e != 0u;
+    switch (kind) {
+        case 1u: payload = pbase + (int)e; return in;
+        case 2u: payload = 0; return !in;
+        case 3u: payload = in ? pbase + (int)e : 0; return true;
+        case 4u: payload = 0; return true;
+        default: return false;
+    }
+}
+
 __device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload, int cal = 0) {
     const u32 x = dense_index(k, dmin, dspan, cal);
     if (x >= (u32)dspan || db == 0) return;

@@ -134,12 +134,43 @@ int main(int argc, char** argv) {
                     printf("case %d: compact fk=%d got (%d,%d) want (%d,%d)\n", c, fk, cm, cp, want, wp);
                     return 1;
                 }
+                {  // the same probe in locate / load / decode steps; a dead row never matches
+                    int sl = 0, sp = 0;
+                    const u32 wi = compact_locate(fk, true, kmin, len, eb, sl);
+                    const bool sm = compact_decode(ht_word<false>(&cbuf[8], wi), sl, fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, sp);
+                    int dl = 0;
+                    const u32 wd = compact_locate(fk, false, kmin, len, eb, dl);
+                    if (sm != want || (want && pair && sp != wp) || wd != 0u ||
+                        compact_decode(ht_word<false>(&cbuf[8], wd), dl, fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, sp)) {
+                        printf("case %d: compact (split) fk=%d got (%d,%d) want (%d,%d)\n", c, fk, sm, sp, want, wp);
+                        return 1;
+                    }
+                }
                 const bool dm = aligned ? ht_probe_dense<true>(&dbuf[8], fk, kmn, dspan, db, pbase, doff != 0, zmode, zx, dp)
                                         : ht_probe_dense<false>(&dbuf[8], fk, kmn, dspan, db, pbase, doff != 0, zmode, zx, dp);
                 if (dm != want || (want && pair && dp != wp)) {
                     printf("case %d: key-indexed(%s) fk=%d got (%d,%d) want (%d,%d) zmode=%d\n", c, aligned ? "pow2" : "exact", fk,
                            dm, dp, want, wp, zmode);
                     return 1;
+                }
+                for (int live = 0; live < 2; ++live) {  // locate / load / decode steps
+                    u32 x = 0, lo = 0, hi = 0;
+                    int sp = 0;
+                    const u32 kind = dense_locate(fk, live != 0, kmn, dspan, zmode, zx, 0, x);
+                    bool sm;
+                    if (aligned) {
+                        dense_words<true>(&dbuf[8], x, db, lo, hi);
+                        sm = dense_decode<true>(kind, x, lo, hi, db, pbase, doff != 0, sp);
+                    } else {
+                        dense_words<false>(&dbuf[8], x, db, lo, hi);
+                        sm = dense_decode<false>(kind, x, lo, hi, db, pbase, doff != 0, sp);
+                    }
+                    const bool w2 = live && want;
+                    if (sm != w2 || (w2 && pair && sp != wp) || (w2 && !pair && sp != dp)) {
+                        printf("case %d: key-indexed(%s, split, live %d) fk=%d got (%d,%d) want (%d,%d) zmode=%d\n", c,
+                               aligned ? "pow2" : "exact", live, fk, sm, sp, w2, wp, zmode);
+                        return 1;
+                    }
                 }
             }
         }
@@ -215,6 +246,23 @@ int main(int argc, char** argv) {
                 const bool dm = aligned ? ht_probe_dense<true>(&dbuf[8], fk, dmin, dspan, db, pbase, true, 0, 0, dp, cal)
                                         : ht_probe_dense<false>(&dbuf[8], fk, dmin, dspan, db, pbase, true, 0, 0, dp, cal);
                 ++cal_probes;
+                {
+                    u32 x = 0, lo = 0, hi = 0;
+                    int sp = 0;
+                    const u32 kind = dense_locate(fk, true, dmin, dspan, 0, 0, cal, x);
+                    bool sm;
+                    if (aligned) {
+                        dense_words<true>(&dbuf[8], x, db, lo, hi);
+                        sm = dense_decode<true>(kind, x, lo, hi, db, pbase, true, sp);
+                    } else {
+                        dense_words<false>(&dbuf[8], x, db, lo, hi);
+                        sm = dense_decode<false>(kind, x, lo, hi, db, pbase, true, sp);
+                    }
+                    if (sm != dm || (sm && sp != dp)) {
+                        printf("calendar case %d: split probe fk=%d got (%d,%d) want (%d,%d)\n", c, fk, sm, sp, dm, dp);
+                        return 1;
+                    }
+                }
                 if (dm != want || (want && pair && dp != wp)) {
                     printf("calendar case %d: (%s) fk=%d got (%d,%d) want (%d,%d)\n", c, aligned ? "pow2" : "exact", fk, dm, dp,
                            want, wp);

@@ -134,6 +134,32 @@ def test_extra_query_parity(sql):
     assert_same(g, o)
 
 
+LAYOUT_VARIANTS = {
+    # compact tagged tables (split probes), reference int tables, one-call
+    # probes, numeric date index
+    "compact": {"dense_tables": 0},
+    "reference": {"dense_tables": 0, "compact_tables": 0},
+    "one_call_probes": {"split_probes": 0},
+    "numeric_dates": {"calendar_tables": 0},
+}
+LAYOUT_QUERIES = [Q.C3, EXTRA[0], EXTRA[10], EXTRA[17], EXTRA[22]]
+
+
+@pytest.mark.parametrize("variant", list(LAYOUT_VARIANTS))
+def test_layout_variants_parity(variant):
+    spec, data, odb, gdb = dataset("sf100")
+    opts = LAYOUT_VARIANTS[variant]
+    try:
+        for k, v in opts.items():
+            gdb.set_option(k, v)
+        for sql in LAYOUT_QUERIES:
+            print(f"[parity] variant={variant} sql={sql}", flush=True)
+            assert_same(gdb.execute(sql), odb.execute(gdb.explain(sql)))
+    finally:
+        for k in opts:
+            gdb.set_option(k, 1)
+
+
 @pytest.mark.parametrize("host_sort_rows", [0, 4096])
 def test_order_by_host_and_device_sort(host_sort_rows):
     """ORDER BY [LIMIT] results: sorted by the device bitonic kernels (0) or,

@@ -30,11 +30,29 @@ def test_layouts_match_reference_table(tmp_path):
 
 
 def _layout_calls(src):
+    """Table geometry per join as the build and the probe see it:
+    dense (dmin, dspan, db, pbase, cal), compact (kmin, len, eb, tb, jlo, pmin),
+    ref (kmin, len, pair)."""
+    def ints(text):
+        return [a.strip() for a in text.split(",")]
     builds, probes = {}, {}
     for m in re.finditer(r"ht_insert_(\w+)\((?:\(int\*\))?P\.ht\[(\d+)\], k, ([^;]*)\);", src):
-        builds[int(m.group(2))] = (m.group(1), m.group(3))
+        a = ints(m.group(3))
+        kind = m.group(1)
+        geo = {"dense": lambda: a[:4] + [a[5]], "compact": lambda: a[:6], "ref": lambda: a[:3]}[kind]()
+        builds[int(m.group(2))] = (kind, geo)
+    # one-call probes
     for m in re.finditer(r"ht_probe_(compact(?:<\w+>)?|dense<\w+(?:, \w+)?>|ref)\((?:\(const int\*\))?(?:s_ht|P\.ht\[)(\d+)\]?, c\d+\[i\], ([^)]*)\)", src):
-        probes[int(m.group(2))] = (m.group(1).split("<")[0], m.group(3))
+        kind, a = m.group(1).split("<")[0], ints(m.group(3))
+        geo = {"dense": lambda: a[:4] + [a[9] if len(a) > 9 else "0"], "compact": lambda: a[:6], "ref": lambda: a[:3]}[kind]()
+        probes[int(m.group(2))] = (kind, geo)
+    # locate / load / decode probes
+    loc = {int(m.group(2)): ints(m.group(1)) for m in re.finditer(r"dense_locate\(c\d+\[i\], [^,]*, ([^;]*), x(\d+)\[i\]\);", src)}
+    for m in re.finditer(r"dense_decode<\w+>\(q(\d+)\[i\], x\d+\[i\], l\d+\[i\], h\d+\[i\], (-?\d+), (-?\d+), \w+, p\d+\[i\]\)", src):
+        j = int(m.group(1))
+        probes[j] = ("dense", loc[j][:2] + [m.group(2), m.group(3), loc[j][4]])
+    for m in re.finditer(r"compact_decode\(w(\d+)\[i\], s\d+\[i\], c\d+\[i\], ([^;]*), p\d+\[i\]\)\)", src):
+        probes[int(m.group(1))] = ("compact", ints(m.group(2))[:6])
     return builds, probes
 
 
@@ -49,10 +67,5 @@ def test_builds_and_probes_agree_on_layout():
             kind_b, args_b = builds[j]
             kind_p, args_p = probes[j]
             assert kind_b == kind_p, (sql, j, kind_b, kind_p)
-            # same table geometry (key range / width / payload base) on both sides
-            nb = [a.strip() for a in args_b.split(",")]
-            np_ = [a.strip() for a in args_p.split(",")]
-            if kind_b == "dense":
-                assert nb[:4] == np_[:4], (sql, j, nb, np_)
-            elif kind_b == "compact":
-                assert nb[:6] == np_[:6], (sql, j, nb, np_)
+            # same table geometry (key range / width / payload base / index) on both sides
+            assert args_b == args_p, (sql, j, args_b, args_p)
