@@ -17,7 +17,8 @@
  *       (include/core/execution.h:297-301; DynamicLibraryExecutor,
  *       src/core/execution.cpp:161-192; generated ABI, visitor.cpp:4804)
  *   cq_db_create / cq_db_destroy / cq_db_generate_ssb / cq_db_create_table /
- *   cq_db_load_column / cq_db_set_primary_key / cq_db_finalize
+ *   cq_db_load_column / cq_db_set_primary_key / cq_db_finalize /
+ *   cq_db_load_reference_files / cq_reference_rows
  *       DatabaseInstance ctor, initCatalog, loadData
  *       (src/db/database_instance.cpp:21-174)
  *   cq_db_execute + cq_result_*
@@ -80,6 +81,28 @@ int cq_db_generate_ssb(cq_db* db, int sf, long long fact_rows, unsigned long lon
 int cq_db_create_table(cq_db* db, const char* table, long long rows, int is_fact);
 int cq_db_load_column(cq_db* db, const char* table, const char* column, const int32_t* data, const uint64_t* validity);
 int cq_db_set_primary_key(cq_db* db, const char* table, const char* column);
+/* The reference's on-disk column store, replacing DatabaseInstance::initCatalog
+ * + loadData (src/db/database_instance.cpp:57-174) with loadColumn /
+ * loadValidityBitmapIfExists (include/crystal/utils.h:142-254) and the
+ * statistics cache (include/core/catalog_cache.h:17-123):
+ *   data_dir     directory of TABLE<i> int32 column files (and validity files)
+ *   rows         rows of LINEORDER, SUPPLIER, CUSTOMER, PART, DDATE
+ *   stats_cache  "column size mtime min max cardinality" text file; NULL or ""
+ *                for none; stale or missing lines are recomputed and saved
+ *   columns      comma-separated subset of the schema; NULL or "" for all
+ * Finalizes the catalog.  A missing column file fails with "Cannot open
+ * file: <path>"; a validity file of the wrong size fails as the reference's. */
+int cq_db_load_reference_files(cq_db* db, const char* data_dir, const long long rows[5], const char* stats_cache,
+                               const char* columns);
+/* The row counts the reference compiles in for SF 10 and 20
+ * (include/crystal/utils.h:11-25); CQ_ERR_INVALID for any other sf. */
+int cq_reference_rows(int sf, long long rows[5]);
+/* Every loaded SSB column written in the reference's format; nullable
+ * columns get a validity file with the given suffix (".valid64", ".valid",
+ * ".validity", ".valid_bitmap", ".valid8", ".null64", ".null", ".nulls",
+ * ".null_bitmap", ".null8").  No reference counterpart (its loader is
+ * bench/loader); it lets the reference and this library read one data set. */
+int cq_db_save_reference_files(cq_db* db, const char* data_dir, const char* validity_suffix);
 int cq_db_finalize(cq_db* db);
 /* Catalog-only column and explicit statistics (planning without data). */
 int cq_db_declare_column(cq_db* db, const char* table, const char* column, int nullable);

@@ -50,6 +50,9 @@ def _load() -> ctypes.CDLL:
         "cq_db_create_table": (c_int, [c_void_p, c_char_p, c_ll, c_int]),
         "cq_db_load_column": (c_int, [c_void_p, c_char_p, c_char_p, c_void_p, c_void_p]),
         "cq_db_set_primary_key": (c_int, [c_void_p, c_char_p, c_char_p]),
+        "cq_db_load_reference_files": (c_int, [c_void_p, c_char_p, P(c_ll), c_char_p, c_char_p]),
+        "cq_reference_rows": (c_int, [c_int, P(c_ll)]),
+        "cq_db_save_reference_files": (c_int, [c_void_p, c_char_p, c_char_p]),
         "cq_db_finalize": (c_int, [c_void_p]),
         "cq_db_declare_column": (c_int, [c_void_p, c_char_p, c_char_p, c_int]),
         "cq_db_set_column_stats": (c_int, [c_void_p, c_char_p, c_ll, c_ll, c_ull]),
@@ -97,6 +100,14 @@ _lib = _load()
 
 def lib() -> ctypes.CDLL:
     return _lib
+
+
+def reference_rows(sf: int) -> list:
+    """Rows of LINEORDER, SUPPLIER, CUSTOMER, PART, DDATE the reference
+    compiles in for scale factor 10 or 20."""
+    arr = (ctypes.c_longlong * 5)()
+    _check(_lib.cq_reference_rows(sf, arr))
+    return list(arr)
 
 
 def _check(rc: int) -> None:
@@ -198,6 +209,25 @@ class Database:
     # ---- data
     def generate_ssb(self, sf: int = 1, fact_rows: int = 0, seed: int = 42, nullable: str = "") -> None:
         _check(_lib.cq_db_generate_ssb(self._h, sf, fact_rows, seed, nullable.encode()))
+
+    def load_reference_files(self, data_dir: str, rows=None, sf: Optional[int] = None, stats_cache: str = "",
+                             columns=None) -> None:
+        """The reference's on-disk store: TABLE<i> int32 column files and
+        validity files under data_dir, rows per table (LINEORDER, SUPPLIER,
+        CUSTOMER, PART, DDATE) given or taken from the reference's SF 10 / 20
+        constants, statistics through the text cache at stats_cache;
+        columns: all of the schema, or the names given.  Finalizes."""
+        if rows is None:
+            if sf is None:
+                raise ValueError("give rows or sf")
+            rows = reference_rows(sf)
+        arr = (ctypes.c_longlong * 5)(*[int(r) for r in rows])
+        cols = ",".join(columns) if columns else ""
+        _check(_lib.cq_db_load_reference_files(self._h, data_dir.encode(), arr, stats_cache.encode(), cols.encode()))
+
+    def save_reference_files(self, data_dir: str, validity_suffix: str = ".valid64") -> None:
+        """Every loaded SSB column in the reference's on-disk format."""
+        _check(_lib.cq_db_save_reference_files(self._h, data_dir.encode(), validity_suffix.encode()))
 
     def create_table(self, name: str, rows: int, is_fact: bool) -> None:
         _check(_lib.cq_db_create_table(self._h, name.encode(), rows, int(is_fact)))

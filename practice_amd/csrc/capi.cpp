@@ -7,6 +7,7 @@
 #include <string>
 
 #include "runtime/database.h"
+#include "runtime/storage.h"
 #include "runtime/execution.h"
 #include "serve/batch.h"
 #include "serve/server.h"
@@ -132,6 +133,45 @@ int cq_db_generate_ssb(cq_db* d, int sf, long long fact_rows, unsigned long long
         g.seed = seed;
         g.nullable = nullable ? nullable : "";
         d->inst->generateSSB(g);
+        return CQ_OK;
+    });
+}
+
+int cq_db_load_reference_files(cq_db* d, const char* data_dir, const long long rows[5], const char* stats_cache,
+                               const char* columns) {
+    return guarded([&] {
+        if (!data_dir || !rows) return fail(CQ_ERR_INVALID, "data_dir and rows are required");
+        int64_t r[5];
+        for (int i = 0; i < 5; ++i) r[i] = rows[i];
+        std::vector<std::string> only;
+        std::string cur;
+        for (const char* p = columns ? columns : ""; ; ++p) {
+            if (*p == ',' || *p == 0) {
+                if (!cur.empty()) only.push_back(cur);
+                cur.clear();
+                if (*p == 0) break;
+            } else if (*p != ' ') {
+                cur += *p;
+            }
+        }
+        d->inst->loadReferenceFiles(data_dir, r, stats_cache ? stats_cache : "", only);
+        return CQ_OK;
+    });
+}
+
+int cq_reference_rows(int sf, long long rows[5]) {
+    return guarded([&] {
+        int64_t r[5];
+        if (!rows || !db::storage::referenceRows(sf, r)) return fail(CQ_ERR_INVALID, "reference row counts exist for SF 10 and 20");
+        for (int i = 0; i < 5; ++i) rows[i] = r[i];
+        return CQ_OK;
+    });
+}
+
+int cq_db_save_reference_files(cq_db* d, const char* data_dir, const char* validity_suffix) {
+    return guarded([&] {
+        if (!data_dir) return fail(CQ_ERR_INVALID, "data_dir is required");
+        d->inst->saveReferenceFiles(data_dir, validity_suffix ? validity_suffix : ".valid64");
         return CQ_OK;
     });
 }

@@ -1,5 +1,6 @@
 // DatabaseInstance implementation and the synthetic schema definition.
 #include "database.h"
+#include "storage.h"
 
 #include <algorithm>
 #include <cstring>
@@ -341,6 +342,18 @@ ColumnStatistics DatabaseInstance::computeStats(const std::string& column, bool&
     ColumnStatistics st;
     nullable = dc.validity != nullptr;
     if (dc.rows == 0) return st;
+    auto preset = preset_stats_.find(column);
+    if (preset != preset_stats_.end()) {
+        st.min_value_ = preset->second.min_value_;
+        st.max_value_ = preset->second.max_value_;
+        st.cardinality_ = preset->second.cardinality_;
+        if (dc.rows <= (int64_t(1) << 24) && st.min_value_ >= 10000101 && st.max_value_ <= 99991231) {
+            std::vector<int32_t> host(static_cast<size_t>(dc.rows));
+            fetchColumn(column, host.data(), nullptr);
+            st.is_calendar_ = calendarColumn(host.data(), dc.rows, st.min_value_, st.max_value_);
+        }
+        return st;
+    }
     if (device_ < 0) {
         const int32_t* d = dc.data;
         int64_t mn = d[0], mx = d[0];
@@ -399,6 +412,84 @@ void DatabaseInstance::finalizeCatalog() {
             if (st.cardinality_ == meta.getSize() && meta.getSize() > 0) st.is_unique_ = true;
             meta.setColumnStats(c, st);
             if (nul) meta.setColumnNullable(c, true);
+        }
+    }
+}
+
+void DatabaseInstance::loadReferenceFiles(const std::string& dir, const int64_t rows[5], const std::string& stats_cache,
+                                          const std::vector<std::string>& only) {
+    std::string d = dir;
+    if (!d.empty() && d.back() != '/') d += '/';
+    auto wanted = [&](const std::string& c) { return only.empty() || std::find(only.begin(), only.end(), c) != only.end(); };
+    for (const auto& c : only) (void)storage::tableOf(c);  // unknown names are errors
+    std::unique_ptr<storage::StatsCache> cache;
+    if (!stats_cache.empty()) cache = std::make_unique<storage::StatsCache>(stats_cache);
+    struct Pending {
+        std::string column;
+        uintmax_t size, mtime;
+    };
+    std::vector<Pending> pending;
+    const auto& tables = storage::referenceTables();
+    for (size_t t = 0; t < tables.size(); ++t) {
+        const std::string& table = tables[t];
+        bool any = false;
+        for (const auto& c : storage::referenceColumns(table)) any = any || wanted(c);
+        if (!any) continue;
+        if (rows[t] < 0) throw std::runtime_error("negative row count for " + table);
+        createTable(table, rows[t], table == "LINEORDER");
+        for (const auto& c : storage::referenceColumns(table)) {
+            if (!wanted(c)) continue;
+            const std::string path = d + storage::physicalName(c);
+            const std::vector<int32_t> data = storage::readColumnFile(path, rows[t]);
+            const std::vector<uint64_t> valid = storage::readValidity(d, c, rows[t]);
+            loadColumn(table, c, data.data(), valid.empty() ? nullptr : valid.data());
+            if (!cache) continue;
+            const uintmax_t size = storage::StatsCache::fileSize(path), mtime = storage::StatsCache::fileMtime(path);
+            storage::CachedStats cs;
+            if (cache->lookup(c, size, mtime, cs) && cs.cardinality > 0) {
+                ColumnStatistics st;
+                st.min_value_ = cs.min_val;
+                st.max_value_ = cs.max_val;
+                st.cardinality_ = static_cast<uint64_t>(cs.cardinality);
+                preset_stats_[c] = st;
+            } else {
+                pending.push_back({c, size, mtime});
+            }
+        }
+        const std::string pk = storage::referencePrimaryKey(table);
+        if (!pk.empty() && wanted(pk)) setPrimaryKey(table, pk);
+    }
+    finalizeCatalog();
+    if (!cache) return;
+    for (const auto& p : pending) {
+        const auto& meta = catalog_->getTableMetadata(storage::tableOf(p.column));
+        if (!meta.hasColumnStats(p.column)) continue;
+        const ColumnStatistics* st = &meta.getColumnStats(p.column);
+        storage::CachedStats cs;
+        cs.file_size = p.size;
+        cs.mtime = p.mtime;
+        cs.min_val = static_cast<int>(st->min_value_);
+        cs.max_val = static_cast<int>(st->max_value_);
+        cs.cardinality = static_cast<int>(st->cardinality_);
+        cache->put(p.column, cs);
+    }
+    cache->save();
+}
+
+void DatabaseInstance::saveReferenceFiles(const std::string& dir, const std::string& validity_suffix) const {
+    std::string d = dir;
+    if (!d.empty() && d.back() != '/') d += '/';
+    for (const auto& table : storage::referenceTables()) {
+        if (!catalog_->hasTable(table)) continue;
+        for (const auto& c : storage::referenceColumns(table)) {
+            auto it = columns_.find(c);
+            if (it == columns_.end()) continue;
+            const int64_t rows = it->second.rows;
+            std::vector<int32_t> data(static_cast<size_t>(rows));
+            std::vector<uint64_t> valid(static_cast<size_t>((rows + 63) / 64));
+            fetchColumn(c, data.data(), valid.data());
+            storage::writeColumnFile(d + storage::physicalName(c), data.data(), rows);
+            if (it->second.validity) storage::writeValidity(d, c, validity_suffix, valid.data(), rows);
         }
     }
 }

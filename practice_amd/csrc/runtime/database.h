@@ -54,6 +54,19 @@ public:
     void generateSSB(const GenSpec& spec);
     // Statistics for every column (min, max, distinct count, nullability).
     void finalizeCatalog();
+    // The reference's on-disk column store (runtime/storage.h; reference
+    // DatabaseInstance::initCatalog + loadData, database_instance.cpp:57-174):
+    // every SSB column (or those named in `only`) from `dir`, rows[5] per
+    // table in storage::referenceTables() order, validity bitmaps where a
+    // file exists (the column is then nullable), the dimension primary keys,
+    // and statistics from the text cache at `stats_cache` where its line
+    // matches the file (recomputed and written back otherwise; "" for none).
+    // Finalizes the catalog.
+    void loadReferenceFiles(const std::string& dir, const int64_t rows[5], const std::string& stats_cache,
+                            const std::vector<std::string>& only);
+    // Every loaded column of the SSB schema written in that format, with
+    // validity files (suffix ".valid64", ".null8", ...) for nullable columns.
+    void saveReferenceFiles(const std::string& dir, const std::string& validity_suffix) const;
 
     QueryResult executeQuery(const std::string& sql);
     // Scan only fact rows [row_begin, row_end) (serving / batching chunks).
@@ -93,6 +106,8 @@ private:
     std::unordered_map<std::string, std::vector<int32_t>> host_data_;        // host-only mode
     std::unordered_map<std::string, std::vector<uint64_t>> host_validity_;  // host-only mode
     size_t loaded_bytes_ = 0;
+    // min / max / distinct count known ahead of finalizeCatalog (statistics cache)
+    std::unordered_map<std::string, ColumnStatistics> preset_stats_;
     std::shared_ptr<PlanSignatureCache> cache_;
     std::shared_ptr<HipRtcCompiler> compiler_;
     std::shared_ptr<HipModuleExecutor> executor_;
