@@ -562,8 +562,10 @@ static cval float_literal(double v) {
 typedef struct {
     otable* dim;
     ocol* pk;
-    ocol* fk;          /* fact column */
+    ocol* fk;          /* fact column (the first key alternative) */
     int fk_idx;        /* index in fact table */
+    ocol* fks[MAXJ];   /* OR-key join: every fact key alternative, in condition order */
+    int nfk;
     ocol* payload;     /* NULL for PHT_1 */
     const expr* build_pred;
     int32_t kmin;
@@ -684,9 +686,15 @@ static int extract_keys(query* q, joininfo* ji, const expr* e) {
         otable* tb = NULL;
         ocol* cb = find_col(q->db, e->b->name, &tb);
         if (!ca || !cb) FAIL("join key column not found");
-        if (ji->pk) FAIL("joins with more than one key pair (OR-key multi-probe) are not supported");
-        if (ta == ji->dim) { ji->pk = ca; ji->fk = cb; }
-        else { ji->pk = cb; ji->fk = ca; }
+        /* computeBuildInfo's extractKeys (visitor.cpp:3921-3945): every key
+         * pair adds a fact key alternative of the one dimension key */
+        ocol* pk = ta == ji->dim ? ca : cb;
+        ocol* fk = ta == ji->dim ? cb : ca;
+        if (ji->pk && ji->pk != pk) FAIL("OR-key join over two dimension keys (%s, %s)", ji->pk->name, pk->name);
+        if (ji->nfk >= MAXJ) FAIL("too many join key alternatives");
+        ji->pk = pk;
+        ji->fks[ji->nfk++] = fk;
+        if (!ji->fk) ji->fk = fk;
         return 0;
     }
     if (e->k == E_AND || e->k == E_OR) {
@@ -723,9 +731,11 @@ static int build_info(query* q, const plan* join, const plan* build_side, joinin
     }
     if (extract_keys(q, ji, join->pred)) return -1;
     if (!ji->pk) FAIL("hash join without an equi-key condition");
-    otable* ft = NULL;
-    find_col(q->db, ji->fk->name, &ft);
-    if (ft != q->fact) FAIL("join key %s is not a fact-table column", ji->fk->name);
+    for (int k = 0; k < ji->nfk; ++k) {
+        otable* ft = NULL;
+        find_col(q->db, ji->fks[k]->name, &ft);
+        if (ft != q->fact) FAIL("join key %s is not a fact-table column", ji->fks[k]->name);
+    }
     ji->fk_idx = (int)(ji->fk - q->fact->cols);
     if (!(ji->pk->pk || ji->pk->unique || ji->pk->card == (uint64_t)ji->dim->rows))
         FAIL("non-unique build key %s needs a multi-value hash table (not supported)", ji->pk->name);
@@ -1152,10 +1162,14 @@ static int build_tables(query* q) {
     return 0;
 }
 
-static int probe(rowctx* rc, const joininfo* ji, int jidx) {
+static int key_valid(const ocol* c, int64_t r) { return !c->validity || ((c->validity[r >> 6] >> (r & 63)) & 1ULL); }
+
+/* probe with fact key alternative k */
+static int probe(rowctx* rc, const joininfo* ji, int jidx, int k) {
     const int64_t r = rc->row;
-    if (ji->fk->validity && !((ji->fk->validity[r >> 6] >> (r & 63)) & 1ULL)) return 0;
-    int32_t v = ji->fk->data[r];
+    const ocol* fk = ji->fks[k];
+    if (!key_valid(fk, r)) return 0;
+    int32_t v = fk->data[r];
     int32_t x = (int32_t)((uint32_t)v - (uint32_t)ji->kmin);
     if (x < 0) return 0;
     int64_t s = x % ji->len;
@@ -1538,6 +1552,36 @@ static int shape_result(const query* q, aggstate* s, orc_result* out) {
 /* driver                                                               */
 /* ------------------------------------------------------------------ */
 
+/* The pipeline from op i on for one fact row, in plan order.  A join with
+ * several fact key alternatives (lo_orderdate = d_datekey OR lo_commitdate =
+ * d_datekey) runs the rest of the pipeline once per alternative whose key is
+ * valid, differs from every valid earlier alternative's key, and matches
+ * (consumeHashJoinVector, visitor.cpp:4581-4638). */
+static void run_ops(const query* q, rowctx* rc, int i, aggstate* st) {
+    for (; i < q->nops; ++i) {
+        const pipeop* op = &q->ops[i];
+        if (op->kind == 0) {
+            const int flag = op->fast ? fast_filter(rc, op->pred) : (vvalid(rc, op->pred) && truthy(vexpr(rc, op->pred)));
+            if (!flag) return;
+            continue;
+        }
+        const joininfo* ji = &q->joins[op->join];
+        if (ji->nfk == 1) {
+            if (!probe(rc, ji, op->join, 0)) return;
+            continue;
+        }
+        for (int k = 0; k < ji->nfk; ++k) {
+            int dup = 0;
+            for (int p = 0; p < k && !dup; ++p)
+                dup = key_valid(ji->fks[p], rc->row) && ji->fks[p]->data[rc->row] == ji->fks[k]->data[rc->row];
+            if (dup || !probe(rc, ji, op->join, k)) continue;
+            run_ops(q, rc, i + 1, st);
+        }
+        return;
+    }
+    aggregate_row(rc, st);
+}
+
 static int run_scan(query* q, aggstate* total, int threads) {
     const int64_t N = q->fact->rows;
     const int64_t slots = q->G * q->storage_ts;
@@ -1566,17 +1610,7 @@ static int run_scan(query* q, aggstate* total, int threads) {
         rowctx rc;
         rc.q = q;
         rc.row = r;
-        int flag = 1;
-        for (int i = 0; i < q->nops && flag; ++i) {
-            const pipeop* op = &q->ops[i];
-            if (op->kind == 0) {
-                if (op->fast) flag = fast_filter(&rc, op->pred);
-                else flag = vvalid(&rc, op->pred) && truthy(vexpr(&rc, op->pred));
-            } else {
-                flag = probe(&rc, &q->joins[op->join], op->join);
-            }
-        }
-        if (flag) aggregate_row(&rc, &parts[tid]);
+        run_ops(q, &rc, 0, &parts[tid]);
     }
     *total = parts[0];
     for (int t = 1; t < nthr; ++t) {

@@ -390,9 +390,13 @@ private:
                 if (b->left->getType() == ExprType::COLUMN_REF && b->right->getType() == ExprType::COLUMN_REF) {
                     const auto& l = static_cast<const ColumnRefExpr*>(b->left.get())->column_name;
                     const auto& r = static_cast<const ColumnRefExpr*>(b->right.get())->column_name;
-                    if (!jp.pk_col.empty()) unsupported("joins with more than one key pair (OR-key multi-probe) are not supported yet");
-                    if (tableOf(l) == t) jp.pk_col = l, jp.fk_col = r;
-                    else jp.pk_col = r, jp.fk_col = l;
+                    // every key pair adds a fact key alternative (visitor.cpp:3921-3945)
+                    const std::string pk = tableOf(l) == t ? l : r, fk = tableOf(l) == t ? r : l;
+                    if (!jp.pk_col.empty() && jp.pk_col != pk)
+                        unsupported("OR-key join over two dimension keys (" + jp.pk_col + ", " + pk + ")");
+                    jp.pk_col = pk;
+                    jp.fk_alts.push_back(fk);
+                    if (jp.fk_col.empty()) jp.fk_col = fk;
                 }
             } else if (e->getType() == ExprType::OP_AND || e->getType() == ExprType::OP_OR) {
                 const auto* b = static_cast<const BinaryExpr*>(e);
@@ -402,8 +406,16 @@ private:
         };
         keys(join->join_condition.get());
         if (jp.pk_col.empty()) unsupported("hash join without an equi-key condition");
-        if (tableOf(jp.fk_col) != prog_.fact_table && !prog_.fact_table.empty())
-            unsupported("join key " + jp.fk_col + " is not a fact-table column");
+        for (const auto& fk : jp.fk_alts)
+            if (tableOf(fk) != prog_.fact_table && !prog_.fact_table.empty())
+                unsupported("join key " + fk + " is not a fact-table column");
+        if (jp.fk_alts.size() > 1) {
+            if (opt_.vm) unsupported("interpreted tier: OR-key joins run on compiled modules");
+            if (opt_.batch_body) unsupported("OR-key joins do not run in the shared scan");
+            for (const auto& other : prog_.joins)
+                if (&other != &jp && other.fk_alts.size() > 1) unsupported("more than one OR-key join in a plan");
+            if (jp.fk_alts.size() > 8) unsupported("more than 8 key alternatives in one join");
+        }
         if (!uniqueKey(t, jp.pk_col))
             unsupported("non-unique build key " + jp.pk_col + " needs a multi-value hash table (not supported yet)");
         // payload: aggregate columns first, then filter columns above the join
@@ -1090,99 +1102,64 @@ private:
         const int ipt = 4 * vpt;
 
         // ---- pipeline body first: it decides which fact columns are loaded
-        std::ostringstream body;
+        std::ostringstream body, orbody;
         visible_mask_ = 0;
+        // an OR-key join runs last, with the filters that read its payload
+        int multi = -1;
+        for (size_t j = 0; j < prog_.joins.size(); ++j)
+            if (prog_.joins[j].fk_alts.size() > 1) multi = static_cast<int>(j);
+        std::vector<Op> dependent;
         for (const Op& op : pipelineOrder()) {
-            if (op.kind == 0) {
-                if (op.fast) {
-                    const std::string c = fastCond(op.pred);
-                    body << "        // filter (fast path, raw values)\n"
-                         << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                         << "            if (!(" << c << ")) " << flClear() << "\n";
-                } else {
-                    const std::string c = andTerms({vvalid(op.pred), vexpr(op.pred)});
-                    body << "        // filter (three-valued logic)\n"
-                         << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                         << "            if (!(" << c << ")) " << flClear() << "\n";
+            if (multi >= 0 && op.kind == 1 && op.join == multi) continue;
+            if (multi >= 0 && op.kind == 0) {
+                std::vector<std::string> cols;
+                collectColumns(op.pred, cols);
+                bool dep = false;
+                for (const auto& c : cols) dep = dep || tableOf(c) == prog_.joins[static_cast<size_t>(multi)].dim_table;
+                if (dep) {
+                    dependent.push_back(op);
+                    continue;
                 }
-                body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
+            }
+            if (op.kind == 0) {
+                emitFilterOp(body, op);
             } else {
                 const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
                 const int fs = factSlot(jp.fk_col);
-                const std::string j = std::to_string(op.join);
-                const std::string table = jp.lds ? "s_ht" + j : "P.ht[" + j + "]";
-                body << "        // probe " << jp.dim_table << " on " << jp.fk_col << "\n";
-                body << "        int p" << j << "[IPT];\n";
-                if (opt_.split_probes && (jp.dense || jp.compact)) {
-                    // locate every row's entry, issue all loads, then decode:
-                    // one table round trip per tile instead of one per row
-                    const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
-                    const std::string st = stream ? "true" : "false";
-                    const std::string c = "c" + std::to_string(fs) + "[i]";
-                    std::string live = flGet();
-                    if (prog_.fact_nullable[static_cast<size_t>(fs)]) live = "(" + live + " && ((v" + std::to_string(fs) + " >> i) & 1u))";
-                    body << "        {\n";
-                    if (jp.dense) {
-                        const std::string al = (jp.db & (jp.db - 1)) == 0 ? "true" : "false";
-                        body << "        u32 q" << j << "[IPT], x" << j << "[IPT], l" << j << "[IPT], h" << j << "[IPT];\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                             << "            q" << j << "[i] = chip::dense_locate(" << c << ", " << live << ", " << jp.dmin << ", "
-                             << jp.dspan << ", " << jp.zmode << ", " << jp.zx << ", " << jp.cal << ", x" << j << "[i]);\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                             << "            chip::dense_words<" << al << ", " << st << ">(" << table << ", x" << j << "[i], " << jp.db
-                             << ", l" << j << "[i], h" << j << "[i]);\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                             << "            p" << j << "[i] = 0;\n"
-                             << "            if (!chip::dense_decode<" << al << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
-                             << "[i], " << jp.db << ", " << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j
-                             << "[i])) " << flClear() << "\n"
-                             << "        }\n";
-                    } else {
-                        body << "        int s" << j << "[IPT]; u32 w" << j << "[IPT];\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                             << "            w" << j << "[i] = chip::compact_locate(" << c << ", " << live << ", " << jp.kmin << ", "
-                             << jp.len << ", " << jp.eb << ", s" << j << "[i]);\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) w" << j << "[i] = chip::ht_word<" << st << ">("
-                             << table << ", w" << j << "[i]);\n"
-                             << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                             << "            p" << j << "[i] = 0;\n"
-                             << "            if (!chip::compact_decode(w" << j << "[i], s" << j << "[i], " << c << ", " << jp.kmin << ", "
-                             << jp.len << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j << "[i])) "
-                             << flClear() << "\n"
-                             << "        }\n";
-                    }
-                    body << "        }\n"
-                         << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
-                    visible_mask_ |= 1u << op.join;
-                    continue;
-                }
-                body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                     << "            p" << j << "[i] = 0;\n"
-                     << "            if (!" << flGet() << ") continue;\n";
-                if (prog_.fact_nullable[static_cast<size_t>(fs)])
-                    body << "            if (!((v" << fs << " >> i) & 1u)) { " << flClear() << " continue; }\n";
-                // tables beyond L1 (32 KiB) that stay in HBM are read with
-                // streaming loads: L1 keeps the small ones
-                const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
-                const std::string st = stream ? "true" : "false";
-                if (jp.dense)
-                    body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ", " << st << ">("
-                         << table << ", c" << fs << "[i], " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
-                         << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p"
-                         << j << "[i], " << jp.cal << ")) " << flClear() << "\n";
-                else if (jp.compact)
-                    body << "            if (!chip::ht_probe_compact<" << st << ">(" << table << ", c" << fs << "[i], " << jp.kmin << ", " << jp.len
-                         << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j
-                         << "[i])) " << flClear() << "\n";
-                else
-                    body << "            if (!chip::ht_probe_ref((const int*)" << table << ", c" << fs << "[i], " << jp.kmin << ", "
-                         << jp.len << ", " << (jp.pair ? "true" : "false") << ", p" << j << "[i])) " << flClear() << "\n";
-                body << "        }\n";
-                body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
-                visible_mask_ |= 1u << op.join;
+                emitProbe(body, op.join, "c" + std::to_string(fs) + "[i]",
+                          prog_.fact_nullable[static_cast<size_t>(fs)] ? "((v" + std::to_string(fs) + " >> i) & 1u)" : "");
             }
         }
-
+        if (multi >= 0) {
+            // OR-key join: the probe, the filters that read its payload and
+            // the aggregation once per key alternative (a row's later
+            // alternative is skipped where it equals a valid earlier key, so
+            // each matching dimension row counts once; visitor.cpp:4581-4638)
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(multi)];
+            const size_t na = jp.fk_alts.size();
+            std::vector<int> slots;
+            for (const auto& fk : jp.fk_alts) slots.push_back(factSlot(fk));
+            auto key = [&](size_t k) { return "c" + std::to_string(slots[k]) + "[i]"; };
+            auto keyValid = [&](size_t k) {
+                return prog_.fact_nullable[static_cast<size_t>(slots[k])] ? "((v" + std::to_string(slots[k]) + " >> i) & 1u)"
+                                                                          : std::string("1u");
+            };
+            orbody << "        // " << jp.dim_table << " joined on";
+            for (const auto& fk : jp.fk_alts) orbody << " " << fk;
+            orbody << ": the rest of the pipeline once per key alternative\n"
+                   << "        int kor[IPT];\n"
+                   << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                   << "            u32 ok = 1u;\n";
+            for (size_t k = 0; k < na; ++k)
+                orbody << "            " << (k ? "else if" : "if") << " (alt == " << k << ") { kor[i] = " << key(k) << "; ok = " << keyValid(k)
+                       << "; }\n";
+            for (size_t k = 0; k + 1 < na; ++k)
+                orbody << "            if (alt > " << k << " && " << keyValid(k) << " && " << key(k) << " == kor[i]) ok = 0u;\n";
+            orbody << "            if (!ok) " << flClear() << "\n"
+                   << "        }\n";
+            emitProbe(orbody, multi, "kor[i]", "");
+            for (const Op& op : dependent) emitFilterOp(orbody, op);
+        }
         // ---- aggregation
         std::ostringstream aggcode, epilogue, prologue;
         const auto* a = agg_;
@@ -1253,7 +1230,15 @@ private:
             src << "        }\n";
         }
         src << body.str();
-        src << aggcode.str();
+        if (multi >= 0) {
+            src << "        const u32 fl_or = fl;\n"
+                << "#pragma unroll 1\n"
+                << "        for (int alt = 0; alt < " << prog_.joins[static_cast<size_t>(multi)].fk_alts.size() << "; ++alt) {\n"
+                << "        fl = fl_or;\n"
+                << orbody.str() << aggcode.str() << "        }\n";
+        } else {
+            src << aggcode.str();
+        }
         src << "    }\n";
         src << epilogue.str();
         src << "}\n\n";
@@ -1262,6 +1247,91 @@ private:
         prog_.source = src.str();
         (void)a;
         setAlgorithmicBytes();
+    }
+
+    void emitFilterOp(std::ostringstream& body, const Op& op) {
+        if (op.fast) {
+            const std::string c = fastCond(op.pred);
+            body << "        // filter (fast path, raw values)\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                 << "            if (!(" << c << ")) " << flClear() << "\n";
+        } else {
+            const std::string c = andTerms({vvalid(op.pred), vexpr(op.pred)});
+            body << "        // filter (three-valued logic)\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                 << "            if (!(" << c << ")) " << flClear() << "\n";
+        }
+        body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
+    }
+
+    // Probe of join `join` with the per-row key expression `key` (of item i);
+    // keyValid: the key's validity bit expression, "" when it cannot be NULL.
+    void emitProbe(std::ostringstream& body, int join, const std::string& key, const std::string& keyValid) {
+        const JoinProgram& jp = prog_.joins[static_cast<size_t>(join)];
+        const std::string j = std::to_string(join);
+        const std::string table = jp.lds ? "s_ht" + j : "P.ht[" + j + "]";
+        // tables beyond L1 (32 KiB) that stay in HBM may be read with
+        // streaming loads (L1 keeps the small ones)
+        const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
+        const std::string st = stream ? "true" : "false";
+        body << "        // probe " << jp.dim_table << " on " << (key.rfind("kor", 0) == 0 ? "the alternative's key" : jp.fk_col) << "\n";
+        body << "        int p" << j << "[IPT];\n";
+        if (opt_.split_probes && (jp.dense || jp.compact)) {
+            // locate every row's entry, issue all loads, then decode:
+            // one table round trip per tile instead of one per row
+            std::string live = flGet();
+            if (!keyValid.empty()) live = "(" + live + " && " + keyValid + ")";
+            body << "        {\n";
+            if (jp.dense) {
+                const std::string al = (jp.db & (jp.db - 1)) == 0 ? "true" : "false";
+                body << "        u32 q" << j << "[IPT], x" << j << "[IPT], l" << j << "[IPT], h" << j << "[IPT];\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                     << "            q" << j << "[i] = chip::dense_locate(" << key << ", " << live << ", " << jp.dmin << ", "
+                     << jp.dspan << ", " << jp.zmode << ", " << jp.zx << ", " << jp.cal << ", x" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                     << "            chip::dense_words<" << al << ", " << st << ">(" << table << ", x" << j << "[i], " << jp.db
+                     << ", l" << j << "[i], h" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                     << "            p" << j << "[i] = 0;\n"
+                     << "            if (!chip::dense_decode<" << al << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
+                     << "[i], " << jp.db << ", " << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j
+                     << "[i])) " << flClear() << "\n"
+                     << "        }\n";
+            } else {
+                body << "        int s" << j << "[IPT]; u32 w" << j << "[IPT];\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                     << "            w" << j << "[i] = chip::compact_locate(" << key << ", " << live << ", " << jp.kmin << ", "
+                     << jp.len << ", " << jp.eb << ", s" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) w" << j << "[i] = chip::ht_word<" << st << ">("
+                     << table << ", w" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                     << "            p" << j << "[i] = 0;\n"
+                     << "            if (!chip::compact_decode(w" << j << "[i], s" << j << "[i], " << key << ", " << jp.kmin << ", "
+                     << jp.len << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j << "[i])) "
+                     << flClear() << "\n"
+                     << "        }\n";
+            }
+            body << "        }\n";
+        } else {
+            body << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            p" << j << "[i] = 0;\n"
+                 << "            if (!" << flGet() << ") continue;\n";
+            if (!keyValid.empty()) body << "            if (!" << keyValid << ") { " << flClear() << " continue; }\n";
+            if (jp.dense)
+                body << "            if (!chip::ht_probe_dense<" << ((jp.db & (jp.db - 1)) == 0 ? "true" : "false") << ", " << st << ">("
+                     << table << ", " << key << ", " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", " << densePbase(jp) << ", "
+                     << (jp.doff ? "true" : "false") << ", " << jp.zmode << ", " << jp.zx << ", p" << j << "[i], " << jp.cal << ")) "
+                     << flClear() << "\n";
+            else if (jp.compact)
+                body << "            if (!chip::ht_probe_compact<" << st << ">(" << table << ", " << key << ", " << jp.kmin << ", " << jp.len
+                     << ", " << jp.eb << ", " << jp.tb << ", " << jp.jlo << ", " << jp.pmin << ", p" << j << "[i])) " << flClear() << "\n";
+            else
+                body << "            if (!chip::ht_probe_ref((const int*)" << table << ", " << key << ", " << jp.kmin << ", " << jp.len
+                     << ", " << (jp.pair ? "true" : "false") << ", p" << j << "[i])) " << flClear() << "\n";
+            body << "        }\n";
+        }
+        body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
+        visible_mask_ |= 1u << join;
     }
 
     // Loads one tile of every fact column into <pre>c<s>[IPT] / <pre>v<s>

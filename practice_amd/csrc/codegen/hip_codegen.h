@@ -53,6 +53,9 @@ struct CodegenOptions {
 
 struct JoinProgram {
     std::string dim_table, pk_col, fk_col, payload_col;
+    // OR-key join (lo_orderdate = d_datekey OR lo_commitdate = d_datekey):
+    // every fact key alternative in condition order (fk_col is the first)
+    std::vector<std::string> fk_alts;
     int64_t dim_rows = 0;
     int pk_dslot = -1, payload_dslot = -1;
     bool pair = false;        // PHT_2 (carries a payload) vs PHT_1

@@ -817,7 +817,15 @@ void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
     ctx->row_end = row_end;
     auto tree = buildOptimizedTree(sql, catalog_.get());
     const auto t1 = std::chrono::steady_clock::now();
-    ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
+    try {
+        ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
+    } catch (const std::runtime_error& e) {
+        // plans the interpreter does not take (OR-key joins) run compiled
+        if (!ctx->config.codegen.vm || std::string(e.what()).rfind("interpreted tier:", 0) != 0) throw;
+        CodegenOptions co = ctx->config.codegen;
+        co.vm = false;
+        ctx->program = generateQueryProgram(tree.get(), *catalog_, co);
+    }
     const auto t2 = std::chrono::steady_clock::now();
     QueryTimingStats timing;
     timing.parse_ms = elapsedMs(t0, t1);

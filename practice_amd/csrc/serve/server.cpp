@@ -157,7 +157,16 @@ bool QueryServer::prepare(Request& r) {
         co.batch_body = true;
         co.block_threads = bo.block_threads;
         co.vectors_per_thread = bo.vectors_per_thread;
-        r.program = std::make_shared<QueryProgram>(db_.engine().prepareCached(r.sql, co));
+        try {
+            r.program = std::make_shared<QueryProgram>(db_.engine().prepareCached(r.sql, co));
+        } catch (const std::runtime_error&) {
+            // not for the shared scan (an OR-key join): the plain module on a
+            // cold lane; a plan the plain path rejects as well fails here
+            CodegenOptions plain = db_.config().codegen;
+            plain.vm = false;
+            r.program = std::make_shared<QueryProgram>(db_.engine().prepareCached(r.sql, plain));
+            r.plain_only = true;
+        }
         r.footprint = estimateQueryTemporaryBytes(*r.program);
         if (r.footprint > opt_.budget_bytes) {
             finish(r, -5, "query needs " + std::to_string(r.footprint) + " bytes of device memory, over the serving budget of " +
@@ -215,6 +224,15 @@ void QueryServer::laneLoop(Lane* lane) {
             set = scan_set_;
             for (auto& r : ready) {
                 const std::string& sig = r.program->signature;
+                if (r.plain_only) {
+                    if (in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
+                        in_flight_bytes_ += r.footprint;
+                        plain.push_back(std::move(r));
+                    } else {
+                        later.push_back(std::move(r));
+                    }
+                    continue;
+                }
                 if (!inScanSet(sig)) {
                     const int n = ++seen_[sig];
                     if (n >= opt_.promote_after && !promoting_.count(sig)) {

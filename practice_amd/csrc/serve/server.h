@@ -85,6 +85,7 @@ private:
         std::shared_ptr<QueryProgram> program;   // prepared with the per-tile body
         size_t footprint = 0;
         bool interpreted = false;                 // cold request: interpreted tier (else its plain module)
+        bool plain_only = false;                  // the shared scan cannot take the plan (OR-key joins): cold lanes, compiled
     };
     struct Done {
         int rc = 0;

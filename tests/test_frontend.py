@@ -146,6 +146,20 @@ def test_parity_queries_compile_for_gfx950(small_db):
         small_db.compile(sql)
 
 
+def test_or_key_queries_compile_and_stay_off_the_interpreter(small_db):
+    from tests.test_gpu_parity import OR_KEY_SSB
+    for sql in OR_KEY_SSB:
+        small_db.compile(sql)
+        assert "for (int alt = 0; alt < 2; ++alt)" in small_db.generate_code(sql)
+    # the shared scan does not take them
+    with pytest.raises(pa.CrystalError, match="OR-key"):
+        small_db.generate_batch_code([OR_KEY_SSB[0]])
+    # two different dimension keys in one OR condition
+    with pytest.raises(pa.CrystalError, match="two dimension keys"):
+        small_db.describe("select count(*) from lineorder, ddate where lo_orderdate = d_datekey "
+                          "or lo_commitdate = d_yearmonthnum")
+
+
 def test_constants_are_kernel_arguments(small_db):
     a = small_db.describe(Q.c2(3, 1))
     b = small_db.describe(Q.c2(17, 4))

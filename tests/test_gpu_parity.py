@@ -20,6 +20,8 @@ DATASETS = {
     "sf1": (1, 1_000_000, 11, Q.C3_NULLABLE),
     "sf20": (20, 2_000_000, 12, ""),
     "sf100": (100, 2_000_000, 13, Q.C3_NULLABLE),
+    # nullable commit dates for the OR-key joins
+    "sf1c": (1, 1_000_000, 14, "LINEORDER.lo_commitdate:500,LINEORDER.lo_revenue:200"),
 }
 
 _cache = {}
@@ -196,3 +198,51 @@ def test_row_range_execution_sums_to_whole():
     a = gdb.execute(sql, 0, cut).tuples()[0]
     b = gdb.execute(sql, cut, n).tuples()[0]
     assert (a[0] + b[0], a[1] + b[1]) == whole
+
+
+OR_KEY_SSB = [
+    "select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
+    "or lo_commitdate = d_datekey group by d_year order by d_year",
+    "select d_year, p_brand1, sum(lo_revenue) from lineorder, ddate, part, supplier "
+    "where (lo_orderdate = d_datekey or lo_commitdate = d_datekey) and lo_partkey = p_partkey "
+    "and lo_suppkey = s_suppkey and p_category = 12 and s_region = 1 group by d_year, p_brand1 order by d_year, p_brand1",
+    "select count(*), sum(lo_extendedprice * lo_discount) from lineorder, ddate "
+    "where (lo_orderdate = d_datekey or lo_commitdate = d_datekey) and d_yearmonthnum = 199401",
+    "select d_year, count(*) from lineorder, ddate where (lo_orderdate = d_datekey or lo_commitdate = d_datekey) "
+    "and d_year - 1990 > lo_discount group by d_year",
+]
+
+
+@pytest.mark.parametrize("sql", OR_KEY_SSB)
+def test_or_key_join_parity(sql):
+    g, o = run_both("sf1c", sql)
+    assert_same(g, o)
+
+
+def test_or_key_join_crafted_keys():
+    """Equal order / commit dates, NULL commit dates, keys outside DDATE and
+    key 0, against the oracle; every probe form, and the interpreter option
+    (OR-key plans run compiled)."""
+    from tests.test_oracle import OR_KEY_QUERIES, or_key_tables
+    tables = or_key_tables(seed=5, n=300_000)
+    odb = orc.OracleDB()
+    with pa.Database(0) as g:
+        for t in ("LINEORDER", "DDATE"):
+            n = len(next(iter(tables[t].values()))[0])
+            odb.add_table(t, n, t == "LINEORDER")
+            g.create_table(t, n, t == "LINEORDER")
+            for c, (v, m) in tables[t].items():
+                odb.add_column(t, c, v, m)
+                g.load_column(t, c, v, m)
+        odb.set_primary_key("DDATE", "d_datekey")
+        g.set_primary_key("DDATE", "d_datekey")
+        odb.finalize()
+        g.finalize()
+        for opts in ({}, {"split_probes": 0}, {"calendar_tables": 0}, {"interpret": 1}):
+            for k, v in opts.items():
+                g.set_option(k, v)
+            for sql in OR_KEY_QUERIES:
+                print(f"[parity] or-key opts={opts} sql={sql}", flush=True)
+                assert_same(g.execute(sql), odb.execute(g.explain(sql)))
+            for k, v in opts.items():
+                g.set_option(k, 0 if k == "interpret" else 1)

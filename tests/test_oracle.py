@@ -253,3 +253,94 @@ def test_generator_domains_and_calendar():
     # any row range regenerates identically (counter-based streams)
     part, _ = orc.gen_column(spec, "LINEORDER", "lo_partkey", 4096, 1000)
     assert (part == fk[4096:5096]).all()
+
+
+# --------------------------------------------------------------- OR-key join
+def or_key_tables(seed=11, n=20000):
+    """DDATE of 400 dates in 1992-1993 and a fact table whose commit date
+    equals its order date for about a third of the rows, is NULL for a tenth,
+    and whose keys include dates outside DDATE and key 0."""
+    rng = np.random.default_rng(seed)
+    days = np.array([y * 10000 + m * 100 + d for y in (1992, 1993) for m in range(1, 13) for d in range(1, 29)], np.int32)
+    dates = np.sort(rng.choice(days, 400, replace=False)).astype(np.int32)
+    years = (dates // 10000).astype(np.int32)
+    probe = np.concatenate([dates, np.array([19910101, 0, 19940101], np.int32), days[:50]])
+    od = rng.choice(probe, n).astype(np.int32)
+    cd = np.where(rng.random(n) < 0.35, od, rng.choice(probe, n)).astype(np.int32)
+    cd_valid = rng.random(n) > 0.1
+    cd = np.where(cd_valid, cd, 0).astype(np.int32)
+    rev = rng.integers(0, 100000, n).astype(np.int32)
+    disc = rng.integers(0, 11, n).astype(np.int32)
+    valid_words = np.packbits(np.concatenate([cd_valid, np.zeros((-n) % 64, bool)]).astype(np.uint8),
+                              bitorder="little").view("<u8").copy()
+    return {"DDATE": {"d_datekey": (dates, None), "d_year": (years, None)},
+            "LINEORDER": {"lo_orderdate": (od, None), "lo_commitdate": (cd, valid_words), "lo_revenue": (rev, None),
+                          "lo_discount": (disc, None)}}
+
+
+OR_KEY_QUERIES = [
+    "select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
+    "or lo_commitdate = d_datekey group by d_year",
+    "select count(*), sum(lo_revenue) from lineorder, ddate where (lo_orderdate = d_datekey or lo_commitdate = d_datekey) "
+    "and d_year = 1993",
+    "select d_year, count(*) from lineorder, ddate where (lo_orderdate = d_datekey or lo_commitdate = d_datekey) "
+    "and d_year - 1990 > lo_discount group by d_year",
+]
+
+
+def or_key_dbs(tables):
+    import practice_amd as pa
+    odb = orc.OracleDB()
+    db = pa.Database(-1)
+    for t in ("LINEORDER", "DDATE"):
+        n = len(next(iter(tables[t].values()))[0])
+        odb.add_table(t, n, t == "LINEORDER")
+        db.create_table(t, n, t == "LINEORDER")
+        for c, (v, m) in tables[t].items():
+            odb.add_column(t, c, v, m)
+            db.load_column(t, c, v, m)
+    odb.set_primary_key("DDATE", "d_datekey")
+    db.set_primary_key("DDATE", "d_datekey")
+    odb.finalize()
+    db.finalize()
+    return odb, db
+
+
+def or_key_closed_form(tables, q):
+    """Per fact row: the order-date key, then the commit-date key when it is
+    valid and differs; each that names a DDATE row (passing the query's
+    filters) contributes that row's year once."""
+    d = tables["DDATE"]
+    year_of = dict(zip(d["d_datekey"][0].tolist(), d["d_year"][0].tolist()))
+    lo = tables["LINEORDER"]
+    od, cd, rev, disc = (lo[c][0].tolist() for c in ("lo_orderdate", "lo_commitdate", "lo_revenue", "lo_discount"))
+    n = len(od)
+    cv = np.unpackbits(lo["lo_commitdate"][1].view(np.uint8), bitorder="little")[:n].astype(bool).tolist()
+    groups = {}
+    for r in range(n):
+        keys = [od[r]] + ([cd[r]] if cv[r] and cd[r] != od[r] else [])
+        for k in keys:
+            y = year_of.get(k)
+            if y is None:
+                continue
+            if q == 1 and y != 1993:
+                continue
+            if q == 2 and not (y - 1990 > disc[r]):
+                continue
+            g = groups.setdefault(y if q != 1 else 0, [0, 0])
+            g[0] += 1
+            g[1] += rev[r]
+    if q == 0:
+        return sorted((y, c, s) for y, (c, s) in groups.items())
+    if q == 1:
+        c, s = groups.get(0, [0, 0])
+        return [(c, s)]
+    return sorted((y, c) for y, (c, _) in groups.items())
+
+
+def test_or_key_join_closed_form():
+    tables = or_key_tables()
+    odb, db = or_key_dbs(tables)
+    for q, sql in enumerate(OR_KEY_QUERIES):
+        got = odb.execute(db.explain(sql)).tuples()
+        assert sorted(got) == or_key_closed_form(tables, q), sql

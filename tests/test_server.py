@@ -90,3 +90,24 @@ def test_unsupported_request_fails_alone():
         assert_results_equal(srv.wait(good, 60_000), gdb.execute(Q.C1), Q.C1)
     finally:
         srv.stop()
+
+
+@pytest.mark.gpu
+def test_or_key_requests_run_on_cold_lanes():
+    """Plans the shared scan does not take (OR-key joins) run compiled on the
+    cold lanes, beside batched requests, with the same results."""
+    from tests.test_gpu_parity import OR_KEY_SSB, dataset
+    spec, data, odb, gdb = dataset("sf1c")
+    srv = pa.Server(gdb, {"lanes": 2, "max_batch": 16})
+    try:
+        qs = [OR_KEY_SSB[0], Q.C1, OR_KEY_SSB[2], Q.C1, OR_KEY_SSB[0]]
+        tickets = [(q, srv.submit(q)) for q in qs]
+        results = [(q, srv.wait(t, 120_000)) for q, t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st["completed"] == len(qs) and st["rejected"] == 0
+    for q, r in results:
+        print(f"[server] sql={q}", flush=True)
+        assert_results_equal(r, gdb.execute(q), q)
+        assert_same(r, odb.execute(gdb.explain(q)))
