@@ -189,13 +189,18 @@ def bench_served(pa, Q, args):
         return [Q.adhoc_query(rng) if rng.random() < args.serve_adhoc else Q.template_query(rng.randint(1, 4), rng)
                 for _ in range(n)]
 
+    first_error = []
+
     def drain(tickets):
         errors = 0
         for t in tickets:
             try:
                 srv.wait(t, 600_000)
-            except pa.CrystalError:
+            except pa.CrystalError as e:
                 errors += 1
+                if not first_error:
+                    first_error.append(str(e)[:300])
+                    print(f"[served] first error: {first_error[0]}", file=sys.stderr, flush=True)
         return errors
 
     rates = []
@@ -352,6 +357,17 @@ def main():
             extra[wl] = {"ms_per_step": round(ms2, 4), "GBps": round(dd["algorithmic_bytes"] / ms2 / 1e6, 2),
                          "scan_kernel_ms": round(sum(sm) / len(sm), 4)}
             d2.close()
+        # OR-key join over the same 600M-row fact table (two date keys probed per row)
+        for _ in range(args.warmup):
+            db.execute(Q.ORKEY)
+        t1 = time.perf_counter()
+        sm = []
+        for _ in range(args.steps):
+            sm.append(db.execute(Q.ORKEY).timing["scan_kernel_ms"])
+        ms2 = (time.perf_counter() - t1) * 1000 / args.steps
+        dd = db.describe(Q.ORKEY)
+        extra["orkey_sf100"] = {"ms_per_step": round(ms2, 4), "scan_kernel_ms": round(sum(sm) / len(sm), 4),
+                                "scan_GBps": round(dd["algorithmic_bytes"] / (sum(sm) / len(sm)) / 1e6, 2)}
         line["other_configs"] = extra
     db.close()
     if not args.no_batched:

@@ -76,3 +76,9 @@ def adhoc_query(rng) -> str:
         return f"select {agg} from lineorder where " + " and ".join(preds)
     return (f"select d_year, {agg} from lineorder, ddate where lo_orderdate = d_datekey and "
             + " and ".join(preds) + " group by d_year")
+
+
+# OR-key join: a fact row counts once per distinct date key (order / commit)
+# that names a DDATE row
+ORKEY = ("select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
+         "or lo_commitdate = d_datekey group by d_year")
