@@ -210,8 +210,9 @@ typedef struct expr {
     double fval;
     struct expr *a, *b, *c;
     /* resolved */
-    int src;      /* 0 fact column, 1 join payload */
+    int src;      /* 0 fact column, 1 join payload, 2 dimension column at the join's row id */
     int colidx;   /* fact column index or join index */
+    void* dcol;   /* src 2: the dimension column (ocol*) */
 } expr;
 
 typedef enum { P_SCAN, P_FILTER, P_JOIN, P_AGG, P_SORT, P_PROJECT } pkind;
@@ -235,6 +236,9 @@ typedef struct plan {
     int keycol[16];
     int keydesc[16];
     int64_t limit; /* -1: none */
+    /* projection */
+    int nexprs;
+    expr* exprs[64];
 } plan;
 
 typedef struct {
@@ -409,7 +413,16 @@ static plan* parse_plan(parser* p) {
         if (!(n->child[0] = parse_plan(p))) return NULL;
         n->nchild = 1;
     } else if (!strcmp(tok, "project")) {
-        FAILP("projection plans are not supported by the oracle");
+        n->k = P_PROJECT;
+        char t2[32];
+        if (pexpect(p, '(') || ptoken(p, t2, sizeof t2) || strcmp(t2, "exprs")) FAILP("project: expected (exprs");
+        while (!ppeek(p, ')')) {
+            if (n->nexprs >= 64) FAILP("too many projection expressions");
+            if (!(n->exprs[n->nexprs++] = parse_expr(p))) return NULL;
+        }
+        pexpect(p, ')');
+        if (!(n->child[0] = parse_plan(p))) return NULL;
+        n->nchild = 1;
     } else {
         FAILP("unknown plan node '%s'", tok);
     }
@@ -566,6 +579,7 @@ typedef struct {
     int fk_idx;        /* index in fact table */
     ocol* fks[MAXJ];   /* OR-key join: every fact key alternative, in condition order */
     int nfk;
+    int rowid;         /* payload is the dimension row index (projection of its columns) */
     ocol* payload;     /* NULL for PHT_1 */
     const expr* build_pred;
     int32_t kmin;
@@ -583,8 +597,12 @@ typedef struct {
 typedef struct {
     orc_db* db;
     otable* fact;
-    const plan* agg;
+    const plan* agg;   /* the aggregate, or the projection */
     const plan* sort;
+    const plan* proj;  /* projection root (else NULL) */
+    int nexprs;
+    expr* exprs[64];   /* projection output expressions (a star expanded) */
+    expr star_cols[64];
     int nops;
     pipeop ops[MAXOPS];
     int njoins;
@@ -617,6 +635,7 @@ static int resolve_expr(query* q, expr* e, int njoins_visible) {
             return 0;
         }
         for (int j = 0; j < njoins_visible; ++j) {
+            if (q->joins[j].rowid && q->joins[j].dim == t) { e->src = 2; e->colidx = j; e->dcol = c; return 0; }
             if (q->joins[j].payload == c) { e->src = 1; e->colidx = j; return 0; }
         }
         FAIL("column '%s' of dimension %s is not available in the probe pipeline "
@@ -754,7 +773,24 @@ static int build_info(query* q, const plan* join, const plan* build_side, joinin
         }
     }
     ji->payload = NULL;
-    for (int pass = 0; pass < 2 && !ji->payload; ++pass) {
+    ji->rowid = 0;
+    /* projectionRequiresRowIdPayload (visitor.cpp:3300-3317): a projection
+     * above that reads this dimension gathers its columns by row id */
+    for (const plan* cur = join->parent; cur; cur = cur->parent) {
+        if (cur->k != P_PROJECT) continue;
+        for (int i = 0; i < cur->nexprs; ++i) {
+            const char* cols[64];
+            int nc = 0;
+            if (cur->exprs[i]->k == E_STAR) ji->rowid = 1;
+            collect_cols(cur->exprs[i], cols, &nc, 64);
+            for (int c = 0; c < nc; ++c) {
+                otable* ct = NULL;
+                find_col(q->db, cols[c], &ct);
+                if (ct == ji->dim) ji->rowid = 1;
+            }
+        }
+    }
+    for (int pass = 0; pass < 2 && !ji->payload && !ji->rowid; ++pass) {
         const char** lst = pass == 0 ? aggc : fltc;
         int n = pass == 0 ? nagg : nflt;
         for (int i = 0; i < n; ++i) {
@@ -842,11 +878,51 @@ static int fast_sparse_eligible(query* q) {
     return 0;
 }
 
+/* Projection (consumeProjectionVector, visitor.cpp:3466-3548): one output
+ * row per row leaving the pipeline, each expression cast to its result type
+ * (resultLogicalTypeForExpr), validity from the expression's operands. */
+static int analyze_projection(query* q) {
+    const plan* pr = q->proj;
+    q->nexprs = 0;
+    for (int i = 0; i < pr->nexprs; ++i) {
+        expr* e = pr->exprs[i];
+        if (e->k != E_STAR) {
+            if (q->nexprs >= 64) FAIL("too many projection columns");
+            q->exprs[q->nexprs++] = e;
+            continue;
+        }
+        if (q->njoins > 0) FAIL("SELECT * over joins is not supported; list the columns");
+        for (int c = 0; c < q->fact->ncols; ++c) {
+            if (q->nexprs >= 64) FAIL("too many projection columns");
+            expr* x = &q->star_cols[q->nexprs];
+            memset(x, 0, sizeof *x);
+            x->k = E_COL;
+            snprintf(x->name, sizeof x->name, "%s", q->fact->cols[c].name);
+            q->exprs[q->nexprs++] = x;
+        }
+    }
+    if (q->nexprs == 0) FAIL("projection without columns");
+    for (int i = 0; i < q->nexprs; ++i)
+        if (resolve_expr(q, q->exprs[i], q->njoins)) return -1;
+    q->ngroups = 0;
+    q->naggs = 0;
+    q->visible_ts = q->storage_ts = q->nexprs;
+    for (int c = 0; c < q->nexprs; ++c) {
+        q->out_type[c] = expr_out_type(q->exprs[c]);
+        q->out_nullable[c] = expr_nullable(q->db, q->exprs[c]);
+    }
+    q->G = 0;
+    q->fast_sparse = 0;
+    return 0;
+}
+
 static int analyze(query* q, const plan* root) {
     const plan* n = root;
     q->sort = NULL;
     if (n->k == P_SORT) { q->sort = n; n = n->child[0]; }
-    if (n->k != P_AGG) FAIL("plan root must be an aggregate (projection is not supported)");
+    q->proj = NULL;
+    if (n->k == P_PROJECT) q->proj = n;
+    else if (n->k != P_AGG) FAIL("plan root must be an aggregate or a projection");
     q->agg = n;
     /* collect the probe chain from the aggregate down to the fact scan */
     const plan* chain[MAXOPS];
@@ -920,6 +996,7 @@ static int analyze(query* q, const plan* root) {
             }
         }
     }
+    if (q->proj) return analyze_projection(q);
     const plan* a = q->agg;
     for (int g = 0; g < a->ngroups; ++g)
         if (resolve_expr(q, a->groups[g], q->njoins)) return -1;
@@ -994,11 +1071,17 @@ typedef struct {
 } rowctx;
 
 static int32_t col_value(const rowctx* rc, const expr* e) {
+    if (e->src == 2) return ((const ocol*)e->dcol)->data[rc->payload[e->colidx]];
     if (e->src == 1) return rc->payload[e->colidx];
     return rc->q->fact->cols[e->colidx].data[rc->row];
 }
 
 static int col_valid(const rowctx* rc, const expr* e) {
+    if (e->src == 2) {  /* gathered by row id with its validity (itemColumnValidExpr, visitor.cpp:3570-3582) */
+        const ocol* c = (const ocol*)e->dcol;
+        const int64_t r = rc->payload[e->colidx];
+        return !c->validity || (int)((c->validity[r >> 6] >> (r & 63)) & 1ULL);
+    }
     if (e->src == 1) return 1; /* payload registers carry no validity */
     const uint64_t* v = rc->q->fact->cols[e->colidx].validity;
     if (!v) return 1;
@@ -1151,9 +1234,9 @@ static int build_tables(query* q) {
             int32_t x = (int32_t)((uint32_t)k - (uint32_t)ji->kmin);
             if (x < 0) continue;
             int64_t s = x % ji->len;
-            if (ji->payload) {
+            if (ji->payload || ji->rowid) {
                 if (ji->ht[2 * s] == 0) ji->ht[2 * s] = k;
-                ji->ht[2 * s + 1] = ji->payload->data[r];
+                ji->ht[2 * s + 1] = ji->rowid ? (int32_t)r : ji->payload->data[r];
             } else {
                 if (ji->ht[s] == 0) ji->ht[s] = k;
             }
@@ -1173,7 +1256,7 @@ static int probe(rowctx* rc, const joininfo* ji, int jidx, int k) {
     int32_t x = (int32_t)((uint32_t)v - (uint32_t)ji->kmin);
     if (x < 0) return 0;
     int64_t s = x % ji->len;
-    if (ji->payload) {
+    if (ji->payload || ji->rowid) {
         if (ji->ht[2 * s] != v) return 0;
         rc->payload[jidx] = ji->ht[2 * s + 1];
         return 1;
@@ -1190,6 +1273,9 @@ typedef struct {
     uint8_t* bit;      /* validity bit per slot */
     uint8_t* written;  /* group column slot written */
     uint8_t* seen;     /* scalar path: any valid input per aggregate */
+    uint64_t* pv;      /* projection: output rows (values, row-major) */
+    uint8_t* pb;       /* projection: validity per cell */
+    int64_t prows, pcap;
 } aggstate;
 
 static int state_alloc(aggstate* s, int64_t slots) {
@@ -1205,6 +1291,33 @@ static void state_free(aggstate* s) {
     free(s->bit);
     free(s->written);
     free(s->seen);
+    free(s->pv);
+    free(s->pb);
+}
+
+static void emit_row(const rowctx* rc, aggstate* s) {
+    const query* q = rc->q;
+    const int n = q->nexprs;
+    if (s->prows == s->pcap) {
+        s->pcap = s->pcap ? 2 * s->pcap : 1024;
+        s->pv = (uint64_t*)realloc(s->pv, (size_t)(s->pcap * n) * sizeof(uint64_t));
+        s->pb = (uint8_t*)realloc(s->pb, (size_t)(s->pcap * n));
+    }
+    uint64_t* v = s->pv + s->prows * n;
+    uint8_t* b = s->pb + s->prows * n;
+    for (int c = 0; c < n; ++c) {
+        const cval x = vexpr(rc, q->exprs[c]);
+        b[c] = (uint8_t)(vvalid(rc, q->exprs[c]) != 0);
+        if (q->out_type[c] == ORC_T_FLOAT64) {
+            const double d = conv(x, T_F64).f;
+            memcpy(&v[c], &d, 8);
+        } else if (q->out_type[c] == ORC_T_INT64) {
+            v[c] = (uint64_t)conv(x, T_I64).i;
+        } else {
+            v[c] = conv(x, T_U64).u;
+        }
+    }
+    ++s->prows;
 }
 
 static void init_min_slots(const query* q, aggstate* s) {
@@ -1424,6 +1537,45 @@ static int shape_result(const query* q, aggstate* s, orc_result* out) {
     for (int c = 0; c < vts; ++c) { out->types[c] = q->out_type[c]; out->nullable[c] = q->out_nullable[c]; }
     out->ordered = 1;
 
+    if (q->proj) {
+        /* rows in scan order (the device's within-tile order is not fixed:
+         * compare unordered projections as multisets), then ORDER BY / LIMIT
+         * with the row position as the final tie-break */
+        const int64_t rows = s->prows;
+        uint64_t* dval = (uint64_t*)calloc((size_t)(rows * vts + 1), sizeof(uint64_t));
+        uint8_t* dvld = (uint8_t*)calloc((size_t)(rows * vts + 1), 1);
+        for (int64_t r = 0; r < rows; ++r)
+            for (int c = 0; c < vts; ++c) {
+                dval[r * vts + c] = s->pv[r * vts + c];
+                dvld[r * vts + c] = q->out_nullable[c] ? s->pb[r * vts + c] : 1;
+            }
+        int64_t out_rows = rows;
+        int64_t* order = (int64_t*)malloc((size_t)(rows + 1) * sizeof(int64_t));
+        for (int64_t r = 0; r < rows; ++r) order[r] = r;
+        out->ordered = q->sort != NULL;
+        if (q->sort) {
+            if (q->sort->nkeys > 0 && rows > 1) {
+                g_sort.q = q;
+                g_sort.val = dval;
+                g_sort.vld = dvld;
+                qsort(order, (size_t)rows, sizeof(int64_t), row_cmp_q);
+            }
+            if (q->sort->limit >= 0 && q->sort->limit < out_rows) out_rows = q->sort->limit;
+        }
+        out->nrows = out_rows;
+        out->values = (uint64_t*)calloc((size_t)(out_rows * vts + 1), sizeof(uint64_t));
+        out->valid = (uint8_t*)calloc((size_t)(out_rows * vts + 1), 1);
+        for (int64_t r = 0; r < out_rows; ++r)
+            for (int c = 0; c < vts; ++c) {
+                out->values[r * vts + c] = dval[order[r] * vts + c];
+                out->valid[r * vts + c] = dvld[order[r] * vts + c];
+            }
+        free(order);
+        free(dval);
+        free(dvld);
+        return 0;
+    }
+
     /* scalar aggregates: validity bits as the per-block reductions set them */
     if (a->ngroups == 0) {
         for (int i = 0; i < a->naggs; ++i) {
@@ -1579,7 +1731,8 @@ static void run_ops(const query* q, rowctx* rc, int i, aggstate* st) {
         }
         return;
     }
-    aggregate_row(rc, st);
+    if (q->proj) emit_row(rc, st);
+    else aggregate_row(rc, st);
 }
 
 static int run_scan(query* q, aggstate* total, int threads) {
@@ -1614,7 +1767,22 @@ static int run_scan(query* q, aggstate* total, int threads) {
     }
     *total = parts[0];
     for (int t = 1; t < nthr; ++t) {
-        merge_state(q, total, &parts[t]);
+        if (q->proj) {
+            /* static schedule: thread t scanned the t-th contiguous row range */
+            aggstate* a = total;
+            const aggstate* b = &parts[t];
+            const int n = q->nexprs;
+            if (b->prows) {
+                a->pv = (uint64_t*)realloc(a->pv, (size_t)((a->prows + b->prows) * n) * sizeof(uint64_t));
+                a->pb = (uint8_t*)realloc(a->pb, (size_t)((a->prows + b->prows) * n));
+                memcpy(a->pv + a->prows * n, b->pv, (size_t)(b->prows * n) * sizeof(uint64_t));
+                memcpy(a->pb + a->prows * n, b->pb, (size_t)(b->prows * n));
+                a->prows += b->prows;
+                a->pcap = a->prows;
+            }
+        } else {
+            merge_state(q, total, &parts[t]);
+        }
         state_free(&parts[t]);
     }
     free(parts);

@@ -344,3 +344,44 @@ def test_or_key_join_closed_form():
     for q, sql in enumerate(OR_KEY_QUERIES):
         got = odb.execute(db.explain(sql)).tuples()
         assert sorted(got) == or_key_closed_form(tables, q), sql
+
+
+# --------------------------------------------------------------- projection
+PROJECTION_QUERIES = [
+    "select lo_revenue, lo_discount * 2, lo_discount < 3 from lineorder where lo_discount < 4",
+    "select lo_revenue, d_year from lineorder, ddate where lo_orderdate = d_datekey and lo_discount = 1",
+    "select lo_commitdate, lo_discount from lineorder where lo_discount = 2",
+    "select lo_revenue, lo_discount from lineorder where lo_discount = 3 order by lo_revenue desc limit 10",
+    "select * from lineorder where lo_discount = 9",
+]
+
+
+def projection_closed_form(tables, q):
+    lo = tables["LINEORDER"]
+    od, cd, rev, disc = (lo[c][0].astype(np.int64) for c in ("lo_orderdate", "lo_commitdate", "lo_revenue", "lo_discount"))
+    n = len(od)
+    cv = np.unpackbits(lo["lo_commitdate"][1].view(np.uint8), bitorder="little")[:n].astype(bool)
+    d = tables["DDATE"]
+    year_of = dict(zip(d["d_datekey"][0].tolist(), d["d_year"][0].tolist()))
+    rows = range(n)
+    if q == 0:
+        return [(int(rev[r]), int(disc[r]) * 2, int(disc[r] < 3)) for r in rows if disc[r] < 4]
+    if q == 1:
+        return [(int(rev[r]), year_of[int(od[r])]) for r in rows if disc[r] == 1 and int(od[r]) in year_of]
+    if q == 2:
+        return [(int(cd[r]) if cv[r] else None, 2) for r in rows if disc[r] == 2]
+    if q == 3:
+        sel = sorted((r for r in rows if disc[r] == 3), key=lambda r: (-rev[r], r))[:10]
+        return [(int(rev[r]), 3) for r in sel]
+    return [(int(od[r]), int(cd[r]) if cv[r] else None, int(rev[r]), 9) for r in rows if disc[r] == 9]
+
+
+def test_projection_closed_form():
+    tables = or_key_tables(seed=12, n=5000)
+    odb, db = or_key_dbs(tables)
+    for q, sql in enumerate(PROJECTION_QUERIES):
+        got = odb.execute(db.explain(sql))
+        assert got.tuples() == projection_closed_form(tables, q), sql
+    r = odb.execute(db.explain(PROJECTION_QUERIES[0]))
+    assert r.types == [0, 0, 1] and r.nullable == [False, False, False]
+    assert odb.execute(db.explain(PROJECTION_QUERIES[2])).nullable == [True, False]
