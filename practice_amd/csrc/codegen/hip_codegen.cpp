@@ -243,6 +243,7 @@ private:
     QueryProgram prog_;
     const AggregateNode* agg_ = nullptr;
     const SortLimitNode* sort_ = nullptr;
+    const ProjectionNode* proj_ = nullptr;
     std::vector<Op> ops_;
     std::vector<const ExprNode*> build_pred_;
     bool build_filtered_ = false;  // join being analysed has a build-side filter
@@ -289,13 +290,18 @@ private:
             sort_ = static_cast<const SortLimitNode*>(n);
             n = n->getChildren()[0].get();
         }
-        if (n->getType() == OperatorType::PROJECTION)
-            unsupported("Projection (non-aggregate SELECT) is not supported by the HIP backend yet");
-        if (n->getType() != OperatorType::AGGREGATE) unsupported("plan root must be an aggregate");
-        agg_ = static_cast<const AggregateNode*>(n);
+        if (n->getType() == OperatorType::PROJECTION) {
+            if (opt_.vm) unsupported("interpreted tier: projections run compiled");
+            if (opt_.batch_body) unsupported("projections do not run in the shared scan");
+            proj_ = static_cast<const ProjectionNode*>(n);
+        } else if (n->getType() != OperatorType::AGGREGATE) {
+            unsupported("plan root must be an aggregate or a projection");
+        } else {
+            agg_ = static_cast<const AggregateNode*>(n);
+        }
 
         std::vector<const OperatorNode*> chain, build;
-        const OperatorNode* cur = agg_->getChildren()[0].get();
+        const OperatorNode* cur = n->getChildren()[0].get();
         while (cur->getType() != OperatorType::TABLE_SCAN) {
             if (cur->getType() == OperatorType::FILTER) {
                 chain.push_back(cur);
@@ -346,7 +352,92 @@ private:
             ops_.push_back(op);
         }
         if (prog_.joins.size() > CQ_MAX_JOINS_HOST) unsupported("too many joins");
-        analyzeAggregate();
+        if (proj_) analyzeProjection();
+        else analyzeAggregate();
+    }
+
+    // consumeProjectionVector (visitor.cpp:3466-3548): the output columns,
+    // each expression's result type and nullability as for GROUP BY columns
+    std::vector<const ExprNode*> proj_exprs_;
+    std::vector<std::unique_ptr<ExprNode>> proj_star_;
+    void analyzeProjection() {
+        for (const auto& e : proj_->select_exprs) {
+            if (!e) continue;
+            if (e->getType() != ExprType::STAR) {
+                proj_exprs_.push_back(e.get());
+                continue;
+            }
+            if (!prog_.joins.empty()) unsupported("SELECT * over joins is not supported; list the columns");
+            for (const auto& c : meta(prog_.fact_table).getColumnNames()) {
+                proj_star_.push_back(std::make_unique<ColumnRefExpr>(c));
+                proj_exprs_.push_back(proj_star_.back().get());
+            }
+        }
+        if (proj_exprs_.empty()) unsupported("projection without columns");
+        if (proj_exprs_.size() > 64) unsupported("more than 64 projection columns");
+        for (const auto& jp : prog_.joins)
+            if (jp.fk_alts.size() > 1) unsupported("OR-key joins in a projection are not supported");
+        prog_.projection = true;
+        prog_.ngroups = 0;
+        prog_.naggs = 0;
+        prog_.grouped = false;
+        prog_.visible_ts = prog_.storage_ts = static_cast<int>(proj_exprs_.size());
+        prog_.groups = 0;
+        prog_.fast_sparse = false;
+        for (const ExprNode* e : proj_exprs_) {
+            OutputColumn oc;
+            oc.type = resultType(e);
+            oc.nullable = resultNullable(e);
+            prog_.out.push_back(oc);
+        }
+        for (size_t c = 0; c < prog_.out.size() && c < proj_->output_aliases.size(); ++c) prog_.out[c].name = proj_->output_aliases[c];
+        prog_.has_sort_node = sort_ != nullptr;
+        if (sort_) {
+            prog_.sort_keys = sort_->sort_keys;
+            prog_.has_limit = sort_->has_limit;
+            prog_.limit = sort_->limit;
+            for (const auto& k : prog_.sort_keys)
+                if (k.column_index >= static_cast<size_t>(prog_.visible_ts))
+                    unsupported("ORDER BY key index exceeds generated result width");
+        }
+    }
+    // resultLogicalTypeForExpr (visitor.cpp:77-107)
+    static LogicalType resultType(const ExprNode* e) {
+        switch (e->getType()) {
+            case ExprType::OP_EQ: case ExprType::OP_NEQ: case ExprType::OP_LT: case ExprType::OP_LTE:
+            case ExprType::OP_GT: case ExprType::OP_GTE: case ExprType::OP_AND: case ExprType::OP_OR:
+            case ExprType::OP_NOT: case ExprType::OP_IS_NULL: case ExprType::OP_IS_NOT_NULL:
+            case ExprType::LITERAL_NULL:
+                return LogicalType::UInt64;
+            case ExprType::LITERAL_FLOAT: return LogicalType::Float64;
+            case ExprType::CASE_WHEN: {
+                const auto* c = static_cast<const CaseWhenExpr*>(e);
+                const auto x = resultType(c->then_expr.get()), y = resultType(c->else_expr.get());
+                if (x == LogicalType::Float64 || y == LogicalType::Float64) return LogicalType::Float64;
+                if (x == LogicalType::Int64 || y == LogicalType::Int64) return LogicalType::Int64;
+                return LogicalType::UInt64;
+            }
+            default: return LogicalType::Int64;
+        }
+    }
+    bool resultNullable(const ExprNode* e) const {
+        if (!e) return true;
+        switch (e->getType()) {
+            case ExprType::LITERAL_NULL: return true;
+            case ExprType::STAR: case ExprType::LITERAL_INT: case ExprType::LITERAL_FLOAT:
+            case ExprType::OP_IS_NULL: case ExprType::OP_IS_NOT_NULL:
+                return false;
+            case ExprType::COLUMN_REF: return nullable(static_cast<const ColumnRefExpr*>(e)->column_name);
+            case ExprType::OP_NOT: return resultNullable(static_cast<const BinaryExpr*>(e)->left.get());
+            case ExprType::CASE_WHEN: {
+                const auto* c = static_cast<const CaseWhenExpr*>(e);
+                return resultNullable(c->then_expr.get()) || resultNullable(c->else_expr.get());
+            }
+            default: {
+                const auto* b = static_cast<const BinaryExpr*>(e);
+                return resultNullable(b->left.get()) || resultNullable(b->right.get());
+            }
+        }
     }
     static constexpr size_t CQ_MAX_JOINS_HOST = 8;
 
@@ -418,6 +509,23 @@ private:
         }
         if (!uniqueKey(t, jp.pk_col))
             unsupported("non-unique build key " + jp.pk_col + " needs a multi-value hash table (not supported yet)");
+        // a projection above that reads this dimension gathers its columns
+        // by row id (projectionRequiresRowIdPayload, visitor.cpp:3300-3317)
+        for (const OperatorNode* p = join->parent_; p; p = p->parent_) {
+            if (p->getType() != OperatorType::PROJECTION) continue;
+            for (const auto& e : static_cast<const ProjectionNode*>(p)->select_exprs) {
+                if (!e) continue;
+                if (e->getType() == ExprType::STAR) jp.rowid = true;
+                std::vector<std::string> cols;
+                collectColumns(e.get(), cols);
+                for (const auto& c : cols) jp.rowid = jp.rowid || tableOf(c) == t;
+            }
+        }
+        if (jp.rowid) {
+            jp.pair = true;
+            chooseLayout(jp);
+            return;
+        }
         // payload: aggregate columns first, then filter columns above the join
         std::vector<std::string> aggc, fltc;
         for (const OperatorNode* p = join->parent_; p; p = p->parent_) {
@@ -453,7 +561,9 @@ private:
                 int code_bits = 0;
                 jp.pmin = 0;
                 bool ok = true;
-                if (jp.pair) {
+                if (jp.rowid) {
+                    code_bits = bitsFor(static_cast<uint64_t>(std::max<int64_t>(jp.dim_rows - 1, 0)));
+                } else if (jp.pair) {
                     const auto* ps = stats(jp.payload_col);
                     if (!ps) ok = false;
                     else {
@@ -483,7 +593,9 @@ private:
                                      ks2->cardinality_ == tm.getSize();
             jp.doff = all_present ? 0 : 1;
             uint64_t range = 0;
-            if (jp.pair) {
+            if (jp.rowid) {
+                range = static_cast<uint64_t>(std::max<int64_t>(jp.dim_rows - 1, 0));
+            } else if (jp.pair) {
                 const auto* ps = stats(jp.payload_col);
                 range = static_cast<uint64_t>(ps->max_value_ - ps->min_value_);
             }
@@ -550,7 +662,7 @@ private:
         }
         if (jp.compact) jp.ht_words = (static_cast<uint64_t>(jp.len) * static_cast<uint64_t>(jp.eb) + 31) / 32;
     }
-    int64_t densePbase(const JoinProgram& jp) const { return (jp.pair ? jp.pmin : 0) - jp.doff; }
+    int64_t densePbase(const JoinProgram& jp) const { return (jp.pair && !jp.rowid ? jp.pmin : 0) - jp.doff; }
     // LDS bytes the table needs when placed there (0: not placeable)
     uint64_t ldsBytes(const JoinProgram& jp) const {
         if (jp.dense_ok) return denseWords(jp, jp.dbits) * 4;
@@ -748,8 +860,15 @@ private:
             if (((visible_mask_ >> j) & 1u) && prog_.joins[static_cast<size_t>(j)].payload_col == col) return j;
         return -1;
     }
+    int rowidJoin(const std::string& col) const {
+        for (size_t j = 0; j < prog_.joins.size(); ++j)
+            if (prog_.joins[j].rowid && prog_.joins[j].dim_table == tableOf(col)) return static_cast<int>(j);
+        return -1;
+    }
     std::string colValue(const std::string& col) {
         if (tableOf(col) == prog_.fact_table) return "c" + std::to_string(factSlot(col)) + "[i]";
+        if (const int rj = rowidJoin(col); rj >= 0)
+            return "P.dcol[" + std::to_string(dimSlot(col)) + "][p" + std::to_string(rj) + "[i]]";
         const int j = payloadJoin(col);
         if (j >= 0) return "p" + std::to_string(j) + "[i]";
         unsupported("column '" + col + "' of dimension " + tableOf(col) +
@@ -760,6 +879,8 @@ private:
             const int s = factSlot(col);
             return prog_.fact_nullable[static_cast<size_t>(s)] ? "((v" + std::to_string(s) + " >> i) & 1u)" : "1";
         }
+        if (const int rj = rowidJoin(col); rj >= 0)  // gathered by row id with its validity
+            return nullable(col) ? "chip::bit32(P.dvalid[" + std::to_string(dimSlot(col)) + "], p" + std::to_string(rj) + "[i])" : "1";
         if (payloadJoin(col) >= 0) return "1";  // payload registers carry no validity
         colValue(col);                           // raises the diagnostic
         return "1";
@@ -1014,7 +1135,7 @@ private:
         const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
         std::string cond = bp ? buildCond(bp) : "";
         jp.pk_dslot = dimSlot(jp.pk_col);
-        if (jp.pair) jp.payload_dslot = dimSlot(jp.payload_col);
+        if (jp.pair && !jp.rowid) jp.payload_dslot = dimSlot(jp.payload_col);
         src << "extern \"C\" __global__ __launch_bounds__(256) void cq_build_" << j << "(" << paramDecl() << ") {\n"
             << paramAlias()
             << "    const long long n = P.dim_rows[" << j << "];\n"
@@ -1023,7 +1144,8 @@ private:
         if (nullable(jp.pk_col))
             src << "        if (!chip::bit32(P.dvalid[" << jp.pk_dslot << "], r)) continue;\n";
         src << "        const int k = P.dcol[" << jp.pk_dslot << "][r];\n";
-        src << "        const int pay = " << (jp.pair ? "P.dcol[" + std::to_string(jp.payload_dslot) + "][r]" : "0") << ";\n";
+        src << "        const int pay = "
+            << (jp.rowid ? std::string("(int)r") : jp.pair ? "P.dcol[" + std::to_string(jp.payload_dslot) + "][r]" : "0") << ";\n";
         if (jp.dense)
             src << "        chip::ht_insert_dense(P.ht[" << j << "], k, " << jp.dmin << ", " << jp.dspan << ", " << jp.db << ", "
                 << densePbase(jp) << ", pay, " << jp.cal << ");\n";
@@ -1160,6 +1282,13 @@ private:
             emitProbe(orbody, multi, "kor[i]", "");
             for (const Op& op : dependent) emitFilterOp(orbody, op);
         }
+        if (prog_.projection) {
+            emitProjection(src, body.str(), bt, vpt, ipt);
+            prog_.source = src.str();
+            setAlgorithmicBytes();
+            return;
+        }
+
         // ---- aggregation
         std::ostringstream aggcode, epilogue, prologue;
         const auto* a = agg_;
@@ -1247,6 +1376,83 @@ private:
         prog_.source = src.str();
         (void)a;
         setAlgorithmicBytes();
+    }
+
+    // Projection: cq_pcount stores the live rows of each (tile, vector, wave);
+    // after an exclusive scan of those counts cq_pwrite writes every live row
+    // at its offset, so the output is in scan (row) order.  No block barrier:
+    // a wave that finds no live row in a tile leaves the tile early.
+    void emitProjection(std::ostringstream& src, const std::string& body, int bt, int vpt, int ipt) {
+        const int waves = bt / 64;
+        prog_.proj_waves = waves;
+        prog_.block_threads = bt;
+        prog_.vectors_per_thread = vpt;
+        (void)ipt;
+        const int ts = prog_.visible_ts;
+        // output expressions first: they decide the fact columns each tile loads
+        std::vector<std::string> vals, oks;
+        for (int c = 0; c < ts; ++c) {
+            vals.push_back(vexpr(proj_exprs_[static_cast<size_t>(c)]));
+            oks.push_back(vvalid(proj_exprs_[static_cast<size_t>(c)]));
+        }
+        src << "#define BT " << bt << "\n#define VPT " << vpt << "\n#define IPT " << 4 * vpt << "\n#define TILE (BT * IPT)\n#define WAVES "
+            << waves << "\n\n";
+        auto head = [&](const char* name) {
+            src << "extern \"C\" __global__ __launch_bounds__(BT) void " << name << "(const cq_query_params P) {\n"
+                << "    using namespace chip;\n"
+                << "    const int tid = threadIdx.x;\n"
+                << "    const int wave = tid >> 6, lane = tid & 63;\n";
+            for (const auto& jp : prog_.joins) {
+                if (!jp.lds) continue;
+                const int j = static_cast<int>(&jp - prog_.joins.data());
+                src << "    __shared__ u32 s_ht" << j << "[" << jp.ht_words << "];\n"
+                    << "    for (int k = tid; k < " << jp.ht_words << "; k += BT) s_ht" << j << "[k] = P.ht[" << j << "][k];\n";
+            }
+            src << "    __syncthreads();\n"
+                << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n"
+                << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
+            for (size_t c = 0; c < prog_.fact_cols.size(); ++c) {
+                src << "        int c" << c << "[IPT];\n";
+                if (prog_.fact_nullable[c]) src << "        u32 v" << c << " = 0u;\n";
+            }
+            src << "        u32 fl = 0u;\n";
+            emitTileLoad(src, "        ", "", "base", "fl");
+            src << body;
+        };
+        head("cq_pcount");
+        src << "#pragma unroll\n        for (int v = 0; v < VPT; ++v) {\n"
+            << "            const u64 n = wave_sum((u64)__popc((fl >> (4 * v)) & 0xfu));\n"
+            << "            if (lane == 0) P.tile_count[(tile * VPT + v) * WAVES + wave] = n;\n"
+            << "        }\n    }\n    (void)lane;\n}\n\n";
+        head("cq_pwrite");
+        src << "#pragma unroll\n        for (int v = 0; v < VPT; ++v) {\n"
+            << "            const u32 m4 = (fl >> (4 * v)) & 0xfu;\n"
+            << "            const u32 cnt = __popc(m4);\n"
+            << "            const u64 below = (1ull << lane) - 1ull;\n"
+            << "            const u32 pre = __popcll(__ballot(cnt & 1u) & below) + 2u * __popcll(__ballot((cnt >> 1) & 1u) & below) +\n"
+            << "                            4u * __popcll(__ballot((cnt >> 2) & 1u) & below);\n"
+            << "            const u64 first = P.tile_off[(tile * VPT + v) * WAVES + wave] + pre;\n"
+            << "#pragma unroll\n            for (int k = 0; k < 4; ++k) {\n"
+            << "                const int i = 4 * v + k;\n"
+            << "                if (!((m4 >> k) & 1u)) continue;\n"
+            << "                const u64 row = first + (u64)__popc(m4 & ((1u << k) - 1u));\n";
+        for (int c = 0; c < ts; ++c) {
+            const std::string& v = vals[static_cast<size_t>(c)];
+            std::string cast;
+            switch (prog_.out[static_cast<size_t>(c)].type) {
+                case LogicalType::Float64: cast = "(u64)__double_as_longlong((double)(" + v + "))"; break;
+                case LogicalType::Int64: cast = "(u64)(long long)(" + v + ")"; break;
+                default: cast = "(u64)(" + v + ")"; break;
+            }
+            src << "                P.result[row * " << ts << " + " << c << "] = " << cast << ";\n";
+            const std::string& ok = oks[static_cast<size_t>(c)];
+            if (litFalse(ok)) continue;
+            const std::string set = "{ const u64 b = row * " + std::to_string(ts) + " + " + std::to_string(c) +
+                                    "; atomicOr(&P.result_valid[b >> 6], 1ull << (b & 63)); }";
+            if (litTrue(ok)) src << "                " << set << "\n";
+            else src << "                if (" << ok << ") " << set << "\n";
+        }
+        src << "            }\n        }\n    }\n}\n\n";
     }
 
     void emitFilterOp(std::ostringstream& body, const Op& op) {
@@ -2954,8 +3160,9 @@ void HipOperatorVisitor::visit(const AggregateNode& node) {
     program_ = Gen(catalog_, options_).run(&node);
     done_ = true;
 }
-void HipOperatorVisitor::visit(const ProjectionNode&) {
-    throw std::runtime_error("Projection (non-aggregate SELECT) is not supported by the HIP backend yet");
+void HipOperatorVisitor::visit(const ProjectionNode& node) {
+    program_ = Gen(catalog_, options_).run(&node);
+    done_ = true;
 }
 void HipOperatorVisitor::visit(const SortLimitNode& node) {
     program_ = Gen(catalog_, options_).run(&node);

@@ -56,6 +56,9 @@ struct JoinProgram {
     // OR-key join (lo_orderdate = d_datekey OR lo_commitdate = d_datekey):
     // every fact key alternative in condition order (fk_col is the first)
     std::vector<std::string> fk_alts;
+    // projection of this dimension's columns: the payload is the build row's
+    // index and the columns are gathered by it (with their validity)
+    bool rowid = false;
     int64_t dim_rows = 0;
     int pk_dslot = -1, payload_dslot = -1;
     bool pair = false;        // PHT_2 (carries a payload) vs PHT_1
@@ -95,6 +98,11 @@ struct OutputColumn {
 };
 
 struct QueryProgram {
+    // projection (non-aggregate SELECT): cq_pcount counts each wave's rows per
+    // tile, cq_pwrite writes them at the scanned offsets (row order); the
+    // output columns are the visible slots of `out`
+    bool projection = false;
+    int proj_waves = 0;
     std::string source;
     std::string signature;    // hex hash of `source`
     std::string plan_sexpr;

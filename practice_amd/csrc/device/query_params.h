@@ -31,6 +31,8 @@ struct cq_query_params {
     // with one validity bit per slot (reference d_result / d_result_validity)
     unsigned long long* result;
     unsigned long long* result_valid;
+    unsigned long long* tile_count;   // projection: rows per (tile, vector, wave)
+    unsigned long long* tile_off;     // projection: exclusive scan of tile_count
     // predicate constants (runtime parameters, so one compiled module serves
     // every query of the same plan signature)
     long long lit[CQ_MAX_LITS];

@@ -147,6 +147,13 @@ void PlanSignatureCache::clear() {
 size_t estimateQueryTemporaryBytes(const QueryProgram& p) {
     size_t bytes = 0;
     for (const auto& j : p.joins) bytes += j.ht_words * 4;
+    if (p.projection) {
+        // per-(tile, vector, wave) counts and offsets; the rows are checked after the count
+        const int64_t tile = static_cast<int64_t>(std::max(p.block_threads, 64)) * 4 * std::max(p.vectors_per_thread, 1);
+        const size_t n = static_cast<size_t>((p.fact_rows + tile - 1) / tile) * static_cast<size_t>(std::max(p.vectors_per_thread, 1)) *
+                         static_cast<size_t>(std::max(p.proj_waves, 1));
+        return bytes + 2 * n * 8 + 64;
+    }
     const size_t slots = static_cast<size_t>(p.groups) * static_cast<size_t>(p.storage_ts);
     bytes += slots * 8 + ((slots + 63) / 64) * 8;                              // aggregate state
     const size_t vslots = static_cast<size_t>(p.groups) * static_cast<size_t>(p.visible_ts);
@@ -263,7 +270,7 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
         P.dim_rows[j] = p.joins[j].dim_rows;
         if (!p.vm) run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
     }
-    run.dense_count = (p.fast_sparse || p.vm) ? nullptr : module->function("cq_dense_count");
+    run.dense_count = (p.fast_sparse || p.vm || p.projection) ? nullptr : module->function("cq_dense_count");
     run.keys = p.keys_kernel ? module->function("cq_keys") : nullptr;
     run.vm = p.vm ? static_cast<cq_vm_program*>(take("vm", sizeof(cq_vm_program))) : nullptr;
 
@@ -429,6 +436,81 @@ void HipModuleExecutor::shape(QueryRun& run, ExecutionContext* ctx, const cq_que
     hipCheck(cq_launch_write_rows(run.res, run.rv, G, vts, run.fl, run.off, nmask, p.fast_sparse ? 0 : 1, run.o, run.ov,
                                   run.cap_rows, run.vwords, s),
              "write rows");
+}
+
+unsigned long long HipModuleExecutor::project(QueryRun& run, ExecutionContext* ctx, hipEvent_t kernel_start, hipEvent_t kernel_end) {
+    const QueryProgram& p = *run.program;
+    cq_query_params& P = run.P;
+    hipStream_t s = ctx->stream;
+    auto take64 = [&](const std::string& name, size_t bytes) {
+        DeviceBuffer& b = ctx->scratch(name);
+        b.ensure(std::max<size_t>(bytes, 8));
+        return static_cast<unsigned long long*>(b.data());
+    };
+    // one count per (tile, vector of 4 rows per lane, wave) in row order
+    const long long n = std::max<long long>(P.num_tiles * p.vectors_per_thread * p.proj_waves, 1);
+    P.tile_count = take64("proj_counts", static_cast<size_t>(n) * 8);
+    P.tile_off = take64("proj_offsets", static_cast<size_t>(n) * 8 + 8);
+    unsigned long long* bs = take64("proj_block_sums", static_cast<size_t>(cq_scan_block_count(n) + 1) * 8);
+    hipCheck(cq_launch_fill_u64(P.tile_count, 0ull, n, s), "fill projection counts");
+    hipFunction_t fc = run.module->function("cq_pcount");
+    hipFunction_t fw = run.module->function("cq_pwrite");
+    const unsigned grid = scanGrid(fc, p.block_threads, ctx->device, P.num_tiles);
+    void* args[] = {&P};
+    if (kernel_start) hipCheck(hipEventRecord(kernel_start, s), "hipEventRecord");
+    if (P.num_tiles > 0) launch(fc, grid, static_cast<unsigned>(p.block_threads), s, args, "launch projection count");
+    hipCheck(cq_launch_exclusive_scan(P.tile_count, P.tile_off, n, bs, P.tile_off + n, s), "scan projection counts");
+    unsigned long long rows = 0;
+    hipCheck(hipMemcpyAsync(&rows, P.tile_off + n, 8, hipMemcpyDeviceToHost, s), "copy projection rows");
+    hipCheck(hipStreamSynchronize(s), "sync");
+    // the exact result must fit beside the resident data (the reference's
+    // check after its count pass)
+    const int vts = p.visible_ts;
+    const size_t words = static_cast<size_t>((rows + 63) / 64);
+    const size_t need = static_cast<size_t>(rows) * static_cast<size_t>(vts) * 8 * 3 + words * 8 * static_cast<size_t>(vts) * 3 +
+                        static_cast<size_t>(rows) * 8 * 3;
+    if (ctx->config.memory_guard_enabled) {
+        size_t free_b = 0, total = 0;
+        if (hipMemGetInfo(&free_b, &total) == hipSuccess && need > free_b)
+            throw std::runtime_error("Insufficient GPU memory after projection count: exact materialized result does not fit device memory");
+    }
+    // result rows in the aggregate-state layout, then the shared shaping path
+    run.G = static_cast<int64_t>(rows);
+    run.slots = static_cast<size_t>(rows) * static_cast<size_t>(p.storage_ts);
+    P.result = take64("result", run.slots * 8);
+    P.result_valid = take64("result_valid", ((run.slots + 63) / 64) * 8);
+    run.res = P.result;
+    run.rv = P.result_valid;
+    run.single_cell = false;
+    run.cap_rows = std::max<long long>(static_cast<long long>(rows), 1);
+    run.vwords = (run.cap_rows + 63) / 64;
+    run.o = take64("out", static_cast<size_t>(vts) * run.cap_rows * 8);
+    run.ov = take64("out_valid", static_cast<size_t>(vts) * run.vwords * 8);
+    run.fl = take64("flags", run.cap_rows * 8);
+    run.off = take64("offsets", run.cap_rows * 8 + 8);
+    run.bsums = take64("block_sums", static_cast<size_t>(cq_scan_block_count(run.cap_rows) + 1) * 8);
+    if (p.has_sort_node && !p.sort_keys.empty()) {
+        run.sidx = take64("sort_idx", static_cast<size_t>(cq_sort_padded(run.cap_rows)) * 8);
+        run.o2 = take64("out2", static_cast<size_t>(vts) * run.cap_rows * 8);
+        run.ov2 = take64("out2_valid", static_cast<size_t>(vts) * run.vwords * 8);
+    }
+    hipCheck(cq_launch_fill_u64(P.result_valid, 0ull, static_cast<long long>((run.slots + 63) / 64), s), "fill result validity");
+    if (P.num_tiles > 0 && rows > 0) launch(fw, grid, static_cast<unsigned>(p.block_threads), s, args, "launch projection write");
+    if (kernel_end) hipCheck(hipEventRecord(kernel_end, s), "hipEventRecord");
+    // every row is present: offsets are the row indices
+    hipCheck(cq_launch_fill_u64(run.ov, 0ull, static_cast<long long>(vts) * run.vwords, s), "fill out validity");
+    if (rows > 0) {
+        hipCheck(cq_launch_fill_u64(run.fl, 1ull, static_cast<long long>(rows), s), "fill projection flags");
+        hipCheck(cq_launch_exclusive_scan(run.fl, run.off, static_cast<long long>(rows), run.bsums, run.off + run.cap_rows, s), "scan");
+    }
+    unsigned int nmask = 0;
+    for (int c = 0; c < vts; ++c)
+        if (p.out[static_cast<size_t>(c)].nullable) nmask |= 1u << c;
+    if (rows > 0)
+        hipCheck(cq_launch_write_rows(run.res, run.rv, static_cast<long long>(rows), vts, run.fl, run.off, nmask, 1, run.o, run.ov,
+                                      run.cap_rows, run.vwords, s),
+                 "write rows");
+    return rows;
 }
 
 size_t HipModuleExecutor::fetchBytes(const QueryRun& run, unsigned long long rows) const {
@@ -619,12 +701,16 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
     bind(module, ctx->program, ctx, "", run, nullptr);
     hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
     setup(run, ctx, nullptr);
-    if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");
-    scan(run, ctx);
-    if (ke) hipCheck(hipEventRecord(ke, s), "hipEventRecord");
-    shape(run, ctx, nullptr, nullptr);
     unsigned long long rows = 0;
-    if (!run.single_cell) {
+    if (ctx->program.projection) {
+        rows = project(run, ctx, ks, ke);
+    } else {
+        if (ks) hipCheck(hipEventRecord(ks, s), "hipEventRecord");
+        scan(run, ctx);
+        if (ke) hipCheck(hipEventRecord(ke, s), "hipEventRecord");
+        shape(run, ctx, nullptr, nullptr);
+    }
+    if (!run.single_cell && !ctx->program.projection) {
         hipCheck(hipMemcpyAsync(&rows, run.off + run.cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
         hipCheck(hipStreamSynchronize(s), "sync");
     }

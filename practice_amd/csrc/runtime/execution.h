@@ -275,6 +275,9 @@ public:
     // AVG finalization, present/HAVING flags, compaction; the number of
     // result rows lands in device memory at `row_count` (default: after the offsets)
     void shape(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params, unsigned long long* row_count);
+    // projection: count pass, scan of the counts, result buffers for the
+    // rows found, write pass, columnar rows (returns the row count)
+    unsigned long long project(QueryRun& run, ExecutionContext* ctx, hipEvent_t kernel_start, hipEvent_t kernel_end);
     // sort / limit and copy the result to the host (synchronizes)
     void fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows);
     // the same in two steps, so many runs share one synchronization:

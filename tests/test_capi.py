@@ -71,8 +71,10 @@ def test_unknown_column_and_unsupported_plan_codes():
     db.create_table("T", 4, True)
     db.load_column("T", "t_x", [1, 2, 3, 4])
     db.finalize()
+    # projections generate (count and write kernels); an unsupported plan shape is code -4
+    assert "cq_pwrite" in db.generate_code("select t_x from t")
     with pytest.raises(pa.CrystalError) as e:
-        db.generate_code("select t_x from t")
+        db.generate_code("select sum(t_x * 1.5) from t")
     assert e.value.code == -4
     with pytest.raises(pa.CrystalError):
         db.generate_code("select sum(t_y) from t")

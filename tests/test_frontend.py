@@ -104,7 +104,8 @@ def test_order_limit_having_translation():
     ("select from lineorder", "SQL parse error"),
     ("select sum(lo_revenue) from lineorder where 5 < lo_quantity", "Unsupported comparison shape"),
     ("select sum(lo_revenue) from lineorder where lo_quantity < 2.5", "float literal"),
-    ("select lo_quantity from lineorder", "Projection"),
+    ("select lo_quantity from lineorder, ddate where lo_orderdate = d_datekey or lo_commitdate = d_datekey",
+     "OR-key joins in a projection"),
 ])
 def test_rejections_raise(sql, msg):
     db = reference_test_catalog()
@@ -158,6 +159,21 @@ def test_or_key_queries_compile_and_stay_off_the_interpreter(small_db):
     with pytest.raises(pa.CrystalError, match="two dimension keys"):
         small_db.describe("select count(*) from lineorder, ddate where lo_orderdate = d_datekey "
                           "or lo_commitdate = d_yearmonthnum")
+
+
+def test_projection_queries_compile(small_db):
+    from tests.test_gpu_parity import PROJECTIONS_SSB
+    for sql in PROJECTIONS_SSB:
+        small_db.compile(sql)
+        src = small_db.generate_code(sql)
+        assert "cq_pcount" in src and "cq_pwrite" in src
+    d = small_db.describe(PROJECTIONS_SSB[1])
+    assert all(j["layout"] in ("key-indexed", "compact") for j in d["joins"])
+    assert [o["nullable"] for o in d["outputs"]] == [1, 0, 1, 0]
+    with pytest.raises(pa.CrystalError, match="SELECT \\* over joins"):
+        small_db.describe("select * from lineorder, ddate where lo_orderdate = d_datekey")
+    with pytest.raises(pa.CrystalError, match="shared scan"):
+        small_db.generate_batch_code([PROJECTIONS_SSB[0]])
 
 
 def test_constants_are_kernel_arguments(small_db):

@@ -83,9 +83,10 @@ def test_unsupported_request_fails_alone():
     spec, data, odb, gdb = dataset("sf1")
     srv = pa.Server(gdb, {"lanes": 1})
     try:
-        bad = srv.submit("select lo_revenue from lineorder")
+        bad = srv.submit("select lo_revenue from lineorder, ddate where lo_orderdate = d_datekey "
+                         "or lo_commitdate = d_datekey")
         good = srv.submit(Q.C1)
-        with pytest.raises(pa.CrystalError, match="Projection"):
+        with pytest.raises(pa.CrystalError, match="OR-key joins in a projection"):
             srv.wait(bad, 60_000)
         assert_results_equal(srv.wait(good, 60_000), gdb.execute(Q.C1), Q.C1)
     finally:
@@ -111,3 +112,16 @@ def test_or_key_requests_run_on_cold_lanes():
         print(f"[server] sql={q}", flush=True)
         assert_results_equal(r, gdb.execute(q), q)
         assert_same(r, odb.execute(gdb.explain(q)))
+
+
+@pytest.mark.gpu
+def test_projection_requests_run_on_cold_lanes():
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    q = "select lo_revenue, lo_discount from lineorder where lo_quantity = 7 and lo_discount > 8"
+    srv = pa.Server(gdb, {"lanes": 1})
+    try:
+        r = srv.wait(srv.submit(q), 120_000)
+    finally:
+        srv.stop()
+    assert_same(r, odb.execute(gdb.explain(q)))
