@@ -368,6 +368,18 @@ def main():
         dd = db.describe(Q.ORKEY)
         extra["orkey_sf100"] = {"ms_per_step": round(ms2, 4), "scan_kernel_ms": round(sum(sm) / len(sm), 4),
                                 "scan_GBps": round(dd["algorithmic_bytes"] / (sum(sm) / len(sm)) / 1e6, 2)}
+        # projection over the same fact table (count pass + write pass)
+        for _ in range(args.warmup):
+            db.execute(Q.PROJ)
+        t1 = time.perf_counter()
+        sm = []
+        for _ in range(args.steps):
+            r = db.execute(Q.PROJ)
+            sm.append(r.timing["scan_kernel_ms"])
+        ms2 = (time.perf_counter() - t1) * 1000 / args.steps
+        dd = db.describe(Q.PROJ)
+        extra["projection_sf100"] = {"ms_per_step": round(ms2, 4), "kernels_ms": round(sum(sm) / len(sm), 4), "rows": r.rows,
+                                     "scan_GBps": round(dd["algorithmic_bytes"] / (sum(sm) / len(sm)) / 1e6, 2)}
         line["other_configs"] = extra
     db.close()
     if not args.no_batched:

@@ -82,3 +82,6 @@ def adhoc_query(rng) -> str:
 # that names a DDATE row
 ORKEY = ("select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
          "or lo_commitdate = d_datekey group by d_year")
+
+# projection: about 0.4 % of the fact rows, two columns each
+PROJ = "select lo_revenue, lo_discount from lineorder where lo_quantity = 7 and lo_discount > 8"
