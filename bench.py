@@ -229,6 +229,8 @@ def bench_served(pa, Q, args):
                       "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2),
                       "mean_batch": round(st["mean_batch"], 1), "interpreted": st["interpreted_queries"],
                       "promotions": st["promotions"], "errors": errors})
+        if errors:
+            rates[-1]["last_error"] = st.get("last_error", "")[:200]
         print(f"[served] rate {rate:g}: {rates[-1]}", file=sys.stderr, flush=True)
     # closed loop: every request queued at once
     srv.reset_stats()
@@ -240,6 +242,8 @@ def bench_served(pa, Q, args):
     st = srv.stats()
     burst = {"requests": len(reqs), "qps": round(st["completed"] / wall, 1), "p50_ms": round(st["p50_ms"], 2),
              "p99_ms": round(st["p99_ms"], 2), "mean_batch": round(st["mean_batch"], 1), "errors": errors}
+    if errors:
+        burst["last_error"] = st.get("last_error", "")[:200]
     print(f"[served] burst: {burst}", file=sys.stderr, flush=True)
     srv.stop()
     db.close()

@@ -7,6 +7,18 @@
 
 namespace db {
 
+namespace {
+std::string jsonEscape(const std::string& t) {
+    std::string o;
+    for (char c : t.substr(0, 300)) {
+        if (c == '"' || c == '\\') o += '\\', o += c;
+        else if (static_cast<unsigned char>(c) < 0x20) o += ' ';
+        else o += c;
+    }
+    return o;
+}
+}  // namespace
+
 ServerOptions parseServerOptions(const std::string& text) {
     ServerOptions o;
     std::string batch_opts;
@@ -33,7 +45,10 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb") batch_opts += (batch_opts.empty() ? "" : ",") + item;
         else throw std::runtime_error("unknown server option: " + k);
     }
-    o.batch = parseBatchOptions(batch_opts);
+    // served batches vary in composition and buffer placement, so captured
+    // graphs are rarely replayed: eager launches unless asked for
+    o.batch = parseBatchOptions(batch_opts.find("graph=") == std::string::npos ? (batch_opts.empty() ? "graph=0" : batch_opts + ",graph=0")
+                                                                                 : batch_opts);
     if (o.lanes < 1 || o.lanes > 8) throw std::runtime_error("server lanes must be in [1, 8]");
     if (o.max_batch < 1) throw std::runtime_error("server max_batch must be positive");
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
@@ -304,6 +319,13 @@ void QueryServer::laneLoop(Lane* lane) {
                 }
             } catch (const std::exception& e) {
                 err = e.what();
+                // start the lane over with a fresh executor (arena, graphs, events)
+                try {
+                    lane->exec = std::make_unique<BatchExecutor>(db_, lane->ctx.get(), &lane->mu);
+                } catch (const std::exception&) {
+                }
+                std::lock_guard<std::mutex> lock(mu_);
+                last_error_ = err;
             }
             for (size_t i = 0; i < batch.size(); ++i) {
                 released += batch[i].footprint;
@@ -358,6 +380,8 @@ void QueryServer::coldLoop(Lane* lane) {
         } catch (const std::exception& e) {
             ctx->config.codegen.vm = false;
             finish(r, -3, e.what(), nullptr);
+            std::lock_guard<std::mutex> lock(mu_);
+            last_error_ = e.what();
         }
         {
             std::lock_guard<std::mutex> lock(mu_);
@@ -526,6 +550,7 @@ std::string QueryServer::statsJson() {
       << ",\"plain_queries\":" << plain_queries_ << ",\"interpreted_queries\":" << interpreted_queries_
       << ",\"admission_waits\":" << admission_waits_
       << ",\"compiles\":" << compiles_ << ",\"promotions\":" << promotions_ << ",\"rejected\":" << rejected_
+      << ",\"last_error\":\"" << jsonEscape(last_error_) << "\""
       << ",\"lanes\":" << opt_.lanes << ",\"budget_bytes\":" << opt_.budget_bytes << ",\"in_flight_bytes\":" << in_flight_bytes_
       << ",\"scan_set\":" << scan_set_.size() << "}";
     return o.str();

@@ -127,6 +127,7 @@ private:
     // statistics
     std::vector<double> latencies_;
     std::chrono::steady_clock::time_point stats_start_;
+    std::string last_error_;            // message of the last failed batch or cold request
     long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, interpreted_queries_ = 0, admission_waits_ = 0,
               compiles_ = 0, promotions_ = 0, rejected_ = 0;
 
