@@ -891,15 +891,31 @@ static int analyze_projection(query* q) {
             q->exprs[q->nexprs++] = e;
             continue;
         }
-        if (q->njoins > 0) FAIL("SELECT * over joins is not supported; list the columns");
-        for (int c = 0; c < q->fact->ncols; ++c) {
-            if (q->nexprs >= 64) FAIL("too many projection columns");
-            expr* x = &q->star_cols[q->nexprs];
-            memset(x, 0, sizeof *x);
-            x->k = E_COL;
-            snprintf(x->name, sizeof x->name, "%s", q->fact->cols[c].name);
-            q->exprs[q->nexprs++] = x;
+        /* every column of every scanned table, tables in plan order (depth
+         * first, children in order; collectTableScans, visitor.cpp:3426-3432) */
+        const plan* stack[64];
+        const otable* tabs[16];
+        int ns = 0, nt = 0;
+        stack[ns++] = pr->child[0];
+        while (ns > 0) {
+            const plan* c = stack[--ns];
+            if (c->k == P_SCAN) {
+                const otable* t = find_table(q->db, c->table);
+                if (!t) FAIL("unknown table %s", c->table);
+                if (nt < 16) tabs[nt++] = t;
+                continue;
+            }
+            for (int k = c->nchild - 1; k >= 0 && ns < 64; --k) stack[ns++] = c->child[k];
         }
+        for (int k = 0; k < nt; ++k)
+            for (int c = 0; c < tabs[k]->ncols; ++c) {
+                if (q->nexprs >= 64) FAIL("too many projection columns");
+                expr* x = &q->star_cols[q->nexprs];
+                memset(x, 0, sizeof *x);
+                x->k = E_COL;
+                snprintf(x->name, sizeof x->name, "%s", tabs[k]->cols[c].name);
+                q->exprs[q->nexprs++] = x;
+            }
     }
     if (q->nexprs == 0) FAIL("projection without columns");
     for (int i = 0; i < q->nexprs; ++i)

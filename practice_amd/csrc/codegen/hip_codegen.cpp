@@ -367,11 +367,20 @@ private:
                 proj_exprs_.push_back(e.get());
                 continue;
             }
-            if (!prog_.joins.empty()) unsupported("SELECT * over joins is not supported; list the columns");
-            for (const auto& c : meta(prog_.fact_table).getColumnNames()) {
-                proj_star_.push_back(std::make_unique<ColumnRefExpr>(c));
-                proj_exprs_.push_back(proj_star_.back().get());
-            }
+            // every column of every scanned table, tables in plan order
+            // (collectTableScans: depth first, children in order), dimension
+            // columns through row ids (consumeProjectionVector, visitor.cpp:3475-3490)
+            std::vector<std::string> tables;
+            std::function<void(const OperatorNode*)> scans = [&](const OperatorNode* n) {
+                if (n->getType() == OperatorType::TABLE_SCAN) tables.push_back(static_cast<const TableScanNode*>(n)->table_name);
+                for (const auto& c : n->getChildren()) scans(c.get());
+            };
+            scans(proj_->getChildren()[0].get());
+            for (const auto& t : tables)
+                for (const auto& c : meta(t).getColumnNames()) {
+                    proj_star_.push_back(std::make_unique<ColumnRefExpr>(c));
+                    proj_exprs_.push_back(proj_star_.back().get());
+                }
         }
         if (proj_exprs_.empty()) unsupported("projection without columns");
         if (proj_exprs_.size() > 64) unsupported("more than 64 projection columns");

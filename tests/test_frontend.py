@@ -170,8 +170,8 @@ def test_projection_queries_compile(small_db):
     d = small_db.describe(PROJECTIONS_SSB[1])
     assert all(j["layout"] in ("key-indexed", "compact") for j in d["joins"])
     assert [o["nullable"] for o in d["outputs"]] == [1, 0, 1, 0]
-    with pytest.raises(pa.CrystalError, match="SELECT \\* over joins"):
-        small_db.describe("select * from lineorder, ddate where lo_orderdate = d_datekey")
+    star = small_db.describe("select * from lineorder, ddate where lo_orderdate = d_datekey")
+    assert len(star["outputs"]) == 11 + 8 and star["joins"][0]["dim"] == "DDATE"
     with pytest.raises(pa.CrystalError, match="shared scan"):
         small_db.generate_batch_code([PROJECTIONS_SSB[0]])
 

@@ -257,6 +257,7 @@ PROJECTIONS_SSB = [
     "order by lo_revenue desc, lo_orderdate limit 100",
     "select lo_revenue is null, lo_revenue from lineorder where lo_quantity = 50",
     "select * from lineorder where lo_quantity = 1 and lo_discount = 0",
+    "select * from lineorder, supplier where lo_suppkey = s_suppkey and lo_quantity = 1 and lo_discount = 0",
     "select d_year, p_brand1 from lineorder, ddate, part where lo_orderdate = d_datekey and lo_partkey = p_partkey "
     "and d_yearmonthnum = 199401 and p_category = 3",
 ]

@@ -353,6 +353,8 @@ PROJECTION_QUERIES = [
     "select lo_commitdate, lo_discount from lineorder where lo_discount = 2",
     "select lo_revenue, lo_discount from lineorder where lo_discount = 3 order by lo_revenue desc limit 10",
     "select * from lineorder where lo_discount = 9",
+    # every column of every scanned table, in plan order (DDATE, then LINEORDER)
+    "select * from lineorder, ddate where lo_orderdate = d_datekey and lo_discount = 8",
 ]
 
 
@@ -373,7 +375,10 @@ def projection_closed_form(tables, q):
     if q == 3:
         sel = sorted((r for r in rows if disc[r] == 3), key=lambda r: (-rev[r], r))[:10]
         return [(int(rev[r]), 3) for r in sel]
-    return [(int(od[r]), int(cd[r]) if cv[r] else None, int(rev[r]), 9) for r in rows if disc[r] == 9]
+    if q == 4:
+        return [(int(od[r]), int(cd[r]) if cv[r] else None, int(rev[r]), 9) for r in rows if disc[r] == 9]
+    return [(int(od[r]), year_of[int(od[r])], int(od[r]), int(cd[r]) if cv[r] else None, int(rev[r]), 8)
+            for r in rows if disc[r] == 8 and int(od[r]) in year_of]
 
 
 def test_projection_closed_form():
