@@ -327,8 +327,17 @@ void QueryServer::laneLoop(Lane* lane) {
                 std::lock_guard<std::mutex> lock(mu_);
                 last_error_ = err;
             }
+            // the batch's reservation is returned before its results are
+            // published, so a client that has every result sees it released
+            for (const auto& r : batch) released += r.footprint;
+            {
+                std::lock_guard<std::mutex> lock(mu_);
+                in_flight_bytes_ -= released;
+                ++batches_;
+                batched_queries_ += static_cast<long long>(batch.size());
+            }
+            released = 0;
             for (size_t i = 0; i < batch.size(); ++i) {
-                released += batch[i].footprint;
                 if (err.empty()) finish(batch[i], 0, "", &results[i]);
                 else finish(batch[i], -3, err, nullptr);
             }
@@ -346,14 +355,18 @@ void QueryServer::laneLoop(Lane* lane) {
                 cold_q_.push_back(std::move(r));
             }
             in_flight_bytes_ -= released;
-            if (!batch.empty()) {
-                ++batches_;
-                batched_queries_ += static_cast<long long>(batch.size());
-            }
         }
         if (!interp.empty() || !plain.empty()) cold_cv_.notify_all();
         cv_.notify_all();
     }
+}
+
+// returns a cold request's reservation (before its result is published)
+void QueryServer::release(const Request& r) {
+    std::lock_guard<std::mutex> lock(mu_);
+    in_flight_bytes_ -= r.footprint;
+    if (r.interpreted) ++interpreted_queries_;
+    else ++plain_queries_;
 }
 
 void QueryServer::coldLoop(Lane* lane) {
@@ -369,6 +382,7 @@ void QueryServer::coldLoop(Lane* lane) {
         // a cold signature: its own scan on the interpreted tier (no
         // compilation) or on its plain module once that is compiled
         ExecutionContext* ctx = lane->ctx.get();
+        bool returned = false;
         try {
             ctx->row_begin = 0;
             ctx->row_end = -1;
@@ -376,18 +390,15 @@ void QueryServer::coldLoop(Lane* lane) {
             db_.engine().executeQuery(r.sql, ctx);
             ctx->config.codegen.vm = false;
             QueryResult res = std::move(ctx->result);
+            release(r);
+            returned = true;
             finish(r, 0, "", &res);
         } catch (const std::exception& e) {
             ctx->config.codegen.vm = false;
+            if (!returned) release(r);
             finish(r, -3, e.what(), nullptr);
             std::lock_guard<std::mutex> lock(mu_);
             last_error_ = e.what();
-        }
-        {
-            std::lock_guard<std::mutex> lock(mu_);
-            in_flight_bytes_ -= r.footprint;
-            if (r.interpreted) ++interpreted_queries_;
-            else ++plain_queries_;
         }
         cv_.notify_all();
     }

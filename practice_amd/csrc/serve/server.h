@@ -134,6 +134,7 @@ private:
     void start();
     void laneLoop(Lane* lane);
     void coldLoop(Lane* lane);
+    void release(const Request& r);
     void compileLoop();
     bool compileOne();  // one queued compilation job; false when there is none
     void finish(Request& r, int rc, const std::string& err, QueryResult* result);
