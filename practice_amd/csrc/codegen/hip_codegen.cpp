@@ -783,11 +783,11 @@ private:
             if (gexpr->getType() == ExprType::COLUMN_REF) {
                 const auto& col = static_cast<const ColumnRefExpr*>(gexpr.get())->column_name;
                 const auto* s = stats(col);
-                // dimension columns arrive in payload registers, which carry
-                // no validity: their NULL code can never occur
-                bool payload = false;
-                for (const auto& jp : prog_.joins) payload = payload || jp.payload_col == col;
-                nul = nullable(col) && !payload;
+                // nullable from the catalog alone (visitor.cpp:2077): a
+                // dimension column arriving in a payload register (valid
+                // expression 1) still takes the +1 term and card + 1
+                // (visitor.cpp:2105-2108), so its code 0 is never used
+                nul = nullable(col);
                 if (s) {
                     mn = s->min_value_;
                     card = s->cardinality_;

@@ -198,8 +198,10 @@ def test_calendar_index_option(small_db):
 
 def test_layout_decisions_for_configs(small_db):
     c3 = small_db.describe(Q.C3)
-    # nation keys arrive in payload registers (no validity): no NULL code in the domain
-    assert c3["lds_aggregation"] == 1 and c3["groups"] == 25 * 25 * 7
+    # s_nation is nullable in the catalog: its term keeps the +1 shift and
+    # card + 1 although it arrives in a payload register (visitor.cpp:2077,
+    # 2105-2108), 25 x 26 x 7 = 4,550 slots
+    assert c3["lds_aggregation"] == 1 and c3["groups"] == 25 * 26 * 7
     joins = {j["dim"]: j for j in c3["joins"]}
     assert joins["SUPPLIER"]["payload"] == "s_nation" and joins["CUSTOMER"]["payload"] == "c_nation"
     # every supplier / customer key present: 5-bit nation codes indexed by key, no absent code

@@ -390,3 +390,98 @@ def test_projection_closed_form():
     r = odb.execute(db.explain(PROJECTION_QUERIES[0]))
     assert r.types == [0, 0, 1] and r.nullable == [False, False, False]
     assert odb.execute(db.explain(PROJECTION_QUERIES[2])).nullable == [True, False]
+
+
+# --------------------------------------------------------------- aggregate shaping: HAVING, MIN / MAX, ORDER BY
+def _year_groups(data):
+    lo, d = data["LINEORDER"], data["DDATE"]
+    hit, year = _lookup(d["d_datekey"][0], d["d_year"][0], lo["lo_orderdate"][0])
+    rev = lo["lo_revenue"][0].astype(np.int64)
+    rv = _bits(lo["lo_revenue"][1], len(rev))
+    cnt, tot = {}, {}
+    for y in range(1992, 1999):
+        m = hit & (year == y)
+        cnt[y] = int(m.sum())
+        tot[y] = int(rev[m & rv].sum())
+    return cnt, tot
+
+
+def test_having_closed_form(small_ssb):
+    """HAVING keeps the groups whose visible aggregates satisfy the predicate
+    (visitor.cpp:2441-2450), thresholds chosen between the actual values so
+    some years are cut by each arm."""
+    data, odb, db = small_ssb
+    cnt, tot = _year_groups(data)
+    c_thr = sorted(cnt.values())[2]
+    s_thr = sorted(tot.values())[4]
+    sql = (f"select d_year, sum(lo_revenue), count(*) from lineorder, ddate where lo_orderdate = d_datekey "
+           f"group by d_year having count(*) >= {c_thr} and sum(lo_revenue) > {s_thr} order by d_year")
+    want = [(y, tot[y], cnt[y]) for y in range(1992, 1999) if cnt[y] >= c_thr and tot[y] > s_thr]
+    assert 0 < len(want) < 7
+    assert odb.execute(db.explain(sql)).tuples() == want
+    sql = (f"select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
+           f"group by d_year having not (count(*) < {c_thr}) or sum(lo_revenue) <= {s_thr} order by d_year")
+    want = [(y, cnt[y], tot[y]) for y in range(1992, 1999) if not (cnt[y] < c_thr) or tot[y] <= s_thr]
+    assert odb.execute(db.explain(sql)).tuples() == want
+
+
+def test_min_max_closed_form(small_ssb):
+    """MIN / MAX over u64 values of the leftmost-cast expression (inline_math.h
+    atomic min / max), NULL inputs skipped, per group and scalar."""
+    data, odb, db = small_ssb
+    lo = data["LINEORDER"]
+    tax, qty, ext, disc = (lo[c][0].astype(np.int64) for c in ("lo_tax", "lo_quantity", "lo_extendedprice", "lo_discount"))
+    rev = lo["lo_revenue"][0].astype(np.int64)
+    rv = _bits(lo["lo_revenue"][1], len(rev))
+    m64 = (1 << 64) - 1
+    sql = ("select lo_tax, min(lo_quantity), max(lo_extendedprice), min(lo_revenue), max(lo_revenue), "
+           "count(lo_revenue), min(lo_discount - 5), max(lo_discount - 5) from lineorder where lo_discount = 5 "
+           "or lo_discount = 2 group by lo_tax order by lo_tax")
+    want = []
+    for t in sorted(set(tax.tolist())):
+        m = (tax == t) & ((disc == 5) | (disc == 2))
+        if not m.any():
+            continue
+        mv = m & rv
+        wrapped = [(x - 5) & m64 for x in disc[m].tolist()]
+        want.append((t, int(qty[m].min()), int(ext[m].max()), int(rev[mv].min()), int(rev[mv].max()), int(mv.sum()),
+                     min(wrapped), max(wrapped)))
+    assert odb.execute(db.explain(sql)).tuples() == want
+    m = qty == 1
+    sql = "select min(lo_revenue), max(lo_revenue), min(lo_extendedprice) from lineorder where lo_quantity = 1"
+    assert odb.execute(db.explain(sql)).tuples() == [(int(rev[m & rv].min()), int(rev[m & rv].max()), int(ext[m].min()))]
+
+
+def ties_tables():
+    """A fact table of ten groups whose counts tie (5, 3 and 2 rows) and whose
+    values are NULL for every row of groups 3 and 6."""
+    counts = [5, 3, 5, 2, 5, 3, 1, 5, 4, 2]
+    g = np.repeat(np.arange(10, dtype=np.int32), counts)
+    rng = np.random.default_rng(4)
+    g = g[rng.permutation(len(g))]
+    valid = ~np.isin(g, [3, 6])
+    v = np.where(valid, 1, 0).astype(np.int32)
+    return {"TIES": (len(g), True, {"t_g": (g, None), "t_v": (v, bitmap_from_bools(valid))})}
+
+
+# ORDER BY over the dense result: valid before NULL in either direction, then
+# the value, then the group index (visitor.cpp:320-358); group index = t_g here
+TIES_QUERIES = [
+    ("select t_g, count(*) from ties group by t_g order by 2 desc limit 6",
+     [(0, 5), (2, 5), (4, 5), (7, 5), (8, 4), (1, 3)]),
+    ("select t_g, sum(t_v) from ties group by t_g order by 2 desc, t_g desc",
+     [(7, 5), (4, 5), (2, 5), (0, 5), (8, 4), (5, 3), (1, 3), (9, 2), (6, None), (3, None)]),
+    ("select t_g, sum(t_v) from ties group by t_g order by 2 limit 3", [(9, 2), (1, 3), (5, 3)]),
+    ("select t_g, sum(t_v) from ties group by t_g order by 2", [(9, 2), (1, 3), (5, 3), (8, 4), (0, 5), (2, 5), (4, 5),
+                                                                 (7, 5), (3, None), (6, None)]),
+]
+
+
+def test_order_by_desc_limit_tie_break_known_answers():
+    tables = ties_tables()
+    g = tables["TIES"][2]["t_g"][0]
+    assert np.bincount(g).tolist() == [5, 3, 5, 2, 5, 3, 1, 5, 4, 2]
+    db, odb = build_pair(tables)
+    for sql, want in TIES_QUERIES:
+        got = odb.execute(db.explain(sql))
+        assert got.ordered and got.tuples() == want, sql
