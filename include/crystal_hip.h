@@ -56,7 +56,7 @@ typedef struct cq_module cq_module;
 typedef struct cq_db cq_db;
 typedef struct cq_result cq_result;
 typedef struct cq_server cq_server;
-struct cq_query_params; /* practice_amd/csrc/device/query_params.h */
+#include "cq_query_params.h" /* struct cq_query_params: the kernel argument block cq_launch passes */
 
 const char* cq_last_error(void);
 const char* cq_version(void);
@@ -66,10 +66,16 @@ const char* cq_version(void);
  * library crystal_hip.h is available as an include. */
 int cq_compile(const char* source, size_t length, const char* arch, cq_module** out);
 /* Launch kernel `name` of a module with a parameter block on a HIP stream
- * (NULL = default stream).  grid/block in work-groups / work-items. */
+ * (NULL = default stream).  grid/block in work-groups / work-items.  A
+ * module of cq_db_generate_code holds cq_build_<j> (one per join, any grid
+ * of 256-thread blocks: grid-stride over the dimension rows) and cq_scan
+ * (blocks of "block_threads", any grid: grid-stride over the tiles); the
+ * caller zero-fills the hash tables and the result slots first. */
 int cq_launch(cq_module* module, const char* name, const struct cq_query_params* params, unsigned grid, unsigned block,
               void* stream);
 size_t cq_module_code_size(const cq_module* module);
+/* sizeof(struct cq_query_params) as this library was built (binding check) */
+size_t cq_query_params_size(void);
 void cq_module_release(cq_module* module);
 
 /* ---- database ------------------------------------------------------ */
@@ -114,6 +120,11 @@ long long cq_db_result_size(cq_db* db, const char* sql);
 int cq_db_column_stats(cq_db* db, const char* column, long long* min_value, long long* max_value,
                        unsigned long long* cardinality, int* nullable, int* unique);
 int cq_db_fetch_column(cq_db* db, const char* column, int32_t* out, uint64_t* validity_out);
+/* Device address of a resident column and of its LSB-first u64 validity
+ * bitmap (NULL when the column has none) and its rows, for binding to
+ * cq_query_params slots: ExecutionContext::getBuffer<int>("d_<col>") and
+ * getNullBitmap("d_<col>") (include/core/execution.h:163-177). */
+int cq_db_column_device(cq_db* db, const char* column, const int32_t** data, const uint64_t** validity, long long* rows);
 long long cq_db_table_rows(cq_db* db, const char* table);
 /* option names: "kernel_timing", "memory_guard", "lds_budget", "block_threads",
  * "vectors_per_thread", "lds_tables", "lds_aggregation", "compact_tables",
@@ -122,9 +133,10 @@ long long cq_db_table_rows(cq_db* db, const char* table);
  * "memory_guard_reserve_bytes", "prefetch" (load the next tile while the
  * current one is probed), "interpret" (1: run plans on the ahead-of-time
  * interpreter instead of hipRTC-compiled kernels; same results, no compile),
- * "vm_row_kernel", "host_sort_rows", "prepared_cache" */
+ * "vm_row_kernel", "host_sort_rows", "prepared_cache", "module_cache_capacity" (compiled modules kept,
+ * least recently used evicted) */
 int cq_db_set_option(cq_db* db, const char* name, long long value);
-long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","compiles" */
+long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","evictions","compiles" */
 int cq_db_clear_cache(cq_db* db);
 
 int cq_db_execute(cq_db* db, const char* sql, cq_result** out);

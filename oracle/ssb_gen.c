@@ -234,17 +234,23 @@ int orc_gen_column(const orc_genspec* spec, const char* table, const char* colum
     if (resolve(spec, table, column, &rc) != 0) return -1;
     const int bp = orc_column_null_bp(spec, table, column);
     const uint64_t vseed = stream_seed(spec->seed, table, column, "#valid");
-    if (validity) {
-        if (start % 64 != 0) return -1;
-        memset(validity, 0, (size_t)((count + 63) / 64) * sizeof(uint64_t));
-    }
-    for (int64_t i = 0; i < count; ++i) {
-        const int64_t r = start + i;
-        const int32_t v = cell(&rc, r);
-        int valid = 1;
-        if (bp > 0) valid = uni(vseed, r, 0, 9999) >= bp;
-        out[i] = valid ? v : 0;
-        if (validity && valid) validity[i >> 6] |= 1ULL << (i & 63);
+    if (validity && start % 64 != 0) return -1;
+    /* counter-based cells: any row range is generated independently, so
+     * 64-row groups (one validity word each) are spread over the threads */
+    const int64_t groups = (count + 63) / 64;
+#pragma omp parallel for schedule(static)
+    for (int64_t w = 0; w < groups; ++w) {
+        uint64_t word = 0;
+        const int64_t end = (w + 1) * 64 < count ? (w + 1) * 64 : count;
+        for (int64_t i = w * 64; i < end; ++i) {
+            const int64_t r = start + i;
+            const int32_t v = cell(&rc, r);
+            int valid = 1;
+            if (bp > 0) valid = uni(vseed, r, 0, 9999) >= bp;
+            out[i] = valid ? v : 0;
+            if (valid) word |= 1ULL << (i & 63);
+        }
+        if (validity) validity[w] = word;
     }
     return 0;
 }

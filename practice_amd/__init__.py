@@ -43,6 +43,7 @@ def _load() -> ctypes.CDLL:
         "cq_compile": (c_int, [c_char_p, c_sz, c_char_p, P(c_void_p)]),
         "cq_launch": (c_int, [c_void_p, c_char_p, c_void_p, ctypes.c_uint, ctypes.c_uint, c_void_p]),
         "cq_module_code_size": (c_sz, [c_void_p]),
+        "cq_query_params_size": (c_sz, []),
         "cq_module_release": (None, [c_void_p]),
         "cq_db_create": (c_int, [c_int, P(c_void_p)]),
         "cq_db_destroy": (None, [c_void_p]),
@@ -60,6 +61,7 @@ def _load() -> ctypes.CDLL:
         "cq_db_column_stats": (c_int, [c_void_p, c_char_p, P(c_ll), P(c_ll), P(c_ull), P(c_int), P(c_int)]),
         "cq_db_fetch_column": (c_int, [c_void_p, c_char_p, c_void_p, c_void_p]),
         "cq_db_table_rows": (c_ll, [c_void_p, c_char_p]),
+        "cq_db_column_device": (c_int, [c_void_p, c_char_p, P(c_void_p), P(c_void_p), P(c_ll)]),
         "cq_db_set_option": (c_int, [c_void_p, c_char_p, c_ll]),
         "cq_db_cache_stat": (c_ll, [c_void_p, c_char_p]),
         "cq_db_clear_cache": (c_int, [c_void_p]),
@@ -273,6 +275,13 @@ class Database:
             raise CrystalError(-1, _lib.cq_last_error().decode())
         return n
 
+    def column_device(self, column: str):
+        """(device address of the int32 data, of the u64 validity bitmap or 0, rows)."""
+        data, valid, rows = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_longlong()
+        _check(_lib.cq_db_column_device(self._h, column.encode(), ctypes.byref(data), ctypes.byref(valid),
+                                        ctypes.byref(rows)))
+        return data.value or 0, valid.value or 0, rows.value
+
     def fetch_column(self, table: str, column: str):
         n = self.table_rows(table)
         out = np.empty(n, np.int32)
@@ -370,6 +379,45 @@ class Server:
             self.stop()
         except Exception:
             pass
+
+
+# include/cq_query_params.h
+CQ_MAX_FACT_COLS, CQ_MAX_DIM_COLS, CQ_MAX_JOINS, CQ_MAX_LITS, CQ_MAX_FLITS = 16, 24, 8, 48, 8
+
+
+class QueryParams(ctypes.Structure):
+    """struct cq_query_params: the kernel argument block of a generated module."""
+    _fields_ = [("col", ctypes.c_void_p * CQ_MAX_FACT_COLS), ("valid", ctypes.c_void_p * CQ_MAX_FACT_COLS),
+                ("dcol", ctypes.c_void_p * CQ_MAX_DIM_COLS), ("dvalid", ctypes.c_void_p * CQ_MAX_DIM_COLS),
+                ("ht", ctypes.c_void_p * CQ_MAX_JOINS), ("dim_rows", ctypes.c_longlong * CQ_MAX_JOINS),
+                ("result", ctypes.c_void_p), ("result_valid", ctypes.c_void_p), ("tile_count", ctypes.c_void_p),
+                ("tile_off", ctypes.c_void_p), ("lit", ctypes.c_longlong * CQ_MAX_LITS),
+                ("flit", ctypes.c_double * CQ_MAX_FLITS), ("row_begin", ctypes.c_longlong),
+                ("row_end", ctypes.c_longlong), ("num_tiles", ctypes.c_longlong)]
+
+
+class Module:
+    """A hipRTC-compiled module (cq_compile) whose kernels the caller launches
+    with its own parameter block (cq_launch)."""
+
+    def __init__(self, source: str, arch: str = "gfx950"):
+        self._h = ctypes.c_void_p()
+        data = source.encode()
+        _check(_lib.cq_compile(data, len(data), arch.encode(), ctypes.byref(self._h)))
+
+    def launch(self, kernel: str, params: QueryParams, grid: int, block: int, stream: int = 0) -> None:
+        _check(_lib.cq_launch(self._h, kernel.encode(), ctypes.byref(params), grid, block, stream or None))
+
+    def close(self) -> None:
+        if self._h:
+            _lib.cq_module_release(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def compile_source(source: str, arch: str = "gfx950") -> int:

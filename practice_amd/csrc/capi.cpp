@@ -111,6 +111,7 @@ int cq_launch(cq_module* module, const char* name, const struct cq_query_params*
 }
 
 size_t cq_module_code_size(const cq_module* module) { return module ? module->handle->codeSize() : 0; }
+size_t cq_query_params_size(void) { return sizeof(cq_query_params); }
 void cq_module_release(cq_module* module) { delete module; }
 
 int cq_db_create(int device, cq_db** out) {
@@ -252,6 +253,20 @@ int cq_db_fetch_column(cq_db* d, const char* column, int32_t* out, uint64_t* val
     });
 }
 
+int cq_db_column_device(cq_db* d, const char* column, const int32_t** data, const uint64_t** validity, long long* rows) {
+    return guarded([&] {
+        if (!d || !column || !data) return fail(CQ_ERR_INVALID, "cq_db_column_device: null argument");
+        if (d->inst->hostOnly()) return fail(CQ_ERR_INVALID, "cq_db_column_device: host-only database");
+        const auto& cols = d->inst->columns();
+        auto it = cols.find(column);
+        if (it == cols.end()) return fail(CQ_ERR_INVALID, std::string("column not loaded on the device: ") + column);
+        *data = it->second.data;
+        if (validity) *validity = it->second.validity;
+        if (rows) *rows = it->second.rows;
+        return CQ_OK;
+    });
+}
+
 long long cq_db_table_rows(cq_db* d, const char* table) {
     try {
         return static_cast<long long>(d->inst->catalog().getTableMetadata(table).getSize());
@@ -283,6 +298,11 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else if (n == "host_sort_rows") c.host_sort_rows = v;
         else if (n == "prepared_cache") c.prepared_cache = v != 0;
+        else if (n == "module_cache_capacity") {
+            if (v < 1) return fail(CQ_ERR_INVALID, "module_cache_capacity must be positive");
+            c.module_cache_capacity = static_cast<size_t>(v);
+            d->inst->moduleCache().setCapacity(static_cast<size_t>(v));
+        }
         else return fail(CQ_ERR_INVALID, "unknown option " + n);
         return CQ_OK;
     });
@@ -294,6 +314,7 @@ long long cq_db_cache_stat(cq_db* d, const char* name) {
     if (n == "size") return static_cast<long long>(c.size());
     if (n == "hits") return static_cast<long long>(c.hits());
     if (n == "misses") return static_cast<long long>(c.misses());
+    if (n == "evictions") return static_cast<long long>(c.evictions());
     if (n == "compiles") return static_cast<long long>(d->inst->compiler().compileCount());
     return -1;
 }

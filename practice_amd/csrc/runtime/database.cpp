@@ -177,6 +177,7 @@ DatabaseInstance::DatabaseInstance(int device, EngineConfig config) : device_(de
         hipCheck(hipGetDeviceCount(&n), "hipGetDeviceCount");
         if (device_ >= n) throw std::runtime_error("HIP device " + std::to_string(device_) + " not present");
         hipCheck(hipSetDevice(device_), "hipSetDevice");
+        DeviceMapLock lock;
         hipCheck(hipStreamCreate(&ctx_->stream), "hipStreamCreate");
         ctx_->owns_stream = true;
     }
@@ -190,6 +191,7 @@ std::unique_ptr<ExecutionContext> DatabaseInstance::newContext() {
     c->columns = &columns_;
     c->loaded_device_bytes = loaded_bytes_;
     hipCheck(hipSetDevice(device_), "hipSetDevice");
+    DeviceMapLock lock;
     hipCheck(hipStreamCreate(&c->stream), "hipStreamCreate");
     c->owns_stream = true;
     return c;
@@ -200,6 +202,7 @@ DatabaseInstance::~DatabaseInstance() {
     engine_.reset();
     cache_.reset();
     if (device_ >= 0) {
+        DeviceMapLock lock;
         (void)hipDeviceSynchronize();
         for (void* p : allocations_) (void)hipFree(p);
     }
@@ -207,6 +210,7 @@ DatabaseInstance::~DatabaseInstance() {
 
 void* DatabaseInstance::deviceAlloc(size_t bytes) {
     void* p = nullptr;
+    DeviceMapLock lock;
     hipCheck(hipMalloc(&p, std::max<size_t>(bytes, 256)), "hipMalloc column");
     allocations_.push_back(p);
     loaded_bytes_ += bytes;
@@ -236,14 +240,20 @@ void DatabaseInstance::loadColumn(const std::string& table, const std::string& c
     dc.rows = rows;
     if (device_ >= 0) {
         void* d = deviceAlloc(static_cast<size_t>(rows) * 4);
-        if (rows) hipCheck(hipMemcpy(d, data, static_cast<size_t>(rows) * 4, hipMemcpyHostToDevice), "upload column");
+        {
+            DeviceMapLock lock;  // null-stream copy: synchronizes with every stream
+            if (rows) hipCheck(hipMemcpy(d, data, static_cast<size_t>(rows) * 4, hipMemcpyHostToDevice), "upload column");
+        }
         dc.data = static_cast<const int*>(d);
         if (validity) {
             void* v = deviceAlloc(words * 8);
             // mask bits past the last row (maskUnusedValidityBits, utils.h:111-115)
             std::vector<uint64_t> vv(validity, validity + words);
             if (rows % 64 && !vv.empty()) vv.back() &= (1ULL << (rows % 64)) - 1;
-            if (words) hipCheck(hipMemcpy(v, vv.data(), words * 8, hipMemcpyHostToDevice), "upload validity");
+            {
+                DeviceMapLock lock;  // null-stream copy: synchronizes with every stream
+                if (words) hipCheck(hipMemcpy(v, vv.data(), words * 8, hipMemcpyHostToDevice), "upload validity");
+            }
             dc.validity = static_cast<const uint64_t*>(v);
         }
     } else {
@@ -545,10 +555,17 @@ void DatabaseInstance::fetchColumn(const std::string& column, int32_t* out, uint
         }
         return;
     }
-    hipCheck(hipMemcpy(out, dc.data, static_cast<size_t>(dc.rows) * 4, hipMemcpyDeviceToHost), "fetch column");
+    {
+        DeviceMapLock lock;  // null-stream copy: synchronizes with every stream
+        hipCheck(hipMemcpy(out, dc.data, static_cast<size_t>(dc.rows) * 4, hipMemcpyDeviceToHost), "fetch column");
+    }
     if (validity_out) {
-        if (dc.validity) hipCheck(hipMemcpy(validity_out, dc.validity, words * 8, hipMemcpyDeviceToHost), "fetch validity");
-        else std::fill(validity_out, validity_out + words, ~0ULL);
+        if (dc.validity) {
+            DeviceMapLock lock;  // null-stream copy: synchronizes with every stream
+            hipCheck(hipMemcpy(validity_out, dc.validity, words * 8, hipMemcpyDeviceToHost), "fetch validity");
+        } else {
+            std::fill(validity_out, validity_out + words, ~0ULL);
+        }
     }
 }
 

@@ -16,6 +16,7 @@
 extern const char* const kCrystalHipHeaderSource;
 extern const char* const kQueryParamsHeaderSource;
 extern const char* const kHtLayoutsHeaderSource;
+extern const char* const kParamsLayoutHeaderSource;
 
 namespace db {
 
@@ -23,12 +24,52 @@ void hipCheck(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// ------------------------------------------------------------------ device map lock
+
+namespace {
+thread_local bool t_capturing = false;
+}
+
+std::shared_mutex& deviceMapMutex() {
+    static std::shared_mutex mu;
+    return mu;
+}
+
+bool threadIsCapturing() { return t_capturing; }
+
+DeviceMapLock::DeviceMapLock() {
+    if (t_capturing)
+        throw std::logic_error("device allocation, free or synchronization inside a HIP graph capture");
+    deviceMapMutex().lock();
+}
+
+DeviceMapLock::~DeviceMapLock() { deviceMapMutex().unlock(); }
+
+CaptureLock::CaptureLock() {
+    if (t_capturing) throw std::logic_error("nested HIP graph capture");
+    deviceMapMutex().lock_shared();
+    t_capturing = true;
+}
+
+CaptureLock::~CaptureLock() {
+    t_capturing = false;
+    deviceMapMutex().unlock_shared();
+}
+
 // ------------------------------------------------------------------ DeviceBuffer
 
-DeviceBuffer::~DeviceBuffer() { release(); }
+DeviceBuffer::~DeviceBuffer() {
+    if (!ptr_) return;
+    if (t_capturing) {  // unwinding out of a capture: leak rather than break it
+        ptr_ = nullptr;
+        return;
+    }
+    release();
+}
 
 void DeviceBuffer::release() {
     if (ptr_) {
+        DeviceMapLock lock;
         (void)hipDeviceSynchronize();
         (void)hipFree(ptr_);
     }
@@ -42,6 +83,7 @@ void DeviceBuffer::ensure(size_t bytes) {
     if (bytes < (8ull << 20)) want = std::max(bytes, bytes_ + bytes_ / 2);
     if (want == 0) want = 256;
     release();
+    DeviceMapLock lock;
     hipCheck(hipMalloc(&ptr_, want), "hipMalloc scratch");
     bytes_ = want;
 }
@@ -52,12 +94,18 @@ CompiledModule::CompiledModule(std::string signature, std::vector<char> code)
     : signature_(std::move(signature)), code_(std::move(code)) {}
 
 CompiledModule::~CompiledModule() {
-    if (module_) (void)hipModuleUnload(module_);
+    if (!module_) return;
+    if (threadIsCapturing()) return;  // last handle dropped while unwinding a capture: keep the code loaded
+    DeviceMapLock lock;
+    (void)hipModuleUnload(module_);
 }
 
 hipFunction_t CompiledModule::function(const std::string& name) {
     std::lock_guard<std::mutex> lock(mu_);
-    if (!module_) hipCheck(hipModuleLoadData(&module_, code_.data()), "hipModuleLoadData");
+    if (!module_) {
+        DeviceMapLock dl;
+        hipCheck(hipModuleLoadData(&module_, code_.data()), "hipModuleLoadData");
+    }
     auto it = functions_.find(name);
     if (it != functions_.end()) return it->second;
     hipFunction_t f = nullptr;
@@ -73,10 +121,10 @@ HipRtcCompiler::HipRtcCompiler(std::string arch, std::vector<std::string> extra)
 
 ModuleHandle HipRtcCompiler::compile(const std::string& source, const std::string& query_hash) {
     hiprtcProgram prog = nullptr;
-    const char* headers[] = {kCrystalHipHeaderSource, kQueryParamsHeaderSource, kHtLayoutsHeaderSource};
-    const char* names[] = {"crystal_hip.h", "query_params.h", "ht_layouts.h"};
+    const char* headers[] = {kCrystalHipHeaderSource, kQueryParamsHeaderSource, kParamsLayoutHeaderSource, kHtLayoutsHeaderSource};
+    const char* names[] = {"crystal_hip.h", "query_params.h", "cq_query_params.h", "ht_layouts.h"};
     const std::string file = "q_" + query_hash + ".hip";
-    if (hiprtcCreateProgram(&prog, source.c_str(), file.c_str(), 3, headers, names) != HIPRTC_SUCCESS)
+    if (hiprtcCreateProgram(&prog, source.c_str(), file.c_str(), 4, headers, names) != HIPRTC_SUCCESS)
         throw std::runtime_error("hiprtcCreateProgram failed");
     std::vector<std::string> opts = {"--offload-arch=" + arch_, "-O3", "-std=c++17", "-fwrapv",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
@@ -116,19 +164,36 @@ ModuleHandle PlanSignatureCache::get(const std::string& key) {
 }
 
 void PlanSignatureCache::put(const std::string& key, ModuleHandle module) {
-    std::lock_guard<std::mutex> lock(mu_);
-    auto it = index_.find(key);
-    if (it != index_.end()) {
-        it->second->second = std::move(module);
-        lru_.splice(lru_.begin(), lru_, it->second);
-        return;
+    std::vector<ModuleHandle> evicted;  // unloaded (if last held) after the cache lock is dropped
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = index_.find(key);
+        if (it != index_.end()) {
+            evicted.push_back(std::move(it->second->second));
+            it->second->second = std::move(module);
+            lru_.splice(lru_.begin(), lru_, it->second);
+        } else {
+            lru_.emplace_front(key, std::move(module));
+            index_[key] = lru_.begin();
+            trimLocked(evicted);
+        }
     }
-    lru_.emplace_front(key, std::move(module));
-    index_[key] = lru_.begin();
+}
+
+void PlanSignatureCache::trimLocked(std::vector<ModuleHandle>& evicted) {
     while (lru_.size() > capacity_) {
+        evicted.push_back(std::move(lru_.back().second));
         index_.erase(lru_.back().first);
         lru_.pop_back();
+        ++evictions_;
     }
+}
+
+void PlanSignatureCache::setCapacity(size_t capacity) {
+    std::vector<ModuleHandle> evicted;
+    std::lock_guard<std::mutex> lock(mu_);
+    capacity_ = std::max<size_t>(1, capacity);
+    trimLocked(evicted);
 }
 
 size_t PlanSignatureCache::size() const {
@@ -137,8 +202,9 @@ size_t PlanSignatureCache::size() const {
 }
 
 void PlanSignatureCache::clear() {
+    std::list<std::pair<std::string, ModuleHandle>> dropped;
     std::lock_guard<std::mutex> lock(mu_);
-    lru_.clear();
+    dropped.swap(lru_);
     index_.clear();
 }
 

@@ -28,6 +28,7 @@
 #include <memory>
 #include <mutex>
 #include <set>
+#include <shared_mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -40,6 +41,34 @@
 namespace db {
 
 void hipCheck(hipError_t e, const char* what);
+
+// HIP calls that synchronize the whole device or change its memory map or
+// loaded code objects (hipMalloc, hipFree, hipDeviceSynchronize,
+// hipHostMalloc / hipHostFree, hipModuleLoadData / hipModuleUnload,
+// hipStreamCreate / Destroy, graph destruction) synchronize implicitly with
+// every stream, so one of them issued by any thread while another thread
+// captures a HIP graph invalidates that capture.  Such calls hold this lock
+// exclusively (DeviceMapLock); a capture holds it shared from begin to
+// instantiation (CaptureLock).  A thread that captures must not make such a
+// call: DeviceMapLock throws instead of deadlocking.
+std::shared_mutex& deviceMapMutex();
+bool threadIsCapturing();
+
+class DeviceMapLock {
+public:
+    DeviceMapLock();
+    ~DeviceMapLock();
+    DeviceMapLock(const DeviceMapLock&) = delete;
+    DeviceMapLock& operator=(const DeviceMapLock&) = delete;
+};
+
+class CaptureLock {
+public:
+    CaptureLock();
+    ~CaptureLock();
+    CaptureLock(const CaptureLock&) = delete;
+    CaptureLock& operator=(const CaptureLock&) = delete;
+};
 
 // Grow-only device allocation.  Grows by 1.5x below 1 Mi elements and exactly
 // above (device_buffer.h:59-75); frees after synchronizing its stream.
@@ -129,7 +158,10 @@ struct ExecutionContext {
     ExecutionContext& operator=(const ExecutionContext&) = delete;
     ~ExecutionContext() {
         scratch_.clear();
-        if (stream && owns_stream) (void)hipStreamDestroy(stream);
+        if (stream && owns_stream) {
+            DeviceMapLock lock;
+            (void)hipStreamDestroy(stream);
+        }
     }
     int device = 0;
     hipStream_t stream = nullptr;
@@ -206,9 +238,13 @@ public:
     void clear() override;
     size_t hits() const { return hits_; }
     size_t misses() const { return misses_; }
+    size_t evictions() const { return evictions_; }
+    void setCapacity(size_t capacity);
 
 private:
+    void trimLocked(std::vector<ModuleHandle>& evicted);
     size_t capacity_;
+    size_t evictions_ = 0;
     mutable std::mutex mu_;
     std::list<std::pair<std::string, ModuleHandle>> lru_;
     std::unordered_map<std::string, std::list<std::pair<std::string, ModuleHandle>>::iterator> index_;

@@ -15,6 +15,7 @@ namespace db {
 namespace {
 constexpr int kHashMinBits = 7, kHashMaxBits = 10;  // per-query LDS hash tables: 128..1024 slots
 constexpr int kMaskQueries = 64;                    // queries of a mask-form template per launch
+constexpr int kMaxGraphs = 64;                      // captured batch shapes kept per executor
 // dynamic LDS for the per-query regions: what lds_kb allows, minus the room
 // kept for shared dimension tables staged into LDS
 unsigned ldsLimit(const BatchOptions& opt) {
@@ -103,19 +104,27 @@ BatchExecutor::~BatchExecutor() {
     for (auto& kv : graphs_) dropGraph(kv.second);
     for (auto& e : ev_)
         if (e) (void)hipEventDestroy(e);
-    if (pinned_) (void)hipHostFree(pinned_);
+    if (pinned_) {
+        DeviceMapLock lock;
+        (void)hipHostFree(pinned_);
+    }
 }
 
 void BatchExecutor::dropGraph(Graph& g) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.exec || g.graph) {
+        DeviceMapLock lock;
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     g.exec = nullptr;
     g.graph = nullptr;
     g.key.clear();
+    g.modules.clear();
 }
 
 void* BatchExecutor::pinned(size_t bytes) {
     if (bytes <= pinned_bytes_) return pinned_;
+    DeviceMapLock lock;
     if (pinned_) hipCheck(hipHostFree(pinned_), "hipHostFree");
     pinned_ = nullptr;
     pinned_bytes_ = 0;
@@ -247,7 +256,9 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
           << ",\"host_ms\":{\"modules\":" << rep.modules_ms << ",\"bind\":" << rep.bind_ms << ",\"enqueue\":" << rep.enqueue_ms
           << ",\"wait\":" << rep.wait_ms << ",\"fetch\":" << rep.fetch_ms << "},\"arena_bytes\":" << rep.arena_bytes << "}";
     }
-    r << "],\"wall_ms\":" << elapsedMs(t0, std::chrono::steady_clock::now()) << "}";
+    r << "],\"graphs\":" << graphs_.size() << ",\"graph_failures\":" << graph_failures_.load() << ",\"last_graph_error\":\"";
+    for (char c : last_graph_error_.substr(0, 200)) r << ((c == '"' || c == '\\' || static_cast<unsigned char>(c) < 0x20) ? ' ' : c);
+    r << "\",\"wall_ms\":" << elapsedMs(t0, std::chrono::steady_clock::now()) << "}";
     report_ = r.str();
     return results;
 }
@@ -509,7 +520,8 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     if (opt.graph) {
         // everything a replay depends on besides the uploaded parameter blocks
         std::ostringstream key;
-        key << bm->signature() << "|" << grid << "|" << B.num_tiles << "|" << dqp << "|" << dtot;
+        key << bm->signature() << "|" << grid << "|" << cfg.block_threads << "|" << lds_bytes << "|" << B.num_tiles << "|" << dqp
+            << "|" << dtot;
         for (size_t u = 0; u < ucols.size(); ++u) key << "|" << B.col[u] << "," << B.valid[u];
         for (int k = 0; k < n; ++k) {
             const QueryRun& rr = runs[static_cast<size_t>(k)];
@@ -522,28 +534,59 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
             replayed = true;
             timed = false;
             rep.graph_mode = "replayed";
+            ++graph_replays_;
         } else if (++graph_seen_[ks] >= 2) {
-            if (graphs_.size() >= 8) {  // keep a few shapes
+            if (graphs_.size() >= static_cast<size_t>(kMaxGraphs)) {  // keep a bounded set of shapes
                 dropGraph(graphs_.begin()->second);
                 graphs_.erase(graphs_.begin());
             }
-            Graph& g = graphs_[ks];
-            hipCheck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            try {
-                enqueue(false);
-            } catch (...) {
-                hipGraph_t broken = nullptr;
-                (void)hipStreamEndCapture(s, &broken);
-                if (broken) (void)hipGraphDestroy(broken);
-                graphs_.erase(ks);
-                throw;
+            // Capture under the shared device-map lock: no thread of this
+            // process allocates, frees or synchronizes the device meanwhile.
+            // A capture that fails leaves no entry behind and the batch runs
+            // eagerly instead (the captured work was never launched).
+            Graph g;
+            bool ok = false;
+            std::string why;
+            {
+                CaptureLock capture;
+                hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    try {
+                        enqueue(false);
+                    } catch (const std::exception& ex) {
+                        why = ex.what();
+                    }
+                    hipGraph_t captured = nullptr;
+                    const hipError_t ee = hipStreamEndCapture(s, &captured);
+                    if (why.empty() && ee != hipSuccess) why = std::string("hipStreamEndCapture: ") + hipGetErrorString(ee);
+                    if (why.empty() && captured) {
+                        e = hipGraphInstantiate(&g.exec, captured, nullptr, nullptr, 0);
+                        if (e != hipSuccess) why = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+                    }
+                    g.graph = captured;
+                    ok = why.empty() && g.exec != nullptr;
+                } else {
+                    why = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
+                }
             }
-            hipCheck(hipStreamEndCapture(s, &g.graph), "hipStreamEndCapture");
-            hipCheck(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-            g.key = ks;
-            hipCheck(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
-            timed = false;
-            rep.graph_mode = "captured";
+            if (ok) {
+                g.key = ks;
+                g.modules = tmodules;
+                g.modules.push_back(bm);
+                for (const QueryRun& rr : runs) g.modules.push_back(rr.module);
+                hipCheck(hipGraphLaunch(g.exec, s), "hipGraphLaunch");
+                graphs_.emplace(ks, std::move(g));
+                ++graph_captures_;
+                timed = false;
+                rep.graph_mode = "captured";
+            } else {
+                (void)hipGetLastError();
+                dropGraph(g);
+                ++graph_failures_;
+                last_graph_error_ = why;
+                enqueue(true);
+                rep.graph_mode = "capture-failed";
+            }
         } else {
             enqueue(true);
         }

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -83,12 +84,19 @@ public:
     CodegenOptions codegenOptions(const BatchOptions& options) const;
     // JSON: per shared scan counts, timings, tuning choice, graph use.
     const std::string& lastReport() const { return report_; }
+    // graph captures, replays and failed captures since construction
+    long long graphCaptures() const { return graph_captures_.load(); }
+    long long graphReplays() const { return graph_replays_.load(); }
+    long long graphFailures() const { return graph_failures_.load(); }
 
 private:
     struct Graph {
         std::string key;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        // code objects the captured kernel nodes run: held so that a module
+        // cache eviction cannot unload them while the graph may replay
+        std::vector<ModuleHandle> modules;
     };
     struct Tuned {
         int block_threads = 1024, vectors_per_thread = 1;
@@ -101,6 +109,9 @@ private:
     std::map<std::string, int> graph_seen_; // shapes seen (captured on the second)
     std::map<std::string, Tuned> tuned_;    // keyed by template-set signature
     double plan_ms_ = 0;                    // SQL -> programs time of the current call
+    std::atomic<long long> graph_failures_{0};  // captures that failed (the batch then ran eagerly)
+    std::atomic<long long> graph_captures_{0}, graph_replays_{0};
+    std::string last_graph_error_;
     void* pinned_ = nullptr;                // staging for parameter blocks / row counts
     size_t pinned_bytes_ = 0;
     hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};

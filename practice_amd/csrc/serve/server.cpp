@@ -321,7 +321,12 @@ void QueryServer::laneLoop(Lane* lane) {
                 err = e.what();
                 // start the lane over with a fresh executor (arena, graphs, events)
                 try {
-                    lane->exec = std::make_unique<BatchExecutor>(db_, lane->ctx.get(), &lane->mu);
+                    auto fresh = std::make_unique<BatchExecutor>(db_, lane->ctx.get(), &lane->mu);
+                    std::lock_guard<std::mutex> lock(mu_);
+                    lane->graph_captures += lane->exec->graphCaptures();
+                    lane->graph_replays += lane->exec->graphReplays();
+                    lane->graph_failures += lane->exec->graphFailures();
+                    lane->exec = std::move(fresh);
                 } catch (const std::exception&) {
                 }
                 std::lock_guard<std::mutex> lock(mu_);
@@ -553,6 +558,12 @@ std::string QueryServer::statsJson() {
         return l[std::min(l.size() - 1, i)];
     };
     const double secs = elapsedMs(stats_start_, std::chrono::steady_clock::now()) / 1000.0;
+    long long g_capt = 0, g_replay = 0, g_fail = 0;
+    for (const auto& l : lanes_) {
+        g_capt += l->graph_captures + (l->exec ? l->exec->graphCaptures() : 0);
+        g_replay += l->graph_replays + (l->exec ? l->exec->graphReplays() : 0);
+        g_fail += l->graph_failures + (l->exec ? l->exec->graphFailures() : 0);
+    }
     std::ostringstream o;
     o << "{\"completed\":" << l.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p99_ms\":" << pct(0.99)
       << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
@@ -561,6 +572,7 @@ std::string QueryServer::statsJson() {
       << ",\"plain_queries\":" << plain_queries_ << ",\"interpreted_queries\":" << interpreted_queries_
       << ",\"admission_waits\":" << admission_waits_
       << ",\"compiles\":" << compiles_ << ",\"promotions\":" << promotions_ << ",\"rejected\":" << rejected_
+      << ",\"graph_captures\":" << g_capt << ",\"graph_replays\":" << g_replay << ",\"graph_failures\":" << g_fail
       << ",\"last_error\":\"" << jsonEscape(last_error_) << "\""
       << ",\"lanes\":" << opt_.lanes << ",\"budget_bytes\":" << opt_.budget_bytes << ",\"in_flight_bytes\":" << in_flight_bytes_
       << ",\"scan_set\":" << scan_set_.size() << "}";

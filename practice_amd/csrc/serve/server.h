@@ -96,8 +96,9 @@ private:
     struct Lane {
         std::unique_ptr<ExecutionContext> ctx;
         std::mutex mu;
-        std::unique_ptr<BatchExecutor> exec;
+        std::unique_ptr<BatchExecutor> exec;   // replaced under the server mutex
         std::thread thread;
+        long long graph_captures = 0, graph_replays = 0, graph_failures = 0;  // of replaced executors
     };
 
     DatabaseInstance& db_;

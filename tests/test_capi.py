@@ -12,9 +12,14 @@ ROOT = pathlib.Path(__file__).resolve().parents[1]
 
 
 def declared_symbols():
-    text = (ROOT / "include" / "crystal_hip.h").read_text()
+    text = "".join(p.read_text() for p in sorted((ROOT / "include").glob("*.h")))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(cq_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_params_block_layout_matches_binding():
+    # include/cq_query_params.h as the library was built vs the ctypes mirror
+    assert pa.lib().cq_query_params_size() == ctypes.sizeof(pa.QueryParams) == 1272
 
 
 def test_library_exports_every_declared_symbol():

@@ -292,3 +292,45 @@ def test_projection_crafted_keys():
             print(f"[parity] projection sql={sql}", flush=True)
             got, want = g.execute(sql), odb.execute(g.explain(sql))
             assert got.tuples() == want.tuples(), sql  # row order as well
+
+
+FULL_COLUMNS = {
+    "C1": {"LINEORDER": ["lo_orderdate", "lo_discount", "lo_quantity", "lo_extendedprice"],
+           "DDATE": ["d_datekey", "d_year"]},
+    "C2": {"LINEORDER": ["lo_orderdate", "lo_partkey", "lo_suppkey", "lo_revenue"],
+           "PART": ["p_partkey", "p_category", "p_brand1"], "SUPPLIER": ["s_suppkey", "s_region"],
+           "DDATE": ["d_datekey", "d_year"]},
+    "C3": {"LINEORDER": ["lo_custkey", "lo_suppkey", "lo_orderdate", "lo_revenue"],
+           "CUSTOMER": ["c_custkey", "c_nation"], "SUPPLIER": ["s_suppkey", "s_nation"],
+           "DDATE": ["d_datekey", "d_year"]},
+}
+
+
+@pytest.mark.parametrize("name", ["C1", "C2", "C3"])
+def test_config_at_full_size(name):
+    """C1 (6M fact rows), C2 (120M) and C3 (600M) at their stated sizes: the
+    GPU generator and the oracle's build the same columns from one seed, the
+    oracle runs the query on every host core, the rows must be identical."""
+    import time
+    sf, rows = Q.FULL[name]
+    nullable = Q.C3_NULLABLE if name == "C3" else ""
+    sql = getattr(Q, name)
+    seed = 101
+    t0 = time.perf_counter()
+    print(f"[full] {name}: {rows:,} fact rows, oracle generation", flush=True)
+    spec = orc.GenSpec(sf=sf, fact_rows=rows, seed=seed, nullable=nullable)
+    cols = FULL_COLUMNS[name]
+    odb = orc.load_ssb(orc.gen_tables(spec, tables=list(cols), columns=cols), spec)
+    print(f"[full] {name}: oracle data {time.perf_counter() - t0:.1f} s; GPU generation", flush=True)
+    with pa.Database(0) as gdb:
+        gdb.generate_ssb(sf=sf, fact_rows=rows, seed=seed, nullable=nullable)
+        gdb.finalize()
+        g = gdb.execute(sql)
+        plan = gdb.explain(sql)
+        groups = gdb.describe(sql)["groups"]
+    print(f"[full] {name}: GPU done ({g.rows} rows); oracle query", flush=True)
+    o = odb.execute(plan)
+    print(f"[full] {name}: oracle done {time.perf_counter() - t0:.1f} s", flush=True)
+    assert_same(g, o, ordered=(name == "C2"))
+    if name == "C3":
+        assert groups == 25 * 26 * 7 and g.rows == 25 * 25 * 7

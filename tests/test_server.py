@@ -125,3 +125,61 @@ def test_projection_requests_run_on_cold_lanes():
     finally:
         srv.stop()
     assert_same(r, odb.execute(gdb.explain(q)))
+
+
+def _graph_shapes(rng):
+    """Two batch shapes of fresh constants: 1- and 2-dimension templates."""
+    return [[Q.template_query(1, rng), Q.template_query(1, rng)],
+            [Q.template_query(2, rng), Q.template_query(2, rng), Q.template_query(1, rng)]]
+
+
+@pytest.mark.gpu
+def test_batch_graph_replay_survives_module_eviction():
+    """A captured batch keeps the code objects its kernel nodes run: with a
+    one-module cache every batch evicts the other shape's modules, and the
+    replays (fresh constants each round) still match the oracle."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(21)
+    modes = []
+    gdb.set_option("module_cache_capacity", 1)
+    try:
+        for rnd in range(5):
+            for qs in _graph_shapes(rng):
+                print(f"[graph] round={rnd} queries={len(qs)}", flush=True)
+                res = gdb.batch_execute(qs, {"graph": 1})
+                modes += [s["graph"] for s in gdb.batch_report()["scans"]]
+                for q, r in zip(qs, res):
+                    assert_same(r, odb.execute(gdb.explain(q)))
+        evictions = gdb.cache_stat("evictions")
+    finally:
+        gdb.set_option("module_cache_capacity", 256)
+    assert evictions > 0
+    assert "replayed" in modes and "capture-failed" not in modes, modes
+
+
+@pytest.mark.gpu
+def test_server_graph_launches_with_changing_batches():
+    """graph=1 on the server: batches of changing composition on two lanes,
+    cold requests (interpreted, allocating their scratch) on the cold lanes
+    meanwhile; every result matches the oracle and no capture fails."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(33)
+    srv = pa.Server(gdb, {"lanes": 2, "max_batch": 8, "graph": 1, "cold_lanes": 2})
+    try:
+        done = []
+        for wave in range(6):
+            qs = [Q.template_query(rng.randint(1, 4), rng) for _ in range(rng.randint(3, 12))]
+            qs += [Q.adhoc_query(rng) for _ in range(2)]
+            tickets = [(q, srv.submit(q)) for q in qs]
+            done += [(q, srv.wait(t, 120_000)) for q, t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st["rejected"] == 0 and st["graph_failures"] == 0, st
+    for q, r in done:
+        print(f"[server-graph] sql={q}", flush=True)
+        assert_same(r, odb.execute(gdb.explain(q)))
