@@ -104,7 +104,7 @@ int64_t orc_table_rows(const orc_genspec* spec, const char* table) {
 
 static const char* LO_COLS[] = {"lo_custkey", "lo_partkey", "lo_suppkey", "lo_orderdate", "lo_commitdate",
                                 "lo_quantity", "lo_extendedprice", "lo_discount", "lo_revenue",
-                                "lo_supplycost", "lo_tax"};
+                                "lo_supplycost", "lo_tax", "lo_orderkey", "lo_linenumber"};
 static const char* P_COLS[] = {"p_partkey", "p_mfgr", "p_category", "p_brand1", "p_size"};
 static const char* S_COLS[] = {"s_suppkey", "s_city", "s_nation", "s_region"};
 static const char* C_COLS[] = {"c_custkey", "c_city", "c_nation", "c_region", "c_mktsegment"};
@@ -140,7 +140,7 @@ int orc_column_null_bp(const orc_genspec* spec, const char* table, const char* c
 
 /* Column recipe: resolved once per call so the per-row loop does no string work. */
 enum { K_KEY, K_UNI, K_DATEFK, K_PCAT, K_PMFGR, K_PBRAND, K_NATION, K_REGION, K_CITY,
-       K_DKEY, K_DYEAR, K_DMONTH, K_DYM, K_DDOW, K_DDOM, K_DDOY, K_DWEEK };
+       K_DKEY, K_DYEAR, K_DMONTH, K_DYM, K_DDOW, K_DDOM, K_DDOY, K_DWEEK, K_ORDERKEY, K_LINENUMBER };
 typedef struct { int kind; uint64_t s0, s1; int64_t lo, hi; } recipe;
 
 static int resolve(const orc_genspec* sp, const char* t, const char* c, recipe* rc) {
@@ -161,6 +161,9 @@ static int resolve(const orc_genspec* sp, const char* t, const char* c, recipe* 
         if (!strcmp(c, "lo_revenue")) UNI(100000, 10000000);
         if (!strcmp(c, "lo_supplycost")) UNI(60000, 600000);
         if (!strcmp(c, "lo_tax")) UNI(0, 8);
+        /* orders of 4 lines: order key row / 4 + 1, line number row % 4 + 1 */
+        if (!strcmp(c, "lo_orderkey")) { rc->kind = K_ORDERKEY; return 0; }
+        if (!strcmp(c, "lo_linenumber")) { rc->kind = K_LINENUMBER; return 0; }
         return -1;
     }
     if (!strcmp(t, "PART")) {
@@ -212,6 +215,8 @@ static int32_t cell(const recipe* rc, int64_t r) {
         case K_NATION: return (int32_t)uni(rc->s0, r, 0, 24);
         case K_REGION: return (int32_t)(uni(rc->s0, r, 0, 24) / 5);
         case K_CITY: return (int32_t)(uni(rc->s0, r, 0, 24) * 10 + uni(rc->s1, r, 0, 9));
+        case K_ORDERKEY: return (int32_t)(r / 4 + 1);
+        case K_LINENUMBER: return (int32_t)(r % 4 + 1);
         default: break;
     }
     civil(r, &y, &m, &dom, &doy);

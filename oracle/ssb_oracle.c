@@ -17,6 +17,10 @@
  *                      BlockBuildSelectivePHT  include/crystal/join.h:124-253
  *   probe()            consumeHashJoinVector   src/core/visitor.cpp:4556-4731
  *                      BlockProbeAndPHT_1/_2   include/crystal/join.h:7-120
+ *   build_tables() /   MHT choice (non-unique key) visitor.cpp:3946-3970,
+ *   mht_probe()        emitMHTBuildKernels     visitor.cpp:4295-4464,
+ *                      BlockBuildMHT_Count/_Write, ProbeMultiHT join.h:276-364,
+ *                      the row-expanding probe visitor.cpp:4671-4730
  *   fast_filter()      consumeFilterVector     src/core/visitor.cpp:1823-1893
  *                      BlockPred* (int items)  include/crystal/pred.h:70-275
  *   vexpr()/vvalid()   expressionValueNoEmit / expressionValidExpr
@@ -43,7 +47,11 @@
  *   - closed forms: brute-force nested-loop joins and Python evaluation of the
  *     star-join queries on small instances; u64 sums compared exactly.
  * Parity unpinned: hash-table collision behaviour (the reference's first-CAS
- * winner is nondeterministic); the synthetic data never produces collisions.
+ * winner is nondeterministic, and so is the key an MHT directory slot keeps
+ * when two keys share it); the synthetic data never produces collisions.
+ * MHT rows within one directory slot are kept in build-row order here (the
+ * reference's order is that of its atomic write counters), so only the
+ * order of projected rows can differ.
  *
  * Deliberate, documented definitions where the reference is undefined:
  *   - integer division by zero yields 0;  INT_MIN / -1 wraps;
@@ -580,6 +588,11 @@ typedef struct {
     ocol* fks[MAXJ];   /* OR-key join: every fact key alternative, in condition order */
     int nfk;
     int rowid;         /* payload is the dimension row index (projection of its columns) */
+    int mht;           /* non-unique build key: multi-value table, one pipeline pass per build row */
+    int32_t* mkey;     /* MHT directory: key, offset and count per slot (len each) */
+    int32_t* moff;
+    int32_t* mcnt;
+    int32_t* mrow;     /* MHT payload: build row ids grouped by slot */
     ocol* payload;     /* NULL for PHT_1 */
     const expr* build_pred;
     int32_t kmin;
@@ -756,8 +769,11 @@ static int build_info(query* q, const plan* join, const plan* build_side, joinin
         if (ft != q->fact) FAIL("join key %s is not a fact-table column", ji->fks[k]->name);
     }
     ji->fk_idx = (int)(ji->fk - q->fact->cols);
-    if (!(ji->pk->pk || ji->pk->unique || ji->pk->card == (uint64_t)ji->dim->rows))
-        FAIL("non-unique build key %s needs a multi-value hash table (not supported)", ji->pk->name);
+    /* MHT only when the catalog proves the key non-unique (visitor.cpp:3951-3961);
+     * it always carries the build row id (3963-3970) */
+    ji->mht = !(ji->pk->pk || ji->pk->unique || ji->pk->card == (uint64_t)ji->dim->rows) &&
+              ji->pk->card > 0 && ji->pk->card < (uint64_t)ji->dim->rows;
+    if (ji->mht && ji->nfk > 1) FAIL("OR-key join on a non-unique build key");
     /* payload: first column of this dimension needed above the join, aggregate
      * columns (aggregates, then group-bys) before filter columns */
     const char* aggc[64];
@@ -790,6 +806,7 @@ static int build_info(query* q, const plan* join, const plan* build_side, joinin
             }
         }
     }
+    if (ji->mht) ji->rowid = 1;
     for (int pass = 0; pass < 2 && !ji->payload && !ji->rowid; ++pass) {
         const char** lst = pass == 0 ? aggc : fltc;
         int n = pass == 0 ? nagg : nflt;
@@ -1237,9 +1254,46 @@ static int build_filter(const otable* dim, int64_t row, const expr* e, orc_db* d
     return int_pred(op, c->data[row], lit_int32(lit));
 }
 
+/* the build rows an MHT stores: build filter, key validity, key >= minimum */
+static int mht_slot(const joininfo* ji, int64_t r, orc_db* db, int64_t* slot) {
+    int flag = ji->build_pred ? build_filter(ji->dim, r, ji->build_pred, db) : 1;
+    if (ji->pk->validity) flag = flag && ((ji->pk->validity[r >> 6] >> (r & 63)) & 1ULL);
+    if (!flag) return 0;
+    int32_t x = (int32_t)((uint32_t)ji->pk->data[r] - (uint32_t)ji->kmin);
+    if (x < 0) return 0;
+    *slot = x % ji->len;
+    return 1;
+}
+
+/* two passes as BlockBuildMHT_Count / prefix sum / BlockBuildMHT_Write */
+static int build_mht(query* q, joininfo* ji) {
+    ji->mkey = (int32_t*)calloc((size_t)ji->len, sizeof(int32_t));
+    ji->moff = (int32_t*)calloc((size_t)ji->len, sizeof(int32_t));
+    ji->mcnt = (int32_t*)calloc((size_t)ji->len, sizeof(int32_t));
+    ji->mrow = (int32_t*)calloc((size_t)(ji->dim->rows + 1), sizeof(int32_t));
+    int32_t* pos = (int32_t*)calloc((size_t)ji->len, sizeof(int32_t));
+    if (!ji->mkey || !ji->moff || !ji->mcnt || !ji->mrow || !pos) { free(pos); FAIL("out of memory for hash table"); }
+    int64_t s;
+    for (int64_t r = 0; r < ji->dim->rows; ++r)
+        if (mht_slot(ji, r, q->db, &s)) ++ji->mcnt[s];
+    int64_t run = 0;
+    for (int64_t k = 0; k < ji->len; ++k) { ji->moff[k] = (int32_t)run; run += ji->mcnt[k]; }
+    for (int64_t r = 0; r < ji->dim->rows; ++r) {
+        if (!mht_slot(ji, r, q->db, &s)) continue;
+        ji->mrow[ji->moff[s] + pos[s]++] = (int32_t)r;
+        ji->mkey[s] = ji->pk->data[r];
+    }
+    free(pos);
+    return 0;
+}
+
 static int build_tables(query* q) {
     for (int j = 0; j < q->njoins; ++j) {
         joininfo* ji = &q->joins[j];
+        if (ji->mht) {
+            if (build_mht(q, ji)) return -1;
+            continue;
+        }
         ji->ht = (int32_t*)calloc((size_t)(2 * ji->len), sizeof(int32_t));
         if (!ji->ht) FAIL("out of memory for hash table");
         for (int64_t r = 0; r < ji->dim->rows; ++r) {
@@ -1734,6 +1788,22 @@ static void run_ops(const query* q, rowctx* rc, int i, aggstate* st) {
             continue;
         }
         const joininfo* ji = &q->joins[op->join];
+        if (ji->mht) {
+            /* ProbeMultiHT, then the rest of the pipeline once per build row
+             * of the slot (visitor.cpp:4671-4730) */
+            const ocol* fk = ji->fks[0];
+            if (!key_valid(fk, rc->row)) return;
+            const int32_t v = fk->data[rc->row];
+            const int32_t x = (int32_t)((uint32_t)v - (uint32_t)ji->kmin);
+            if (x < 0) return;
+            const int64_t s = x % ji->len;
+            if (ji->mkey[s] != v || ji->mcnt[s] <= 0) return;
+            for (int32_t m = 0; m < ji->mcnt[s]; ++m) {
+                rc->payload[op->join] = ji->mrow[ji->moff[s] + m];
+                run_ops(q, rc, i + 1, st);
+            }
+            return;
+        }
         if (ji->nfk == 1) {
             if (!probe(rc, ji, op->join, 0)) return;
             continue;
@@ -1825,7 +1895,13 @@ int orc_execute(orc_db* db, const char* plan_sexpr, int threads, orc_result* out
     rc = 0;
 done:
     if (rc && err && errlen) snprintf(err, errlen, "%s", g_err[0] ? g_err : "oracle failure");
-    for (int j = 0; j < q->njoins; ++j) free(q->joins[j].ht);
+    for (int j = 0; j < q->njoins; ++j) {
+        free(q->joins[j].ht);
+        free(q->joins[j].mkey);
+        free(q->joins[j].moff);
+        free(q->joins[j].mcnt);
+        free(q->joins[j].mrow);
+    }
     state_free(&st);
     free(q);
     parser_free(&p);

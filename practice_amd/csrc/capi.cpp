@@ -364,7 +364,7 @@ int cq_db_describe(cq_db* d, const char* sql, char* buf, size_t len, size_t* nee
             const auto& jp = p.joins[j];
             o << (j ? "," : "") << "{\"dim\":\"" << jp.dim_table << "\",\"pk\":\"" << jp.pk_col << "\",\"fk\":\"" << jp.fk_col
               << "\",\"payload\":\"" << jp.payload_col << "\",\"kmin\":" << jp.kmin << ",\"len\":" << jp.len
-              << ",\"layout\":\"" << (jp.dense ? "key-indexed" : (jp.compact ? "compact" : "reference"))
+              << ",\"layout\":\"" << (jp.mht ? "multi-value" : jp.dense ? "key-indexed" : (jp.compact ? "compact" : "reference"))
               << "\",\"compact\":" << (jp.compact ? 1 : 0) << ",\"entry_bits\":" << (jp.dense ? jp.db : jp.eb)
               << ",\"tag_bits\":" << (jp.dense ? 0 : jp.tb) << ",\"absent_code\":" << (jp.dense ? jp.doff : 1)
               << ",\"calendar\":" << (jp.dense ? jp.cal : 0) << ",\"lds\":" << (jp.lds ? 1 : 0) << ",\"bytes\":" << jp.ht_words * 4

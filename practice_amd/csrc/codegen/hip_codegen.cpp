@@ -21,6 +21,7 @@
 // merge per block.
 #include "hip_codegen.h"
 #include "vm_expr.h"
+#include "../kernels/aot_kernels.h"
 
 #include <algorithm>
 #include <cstring>
@@ -516,8 +517,33 @@ private:
                 if (&other != &jp && other.fk_alts.size() > 1) unsupported("more than one OR-key join in a plan");
             if (jp.fk_alts.size() > 8) unsupported("more than 8 key alternatives in one join");
         }
-        if (!uniqueKey(t, jp.pk_col))
-            unsupported("non-unique build key " + jp.pk_col + " needs a multi-value hash table (not supported yet)");
+        if (!uniqueKey(t, jp.pk_col)) {
+            // MHT when the catalog proves the key non-unique (visitor.cpp:
+            // 3946-3961); it always carries the build row id (3963-3970)
+            const auto* ks = stats(jp.pk_col);
+            if (!ks || ks->cardinality_ == 0 || ks->cardinality_ >= m.getSize())
+                unsupported("build key " + jp.pk_col + " is neither unique nor known to be non-unique in the catalog");
+            if (opt_.vm) unsupported("interpreted tier: multi-value joins run on compiled modules");
+            if (opt_.batch_body) unsupported("multi-value joins do not run in the shared scan");
+            if (jp.fk_alts.size() > 1) unsupported("OR-key join on a non-unique build key");
+            for (const auto& other : prog_.joins)
+                if (&other != &jp && (other.mht || other.fk_alts.size() > 1))
+                    unsupported("a multi-value join with another multi-value or OR-key join in one plan");
+            if (jp.len > (int64_t(1) << 30) || jp.dim_rows > (int64_t(1) << 31) - 1)
+                unsupported("multi-value table too large");
+            jp.mht = true;
+            jp.rowid = true;
+            jp.pair = true;
+            const uint64_t len = static_cast<uint64_t>(jp.len);
+            const uint64_t sums = 2 * static_cast<uint64_t>(cq_scan_block_count(jp.len) + 2);
+            jp.mht_cnt = 4 * len;
+            jp.mht_off = 6 * len;
+            jp.mht_sums = 8 * len;
+            jp.mht_pos = 8 * len + sums;
+            jp.mht_rows = 9 * len + sums;
+            jp.ht_words = jp.mht_rows + static_cast<uint64_t>(jp.dim_rows);
+            return;
+        }
         // a projection above that reads this dimension gathers its columns
         // by row id (projectionRequiresRowIdPayload, visitor.cpp:3300-3317)
         for (const OperatorNode* p = join->parent_; p; p = p->parent_) {
@@ -1139,11 +1165,45 @@ private:
         return out;
     }
 
+    // Multi-value table: cq_build_<j> counts the build rows per slot
+    // (BlockBuildMHT_Count), the runtime scans the counts into first
+    // positions, cq_mhtw_<j> places each row id and writes the slot's
+    // directory entry (BlockBuildMHT_Write, join.h:301-364).  Build filter
+    // and key validity as for the other tables (visitor.cpp:4295-4464).
+    void emitMhtBuild(std::ostringstream& src, int j, const std::string& cond) {
+        const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+        auto head = [&](const std::string& name) {
+            src << "extern \"C\" __global__ __launch_bounds__(256) void " << name << "(" << paramDecl() << ") {\n"
+                << paramAlias() << "    using namespace chip;\n"
+                << "    const long long n = P.dim_rows[" << j << "];\n"
+                << "    u32* const T = P.ht[" << j << "];\n"
+                << "    for (long long r = (long long)blockIdx.x * 256 + threadIdx.x; r < n; r += (long long)gridDim.x * 256) {\n";
+            if (!cond.empty()) src << "        if (!" << cond << ") continue;\n";
+            if (nullable(jp.pk_col)) src << "        if (!chip::bit32(P.dvalid[" << jp.pk_dslot << "], r)) continue;\n";
+            src << "        const int k = P.dcol[" << jp.pk_dslot << "][r];\n"
+                << "        long long s;\n"
+                << "        if (!chip::mht_slot(k, " << jp.kmin << ", " << jp.len << "LL, s)) continue;\n";
+        };
+        head("cq_build_" + std::to_string(j));
+        src << "        atomicAdd((u64*)(T + " << jp.mht_cnt << ") + s, 1ull);\n    }\n}\n\n";
+        head("cq_mhtw_" + std::to_string(j));
+        src << "        const u32 first = (u32)((const u64*)(T + " << jp.mht_off << "))[s];\n"
+            << "        const u32 count = (u32)((const u64*)(T + " << jp.mht_cnt << "))[s];\n"
+            << "        const u32 at = atomicAdd(T + " << jp.mht_pos << " + s, 1u);\n"
+            << "        T[" << jp.mht_rows << " + first + at] = (u32)r;\n"
+            << "        v4u d; d.x = (u32)k; d.y = first; d.z = count; d.w = 0u;\n"
+            << "        *(v4u*)(T + 4 * s) = d;\n    }\n}\n\n";
+    }
+
     void emitBuild(std::ostringstream& src, int j) {
         JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
         const ExprNode* bp = build_pred_[static_cast<size_t>(j)];
         std::string cond = bp ? buildCond(bp) : "";
         jp.pk_dslot = dimSlot(jp.pk_col);
+        if (jp.mht) {
+            emitMhtBuild(src, j, cond);
+            return;
+        }
         if (jp.pair && !jp.rowid) jp.payload_dslot = dimSlot(jp.payload_col);
         src << "extern \"C\" __global__ __launch_bounds__(256) void cq_build_" << j << "(" << paramDecl() << ") {\n"
             << paramAlias()
@@ -1235,18 +1295,22 @@ private:
         // ---- pipeline body first: it decides which fact columns are loaded
         std::ostringstream body, orbody;
         visible_mask_ = 0;
-        // an OR-key join runs last, with the filters that read its payload
-        int multi = -1;
-        for (size_t j = 0; j < prog_.joins.size(); ++j)
+        // an OR-key or multi-value join runs last, with the filters that
+        // read its payload
+        int multi = -1, mhtj = -1;
+        for (size_t j = 0; j < prog_.joins.size(); ++j) {
             if (prog_.joins[j].fk_alts.size() > 1) multi = static_cast<int>(j);
+            if (prog_.joins[j].mht) mhtj = static_cast<int>(j);
+        }
+        const int last = multi >= 0 ? multi : mhtj;
         std::vector<Op> dependent;
         for (const Op& op : pipelineOrder()) {
-            if (multi >= 0 && op.kind == 1 && op.join == multi) continue;
-            if (multi >= 0 && op.kind == 0) {
+            if (last >= 0 && op.kind == 1 && op.join == last) continue;
+            if (last >= 0 && op.kind == 0) {
                 std::vector<std::string> cols;
                 collectColumns(op.pred, cols);
                 bool dep = false;
-                for (const auto& c : cols) dep = dep || tableOf(c) == prog_.joins[static_cast<size_t>(multi)].dim_table;
+                for (const auto& c : cols) dep = dep || tableOf(c) == prog_.joins[static_cast<size_t>(last)].dim_table;
                 if (dep) {
                     dependent.push_back(op);
                     continue;
@@ -1291,7 +1355,41 @@ private:
             emitProbe(orbody, multi, "kor[i]", "");
             for (const Op& op : dependent) emitFilterOp(orbody, op);
         }
+        // multi-value join: directory lookup per item (mlocate), then per
+        // match index m the build row ids and the filters that read them
+        // (mbody, inside `for (m ...)`, where `continue` moves to the next m)
+        std::ostringstream mlocate, mbody;
+        if (mhtj >= 0) {
+            const JoinProgram& jp = prog_.joins[static_cast<size_t>(mhtj)];
+            const std::string J = std::to_string(mhtj);
+            const int fs = factSlot(jp.fk_col);
+            std::string live = flGet();
+            if (prog_.fact_nullable[static_cast<size_t>(fs)]) live = "(" + live + " && ((v" + std::to_string(fs) + " >> i) & 1u))";
+            mlocate << "        // " << jp.dim_table << " joined on non-unique " << jp.pk_col
+                    << ": the rest of the pipeline once per matching build row\n"
+                    << "        u32 mn[IPT], mf[IPT], mmax = 0u;\n"
+                    << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                    << "            mn[i] = 0u; mf[i] = 0u;\n"
+                    << "            if (" << live << ") mn[i] = chip::mht_find(P.ht[" << J << "], c" << fs << "[i], " << jp.kmin << ", "
+                    << jp.len << "LL, mf[i]);\n"
+                    << "            mmax = mn[i] > mmax ? mn[i] : mmax;\n"
+                    << "        }\n";
+            mbody << "            fl = 0u;\n"
+                  << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) if (m < mn[i]) fl |= 1u << i;\n"
+                  << "            int p" << J << "[IPT];\n"
+                  << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) p" << J << "[i] = ((fl >> i) & 1u) ? (int)P.ht[" << J
+                  << "][" << jp.mht_rows << " + mf[i] + m] : 0;\n";
+            visible_mask_ |= 1u << mhtj;
+            for (const Op& op : dependent) emitFilterOp(mbody, op);
+        }
         if (prog_.projection) {
+            if (mhtj >= 0) {
+                prog_.rows_unordered = true;
+                emitProjectionExpanded(src, body.str(), mlocate.str(), mbody.str(), bt, vpt);
+                prog_.source = src.str();
+                setAlgorithmicBytes();
+                return;
+            }
             emitProjection(src, body.str(), bt, vpt, ipt);
             prog_.source = src.str();
             setAlgorithmicBytes();
@@ -1374,6 +1472,11 @@ private:
                 << "        for (int alt = 0; alt < " << prog_.joins[static_cast<size_t>(multi)].fk_alts.size() << "; ++alt) {\n"
                 << "        fl = fl_or;\n"
                 << orbody.str() << aggcode.str() << "        }\n";
+        } else if (mhtj >= 0) {
+            src << mlocate.str()
+                << "#pragma unroll 1\n"
+                << "        for (u32 m = 0u; __ballot(m < mmax) != 0ull; ++m) {\n"
+                << mbody.str() << aggcode.str() << "        }\n";
         } else {
             src << aggcode.str();
         }
@@ -1462,6 +1565,87 @@ private:
             else src << "                if (" << ok << ") " << set << "\n";
         }
         src << "            }\n        }\n    }\n}\n\n";
+    }
+
+    // Projection through a multi-value join: as emitProjection, with each
+    // live row expanded into one output row per matching build row.  The
+    // count pass stores each wave's surviving (row, match) pairs per (tile,
+    // vector); the write pass counts them again per item, ranks them in the
+    // wave (exclusive lane scan) and writes them in (lane, item, match) order.
+    void emitProjectionExpanded(std::ostringstream& src, const std::string& body, const std::string& mlocate,
+                                const std::string& mbody, int bt, int vpt) {
+        const int waves = bt / 64;
+        prog_.proj_waves = waves;
+        prog_.block_threads = bt;
+        prog_.vectors_per_thread = vpt;
+        const int ts = prog_.visible_ts;
+        std::vector<std::string> vals, oks;
+        for (int c = 0; c < ts; ++c) {
+            vals.push_back(vexpr(proj_exprs_[static_cast<size_t>(c)]));
+            oks.push_back(vvalid(proj_exprs_[static_cast<size_t>(c)]));
+        }
+        src << "#define BT " << bt << "\n#define VPT " << vpt << "\n#define IPT " << 4 * vpt << "\n#define TILE (BT * IPT)\n#define WAVES "
+            << waves << "\n\n";
+        const std::string loop = "#pragma unroll 1\n        for (u32 m = 0u; __ballot(m < mmax) != 0ull; ++m) {\n";
+        auto head = [&](const char* name) {
+            src << "extern \"C\" __global__ __launch_bounds__(BT) void " << name << "(const cq_query_params P) {\n"
+                << "    using namespace chip;\n"
+                << "    const int tid = threadIdx.x;\n"
+                << "    const int wave = tid >> 6, lane = tid & 63;\n";
+            for (const auto& jp : prog_.joins) {
+                if (!jp.lds) continue;
+                const int j = static_cast<int>(&jp - prog_.joins.data());
+                src << "    __shared__ u32 s_ht" << j << "[" << jp.ht_words << "];\n"
+                    << "    for (int k = tid; k < " << jp.ht_words << "; k += BT) s_ht" << j << "[k] = P.ht[" << j << "][k];\n";
+            }
+            src << "    __syncthreads();\n"
+                << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n"
+                << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
+            for (size_t c = 0; c < prog_.fact_cols.size(); ++c) {
+                src << "        int c" << c << "[IPT];\n";
+                if (prog_.fact_nullable[c]) src << "        u32 v" << c << " = 0u;\n";
+            }
+            src << "        u32 fl = 0u;\n";
+            emitTileLoad(src, "        ", "", "base", "fl");
+            src << body << mlocate
+                << "        u32 sc[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) sc[i] = 0u;\n"
+                << loop << mbody
+                << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) sc[i] += (fl >> i) & 1u;\n"
+                << "        }\n";
+        };
+        head("cq_pcount");
+        src << "#pragma unroll\n        for (int v = 0; v < VPT; ++v) {\n"
+            << "            const u64 n = wave_sum((u64)(sc[4 * v] + sc[4 * v + 1] + sc[4 * v + 2] + sc[4 * v + 3]));\n"
+            << "            if (lane == 0) P.tile_count[(tile * VPT + v) * WAVES + wave] = n;\n"
+            << "        }\n    }\n    (void)lane;\n}\n\n";
+        head("cq_pwrite");
+        src << "        u64 at[IPT];\n"
+            << "#pragma unroll\n        for (int v = 0; v < VPT; ++v) {\n"
+            << "            u64 run = P.tile_off[(tile * VPT + v) * WAVES + wave] +\n"
+            << "                      wave_exclusive(sc[4 * v] + sc[4 * v + 1] + sc[4 * v + 2] + sc[4 * v + 3]);\n"
+            << "#pragma unroll\n            for (int k = 0; k < 4; ++k) { at[4 * v + k] = run; run += sc[4 * v + k]; }\n"
+            << "        }\n"
+            << loop << mbody
+            << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+            << "                if (!((fl >> i) & 1u)) continue;\n"
+            << "                const u64 row = at[i]++;\n";
+        for (int c = 0; c < ts; ++c) {
+            const std::string& v = vals[static_cast<size_t>(c)];
+            std::string cast;
+            switch (prog_.out[static_cast<size_t>(c)].type) {
+                case LogicalType::Float64: cast = "(u64)__double_as_longlong((double)(" + v + "))"; break;
+                case LogicalType::Int64: cast = "(u64)(long long)(" + v + ")"; break;
+                default: cast = "(u64)(" + v + ")"; break;
+            }
+            src << "                P.result[row * " << ts << " + " << c << "] = " << cast << ";\n";
+            const std::string& ok = oks[static_cast<size_t>(c)];
+            if (litFalse(ok)) continue;
+            const std::string set = "{ const u64 b = row * " + std::to_string(ts) + " + " + std::to_string(c) +
+                                    "; atomicOr(&P.result_valid[b >> 6], 1ull << (b & 63)); }";
+            if (litTrue(ok)) src << "                " << set << "\n";
+            else src << "                if (" << ok << ") " << set << "\n";
+        }
+        src << "            }\n        }\n    }\n    (void)lane;\n}\n\n";
     }
 
     void emitFilterOp(std::ostringstream& body, const Op& op) {

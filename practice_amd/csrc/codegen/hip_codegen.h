@@ -89,6 +89,13 @@ struct JoinProgram {
     // kernel points at LDS or at the table in HBM)
     bool lds = false;
     uint64_t ht_words = 0;    // u32 words of the table
+    // multi-value table of a non-unique build key (MHT, visitor.cpp:3946-3970,
+    // 4295-4464): one 16-byte directory entry {key, first, count, 0} per slot
+    // (key - kmin) % len, u64 counts and offsets, the prefix scan's block
+    // sums, u32 write cursors and the build row ids grouped by slot; the
+    // payload is always the build row id.  Word offsets in the table:
+    bool mht = false;
+    uint64_t mht_cnt = 0, mht_off = 0, mht_sums = 0, mht_pos = 0, mht_rows = 0;
 };
 
 struct OutputColumn {
@@ -103,6 +110,9 @@ struct QueryProgram {
     // output columns are the visible slots of `out`
     bool projection = false;
     int proj_waves = 0;
+    // rows of a projection through a multi-value join come out grouped by
+    // scan position, in table order within one key (unspecified order)
+    bool rows_unordered = false;
     std::string source;
     std::string signature;    // hex hash of `source`
     std::string plan_sexpr;

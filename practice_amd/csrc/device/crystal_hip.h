@@ -29,6 +29,7 @@ typedef unsigned long long u64;
 typedef long long i64;
 typedef unsigned int u32;
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ arithmetic
 // Integer division with defined behaviour: x / 0 == 0, INT_MIN / -1 wraps.
@@ -87,6 +88,27 @@ __device__ __forceinline__ bool any_of(const bool (&f)[N]) {
     return a;
 }
 
+// ------------------------------------------------------------------ multi-value tables
+// Slot of a build or probe key in a multi-value table: (key - kmin) % len in
+// int32 arithmetic (HASH, join.h:3); a key below the minimum has no slot.
+__device__ __forceinline__ bool mht_slot(int key, int kmin, long long len, long long& slot) {
+    const int x = (int)((u32)key - (u32)kmin);
+    if (x < 0) return false;
+    slot = (long long)x % len;
+    return true;
+}
+// ProbeMultiHT (join.h:278-299): the directory entry of the key's slot
+// matches when it holds that key and a non-zero count; returns the count and
+// the first payload index.  An empty entry ({0, 0, 0, 0}) never matches.
+__device__ __forceinline__ u32 mht_find(const u32* __restrict__ ht, int key, int kmin, long long len, u32& first) {
+    long long s;
+    if (!mht_slot(key, kmin, len, s)) return 0u;
+    const v4u d = *(const v4u*)(ht + 4 * s);
+    if ((int)d.x != key || d.z == 0u) return 0u;
+    first = d.y;
+    return d.z;
+}
+
 // ------------------------------------------------------------------ wave reductions
 __device__ __forceinline__ u64 wave_sum(u64 v) {
 #pragma unroll
@@ -108,6 +130,18 @@ __device__ __forceinline__ u64 wave_max(u64 v) {
         v = t > v ? t : v;
     }
     return v;
+}
+
+// exclusive prefix sum over the 64 lanes of a wave
+__device__ __forceinline__ u32 wave_exclusive(u32 v) {
+    const int lane = threadIdx.x & 63;
+    u32 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 t = __shfl_up(x, o);
+        if (lane >= o) x += t;
+    }
+    return x - v;
 }
 
 }  // namespace chip

@@ -6,7 +6,7 @@
 enum {
     CQ_GEN_KEY = 0, CQ_GEN_UNIFORM, CQ_GEN_DATE_FK, CQ_GEN_PCAT, CQ_GEN_PMFGR, CQ_GEN_PBRAND, CQ_GEN_NATION,
     CQ_GEN_REGION, CQ_GEN_CITY, CQ_GEN_DKEY, CQ_GEN_DYEAR, CQ_GEN_DMONTH, CQ_GEN_DYM, CQ_GEN_DDOW, CQ_GEN_DDOM,
-    CQ_GEN_DDOY, CQ_GEN_DWEEK
+    CQ_GEN_DDOY, CQ_GEN_DWEEK, CQ_GEN_ORDERKEY, CQ_GEN_LINENUMBER
 };
 
 // One generated column: value kind + stream seeds (see practice_amd/csrc/runtime/generator.cpp)

@@ -71,6 +71,8 @@ __device__ int gen_cell(const cq_gen_recipe& rc, i64 r) {
         case CQ_GEN_NATION: return (int)uni(rc.s0, r, 0, 24);
         case CQ_GEN_REGION: return (int)(uni(rc.s0, r, 0, 24) / 5);
         case CQ_GEN_CITY: return (int)(uni(rc.s0, r, 0, 24) * 10 + uni(rc.s1, r, 0, 9));
+        case CQ_GEN_ORDERKEY: return (int)(r / 4 + 1);
+        case CQ_GEN_LINENUMBER: return (int)(r % 4 + 1);
         default: break;
     }
     civil(r, y, m, dom, doy);

@@ -97,6 +97,8 @@ cq_gen_recipe recipeFor(const GenSpec& sp, const std::string& t, const std::stri
         else if (c == "lo_revenue") uniform(100000, 10000000);
         else if (c == "lo_supplycost") uniform(60000, 600000);
         else if (c == "lo_tax") uniform(0, 8);
+        else if (c == "lo_orderkey") r.kind = CQ_GEN_ORDERKEY;  // orders of 4 lines
+        else if (c == "lo_linenumber") r.kind = CQ_GEN_LINENUMBER;
         else throw std::runtime_error("unknown generated column " + c);
     } else if (t == "PART") {
         r.s0 = S("p_category");
@@ -150,7 +152,7 @@ std::vector<std::string> genTables() { return {"LINEORDER", "PART", "SUPPLIER", 
 std::vector<std::string> genTableColumns(const std::string& table) {
     if (table == "LINEORDER")
         return {"lo_custkey", "lo_partkey", "lo_suppkey", "lo_orderdate", "lo_commitdate", "lo_quantity",
-                "lo_extendedprice", "lo_discount", "lo_revenue", "lo_supplycost", "lo_tax"};
+                "lo_extendedprice", "lo_discount", "lo_revenue", "lo_supplycost", "lo_tax", "lo_orderkey", "lo_linenumber"};
     if (table == "PART") return {"p_partkey", "p_mfgr", "p_category", "p_brand1", "p_size"};
     if (table == "SUPPLIER") return {"s_suppkey", "s_city", "s_nation", "s_region"};
     if (table == "CUSTOMER") return {"c_custkey", "c_city", "c_nation", "c_region", "c_mktsegment"};

@@ -329,12 +329,14 @@ void HipModuleExecutor::bind(const ModuleHandle& module, const QueryProgram& p, 
     auto take64 = [&](const std::string& name, size_t bytes) { return static_cast<unsigned long long*>(take(name, bytes)); };
 
     run.builds.clear();
+    run.mht_writes.clear();
     for (size_t j = 0; j < p.joins.size(); ++j) {
         // shared (template) tables are owned and built by the batch
         P.ht[j] = p.shared_tables ? nullptr
                                   : static_cast<unsigned int*>(take("ht" + std::to_string(j), std::max<size_t>(p.joins[j].ht_words * 4, 16)));
         P.dim_rows[j] = p.joins[j].dim_rows;
         if (!p.vm) run.builds.push_back(module->function("cq_build_" + std::to_string(j)));
+        run.mht_writes.push_back(p.joins[j].mht ? module->function("cq_mhtw_" + std::to_string(j)) : nullptr);
     }
     run.dense_count = (p.fast_sparse || p.vm || p.projection) ? nullptr : module->function("cq_dense_count");
     run.keys = p.keys_kernel ? module->function("cq_keys") : nullptr;
@@ -412,12 +414,23 @@ void HipModuleExecutor::setup(QueryRun& run, ExecutionContext* ctx, const cq_que
 void HipModuleExecutor::buildTables(QueryRun& run, ExecutionContext* ctx, const cq_query_params* device_params) {
     const QueryProgram& p = *run.program;
     for (size_t j = 0; j < p.joins.size(); ++j) {
-        const long long rows = p.joins[j].dim_rows;
+        const JoinProgram& jp = p.joins[j];
+        const long long rows = jp.dim_rows;
         const unsigned grid = static_cast<unsigned>(std::min<long long>(std::max<long long>((rows + 255) / 256, 1), 8192));
         const cq_query_params* dp = device_params;
         void* by_value[] = {&run.P};
         void* by_pointer[] = {&dp};
         launch(run.builds[j], grid, 256, ctx->stream, device_params ? by_pointer : by_value, "launch build");
+        if (!jp.mht) continue;
+        // multi-value table: first positions = exclusive scan of the slot
+        // counts, then the row ids and directory entries
+        auto* T = reinterpret_cast<unsigned long long*>(run.P.ht[j]);
+        unsigned long long* cnt = T + jp.mht_cnt / 2;
+        unsigned long long* off = T + jp.mht_off / 2;
+        unsigned long long* sums = T + jp.mht_sums / 2;
+        const long long nb = cq_scan_block_count(jp.len);
+        hipCheck(cq_launch_exclusive_scan(cnt, off, jp.len, sums, sums + nb + 1, ctx->stream), "scan multi-value counts");
+        launch(run.mht_writes[j], grid, 256, ctx->stream, device_params ? by_pointer : by_value, "launch multi-value write");
     }
 }
 
@@ -661,7 +674,7 @@ void HipModuleExecutor::fetchFinish(QueryRun& run) {
     for (const auto& oc : p.out) out.columns.push_back(ResultColumnDesc{oc.type, oc.nullable, oc.name});
     out.data.assign(static_cast<size_t>(vts), {});
     out.validity.assign(static_cast<size_t>(vts), {});
-    out.ordered = !p.fast_sparse || run.single_cell;
+    out.ordered = (!p.fast_sparse || run.single_cell) && !(p.rows_unordered && !(p.has_sort_node && !p.sort_keys.empty()));
     out.dense_result = !p.fast_sparse;
     const char* h = run.fetch_host;
     if (run.single_cell) {

@@ -261,6 +261,7 @@ struct QueryRun {
     std::string prefix;
     cq_query_params P;
     std::vector<hipFunction_t> builds;
+    std::vector<hipFunction_t> mht_writes;  // per join: cq_mhtw_<j> of a multi-value table, else null
     hipFunction_t dense_count = nullptr;
     hipFunction_t keys = nullptr;       // cq_keys (group keys decoded after the scan)
     cq_vm_program* vm = nullptr;        // interpreted tier: the program in device memory

@@ -171,7 +171,7 @@ def test_projection_queries_compile(small_db):
     assert all(j["layout"] in ("key-indexed", "compact") for j in d["joins"])
     assert [o["nullable"] for o in d["outputs"]] == [1, 0, 1, 0]
     star = small_db.describe("select * from lineorder, ddate where lo_orderdate = d_datekey")
-    assert len(star["outputs"]) == 11 + 8 and star["joins"][0]["dim"] == "DDATE"
+    assert len(star["outputs"]) == 13 + 8 and star["joins"][0]["dim"] == "DDATE"
     with pytest.raises(pa.CrystalError, match="shared scan"):
         small_db.generate_batch_code([PROJECTIONS_SSB[0]])
 
