@@ -140,6 +140,19 @@ long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misse
 int cq_db_clear_cache(cq_db* db);
 
 int cq_db_execute(cq_db* db, const char* sql, cq_result** out);
+/* cq_db_execute for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No reference counterpart.  Calls no hipMalloc, hipFree, hipHostMalloc or
+ * other device-allocating routine (the cq_result it returns is host memory,
+ * as cq_db_execute's): every scratch buffer of the query -- hash tables,
+ * aggregate slots, materialization and sort buffers -- comes from one static
+ * device array of 32 MiB per GPU, held until the process ends and shared by
+ * every call on that GPU (calls that overlap on one GPU race; one database
+ * serializes its own calls).  Aggregate queries only; the plan's module
+ * must be compiled beforehand with cq_db_compile (which also loads it).
+ * CQ_ERR_MEMORY when the query needs more scratch than the array holds,
+ * CQ_ERR_INVALID for a projection or a plan not compiled yet. */
+int cq_db_execute_noalloc(cq_db* db, const char* sql, cq_result** out);
 /* scan only fact rows [row_begin, row_end) */
 int cq_db_execute_range(cq_db* db, const char* sql, long long row_begin, long long row_end, cq_result** out);
 /* Copies a NUL-terminated string into buf (truncating); *needed gets the full size. */

@@ -67,6 +67,7 @@ def _load() -> ctypes.CDLL:
         "cq_db_clear_cache": (c_int, [c_void_p]),
         "cq_db_execute": (c_int, [c_void_p, c_char_p, P(c_void_p)]),
         "cq_db_execute_range": (c_int, [c_void_p, c_char_p, c_ll, c_ll, P(c_void_p)]),
+        "cq_db_execute_noalloc": (c_int, [c_void_p, c_char_p, P(c_void_p)]),
         "cq_db_explain": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
         "cq_db_generate_code": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
         "cq_db_describe": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
@@ -303,6 +304,13 @@ class Database:
     def execute(self, sql: str, row_begin: int = 0, row_end: int = -1) -> Result:
         h = ctypes.c_void_p()
         _check(_lib.cq_db_execute_range(self._h, sql.encode(), row_begin, row_end, ctypes.byref(h)))
+        return _fetch_result(h)
+
+    def execute_noalloc(self, sql: str) -> Result:
+        """execute() with every device scratch buffer taken from the library's
+        static per-GPU array (no device allocation); compile(sql) first."""
+        h = ctypes.c_void_p()
+        _check(_lib.cq_db_execute_noalloc(self._h, sql.encode(), ctypes.byref(h)))
         return _fetch_result(h)
 
     def explain(self, sql: str) -> str:

@@ -11,6 +11,7 @@
 #include "runtime/execution.h"
 #include "serve/batch.h"
 #include "serve/server.h"
+#include "kernels/noalloc.h"
 
 struct cq_module {
     db::ModuleHandle handle;
@@ -119,6 +120,8 @@ int cq_db_create(int device, cq_db** out) {
         if (!out) return fail(CQ_ERR_INVALID, "cq_db_create: null out");
         auto d = std::make_unique<cq_db>();
         d->inst = std::make_unique<db::DatabaseInstance>(device);
+        // the allocation-free execute's scratch address on this GPU, once
+        if (device >= 0) (void)db::noAllocScratch(device);
         *out = d.release();
         return CQ_OK;
     });
@@ -333,6 +336,24 @@ int cq_db_execute_range(cq_db* d, const char* sql, long long b, long long e, cq_
         if (!d || !sql || !out) return fail(CQ_ERR_INVALID, "cq_db_execute: null argument");
         auto r = std::make_unique<cq_result>();
         r->r = d->inst->executeQueryRange(sql, b, e);
+        *out = r.release();
+        return CQ_OK;
+    });
+}
+
+int cq_db_execute_noalloc(cq_db* d, const char* sql, cq_result** out) {
+    return guarded([&] {
+        if (!d || !sql || !out) return fail(CQ_ERR_INVALID, "cq_db_execute_noalloc: null argument");
+        if (d->inst->hostOnly()) return fail(CQ_ERR_INVALID, "cq_db_execute_noalloc: host-only database");
+        void* scratch = db::noAllocScratch(d->inst->device());
+        auto r = std::make_unique<cq_result>();
+        try {
+            r->r = d->inst->executeQueryInto(sql, scratch, db::noAllocScratchBytes());
+        } catch (const std::length_error& e) {
+            return fail(CQ_ERR_MEMORY, e.what());
+        } catch (const std::invalid_argument& e) {
+            return fail(CQ_ERR_INVALID, e.what());
+        }
         *out = r.release();
         return CQ_OK;
     });

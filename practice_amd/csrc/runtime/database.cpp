@@ -541,7 +541,23 @@ std::string DatabaseInstance::generateQueryCode(const std::string& sql) const { 
 
 ModuleHandle DatabaseInstance::compile(const std::string& sql) {
     QueryProgram p = prepare(sql);
-    return engine_->moduleFor(p, nullptr);
+    ModuleHandle m = engine_->moduleFor(p, nullptr);
+    if (device_ >= 0) {
+        hipCheck(hipSetDevice(device_), "hipSetDevice");
+        m->load();  // loaded now, so later executions load nothing
+    }
+    return m;
+}
+
+QueryResult DatabaseInstance::executeQueryInto(const std::string& sql, void* scratch, size_t bytes) {
+    if (device_ < 0) throw std::runtime_error("executeQuery needs a GPU device (host-only database)");
+    std::lock_guard<std::mutex> lock(mu_);
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    ctx_->resetForQuery();
+    ctx_->program = prepare(sql);
+    ModuleHandle m = ctx_->program.vm ? nullptr : engine_->cachedModule(ctx_->program);
+    executor_->executeInto(m, ctx_.get(), scratch, bytes);
+    return ctx_->result;
 }
 
 void DatabaseInstance::fetchColumn(const std::string& column, int32_t* out, uint64_t* validity_out) const {

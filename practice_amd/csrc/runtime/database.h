@@ -71,6 +71,10 @@ public:
     QueryResult executeQuery(const std::string& sql);
     // Scan only fact rows [row_begin, row_end) (serving / batching chunks).
     QueryResult executeQueryRange(const std::string& sql, int64_t row_begin, int64_t row_end);
+    // executeQuery with no device or pinned allocation: scratch from the
+    // caller's device block (HipModuleExecutor::executeInto); the plan's
+    // module must have been compiled (compile()) beforehand.
+    QueryResult executeQueryInto(const std::string& sql, void* scratch, size_t bytes);
     std::string explain(const std::string& sql) const;           // optimized plan, S-expression
     std::string explainTree(const std::string& sql) const;       // optimized plan, indented
     std::string generateQueryCode(const std::string& sql) const; // HIP source

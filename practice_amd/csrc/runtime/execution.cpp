@@ -100,6 +100,18 @@ CompiledModule::~CompiledModule() {
     (void)hipModuleUnload(module_);
 }
 
+void CompiledModule::load() {
+    std::lock_guard<std::mutex> lock(mu_);
+    if (module_) return;
+    DeviceMapLock dl;
+    hipCheck(hipModuleLoadData(&module_, code_.data()), "hipModuleLoadData");
+}
+
+bool CompiledModule::loaded() {
+    std::lock_guard<std::mutex> lock(mu_);
+    return module_ != nullptr;
+}
+
 hipFunction_t CompiledModule::function(const std::string& name) {
     std::lock_guard<std::mutex> lock(mu_);
     if (!module_) {
@@ -812,6 +824,36 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
     (void)hipEventDestroy(ev1);
     if (ks) (void)hipEventDestroy(ks);
     if (ke) (void)hipEventDestroy(ke);
+}
+
+void HipModuleExecutor::executeInto(const ModuleHandle& module, ExecutionContext* ctx, void* scratch, size_t bytes) {
+    const QueryProgram& p = ctx->program;
+    if (p.projection)
+        throw std::invalid_argument("the allocation-free execute takes aggregate queries only (a projection's result size is known "
+                                    "only after its scan)");
+    if (!p.vm && (!module || !module->loaded()))
+        throw std::invalid_argument("the allocation-free execute needs the query's module compiled and loaded on the device "
+                                    "(cq_db_compile first)");
+    hipStream_t s = ctx->stream;
+    QueryRun run;
+    ScratchArena arena;
+    arena.measure();
+    bind(module, p, ctx, "", run, &arena);
+    if (arena.used() > bytes)
+        throw std::length_error("the allocation-free execute needs " + std::to_string(arena.used()) +
+                                " bytes of scratch, over its static " + std::to_string(bytes) + " bytes");
+    arena.assign(scratch, bytes);
+    bind(module, p, ctx, "", run, &arena);
+    setup(run, ctx, nullptr);
+    scan(run, ctx);
+    shape(run, ctx, nullptr, nullptr);
+    unsigned long long rows = 0;
+    if (!run.single_cell) {
+        hipCheck(hipMemcpyAsync(&rows, run.off + run.cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
+        hipCheck(hipStreamSynchronize(s), "sync");
+    }
+    fetch(run, ctx, rows);
+    ctx->result = std::move(run.result);
 }
 
 // ------------------------------------------------------------------ QueryEngine

@@ -142,6 +142,9 @@ public:
     const std::vector<char>& code() const { return code_; }
     // Loads the module on the current device on first use (thread-safe).
     hipFunction_t function(const std::string& name);
+    // Loads the code object now (hipModuleLoadData) / whether it is loaded.
+    void load();
+    bool loaded();
 
 private:
     std::string signature_;
@@ -329,6 +332,11 @@ public:
                        std::vector<cq_copy_seg>& segs);
     void fetchFinish(QueryRun& run);
     void sortHost(QueryRun& run);
+    // execute() without device or pinned allocation: every scratch buffer
+    // (hash tables, aggregate slots, materialization, sort) is carved out of
+    // `scratch` (`bytes` long), the module must be loaded, no events.
+    // Aggregate plans only (a projection's size is known after its scan).
+    void executeInto(const ModuleHandle& module, ExecutionContext* ctx, void* scratch, size_t bytes);
 };
 
 // Persistent grid for a scan kernel: occupancy x CUs, at most one block per tile.
@@ -356,6 +364,8 @@ public:
     // compile or fetch the module for a program
     ModuleHandle moduleFor(const QueryProgram& program, QueryTimingStats* timing);
     ModuleHandle moduleForSource(const std::string& source, const std::string& signature, QueryTimingStats* timing);
+    // the cached module of a program, or null (never compiles)
+    ModuleHandle cachedModule(const QueryProgram& program) { return cache_->get(program.signature); }
     HipModuleExecutor* hipExecutor() { return dynamic_cast<HipModuleExecutor*>(executor_.get()); }
     void setCatalog(std::shared_ptr<const Catalog> catalog) { catalog_ = std::move(catalog); }
     IQueryCache& cache() { return *cache_; }
