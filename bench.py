@@ -40,6 +40,9 @@ def parse_args():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     p.add_argument("--fact-rows", type=int, default=0, help="override the fact-table size")
+    p.add_argument("--query", default="", help="time this query of tests/queries.py (by name) on the workload's data")
+    p.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                   help="database option (cq_db_set_option) for the timed query, repeatable")
     p.add_argument("--seed", type=int, default=42)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
@@ -286,13 +289,16 @@ def main():
     sf, rows, nullable, key = WORKLOADS[args.workload]
     if args.fact_rows:
         rows = args.fact_rows
-    sql = getattr(Q, key)
+    sql = getattr(Q, args.query or key)
     db = pa.Database(0)
     t0 = time.perf_counter()
     db.generate_ssb(sf=sf, fact_rows=rows, seed=args.seed, nullable=nullable)
     db.finalize()
     gen_s = time.perf_counter() - t0
     db.set_option("kernel_timing", 1)
+    for kv in args.option:
+        k, v = kv.split("=")
+        db.set_option(k, int(v))
     desc = db.describe(sql)
     for _ in range(args.warmup):
         db.execute(sql)
@@ -314,7 +320,7 @@ def main():
     achieved = fact_bytes / (scan_avg / 1000.0) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"{args.workload}_scan_traffic.json")
-    if os.path.exists(tf) and not args.fact_rows:
+    if os.path.exists(tf) and not args.fact_rows and not args.query:
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     line = {
@@ -331,7 +337,8 @@ def main():
         "dtype": "int32 columns, u64 aggregates",
         "data": "synthetic (seeded star schema generated in HBM)",
         "config": {"workload": f"{args.workload.upper()}: scale {sf}, {rows:,} fact rows, "
-                               f"{len(desc['joins'])}-way join, {desc['groups']} group slots",
+                               f"{len(desc['joins'])}-way join, {desc['groups']} group slots"
+                               + (f", query {args.query}" if args.query else ""),
                    "fact_rows": rows, "groups": desc["groups"], "algorithmic_bytes": desc["algorithmic_bytes"],
                    "result_rows": res.rows, "block_threads": desc["block_threads"],
                    "lds_bytes": desc["scan_lds_bytes"], "generate_s": round(gen_s, 2)},

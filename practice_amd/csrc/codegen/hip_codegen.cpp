@@ -649,6 +649,15 @@ private:
                     jp.zmode = k0 == 0 ? 3 : 2;
                     jp.zx = static_cast<int>(k0 - jp.dmin);
                 }
+                // every key of [dmin, dmin + dspan) present: the slot owner
+                // is always in the table, so key 0 matches exactly when it
+                // is a key itself (mode 3, its own entry) and never otherwise
+                // (mode 2, where 0 lies outside the key range): the plain
+                // entry lookup already says so
+                if (jp.doff == 0 && jp.zmode >= 2) {
+                    jp.zmode = 0;
+                    jp.zx = 0;
+                }
             }
             // calendar keys (yyyymmdd dates): index by (year, month, day),
             // 372 entries a year instead of 10,000, when that is smaller.
@@ -1107,6 +1116,13 @@ private:
             else term = "(" + v + " - " + mn + ")";
             h = h.empty() ? term : "(" + h + " * " + std::to_string(gcard_[g]) + " + " + term + ")";
         }
+        // Every term of a dense domain lies in [0, card) (catalog min / max /
+        // distinct count bound every value, the NULL code is 0), so the
+        // Horner value lies in [0, groups) and the reference's `% groups`
+        // is the identity: left out.
+        bool dense = true;
+        for (bool d : gdense_) dense = dense && d;
+        if (dense) return "(int)(" + h + ")";
         return "(int)((" + h + ") % " + std::to_string(prog_.groups) + ")";
     }
 
@@ -1429,6 +1445,7 @@ private:
             src << "    __shared__ u32 s_ht" << j << "[" << jp.ht_words << "];\n"
                 << "    for (int k = tid; k < " << jp.ht_words << "; k += BT) s_ht" << j << "[k] = P.ht[" << j << "][k];\n";
         }
+        emitTableRsrc(src);
         src << prologue.str();
         src << "    __syncthreads();\n";
         const size_t nf = prog_.fact_cols.size();
@@ -1520,6 +1537,7 @@ private:
                 src << "    __shared__ u32 s_ht" << j << "[" << jp.ht_words << "];\n"
                     << "    for (int k = tid; k < " << jp.ht_words << "; k += BT) s_ht" << j << "[k] = P.ht[" << j << "][k];\n";
             }
+            emitTableRsrc(src);
             src << "    __syncthreads();\n"
                 << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n"
                 << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
@@ -1598,6 +1616,7 @@ private:
                 src << "    __shared__ u32 s_ht" << j << "[" << jp.ht_words << "];\n"
                     << "    for (int k = tid; k < " << jp.ht_words << "; k += BT) s_ht" << j << "[k] = P.ht[" << j << "][k];\n";
             }
+            emitTableRsrc(src);
             src << "    __syncthreads();\n"
                 << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n"
                 << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
@@ -1663,6 +1682,20 @@ private:
         body << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
     }
 
+    // key-indexed table in HBM read through a buffer resource (rt<j>, set up
+    // at the top of the kernel by emitTableRsrc)
+    bool useRsrc(const JoinProgram& jp) const {
+        const bool stream = opt_.stream_large_tables && !jp.lds && jp.ht_words * 4 > 32 * 1024;
+        return jp.dense && !jp.lds && opt_.buffer_tables && !opt_.batch_body && !stream && opt_.split_probes &&
+               jp.ht_words * 4 < (uint64_t(1) << 31);
+    }
+    void emitTableRsrc(std::ostringstream& src) const {
+        for (size_t j = 0; j < prog_.joins.size(); ++j)
+            if (useRsrc(prog_.joins[j]))
+                src << "    const chip::table_rsrc rt" << j << " = chip::make_table_rsrc(P.ht[" << j << "], "
+                    << prog_.joins[j].ht_words * 4 << "u);\n";
+    }
+
     // Probe of join `join` with the per-row key expression `key` (of item i);
     // keyValid: the key's validity bit expression, "" when it cannot be NULL.
     void emitProbe(std::ostringstream& body, int join, const std::string& key, const std::string& keyValid) {
@@ -1687,12 +1720,17 @@ private:
                      << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
                      << "            q" << j << "[i] = chip::dense_locate(" << key << ", " << live << ", " << jp.dmin << ", "
                      << jp.dspan << ", " << jp.zmode << ", " << jp.zx << ", " << jp.cal << ", x" << j << "[i]);\n"
-                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
-                     << "            chip::dense_words<" << al << ", " << st << ">(" << table << ", x" << j << "[i], " << jp.db
-                     << ", l" << j << "[i], h" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n";
+                if (useRsrc(jp))
+                    body << "            chip::dense_words_rsrc<" << al << ">(rt" << j << ", x" << j << "[i], " << jp.db << ", l" << j
+                         << "[i], h" << j << "[i]);\n";
+                else
+                    body << "            chip::dense_words<" << al << ", " << st << ">(" << table << ", x" << j << "[i], " << jp.db
+                         << ", l" << j << "[i], h" << j << "[i]);\n";
+                body
                      << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
                      << "            p" << j << "[i] = 0;\n"
-                     << "            if (!chip::dense_decode<" << al << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
+                     << "            if (!chip::dense_decode<" << al << ", " << jp.zmode << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
                      << "[i], " << jp.db << ", " << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j
                      << "[i])) " << flClear() << "\n"
                      << "        }\n";

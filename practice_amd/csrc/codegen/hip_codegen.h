@@ -38,6 +38,7 @@ struct CodegenOptions {
     bool prefetch = false;            // load tile t+stride while tile t is processed
     bool stream_large_tables = false; // streaming loads for probes of HBM tables beyond L1 (measured 2x slower on C3: off)
     bool split_probes = true;         // probes as locate / load / decode loops (all of a thread's loads in flight at once)
+    bool buffer_tables = true;        // key-indexed tables in HBM read through a buffer resource (32-bit offsets, range-checked)
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;

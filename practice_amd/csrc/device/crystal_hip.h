@@ -67,6 +67,14 @@ template <class A, class B> __device__ __forceinline__ int safe_lte(A a, B b) { 
 template <class A, class B> __device__ __forceinline__ int safe_gte(A a, B b) { return prom(a) >= prom(b) ? 1 : 0; }
 template <class A, class B> __device__ __forceinline__ int safe_eq(A a, B b) { return prom(a) == prom(b) ? 1 : 0; }
 template <class A, class B> __device__ __forceinline__ int safe_neq(A a, B b) { return prom(a) != prom(b) ? 1 : 0; }
+// int32 against int32: the int64 promotion cannot change the outcome, so the
+// comparison runs in 32 bits
+__device__ __forceinline__ int safe_lt(int a, int b) { return a < b ? 1 : 0; }
+__device__ __forceinline__ int safe_gt(int a, int b) { return a > b ? 1 : 0; }
+__device__ __forceinline__ int safe_lte(int a, int b) { return a <= b ? 1 : 0; }
+__device__ __forceinline__ int safe_gte(int a, int b) { return a >= b ? 1 : 0; }
+__device__ __forceinline__ int safe_eq(int a, int b) { return a == b ? 1 : 0; }
+__device__ __forceinline__ int safe_neq(int a, int b) { return a != b ? 1 : 0; }
 
 __device__ __forceinline__ double bits_to_double(u64 v) { return __builtin_bit_cast(double, v); }
 __device__ __forceinline__ u64 double_to_bits(double v) { return __builtin_bit_cast(u64, v); }

@@ -27,6 +27,10 @@ __device__ __forceinline__ u32 ht_word(const u32* w, u32 i) {
     return w[i];
 }
 
+// low 32 bits of the product of the low 24 bits of a and b (the masks let
+// the compiler use the full-rate 24-bit multiply)
+__device__ __forceinline__ u32 mul24(u32 a, u32 b) { return (a & 0xffffffu) * (b & 0xffffffu); }
+
 // Reference slot: (k - kmin) % len in int arithmetic.  A negative k - kmin
 // (an out-of-bounds access in the reference) never matches here.
 __device__ __forceinline__ int ht_slot(int k, int kmin, int len, int& x) {
@@ -124,10 +128,25 @@ __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) 
 // that is not such a date, since no table key is one).
 __device__ __forceinline__ u32 dense_index(int k, int dmin, int dspan, int cal) {
     if (!cal) return (u32)k - (u32)dmin;
-    if (k < 0) return (u32)dspan;
-    const int y = k / 10000, md = k - y * 10000, m = md / 100, d = md - m * 100;
-    if (m < 1 || m > 12 || d < 1 || d > 31 || y < cal) return (u32)dspan;
-    return (u32)(((y - cal) * 12 + (m - 1)) * 31 + (d - 1));
+    // Branch-free (selects only).  u = (y - cal) * 10000 + m * 100 + d must
+    // lie below years * 10000 (negative keys and earlier years wrap above).
+    // With at most 13 years, u < 130,000 and y - cal = floor((u + 0.5) / 10^4)
+    // in single precision is exact: (u + 0.5) / 10^4 stays at least 5e-5 from
+    // an integer, far beyond the float rounding (< 2e-6 here); the month is
+    // md * 10486 >> 20, exact for md < 43,690.  All full-rate 24-bit / fp32
+    // operations instead of 32-bit division.
+    const u32 years = (u32)dspan / 372u;
+    const u32 u = (u32)k - (u32)cal * 10000u;
+    const bool inr = u < years * 10000u;
+    const u32 yo = years <= 13u ? (u32)(((float)u + 0.5f) * 1.0e-4f) : u / 10000u;
+    // every factor below 2^24 whenever the key is in range (out of range,
+    // the products are ignored): 24-bit multiplies
+    const u32 md = u - mul24(yo, 10000u);
+    const u32 m = mul24(md, 10486u) >> 20;
+    const u32 d = md - mul24(m, 100u);
+    const bool ok = inr && m - 1u < 12u && d - 1u < 31u;
+    const u32 x = mul24(mul24(yo, 12u) + m - 1u, 31u) + (d - 1u);
+    return ok ? x : (u32)dspan;
 }
 
 // Probe key 0 in the zero-filled reference table matches an EMPTY slot
@@ -167,38 +186,66 @@ __device__ __forceinline__ bool ht_probe_dense(const u32* w, int fk, int dmin, i
 //                 present), 4 key 0 matches with payload 0
 //   dense_words   the entry's word (and the next one for exact widths)
 //   dense_decode  match and payload, as ht_probe_dense
+//                 (selects only: the generated code passes zmode as a
+//                 literal, so with zmode 0 the key-0 cases vanish)
 __device__ __forceinline__ u32 dense_locate(int fk, bool live, int dmin, int dspan, int zmode, int zx, int cal, u32& x) {
-    x = 0u;
-    if (!live) return 0u;
-    if (zmode != 0 && fk == 0) {
-        if (zmode == 1) return 4u;
-        x = (u32)zx;
-        return (u32)zmode;
-    }
     const u32 y = dense_index(fk, dmin, dspan, cal);
-    if (y >= (u32)dspan) return 0u;
-    x = y;
-    return 1u;
+    const bool in = y < (u32)dspan;
+    u32 kind = in ? 1u : 0u;
+    u32 at = in ? y : 0u;
+    if (zmode != 0) {
+        const bool z = fk == 0;
+        kind = z ? (zmode == 1 ? 4u : (u32)zmode) : kind;
+        at = z ? (zmode == 1 ? 0u : (u32)zx) : at;
+    }
+    x = live ? at : 0u;  // a dead item reads entry 0 (in the table, one shared line)
+    return live ? kind : 0u;
 }
+#if defined(__HIP_DEVICE_COMPILE__)
+// A key-indexed table in HBM read through a buffer resource: the per-lane
+// address is a 32-bit byte offset (no 64-bit address arithmetic per lane)
+// and reads beyond `bytes` return 0.  dword 3 of the descriptor: the gfx9
+// family's raw 32-bit buffer format.
+typedef __amdgpu_buffer_rsrc_t table_rsrc;
+__device__ __forceinline__ table_rsrc make_table_rsrc(const u32* w, u32 bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<u32*>(w), (short)0, (int)bytes, 0x00020000);
+}
+// entry words as dense_words<ALIGNED> (byte-wide entries: the byte itself)
+template <bool ALIGNED>
+__device__ __forceinline__ void dense_words_rsrc(table_rsrc r, u32 x, int db, u32& lo, u32& hi) {
+    const u32 b = x * (u32)db;
+    if (db == 8) {
+        lo = (u32)(unsigned char)__builtin_amdgcn_raw_buffer_load_b8(r, x, 0, 0) << (b & 31);
+        hi = 0u;
+        return;
+    }
+    lo = db == 0 ? 0u : (u32)__builtin_amdgcn_raw_buffer_load_b32(r, (b >> 5) * 4u, 0, 0);
+    hi = (ALIGNED || db == 0) ? 0u : (u32)__builtin_amdgcn_raw_buffer_load_b32(r, ((b >> 5) + 1u) * 4u, 0, 0);
+}
+#endif
+
 template <bool ALIGNED, bool STREAM = false>
 __device__ __forceinline__ void dense_words(const u32* w, u32 x, int db, u32& lo, u32& hi) {
     const u32 b = x * (u32)db;
     lo = db == 0 ? 0u : ht_word<STREAM>(w, b >> 5);
     hi = (ALIGNED || db == 0) ? 0u : ht_word<STREAM>(w, (b >> 5) + 1);  // the padding word keeps this in the table
 }
-template <bool ALIGNED>
+// (selects only; ZMODE as in dense_locate: with 0, kind is 0 or 1)
+template <bool ALIGNED, int ZMODE = -1>
 __device__ __forceinline__ bool dense_decode(u32 kind, u32 x, u32 lo, u32 hi, int db, int pbase, bool off, int& payload) {
     const u32 b = x * (u32)db;
     const u32 mask = (1u << db) - 1u;
     const u32 e = db == 0 ? 0u : (ALIGNED ? (lo >> (b & 31)) & mask : (u32)((((u64)hi << 32) | lo) >> (b & 31)) & mask);
     const bool in = db == 0 || !off || e != 0u;
-    switch (kind) {
-        case 1u: payload = pbase + (int)e; return in;
-        case 2u: payload = 0; return !in;
-        case 3u: payload = in ? pbase + (int)e : 0; return true;
-        case 4u: payload = 0; return true;
-        default: return false;
+    const int pe = pbase + (int)e;
+    if (ZMODE == 0) {
+        payload = pe;
+        return kind != 0u && in;
     }
+    // kind 1: the entry; 2: key 0 matches iff entry zx is absent; 3: key 0
+    // matches, payload if present; 4: key 0 matches, payload 0
+    payload = (kind == 1u || (kind == 3u && in)) ? pe : 0;
+    return kind == 1u ? in : (kind == 2u ? !in : kind >= 3u);
 }
 
 __device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dspan, int db, int pbase, int payload, int cal = 0) {

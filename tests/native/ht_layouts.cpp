@@ -161,9 +161,19 @@ int main(int argc, char** argv) {
                     if (aligned) {
                         dense_words<true>(&dbuf[8], x, db, lo, hi);
                         sm = dense_decode<true>(kind, x, lo, hi, db, pbase, doff != 0, sp);
+                        int sp0 = 0;  // the generated code's form when the key-0 corner cannot occur
+                        if (zmode == 0 && (dense_decode<true, 0>(kind, x, lo, hi, db, pbase, doff != 0, sp0) != sm || (sm && sp0 != sp))) {
+                            printf("case %d: ZMODE 0 decode differs fk=%d\n", c, fk);
+                            return 1;
+                        }
                     } else {
                         dense_words<false>(&dbuf[8], x, db, lo, hi);
                         sm = dense_decode<false>(kind, x, lo, hi, db, pbase, doff != 0, sp);
+                        int sp0 = 0;
+                        if (zmode == 0 && (dense_decode<false, 0>(kind, x, lo, hi, db, pbase, doff != 0, sp0) != sm || (sm && sp0 != sp))) {
+                            printf("case %d: ZMODE 0 decode differs fk=%d\n", c, fk);
+                            return 1;
+                        }
                     }
                     const bool w2 = live && want;
                     if (sm != w2 || (w2 && pair && sp != wp) || (w2 && !pair && sp != dp)) {
@@ -186,7 +196,7 @@ int main(int argc, char** argv) {
     // reference table
     long long cal_probes = 0;
     for (int c = 0; c < 1 + cases / 10; ++c) {
-        const int y0 = 1990 + (int)(rng() % 20), ny = 1 + (int)(rng() % 8);
+        const int y0 = 1990 + (int)(rng() % 20), ny = 1 + (int)(rng() % 20);  // up to 13 years: float index path
         const bool pair = c % 3 != 0;
         std::vector<int> keys;
         for (int y = y0; y < y0 + ny; ++y)
@@ -253,10 +263,10 @@ int main(int argc, char** argv) {
                     bool sm;
                     if (aligned) {
                         dense_words<true>(&dbuf[8], x, db, lo, hi);
-                        sm = dense_decode<true>(kind, x, lo, hi, db, pbase, true, sp);
+                        sm = dense_decode<true, 0>(kind, x, lo, hi, db, pbase, true, sp);
                     } else {
                         dense_words<false>(&dbuf[8], x, db, lo, hi);
-                        sm = dense_decode<false>(kind, x, lo, hi, db, pbase, true, sp);
+                        sm = dense_decode<false, 0>(kind, x, lo, hi, db, pbase, true, sp);
                     }
                     if (sm != dm || (sm && sp != dp)) {
                         printf("calendar case %d: split probe fk=%d got (%d,%d) want (%d,%d)\n", c, fk, sm, sp, dm, dp);

@@ -21,6 +21,16 @@ C3 = ("select c_nation, s_nation, d_year, sum(lo_revenue) from lineorder, custom
 
 C3_NULLABLE = "LINEORDER.lo_revenue:200,SUPPLIER.s_nation:200"
 
+# C3's pieces over the same data, for profiling what bounds it: the CUSTOMER
+# join alone (one random gather per row into the 3M-key table), and the same
+# four fact columns, filter, SUPPLIER / DDATE probes and grouping with no
+# CUSTOMER gather (lo_custkey read by a filter that keeps every row)
+C3_CUSTOMER_ONLY = ("select c_nation, sum(lo_revenue) from lineorder, customer where lo_custkey = c_custkey "
+                    "and (lo_revenue > 1000000 or lo_revenue is null) group by c_nation;")
+C3_NO_GATHER = ("select s_nation, d_year, sum(lo_revenue) from lineorder, supplier, ddate "
+                "where lo_suppkey = s_suppkey and lo_orderdate = d_datekey and lo_custkey > 0 "
+                "and (lo_revenue > 1000000 or lo_revenue is null) group by s_nation, d_year;")
+
 # (sf, fact_rows) of each configuration at full size
 FULL = {"C1": (1, 6_000_000), "C2": (20, 120_000_000), "C3": (100, 600_000_000)}
 

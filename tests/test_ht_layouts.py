@@ -48,7 +48,7 @@ def _layout_calls(src):
         probes[int(m.group(2))] = (kind, geo)
     # locate / load / decode probes
     loc = {int(m.group(2)): ints(m.group(1)) for m in re.finditer(r"dense_locate\(c\d+\[i\], [^,]*, ([^;]*), x(\d+)\[i\]\);", src)}
-    for m in re.finditer(r"dense_decode<\w+>\(q(\d+)\[i\], x\d+\[i\], l\d+\[i\], h\d+\[i\], (-?\d+), (-?\d+), \w+, p\d+\[i\]\)", src):
+    for m in re.finditer(r"dense_decode<\w+(?:, -?\d+)?>\(q(\d+)\[i\], x\d+\[i\], l\d+\[i\], h\d+\[i\], (-?\d+), (-?\d+), \w+, p\d+\[i\]\)", src):
         j = int(m.group(1))
         probes[j] = ("dense", loc[j][:2] + [m.group(2), m.group(3), loc[j][4]])
     for m in re.finditer(r"compact_decode\(w(\d+)\[i\], s\d+\[i\], c\d+\[i\], ([^;]*), p\d+\[i\]\)\)", src):
