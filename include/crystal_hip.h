@@ -133,7 +133,8 @@ long long cq_db_table_rows(cq_db* db, const char* table);
  * "memory_guard_reserve_bytes", "prefetch" (load the next tile while the
  * current one is probed), "interpret" (1: run plans on the ahead-of-time
  * interpreter instead of hipRTC-compiled kernels; same results, no compile),
- * "vm_row_kernel", "host_sort_rows", "prepared_cache", "module_cache_capacity" (compiled modules kept,
+ * "vm_row_kernel", "vm_linear" (0: interpreted plans always run their bytecode,
+ * also those with a linear form), "host_sort_rows", "prepared_cache", "module_cache_capacity" (compiled modules kept,
  * least recently used evicted) */
 int cq_db_set_option(cq_db* db, const char* name, long long value);
 long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","evictions","compiles" */

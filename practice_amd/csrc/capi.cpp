@@ -300,6 +300,7 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "buffer_tables") c.codegen.buffer_tables = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
+        else if (n == "vm_linear") c.vm_linear = v != 0;
         else if (n == "host_sort_rows") c.host_sort_rows = v;
         else if (n == "prepared_cache") c.prepared_cache = v != 0;
         else if (n == "module_cache_capacity") {
@@ -377,6 +378,7 @@ int cq_db_describe(cq_db* d, const char* sql, char* buf, size_t len, size_t* nee
           << ",\"fast_sparse\":" << (p.fast_sparse ? 1 : 0) << ",\"lds_aggregation\":" << (p.lds_aggregation ? 1 : 0)
           << ",\"block_threads\":" << p.block_threads << ",\"vectors_per_thread\":" << p.vectors_per_thread
           << ",\"scan_lds_bytes\":" << p.scan_lds_bytes << ",\"algorithmic_bytes\":" << p.algorithmic_bytes
+          << ",\"interpreted\":" << (p.vm ? 1 : 0) << ",\"linear\":" << ((p.vm && p.vm->lin.enabled) ? 1 : 0)
           << ",\"fact_cols\":[";
         for (size_t i = 0; i < p.fact_cols.size(); ++i) o << (i ? "," : "") << "\"" << p.fact_cols[i] << "\"";
         o << "],\"dim_cols\":[";

@@ -1806,6 +1806,149 @@ private:
     // ------------------------------------------------ interpreted tier
     // The same expressions the HIP source would contain, compiled to
     // register code for the ahead-of-time interpreter (no hipRTC).
+    // The linear form of an interpreted plan (cq_lin_program), when it has
+    // one; otherwise lin.enabled stays 0 and the bytecode runs.  fcmp_lits:
+    // the P.lit index of every fast-path compare in emission order;
+    // agg_lit_base: per aggregate, the first P.lit index its value
+    // expression took (its literals follow in left-to-right order).
+    void analyzeLinear(cq_vm_program* vp, const std::vector<int>& fcmp_lits, const std::vector<int>& agg_lit_base) {
+        cq_lin_program L;
+        std::memset(&L, 0, sizeof L);
+        L.join = -1;
+        auto pos = [&](const std::string& col) -> int {  // column position, -1: no room
+            const int s = factSlot(col);
+            for (int k = 0; k < L.ncols; ++k)
+                if (L.slot[k] == s) return k;
+            if (L.ncols >= CQ_LIN_COLS) return -1;
+            L.slot[L.ncols] = s;
+            L.nullable[L.ncols] = prog_.fact_nullable[static_cast<size_t>(s)] ? 1 : 0;
+            return L.ncols++;
+        };
+        if (prog_.projection || prog_.joins.size() > 1) return;
+        if (prog_.naggs > CQ_LIN_AGGS || (!prog_.grouped && prog_.storage_ts > CQ_LIN_RACC)) return;
+        if (!prog_.joins.empty()) {
+            const JoinProgram& jp = prog_.joins[0];
+            if (!jp.dense || jp.mht || jp.rowid || jp.fk_alts.size() != 1) return;
+            L.join = 0;
+        }
+        size_t next_fcmp = 0;
+        for (const Op& op : pipelineOrder()) {
+            if (op.kind == 1) continue;
+            if (!op.fast) return;
+            std::vector<const BinaryExpr*> leaves;
+            andLeaves(op.pred, leaves);
+            for (const BinaryExpr* lf : leaves) {
+                const auto& col = static_cast<const ColumnRefExpr*>(lf->left.get())->column_name;
+                if (L.nterms >= CQ_LIN_TERMS || next_fcmp >= fcmp_lits.size()) return;
+                cq_lin_term& t = L.term[L.nterms++];
+                if ((t.col = pos(col)) < 0) return;
+                switch (lf->op_type) {
+                    case ExprType::OP_EQ: t.op = CQ_OP_EQ; break;
+                    case ExprType::OP_NEQ: t.op = CQ_OP_NE; break;
+                    case ExprType::OP_LT: t.op = CQ_OP_LT; break;
+                    case ExprType::OP_LTE: t.op = CQ_OP_LE; break;
+                    case ExprType::OP_GT: t.op = CQ_OP_GT; break;
+                    case ExprType::OP_GTE: t.op = CQ_OP_GE; break;
+                    default: return;
+                }
+                t.lit = fcmp_lits[next_fcmp++];
+            }
+        }
+        // filtered columns take the first positions; terms grouped by
+        // position (every term must hold, so their order is free)
+        L.nfilter_cols = L.ncols;
+        std::stable_sort(L.term, L.term + L.nterms, [](const cq_lin_term& x, const cq_lin_term& y) { return x.col < y.col; });
+        for (int c = 0, t = 0; c < CQ_LIN_COLS; ++c) {
+            while (t < L.nterms && L.term[t].col == c) ++t;
+            L.term_end[c] = t;
+        }
+        if (L.join == 0 && (L.join_col = pos(prog_.joins[0].fk_col)) < 0) return;
+        // one operand: a fact column, the join's payload or (with `litc`)
+        // the next literal of the expression, in the type lit() gave it
+        auto operand = [&](const ExprNode* e, cq_lin_step& st, int* litc) -> bool {
+            if (e->getType() == ExprType::LITERAL_INT) {
+                if (!litc) return false;
+                const int64_t v = static_cast<const LiteralIntExpr*>(e)->value;
+                st.kind = CQ_LIN_LIT;
+                st.idx = (*litc)++;
+                st.i32 = (v >= -2147483647LL && v <= 2147483647LL) ? 1 : 0;
+                return true;
+            }
+            if (e->getType() != ExprType::COLUMN_REF) return false;
+            const auto& col = static_cast<const ColumnRefExpr*>(e)->column_name;
+            if (tableOf(col) == prog_.fact_table) {
+                st.kind = CQ_LIN_COL;
+                return (st.idx = pos(col)) >= 0;
+            }
+            if (L.join == 0 && prog_.joins[0].payload_col == col) {
+                st.kind = CQ_LIN_PAY;
+                st.idx = 0;
+                return true;
+            }
+            return false;
+        };
+        // left-deep chain: the leftmost leaf a column, right operands leaves
+        std::function<bool(const ExprNode*, cq_lin_agg&, int*)> chain = [&](const ExprNode* e, cq_lin_agg& g, int* litc) -> bool {
+            const ExprType t = e->getType();
+            if (t == ExprType::COLUMN_REF) {
+                if (g.nsteps >= CQ_LIN_SRCS) return false;
+                return operand(e, g.step[g.nsteps++], nullptr);
+            }
+            if (t != ExprType::OP_ADD && t != ExprType::OP_SUB && t != ExprType::OP_MUL) return false;
+            const auto* b = static_cast<const BinaryExpr*>(e);
+            if (!chain(b->left.get(), g, litc) || g.nsteps >= CQ_LIN_SRCS) return false;
+            cq_lin_step& st = g.step[g.nsteps++];
+            st.op = t == ExprType::OP_ADD ? CQ_OP_ADD : (t == ExprType::OP_SUB ? CQ_OP_SUB : CQ_OP_MUL);
+            return operand(b->right.get(), st, litc);
+        };
+        // validity of arithmetic over leaves: AND of the nullable fact
+        // columns' bits (literals and payload registers are always valid)
+        std::function<bool(const ExprNode*, cq_lin_agg&)> validity = [&](const ExprNode* e, cq_lin_agg& g) -> bool {
+            const ExprType t = e->getType();
+            if (t == ExprType::LITERAL_INT) return true;
+            if (t == ExprType::COLUMN_REF) {
+                cq_lin_step st;
+                std::memset(&st, 0, sizeof st);
+                if (!operand(e, st, nullptr)) return false;
+                if (st.kind != CQ_LIN_COL || !L.nullable[st.idx]) return true;
+                for (int k = 0; k < g.nvalid; ++k)
+                    if (g.valid_col[k] == st.idx) return true;
+                if (g.nvalid >= CQ_LIN_SRCS) return false;
+                g.valid_col[g.nvalid++] = st.idx;
+                return true;
+            }
+            if (t != ExprType::OP_ADD && t != ExprType::OP_SUB && t != ExprType::OP_MUL) return false;
+            const auto* b = static_cast<const BinaryExpr*>(e);
+            return validity(b->left.get(), g) && validity(b->right.get(), g);
+        };
+        const auto* a = agg_;
+        L.naggs = prog_.naggs;
+        for (int i = 0; i < prog_.naggs; ++i) {
+            const auto& d = a->aggregates[static_cast<size_t>(i)];
+            cq_lin_agg& g = L.agg[i];
+            if (d.hasStarArgument()) continue;  // constant 1, always valid
+            if (!validity(d.agg_expr.get(), g)) return;
+            if (d.isCount()) continue;
+            int litc = agg_lit_base[static_cast<size_t>(i)];
+            if (!chain(d.agg_expr.get(), g, &litc)) return;
+        }
+        if (prog_.grouped) {
+            if (prog_.ngroups > CQ_LIN_KEYS || !vp->decodable) return;
+            L.ngroups = prog_.ngroups;
+            for (int g = 0; g < prog_.ngroups; ++g) {
+                const ExprNode* e = a->group_by_exprs[static_cast<size_t>(g)].get();
+                cq_lin_step st;
+                std::memset(&st, 0, sizeof st);
+                if (!operand(e, st, nullptr)) return;
+                L.key[g].kind = st.kind;
+                L.key[g].col = st.idx;
+                L.key[g].valid_col = (st.kind == CQ_LIN_COL && L.nullable[st.idx]) ? st.idx : -1;
+            }
+        }
+        L.enabled = 1;
+        vp->lin = L;
+    }
+
     void emitVm() {
         auto vp = std::make_shared<cq_vm_program>();
         std::memset(vp.get(), 0, sizeof(cq_vm_program));
@@ -1847,6 +1990,7 @@ private:
             b.setPayloadReg(j, pay_reg[static_cast<size_t>(j)]);
         }
         visible_mask_ = 0;
+        std::vector<int> fcmp_lits, agg_lit_base;
         for (const Op& op : pipelineOrder()) {
             if (op.kind == 0) {
                 if (op.fast) {
@@ -1859,6 +2003,7 @@ private:
                             unsupported("fast-path filter on dimension column " + col + " above its join");
                         const int k = static_cast<int>(prog_.lits.size());
                         (void)litInt32(static_cast<const LiteralIntExpr*>(lf->right.get())->value);
+                        fcmp_lits.push_back(k);
                         int cmp = CQ_OP_EQ;
                         switch (lf->op_type) {
                             case ExprType::OP_EQ: cmp = CQ_OP_EQ; break;
@@ -1921,6 +2066,7 @@ private:
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             vp->agg_fn[i] = f == "MIN" ? CQ_VM_MIN : (f == "MAX" ? CQ_VM_MAX : (f == "COUNT" ? CQ_VM_COUNT : CQ_VM_SUM));
             vp->agg_hidden[i] = prog_.hidden_slot[static_cast<size_t>(i)];
+            agg_lit_base.push_back(static_cast<int>(prog_.lits.size()));
             const AggRow r = aggRow(i);
             vp->agg_val[i] = f == "COUNT" ? -1 : b.compile(r.value).reg;
             vp->agg_valid[i] = litTrue(r.valid) ? -1 : b.compile(r.valid).reg;
@@ -1945,9 +2091,11 @@ private:
         const uint64_t nas = static_cast<uint64_t>(prog_.storage_ts - prog_.ngroups);
         const uint64_t lds = static_cast<uint64_t>(prog_.groups) * (nas * 8 + 4);
         prog_.vm_lds_bytes = (prog_.grouped && vp->decodable && lds <= 64 * 1024) ? lds : 0;
+        analyzeLinear(vp.get(), fcmp_lits, agg_lit_base);
         prog_.vm = vp;
         std::ostringstream src;
-        src << "// interpreted: " << code.size() << " instructions, " << b.maxRegs() << " registers\n";
+        src << "// interpreted: " << code.size() << " instructions, " << b.maxRegs() << " registers"
+            << (vp->lin.enabled ? ", linear form" : "") << "\n";
         for (size_t k = 0; k < code.size(); ++k)
             src << "// " << k << ": op " << int(code[k].op) << " t" << int(code[k].t) << "/" << int(code[k].t2) << " d" << int(code[k].d)
                 << " a" << int(code[k].a) << " b" << int(code[k].b) << " c" << int(code[k].c) << " imm " << code[k].imm << "\n";

@@ -66,6 +66,64 @@ struct cq_vm_join {
 
 enum cq_vm_agg_fn { CQ_VM_SUM = 0, CQ_VM_MIN = 1, CQ_VM_MAX = 2, CQ_VM_COUNT = 3 };
 
+// Linear form of the same plan, when it has one (cq_lin_program.enabled):
+// fast-path fact filters only (AND of `(int)column op (int)literal` on raw
+// values), at most one join, on a key-indexed table, group keys that are
+// fact or payload columns of dense domains, and aggregate inputs that are
+// left-deep chains in unsigned long long (the leftmost leaf a column cast to
+// u64, every right operand a column, the payload or a literal, each
+// converted to u64 as C converts it).  Literals stay P.lit references, so
+// one program serves every query of its shape.  An ahead-of-time kernel
+// evaluates the form with no per-instruction dispatch and produces the
+// values the bytecode would.
+#define CQ_LIN_COLS 8
+#define CQ_LIN_TERMS 8
+#define CQ_LIN_AGGS 4
+#define CQ_LIN_SRCS 4
+#define CQ_LIN_KEYS 2
+#define CQ_LIN_RACC 8    // scalar result slots (aggregates + AVG counts)
+struct cq_lin_term {
+    int col;   // column position (cq_lin_program.slot)
+    int op;    // CQ_OP_EQ .. CQ_OP_GE
+    int lit;   // P.lit index
+    int pad;
+};
+// one step of a left-deep chain: x = x op operand (step 0: x = operand)
+enum cq_lin_kind { CQ_LIN_COL = 0, CQ_LIN_PAY = 1, CQ_LIN_LIT = 2 };
+struct cq_lin_step {
+    int op;    // CQ_OP_ADD / CQ_OP_SUB / CQ_OP_MUL (ignored for step 0)
+    int kind;  // cq_lin_kind
+    int idx;   // column position, or P.lit index
+    int i32;   // literal read as (int)P.lit (else as long long)
+};
+struct cq_lin_agg {
+    int nsteps;                         // 0: the constant 1 (COUNT)
+    int nvalid;                         // validity: AND of these column positions' bits
+    int valid_col[CQ_LIN_SRCS];
+    struct cq_lin_step step[CQ_LIN_SRCS];
+};
+struct cq_lin_key {
+    int kind, col;                      // CQ_LIN_COL (position) or CQ_LIN_PAY
+    int valid_col;                      // nullable fact column: its position, else -1
+    int pad;
+};
+struct cq_lin_program {
+    int enabled;
+    int ncols;                          // column positions: the filters' first
+    int nfilter_cols;                   // positions [0, nfilter_cols) hold every filtered column
+    int slot[CQ_LIN_COLS];              // fact column slot (P.col / P.valid) per position
+    int nullable[CQ_LIN_COLS];
+    int nterms;                         // sorted by column position
+    int term_end[CQ_LIN_COLS];          // terms of position c: [term_end[c - 1], term_end[c])
+    struct cq_lin_term term[CQ_LIN_TERMS];
+    int join;                           // -1: none; else the join (key-indexed layout) probed
+    int join_col;                       // position of its fact key
+    int naggs;                          // = cq_vm_program.naggs
+    struct cq_lin_agg agg[CQ_LIN_AGGS];
+    int ngroups;
+    struct cq_lin_key key[CQ_LIN_KEYS];
+};
+
 struct cq_vm_program {
     int ncode;
     struct cq_vm_instr code[CQ_VM_MAX_CODE];
@@ -93,6 +151,7 @@ struct cq_vm_program {
     // HAVING (one "row" per group)
     int having_begin, having_end, having_reg;  // having_reg < 0: no HAVING
     int having_type;
+    struct cq_lin_program lin;
 };
 
 #endif

@@ -21,3 +21,11 @@ bool cq_vm_scan4_supported(const cq_vm_program* V);
 // the grid fills the device
 hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,
                               hipStream_t s);
+// the plan's linear form (cq_lin_program) when it has one (V: host copy)
+bool cq_lin_supported(const cq_vm_program* V);
+// row_begin must be a multiple of cq_lin_row_align(); the grid fills the
+// device (V: the program in device memory, host: its host copy, which
+// picks the kernel instantiation)
+int cq_lin_row_align();
+hipError_t cq_lin_launch_scan(const cq_query_params* P, const cq_vm_program* V, const cq_vm_program* host, unsigned lds_group_bytes,
+                              int device, hipStream_t s);

@@ -544,6 +544,92 @@ __device__ __forceinline__ void vm4_aggregate_row(const cq_query_params& P, cons
     }
 }
 
+// Block-level aggregation state shared by the 4-row kernels: scalar slots
+// in s_acc / s_seen, or G groups of nas slots in s_agg with a flag word per
+// group in s_flag (bit 31 hit, bit s validity of slot s).
+template <int BT>
+__device__ __forceinline__ void agg_init(const cq_vm_program* __restrict__ V, int lds_groups, u64* s_acc, u32* s_seen, u64* s_agg,
+                                         u32* s_flag) {
+    const int tid = threadIdx.x;
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += BT) {
+            s_acc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
+            s_seen[k] = 0u;
+        }
+    } else if (lds_groups) {
+        for (long long k = tid; k < G * nas; k += BT) s_agg[k] = slot_is_min(V, (int)(k % nas)) ? ~0ull : 0ull;
+        for (long long k = tid; k < G; k += BT) s_flag[k] = 0u;
+    }
+    __syncthreads();
+}
+
+// Merge of a block: per-thread scalar slots racc (wave reduction, then one
+// atomic per slot and block), or the LDS groups with their keys decoded
+// from the group index.
+template <int BT, int NR>
+__device__ __forceinline__ void agg_flush(const cq_query_params& P, const cq_vm_program* __restrict__ V, int lds_groups, u64* s_acc,
+                                          u32* s_seen, const u64* s_agg, const u32* s_flag, const u64 (&racc)[NR], u32 rseen) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    if (ng == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (k >= sts) break;
+            const int fn = slot_fn(V, k);
+            const u64 w = wave_reduce(racc[k], fn);
+            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
+            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
+            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
+            if (lane == 0 && any) s_seen[k] = 1u;
+        }
+    }
+    __syncthreads();
+    if (ng == 0) {
+        for (int k = tid; k < sts; k += BT) {
+            const int fn = slot_fn(V, k);
+            bool hidden = false;
+            for (int i = 0; i < V->naggs; ++i) hidden = hidden || V->agg_hidden[i] == k;
+            const bool scalar_valid_bit = k < V->naggs && fn != CQ_VM_COUNT && V->agg_hidden[k] < 0;
+            const u64 x = s_acc[k];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
+            else if (x) atomicAdd(&P.result[k], x);
+            if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
+        }
+        return;
+    }
+    if (!lds_groups) return;
+    for (long long g = tid; g < G; g += BT) {
+        const u32 f = s_flag[g];
+        if (!(f & 0x80000000u)) continue;
+        const long long base = g * sts;
+        long long rem = g;
+        long long dg[CQ_VM_MAX_AGGS];
+        for (int k = ng - 1; k >= 0; --k) {
+            dg[k] = rem % V->key_card[k];
+            rem /= V->key_card[k];
+        }
+        for (int k = 0; k < ng; ++k) {
+            const long long v = V->key_nullable[k] ? (dg[k] == 0 ? 0ll : V->key_min[k] + dg[k] - 1) : V->key_min[k] + dg[k];
+            P.result[base + k] = (u64)v;
+        }
+        for (int s2 = 0; s2 < nas; ++s2) {
+            const int fn = s2 < V->naggs ? V->agg_fn[s2] : CQ_VM_SUM;
+            const u64 x = s_agg[g * nas + s2];
+            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[base + ng + s2], x); }
+            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[base + ng + s2], x); }
+            else if (x) atomicAdd(&P.result[base + ng + s2], x);
+        }
+        if (!V->fast_sparse)
+            for (int s2 = 0; s2 < sts; ++s2)
+                if ((f >> s2) & 1u) chip::set_bit64(P.result_valid, base + s2);
+    }
+}
+
 // dynamic LDS: register file (nregs x 4 x VM4_BT u64), then the group
 // aggregates when lds_groups (G * nas u64 + G u32 flag words)
 __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
@@ -551,22 +637,12 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
     __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
     extern __shared__ u64 s_dyn[];
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
     const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
     const long long G = V->groups;
     u64* RFt = s_dyn + tid;
     u64* s_agg = s_dyn + (long long)V->nregs * 4 * VM4_BT;
     u32* s_flag = (u32*)(s_agg + (lds_groups ? G * nas : 0));
-    if (ng == 0) {
-        for (int k = tid; k < sts; k += VM4_BT) {
-            s_acc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
-            s_seen[k] = 0u;
-        }
-    } else if (lds_groups) {
-        for (long long k = tid; k < G * nas; k += VM4_BT) s_agg[k] = slot_is_min(V, (int)(k % nas)) ? ~0ull : 0ull;
-        for (long long k = tid; k < G; k += VM4_BT) s_flag[k] = 0u;
-    }
-    __syncthreads();
+    agg_init<VM4_BT>(V, lds_groups, s_acc, s_seen, s_agg, s_flag);
     u64 racc[VM_RACC];
     u32 rseen = 0u;
 #pragma unroll
@@ -741,59 +817,404 @@ __global__ __launch_bounds__(VM4_BT) void k_vm_scan4(const cq_query_params P, co
 #pragma unroll
         for (int k = 0; k < 4; ++k) vm4_aggregate_row(P, V, VmReg{RFt + k * VM4_BT}, alive[k], lds_groups, s_agg, s_flag, racc, rseen);
     }
-    if (ng == 0) {
+    agg_flush<VM4_BT>(P, V, lds_groups, s_acc, s_seen, s_agg, s_flag, racc, rseen);
+}
+
+// ---------------------------------------------------------------- linear form
+// k_lin_scan evaluates cq_lin_program (a plan's filters, one key-indexed
+// probe, group index and aggregate chains) directly, with no register file
+// and no per-instruction dispatch.  A wave owns tiles of 512 rows; it
+// stages every column of the plan for its tile into its own LDS region with
+// direct global-to-LDS loads (global_load_lds_dwordx4: 1 KiB per
+// instruction, all of the tile's columns in flight at once, no registers
+// held while they fly), then every filter term, the probe key, the group
+// keys and the chain operands read their column from LDS at a computed
+// address.  Thread `lane` holds rows t0 + k * 256 + lane * 4 + j (k < 2,
+// j < 4) of the tile starting at t0, as bits 4k + j of its row masks.  It
+// produces the values k_vm_scan4 produces from the same program's bytecode.
+constexpr int LIN_BT = 256;
+constexpr int LIN_R = 8;                    // rows per thread
+constexpr int LIN_TILE = 64 * LIN_R;        // rows per wave tile
+constexpr int LIN_COL_INTS = LIN_TILE;      // LDS ints per column per wave
+constexpr u32 LIN_ALL = (1u << LIN_R) - 1u;
+
+// one wave tile: its first row, whether all its rows lie in the range
+// (wave-uniform), this lane's live-row mask, the range's last row
+struct LinRows {
+    long long t0, last;
+    bool full;
+    u32 inm;
+    int lane;
+};
+__device__ __forceinline__ long long lin_row(const LinRows& q, int bit) { return q.t0 + (bit >> 2) * 256 + q.lane * 4 + (bit & 3); }
+
+// stage the tile's rows of a column into the wave's LDS column `wl`
+__device__ __forceinline__ void lin_stage(const int* __restrict__ col, int* wl, const LinRows& q) {
+    if (q.full) {
+        __builtin_amdgcn_global_load_lds((const void*)(col + q.t0 + q.lane * 4), (__attribute__((address_space(3))) void*)wl, 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(col + q.t0 + 256 + q.lane * 4),
+                                         (__attribute__((address_space(3))) void*)(wl + 256), 16, 0, 0);
+    } else {
+        // the range's last, partial tile: rows past the end read the last row
 #pragma unroll
-        for (int k = 0; k < VM_RACC; ++k) {
-            if (k >= sts) break;
-            const int fn = slot_fn(V, k);
-            const u64 w = wave_reduce(racc[k], fn);
-            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
-            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
-            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
-            if (lane == 0 && any) s_seen[k] = 1u;
+        for (int k = 0; k < 2; ++k) {
+            chip::v4i v;
+            long long r = q.t0 + k * 256 + q.lane * 4;
+            v.x = col[r < q.last ? r : q.last];
+            v.y = col[r + 1 < q.last ? r + 1 : q.last];
+            v.z = col[r + 2 < q.last ? r + 2 : q.last];
+            v.w = col[r + 3 < q.last ? r + 3 : q.last];
+            *(chip::v4i*)(wl + k * 256 + q.lane * 4) = v;
         }
     }
-    __syncthreads();
-    if (ng == 0) {
-        for (int k = tid; k < sts; k += VM4_BT) {
-            const int fn = slot_fn(V, k);
-            bool hidden = false;
-            for (int i = 0; i < V->naggs; ++i) hidden = hidden || V->agg_hidden[i] == k;
-            const bool scalar_valid_bit = k < V->naggs && fn != CQ_VM_COUNT && V->agg_hidden[k] < 0;
-            const u64 x = s_acc[k];
-            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
-            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
-            else if (x) atomicAdd(&P.result[k], x);
-            if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
-        }
-        return;
+}
+// this lane's LIN_R values of a staged column
+__device__ __forceinline__ void lin_read(const int* wl, const LinRows& q, int (&x)[LIN_R]) {
+    const chip::v4i a = *(const chip::v4i*)(wl + q.lane * 4);
+    const chip::v4i b = *(const chip::v4i*)(wl + 256 + q.lane * 4);
+    x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w;
+    x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
+}
+// validity bits of this lane's rows; a column without a bitmap is valid
+// everywhere
+__device__ __forceinline__ u32 lin_valid(const u32* vb, const LinRows& q) {
+    if (!vb) return q.inm;
+    u32 m = 0u;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        long long r = q.t0 + k * 256 + q.lane * 4;  // a multiple of 4: 4 bits of one word
+        r = r < q.last ? r : q.last & ~3ll;
+        m |= ((vb[r >> 5] >> (r & 31)) & 0xfu) << (4 * k);
     }
-    if (!lds_groups) return;
-    for (long long g = tid; g < G; g += VM4_BT) {
-        const u32 f = s_flag[g];
-        if (!(f & 0x80000000u)) continue;
-        const long long base = g * sts;
-        long long rem = g;
-        long long dg[CQ_VM_MAX_AGGS];
-        for (int k = ng - 1; k >= 0; --k) {
-            dg[k] = rem % V->key_card[k];
-            rem /= V->key_card[k];
+    return m & q.inm;
+}
+
+template <bool ALIGNED>
+__device__ __forceinline__ u32 lin_probe(const cq_vm_join& J, const u32* ht, const int (&fk)[LIN_R], u32 alive, int (&pay)[LIN_R]) {
+#pragma unroll
+    for (int r = 0; r < LIN_R; ++r) {
+        int pv = 0;
+        if ((alive >> r) & 1u) {
+            if (!chip::ht_probe_dense<ALIGNED>(ht, fk[r], J.dmin, J.dspan, J.db, J.pbase, J.doff != 0, J.zmode, J.zx, pv, J.cal))
+                alive &= ~(1u << r);
         }
-        for (int k = 0; k < ng; ++k) {
-            const long long v = V->key_nullable[k] ? (dg[k] == 0 ? 0ll : V->key_min[k] + dg[k] - 1) : V->key_min[k] + dg[k];
-            P.result[base + k] = (u64)v;
-        }
-        for (int s2 = 0; s2 < nas; ++s2) {
-            const int fn = s2 < V->naggs ? V->agg_fn[s2] : CQ_VM_SUM;
-            const u64 x = s_agg[g * nas + s2];
-            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[base + ng + s2], x); }
-            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[base + ng + s2], x); }
-            else if (x) atomicAdd(&P.result[base + ng + s2], x);
-        }
-        if (!V->fast_sparse)
-            for (int s2 = 0; s2 < sts; ++s2)
-                if ((f >> s2) & 1u) chip::set_bit64(P.result_valid, base + s2);
+        pay[r] = pv;
     }
+    return alive;
+}
+
+// one aggregate's input for this lane's rows: the left-deep chain in u64
+// (column and payload operands stay int until each step converts them)
+__device__ __forceinline__ void lin_chain(const cq_query_params& P, const cq_lin_agg& A, int ns, const int* wcols, const LinRows& q,
+                                          const int (&pay)[LIN_R], u64 (&x)[LIN_R]) {
+#pragma unroll
+    for (int r = 0; r < LIN_R; ++r) x[r] = 1ull;
+    for (int s = 0; s < ns; ++s) {
+        const cq_lin_step& st = A.step[s];
+        const int op = s == 0 ? 0 : st.op;
+        if (st.kind == CQ_LIN_LIT) {
+            const u64 l = st.i32 ? (u64)(i64)(int)P.lit[st.idx] : (u64)P.lit[st.idx];
+            switch (op) {
+                case CQ_OP_ADD:
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) x[r] += l;
+                    break;
+                case CQ_OP_SUB:
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) x[r] -= l;
+                    break;
+                case CQ_OP_MUL:
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) x[r] *= l;
+                    break;
+                default:
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) x[r] = l;
+                    break;
+            }
+            continue;
+        }
+        int v[LIN_R];
+        if (st.kind == CQ_LIN_COL) lin_read(wcols + st.idx * LIN_COL_INTS, q, v);
+        else
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) v[r] = pay[r];
+        switch (op) {
+            case CQ_OP_ADD:
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) x[r] += (u64)(i64)v[r];
+                break;
+            case CQ_OP_SUB:
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) x[r] -= (u64)(i64)v[r];
+                break;
+            case CQ_OP_MUL:
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) x[r] *= (u64)(i64)v[r];
+                break;
+            default:
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) x[r] = (u64)(i64)v[r];
+                break;
+        }
+    }
+}
+
+// per-row atomics of one aggregate into group slots base[h * stride]
+// (rows of vm only; one branch on the function)
+__device__ __forceinline__ void lin_group_atomics(u64* base, long long stride, const int (&h)[LIN_R], const u64 (&x)[LIN_R], u32 vm,
+                                                  int fn) {
+    switch (fn) {
+        case CQ_VM_MIN:
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r)
+                if ((vm >> r) & 1u) atomicMin(&base[(long long)h[r] * stride], x[r]);
+            break;
+        case CQ_VM_MAX:
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r)
+                if ((vm >> r) & 1u) atomicMax(&base[(long long)h[r] * stride], x[r]);
+            break;
+        default:
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r)
+                if ((vm >> r) & 1u) atomicAdd(&base[(long long)h[r] * stride], x[r]);
+            break;
+    }
+}
+
+// GROUPED: GROUP BY plans (the scalar accumulators exist only without).
+// Dynamic LDS: per wave ncols staged columns (LIN_COL_INTS ints each), then
+// the group aggregates when lds_groups (G * nas u64 + G u32 flag words).
+template <bool GROUPED>
+__global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
+    __shared__ u64 s_acc[CQ_VM_MAX_AGGS * 2];
+    __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
+    extern __shared__ u64 s_dyn[];
+    constexpr int NR = GROUPED ? 1 : CQ_LIN_RACC;  // scalar slots kept per thread
+    const cq_lin_program& L = V->lin;
+    const int tid = threadIdx.x;
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    const bool fast_sparse = V->fast_sparse != 0;
+    const int nc = L.ncols, nterms = L.nterms, naggs = L.naggs;
+    int* wcols = (int*)s_dyn + (tid >> 6) * nc * LIN_COL_INTS;  // this wave's staged columns
+    u64* s_agg = s_dyn + (long long)(LIN_BT / 64) * nc * LIN_COL_INTS / 2;
+    u32* s_flag = (u32*)(s_agg + (lds_groups ? G * nas : 0));
+    agg_init<LIN_BT>(V, lds_groups, s_acc, s_seen, s_agg, s_flag);
+    u64 racc[NR];
+    u32 rseen = 0u;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) racc[k] = slot_is_min(V, k) ? ~0ull : 0ull;
+    const long long nrows = P.row_end - P.row_begin;
+    const long long ntiles = (nrows + LIN_TILE - 1) / LIN_TILE;
+    const long long waves = (long long)gridDim.x * (LIN_BT / 64);
+    LinRows q;
+    q.lane = tid & 63;
+    q.last = P.row_end - 1;
+    for (long long tile = (long long)blockIdx.x * (LIN_BT / 64) + (tid >> 6); tile < ntiles; tile += waves) {
+        q.t0 = P.row_begin + tile * LIN_TILE;
+        q.full = q.t0 + LIN_TILE <= P.row_end;
+        q.inm = 0u;
+#pragma unroll
+        for (int b = 0; b < LIN_R; ++b) q.inm |= (u32)(lin_row(q, b) <= q.last) << b;
+        // every column of the plan into LDS, all in flight at once
+        __builtin_amdgcn_s_waitcnt(0);  // the previous tile's LDS reads are done
+        for (int c = 0; c < nc; ++c) lin_stage(P.col[L.slot[c]], wcols + c * LIN_COL_INTS, q);
+        __builtin_amdgcn_s_waitcnt(0);  // the staged columns have landed
+        u32 alive = q.inm;
+        // filters
+        for (int t = 0; t < nterms; ++t) {
+            int x[LIN_R];
+            lin_read(wcols + L.term[t].col * LIN_COL_INTS, q, x);
+            const int l = (int)P.lit[L.term[t].lit];
+            u32 m = 0u;
+            switch (L.term[t].op) {
+#define LIN_CMP(OPC, OPR)                                                              \
+    case OPC:                                                                           \
+        _Pragma("unroll") for (int r = 0; r < LIN_R; ++r) m |= (u32)(x[r] OPR l) << r; \
+        break;
+                LIN_CMP(CQ_OP_EQ, ==)
+                LIN_CMP(CQ_OP_NE, !=)
+                LIN_CMP(CQ_OP_LT, <)
+                LIN_CMP(CQ_OP_LE, <=)
+                LIN_CMP(CQ_OP_GT, >)
+                LIN_CMP(CQ_OP_GE, >=)
+#undef LIN_CMP
+                default: break;
+            }
+            alive &= m;
+        }
+        if (__ballot(alive != 0u) == 0ull) continue;
+        int pay[LIN_R];
+#pragma unroll
+        for (int r = 0; r < LIN_R; ++r) pay[r] = 0;
+        if (L.join >= 0) {
+            const cq_vm_join& J = V->joins[L.join];
+            const int jc = L.join_col;
+            int fk[LIN_R];
+            lin_read(wcols + jc * LIN_COL_INTS, q, fk);
+            if (L.nullable[jc]) alive &= lin_valid(P.valid[L.slot[jc]], q);
+            const u32* ht = P.ht[L.join];
+            alive = (J.db & (J.db - 1)) == 0 ? lin_probe<true>(J, ht, fk, alive, pay) : lin_probe<false>(J, ht, fk, alive, pay);
+            if (__ballot(alive != 0u) == 0ull) continue;
+        }
+        if (!GROUPED) {
+            // scalar aggregates: this lane's rows folded, then one update of
+            // the aggregate's slot (and its AVG count)
+#pragma unroll
+            for (int i = 0; i < CQ_LIN_AGGS; ++i) {
+                if (i >= naggs) break;
+                const cq_lin_agg& A = L.agg[i];
+                const int fn = V->agg_fn[i];
+                const int hs = V->agg_hidden[i];
+                u32 vm = alive;
+                for (int c = 0; c < A.nvalid; ++c) vm &= lin_valid(P.valid[L.slot[A.valid_col[c]]], q);
+                u64 part;
+                if (fn == CQ_VM_COUNT) {
+                    part = (u64)__builtin_popcount(vm);
+                } else {
+                    u64 x[LIN_R];
+                    lin_chain(P, A, A.nsteps, wcols, q, pay, x);
+                    if (fn == CQ_VM_MIN) {
+                        part = ~0ull;
+#pragma unroll
+                        for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] < part) ? x[r] : part;
+                    } else if (fn == CQ_VM_MAX) {
+                        part = 0ull;
+#pragma unroll
+                        for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] > part) ? x[r] : part;
+                    } else {
+                        part = 0ull;
+#pragma unroll
+                        for (int r = 0; r < LIN_R; ++r) part += ((vm >> r) & 1u) ? x[r] : 0ull;
+                    }
+                }
+                if (fn == CQ_VM_MIN) racc[i] = part < racc[i] ? part : racc[i];
+                else if (fn == CQ_VM_MAX) racc[i] = part > racc[i] ? part : racc[i];
+                else racc[i] += part;
+                rseen |= vm ? 1u << i : 0u;
+                const u64 cnt = (u64)__builtin_popcount(vm);
+#pragma unroll
+                for (int k = 0; k < NR; ++k) racc[k] += k == hs ? cnt : 0ull;
+            }
+            continue;
+        }
+        // group index (Horner over dense domains, no modulo)
+        int h[LIN_R];
+        u32 keyok[CQ_LIN_KEYS] = {0u, 0u};
+        {
+            u64 hv[LIN_R];
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) hv[r] = 0ull;
+#pragma unroll
+            for (int g = 0; g < CQ_LIN_KEYS; ++g) {
+                if (g >= ng) break;
+                const cq_lin_key& K = L.key[g];
+                int kv[LIN_R];
+                if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
+                else
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
+                keyok[g] = K.valid_col >= 0 ? lin_valid(P.valid[L.slot[K.valid_col]], q) : q.inm;
+                const u64 mn = (u64)V->key_min[g], card = (u64)V->key_card[g];
+                const bool nl = V->key_nullable[g] != 0;
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) {
+                    const u64 v = (u64)(i64)kv[r];
+                    const u64 term = nl ? (((keyok[g] >> r) & 1u) ? v - mn + 1ull : 0ull) : v - mn;
+                    hv[r] = g == 0 ? term : hv[r] * card + term;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) {
+                h[r] = (int)(u32)hv[r];
+                if ((unsigned)h[r] >= (unsigned long long)G) alive &= ~(1u << r);
+            }
+        }
+        if (__ballot(alive != 0u) == 0ull) continue;
+        if (!lds_groups) {
+            // group rows in global memory: keys and their validity
+#pragma unroll
+            for (int g = 0; g < CQ_LIN_KEYS; ++g) {
+                if (g >= ng) break;
+                const cq_lin_key& K = L.key[g];
+                int kv[LIN_R];
+                if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
+                else
+#pragma unroll
+                    for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) {
+                    if (!((alive >> r) & 1u)) continue;
+                    const long long out = (long long)h[r] * sts;
+                    P.result[out + g] = (u64)(i64)kv[r];
+                    if (!fast_sparse && ((keyok[g] >> r) & 1u)) set_bit_once(P.result_valid, out + g);
+                }
+            }
+        }
+        // slots that turn valid, per aggregate: COUNT on every hit, others
+        // on a valid input (AVG keeps its count instead)
+        u32 slotbits[CQ_LIN_AGGS] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int i = 0; i < CQ_LIN_AGGS; ++i) {
+            if (i >= naggs) break;
+            const cq_lin_agg& A = L.agg[i];
+            const int fn = V->agg_fn[i];
+            const int hs = V->agg_hidden[i];
+            u32 vm = alive;
+            for (int c = 0; c < A.nvalid; ++c) vm &= lin_valid(P.valid[L.slot[A.valid_col[c]]], q);
+            u64 x[LIN_R];
+            lin_chain(P, A, fn == CQ_VM_COUNT ? 0 : A.nsteps, wcols, q, pay, x);
+            const long long stride = lds_groups ? nas : sts;
+            lin_group_atomics(lds_groups ? s_agg + i : P.result + ng + i, stride, h, x, vm, fn);
+            if (hs >= 0) {
+                u64 one[LIN_R];
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) one[r] = 1ull;
+                lin_group_atomics(lds_groups ? s_agg + (hs - ng) : P.result + hs, stride, h, one, vm, CQ_VM_SUM);
+            }
+            slotbits[i] = fast_sparse ? 0u : (fn == CQ_VM_COUNT ? alive : (hs >= 0 ? 0u : vm));
+            if (!lds_groups && slotbits[i]) {
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r)
+                    if ((slotbits[i] >> r) & 1u) set_bit_once(P.result_valid, (long long)h[r] * sts + ng + i);
+            }
+        }
+        if (lds_groups) {
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) {
+                if (!((alive >> r) & 1u)) continue;
+                u32 want = 0x80000000u;
+                if (!fast_sparse) {
+#pragma unroll
+                    for (int g = 0; g < CQ_LIN_KEYS; ++g)
+                        if (g < ng) want |= ((keyok[g] >> r) & 1u) << g;
+#pragma unroll
+                    for (int k = 0; k < CQ_LIN_AGGS; ++k) want |= ((slotbits[k] >> r) & 1u) << (ng + k);
+                }
+                if ((s_flag[h[r]] & want) != want) atomicOr(&s_flag[h[r]], want);
+            }
+        }
+    }
+    agg_flush<LIN_BT>(P, V, lds_groups, s_acc, s_seen, s_agg, s_flag, racc, rseen);
+}
+
+// lds_bytes: staged columns and (group_bytes of them) LDS group aggregates
+template <bool GROUPED>
+hipError_t lin_launch(const cq_query_params* P, const cq_vm_program* V, unsigned lds_bytes, unsigned group_bytes, int device,
+                      hipStream_t s) {
+    const long long rows = P->row_end - P->row_begin;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_lin_scan<GROUPED>), LIN_BT, lds_bytes) !=
+            hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
+    const long long tiles = (rows + (long long)LIN_BT * LIN_R - 1) / ((long long)LIN_BT * LIN_R);  // blocks' worth of wave tiles
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, (long long)per_cu * cus));
+    hipLaunchKernelGGL(k_lin_scan<GROUPED>, dim3(grid), dim3(LIN_BT), lds_bytes, s, *P, V, group_bytes ? 1 : 0);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -828,6 +1249,28 @@ bool cq_vm_scan4_supported(const cq_vm_program* V) {
         if (op == CQ_OP_DCOL || op == CQ_OP_DVALID || op == CQ_OP_SLOT || op == CQ_OP_SLOTV) return false;
     }
     return true;
+}
+
+int cq_lin_row_align() { return LIN_R; }
+
+bool cq_lin_supported(const cq_vm_program* V) {
+    const cq_lin_program& L = V->lin;
+    if (!L.enabled || L.ncols < 1 || L.ncols > CQ_LIN_COLS || L.nterms > CQ_LIN_TERMS || L.naggs != V->naggs || L.naggs > CQ_LIN_AGGS)
+        return false;
+    if (L.ngroups != V->ngroups || L.ngroups > CQ_LIN_KEYS || (V->ngroups == 0 && V->storage_ts > CQ_LIN_RACC)) return false;
+    if (L.nfilter_cols < 0 || L.nfilter_cols > L.ncols || L.join_col < 0 || L.join_col >= L.ncols) return false;
+    if (L.join >= 0 && (L.join >= V->njoins || V->joins[L.join].layout != CQ_L_DENSE)) return false;
+    return true;
+}
+
+hipError_t cq_lin_launch_scan(const cq_query_params* P, const cq_vm_program* V, const cq_vm_program* host, unsigned lds_group_bytes,
+                              int device, hipStream_t s) {
+    const long long rows = P->row_end - P->row_begin;
+    if (rows <= 0) return hipSuccess;
+    if (!cq_lin_supported(host)) return hipErrorInvalidValue;
+    const unsigned stage = (unsigned)(LIN_BT / 64) * (unsigned)host->lin.ncols * LIN_COL_INTS * 4u;
+    if (host->ngroups > 0) return lin_launch<true>(P, V, stage + lds_group_bytes, lds_group_bytes, device, s);
+    return lin_launch<false>(P, V, stage, 0u, device, s);
 }
 
 hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,

@@ -458,6 +458,12 @@ void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
     if (p.vm) {
         const long long rows = run.P.row_end - run.P.row_begin;
         if (rows <= 0) return;
+        if (!ctx->config.vm_row_kernel && ctx->config.vm_linear && cq_lin_supported(p.vm.get()) &&
+            run.P.row_begin % cq_lin_row_align() == 0) {
+            hipCheck(cq_lin_launch_scan(&run.P, run.vm, p.vm.get(), static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
+                     "interpreted scan (linear form)");
+            return;
+        }
         if (!ctx->config.vm_row_kernel && cq_vm_scan4_supported(p.vm.get()) && run.P.row_begin % 4 == 0) {
             hipCheck(cq_vm_launch_scan4(&run.P, run.vm, p.vm->nregs, static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
                      "interpreted scan");

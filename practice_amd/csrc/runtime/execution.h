@@ -121,6 +121,7 @@ struct EngineConfig {
     double memory_guard_reserve_fraction = 0.10;
     bool kernel_timing = false;          // record cq_scan duration with events
     bool vm_row_kernel = false;          // interpreted tier: always the one-row-per-thread scan
+    bool vm_linear = true;               // interpreted tier: the linear-form kernel for plans that have one
     long long host_sort_rows = 4096;     // ORDER BY results up to this size are sorted on the host
     bool prepared_cache = false;         // DatabaseInstance::prepare goes through QueryEngine::prepareCached
     CodegenOptions codegen;

@@ -88,6 +88,13 @@ def adhoc_query(rng) -> str:
             + " and ".join(preds) + " group by d_year")
 
 
+# two ad-hoc shapes of the served mix (adhoc_query), for timing the
+# interpreted tier against compiled modules
+ADHOC_1 = ("select count(*), sum(lo_extendedprice + lo_revenue) from lineorder where lo_quantity < 20 "
+           "and lo_discount > 3 and lo_tax <> 5")
+ADHOC_2 = ("select d_year, max(lo_revenue), min(lo_supplycost) from lineorder, ddate where lo_orderdate = d_datekey "
+           "and lo_revenue >= 500000 and lo_extendedprice < 800000 group by d_year")
+
 # OR-key join: a fact row counts once per distinct date key (order / commit)
 # that names a DDATE row
 ORKEY = ("select d_year, count(*), sum(lo_revenue) from lineorder, ddate where lo_orderdate = d_datekey "
