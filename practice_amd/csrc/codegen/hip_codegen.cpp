@@ -1863,6 +1863,10 @@ private:
             L.term_end[c] = t;
         }
         if (L.join == 0 && (L.join_col = pos(prog_.joins[0].fk_col)) < 0) return;
+        // a small table is probed from LDS (the budget is what is left
+        // beside the staged columns and LDS group aggregates)
+        if (L.join == 0 && opt_.allow_lds_tables && prog_.joins[0].ht_words <= CQ_LIN_LDS_TABLE_WORDS)
+            L.join_lds_words = static_cast<int>(prog_.joins[0].ht_words);
         // one operand: a fact column, the join's payload or (with `litc`)
         // the next literal of the expression, in the type lit() gave it
         auto operand = [&](const ExprNode* e, cq_lin_step& st, int* litc) -> bool {

@@ -82,6 +82,7 @@ enum cq_vm_agg_fn { CQ_VM_SUM = 0, CQ_VM_MIN = 1, CQ_VM_MAX = 2, CQ_VM_COUNT = 3
 #define CQ_LIN_SRCS 4
 #define CQ_LIN_KEYS 2
 #define CQ_LIN_RACC 8    // scalar result slots (aggregates + AVG counts)
+#define CQ_LIN_LDS_TABLE_WORDS 4096  // largest join table probed from LDS (16 KiB)
 struct cq_lin_term {
     int col;   // column position (cq_lin_program.slot)
     int op;    // CQ_OP_EQ .. CQ_OP_GE
@@ -118,6 +119,7 @@ struct cq_lin_program {
     struct cq_lin_term term[CQ_LIN_TERMS];
     int join;                           // -1: none; else the join (key-indexed layout) probed
     int join_col;                       // position of its fact key
+    int join_lds_words;                 // > 0: its table (this many words) is copied to LDS and probed there
     int naggs;                          // = cq_vm_program.naggs
     struct cq_lin_agg agg[CQ_LIN_AGGS];
     int ngroups;

@@ -889,6 +889,9 @@ __device__ __forceinline__ u32 lin_valid(const u32* vb, const LinRows& q) {
     return m & q.inm;
 }
 
+// validity bits of column position p from the tile's packed bits
+__device__ __forceinline__ u32 lin_bits(u64 vbits, int p) { return (u32)(vbits >> (LIN_R * p)) & LIN_ALL; }
+
 template <bool ALIGNED>
 __device__ __forceinline__ u32 lin_probe(const cq_vm_join& J, const u32* ht, const int (&fk)[LIN_R], u32 alive, int (&pay)[LIN_R]) {
 #pragma unroll
@@ -961,33 +964,37 @@ __device__ __forceinline__ void lin_chain(const cq_query_params& P, const cq_lin
 }
 
 // per-row atomics of one aggregate into group slots base[h * stride]
-// (rows of vm only; one branch on the function)
-__device__ __forceinline__ void lin_group_atomics(u64* base, long long stride, const int (&h)[LIN_R], const u64 (&x)[LIN_R], u32 vm,
-                                                  int fn) {
+// (rows of vm only; one branch on the function).  Callers pass an LDS or a
+// global pointer, never one that may be either, so the atomics are ds_* or
+// global_* instructions, not flat ones.
+template <typename Index>
+__device__ __forceinline__ void lin_group_atomics(u64* base, Index stride, const int (&h)[LIN_R], const u64 (&x)[LIN_R], u32 vm, int fn) {
     switch (fn) {
         case CQ_VM_MIN:
 #pragma unroll
             for (int r = 0; r < LIN_R; ++r)
-                if ((vm >> r) & 1u) atomicMin(&base[(long long)h[r] * stride], x[r]);
+                if ((vm >> r) & 1u) atomicMin(&base[(Index)h[r] * stride], x[r]);
             break;
         case CQ_VM_MAX:
 #pragma unroll
             for (int r = 0; r < LIN_R; ++r)
-                if ((vm >> r) & 1u) atomicMax(&base[(long long)h[r] * stride], x[r]);
+                if ((vm >> r) & 1u) atomicMax(&base[(Index)h[r] * stride], x[r]);
             break;
         default:
 #pragma unroll
             for (int r = 0; r < LIN_R; ++r)
-                if ((vm >> r) & 1u) atomicAdd(&base[(long long)h[r] * stride], x[r]);
+                if ((vm >> r) & 1u) atomicAdd(&base[(Index)h[r] * stride], x[r]);
             break;
     }
 }
 
 // GROUPED: GROUP BY plans (the scalar accumulators exist only without).
-// Dynamic LDS: per wave ncols staged columns (LIN_COL_INTS ints each), then
-// the group aggregates when lds_groups (G * nas u64 + G u32 flag words).
+// Dynamic LDS: per wave ncols staged columns (LIN_COL_INTS ints each), the
+// join table when it is probed from LDS (join_lds_words, rounded up to an
+// even count), then the group aggregates when lds_groups (G * nas u64 +
+// G u32 flag words).
 template <bool GROUPED>
-__global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
+__global__ __launch_bounds__(LIN_BT, GROUPED ? 3 : 4) void k_lin_scan(const cq_query_params P, const cq_vm_program* __restrict__ V, int lds_groups) {
     __shared__ u64 s_acc[CQ_VM_MAX_AGGS * 2];
     __shared__ u32 s_seen[CQ_VM_MAX_AGGS * 2];
     extern __shared__ u64 s_dyn[];
@@ -999,8 +1006,13 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
     const bool fast_sparse = V->fast_sparse != 0;
     const int nc = L.ncols, nterms = L.nterms, naggs = L.naggs;
     int* wcols = (int*)s_dyn + (tid >> 6) * nc * LIN_COL_INTS;  // this wave's staged columns
-    u64* s_agg = s_dyn + (long long)(LIN_BT / 64) * nc * LIN_COL_INTS / 2;
+    u32* s_table = (u32*)(s_dyn + (long long)(LIN_BT / 64) * nc * LIN_COL_INTS / 2);
+    const int tw = L.join >= 0 ? L.join_lds_words : 0;
+    u64* s_agg = (u64*)(s_table + ((tw + 1) & ~1));
     u32* s_flag = (u32*)(s_agg + (lds_groups ? G * nas : 0));
+    const u32* ht = L.join >= 0 ? P.ht[L.join] : nullptr;
+    // agg_init's barrier orders the copy before the probes
+    for (int k = tid; k < tw; k += LIN_BT) s_table[k] = ht[k];
     agg_init<LIN_BT>(V, lds_groups, s_acc, s_seen, s_agg, s_flag);
     u64 racc[NR];
     u32 rseen = 0u;
@@ -1015,12 +1027,20 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
     for (long long tile = (long long)blockIdx.x * (LIN_BT / 64) + (tid >> 6); tile < ntiles; tile += waves) {
         q.t0 = P.row_begin + tile * LIN_TILE;
         q.full = q.t0 + LIN_TILE <= P.row_end;
-        q.inm = 0u;
+        q.inm = LIN_ALL;
+        if (!q.full) {
+            q.inm = 0u;
 #pragma unroll
-        for (int b = 0; b < LIN_R; ++b) q.inm |= (u32)(lin_row(q, b) <= q.last) << b;
-        // every column of the plan into LDS, all in flight at once
+            for (int b = 0; b < LIN_R; ++b) q.inm |= (u32)(lin_row(q, b) <= q.last) << b;
+        }
+        // every column of the plan into LDS and the validity bits of its
+        // nullable columns (LIN_R bits per position), all in flight at once
         __builtin_amdgcn_s_waitcnt(0);  // the previous tile's LDS reads are done
-        for (int c = 0; c < nc; ++c) lin_stage(P.col[L.slot[c]], wcols + c * LIN_COL_INTS, q);
+        u64 vbits = 0ull;
+        for (int c = 0; c < nc; ++c) {
+            lin_stage(P.col[L.slot[c]], wcols + c * LIN_COL_INTS, q);
+            vbits |= (u64)(L.nullable[c] ? lin_valid(P.valid[L.slot[c]], q) : q.inm) << (LIN_R * c);
+        }
         __builtin_amdgcn_s_waitcnt(0);  // the staged columns have landed
         u32 alive = q.inm;
         // filters
@@ -1054,9 +1074,10 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
             const int jc = L.join_col;
             int fk[LIN_R];
             lin_read(wcols + jc * LIN_COL_INTS, q, fk);
-            if (L.nullable[jc]) alive &= lin_valid(P.valid[L.slot[jc]], q);
-            const u32* ht = P.ht[L.join];
-            alive = (J.db & (J.db - 1)) == 0 ? lin_probe<true>(J, ht, fk, alive, pay) : lin_probe<false>(J, ht, fk, alive, pay);
+            alive &= lin_bits(vbits, jc);
+            const bool al = (J.db & (J.db - 1)) == 0;
+            if (tw > 0) alive = al ? lin_probe<true>(J, s_table, fk, alive, pay) : lin_probe<false>(J, s_table, fk, alive, pay);
+            else alive = al ? lin_probe<true>(J, ht, fk, alive, pay) : lin_probe<false>(J, ht, fk, alive, pay);
             if (__ballot(alive != 0u) == 0ull) continue;
         }
         if (!GROUPED) {
@@ -1069,7 +1090,7 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
                 const int fn = V->agg_fn[i];
                 const int hs = V->agg_hidden[i];
                 u32 vm = alive;
-                for (int c = 0; c < A.nvalid; ++c) vm &= lin_valid(P.valid[L.slot[A.valid_col[c]]], q);
+                for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
                 u64 part;
                 if (fn == CQ_VM_COUNT) {
                     part = (u64)__builtin_popcount(vm);
@@ -1100,13 +1121,15 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
             }
             continue;
         }
-        // group index (Horner over dense domains, no modulo)
+        // group index (Horner over dense domains, no modulo), in 32 bits:
+        // the index is the low word of the u64 Horner value, and add, sub
+        // and mul commute with taking the low word
         int h[LIN_R];
         u32 keyok[CQ_LIN_KEYS] = {0u, 0u};
         {
-            u64 hv[LIN_R];
+            u32 hv[LIN_R];
 #pragma unroll
-            for (int r = 0; r < LIN_R; ++r) hv[r] = 0ull;
+            for (int r = 0; r < LIN_R; ++r) hv[r] = 0u;
 #pragma unroll
             for (int g = 0; g < CQ_LIN_KEYS; ++g) {
                 if (g >= ng) break;
@@ -1116,19 +1139,19 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
                 else
 #pragma unroll
                     for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
-                keyok[g] = K.valid_col >= 0 ? lin_valid(P.valid[L.slot[K.valid_col]], q) : q.inm;
-                const u64 mn = (u64)V->key_min[g], card = (u64)V->key_card[g];
+                keyok[g] = K.valid_col >= 0 ? lin_bits(vbits, K.valid_col) : q.inm;
+                const u32 mn = (u32)V->key_min[g], card = (u32)V->key_card[g];
                 const bool nl = V->key_nullable[g] != 0;
 #pragma unroll
                 for (int r = 0; r < LIN_R; ++r) {
-                    const u64 v = (u64)(i64)kv[r];
-                    const u64 term = nl ? (((keyok[g] >> r) & 1u) ? v - mn + 1ull : 0ull) : v - mn;
+                    const u32 v = (u32)kv[r];
+                    const u32 term = nl ? (((keyok[g] >> r) & 1u) ? v - mn + 1u : 0u) : v - mn;
                     hv[r] = g == 0 ? term : hv[r] * card + term;
                 }
             }
 #pragma unroll
             for (int r = 0; r < LIN_R; ++r) {
-                h[r] = (int)(u32)hv[r];
+                h[r] = (int)hv[r];
                 if ((unsigned)h[r] >= (unsigned long long)G) alive &= ~(1u << r);
             }
         }
@@ -1163,16 +1186,18 @@ __global__ __launch_bounds__(LIN_BT) void k_lin_scan(const cq_query_params P, co
             const int fn = V->agg_fn[i];
             const int hs = V->agg_hidden[i];
             u32 vm = alive;
-            for (int c = 0; c < A.nvalid; ++c) vm &= lin_valid(P.valid[L.slot[A.valid_col[c]]], q);
+            for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
             u64 x[LIN_R];
             lin_chain(P, A, fn == CQ_VM_COUNT ? 0 : A.nsteps, wcols, q, pay, x);
-            const long long stride = lds_groups ? nas : sts;
-            lin_group_atomics(lds_groups ? s_agg + i : P.result + ng + i, stride, h, x, vm, fn);
-            if (hs >= 0) {
-                u64 one[LIN_R];
+            u64 one[LIN_R];
 #pragma unroll
-                for (int r = 0; r < LIN_R; ++r) one[r] = 1ull;
-                lin_group_atomics(lds_groups ? s_agg + (hs - ng) : P.result + hs, stride, h, one, vm, CQ_VM_SUM);
+            for (int r = 0; r < LIN_R; ++r) one[r] = 1ull;
+            if (lds_groups) {
+                lin_group_atomics<int>(s_agg + i, nas, h, x, vm, fn);
+                if (hs >= 0) lin_group_atomics<int>(s_agg + (hs - ng), nas, h, one, vm, CQ_VM_SUM);
+            } else {
+                lin_group_atomics<long long>(P.result + ng + i, (long long)sts, h, x, vm, fn);
+                if (hs >= 0) lin_group_atomics<long long>(P.result + hs, (long long)sts, h, one, vm, CQ_VM_SUM);
             }
             slotbits[i] = fast_sparse ? 0u : (fn == CQ_VM_COUNT ? alive : (hs >= 0 ? 0u : vm));
             if (!lds_groups && slotbits[i]) {
@@ -1259,6 +1284,7 @@ bool cq_lin_supported(const cq_vm_program* V) {
         return false;
     if (L.ngroups != V->ngroups || L.ngroups > CQ_LIN_KEYS || (V->ngroups == 0 && V->storage_ts > CQ_LIN_RACC)) return false;
     if (L.nfilter_cols < 0 || L.nfilter_cols > L.ncols || L.join_col < 0 || L.join_col >= L.ncols) return false;
+    if (L.join_lds_words < 0 || L.join_lds_words > CQ_LIN_LDS_TABLE_WORDS) return false;
     if (L.join >= 0 && (L.join >= V->njoins || V->joins[L.join].layout != CQ_L_DENSE)) return false;
     return true;
 }
@@ -1269,8 +1295,9 @@ hipError_t cq_lin_launch_scan(const cq_query_params* P, const cq_vm_program* V, 
     if (rows <= 0) return hipSuccess;
     if (!cq_lin_supported(host)) return hipErrorInvalidValue;
     const unsigned stage = (unsigned)(LIN_BT / 64) * (unsigned)host->lin.ncols * LIN_COL_INTS * 4u;
-    if (host->ngroups > 0) return lin_launch<true>(P, V, stage + lds_group_bytes, lds_group_bytes, device, s);
-    return lin_launch<false>(P, V, stage, 0u, device, s);
+    const unsigned table = host->lin.join >= 0 ? (unsigned)((host->lin.join_lds_words + 1) & ~1) * 4u : 0u;
+    if (host->ngroups > 0) return lin_launch<true>(P, V, stage + table + lds_group_bytes, lds_group_bytes, device, s);
+    return lin_launch<false>(P, V, stage + table, 0u, device, s);
 }
 
 hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,
