@@ -11,14 +11,17 @@ value = algorithmic bytes of the query / ms per step, where
                     + 4 B x rows x touched dimension columns.
 roofline: the fused scan kernel alone (fact bytes / its HIP-event duration on
 the query stream) against the 8 TB/s HBM peak.
-cpu_baseline: the CPU oracle (C restatement of the reference kernels,
-OpenMP) on a bounded sample of the same workload.
+cpu_baseline: a tiled C/OpenMP restatement of the kernels the reference
+generates for the query (oracle/ssb_tiled.c), on a bounded sample of the same
+workload, on all the process's cores and on one.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -45,7 +48,10 @@ def parse_args():
                    help="database option (cq_db_set_option) for the timed query, repeatable")
     p.add_argument("--seed", type=int, default=42)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
+    p.add_argument("--cpu-sample-rows", type=int, default=120_000_000,
+                   help="fact rows of the all-core CPU baseline sample (capped at the configuration's size)")
+    p.add_argument("--cpu-one-core-rows", type=int, default=20_000_000,
+                   help="fact rows of the one-core CPU baseline sample")
     p.add_argument("--extra", action="store_true", help="also time the other plain configurations")
     p.add_argument("--no-batched", action="store_true", help="skip the batched shared-scan measurement")
     p.add_argument("--no-served", action="store_true", help="skip the serving measurement")
@@ -61,49 +67,69 @@ def parse_args():
     return p.parse_args()
 
 
-def cpu_baseline(workload, sql_text, sample_rows, seed):
-    from oracle import oracle as orc
-    import practice_amd as pa
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    sf, _, nullable, _ = WORKLOADS[workload]
-    spec = orc.GenSpec(sf=sf, fact_rows=sample_rows, seed=seed, nullable=nullable)
-    # plan and touched columns from a host-only catalog of the sample
-    hdb = pa.Database(-1)
-    # generate only the columns the query reads
-    touched = {"LINEORDER": [], "PART": [], "SUPPLIER": [], "CUSTOMER": [], "DDATE": []}
-    words = sql_text.replace(",", " ").replace("(", " ").replace(")", " ").split()
-    for t in touched:
-        for c in orc.table_columns(t):
-            if c in words:
-                touched[t].append(c)
-    touched = {t: cs for t, cs in touched.items() if cs}
-    for t in touched:
-        if t in orc.SSB_PK and orc.SSB_PK[t] not in touched[t]:
-            touched[t].append(orc.SSB_PK[t])
-    data = orc.gen_tables(spec, tables=list(touched), columns=touched)
-    odb = orc.load_ssb(data, spec)
-    for t, cols in data.items():
-        hdb.create_table(t, spec.rows(t), t == "LINEORDER")
-        for c, (v, m) in cols.items():
-            hdb.load_column(t, c, v, m)
-        if t in orc.SSB_PK:
-            hdb.set_primary_key(t, orc.SSB_PK[t])
-    hdb.finalize()
-    plan = hdb.explain(sql_text)
-    desc = hdb.describe(sql_text)
-    threads = min(16, os.cpu_count() or 1)
-    odb.execute(plan, threads)  # warm-up
-    times = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        odb.execute(plan, threads)
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
-    gbs = desc["algorithmic_bytes"] / t / 1e9
-    hdb.close()
-    return {"value": round(gbs, 3), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{workload.upper()} query over a {sample_rows:,}-row fact sample (same generator, scale-{sf} "
-                      f"dimensions), CPU oracle restatement with OpenMP, median of 3 runs: {t * 1000:.0f} ms"}
+
+def cpu_baseline(workload, sample_rows, one_core_rows, seed, runs=5):
+    """The tiled CPU restatement of the reference's generated kernels for this
+    configuration (oracle/ssb_tiled.c: 512-row tiles, int[2*len] hash tables,
+    Horner group hash, u64 atomics), OpenMP over the host's cores, on a
+    bounded fact sample of the same generator: one warm-up, then the median
+    of `runs` timed runs (dimension builds + fact scan), on every core the
+    process may use and on one core."""
+    from oracle import oracle as orc
+    from oracle import tiled
+
+    sf, full_rows, nullable, key = WORKLOADS[workload]
+    cols = tiled.COLUMNS[key]
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or nproc
+    out = {}
+    for label, rows, th in (("all", sample_rows, threads), ("one", one_core_rows, 1)):
+        spec = orc.GenSpec(sf=sf, fact_rows=rows, seed=seed, nullable=nullable)
+        data = orc.gen_tables(spec, tables=list(cols), columns=cols)
+
+        def stats(column, data=data):
+            for t, cs in data.items():
+                if column in cs:
+                    v, m = cs[column]
+                    if m is not None:
+                        ok = np.unpackbits(m.view(np.uint8), bitorder="little")[:len(v)].astype(bool)
+                        v = v[ok]
+                    return {"min": int(v.min()), "card": int(len(np.unique(v))), "nullable": m is not None}
+            raise KeyError(column)
+
+        fact = data["LINEORDER"]
+        nbytes = 4 * rows * len(fact) + 8 * ((rows + 63) // 64) * sum(m is not None for _, m in fact.values())
+        nbytes += sum(4 * len(v) for t, cs in data.items() if t != "LINEORDER" for v, _ in cs.values())
+        tiled.run(key, data, stats, th)  # warm-up
+        times = []
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            tiled.run(key, data, stats, th)
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[len(times) // 2]
+        out[label] = {"rows": rows, "threads": th, "ms": t * 1000, "GBps": nbytes / t / 1e9,
+                      "rows_per_s": rows / t}
+        del data
+    a, o = out["all"], out["one"]
+    return {"value": round(a["GBps"], 3), "unit": "GB/s", "cores": a["threads"], "kind": "port",
+            "sample": (f"{key} over a {a['rows']:,}-row fact sample of the {full_rows:,}-row configuration (same "
+                       f"generator, scale-{sf} dimensions): tiled C/OpenMP restatement of the reference's generated "
+                       f"kernels (not AdaptiveCpp), dimension builds + fact scan, median of {runs} runs after a "
+                       f"warm-up: {a['ms']:.0f} ms on {a['threads']} threads ({a['rows_per_s'] / 1e6:.0f} M fact "
+                       f"rows/s)"),
+            "one_core": {"value": round(o["GBps"], 3), "rows": o["rows"], "ms": round(o["ms"], 1),
+                         "rows_per_s": round(o["rows_per_s"])},
+            "host": {"cpu_model": _cpu_model(), "nproc": nproc, "os_cpu_count": os.cpu_count(),
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}}
 
 
 def same_rows(a, b):
@@ -398,7 +424,8 @@ def main():
     if not args.no_served:
         line["served"] = bench_served(pa, Q, args)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.workload, sql, min(args.cpu_sample_rows, rows), args.seed)
+        line["cpu_baseline"] = cpu_baseline(args.workload, min(args.cpu_sample_rows, rows),
+                                            min(args.cpu_one_core_rows, rows), args.seed)
     print(json.dumps(line), flush=True)
 
 
