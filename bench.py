@@ -329,11 +329,14 @@ def main():
     for _ in range(args.warmup):
         db.execute(sql)
     scan_ms = []
+    parts = {k: 0.0 for k in ("parse_ms", "codegen_ms", "gpu_execute_ms", "host_fetch_ms", "engine_ms")}
     t0 = time.perf_counter()
     res = None
     for _ in range(args.steps):
         res = db.execute(sql)
         scan_ms.append(res.timing["scan_kernel_ms"])
+        for k in parts:
+            parts[k] += res.timing[k]
     elapsed = time.perf_counter() - t0
     ms_per_step = elapsed * 1000.0 / args.steps
     gbs = desc["algorithmic_bytes"] / (ms_per_step / 1000.0) / 1e9
@@ -368,6 +371,7 @@ def main():
                    "fact_rows": rows, "groups": desc["groups"], "algorithmic_bytes": desc["algorithmic_bytes"],
                    "result_rows": res.rows, "block_threads": desc["block_threads"],
                    "lds_bytes": desc["scan_lds_bytes"], "generate_s": round(gen_s, 2)},
+        "step_ms_breakdown": {k: round(v / args.steps, 4) for k, v in parts.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "cq_scan", "kernel_ms": round(scan_avg, 4), "kernel_bytes": fact_bytes},

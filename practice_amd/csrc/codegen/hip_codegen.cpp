@@ -1262,10 +1262,14 @@ private:
         }
         bool lds_agg = grouped && opt_.allow_lds_aggregation && !batch && flag_w_ > 0 && prog_.storage_ts <= 64;
         for (size_t g = 0; g < gdense_.size(); ++g) lds_agg = lds_agg && gdense_[g];
-        const uint64_t agg_lds = grouped ? aggLdsBytes() : 0;
+        // scalar aggregates: per-wave partials of the block reduction (16
+        // waves at most, 8 + 4 bytes per aggregate and hidden count)
+        const uint64_t agg_lds = grouped ? aggLdsBytes() : static_cast<uint64_t>(16 * 12 * 2 * std::max(1, prog_.naggs));
         if (agg_lds > opt_.lds_budget) lds_agg = false;
         prog_.lds_aggregation = lds_agg;
         uint64_t lds_used = lds_agg ? agg_lds : 0;
+        const uint64_t red_lds = grouped ? 0 : agg_lds;  // outside the launch-shape choice below
+        lds_used += red_lds;
         // LDS placement of the dimension tables: the subset that avoids the
         // most HBM-side gathers within the budget (tables beyond the 32 KiB
         // L1 weigh more: their probes go to L2), then the final layouts
@@ -1302,7 +1306,7 @@ private:
         for (int j = 0; j < nj; ++j) emitBuild(src, j);
         prog_.scan_lds_bytes = lds_used;
         int bt = opt_.block_threads;
-        if (bt <= 0) bt = lds_used > 40 * 1024 ? 1024 : (lds_used > 0 ? 512 : 256);
+        if (bt <= 0) bt = lds_used - red_lds > 40 * 1024 ? 1024 : (lds_used - red_lds > 0 ? 512 : 256);
         prog_.block_threads = bt;
         const int vpt = std::max(1, std::min(8, opt_.vectors_per_thread));
         prog_.vectors_per_thread = vpt;
@@ -1320,7 +1324,41 @@ private:
         }
         const int last = multi >= 0 ? multi : mhtj;
         std::vector<Op> dependent;
-        for (const Op& op : pipelineOrder()) {
+        const std::vector<Op> order = pipelineOrder();
+        // Key-indexed HBM probes whose earlier probes remove no rows (no
+        // build filter): their loads are issued with the first probe, so the
+        // L2 round trip overlaps the LDS probes in between.  Rows are only
+        // ever removed, so loading for rows an LDS probe drops later reads
+        // entries nobody uses and changes no result.
+        uint32_t hoisted = 0;
+        if (opt_.hoist_gathers && !opt_.batch_body && !opt_.vm) {
+            bool ahead_keeps_all = true;
+            size_t probes_before = 0;
+            for (const Op& op : order) {
+                if (op.kind != 1 || op.join == last) continue;
+                const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
+                if (jp.dense && !jp.lds && useRsrc(jp) && ahead_keeps_all && probes_before > 0)
+                    hoisted |= 1u << op.join;
+                ahead_keeps_all = ahead_keeps_all && build_pred_[static_cast<size_t>(op.join)] == nullptr;
+                ++probes_before;
+            }
+        }
+        auto probeArgs = [&](const JoinProgram& jp, std::string& key, std::string& kv) {
+            const int fs = factSlot(jp.fk_col);
+            key = "c" + std::to_string(fs) + "[i]";
+            kv = prog_.fact_nullable[static_cast<size_t>(fs)] ? "((v" + std::to_string(fs) + " >> i) & 1u)" : "";
+        };
+        bool issued = false;
+        for (const Op& op : order) {
+            if (op.kind == 1 && hoisted && !issued) {
+                for (size_t j = 0; j < prog_.joins.size(); ++j) {
+                    if (!((hoisted >> j) & 1u)) continue;
+                    std::string key, kv;
+                    probeArgs(prog_.joins[j], key, kv);
+                    emitProbe(body, static_cast<int>(j), key, kv, 1);
+                }
+                issued = true;
+            }
             if (last >= 0 && op.kind == 1 && op.join == last) continue;
             if (last >= 0 && op.kind == 0) {
                 std::vector<std::string> cols;
@@ -1335,10 +1373,9 @@ private:
             if (op.kind == 0) {
                 emitFilterOp(body, op);
             } else {
-                const JoinProgram& jp = prog_.joins[static_cast<size_t>(op.join)];
-                const int fs = factSlot(jp.fk_col);
-                emitProbe(body, op.join, "c" + std::to_string(fs) + "[i]",
-                          prog_.fact_nullable[static_cast<size_t>(fs)] ? "((v" + std::to_string(fs) + " >> i) & 1u)" : "");
+                std::string key, kv;
+                probeArgs(prog_.joins[static_cast<size_t>(op.join)], key, kv);
+                emitProbe(body, op.join, key, kv, ((hoisted >> op.join) & 1u) ? 2 : 0);
             }
         }
         if (multi >= 0) {
@@ -1449,7 +1486,7 @@ private:
         src << prologue.str();
         src << "    __syncthreads();\n";
         const size_t nf = prog_.fact_cols.size();
-        if (!opt_.prefetch) {
+        if (!opt_.prefetch || nf * static_cast<size_t>(ipt) > 48) {
             src << "    for (long long tile = blockIdx.x; tile < P.num_tiles; tile += gridDim.x) {\n";
             src << "        const long long base = P.row_begin + tile * (long long)TILE;\n";
             for (size_t s = 0; s < nf; ++s) {
@@ -1698,9 +1735,38 @@ private:
 
     // Probe of join `join` with the per-row key expression `key` (of item i);
     // keyValid: the key's validity bit expression, "" when it cannot be NULL.
-    void emitProbe(std::ostringstream& body, int join, const std::string& key, const std::string& keyValid) {
+    // part 1 emits only the locate and load loops of a key-indexed HBM
+    // probe (hoisted ahead of LDS work), part 2 the decode that completes it.
+    void emitProbe(std::ostringstream& body, int join, const std::string& key, const std::string& keyValid, int part = 0) {
         const JoinProgram& jp = prog_.joins[static_cast<size_t>(join)];
         const std::string j = std::to_string(join);
+        if (part != 0) {
+            const std::string al = (jp.db & (jp.db - 1)) == 0 ? "true" : "false";
+            if (part == 1) {
+                std::string live = flGet();
+                if (!keyValid.empty()) live = "(" + live + " && " + keyValid + ")";
+                body << "        // " << jp.dim_table << " on " << jp.fk_col << ": table loads issued ahead of the LDS probes\n"
+                     << "        u32 q" << j << "[IPT], x" << j << "[IPT], l" << j << "[IPT], h" << j << "[IPT];\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                     << "            q" << j << "[i] = chip::dense_locate(" << key << ", " << live << ", " << jp.dmin << ", "
+                     << jp.dspan << ", " << jp.zmode << ", " << jp.zx << ", " << jp.cal << ", x" << j << "[i]);\n"
+                     << "#pragma unroll\n        for (int i = 0; i < IPT; ++i)\n"
+                     << "            chip::dense_words_rsrc<" << al << ">(rt" << j << ", x" << j << "[i], " << jp.db << ", l" << j
+                     << "[i], h" << j << "[i]);\n";
+                return;
+            }
+            body << "        // probe " << jp.dim_table << " on " << jp.fk_col << " (loads issued above)\n"
+                 << "        int p" << j << "[IPT];\n"
+                 << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                 << "            p" << j << "[i] = 0;\n"
+                 << "            if (!chip::dense_decode<" << al << ", " << jp.zmode << ">(q" << j << "[i], x" << j << "[i], l" << j << "[i], h" << j
+                 << "[i], " << jp.db << ", " << densePbase(jp) << ", " << (jp.doff ? "true" : "false") << ", p" << j
+                 << "[i])) " << flClear() << "\n"
+                 << "        }\n"
+                 << "        if (__ballot(" << flAny() << ") == 0ull) continue;\n";
+            visible_mask_ |= 1u << join;
+            return;
+        }
         const std::string table = jp.lds ? "s_ht" + j : "P.ht[" + j + "]";
         // tables beyond L1 (32 KiB) that stay in HBM may be read with
         // streaming loads (L1 keeps the small ones)
@@ -2798,23 +2864,48 @@ private:
             code << "            }\n";
         }
         code << "        }\n";
-        epi << "    const int lane = tid & 63;\n";
+        // one atomic per block and aggregate: wave reductions, then the
+        // first wave reduces the waves' partials (same-address global atomics
+        // from every wave serialize in L2); identity partials are skipped
+        // (the slot already holds the identity: 0, or ~0 for MIN)
+        int nslots = na;
+        for (int i = 0; i < na; ++i) nslots += prog_.hidden_slot[static_cast<size_t>(i)] >= 0 ? 1 : 0;
+        epi << "    const int lane = tid & 63, wid = tid >> 6;\n"
+            << "    __shared__ u64 s_red[" << nslots << "][BT / 64];\n"
+            << "    __shared__ u32 s_any[" << na << "][BT / 64];\n";
+        int hs = na;
+        for (int i = 0; i < na; ++i) {
+            const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
+            const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
+            epi << "    { const u64 w = " << red << "(acc" << i << "); const u32 any = __ballot(seen" << i << " != 0u) != 0ull ? 1u : 0u;\n"
+                << "      if (lane == 0) { s_red[" << i << "][wid] = w; s_any[" << i << "][wid] = any; } }\n";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
+                epi << "    { const u64 c = wave_sum(cnt" << i << "); if (lane == 0) s_red[" << hs++ << "][wid] = c; }\n";
+        }
+        epi << "    __syncthreads();\n"
+            << "    if (wid == 0) {\n";
+        hs = na;
         for (int i = 0; i < na; ++i) {
             const std::string& f = prog_.agg_funcs[static_cast<size_t>(i)];
             const char* red = f == "MIN" ? "wave_min" : (f == "MAX" ? "wave_max" : "wave_sum");
             const char* atom = f == "MIN" ? "atomicMin" : (f == "MAX" ? "atomicMax" : "atomicAdd");
-            epi << "    { const u64 w = " << red << "(acc" << i << "); const bool any = __ballot(seen" << i << " != 0u) != 0ull;\n"
-                << "      if (lane == 0) { " << atom << "(&P.result[" << i << "], w);";
-            if (f == "COUNT") epi << " }\n";
-            else if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) epi << " }\n";
-            else epi << " if (any) chip::set_bit64(P.result_valid, " << i << "); }\n";
-            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0)
-                epi << "      { const u64 c = wave_sum(cnt" << i << "); if (lane == 0) atomicAdd(&P.result["
+            const char* idn = f == "MIN" ? "~0ull" : "0ull";
+            epi << "        { const u64 w = " << red << "(lane < BT / 64 ? s_red[" << i << "][lane] : " << idn << ");\n"
+                << "          const bool any = __ballot(lane < BT / 64 && s_any[" << i << "][lane] != 0u) != 0ull;\n"
+                << "          if (lane == 0 && w != " << idn << ") " << atom << "(&P.result[" << i << "], w);\n";
+            if (f != "COUNT" && prog_.hidden_slot[static_cast<size_t>(i)] < 0)
+                epi << "          if (lane == 0 && any) chip::set_bit64(P.result_valid, " << i << ");\n";
+            else
+                epi << "          (void)any;\n";
+            if (prog_.hidden_slot[static_cast<size_t>(i)] >= 0) {
+                epi << "          { const u64 c = wave_sum(lane < BT / 64 ? s_red[" << hs++ << "][lane] : 0ull); if (lane == 0 && c) atomicAdd(&P.result["
                     << prog_.hidden_slot[static_cast<size_t>(i)] << "], c); }\n";
-            epi << "    }\n";
+            }
+            epi << "        }\n";
             if (f == "COUNT")
-                epi << "    if (blockIdx.x == 0 && tid == 0 && P.row_end > P.row_begin) chip::set_bit64(P.result_valid, " << i << ");\n";
+                epi << "        if (blockIdx.x == 0 && lane == 0 && P.row_end > P.row_begin) chip::set_bit64(P.result_valid, " << i << ");\n";
         }
+        epi << "    }\n";
     }
 
     // scalar aggregates reduced per tile (batched shared scan)

@@ -35,10 +35,16 @@ struct CodegenOptions {
     bool allow_compact_tables = true;
     bool allow_dense_tables = true;   // key-indexed tables where the keys allow
     bool calendar_tables = true;      // key-indexed tables of yyyymmdd keys indexed by (year, month, day)
-    bool prefetch = false;            // load tile t+stride while tile t is processed
+    // load tile t + stride while tile t is processed (scan kernels whose
+    // prefetched columns take at most 48 registers per lane)
+    bool prefetch = true;
     bool stream_large_tables = false; // streaming loads for probes of HBM tables beyond L1 (measured 2x slower on C3: off)
     bool split_probes = true;         // probes as locate / load / decode loops (all of a thread's loads in flight at once)
     bool buffer_tables = true;        // key-indexed tables in HBM read through a buffer resource (32-bit offsets, range-checked)
+    // issue the loads of an HBM table's probe right after the fact filters,
+    // ahead of LDS probes that remove no rows, so the L2 round trip overlaps
+    // their work
+    bool hoist_gathers = true;
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;

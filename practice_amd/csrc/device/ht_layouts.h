@@ -252,6 +252,22 @@ __device__ __forceinline__ void ht_insert_dense(u32* w, int k, int dmin, int dsp
     const u32 x = dense_index(k, dmin, dspan, cal);
     if (x >= (u32)dspan || db == 0) return;
     const u32 c = (u32)(payload - pbase);
+    // Byte-, halfword- and word-wide entries own their bytes (the layout
+    // is used only for unique build keys, so one row writes each entry):
+    // plain stores, which neighbouring lanes with consecutive keys coalesce,
+    // instead of atomic ORs that serialize on a shared word.
+    if (db == 8) {
+        reinterpret_cast<unsigned char*>(w)[x] = (unsigned char)c;
+        return;
+    }
+    if (db == 16) {
+        reinterpret_cast<unsigned short*>(w)[x] = (unsigned short)c;
+        return;
+    }
+    if (db == 32) {
+        w[x] = c;
+        return;
+    }
     const u32 b = x * (u32)db;
     atomicOr(&w[b >> 5], c << (b & 31));
     if ((b & 31) + (u32)db > 32u) atomicOr(&w[(b >> 5) + 1], c >> (32u - (b & 31)));

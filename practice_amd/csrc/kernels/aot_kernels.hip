@@ -279,18 +279,50 @@ __global__ __launch_bounds__(SCAN_BT) void k_scan_add(u64* __restrict__ out, i64
         if (base + k < n) out[base + k] += add;
 }
 
-// dense / sparse write: rows of present groups, in group order
+// dense / sparse write: rows of present groups, in group order.  `offsets`
+// is the exclusive scan of the 0/1 `flags`, so the present groups of one
+// wave land on consecutive rows, which span at most two 64-bit validity
+// words: each column's validity bits are OR-reduced across the wave and
+// written with two atomics per wave (not one per cell on a shared word).
+__device__ __forceinline__ u64 wave_or64(u64 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+    return v;
+}
 __global__ __launch_bounds__(256) void k_write_rows(const u64* __restrict__ res, const u64* __restrict__ rv, i64 groups,
                                                     int vts, const u64* __restrict__ flags, const u64* __restrict__ offsets,
                                                     u32 nullable_mask, int use_validity, u64* __restrict__ out,
                                                     u64* __restrict__ out_valid, i64 cap_rows, i64 valid_words) {
-    for (i64 g = (i64)blockIdx.x * 256 + threadIdx.x; g < groups; g += (i64)gridDim.x * 256) {
-        if (!flags[g]) continue;
-        const i64 row = (i64)offsets[g];
+    const int lane = threadIdx.x & 63;
+    // wave-uniform trip count: every lane takes part in the reductions
+    for (i64 g0 = (i64)blockIdx.x * 256 + (threadIdx.x & ~63); g0 < groups; g0 += (i64)gridDim.x * 256) {
+        const i64 g = g0 + lane;
+        const bool present = g < groups && flags[g] != 0ull;
+        const u64 pm = __ballot(present);
+        if (pm == 0ull) continue;
+        const i64 row = present ? (i64)offsets[g] : 0;
+        const int first = __ffsll((unsigned long long)pm) - 1;
+        const i64 base = __shfl(row, first) & ~(i64)63;
+        const i64 off = row - base;
         for (int c = 0; c < vts; ++c) {
-            out[(i64)c * cap_rows + row] = res[g * vts + c];
-            const bool valid = !use_validity || !((nullable_mask >> c) & 1u) || bit64(rv, g * vts + c);
-            if (valid) set_bit64(out_valid + (i64)c * valid_words, row);
+            bool valid = false;
+            if (present) {
+                out[(i64)c * cap_rows + row] = res[g * vts + c];
+                valid = !use_validity || !((nullable_mask >> c) & 1u) || bit64(rv, g * vts + c);
+            }
+            u64* ov = out_valid + (i64)c * valid_words;
+            u64 lo = 0ull, hi = 0ull;
+            if (valid) {
+                if (off < 64) lo = 1ull << off;
+                else if (off < 128) hi = 1ull << (off - 64);
+                else set_bit64(ov, row);  // not reached when flags are 0/1
+            }
+            lo = wave_or64(lo);
+            hi = wave_or64(hi);
+            if (lane == 0) {
+                if (lo) atomicOr(&ov[base >> 6], lo);
+                if (hi) atomicOr(&ov[(base >> 6) + 1], hi);
+            }
         }
     }
 }
@@ -384,11 +416,16 @@ __global__ __launch_bounds__(256) void k_copy_segments(const cq_copy_seg* __rest
 __global__ __launch_bounds__(256) void k_gather(const u64* __restrict__ cols, const u64* __restrict__ valid,
                                                 const u64* __restrict__ idx, i64 rows, int vts, i64 cap_rows,
                                                 i64 valid_words, u64* __restrict__ out, u64* __restrict__ out_valid) {
-    for (i64 r = (i64)blockIdx.x * 256 + threadIdx.x; r < rows; r += (i64)gridDim.x * 256) {
-        const i64 src = (i64)idx[r];
+    // a wave writes rows 64k .. 64k + 63: it owns their validity word, which
+    // is one ballot (plain store, no atomics on a shared word)
+    for (i64 r0 = (i64)blockIdx.x * 256 + (threadIdx.x & ~63); r0 < rows; r0 += (i64)gridDim.x * 256) {
+        const i64 r = r0 + (threadIdx.x & 63);
+        const bool in = r < rows;
+        const i64 src = in ? (i64)idx[r] : 0;
         for (int c = 0; c < vts; ++c) {
-            out[(i64)c * cap_rows + r] = cols[(i64)c * cap_rows + src];
-            if (bit64(valid + (i64)c * valid_words, src)) set_bit64(out_valid + (i64)c * valid_words, r);
+            if (in) out[(i64)c * cap_rows + r] = cols[(i64)c * cap_rows + src];
+            const u64 w = __ballot(in && bit64(valid + (i64)c * valid_words, src));
+            if ((threadIdx.x & 63) == 0) out_valid[(i64)c * valid_words + (r0 >> 6)] = w;
         }
     }
 }

@@ -225,6 +225,7 @@ void DatabaseInstance::ensureTable(const std::string& table) const {
 }
 
 void DatabaseInstance::createTable(const std::string& name, int64_t rows, bool is_fact) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     if (catalog_->hasTable(name)) throw std::runtime_error("table already exists: " + name);
     catalog_->pushTableMetadata(TableMetadata(name, {}, static_cast<uint64_t>(rows), is_fact));
@@ -232,6 +233,7 @@ void DatabaseInstance::createTable(const std::string& name, int64_t rows, bool i
 
 void DatabaseInstance::loadColumn(const std::string& table, const std::string& column, const int32_t* data,
                                   const uint64_t* validity) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     ensureTable(table);
     auto& meta = catalog_->getTableMetadataMutable(table);
@@ -279,12 +281,14 @@ void DatabaseInstance::loadColumn(const std::string& table, const std::string& c
 }
 
 void DatabaseInstance::setPrimaryKey(const std::string& table, const std::string& column) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     ensureTable(table);
     catalog_->getTableMetadataMutable(table).setColumnPrimaryKey(column, true);
 }
 
 void DatabaseInstance::declareColumn(const std::string& table, const std::string& column, bool nullable) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     ensureTable(table);
     auto& meta = catalog_->getTableMetadataMutable(table);
@@ -298,6 +302,7 @@ void DatabaseInstance::declareColumn(const std::string& table, const std::string
 }
 
 void DatabaseInstance::setColumnStats(const std::string& column, int64_t mn, int64_t mx, uint64_t card) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     const std::string t = catalog_->tableOfColumn(column);
     ensureTable(t);
@@ -317,6 +322,7 @@ uint64_t DatabaseInstance::resultSize(const std::string& sql) const {
 }
 
 void DatabaseInstance::generateSSB(const GenSpec& spec) {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     if (device_ < 0) throw std::runtime_error("generateSSB needs a GPU device");
     for (const auto& t : genTables()) createTable(t, genTableRows(spec, t), t == "LINEORDER");
     for (const auto& t : genTables()) {
@@ -416,6 +422,7 @@ ColumnStatistics DatabaseInstance::computeStats(const std::string& column, bool&
 }
 
 void DatabaseInstance::finalizeCatalog() {
+    engine_->invalidate();  // cached programs were planned on the old catalog
     std::lock_guard<std::mutex> lock(mu_);
     for (auto& meta : catalog_->getTablesMetadataMutable()) {
         for (const auto& c : meta.getColumnNames()) {

@@ -88,6 +88,25 @@ void DeviceBuffer::ensure(size_t bytes) {
     bytes_ = want;
 }
 
+PinnedBuffer::~PinnedBuffer() {
+    if (ptr_) {
+        DeviceMapLock lock;
+        (void)hipHostFree(ptr_);
+    }
+}
+
+void* PinnedBuffer::ensure(size_t bytes) {
+    if (bytes <= bytes_ && ptr_) return ptr_;
+    DeviceMapLock lock;
+    if (ptr_) hipCheck(hipHostFree(ptr_), "hipHostFree");
+    ptr_ = nullptr;
+    bytes_ = 0;
+    const size_t want = std::max<size_t>(bytes, 64 * 1024);
+    hipCheck(hipHostMalloc(&ptr_, want, hipHostMallocDefault), "hipHostMalloc");
+    bytes_ = want;
+    return ptr_;
+}
+
 // ------------------------------------------------------------------ CompiledModule
 
 CompiledModule::CompiledModule(std::string signature, std::vector<char> code)
@@ -785,6 +804,35 @@ void HipModuleExecutor::fetch(QueryRun& run, ExecutionContext* ctx, unsigned lon
     fetchFinish(run);
 }
 
+void HipModuleExecutor::fetchPinned(QueryRun& run, ExecutionContext* ctx, unsigned long long rows_found) {
+    hipStream_t s = ctx->stream;
+    const size_t bytes = fetchBytes(run, rows_found);
+    std::vector<cq_copy_seg> segs;
+    fetchSegments(run, ctx, rows_found, nullptr, 0, segs);
+    // the result's columns and validity words gathered on the device into
+    // one range, then one copy into pinned memory (not one copy per column
+    // into pageable memory)
+    const size_t seg_at = (bytes + 255) & ~size_t(255);
+    const size_t seg_bytes = segs.size() * sizeof(cq_copy_seg);
+    char* host = static_cast<char*>(ctx->pinned().ensure(seg_at + seg_bytes + 8));
+    if (segs.size() == 1) {
+        hipCheck(hipMemcpyAsync(host + segs[0].dst_word * 8, segs[0].src, segs[0].words * 8, hipMemcpyDeviceToHost, s), "fetch");
+    } else if (!segs.empty()) {
+        std::memcpy(host + seg_at, segs.data(), seg_bytes);
+        DeviceBuffer& stage = ctx->scratch(run.prefix + "fetch_stage");
+        stage.ensure(seg_at + seg_bytes + 8);
+        char* d = static_cast<char*>(stage.data());
+        auto* dseg = reinterpret_cast<cq_copy_seg*>(d + seg_at);
+        hipCheck(hipMemcpyAsync(dseg, host + seg_at, seg_bytes, hipMemcpyHostToDevice, s), "upload fetch segments");
+        hipCheck(cq_launch_copy_segments(dseg, static_cast<int>(segs.size()), reinterpret_cast<unsigned long long*>(d), s),
+                 "gather result");
+        hipCheck(hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, s), "fetch");
+    }
+    run.fetch_host = host;
+    hipCheck(hipStreamSynchronize(s), "sync");
+    fetchFinish(run);
+}
+
 void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ctx) {
     hipStream_t s = ctx->stream;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ks = nullptr, ke = nullptr;
@@ -808,11 +856,13 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
         shape(run, ctx, nullptr, nullptr);
     }
     if (!run.single_cell && !ctx->program.projection) {
-        hipCheck(hipMemcpyAsync(&rows, run.off + run.cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
+        auto* pr = static_cast<unsigned long long*>(ctx->pinned().ensure(64));
+        hipCheck(hipMemcpyAsync(pr, run.off + run.cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
         hipCheck(hipStreamSynchronize(s), "sync");
+        rows = *pr;
     }
     const auto f0 = std::chrono::steady_clock::now();
-    fetch(run, ctx, rows);
+    fetchPinned(run, ctx, rows);
     hipCheck(hipEventRecord(ev1, s), "hipEventRecord");
     hipCheck(hipEventSynchronize(ev1), "sync");
     ctx->result = std::move(run.result);
@@ -910,7 +960,7 @@ bool splitLiterals(const std::string& sql, std::string& shape, std::vector<long 
 std::string optionsKey(const CodegenOptions& o) {
     std::ostringstream k;
     k << o.block_threads << ',' << o.vectors_per_thread << ',' << o.lds_budget << ',' << o.allow_lds_tables << o.allow_lds_aggregation
-      << o.allow_compact_tables << o.allow_dense_tables << o.calendar_tables << o.split_probes << o.buffer_tables << o.prefetch << o.stream_large_tables << o.batch_body << o.batch_shared_tables << o.batch_masks
+      << o.allow_compact_tables << o.allow_dense_tables << o.calendar_tables << o.split_probes << o.buffer_tables << o.hoist_gathers << o.prefetch << o.stream_large_tables << o.batch_body << o.batch_shared_tables << o.batch_masks
       << o.vm;
     return k.str();
 }
@@ -1028,21 +1078,35 @@ void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
     ctx->resetForQuery();
     ctx->row_begin = row_begin;
     ctx->row_end = row_end;
-    auto tree = buildOptimizedTree(sql, catalog_.get());
-    const auto t1 = std::chrono::steady_clock::now();
-    try {
-        ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
-    } catch (const std::runtime_error& e) {
-        // plans the interpreter does not take (OR-key joins) run compiled
-        if (!ctx->config.codegen.vm || std::string(e.what()).rfind("interpreted tier:", 0) != 0) throw;
-        CodegenOptions co = ctx->config.codegen;
-        co.vm = false;
-        ctx->program = generateQueryProgram(tree.get(), *catalog_, co);
+    const std::string key = optionsKey(ctx->config.codegen) + "|" + sql;
+    std::shared_ptr<const QueryProgram> known;
+    {
+        std::lock_guard<std::mutex> lock(programs_mu_);
+        auto it = programs_.find(key);
+        if (it != programs_.end()) known = it->second;
     }
-    const auto t2 = std::chrono::steady_clock::now();
     QueryTimingStats timing;
-    timing.parse_ms = elapsedMs(t0, t1);
-    timing.codegen_ms = elapsedMs(t1, t2);
+    if (known) {
+        ctx->program = *known;
+    } else {
+        auto tree = buildOptimizedTree(sql, catalog_.get());
+        const auto t1 = std::chrono::steady_clock::now();
+        try {
+            ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
+        } catch (const std::runtime_error& e) {
+            // plans the interpreter does not take (OR-key joins) run compiled
+            if (!ctx->config.codegen.vm || std::string(e.what()).rfind("interpreted tier:", 0) != 0) throw;
+            CodegenOptions co = ctx->config.codegen;
+            co.vm = false;
+            ctx->program = generateQueryProgram(tree.get(), *catalog_, co);
+        }
+        const auto t2 = std::chrono::steady_clock::now();
+        timing.parse_ms = elapsedMs(t0, t1);
+        timing.codegen_ms = elapsedMs(t1, t2);
+        std::lock_guard<std::mutex> lock(programs_mu_);
+        if (programs_.size() >= 1024) programs_.clear();
+        programs_[key] = std::make_shared<const QueryProgram>(ctx->program);
+    }
     preflightDeviceMemoryOrThrow(*ctx, ctx->program);
     ModuleHandle m = ctx->program.vm ? nullptr : moduleFor(ctx->program, &timing);
     executor_->execute(m, ctx);

@@ -128,6 +128,21 @@ struct EngineConfig {
     size_t module_cache_capacity = 256;  // compiled modules kept (LRU)
 };
 
+// Grow-only pinned host allocation (hipHostMalloc), for asynchronous
+// device-to-host copies.
+class PinnedBuffer {
+public:
+    PinnedBuffer() = default;
+    ~PinnedBuffer();
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    void* ensure(size_t bytes);
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
 struct DeviceColumn {
     const int* data = nullptr;
     const uint64_t* validity = nullptr;  // LSB-first u64 words, null = all valid
@@ -181,6 +196,8 @@ struct ExecutionContext {
     int64_t row_begin = 0, row_end = -1;  // fact rows to scan (-1: all)
 
     DeviceBuffer& scratch(const std::string& name) { return scratch_[name]; }
+    // pinned host staging of result fetches
+    PinnedBuffer& pinned() { return pinned_; }
     size_t scratchBytes() const {
         size_t b = 0;
         for (const auto& kv : scratch_) b += kv.second.capacity();
@@ -195,6 +212,7 @@ struct ExecutionContext {
 
 private:
     std::map<std::string, DeviceBuffer> scratch_;
+    PinnedBuffer pinned_;
 };
 
 class ICompiler {
@@ -319,8 +337,11 @@ public:
     // projection: count pass, scan of the counts, result buffers for the
     // rows found, write pass, columnar rows (returns the row count)
     unsigned long long project(QueryRun& run, ExecutionContext* ctx, hipEvent_t kernel_start, hipEvent_t kernel_end);
-    // sort / limit and copy the result to the host (synchronizes)
+    // sort / limit and copy the result to the host (synchronizes);
+    // fetchPinned: the same through the context's pinned staging, the
+    // result gathered into one range on the device and copied once
     void fetch(QueryRun& run, ExecutionContext* ctx, unsigned long long rows);
+    void fetchPinned(QueryRun& run, ExecutionContext* ctx, unsigned long long rows);
     // the same in two steps, so many runs share one synchronization:
     // fetchLaunch enqueues sort / gather and the copies into `host`
     // (fetchBytes(run, rows) bytes, pinned for asynchronous copies),
@@ -368,7 +389,20 @@ public:
     // the cached module of a program, or null (never compiles)
     ModuleHandle cachedModule(const QueryProgram& program) { return cache_->get(program.signature); }
     HipModuleExecutor* hipExecutor() { return dynamic_cast<HipModuleExecutor*>(executor_.get()); }
-    void setCatalog(std::shared_ptr<const Catalog> catalog) { catalog_ = std::move(catalog); }
+    // a new or changed catalog (tables, columns, statistics) drops every
+    // cached program
+    void setCatalog(std::shared_ptr<const Catalog> catalog) {
+        catalog_ = std::move(catalog);
+        invalidate();
+    }
+    void invalidate() {
+        {
+            std::lock_guard<std::mutex> a(prepared_mu_);
+            prepared_.clear();
+        }
+        std::lock_guard<std::mutex> b(programs_mu_);
+        programs_.clear();
+    }
     IQueryCache& cache() { return *cache_; }
 
 private:
@@ -390,6 +424,12 @@ private:
     };
     std::mutex prepared_mu_;
     std::unordered_map<std::string, PreparedShape> prepared_;
+    // executeQuery: statement text (and code generation options) -> its
+    // program, so a repeated statement skips parsing and code generation
+    // (the reference skips code generation for a statement it has seen,
+    // execution.cpp:692-699)
+    std::mutex programs_mu_;
+    std::unordered_map<std::string, std::shared_ptr<const QueryProgram>> programs_;
 };
 
 // Throws when the device cannot hold the query (preflightDeviceMemoryOrThrow).
