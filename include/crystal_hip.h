@@ -174,6 +174,18 @@ const unsigned char* cq_result_validity(const cq_result* r, int col);
 /* "gpu_execute_ms","scan_kernel_ms","compile_ms","codegen_ms","parse_ms",
  * "host_fetch_ms","engine_ms","cache_hit" */
 double cq_result_timing(const cq_result* r, const char* name);
+/* The accessors above in bulk, for bindings that read many results (no
+ * reference counterpart: QueryResult's fields, execution.h:56-96, are read
+ * in place by C++ callers).  cq_result_shape: rows, columns, ordered flag,
+ * and per column its type and nullability (arrays of max_cols entries;
+ * CQ_ERR_INVALID when the result has more columns).  cq_result_copy: every
+ * column into values[c * rows + i] and its validity into valid[c * rows + i]
+ * (either may be NULL).  cq_result_timings: out[0..7] = gpu_execute_ms,
+ * scan_kernel_ms, compile_ms, codegen_ms, parse_ms, host_fetch_ms,
+ * engine_ms, cache_hit. */
+int cq_result_shape(const cq_result* r, long long* rows, int* cols, int* ordered, int* types, int* nullable, int max_cols);
+int cq_result_copy(const cq_result* r, unsigned long long* values, unsigned char* valid);
+int cq_result_timings(const cq_result* r, double* out);
 void cq_result_free(cq_result* r);
 
 /* ---- batched shared scan ------------------------------------------- */

@@ -80,6 +80,9 @@ def _load() -> ctypes.CDLL:
         "cq_result_column": (c_void_p, [c_void_p, c_int]),
         "cq_result_validity": (c_void_p, [c_void_p, c_int]),
         "cq_result_timing": (ctypes.c_double, [c_void_p, c_char_p]),
+        "cq_result_shape": (c_int, [c_void_p, P(c_ll), P(c_int), P(c_int), c_void_p, c_void_p, c_int]),
+        "cq_result_copy": (c_int, [c_void_p, c_void_p, c_void_p]),
+        "cq_result_timings": (c_int, [c_void_p, c_void_p]),
         "cq_result_free": (None, [c_void_p]),
         "cq_batch_execute": (c_int, [c_void_p, P(c_char_p), c_int, c_char_p, P(c_void_p)]),
         "cq_batch_last_report": (c_int, [c_void_p, c_char_p, c_sz, P(c_sz)]),
@@ -158,26 +161,32 @@ class Result:
         return out
 
 
+_TIMING_KEYS = ("gpu_execute_ms", "scan_kernel_ms", "compile_ms", "codegen_ms", "parse_ms", "host_fetch_ms",
+                "engine_ms", "cache_hit")
+_MAX_COLS = 256
+_DTYPES = {INT64: np.int64, UINT64: np.uint64, FLOAT64: np.float64}
+
+
 def _fetch_result(ptr: ctypes.c_void_p) -> Result:
+    """One result through the bulk accessors (shape, one copy of every
+    column and validity, timings), then freed."""
     try:
-        n = _lib.cq_result_rows(ptr)
-        ncols = _lib.cq_result_cols(ptr)
-        cols, valid, types, nullable = [], [], [], []
-        for c in range(ncols):
-            t = _lib.cq_result_type(ptr, c)
-            raw = np.ctypeslib.as_array(ctypes.cast(_lib.cq_result_column(ptr, c), ctypes.POINTER(ctypes.c_uint64)),
-                                        shape=(n,)).copy() if n else np.zeros(0, np.uint64)
-            vptr = _lib.cq_result_validity(ptr, c)
-            v = (np.ctypeslib.as_array(ctypes.cast(vptr, ctypes.POINTER(ctypes.c_uint8)), shape=(n,)).copy().astype(bool)
-                 if n else np.zeros(0, bool))
-            cols.append(raw.view({INT64: np.int64, UINT64: np.uint64, FLOAT64: np.float64}[t]))
-            valid.append(v)
-            types.append(t)
-            nullable.append(bool(_lib.cq_result_nullable(ptr, c)))
-        timing = {k: _lib.cq_result_timing(ptr, k.encode()) for k in
-                  ("gpu_execute_ms", "scan_kernel_ms", "compile_ms", "codegen_ms", "parse_ms", "host_fetch_ms",
-                   "engine_ms", "cache_hit")}
-        return Result(cols, valid, types, nullable, bool(_lib.cq_result_ordered(ptr)), timing)
+        n, ncols, ordered = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
+        types = np.zeros(_MAX_COLS, np.int32)
+        nullable = np.zeros(_MAX_COLS, np.int32)
+        _check(_lib.cq_result_shape(ptr, ctypes.byref(n), ctypes.byref(ncols), ctypes.byref(ordered), types.ctypes.data,
+                                    nullable.ctypes.data, _MAX_COLS))
+        rows, nc = n.value, ncols.value
+        values = np.empty((nc, rows), np.uint64)
+        valid = np.empty((nc, rows), np.uint8)
+        if rows and nc:
+            _check(_lib.cq_result_copy(ptr, values.ctypes.data, valid.ctypes.data))
+        tv = np.zeros(8, np.float64)
+        _check(_lib.cq_result_timings(ptr, tv.ctypes.data))
+        cols = [values[c].view(_DTYPES[int(types[c])]) for c in range(nc)]
+        vld = [valid[c].view(bool) for c in range(nc)]
+        return Result(cols, vld, [int(t) for t in types[:nc]], [bool(x) for x in nullable[:nc]], bool(ordered.value),
+                      dict(zip(_TIMING_KEYS, tv.tolist())))
     finally:
         _lib.cq_result_free(ptr)
 

@@ -437,6 +437,43 @@ const unsigned char* cq_result_validity(const cq_result* r, int c) {
     if (!r || c < 0 || c >= static_cast<int>(r->r.validity.size())) return nullptr;
     return r->r.validity[static_cast<size_t>(c)].data();
 }
+int cq_result_shape(const cq_result* r, long long* rows, int* cols, int* ordered, int* types, int* nullable, int max_cols) {
+    if (!r || !rows || !cols || !ordered) return CQ_ERR_INVALID;
+    const auto& q = r->r;
+    *rows = static_cast<long long>(q.row_count);
+    *cols = static_cast<int>(q.columns.size());
+    *ordered = q.ordered ? 1 : 0;
+    if (*cols > max_cols) return CQ_ERR_INVALID;
+    for (int c = 0; c < *cols; ++c) {
+        const auto& cd = q.columns[static_cast<size_t>(c)];
+        if (types) types[c] = static_cast<int>(cd.type);
+        if (nullable) nullable[c] = cd.nullable ? 1 : 0;
+    }
+    return CQ_OK;
+}
+int cq_result_copy(const cq_result* r, unsigned long long* values, unsigned char* valid) {
+    if (!r) return CQ_ERR_INVALID;
+    const auto& q = r->r;
+    const size_t n = q.row_count;
+    for (size_t c = 0; c < q.data.size(); ++c) {
+        if (values && n) std::memcpy(values + c * n, q.data[c].data(), n * sizeof(unsigned long long));
+        if (valid && n) std::memcpy(valid + c * n, q.validity[c].data(), n);
+    }
+    return CQ_OK;
+}
+int cq_result_timings(const cq_result* r, double* out) {
+    if (!r || !out) return CQ_ERR_INVALID;
+    const auto& t = r->r.timing;
+    out[0] = t.gpu_execute_ms;
+    out[1] = t.scan_kernel_ms;
+    out[2] = t.compile_ms;
+    out[3] = t.codegen_ms;
+    out[4] = t.parse_ms;
+    out[5] = t.host_fetch_ms;
+    out[6] = t.engine_ms;
+    out[7] = t.cache_hit ? 1 : 0;
+    return CQ_OK;
+}
 double cq_result_timing(const cq_result* r, const char* name) {
     if (!r || !name) return -1;
     const auto& t = r->r.timing;

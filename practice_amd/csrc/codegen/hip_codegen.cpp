@@ -3475,12 +3475,22 @@ private:
     }
 
     void emitKeysKernel(std::ostringstream& src) {
+        emitKeysKernelAs(src, false);
+        // batched templates: the same for every query of the template in one
+        // launch (blockIdx.y = query, parameter blocks in device memory)
+        if (opt_.batch_body) emitKeysKernelAs(src, true);
+    }
+    void emitKeysKernelAs(std::ostringstream& src, bool batched) {
         const int ng = prog_.ngroups, sts = prog_.storage_ts;
         const int64_t G = prog_.groups;
         unsigned long long smask = 0;
         for (int s : flag_static_) smask |= 1ull << s;
-        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_keys(" << paramDecl() << ") {\n"
-            << paramAlias() << "    using namespace chip;\n"
+        if (batched)
+            src << "extern \"C\" __global__ __launch_bounds__(256) void cq_keys_b(const cq_query_params* __restrict__ QP) {\n"
+                << "    const cq_query_params& P = QP[blockIdx.y];\n";
+        else
+            src << "extern \"C\" __global__ __launch_bounds__(256) void cq_keys(" << paramDecl() << ") {\n" << paramAlias();
+        src << "    using namespace chip;\n"
             << "    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;\n"
             << "    if (g >= " << G << "ll) return;\n"
             << "    const long long base = g * " << sts << ";\n"
@@ -3600,12 +3610,25 @@ private:
     }
 
     void emitDenseCount(std::ostringstream& src) {
+        emitDenseCountAs(src, false);
+        if (opt_.batch_body) emitDenseCountAs(src, true);
+    }
+    void emitDenseCountAs(std::ostringstream& src, bool batched) {
         const int vts = prog_.visible_ts;
-        src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count(" << paramDecl()
-            << ", const unsigned long long* __restrict__ R, const unsigned long long* __restrict__ RV, "
-               "unsigned long long* counts, long long G) {\n"
-            << paramAlias()
-            << "    using namespace chip;\n"
+        if (batched)
+            src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count_b(const cq_query_params* __restrict__ QP, "
+                   "const cq_shape_desc* __restrict__ SD) {\n"
+                << "    const cq_query_params& P = QP[blockIdx.y];\n"
+                << "    const unsigned long long* __restrict__ R = SD[blockIdx.y].res;\n"
+                << "    const unsigned long long* __restrict__ RV = SD[blockIdx.y].rv;\n"
+                << "    unsigned long long* counts = SD[blockIdx.y].fl;\n"
+                << "    const long long G = SD[blockIdx.y].G;\n";
+        else
+            src << "extern \"C\" __global__ __launch_bounds__(256) void cq_dense_count(" << paramDecl()
+                << ", const unsigned long long* __restrict__ R, const unsigned long long* __restrict__ RV, "
+                   "unsigned long long* counts, long long G) {\n"
+                << paramAlias();
+        src << "    using namespace chip;\n"
             << "    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;\n"
             << "    if (g >= G) return;\n"
             << "    const long long base = g * " << vts << ";\n";

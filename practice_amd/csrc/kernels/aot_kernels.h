@@ -69,3 +69,15 @@ struct cq_copy_seg {
     unsigned long long words;
 };
 hipError_t cq_launch_copy_segments(const cq_copy_seg* segs, int n, unsigned long long* dst, hipStream_t s);
+
+// Batched shared scans: aggregate-state initialization and result shaping of
+// every query of a batch in one launch each (descriptors in device memory,
+// device/query_params.h).  The shaping kernel runs one block per query and
+// takes queries of at most kBatchShapeMaxGroups groups and
+// kBatchShapeMaxCols output columns; a descriptor with G < 0 is skipped.
+struct cq_init_desc;
+struct cq_shape_desc;
+constexpr long long kBatchShapeMaxGroups = 16384;
+constexpr int kBatchShapeMaxCols = 8;
+hipError_t cq_launch_batch_init(const cq_init_desc* d, int n, long long max_slots, hipStream_t s);
+hipError_t cq_launch_batch_shape(const cq_shape_desc* d, int n, hipStream_t s);

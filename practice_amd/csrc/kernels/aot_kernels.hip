@@ -327,6 +327,91 @@ __global__ __launch_bounds__(256) void k_write_rows(const u64* __restrict__ res,
     }
 }
 
+// ---- batched shared scans: one launch for all queries of a batch
+// (blockIdx.y / blockIdx.x = query)
+__global__ __launch_bounds__(256) void k_batch_init(const cq_init_desc* __restrict__ D) {
+    const cq_init_desc d = D[blockIdx.y];
+    for (i64 i = (i64)blockIdx.x * 256 + threadIdx.x; i < d.slots; i += (i64)gridDim.x * 256) {
+        u64 v = 0ull;
+        if (d.min_mask) {
+            const int s = (int)(i % d.sts);
+            if (s < 32 && i < d.G * d.sts && ((d.min_mask >> s) & 1u)) v = ~0ull;
+        }
+        d.res[i] = v;
+    }
+    const i64 words = (d.slots + 63) / 64;
+    for (i64 w = (i64)blockIdx.x * 256 + threadIdx.x; w < words; w += (i64)gridDim.x * 256) d.rv[w] = w == 0 ? d.count_bits : 0ull;
+}
+
+// Present groups -> rows in group order, as the dense count / exclusive scan
+// / row writer sequence of a single query, for groups <= 16 x 1024: each
+// thread owns a contiguous run of groups, a block-wide exclusive scan gives
+// its first row, output validity words are assembled in LDS.
+constexpr int BSHAPE_BT = 1024;
+__global__ __launch_bounds__(BSHAPE_BT) void k_batch_shape(const cq_shape_desc* __restrict__ D) {
+    const cq_shape_desc d = D[blockIdx.x];
+    if (d.G < 0) return;
+    __shared__ u64 s_valid[kBatchShapeMaxCols * (kBatchShapeMaxGroups / 64)];
+    __shared__ u32 s_wave[BSHAPE_BT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int per = (int)((d.G + BSHAPE_BT - 1) / BSHAPE_BT);
+    const i64 g0 = (i64)tid * per;
+    const int vts = d.vts;
+    const int vw = (int)d.vwords;
+    for (int i = tid; i < vts * vw; i += BSHAPE_BT) s_valid[i] = 0ull;
+    u32 pm = 0u;
+    u32 cnt = 0u;
+    for (int k = 0; k < per; ++k) {
+        const i64 g = g0 + k;
+        if (g >= d.G) break;
+        bool p = false;
+        if (d.sparse) {
+            for (int c = 0; c < vts && !p; ++c) p = d.res[g * vts + c] != 0ull;
+        } else {
+            p = d.fl[g] != 0ull;
+        }
+        if (p) {
+            pm |= 1u << k;
+            ++cnt;
+        }
+    }
+    // block-wide exclusive scan of cnt
+    u32 x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u32 t = __shfl_up(x, o);
+        if (lane >= o) x += t;
+    }
+    if (lane == 63) s_wave[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        u32 w = lane < BSHAPE_BT / 64 ? s_wave[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const u32 t = __shfl_up(w, o);
+            if (lane >= o) w += t;
+        }
+        if (lane < BSHAPE_BT / 64) s_wave[lane] = w;  // inclusive totals
+    }
+    __syncthreads();
+    const u32 before = (wid ? s_wave[wid - 1] : 0u) + x - cnt;
+    const u32 total = s_wave[BSHAPE_BT / 64 - 1];
+    i64 row = before;
+    for (int k = 0; k < per; ++k) {
+        if (!((pm >> k) & 1u)) continue;
+        const i64 g = g0 + k;
+        for (int c = 0; c < vts; ++c) {
+            d.o[(i64)c * d.cap_rows + row] = d.res[g * vts + c];
+            const bool valid = !d.use_validity || !((d.nullable_mask >> c) & 1u) || bit64(d.rv, g * vts + c);
+            if (valid) atomicOr(&s_valid[c * vw + (int)(row >> 6)], 1ull << (row & 63));
+        }
+        ++row;
+    }
+    __syncthreads();
+    for (int i = tid; i < vts * vw; i += BSHAPE_BT) d.ov[(i64)(i / vw) * d.vwords + (i % vw)] = s_valid[i];
+    if (tid == 0) *d.rows = total;
+}
+
 // ---- ORDER BY
 __device__ bool right_before_left(u64 l, u64 r, const cq_sort_desc& d, const u64* cols, const u64* valid, i64 cap_rows,
                                   i64 valid_words) {
@@ -545,5 +630,20 @@ hipError_t cq_launch_gather(const unsigned long long* cols, const unsigned long 
     if (rows <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gather, dim3(grid_for(rows)), dim3(256), 0, s, cols, valid, idx, rows, vts, cap_rows, valid_words,
                        out, out_valid);
+    return hipGetLastError();
+}
+
+hipError_t cq_launch_batch_init(const cq_init_desc* d, int n, long long max_slots, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    long long bx = (max_slots + 255) / 256;
+    if (bx < 1) bx = 1;
+    if (bx > 64) bx = 64;
+    hipLaunchKernelGGL(k_batch_init, dim3((unsigned)bx, (unsigned)n), dim3(256), 0, s, d);
+    return hipGetLastError();
+}
+
+hipError_t cq_launch_batch_shape(const cq_shape_desc* d, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_shape, dim3((unsigned)n), dim3(BSHAPE_BT), 0, s, d);
     return hipGetLastError();
 }

@@ -327,7 +327,7 @@ unsigned scanGridShared(hipFunction_t f, int block_threads, int device, long lon
 // (from its first query's parameter block), per-query tables in setup().
 void BatchExecutor::prepareTables(const std::vector<const QueryProgram*>& templates,
                                   const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
-                                  const cq_query_params* dqp, ExecutionContext* ctx) {
+                                  const cq_query_params* dqp, const BatchInit& init, ExecutionContext* ctx) {
     HipModuleExecutor* ex = db_.engine().hipExecutor();
     int first = 0;
     for (size_t t = 0; t < templates.size(); ++t) {
@@ -341,7 +341,9 @@ void BatchExecutor::prepareTables(const std::vector<const QueryProgram*>& templa
         }
         first += cnt;
     }
-    for (size_t k = 0; k < runs.size(); ++k) ex->setup(runs[k], ctx, dqp + k);
+    hipCheck(cq_launch_batch_init(init.d, static_cast<int>(runs.size()), init.max_slots, ctx->stream), "batch init");
+    for (size_t k = 0; k < runs.size(); ++k)
+        if (!init.eligible[k]) ex->setup(runs[k], ctx, dqp + k);
 }
 
 void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt,
@@ -415,6 +417,9 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     }
     const std::string pre = "s" + std::to_string(scan_index) + "/";
     const size_t qp_bytes = static_cast<size_t>(n) * sizeof(cq_query_params);
+    const size_t init_bytes = static_cast<size_t>(n) * sizeof(cq_init_desc);
+    const size_t shape_bytes = static_cast<size_t>(n) * sizeof(cq_shape_desc);
+    const size_t upload_bytes = qp_bytes + init_bytes + shape_bytes;
     ScratchArena arena;
     cq_query_params* dqp = nullptr;
     unsigned long long* dtot = nullptr;
@@ -440,22 +445,79 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
             }
             first += cnt;
         }
-        dqp = static_cast<cq_query_params*>(arena.take(qp_bytes));
+        // parameter blocks, then the init and shaping descriptors: one upload
+        dqp = static_cast<cq_query_params*>(arena.take(upload_bytes));
         dtot = static_cast<unsigned long long*>(arena.take(static_cast<size_t>(n) * 8));
     }
+    auto* dinit = reinterpret_cast<cq_init_desc*>(reinterpret_cast<char*>(dqp) + qp_bytes);
+    auto* dshape = reinterpret_cast<cq_shape_desc*>(reinterpret_cast<char*>(dqp) + qp_bytes + init_bytes);
 
-    // parameter blocks -> device (pinned staging, stream ordered)
-    char* stage = static_cast<char*>(pinned(qp_bytes + static_cast<size_t>(n) * 8));
+    // Per-query aggregate-state init and result shaping run as one launch
+    // for the whole batch where the query allows it: compiled, shared
+    // tables (init); at most kBatchShapeMaxGroups groups and
+    // kBatchShapeMaxCols columns, no AVG finalization, not a single cell
+    // (shaping).  The others take the per-query path.
+    BatchInit init;
+    init.d = dinit;
+    init.eligible.assign(static_cast<size_t>(n), 0);
+    std::vector<char> shape_ok(static_cast<size_t>(n), 0);
+    std::vector<cq_init_desc> hinit(static_cast<size_t>(n));
+    std::vector<cq_shape_desc> hshape(static_cast<size_t>(n));
+    for (int k = 0; k < n; ++k) {
+        const QueryRun& rr = runs[static_cast<size_t>(k)];
+        const QueryProgram& p = *rr.program;
+        cq_init_desc& id = hinit[static_cast<size_t>(k)];
+        std::memset(&id, 0, sizeof id);
+        if (p.shared_tables && !p.vm) {
+            id.res = rr.P.result;
+            id.rv = rr.P.result_valid;
+            id.slots = static_cast<long long>(rr.slots);
+            id.G = rr.G;
+            id.sts = p.storage_ts;
+            for (int slot : p.min_slots) id.min_mask |= 1u << slot;
+            if (!p.grouped && rr.P.row_end > rr.P.row_begin)
+                for (int slot : p.count_slots_scalar)
+                    if (slot < 64) id.count_bits |= 1ull << slot;
+            init.eligible[static_cast<size_t>(k)] = 1;
+            init.max_slots = std::max(init.max_slots, id.slots);
+        }
+        cq_shape_desc& sd = hshape[static_cast<size_t>(k)];
+        std::memset(&sd, 0, sizeof sd);
+        sd.G = -1;
+        if (!p.vm && !p.projection && !p.has_avg && !rr.single_cell && rr.G <= kBatchShapeMaxGroups &&
+            p.visible_ts <= kBatchShapeMaxCols) {
+            sd.res = rr.res;
+            sd.rv = rr.rv;
+            sd.fl = rr.fl;
+            sd.o = rr.o;
+            sd.ov = rr.ov;
+            sd.rows = dtot + k;
+            sd.G = rr.G;
+            sd.cap_rows = rr.cap_rows;
+            sd.vwords = rr.vwords;
+            sd.vts = p.visible_ts;
+            sd.sparse = p.fast_sparse ? 1 : 0;
+            sd.use_validity = p.fast_sparse ? 0 : 1;
+            for (int c = 0; c < p.visible_ts; ++c)
+                if (p.out[static_cast<size_t>(c)].nullable) sd.nullable_mask |= 1u << c;
+            shape_ok[static_cast<size_t>(k)] = 1;
+        }
+    }
+
+    // parameter blocks and descriptors -> device (pinned staging, stream ordered)
+    char* stage = static_cast<char*>(pinned(upload_bytes + static_cast<size_t>(n) * 8));
     for (int k = 0; k < n; ++k) std::memcpy(stage + static_cast<size_t>(k) * sizeof(cq_query_params), &runs[static_cast<size_t>(k)].P,
                                             sizeof(cq_query_params));
-    hipCheck(hipMemcpyAsync(dqp, stage, qp_bytes, hipMemcpyHostToDevice, s), "upload parameter blocks");
+    std::memcpy(stage + qp_bytes, hinit.data(), init_bytes);
+    std::memcpy(stage + qp_bytes + init_bytes, hshape.data(), shape_bytes);
+    hipCheck(hipMemcpyAsync(dqp, stage, upload_bytes, hipMemcpyHostToDevice, s), "upload parameter blocks");
 
     // launch shape: tuned per template set when asked
     Tuned cfg{opt.block_threads, opt.vectors_per_thread};
     auto tit = tuned_.find(set_key);
     if (tit != tuned_.end()) cfg = tit->second;
     else if (opt.autotune) {
-        cfg = autotune(templates, per_template, runs, dqp, ldsLimit(opt), ctx);
+        cfg = autotune(templates, per_template, runs, dqp, init, ldsLimit(opt), ctx);
         tuned_[set_key] = cfg;
     }
     std::vector<std::string> ucols;
@@ -489,7 +551,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes);
 
     auto enqueue = [&](bool timed) {
-        prepareTables(templates, per_template, runs, dqp, ctx);
+        prepareTables(templates, per_template, runs, dqp, init, ctx);
         if (timed) hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
         void* args[] = {&B};
         if (B.num_tiles > 0) {
@@ -508,7 +570,35 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
             }
         }
         if (timed) hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
-        for (int k = 0; k < n; ++k) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
+        // result shaping: per template, group keys and present / HAVING
+        // flags of all its queries in one launch each, then the rows of every
+        // eligible query of the batch in one launch
+        int tfirst = 0;
+        for (size_t t = 0; t < templates.size(); ++t) {
+            const int cnt = static_cast<int>(per_template[t].size());
+            if (cnt > 0 && shape_ok[static_cast<size_t>(tfirst)]) {
+                const QueryRun& r0 = runs[static_cast<size_t>(tfirst)];
+                const unsigned gx = static_cast<unsigned>((r0.G + 255) / 256);
+                const cq_query_params* qp = dqp + tfirst;
+                const cq_shape_desc* sdp = dshape + tfirst;
+                if (r0.keys) {
+                    void* kargs[] = {&qp};
+                    hipCheck(hipModuleLaunchKernel(tmodules[t]->function("cq_keys_b"), gx, static_cast<unsigned>(cnt), 1, 256, 1, 1, 0, s,
+                                                   kargs, nullptr),
+                             "launch batched keys");
+                }
+                if (!r0.program->fast_sparse) {
+                    void* dargs[] = {&qp, &sdp};
+                    hipCheck(hipModuleLaunchKernel(tmodules[t]->function("cq_dense_count_b"), gx, static_cast<unsigned>(cnt), 1, 256, 1, 1,
+                                                   0, s, dargs, nullptr),
+                             "launch batched dense count");
+                }
+            }
+            tfirst += cnt;
+        }
+        hipCheck(cq_launch_batch_shape(dshape, n, s), "batch shape");
+        for (int k = 0; k < n; ++k)
+            if (!shape_ok[static_cast<size_t>(k)]) ex->shape(runs[static_cast<size_t>(k)], ctx, dqp + k, dtot + k);
     };
 
     // A graph pays off for a batch shape that comes back: the first time a
@@ -595,7 +685,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
         enqueue(true);
     }
     hipCheck(hipEventRecord(ev_[1], s), "hipEventRecord");
-    auto* htot = reinterpret_cast<unsigned long long*>(stage + qp_bytes);
+    auto* htot = reinterpret_cast<unsigned long long*>(stage + upload_bytes);
     hipCheck(hipMemcpyAsync(htot, dtot, static_cast<size_t>(n) * 8, hipMemcpyDeviceToHost, s), "copy row counts");
     const auto h3 = std::chrono::steady_clock::now();
     hipCheck(hipStreamSynchronize(s), "batch sync");
@@ -660,7 +750,8 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
 
 BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgram*>& templates,
                                              const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
-                                             const cq_query_params* dqp, unsigned lds_limit, ExecutionContext* ctx) {
+                                             const cq_query_params* dqp, const BatchInit& init, unsigned lds_limit,
+                                             ExecutionContext* ctx) {
     QueryEngine& engine = db_.engine();
     hipStream_t s = ctx->stream;
     Tuned best;
@@ -696,7 +787,7 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
             const unsigned grid = scanGridShared(fn, bt, ctx->device, B.num_tiles, lds_bytes);
             float ms = 0;
             for (int rep = 0; rep < 2; ++rep) {  // first run warms the module and caches
-                prepareTables(templates, per_template, runs, dqp, ctx);
+                prepareTables(templates, per_template, runs, dqp, init, ctx);
                 hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
                 void* args[] = {&B};
                 hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(bt), 1, 1, lds_bytes, s, args, nullptr),

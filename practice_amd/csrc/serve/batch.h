@@ -120,10 +120,18 @@ private:
     void runScan(const std::vector<int>& members, std::vector<QueryProgram>& programs, const BatchOptions& opt, int scan_index,
                  std::vector<QueryResult>& results, BatchScanReport& rep,
                  const std::vector<const QueryProgram*>* module_templates);
+    // aggregate-state initialization of a batch's queries: the eligible ones
+    // (compiled, shared tables) in one launch over their descriptors
+    struct BatchInit {
+        const cq_init_desc* d = nullptr;
+        std::vector<char> eligible;
+        long long max_slots = 0;
+    };
     void prepareTables(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
-                       std::vector<QueryRun>& runs, const cq_query_params* dqp, ExecutionContext* ctx);
+                       std::vector<QueryRun>& runs, const cq_query_params* dqp, const BatchInit& init, ExecutionContext* ctx);
     Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
-                   std::vector<QueryRun>& runs, const cq_query_params* dqp, unsigned lds_limit, ExecutionContext* ctx);
+                   std::vector<QueryRun>& runs, const cq_query_params* dqp, const BatchInit& init, unsigned lds_limit,
+                   ExecutionContext* ctx);
     void dropGraph(Graph& g);
 };
 
