@@ -56,10 +56,11 @@ def parse_args():
     p.add_argument("--no-batched", action="store_true", help="skip the batched shared-scan measurement")
     p.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     p.add_argument("--batch-queries", type=int, default=64)
-    p.add_argument("--serve-rates", default="100,500,2000", help="Poisson arrival rates (req/s), comma separated")
-    p.add_argument("--serve-seconds", type=float, default=3.0, help="duration of each arrival rate")
+    p.add_argument("--serve-rates", default="50,100,200,500,1000,2000,2700",
+                   help="Poisson arrival rates (req/s), comma separated")
+    p.add_argument("--serve-seconds", type=float, default=2.0, help="duration of each arrival rate")
     p.add_argument("--serve-burst", type=int, default=1024, help="requests of the closed-loop burst")
-    p.add_argument("--serve-options", default="lanes=2,max_batch=64,cold_lanes=2",
+    p.add_argument("--serve-options", default="lanes=1,max_batch=128,cold_lanes=1",
                    help="server options, key=value,... (cq_server_start)")
     p.add_argument("--serve-adhoc", type=float, default=0.05, help="fraction of requests with a new plan shape")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -247,17 +248,34 @@ def bench_served(pa, Q, args):
                     time.sleep(d)
                 tickets.append(srv.submit(q))
 
+        # admitted-footprint trace: bytes reserved by requests in flight,
+        # sampled every 10 ms while the rate runs
+        samples = []
+        sampling = [True]
+
+        def sampler(t_start):
+            while sampling[0]:
+                samples.append((time.perf_counter() - t_start, srv.stats()["in_flight_bytes"]))
+                time.sleep(0.01)
+
         th = threading.Thread(target=producer)
         t0 = time.perf_counter()
+        sm = threading.Thread(target=sampler, args=(t0,))
+        sm.start()
         th.start()
         th.join()
         errors = drain(tickets)
         wall = time.perf_counter() - t0
+        sampling[0] = False
+        sm.join()
         st = srv.stats()
         rates.append({"rate": rate, "requests": len(reqs), "qps": round(st["completed"] / wall, 1),
                       "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2),
                       "mean_batch": round(st["mean_batch"], 1), "interpreted": st["interpreted_queries"],
-                      "promotions": st["promotions"], "errors": errors})
+                      "promotions": st["promotions"], "errors": errors,
+                      "footprint_mb": {"max": round(max((b for _, b in samples), default=0) / 2**20, 1),
+                                       "mean": round(sum(b for _, b in samples) / max(1, len(samples)) / 2**20, 1),
+                                       "every_100ms": [round(b / 2**20, 1) for t, b in samples[::10]]}})
         if errors:
             rates[-1]["last_error"] = st.get("last_error", "")[:200]
         print(f"[served] rate {rate:g}: {rates[-1]}", file=sys.stderr, flush=True)
@@ -274,12 +292,14 @@ def bench_served(pa, Q, args):
     if errors:
         burst["last_error"] = st.get("last_error", "")[:200]
     print(f"[served] burst: {burst}", file=sys.stderr, flush=True)
+    srv_budget = st["budget_bytes"]
     srv.stop()
     db.close()
     return {"workload": f"Poisson arrivals at {args.serve_rates} req/s for {args.serve_seconds:g} s each, then a burst of "
                         f"{args.serve_burst}; 4 templates + {args.serve_adhoc:.0%} new plan shapes, scale 100 "
                         f"(600,000,000 fact rows)",
-            "server_options": args.serve_options, "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
+            "server_options": args.serve_options, "budget_mb": round(srv_budget / 2**20, 1),
+            "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
             "saturated_speedup": round(burst["qps"] / serial_qps, 2)}
 
 

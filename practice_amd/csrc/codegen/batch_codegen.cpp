@@ -146,6 +146,54 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     if (!tabs.empty() || mask_bytes > 0) src << "    __syncthreads();\n";
     if (table_lds_bytes) *table_lds_bytes = static_cast<unsigned>(tab_bytes + mask_bytes);
 
+    // Launch-level probes: a (layout key, fact key column) that several
+    // templates of the launch probe alike (equal keys: equal table
+    // contents) is probed once per tile; those templates read its entries.
+    // One random L2 gather per row instead of one per template.
+    struct LaunchProbe {
+        std::string key, fk, dim, text;
+        bool fk_null = false;
+        std::vector<std::pair<size_t, size_t>> users;  // (template, join)
+    };
+    std::vector<LaunchProbe> lps;
+    for (size_t t = 0; t < templates.size(); ++t) {
+        const QueryProgram* p = templates[t];
+        if (!p->shared_tables || p->batch_shared_parts.size() != p->joins.size()) continue;
+        for (size_t j = 0; j < p->joins.size(); ++j) {
+            const JoinProgram& jp = p->joins[j];
+            if (!jp.shared || jp.s_probe.empty()) continue;
+            const std::string key = jp.s_key + "#" + jp.fk_col;
+            size_t k = 0;
+            while (k < lps.size() && lps[k].key != key) ++k;
+            if (k == lps.size()) {
+                LaunchProbe lp;
+                lp.key = key;
+                lp.fk = jp.fk_col;
+                lp.dim = jp.dim_table;
+                lp.text = jp.s_probe;
+                const auto it = std::find(p->fact_cols.begin(), p->fact_cols.end(), jp.fk_col);
+                lp.fk_null = it != p->fact_cols.end() && p->fact_nullable[static_cast<size_t>(it - p->fact_cols.begin())];
+                lps.push_back(std::move(lp));
+            }
+            lps[k].users.emplace_back(t, j);
+        }
+    }
+    lps.erase(std::remove_if(lps.begin(), lps.end(), [](const LaunchProbe& lp) { return lp.users.size() < 2; }), lps.end());
+    std::map<std::pair<size_t, size_t>, size_t> lp_of;
+    for (size_t g = 0; g < lps.size(); ++g)
+        for (const auto& u : lps[g].users) lp_of[u] = g;
+    auto subst = [](std::string text, const std::string& from, const std::string& to) {
+        for (size_t at = text.find(from); at != std::string::npos; at = text.find(from, at + to.size())) text.replace(at, from.size(), to);
+        return text;
+    };
+    auto anyQueries = [](const std::vector<std::pair<size_t, size_t>>& users) {
+        std::string c;
+        for (size_t u = 0; u < users.size(); ++u)
+            c += (u ? " || " : "") + std::string("B.tq_begin[") + std::to_string(users[u].first) + "] < B.tq_end[" +
+                 std::to_string(users[u].first) + "]";
+        return c;
+    };
+
     // the last template (kernel order) that reads each union column: its
     // load is nontemporal, earlier templates' loads keep the tile in cache
     std::map<std::string, size_t> last_use;
@@ -160,6 +208,50 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         << "#pragma unroll\n            for (int i = 0; i < IPT; ++i)\n"
         << "                if (base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3) < B.row_end) tile_fl |= 1u << i;\n"
         << "        }\n";
+    for (size_t g = 0; g < lps.size(); ++g) {
+        const LaunchProbe& lp = lps[g];
+        const std::string G = std::to_string(g);
+        const int u = slot[lp.fk];
+        src << "        // launch-level probe " << lp.dim << " on " << lp.fk << " (templates";
+        for (const auto& us : lp.users) src << " " << us.first;
+        src << ")\n"
+            << "        u32 ge" << G << "[IPT]; bool gm" << G << "[IPT], gz" << G << "[IPT];\n"
+            << "        if (" << anyQueries(lp.users) << ") {\n"
+            << "        int gk" << G << "[IPT];\n";
+        if (lp.fk_null) src << "        u32 gv" << G << " = 0u;\n";
+        src << "        if (full) {\n"
+            << "#pragma unroll\n            for (int v = 0; v < VPT; ++v) {\n"
+            << "                const long long r = base + (long long)v * (BT * 4) + tid * 4;\n"
+            << "                { const v4i x = *((const v4i*)(B.col[" << u << "] + r)); gk" << G << "[4 * v] = x.x; gk" << G
+            << "[4 * v + 1] = x.y; gk" << G << "[4 * v + 2] = x.z; gk" << G << "[4 * v + 3] = x.w; }\n";
+        if (lp.fk_null) src << "                gv" << G << " |= ((B.valid[" << u << "][r >> 5] >> (r & 31)) & 0xfu) << (4 * v);\n";
+        src << "            }\n"
+            << "        } else {\n"
+            << "#pragma unroll\n            for (int i = 0; i < IPT; ++i) {\n"
+            << "                const long long r = base + (long long)(i >> 2) * (BT * 4) + tid * 4 + (i & 3);\n"
+            << "                const bool in = r < B.row_end;\n"
+            << "                gk" << G << "[i] = in ? B.col[" << u << "][r] : 0;\n";
+        if (lp.fk_null) src << "                if (in) gv" << G << " |= chip::bit32(B.valid[" << u << "], r) << i;\n";
+        src << "            }\n"
+            << "        }\n"
+            << "        const u32* gt" << G << " = nullptr;\n";
+        for (size_t k = 0; k < lp.users.size(); ++k) {
+            const size_t t = lp.users[k].first, j = lp.users[k].second;
+            const auto it = tab_of.find({t, j});
+            src << "        " << (k ? "else " : "") << "if (B.tq_begin[" << t << "] < B.tq_end[" << t << "]) gt" << G << " = "
+                << (it != tab_of.end() ? "cq_tab" + std::to_string(it->second) : "B.qp[B.tq_begin[" + std::to_string(t) + "]].ht[" + std::to_string(j) + "]")
+                << ";\n";
+        }
+        std::string text = lp.text;
+        text = subst(text, "@FVTEST@", lp.fk_null ? "            if (!((gv" + G + " >> i) & 1u)) continue;\n" : "");
+        text = subst(text, "@X@", "gx" + G);
+        text = subst(text, "@M@", "gm" + G);
+        text = subst(text, "@Z@", "gz" + G);
+        text = subst(text, "@E@", "ge" + G);
+        text = subst(text, "@FK@", "gk" + G + "[i]");
+        text = subst(text, "@TAB@", "gt" + G);
+        src << text << "        }\n";
+    }
     for (size_t t = 0; t < templates.size(); ++t) {
         const QueryProgram* p = templates[t];
         src << "        // template " << t << ": " << p->signature << "\n"
@@ -202,7 +294,17 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
                 src << "        const u32* cq_sht" << j << " = " << (it != tab_of.end() ? "cq_tab" + std::to_string(it->second) : "P0.ht[" + std::to_string(j) + "]")
                     << ";\n";
             }
-            src << p->batch_shared;
+            for (size_t j = 0; j < p->joins.size(); ++j) {
+                const auto it = lp_of.find({t, j});
+                if (it == lp_of.end()) {
+                    src << (p->batch_shared_parts.size() == p->joins.size() ? p->batch_shared_parts[j] : (j ? std::string() : p->batch_shared));
+                    continue;
+                }
+                const std::string J = std::to_string(j), G = std::to_string(it->second);
+                src << "        // " << p->joins[j].dim_table << " on " << p->joins[j].fk_col << ": the launch-level probe's entries\n"
+                    << "        u32 (&e" << J << ")[IPT] = ge" << G << "; bool (&m" << J << ")[IPT] = gm" << G << "; bool (&z" << J
+                    << ")[IPT] = gz" << G << ";\n";
+            }
         }
         if (use_mask[t]) {
             src << withTemplate(p->batch_mask_body, t) << "        }\n";

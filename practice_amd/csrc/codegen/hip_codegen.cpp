@@ -2275,8 +2275,9 @@ private:
 
         // template-level probes, once per tile for all of its queries
         std::ostringstream sh;
+        prog_.batch_shared_parts.assign(static_cast<size_t>(nj), std::string());
         for (int j = 0; j < nj; ++j) {
-            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+            JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
             const int fs = factSlot(jp.fk_col);
             const std::string J = std::to_string(j);
             // e: entry, m: the key is in the table, z: probe key 0 (the
@@ -2284,41 +2285,59 @@ private:
             if (opt_.split_probes) {
                 // entry index of every row first, then all loads, then the
                 // present test (a dead row reads entry 0)
-                sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
-                   << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n"
-                   << "        {\n        u32 x" << J << "[IPT];\n"
-                   << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                   << "            x" << J << "[i] = 0u; m" << J << "[i] = false; z" << J << "[i] = false;\n"
-                   << "            if (!((tile_fl >> i) & 1u)) continue;\n";
-                if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
-                sh << "            const int fk = c" << fs << "[i];\n";
+                std::ostringstream t;
+                t << "        {\n        u32 @X@[IPT];\n"
+                  << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                  << "            @X@[i] = 0u; @M@[i] = false; @Z@[i] = false;\n"
+                  << "            if (!((tile_fl >> i) & 1u)) continue;\n"
+                  << "@FVTEST@"
+                  << "            const int fk = @FK@;\n";
                 if (jp.zmode != 0) {
                     const std::string zx = jp.zmode == 1 ? "0u" : jp.zmode == 2 ? std::to_string(jp.zx) + "u"
                                                                              : "(u32)(0 - " + std::to_string(jp.dmin) + ")";
-                    sh << "            if (fk == 0) { z" << J << "[i] = true; x" << J << "[i] = " << zx << "; continue; }\n";
+                    t << "            if (fk == 0) { @Z@[i] = true; @X@[i] = " << zx << "; continue; }\n";
                 }
-                sh << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
-                   << "            if (x >= " << jp.dspan << "u) continue;\n"
-                   << "            x" << J << "[i] = x; m" << J << "[i] = true;\n"
-                   << "        }\n";
-                if (jp.db == 0) sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) e" << J << "[i] = 0u;\n";
+                t << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
+                  << "            if (x >= " << jp.dspan << "u) continue;\n"
+                  << "            @X@[i] = x; @M@[i] = true;\n"
+                  << "        }\n";
+                if (jp.db == 0) t << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) @E@[i] = 0u;\n";
                 else
-                    sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) e" << J << "[i] = chip::dense_entry_aligned(cq_sht" << J
-                       << ", x" << J << "[i], " << jp.db << ");\n";
-                sh << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n";
-                if (jp.s_present) sh << "            if (m" << J << "[i] && !(e" << J << "[i] & 1u)) m" << J << "[i] = false;\n";
-                if (jp.zmode == 1) sh << "            if (z" << J << "[i]) e" << J << "[i] = 0u;\n";
-                sh << "            if (!m" << J << "[i] && !z" << J << "[i]) e" << J << "[i] = 0u;\n"
-                   << "        }\n        }\n";
+                    t << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) @E@[i] = chip::dense_entry_aligned(@TAB@, @X@[i], " << jp.db
+                      << ");\n";
+                t << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n";
+                if (jp.s_present) t << "            if (@M@[i] && !(@E@[i] & 1u)) @M@[i] = false;\n";
+                if (jp.zmode == 1) t << "            if (@Z@[i]) @E@[i] = 0u;\n";
+                t << "            if (!@M@[i] && !@Z@[i]) @E@[i] = 0u;\n"
+                  << "        }\n        }\n";
+                jp.s_probe = t.str();
+                std::string part = "        // shared probe " + jp.dim_table + " on " + jp.fk_col + "\n" + "        u32 e" + J +
+                                   "[IPT]; bool m" + J + "[IPT], z" + J + "[IPT];\n" + jp.s_probe;
+                const std::string fv = prog_.fact_nullable[static_cast<size_t>(fs)] ? "((v" + std::to_string(fs) + " >> i) & 1u)" : "";
+                auto subst = [](std::string text, const std::string& from, const std::string& to) {
+                    for (size_t at = text.find(from); at != std::string::npos; at = text.find(from, at + to.size())) text.replace(at, from.size(), to);
+                    return text;
+                };
+                part = subst(part, "@FVTEST@", fv.empty() ? "" : "            if (!" + fv + ") continue;\n");
+                part = subst(part, "@X@", "x" + J);
+                part = subst(part, "@M@", "m" + J);
+                part = subst(part, "@Z@", "z" + J);
+                part = subst(part, "@E@", "e" + J);
+                part = subst(part, "@FK@", "c" + std::to_string(fs) + "[i]");
+                part = subst(part, "@TAB@", "cq_sht" + J);
+                prog_.batch_shared_parts[static_cast<size_t>(j)] = part;
+                sh << part;
                 continue;
             }
-            sh << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
+            jp.s_probe.clear();
+            std::ostringstream one;
+            one << "        // shared probe " << jp.dim_table << " on " << jp.fk_col << "\n"
                << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n"
                << "#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
                << "            e" << J << "[i] = 0u; m" << J << "[i] = false; z" << J << "[i] = false;\n"
                << "            if (!((tile_fl >> i) & 1u)) continue;\n";
-            if (prog_.fact_nullable[static_cast<size_t>(fs)]) sh << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
-            sh << "            const int fk = c" << fs << "[i];\n";
+            if (prog_.fact_nullable[static_cast<size_t>(fs)]) one << "            if (!((v" << fs << " >> i) & 1u)) continue;\n";
+            one << "            const int fk = c" << fs << "[i];\n";
             auto entry = [&](const std::string& x) {
                 return jp.db == 0 ? std::string("0u") : "chip::dense_entry_aligned(cq_sht" + J + ", " + x + ", " + std::to_string(jp.db) + ")";
             };
@@ -2326,14 +2345,16 @@ private:
                 std::string ze = "0u";
                 if (jp.zmode == 2) ze = entry(std::to_string(jp.zx) + "u");
                 if (jp.zmode == 3) ze = entry("(u32)(0 - " + std::to_string(jp.dmin) + ")");
-                sh << "            if (fk == 0) { z" << J << "[i] = true; e" << J << "[i] = " << ze << "; continue; }\n";
+                one << "            if (fk == 0) { z" << J << "[i] = true; e" << J << "[i] = " << ze << "; continue; }\n";
             }
-            sh << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
+            one << "            const u32 x = chip::dense_index(fk, " << jp.dmin << ", " << jp.dspan << ", " << jp.cal << ");\n"
                << "            if (x >= " << jp.dspan << "u) continue;\n"
                << "            const u32 e = " << entry("x") << ";\n";
-            if (jp.s_present) sh << "            if (!(e & 1u)) continue;\n";
-            sh << "            e" << J << "[i] = e; m" << J << "[i] = true;\n"
+            if (jp.s_present) one << "            if (!(e & 1u)) continue;\n";
+            one << "            e" << J << "[i] = e; m" << J << "[i] = true;\n"
                << "        }\n";
+            prog_.batch_shared_parts[static_cast<size_t>(j)] = one.str();
+            sh << one.str();
         }
         prog_.batch_shared = sh.str();
 

@@ -92,6 +92,11 @@ struct JoinProgram {
     int s_present = 0;        // 1: bit 0 marks an inserted key
     int s_payload = -1;       // attribute index of the payload column
     std::string s_key;        // layout key: tables with equal keys hold equal entries
+    // the shared probe's loops with placeholders (@E@ entry, @M@ / @Z@ lane
+    // masks, @X@ entry index, @FK@ the key of item i, @FV@ its validity bit
+    // or empty, @TAB@ the table), so a batch can run one probe for several
+    // templates with the same layout and fact key
+    std::string s_probe;
     // (the shared probe reads the table through `cq_sht<j>`, which the batch
     // kernel points at LDS or at the table in HBM)
     bool lds = false;
@@ -179,6 +184,7 @@ struct QueryProgram {
     std::vector<std::pair<std::string, int>> batch_mask_tables;  // LDS u64 tables: name, entries
     std::string batch_mask_build, batch_mask_body;
     std::string batch_shared;
+    std::vector<std::string> batch_shared_parts;  // batch_shared per join (declarations + loops)
     // (scope of the batch kernel: cq_hbits = log2 slots of the query's LDS
     // hash table).  batch_hash_slot_bytes > 0 marks a hash region whose size
     // the launch chooses (stride = slot bytes << hbits, at least
