@@ -50,9 +50,13 @@ def test_shared_scan_loads_each_column_once(hdb):
     # C2 {orderdate, partkey, suppkey, revenue}, C3 {custkey, suppkey, orderdate, revenue}:
     # each of the 8 is streamed from HBM once per tile (nontemporal load by the
     # last template that reads it); the 4 earlier reads of shared columns are
-    # cached loads of the same tile
+    # cached loads of the same tile, and so is the key column read by each
+    # launch-level probe (a probe several templates make alike: here DDATE on
+    # lo_orderdate, made by all three)
     assert src.count("__builtin_nontemporal_load") == 8
-    assert src.count("*((const v4i*)") == 4
+    launch_probes = src.count("        // launch-level probe ")
+    assert launch_probes == 1 and "// launch-level probe DDATE on lo_orderdate" in src
+    assert src.count("*((const v4i*)") == 4 + launch_probes
     # three templates (the two C2 queries share one), one block each per tile
     assert src.count("        // template ") == 3
     pa.compile_source(src)
