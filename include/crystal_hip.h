@@ -174,6 +174,18 @@ const unsigned char* cq_result_validity(const cq_result* r, int col);
 /* "gpu_execute_ms","scan_kernel_ms","compile_ms","codegen_ms","parse_ms",
  * "host_fetch_ms","engine_ms","cache_hit" */
 double cq_result_timing(const cq_result* r, const char* name);
+/* A result column's name: its alias, else the group column's name or the
+ * aggregate's function (inferResultColumnNames, execution.cpp:540-579). */
+const char* cq_result_column_name(const cq_result* r, int col);
+/* The text block the reference console prints for a result
+ * (src/cli/terminal_app.cpp:263-302): "--- Result ---", "Columns: a | b",
+ * the rows ("Row i: x | y", or the bare cell of a one-column result; the
+ * last row_limit rows when row_limit > 0), "Rows returned: N", "Rows shown:
+ * K" when the limit cuts, "GPU execution time: X.XXX ms", with
+ * extended_timing the code generation / compilation / library load /
+ * materialization / engine lines, "--- End ---".  Two-call string protocol
+ * as cq_db_explain. */
+int cq_result_format(const cq_result* r, long long row_limit, int extended_timing, char* buf, size_t len, size_t* needed);
 /* The accessors above in bulk, for bindings that read many results (no
  * reference counterpart: QueryResult's fields, execution.h:56-96, are read
  * in place by C++ callers).  cq_result_shape: rows, columns, ordered flag,

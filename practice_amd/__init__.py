@@ -81,6 +81,8 @@ def _load() -> ctypes.CDLL:
         "cq_result_validity": (c_void_p, [c_void_p, c_int]),
         "cq_result_timing": (ctypes.c_double, [c_void_p, c_char_p]),
         "cq_result_shape": (c_int, [c_void_p, P(c_ll), P(c_int), P(c_int), c_void_p, c_void_p, c_int]),
+        "cq_result_column_name": (c_char_p, [c_void_p, c_int]),
+        "cq_result_format": (c_int, [c_void_p, c_ll, c_int, c_char_p, c_sz, P(c_sz)]),
         "cq_result_copy": (c_int, [c_void_p, c_void_p, c_void_p]),
         "cq_result_timings": (c_int, [c_void_p, c_void_p]),
         "cq_result_free": (None, [c_void_p]),
@@ -314,6 +316,16 @@ class Database:
         h = ctypes.c_void_p()
         _check(_lib.cq_db_execute_range(self._h, sql.encode(), row_begin, row_end, ctypes.byref(h)))
         return _fetch_result(h)
+
+    def execute_text(self, sql: str, row_limit: int = 1000, extended_timing: bool = False) -> str:
+        """execute() and the result as the reference console prints it
+        (cq_result_format): the text block its olap_bench runner parses."""
+        h = ctypes.c_void_p()
+        _check(_lib.cq_db_execute(self._h, sql.encode(), ctypes.byref(h)))
+        try:
+            return _string_out(_lib.cq_result_format, h, row_limit, int(extended_timing))
+        finally:
+            _lib.cq_result_free(h)
 
     def execute_noalloc(self, sql: str) -> Result:
         """execute() with every device scratch buffer taken from the library's

@@ -9,6 +9,7 @@
 #include "runtime/database.h"
 #include "runtime/storage.h"
 #include "runtime/execution.h"
+#include "runtime/result_text.h"
 #include "serve/batch.h"
 #include "serve/server.h"
 #include "kernels/noalloc.h"
@@ -436,6 +437,20 @@ const unsigned long long* cq_result_column(const cq_result* r, int c) {
 const unsigned char* cq_result_validity(const cq_result* r, int c) {
     if (!r || c < 0 || c >= static_cast<int>(r->r.validity.size())) return nullptr;
     return r->r.validity[static_cast<size_t>(c)].data();
+}
+const char* cq_result_column_name(const cq_result* r, int c) {
+    if (!r || c < 0 || c >= static_cast<int>(r->r.columns.size())) return nullptr;
+    return r->r.columns[static_cast<size_t>(c)].name.c_str();
+}
+int cq_result_format(const cq_result* r, long long row_limit, int extended_timing, char* buf, size_t len, size_t* needed) {
+    return guarded([&] {
+        if (!r) return fail(CQ_ERR_INVALID, "cq_result_format: null result");
+        db::ResultTextOptions o;
+        o.limit_enabled = row_limit > 0;
+        o.limit = row_limit > 0 ? static_cast<size_t>(row_limit) : 0;
+        o.extended_timing = extended_timing != 0;
+        return copyOut(db::formatResultText(r->r, o), buf, len, needed);
+    });
 }
 int cq_result_shape(const cq_result* r, long long* rows, int* cols, int* ordered, int* types, int* nullable, int max_cols) {
     if (!r || !rows || !cols || !ordered) return CQ_ERR_INVALID;

@@ -48,6 +48,21 @@ namespace {
 
 [[noreturn]] void unsupported(const std::string& what) { throw std::runtime_error(what); }
 
+// result column name of an expression without an alias
+// (defaultExpressionColumnName, execution.cpp:540-557)
+std::string defaultColumnName(const ExprNode* e) {
+    if (!e) return "expr";
+    switch (e->getType()) {
+        case ExprType::COLUMN_REF: return static_cast<const ColumnRefExpr*>(e)->column_name;
+        case ExprType::STAR: return "*";
+        case ExprType::CASE_WHEN: return "case_when";
+        case ExprType::LITERAL_INT:
+        case ExprType::LITERAL_FLOAT: return "literal";
+        case ExprType::LITERAL_NULL: return "NULL";
+        default: return "expr";
+    }
+}
+
 bool litTrue(const std::string& s) { return s == "1" || s == "true" || s == "(1)" || s == "(true)"; }
 bool litFalse(const std::string& s) { return s == "0" || s == "false" || s == "(0)" || s == "(false)"; }
 
@@ -400,7 +415,10 @@ private:
             oc.nullable = resultNullable(e);
             prog_.out.push_back(oc);
         }
-        for (size_t c = 0; c < prog_.out.size() && c < proj_->output_aliases.size(); ++c) prog_.out[c].name = proj_->output_aliases[c];
+        for (size_t c = 0; c < prog_.out.size(); ++c) {
+            if (c < proj_->output_aliases.size()) prog_.out[c].name = proj_->output_aliases[c];
+            if (prog_.out[c].name.empty()) prog_.out[c].name = defaultColumnName(proj_exprs_[c]);
+        }
         prog_.has_sort_node = sort_ != nullptr;
         if (sort_) {
             prog_.sort_keys = sort_->sort_keys;
@@ -783,7 +801,14 @@ private:
             oc.nullable = d.isCount() ? false : outNullable(d.agg_expr.get());
             prog_.out.push_back(oc);
         }
-        for (size_t c = 0; c < prog_.out.size() && c < a->output_aliases.size(); ++c) prog_.out[c].name = a->output_aliases[c];
+        // names: the alias, else the reference's default (a group column's
+        // name, an aggregate's function; execution.cpp:540-579)
+        for (size_t c = 0; c < prog_.out.size(); ++c) {
+            if (c < a->output_aliases.size()) prog_.out[c].name = a->output_aliases[c];
+            if (!prog_.out[c].name.empty()) continue;
+            const size_t ng = a->group_by_exprs.size();
+            prog_.out[c].name = c < ng ? defaultColumnName(a->group_by_exprs[c].get()) : a->aggregates[c - ng].func_name;
+        }
         // sparse output eligibility
         bool sparse = !prog_.has_avg && !prog_.has_having;
         for (const auto& oc : prog_.out) sparse = sparse && !oc.nullable;

@@ -1,0 +1,67 @@
+"""Failure paths of the plain query path, forced from the host: the device
+memory preflight (preflightDeviceMemoryOrThrow, src/core/execution.cpp:351-404)
+rejects a query its budget cannot hold, and the module cache evicts and
+recompiles when it is full.  Every kernel still exits normally."""
+import ctypes
+
+import pytest
+
+from tests.compare import assert_same
+from tests import queries as Q
+
+pytestmark = pytest.mark.gpu
+
+pa = pytest.importorskip("practice_amd")
+from oracle import oracle as orc  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def dbs():
+    spec = orc.GenSpec(sf=1, fact_rows=500_000, seed=21, nullable=Q.C3_NULLABLE)
+    odb = orc.load_ssb(orc.gen_tables(spec), spec)
+    gdb = pa.Database(0)
+    gdb.generate_ssb(sf=1, fact_rows=500_000, seed=21, nullable=Q.C3_NULLABLE)
+    gdb.finalize()
+    yield gdb, odb
+    gdb.close()
+
+
+def device_total_bytes() -> int:
+    hip = ctypes.CDLL("libamdhip64.so")
+    free_b, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free_b), ctypes.byref(total)) == 0
+    return total.value
+
+
+def test_preflight_rejects_query_beyond_memory_budget(dbs):
+    gdb, odb = dbs
+    total = device_total_bytes()
+    print(f"[failure] preflight: device memory {total} bytes", flush=True)
+    gdb.set_option("memory_guard", 1)
+    # a reserve that leaves 1 MiB of budget: the resident columns alone exceed it
+    gdb.set_option("memory_guard_reserve_bytes", total - (1 << 20))
+    try:
+        with pytest.raises(pa.CrystalError, match="Insufficient GPU memory for query before JIT execution"):
+            gdb.execute(Q.C1)
+    finally:
+        gdb.set_option("memory_guard_reserve_bytes", 512 << 20)
+    # the same query runs once the budget allows it
+    assert_same(gdb.execute(Q.C1), odb.execute(gdb.explain(Q.C1)))
+
+
+def test_module_cache_evicts_and_recompiles(dbs):
+    gdb, odb = dbs
+    gdb.set_option("module_cache_capacity", 1)
+    try:
+        compiles0 = gdb.cache_stat("compiles")
+        evictions0 = gdb.cache_stat("evictions")
+        for sql in (Q.C1, Q.C3, Q.C1, Q.C3):
+            print(f"[failure] eviction: {sql[:40]}", flush=True)
+            g = gdb.execute(sql)
+            assert_same(g, odb.execute(gdb.explain(sql)))
+            assert g.timing["cache_hit"] == 0.0  # the other plan's module took the only slot
+        assert gdb.cache_stat("evictions") - evictions0 >= 3
+        assert gdb.cache_stat("compiles") - compiles0 >= 4
+        assert gdb.cache_stat("size") == 1
+    finally:
+        gdb.set_option("module_cache_capacity", 256)
