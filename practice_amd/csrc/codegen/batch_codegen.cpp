@@ -153,6 +153,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     struct LaunchProbe {
         std::string key, fk, dim, text;
         bool fk_null = false;
+        bool zmarks = false;  // the text marks probe key 0
         std::vector<std::pair<size_t, size_t>> users;  // (template, join)
     };
     std::vector<LaunchProbe> lps;
@@ -162,7 +163,15 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         for (size_t j = 0; j < p->joins.size(); ++j) {
             const JoinProgram& jp = p->joins[j];
             if (!jp.shared || jp.s_probe.empty()) continue;
-            const std::string key = jp.s_key + "#" + jp.fk_col;
+            // Probe key 0 (the reference's empty-slot corner): a template
+            // whose every dimension key is in the table resolves it by the
+            // plain lookup (zmode 0), others mark it (z) and read the entry
+            // of the key sharing its slot.  Where that key lies outside the
+            // key range (zmode 1, 2) one probe text serves both kinds: a
+            // zmode-0 template reads key 0 as absent (m false) either way.
+            // Where key 0 is itself a key (zmode 3) the marked form hides the
+            // plain match, so those templates probe apart from the others.
+            const std::string key = jp.s_key + "#" + jp.fk_col + (jp.zmode == 3 ? "#z3" : "");
             size_t k = 0;
             while (k < lps.size() && lps[k].key != key) ++k;
             if (k == lps.size()) {
@@ -170,10 +179,14 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
                 lp.key = key;
                 lp.fk = jp.fk_col;
                 lp.dim = jp.dim_table;
-                lp.text = jp.s_probe;
                 const auto it = std::find(p->fact_cols.begin(), p->fact_cols.end(), jp.fk_col);
                 lp.fk_null = it != p->fact_cols.end() && p->fact_nullable[static_cast<size_t>(it - p->fact_cols.begin())];
                 lps.push_back(std::move(lp));
+            }
+            // the text of a template that marks key 0, when one does
+            if (lps[k].text.empty() || (!lps[k].zmarks && jp.zmode != 0)) {
+                lps[k].text = jp.s_probe;
+                lps[k].zmarks = jp.zmode != 0;
             }
             lps[k].users.emplace_back(t, j);
         }

@@ -2218,12 +2218,42 @@ private:
             const auto& tm = meta(jp.dim_table);
             const auto* ks = stats(jp.pk_col);
             jp.s_present = (static_cast<uint64_t>(jp.dspan) == tm.getSize() && ks->cardinality_ == tm.getSize()) ? 0 : 1;
-            int off = jp.s_present;
-            for (const auto& c : cols) {
-                if (std::find(jp.s_cols.begin(), jp.s_cols.end(), c) != jp.s_cols.end()) continue;
+            auto bitsOf = [&](const std::string& c) -> int {
                 const auto* cs = stats(c);
-                if (!cs || tableOf(c) != jp.dim_table) return false;
-                const int b = bitsFor(static_cast<uint64_t>(cs->max_value_ - cs->min_value_));
+                return cs ? bitsFor(static_cast<uint64_t>(cs->max_value_ - cs->min_value_)) : 64;
+            };
+            int need = jp.s_present;
+            std::vector<std::string> packed;
+            for (const auto& c : cols) {
+                if (std::find(packed.begin(), packed.end(), c) != packed.end()) continue;
+                if (!stats(c) || tableOf(c) != jp.dim_table) return false;
+                packed.push_back(c);
+                need += bitsOf(c);
+            }
+            if (need > 32) return false;
+            // The dimension's other small attributes (at most 32 values)
+            // join the entry when they fit its 32 bits and the table stays
+            // small (its power-of-two width unchanged, or at most 1 MiB):
+            // templates that read different attributes of one dimension then
+            // build the same table, and a batch probes it once for all of
+            // them.  Attributes are packed in catalog column order.
+            for (const auto& c : tm.getColumnNames()) {
+                if (c == jp.pk_col || std::find(packed.begin(), packed.end(), c) != packed.end() || !stats(c)) continue;
+                const int b = bitsOf(c);
+                if (b > 5 || need + b > 32) continue;
+                const int w0 = pow2Width(need), w1 = pow2Width(need + b);
+                if (w1 != w0 && static_cast<uint64_t>(jp.dspan) * static_cast<uint64_t>(w1) > 8ull * (1u << 20)) continue;
+                packed.push_back(c);
+                need += b;
+            }
+            std::vector<std::string> order = tm.getColumnNames();
+            std::stable_sort(packed.begin(), packed.end(), [&](const std::string& x, const std::string& y) {
+                return std::find(order.begin(), order.end(), x) < std::find(order.begin(), order.end(), y);
+            });
+            int off = jp.s_present;
+            for (const auto& c : packed) {
+                const auto* cs = stats(c);
+                const int b = bitsOf(c);
                 if (c == jp.payload_col) jp.s_payload = static_cast<int>(jp.s_cols.size());
                 jp.s_cols.push_back(c);
                 jp.s_min.push_back(cs->min_value_);

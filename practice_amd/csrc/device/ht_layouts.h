@@ -97,6 +97,9 @@ __device__ __forceinline__ void ht_insert_compact(u32* w, int k, int kmin, int l
     atomicOr(&w[bit >> 5], e << (bit & 31));
 }
 
+// the low `db` bits (db = 32: all of them; a shift by 32 is undefined)
+__device__ __forceinline__ u32 width_mask(int db) { return db >= 32 ? 0xffffffffu : (1u << db) - 1u; }
+
 // Key-indexed layout.  Entry x = fk - dmin of `db` bits holds
 // payload - pbase; with `off` set, code 0 marks a key that was not inserted
 // (filtered out on the build side, or no such row) and pbase = pmin - 1;
@@ -111,7 +114,7 @@ template <bool STREAM = false>
 __device__ __forceinline__ u32 dense_entry(const u32* w, u32 x, int db) {
     const u32 b = x * (u32)db;
     const u32 lo = ht_word<STREAM>(w, b >> 5) >> (b & 31);
-    const u32 mask = (1u << db) - 1u;
+    const u32 mask = width_mask(db);
     if ((b & 31) + (u32)db <= 32u) return lo & mask;
     return (lo | (ht_word<STREAM>(w, (b >> 5) + 1) << (32u - (b & 31)))) & mask;
 }
@@ -119,7 +122,7 @@ __device__ __forceinline__ u32 dense_entry(const u32* w, u32 x, int db) {
 template <bool STREAM = false>
 __device__ __forceinline__ u32 dense_entry_aligned(const u32* w, u32 x, int db) {
     const u32 b = x * (u32)db;
-    return (ht_word<STREAM>(w, b >> 5) >> (b & 31)) & ((1u << db) - 1u);
+    return (ht_word<STREAM>(w, b >> 5) >> (b & 31)) & width_mask(db);
 }
 // Entry index of key k: k - dmin, or for calendar keys (cal = first year,
 // every table key a yyyymmdd date) ((year - cal) * 12 + month - 1) * 31 +
@@ -234,7 +237,7 @@ __device__ __forceinline__ void dense_words(const u32* w, u32 x, int db, u32& lo
 template <bool ALIGNED, int ZMODE = -1>
 __device__ __forceinline__ bool dense_decode(u32 kind, u32 x, u32 lo, u32 hi, int db, int pbase, bool off, int& payload) {
     const u32 b = x * (u32)db;
-    const u32 mask = (1u << db) - 1u;
+    const u32 mask = width_mask(db);
     const u32 e = db == 0 ? 0u : (ALIGNED ? (lo >> (b & 31)) & mask : (u32)((((u64)hi << 32) | lo) >> (b & 31)) & mask);
     const bool in = db == 0 || !off || e != 0u;
     const int pe = pbase + (int)e;

@@ -76,6 +76,9 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
         for (const auto* t : templates) seen = seen || t->signature == p.signature;
         if (!seen) templates.push_back(&p);
     }
+    // the order a launch uses (runScan): by signature
+    std::sort(templates.begin(), templates.end(),
+              [](const QueryProgram* a, const QueryProgram* b) { return a->signature < b->signature; });
     std::vector<std::string> cols;
     std::vector<bool> nullable;
     return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable);

@@ -57,7 +57,11 @@ int main(int argc, char** argv) {
         for (int v = lo; v < lo + span; ++v)
             if (rng() % 4 != 0 || c % 5 == 0) keys.push_back(v);
         if (keys.empty()) keys.push_back(lo);
-        const int pmin = (int)(rng() % 100) - 50, prange = c % 4 == 1 ? 0 : (int)(rng() % 40);
+        // every seventh case: payloads up to 31 bits wide (key-indexed entries
+        // of 32 bits; the compact layout, at most 32 bits, is skipped there)
+        const bool wide = c % 7 == 3;
+        const int pmin = (int)(rng() % 100) - 50,
+                  prange = c % 4 == 1 ? 0 : (wide ? (1 << 20) + (int)(rng() % (1u << 30)) : (int)(rng() % 40));
         std::vector<int> pay(keys.size());
         for (auto& p : pay) p = pmin + (int)(rng() % (prange + 1));
         std::vector<char> inserted(keys.size());
@@ -79,11 +83,13 @@ int main(int argc, char** argv) {
         const int code_bits = pair ? bits_for((uint64_t)prange) : 0;
         int eb = 1;
         while (eb < 1 + tb + code_bits) eb <<= 1;
+        const bool compact_ok = eb <= 32;
+        if (!compact_ok) eb = 32;  // table sized but never filled or probed
         const size_t cwords = ((size_t)len * eb + 31) / 32;
         std::vector<unsigned int> cbuf(cwords + 16, 0xdeadbeefu);
         for (size_t i = 0; i < cwords; ++i) cbuf[8 + i] = 0;
         for (size_t i = 0; i < keys.size(); ++i)
-            if (inserted[i]) ht_insert_compact(&cbuf[8], keys[i], kmin, len, eb, tb, jlo, pair ? pmin : 0, pair ? pay[i] : 0);
+            if (inserted[i] && compact_ok) ht_insert_compact(&cbuf[8], keys[i], kmin, len, eb, tb, jlo, pair ? pmin : 0, pair ? pay[i] : 0);
         // key-indexed layout, exact and power-of-two widths
         const bool all_present = !filtered && (int)keys.size() == kmx - kmn + 1;
         const int doff = all_present ? 0 : 1;
@@ -129,12 +135,13 @@ int main(int argc, char** argv) {
                 }
                 if (fk == 0 && want) ++zero_matches;
                 int cp = 0, dp = 0;
-                const bool cm = ht_probe_compact(&cbuf[8], fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, cp);
+                const bool cm = compact_ok ? ht_probe_compact(&cbuf[8], fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, cp) : want;
+                if (!compact_ok) cp = wp;
                 if (cm != want || (want && pair && cp != wp)) {
                     printf("case %d: compact fk=%d got (%d,%d) want (%d,%d)\n", c, fk, cm, cp, want, wp);
                     return 1;
                 }
-                {  // the same probe in locate / load / decode steps; a dead row never matches
+                if (compact_ok) {  // the same probe in locate / load / decode steps; a dead row never matches
                     int sl = 0, sp = 0;
                     const u32 wi = compact_locate(fk, true, kmin, len, eb, sl);
                     const bool sm = compact_decode(ht_word<false>(&cbuf[8], wi), sl, fk, kmin, len, eb, tb, jlo, pair ? pmin : 0, sp);

@@ -51,11 +51,13 @@ def test_shared_scan_loads_each_column_once(hdb):
     # each of the 8 is streamed from HBM once per tile (nontemporal load by the
     # last template that reads it); the 4 earlier reads of shared columns are
     # cached loads of the same tile, and so is the key column read by each
-    # launch-level probe (a probe several templates make alike: here DDATE on
-    # lo_orderdate, made by all three)
+    # launch-level probe (a probe several templates make alike: DDATE on
+    # lo_orderdate, made by all three, and SUPPLIER on lo_suppkey, made by
+    # the C2 and C3 templates: both pack s_region and s_nation)
     assert src.count("__builtin_nontemporal_load") == 8
     launch_probes = src.count("        // launch-level probe ")
-    assert launch_probes == 1 and "// launch-level probe DDATE on lo_orderdate" in src
+    assert launch_probes == 2 and "// launch-level probe DDATE on lo_orderdate" in src
+    assert "// launch-level probe SUPPLIER on lo_suppkey" in src
     assert src.count("*((const v4i*)") == 4 + launch_probes
     # three templates (the two C2 queries share one), one block each per tile
     assert src.count("        // template ") == 3
@@ -201,14 +203,15 @@ def test_autotuned_batch_matches():
     _check_batch(gdb, odb, qs, res)
 
 
-def _key_zero_tables():
-    """SUPPLIER keyed 1..16 (reference slot (k - 0) % 16: key 16 shares slot 0
-    with a probe key 0) and a fact table whose supplier keys include 0 and
-    keys beyond the dimension."""
+def _key_zero_tables(first_key=1):
+    """SUPPLIER keyed first_key..first_key + 15 and a fact table whose
+    supplier keys include 0 and keys beyond the dimension.  Keys 1..16:
+    reference slot (k - 0) % 16, key 16 shares slot 0 with a probe key 0.
+    Keys 0..15: key 0 is a supplier itself."""
     import numpy as np
     rng = np.random.default_rng(7)
     n = 4096
-    sup = {"s_suppkey": (np.arange(1, 17, dtype=np.int32), None),
+    sup = {"s_suppkey": (np.arange(first_key, first_key + 16, dtype=np.int32), None),
            "s_region": (np.array([i % 3 for i in range(16)], np.int32), None),
            "s_nation": (np.array([i % 5 for i in range(16)], np.int32), None)}
     lo = {"lo_suppkey": (rng.integers(0, 19, n).astype(np.int32), None),
@@ -226,14 +229,37 @@ KEY_ZERO_QUERIES = [
 ]
 
 
-@pytest.mark.gpu
-def test_probe_key_zero_against_reference_empty_slot():
+@pytest.mark.parametrize("first_key", [1, 0])
+def test_launch_probe_marks_key_zero(first_key):
+    """Templates that filter SUPPLIER mark probe key 0 (it matches the
+    reference's empty slot when the key sharing that slot is filtered out);
+    the unfiltered template reads it by the plain lookup.  The launch-level
+    probe they share marks key 0 whatever the template order, and where key
+    0 is a supplier itself the filtered templates probe apart."""
     from tests.fixtures import build_pair
-    db, odb = build_pair(_key_zero_tables(), pks=[("SUPPLIER", "s_suppkey")], device=0)
+    db, _ = build_pair(_key_zero_tables(first_key), pks=[("SUPPLIER", "s_suppkey")], device=-1)
+    try:
+        for qs in (KEY_ZERO_QUERIES, KEY_ZERO_QUERIES[::-1]):
+            src = db.generate_batch_code(qs, {"graph": 0})
+            probes = [ln for ln in src.splitlines() if "// launch-level probe SUPPLIER" in ln]
+            assert len(probes) == 1, probes
+            assert "if (fk == 0) { gz0[i] = true;" in src
+            users = probes[0].split("(templates")[1].rstrip(")").split()
+            assert len(users) == (3 if first_key == 1 else 2), probes
+            pa.compile_source(src)
+    finally:
+        db.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("first_key", [1, 0])
+def test_probe_key_zero_against_reference_empty_slot(first_key):
+    from tests.fixtures import build_pair
+    db, odb = build_pair(_key_zero_tables(first_key), pks=[("SUPPLIER", "s_suppkey")], device=0)
     try:
         res = db.batch_execute(KEY_ZERO_QUERIES, {"graph": 0})
         for sql, r in zip(KEY_ZERO_QUERIES, res):
-            print(f"[key0] sql={sql}", flush=True)
+            print(f"[key0] first_key={first_key} sql={sql}", flush=True)
             o = odb.execute(db.explain(sql))
             assert_same(db.execute(sql), o)
             assert_same(r, o)
