@@ -271,7 +271,9 @@ def bench_served(pa, Q, args):
         st = srv.stats()
         rates.append({"rate": rate, "requests": len(reqs), "qps": round(st["completed"] / wall, 1),
                       "p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2), "max_ms": round(st["max_ms"], 2),
-                      "mean_batch": round(st["mean_batch"], 1), "interpreted": st["interpreted_queries"],
+                      "mean_batch": round(st["mean_batch"], 1), "batch_wall_ms": round(st["batch_wall_ms"], 2),
+                      "cold_p50_ms": round(st["cold_p50_ms"], 2), "cold_p99_ms": round(st["cold_p99_ms"], 2),
+                      "interpreted": st["interpreted_queries"],
                       "promotions": st["promotions"], "errors": errors,
                       "footprint_mb": {"max": round(max((b for _, b in samples), default=0) / 2**20, 1),
                                        "mean": round(sum(b for _, b in samples) / max(1, len(samples)) / 2**20, 1),

@@ -134,13 +134,24 @@ long long cq_db_table_rows(cq_db* db, const char* table);
  * current one is probed), "interpret" (1: run plans on the ahead-of-time
  * interpreter instead of hipRTC-compiled kernels; same results, no compile),
  * "vm_row_kernel", "vm_linear" (0: interpreted plans always run their bytecode,
- * also those with a linear form), "host_sort_rows", "prepared_cache", "module_cache_capacity" (compiled modules kept,
+ * also those with a linear form), "vm_grid_percent" (1-100: linear-form scans
+ * on that share of a full-device grid, leaving CUs to concurrent work), "host_sort_rows", "prepared_cache", "module_cache_capacity" (compiled modules kept,
  * least recently used evicted) */
 int cq_db_set_option(cq_db* db, const char* name, long long value);
 long long cq_db_cache_stat(cq_db* db, const char* name); /* "size","hits","misses","evictions","compiles" */
 int cq_db_clear_cache(cq_db* db);
 
 int cq_db_execute(cq_db* db, const char* sql, cq_result** out);
+/* Several statements run together; no reference counterpart (the
+ * reference runs one statement per call), for servers that hold many
+ * requests at once.  With the "interpret" option on, statements whose
+ * interpreted plans have a linear form over the same fact table share scans
+ * of at most 16 statements and 8 distinct fact columns (the columns are read
+ * once for all of them); every other statement runs as cq_db_execute runs
+ * it.  out[i] receives statement i's result, or NULL when that statement
+ * failed; the call then returns CQ_ERR_RUNTIME with the first failure's
+ * message. */
+int cq_db_execute_group(cq_db* db, const char* const* sqls, int n, cq_result** out);
 /* cq_db_execute for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * No reference counterpart.  Calls no hipMalloc, hipFree, hipHostMalloc or

@@ -68,6 +68,7 @@ def _load() -> ctypes.CDLL:
         "cq_db_execute": (c_int, [c_void_p, c_char_p, P(c_void_p)]),
         "cq_db_execute_range": (c_int, [c_void_p, c_char_p, c_ll, c_ll, P(c_void_p)]),
         "cq_db_execute_noalloc": (c_int, [c_void_p, c_char_p, P(c_void_p)]),
+        "cq_db_execute_group": (c_int, [c_void_p, P(c_char_p), c_int, P(c_void_p)]),
         "cq_db_explain": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
         "cq_db_generate_code": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
         "cq_db_describe": (c_int, [c_void_p, c_char_p, c_char_p, c_sz, P(c_sz)]),
@@ -316,6 +317,19 @@ class Database:
         h = ctypes.c_void_p()
         _check(_lib.cq_db_execute_range(self._h, sql.encode(), row_begin, row_end, ctypes.byref(h)))
         return _fetch_result(h)
+
+    def execute_group(self, sqls: Sequence[str]) -> List[Result]:
+        """Statements run together (cq_db_execute_group): with the "interpret"
+        option on, interpreted plans with a linear form over one fact table
+        share scans.  Raises on the first failure after freeing the rest."""
+        arr = (ctypes.c_char_p * len(sqls))(*[s.encode() for s in sqls])
+        outs = (ctypes.c_void_p * max(1, len(sqls)))()
+        rc = _lib.cq_db_execute_group(self._h, arr, len(sqls), outs)
+        msg = _lib.cq_last_error().decode() if rc != 0 else ""
+        res = [_fetch_result(ctypes.c_void_p(outs[i])) if outs[i] else None for i in range(len(sqls))]
+        if rc != 0:
+            raise CrystalError(rc, msg)
+        return res
 
     def execute_text(self, sql: str, row_limit: int = 1000, extended_timing: bool = False) -> str:
         """execute() and the result as the reference console prints it

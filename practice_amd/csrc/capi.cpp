@@ -303,6 +303,10 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else if (n == "vm_linear") c.vm_linear = v != 0;
+        else if (n == "vm_grid_percent") {
+            if (v < 1 || v > 100) return fail(CQ_ERR_INVALID, "vm_grid_percent must be in [1, 100]");
+            c.vm_grid_percent = static_cast<int>(v);
+        }
         else if (n == "host_sort_rows") c.host_sort_rows = v;
         else if (n == "prepared_cache") c.prepared_cache = v != 0;
         else if (n == "module_cache_capacity") {
@@ -341,6 +345,32 @@ int cq_db_execute_range(cq_db* d, const char* sql, long long b, long long e, cq_
         auto r = std::make_unique<cq_result>();
         r->r = d->inst->executeQueryRange(sql, b, e);
         *out = r.release();
+        return CQ_OK;
+    });
+}
+
+int cq_db_execute_group(cq_db* d, const char* const* sqls, int n, cq_result** out) {
+    return guarded([&] {
+        if (!d || n < 0 || (n > 0 && (!sqls || !out))) return fail(CQ_ERR_INVALID, "cq_db_execute_group: null argument");
+        std::vector<std::string> q;
+        for (int i = 0; i < n; ++i) {
+            if (!sqls[i]) return fail(CQ_ERR_INVALID, "cq_db_execute_group: null statement");
+            q.emplace_back(sqls[i]);
+            out[i] = nullptr;
+        }
+        std::vector<std::string> errs;
+        std::vector<db::QueryResult> res = d->inst->executeGroup(q, errs);
+        std::string first;
+        for (int i = 0; i < n; ++i) {
+            if (!errs[static_cast<size_t>(i)].empty()) {
+                if (first.empty()) first = "statement " + std::to_string(i) + ": " + errs[static_cast<size_t>(i)];
+                continue;
+            }
+            auto r = std::make_unique<cq_result>();
+            r->r = std::move(res[static_cast<size_t>(i)]);
+            out[i] = r.release();
+        }
+        if (!first.empty()) return fail(CQ_ERR_RUNTIME, first);
         return CQ_OK;
     });
 }

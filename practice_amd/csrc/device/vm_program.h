@@ -126,6 +126,25 @@ struct cq_lin_program {
     struct cq_lin_key key[CQ_LIN_KEYS];
 };
 
+// Several linear-form programs over the same fact rows in one pass
+// (k_lin_multi): the union of their columns is staged once per tile and
+// every program reads it.  Program k's linear form indexes the union
+// positions (col / valid); the programs and their parameter blocks are
+// arrays in device memory beside this descriptor.  LDS offsets are bytes
+// into the kernel's dynamic LDS after the staged columns; CQ_LIN_NONE: the
+// join table is probed in HBM / the group aggregates go to global atomics.
+#define CQ_LIN_BATCH 16
+#define CQ_LIN_NONE 0xffffffffu
+struct cq_lin_batch {
+    int nprog;
+    int ncols;
+    long long row_begin, row_end;
+    const int* col[CQ_LIN_COLS];
+    const unsigned int* valid[CQ_LIN_COLS];  // bitmap of a nullable column, else null
+    unsigned int tab_off[CQ_LIN_BATCH];
+    unsigned int grp_off[CQ_LIN_BATCH];
+};
+
 struct cq_vm_program {
     int ncode;
     struct cq_vm_instr code[CQ_VM_MAX_CODE];

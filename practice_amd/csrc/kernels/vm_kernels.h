@@ -27,5 +27,15 @@ bool cq_lin_supported(const cq_vm_program* V);
 // device (V: the program in device memory, host: its host copy, which
 // picks the kernel instantiation)
 int cq_lin_row_align();
+// grid_percent: the grid as a percentage of a full-device persistent grid
 hipError_t cq_lin_launch_scan(const cq_query_params* P, const cq_vm_program* V, const cq_vm_program* host, unsigned lds_group_bytes,
-                              int device, hipStream_t s);
+                              int device, int grid_percent, hipStream_t s);
+// Several linear-form programs over the same rows in one pass (B: the
+// descriptor, V / P: B.nprog programs, whose linear forms index B's union
+// columns, and their parameter blocks, all in device memory).  Dynamic LDS:
+// cq_lin_batch_stage_bytes(ncols) of staged columns, then rest_bytes of
+// join tables and group regions at the descriptor's offsets.  rows:
+// B.row_end - B.row_begin (row_begin a multiple of cq_lin_row_align()).
+unsigned cq_lin_batch_stage_bytes(int ncols);
+hipError_t cq_lin_launch_batch(const cq_lin_batch* B, const cq_vm_program* V, const cq_query_params* P, int ncols, unsigned rest_bytes,
+                               long long rows, int device, int grid_percent, hipStream_t s);

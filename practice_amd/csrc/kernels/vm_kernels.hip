@@ -565,45 +565,33 @@ __device__ __forceinline__ void agg_init(const cq_vm_program* __restrict__ V, in
     __syncthreads();
 }
 
-// Merge of a block: per-thread scalar slots racc (wave reduction, then one
-// atomic per slot and block), or the LDS groups with their keys decoded
-// from the group index.
-template <int BT, int NR>
-__device__ __forceinline__ void agg_flush(const cq_query_params& P, const cq_vm_program* __restrict__ V, int lds_groups, u64* s_acc,
-                                          u32* s_seen, const u64* s_agg, const u32* s_flag, const u64 (&racc)[NR], u32 rseen) {
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
+// A block's scalar slots (s_acc / s_seen, filled by its waves) into the
+// result: one atomic per slot, validity of the slots that saw an input.
+template <int BT>
+__device__ __forceinline__ void flush_scalar_slots(const cq_query_params& P, const cq_vm_program* __restrict__ V, const u64* s_acc,
+                                                   const u32* s_seen) {
+    const int sts = V->storage_ts;
+    for (int k = threadIdx.x; k < sts; k += BT) {
+        const int fn = slot_fn(V, k);
+        bool hidden = false;
+        for (int i = 0; i < V->naggs; ++i) hidden = hidden || V->agg_hidden[i] == k;
+        const bool scalar_valid_bit = k < V->naggs && fn != CQ_VM_COUNT && V->agg_hidden[k] < 0;
+        const u64 x = s_acc[k];
+        if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
+        else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
+        else if (x) atomicAdd(&P.result[k], x);
+        if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
+    }
+}
+
+// A block's LDS groups into the result: keys decoded from the group index,
+// one atomic per touched slot, validity bits from the flag words.
+template <int BT>
+__device__ __forceinline__ void flush_lds_groups(const cq_query_params& P, const cq_vm_program* __restrict__ V, const u64* s_agg,
+                                                 const u32* s_flag) {
     const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
     const long long G = V->groups;
-    if (ng == 0) {
-#pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            if (k >= sts) break;
-            const int fn = slot_fn(V, k);
-            const u64 w = wave_reduce(racc[k], fn);
-            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
-            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
-            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
-            if (lane == 0 && any) s_seen[k] = 1u;
-        }
-    }
-    __syncthreads();
-    if (ng == 0) {
-        for (int k = tid; k < sts; k += BT) {
-            const int fn = slot_fn(V, k);
-            bool hidden = false;
-            for (int i = 0; i < V->naggs; ++i) hidden = hidden || V->agg_hidden[i] == k;
-            const bool scalar_valid_bit = k < V->naggs && fn != CQ_VM_COUNT && V->agg_hidden[k] < 0;
-            const u64 x = s_acc[k];
-            if (fn == CQ_VM_MIN) { if (x != ~0ull) atomicMin(&P.result[k], x); }
-            else if (fn == CQ_VM_MAX) { if (x) atomicMax(&P.result[k], x); }
-            else if (x) atomicAdd(&P.result[k], x);
-            if (!hidden && scalar_valid_bit && s_seen[k]) chip::set_bit64(P.result_valid, k);
-        }
-        return;
-    }
-    if (!lds_groups) return;
-    for (long long g = tid; g < G; g += BT) {
+    for (long long g = threadIdx.x; g < G; g += BT) {
         const u32 f = s_flag[g];
         if (!(f & 0x80000000u)) continue;
         const long long base = g * sts;
@@ -628,6 +616,34 @@ __device__ __forceinline__ void agg_flush(const cq_query_params& P, const cq_vm_
             for (int s2 = 0; s2 < sts; ++s2)
                 if ((f >> s2) & 1u) chip::set_bit64(P.result_valid, base + s2);
     }
+}
+
+// Merge of a block: per-thread scalar slots racc (wave reduction, then one
+// atomic per slot and block), or the LDS groups with their keys decoded
+// from the group index.
+template <int BT, int NR>
+__device__ __forceinline__ void agg_flush(const cq_query_params& P, const cq_vm_program* __restrict__ V, int lds_groups, u64* s_acc,
+                                          u32* s_seen, const u64* s_agg, const u32* s_flag, const u64 (&racc)[NR], u32 rseen) {
+    const int lane = threadIdx.x & 63;
+    const int ng = V->ngroups, sts = V->storage_ts;
+    if (ng == 0) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            if (k >= sts) break;
+            const int fn = slot_fn(V, k);
+            const u64 w = wave_reduce(racc[k], fn);
+            const bool any = __ballot((rseen >> k) & 1u) != 0ull;
+            const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
+            if (lane == 0 && nonid) atomic_fn(&s_acc[k], w, fn);
+            if (lane == 0 && any) s_seen[k] = 1u;
+        }
+    }
+    __syncthreads();
+    if (ng == 0) {
+        flush_scalar_slots<BT>(P, V, s_acc, s_seen);
+        return;
+    }
+    if (lds_groups) flush_lds_groups<BT>(P, V, s_agg, s_flag);
 }
 
 // dynamic LDS: register file (nregs x 4 x VM4_BT u64), then the group
@@ -988,6 +1004,217 @@ __device__ __forceinline__ void lin_group_atomics(u64* base, Index stride, const
     }
 }
 
+// the rows of this lane that pass program L's fact filters
+__device__ __forceinline__ u32 lin_filters(const cq_query_params& P, const cq_lin_program& L, const int* wcols, const LinRows& q,
+                                           u32 alive) {
+    const int nterms = L.nterms;
+    for (int t = 0; t < nterms; ++t) {
+        int x[LIN_R];
+        lin_read(wcols + L.term[t].col * LIN_COL_INTS, q, x);
+        const int l = (int)P.lit[L.term[t].lit];
+        u32 m = 0u;
+        switch (L.term[t].op) {
+#define LIN_CMP(OPC, OPR)                                                              \
+    case OPC:                                                                           \
+        _Pragma("unroll") for (int r = 0; r < LIN_R; ++r) m |= (u32)(x[r] OPR l) << r; \
+        break;
+            LIN_CMP(CQ_OP_EQ, ==)
+            LIN_CMP(CQ_OP_NE, !=)
+            LIN_CMP(CQ_OP_LT, <)
+            LIN_CMP(CQ_OP_LE, <=)
+            LIN_CMP(CQ_OP_GT, >)
+            LIN_CMP(CQ_OP_GE, >=)
+#undef LIN_CMP
+            default: break;
+        }
+        alive &= m;
+    }
+    return alive;
+}
+
+// program L's probe of table `tab` (its LDS copy or the HBM table): the
+// rows with a match and their payloads
+__device__ __forceinline__ u32 lin_join(const cq_vm_program* V, const cq_lin_program& L, const u32* tab, const int* wcols, u64 vbits,
+                                        const LinRows& q, u32 alive, int (&pay)[LIN_R]) {
+    const cq_vm_join& J = V->joins[L.join];
+    int fk[LIN_R];
+    lin_read(wcols + L.join_col * LIN_COL_INTS, q, fk);
+    alive &= lin_bits(vbits, L.join_col);
+    const bool al = (J.db & (J.db - 1)) == 0;
+    return al ? lin_probe<true>(J, tab, fk, alive, pay) : lin_probe<false>(J, tab, fk, alive, pay);
+}
+
+// scalar aggregates: this lane's rows of the tile folded into racc slot by
+// slot (AVG counts into their hidden slots); returns the aggregates that
+// saw a valid input
+template <int NR>
+__device__ __forceinline__ u32 lin_scalar_fold(const cq_query_params& P, const cq_vm_program* V, const cq_lin_program& L,
+                                               const int* wcols, u64 vbits, const LinRows& q, const int (&pay)[LIN_R], u32 alive,
+                                               u64 (&racc)[NR]) {
+    u32 seen = 0u;
+    const int naggs = L.naggs;
+#pragma unroll
+    for (int i = 0; i < CQ_LIN_AGGS; ++i) {
+        if (i >= naggs) break;
+        const cq_lin_agg& A = L.agg[i];
+        const int fn = V->agg_fn[i];
+        const int hs = V->agg_hidden[i];
+        u32 vm = alive;
+        for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
+        u64 part;
+        if (fn == CQ_VM_COUNT) {
+            part = (u64)__builtin_popcount(vm);
+        } else {
+            u64 x[LIN_R];
+            lin_chain(P, A, A.nsteps, wcols, q, pay, x);
+            if (fn == CQ_VM_MIN) {
+                part = ~0ull;
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] < part) ? x[r] : part;
+            } else if (fn == CQ_VM_MAX) {
+                part = 0ull;
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] > part) ? x[r] : part;
+            } else {
+                part = 0ull;
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) part += ((vm >> r) & 1u) ? x[r] : 0ull;
+            }
+        }
+        if (fn == CQ_VM_MIN) racc[i] = part < racc[i] ? part : racc[i];
+        else if (fn == CQ_VM_MAX) racc[i] = part > racc[i] ? part : racc[i];
+        else racc[i] += part;
+        seen |= vm ? 1u << i : 0u;
+        const u64 cnt = (u64)__builtin_popcount(vm);
+#pragma unroll
+        for (int k = 0; k < NR; ++k) racc[k] += k == hs ? cnt : 0ull;
+    }
+    return seen;
+}
+
+// GROUP BY: group index of every live row (Horner over dense domains, no
+// modulo, in 32 bits: the index is the low word of the u64 Horner value,
+// and add, sub and mul commute with taking the low word), then the
+// aggregate atomics into LDS (lds_groups) or global group slots, with the
+// keys written and validity bits set in global memory in the latter case
+__device__ __forceinline__ void lin_group_tile(const cq_query_params& P, const cq_vm_program* V, const cq_lin_program& L,
+                                               const int* wcols, u64 vbits, const LinRows& q, const int (&pay)[LIN_R], u32 alive,
+                                               int lds_groups, u64* s_agg, u32* s_flag) {
+    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+    const long long G = V->groups;
+    const bool fast_sparse = V->fast_sparse != 0;
+    const int naggs = L.naggs;
+    int h[LIN_R];
+    u32 keyok[CQ_LIN_KEYS] = {0u, 0u};
+    {
+        u32 hv[LIN_R];
+#pragma unroll
+        for (int r = 0; r < LIN_R; ++r) hv[r] = 0u;
+#pragma unroll
+        for (int g = 0; g < CQ_LIN_KEYS; ++g) {
+            if (g >= ng) break;
+            const cq_lin_key& K = L.key[g];
+            int kv[LIN_R];
+            if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
+            else
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
+            keyok[g] = K.valid_col >= 0 ? lin_bits(vbits, K.valid_col) : q.inm;
+            const u32 mn = (u32)V->key_min[g], card = (u32)V->key_card[g];
+            const bool nl = V->key_nullable[g] != 0;
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) {
+                const u32 v = (u32)kv[r];
+                const u32 term = nl ? (((keyok[g] >> r) & 1u) ? v - mn + 1u : 0u) : v - mn;
+                hv[r] = g == 0 ? term : hv[r] * card + term;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < LIN_R; ++r) {
+            h[r] = (int)hv[r];
+            if ((unsigned)h[r] >= (unsigned long long)G) alive &= ~(1u << r);
+        }
+    }
+    if (__ballot(alive != 0u) == 0ull) return;
+    if (!lds_groups) {
+        // group rows in global memory: keys and their validity
+#pragma unroll
+        for (int g = 0; g < CQ_LIN_KEYS; ++g) {
+            if (g >= ng) break;
+            const cq_lin_key& K = L.key[g];
+            int kv[LIN_R];
+            if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
+            else
+#pragma unroll
+                for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) {
+                if (!((alive >> r) & 1u)) continue;
+                const long long out = (long long)h[r] * sts;
+                P.result[out + g] = (u64)(i64)kv[r];
+                if (!fast_sparse && ((keyok[g] >> r) & 1u)) set_bit_once(P.result_valid, out + g);
+            }
+        }
+    }
+    // slots that turn valid, per aggregate: COUNT on every hit, others
+    // on a valid input (AVG keeps its count instead)
+    u32 slotbits[CQ_LIN_AGGS] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < CQ_LIN_AGGS; ++i) {
+        if (i >= naggs) break;
+        const cq_lin_agg& A = L.agg[i];
+        const int fn = V->agg_fn[i];
+        const int hs = V->agg_hidden[i];
+        u32 vm = alive;
+        for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
+        u64 x[LIN_R];
+        lin_chain(P, A, fn == CQ_VM_COUNT ? 0 : A.nsteps, wcols, q, pay, x);
+        u64 one[LIN_R];
+#pragma unroll
+        for (int r = 0; r < LIN_R; ++r) one[r] = 1ull;
+        if (lds_groups) {
+            lin_group_atomics<int>(s_agg + i, nas, h, x, vm, fn);
+            if (hs >= 0) lin_group_atomics<int>(s_agg + (hs - ng), nas, h, one, vm, CQ_VM_SUM);
+        } else {
+            lin_group_atomics<long long>(P.result + ng + i, (long long)sts, h, x, vm, fn);
+            if (hs >= 0) lin_group_atomics<long long>(P.result + hs, (long long)sts, h, one, vm, CQ_VM_SUM);
+        }
+        slotbits[i] = fast_sparse ? 0u : (fn == CQ_VM_COUNT ? alive : (hs >= 0 ? 0u : vm));
+        if (!lds_groups && slotbits[i]) {
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r)
+                if ((slotbits[i] >> r) & 1u) set_bit_once(P.result_valid, (long long)h[r] * sts + ng + i);
+        }
+    }
+    if (lds_groups) {
+#pragma unroll
+        for (int r = 0; r < LIN_R; ++r) {
+            if (!((alive >> r) & 1u)) continue;
+            u32 want = 0x80000000u;
+            if (!fast_sparse) {
+#pragma unroll
+                for (int g = 0; g < CQ_LIN_KEYS; ++g)
+                    if (g < ng) want |= ((keyok[g] >> r) & 1u) << g;
+#pragma unroll
+                for (int k = 0; k < CQ_LIN_AGGS; ++k) want |= ((slotbits[k] >> r) & 1u) << (ng + k);
+            }
+            if ((s_flag[h[r]] & want) != want) atomicOr(&s_flag[h[r]], want);
+        }
+    }
+}
+
+// the wave tile starting at row tile * LIN_TILE of [row_begin, row_end)
+__device__ __forceinline__ void lin_tile_rows(LinRows& q, long long row_begin, long long row_end, long long tile) {
+    q.t0 = row_begin + tile * LIN_TILE;
+    q.full = q.t0 + LIN_TILE <= row_end;
+    q.inm = LIN_ALL;
+    if (!q.full) {
+        q.inm = 0u;
+#pragma unroll
+        for (int b = 0; b < LIN_R; ++b) q.inm |= (u32)(lin_row(q, b) <= q.last) << b;
+    }
+}
+
 // GROUPED: GROUP BY plans (the scalar accumulators exist only without).
 // Dynamic LDS: per wave ncols staged columns (LIN_COL_INTS ints each), the
 // join table when it is probed from LDS (join_lds_words, rounded up to an
@@ -1001,10 +1228,9 @@ __global__ __launch_bounds__(LIN_BT, GROUPED ? 3 : 4) void k_lin_scan(const cq_q
     constexpr int NR = GROUPED ? 1 : CQ_LIN_RACC;  // scalar slots kept per thread
     const cq_lin_program& L = V->lin;
     const int tid = threadIdx.x;
-    const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
     const long long G = V->groups;
-    const bool fast_sparse = V->fast_sparse != 0;
-    const int nc = L.ncols, nterms = L.nterms, naggs = L.naggs;
+    const int nas = V->storage_ts - V->ngroups;
+    const int nc = L.ncols;
     int* wcols = (int*)s_dyn + (tid >> 6) * nc * LIN_COL_INTS;  // this wave's staged columns
     u32* s_table = (u32*)(s_dyn + (long long)(LIN_BT / 64) * nc * LIN_COL_INTS / 2);
     const int tw = L.join >= 0 ? L.join_lds_words : 0;
@@ -1025,14 +1251,7 @@ __global__ __launch_bounds__(LIN_BT, GROUPED ? 3 : 4) void k_lin_scan(const cq_q
     q.lane = tid & 63;
     q.last = P.row_end - 1;
     for (long long tile = (long long)blockIdx.x * (LIN_BT / 64) + (tid >> 6); tile < ntiles; tile += waves) {
-        q.t0 = P.row_begin + tile * LIN_TILE;
-        q.full = q.t0 + LIN_TILE <= P.row_end;
-        q.inm = LIN_ALL;
-        if (!q.full) {
-            q.inm = 0u;
-#pragma unroll
-            for (int b = 0; b < LIN_R; ++b) q.inm |= (u32)(lin_row(q, b) <= q.last) << b;
-        }
+        lin_tile_rows(q, P.row_begin, P.row_end, tile);
         // every column of the plan into LDS and the validity bits of its
         // nullable columns (LIN_R bits per position), all in flight at once
         __builtin_amdgcn_s_waitcnt(0);  // the previous tile's LDS reads are done
@@ -1042,193 +1261,145 @@ __global__ __launch_bounds__(LIN_BT, GROUPED ? 3 : 4) void k_lin_scan(const cq_q
             vbits |= (u64)(L.nullable[c] ? lin_valid(P.valid[L.slot[c]], q) : q.inm) << (LIN_R * c);
         }
         __builtin_amdgcn_s_waitcnt(0);  // the staged columns have landed
-        u32 alive = q.inm;
-        // filters
-        for (int t = 0; t < nterms; ++t) {
-            int x[LIN_R];
-            lin_read(wcols + L.term[t].col * LIN_COL_INTS, q, x);
-            const int l = (int)P.lit[L.term[t].lit];
-            u32 m = 0u;
-            switch (L.term[t].op) {
-#define LIN_CMP(OPC, OPR)                                                              \
-    case OPC:                                                                           \
-        _Pragma("unroll") for (int r = 0; r < LIN_R; ++r) m |= (u32)(x[r] OPR l) << r; \
-        break;
-                LIN_CMP(CQ_OP_EQ, ==)
-                LIN_CMP(CQ_OP_NE, !=)
-                LIN_CMP(CQ_OP_LT, <)
-                LIN_CMP(CQ_OP_LE, <=)
-                LIN_CMP(CQ_OP_GT, >)
-                LIN_CMP(CQ_OP_GE, >=)
-#undef LIN_CMP
-                default: break;
-            }
-            alive &= m;
-        }
+        u32 alive = lin_filters(P, L, wcols, q, q.inm);
         if (__ballot(alive != 0u) == 0ull) continue;
         int pay[LIN_R];
 #pragma unroll
         for (int r = 0; r < LIN_R; ++r) pay[r] = 0;
         if (L.join >= 0) {
-            const cq_vm_join& J = V->joins[L.join];
-            const int jc = L.join_col;
-            int fk[LIN_R];
-            lin_read(wcols + jc * LIN_COL_INTS, q, fk);
-            alive &= lin_bits(vbits, jc);
-            const bool al = (J.db & (J.db - 1)) == 0;
-            if (tw > 0) alive = al ? lin_probe<true>(J, s_table, fk, alive, pay) : lin_probe<false>(J, s_table, fk, alive, pay);
-            else alive = al ? lin_probe<true>(J, ht, fk, alive, pay) : lin_probe<false>(J, ht, fk, alive, pay);
+            if (tw > 0) alive = lin_join(V, L, s_table, wcols, vbits, q, alive, pay);
+            else alive = lin_join(V, L, ht, wcols, vbits, q, alive, pay);
             if (__ballot(alive != 0u) == 0ull) continue;
         }
-        if (!GROUPED) {
-            // scalar aggregates: this lane's rows folded, then one update of
-            // the aggregate's slot (and its AVG count)
-#pragma unroll
-            for (int i = 0; i < CQ_LIN_AGGS; ++i) {
-                if (i >= naggs) break;
-                const cq_lin_agg& A = L.agg[i];
-                const int fn = V->agg_fn[i];
-                const int hs = V->agg_hidden[i];
-                u32 vm = alive;
-                for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
-                u64 part;
-                if (fn == CQ_VM_COUNT) {
-                    part = (u64)__builtin_popcount(vm);
-                } else {
-                    u64 x[LIN_R];
-                    lin_chain(P, A, A.nsteps, wcols, q, pay, x);
-                    if (fn == CQ_VM_MIN) {
-                        part = ~0ull;
-#pragma unroll
-                        for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] < part) ? x[r] : part;
-                    } else if (fn == CQ_VM_MAX) {
-                        part = 0ull;
-#pragma unroll
-                        for (int r = 0; r < LIN_R; ++r) part = (((vm >> r) & 1u) && x[r] > part) ? x[r] : part;
-                    } else {
-                        part = 0ull;
-#pragma unroll
-                        for (int r = 0; r < LIN_R; ++r) part += ((vm >> r) & 1u) ? x[r] : 0ull;
-                    }
-                }
-                if (fn == CQ_VM_MIN) racc[i] = part < racc[i] ? part : racc[i];
-                else if (fn == CQ_VM_MAX) racc[i] = part > racc[i] ? part : racc[i];
-                else racc[i] += part;
-                rseen |= vm ? 1u << i : 0u;
-                const u64 cnt = (u64)__builtin_popcount(vm);
-#pragma unroll
-                for (int k = 0; k < NR; ++k) racc[k] += k == hs ? cnt : 0ull;
-            }
-            continue;
-        }
-        // group index (Horner over dense domains, no modulo), in 32 bits:
-        // the index is the low word of the u64 Horner value, and add, sub
-        // and mul commute with taking the low word
-        int h[LIN_R];
-        u32 keyok[CQ_LIN_KEYS] = {0u, 0u};
-        {
-            u32 hv[LIN_R];
-#pragma unroll
-            for (int r = 0; r < LIN_R; ++r) hv[r] = 0u;
-#pragma unroll
-            for (int g = 0; g < CQ_LIN_KEYS; ++g) {
-                if (g >= ng) break;
-                const cq_lin_key& K = L.key[g];
-                int kv[LIN_R];
-                if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
-                else
-#pragma unroll
-                    for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
-                keyok[g] = K.valid_col >= 0 ? lin_bits(vbits, K.valid_col) : q.inm;
-                const u32 mn = (u32)V->key_min[g], card = (u32)V->key_card[g];
-                const bool nl = V->key_nullable[g] != 0;
-#pragma unroll
-                for (int r = 0; r < LIN_R; ++r) {
-                    const u32 v = (u32)kv[r];
-                    const u32 term = nl ? (((keyok[g] >> r) & 1u) ? v - mn + 1u : 0u) : v - mn;
-                    hv[r] = g == 0 ? term : hv[r] * card + term;
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < LIN_R; ++r) {
-                h[r] = (int)hv[r];
-                if ((unsigned)h[r] >= (unsigned long long)G) alive &= ~(1u << r);
-            }
-        }
-        if (__ballot(alive != 0u) == 0ull) continue;
-        if (!lds_groups) {
-            // group rows in global memory: keys and their validity
-#pragma unroll
-            for (int g = 0; g < CQ_LIN_KEYS; ++g) {
-                if (g >= ng) break;
-                const cq_lin_key& K = L.key[g];
-                int kv[LIN_R];
-                if (K.kind == CQ_LIN_COL) lin_read(wcols + K.col * LIN_COL_INTS, q, kv);
-                else
-#pragma unroll
-                    for (int r = 0; r < LIN_R; ++r) kv[r] = pay[r];
-#pragma unroll
-                for (int r = 0; r < LIN_R; ++r) {
-                    if (!((alive >> r) & 1u)) continue;
-                    const long long out = (long long)h[r] * sts;
-                    P.result[out + g] = (u64)(i64)kv[r];
-                    if (!fast_sparse && ((keyok[g] >> r) & 1u)) set_bit_once(P.result_valid, out + g);
-                }
-            }
-        }
-        // slots that turn valid, per aggregate: COUNT on every hit, others
-        // on a valid input (AVG keeps its count instead)
-        u32 slotbits[CQ_LIN_AGGS] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int i = 0; i < CQ_LIN_AGGS; ++i) {
-            if (i >= naggs) break;
-            const cq_lin_agg& A = L.agg[i];
-            const int fn = V->agg_fn[i];
-            const int hs = V->agg_hidden[i];
-            u32 vm = alive;
-            for (int c = 0; c < A.nvalid; ++c) vm &= lin_bits(vbits, A.valid_col[c]);
-            u64 x[LIN_R];
-            lin_chain(P, A, fn == CQ_VM_COUNT ? 0 : A.nsteps, wcols, q, pay, x);
-            u64 one[LIN_R];
-#pragma unroll
-            for (int r = 0; r < LIN_R; ++r) one[r] = 1ull;
-            if (lds_groups) {
-                lin_group_atomics<int>(s_agg + i, nas, h, x, vm, fn);
-                if (hs >= 0) lin_group_atomics<int>(s_agg + (hs - ng), nas, h, one, vm, CQ_VM_SUM);
-            } else {
-                lin_group_atomics<long long>(P.result + ng + i, (long long)sts, h, x, vm, fn);
-                if (hs >= 0) lin_group_atomics<long long>(P.result + hs, (long long)sts, h, one, vm, CQ_VM_SUM);
-            }
-            slotbits[i] = fast_sparse ? 0u : (fn == CQ_VM_COUNT ? alive : (hs >= 0 ? 0u : vm));
-            if (!lds_groups && slotbits[i]) {
-#pragma unroll
-                for (int r = 0; r < LIN_R; ++r)
-                    if ((slotbits[i] >> r) & 1u) set_bit_once(P.result_valid, (long long)h[r] * sts + ng + i);
-            }
-        }
-        if (lds_groups) {
-#pragma unroll
-            for (int r = 0; r < LIN_R; ++r) {
-                if (!((alive >> r) & 1u)) continue;
-                u32 want = 0x80000000u;
-                if (!fast_sparse) {
-#pragma unroll
-                    for (int g = 0; g < CQ_LIN_KEYS; ++g)
-                        if (g < ng) want |= ((keyok[g] >> r) & 1u) << g;
-#pragma unroll
-                    for (int k = 0; k < CQ_LIN_AGGS; ++k) want |= ((slotbits[k] >> r) & 1u) << (ng + k);
-                }
-                if ((s_flag[h[r]] & want) != want) atomicOr(&s_flag[h[r]], want);
-            }
-        }
+        if (!GROUPED) rseen |= lin_scalar_fold<NR>(P, V, L, wcols, vbits, q, pay, alive, racc);
+        else lin_group_tile(P, V, L, wcols, vbits, q, pay, alive, lds_groups, s_agg, s_flag);
     }
     agg_flush<LIN_BT>(P, V, lds_groups, s_acc, s_seen, s_agg, s_flag, racc, rseen);
 }
 
+// Several linear-form programs in one pass (cq_lin_batch): each wave tile
+// of the union columns is staged once and every program evaluates it.
+// Scalar programs fold each tile into a wave reduction and one LDS atomic
+// per slot (the program count is not known at compile time, so no
+// per-thread accumulators); grouped programs aggregate into their LDS
+// region or with global atomics.  Static LDS: the scalar slots of every
+// program; dynamic: the staged columns, then the programs' join tables and
+// group regions at the descriptor's offsets.  The programs and parameter
+// blocks come as restrict arrays, so their uniform fields are read with
+// scalar loads.
+__global__ __launch_bounds__(LIN_BT, 2) void k_lin_multi(const cq_lin_batch* __restrict__ Bp, const cq_vm_program* __restrict__ Vs,
+                                                         const cq_query_params* __restrict__ Ps) {
+    __shared__ u64 s_acc[CQ_LIN_BATCH * CQ_LIN_RACC];
+    __shared__ u32 s_seen[CQ_LIN_BATCH * CQ_LIN_RACC];
+    extern __shared__ u64 s_dyn[];
+    const cq_lin_batch& B = *Bp;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int nc = B.ncols, np = B.nprog;
+    int* wcols = (int*)s_dyn + (tid >> 6) * nc * LIN_COL_INTS;
+    char* s_rest = (char*)(s_dyn + (long long)(LIN_BT / 64) * nc * LIN_COL_INTS / 2);
+    for (int k = 0; k < np; ++k) {
+        const cq_vm_program* V = Vs + k;
+        const cq_query_params& P = Ps[k];
+        const cq_lin_program& L = V->lin;
+        if (B.tab_off[k] != CQ_LIN_NONE) {
+            u32* t = (u32*)(s_rest + B.tab_off[k]);
+            const u32* ht = P.ht[L.join];
+            for (int w = tid; w < L.join_lds_words; w += LIN_BT) t[w] = ht[w];
+        }
+        const int ng = V->ngroups, sts = V->storage_ts, nas = sts - ng;
+        if (ng == 0) {
+            for (int s = tid; s < CQ_LIN_RACC; s += LIN_BT) {
+                s_acc[k * CQ_LIN_RACC + s] = slot_is_min(V, s) ? ~0ull : 0ull;
+                s_seen[k * CQ_LIN_RACC + s] = 0u;
+            }
+        } else if (B.grp_off[k] != CQ_LIN_NONE) {
+            const long long G = V->groups;
+            u64* s_agg = (u64*)(s_rest + B.grp_off[k]);
+            u32* s_flag = (u32*)(s_agg + G * nas);
+            for (long long w = tid; w < G * nas; w += LIN_BT) s_agg[w] = slot_is_min(V, (int)(w % nas)) ? ~0ull : 0ull;
+            for (long long w = tid; w < G; w += LIN_BT) s_flag[w] = 0u;
+        }
+    }
+    __syncthreads();
+    const long long nrows = B.row_end - B.row_begin;
+    const long long ntiles = (nrows + LIN_TILE - 1) / LIN_TILE;
+    const long long waves = (long long)gridDim.x * (LIN_BT / 64);
+    LinRows q;
+    q.lane = lane;
+    q.last = B.row_end - 1;
+    for (long long tile = (long long)blockIdx.x * (LIN_BT / 64) + (tid >> 6); tile < ntiles; tile += waves) {
+        lin_tile_rows(q, B.row_begin, B.row_end, tile);
+        __builtin_amdgcn_s_waitcnt(0);  // the previous tile's LDS reads are done
+        u64 vbits = 0ull;
+        for (int c = 0; c < nc; ++c) {
+            lin_stage(B.col[c], wcols + c * LIN_COL_INTS, q);
+            vbits |= (u64)lin_valid(B.valid[c], q) << (LIN_R * c);
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // the staged columns have landed
+        for (int k = 0; k < np; ++k) {
+            const cq_vm_program* V = Vs + k;
+            const cq_query_params& P = Ps[k];
+            const cq_lin_program& L = V->lin;
+            u32 alive = lin_filters(P, L, wcols, q, q.inm);
+            if (__ballot(alive != 0u) == 0ull) continue;
+            int pay[LIN_R];
+#pragma unroll
+            for (int r = 0; r < LIN_R; ++r) pay[r] = 0;
+            if (L.join >= 0) {
+                if (B.tab_off[k] != CQ_LIN_NONE) alive = lin_join(V, L, (const u32*)(s_rest + B.tab_off[k]), wcols, vbits, q, alive, pay);
+                else alive = lin_join(V, L, P.ht[L.join], wcols, vbits, q, alive, pay);
+                if (__ballot(alive != 0u) == 0ull) continue;
+            }
+            const int ng = V->ngroups, sts = V->storage_ts;
+            if (ng == 0) {
+                u64 racc[CQ_LIN_RACC];
+#pragma unroll
+                for (int s = 0; s < CQ_LIN_RACC; ++s) racc[s] = slot_is_min(V, s) ? ~0ull : 0ull;
+                const u32 seen = lin_scalar_fold<CQ_LIN_RACC>(P, V, L, wcols, vbits, q, pay, alive, racc);
+#pragma unroll
+                for (int s = 0; s < CQ_LIN_RACC; ++s) {
+                    if (s >= sts) break;
+                    const int fn = slot_fn(V, s);
+                    const u64 w = wave_reduce(racc[s], fn);
+                    const bool nonid = fn == CQ_VM_MIN ? w != ~0ull : w != 0ull;
+                    const bool any = __ballot((seen >> s) & 1u) != 0ull;
+                    if (lane == 0 && nonid) atomic_fn(&s_acc[k * CQ_LIN_RACC + s], w, fn);
+                    if (lane == 0 && any) s_seen[k * CQ_LIN_RACC + s] = 1u;
+                }
+            } else {
+                const bool lg = B.grp_off[k] != CQ_LIN_NONE;
+                u64* s_agg = lg ? (u64*)(s_rest + B.grp_off[k]) : nullptr;
+                u32* s_flag = lg ? (u32*)(s_agg + V->groups * (sts - ng)) : nullptr;
+                lin_group_tile(P, V, L, wcols, vbits, q, pay, alive, lg ? 1 : 0, s_agg, s_flag);
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < np; ++k) {
+        const cq_vm_program* V = Vs + k;
+        const cq_query_params& P = Ps[k];
+        if (V->ngroups == 0) {
+            flush_scalar_slots<LIN_BT>(P, V, s_acc + k * CQ_LIN_RACC, s_seen + k * CQ_LIN_RACC);
+        } else if (B.grp_off[k] != CQ_LIN_NONE) {
+            const u64* s_agg = (const u64*)(s_rest + B.grp_off[k]);
+            flush_lds_groups<LIN_BT>(P, V, s_agg, (const u32*)(s_agg + V->groups * (V->storage_ts - V->ngroups)));
+        }
+    }
+}
+
 // lds_bytes: staged columns and (group_bytes of them) LDS group aggregates
+// blocks of a persistent grid: occupancy x CUs, scaled by grid_percent,
+// at most one per `rows_per_block` rows
+unsigned lin_grid(int per_cu, int cus, long long rows, long long rows_per_block, int grid_percent) {
+    const long long full = std::max<long long>(1, (long long)per_cu * cus * std::max(1, std::min(100, grid_percent)) / 100);
+    const long long tiles = (rows + rows_per_block - 1) / rows_per_block;
+    return (unsigned)std::max<long long>(1, std::min<long long>(tiles, full));
+}
+
 template <bool GROUPED>
 hipError_t lin_launch(const cq_query_params* P, const cq_vm_program* V, unsigned lds_bytes, unsigned group_bytes, int device,
-                      hipStream_t s) {
+                      int grid_percent, hipStream_t s) {
     const long long rows = P->row_end - P->row_begin;
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_lin_scan<GROUPED>), LIN_BT, lds_bytes) !=
@@ -1236,8 +1407,7 @@ hipError_t lin_launch(const cq_query_params* P, const cq_vm_program* V, unsigned
         per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
-    const long long tiles = (rows + (long long)LIN_BT * LIN_R - 1) / ((long long)LIN_BT * LIN_R);  // blocks' worth of wave tiles
-    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(tiles, (long long)per_cu * cus));
+    const unsigned grid = lin_grid(per_cu, cus, rows, (long long)LIN_BT * LIN_R, grid_percent);  // blocks' worth of wave tiles
     hipLaunchKernelGGL(k_lin_scan<GROUPED>, dim3(grid), dim3(LIN_BT), lds_bytes, s, *P, V, group_bytes ? 1 : 0);
     return hipGetLastError();
 }
@@ -1290,14 +1460,31 @@ bool cq_lin_supported(const cq_vm_program* V) {
 }
 
 hipError_t cq_lin_launch_scan(const cq_query_params* P, const cq_vm_program* V, const cq_vm_program* host, unsigned lds_group_bytes,
-                              int device, hipStream_t s) {
+                              int device, int grid_percent, hipStream_t s) {
     const long long rows = P->row_end - P->row_begin;
     if (rows <= 0) return hipSuccess;
     if (!cq_lin_supported(host)) return hipErrorInvalidValue;
     const unsigned stage = (unsigned)(LIN_BT / 64) * (unsigned)host->lin.ncols * LIN_COL_INTS * 4u;
     const unsigned table = host->lin.join >= 0 ? (unsigned)((host->lin.join_lds_words + 1) & ~1) * 4u : 0u;
-    if (host->ngroups > 0) return lin_launch<true>(P, V, stage + table + lds_group_bytes, lds_group_bytes, device, s);
-    return lin_launch<false>(P, V, stage + table, 0u, device, s);
+    if (host->ngroups > 0) return lin_launch<true>(P, V, stage + table + lds_group_bytes, lds_group_bytes, device, grid_percent, s);
+    return lin_launch<false>(P, V, stage + table, 0u, device, grid_percent, s);
+}
+
+unsigned cq_lin_batch_stage_bytes(int ncols) { return (unsigned)(LIN_BT / 64) * (unsigned)ncols * LIN_COL_INTS * 4u; }
+
+hipError_t cq_lin_launch_batch(const cq_lin_batch* B, const cq_vm_program* V, const cq_query_params* P, int ncols, unsigned rest_bytes,
+                               long long rows, int device, int grid_percent, hipStream_t s) {
+    if (rows <= 0) return hipSuccess;
+    if (ncols < 1 || ncols > CQ_LIN_COLS) return hipErrorInvalidValue;
+    const unsigned lds = cq_lin_batch_stage_bytes(ncols) + rest_bytes;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_lin_multi), LIN_BT, lds) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 1;
+    const unsigned grid = lin_grid(per_cu, cus, rows, (long long)LIN_BT * LIN_R, grid_percent);
+    hipLaunchKernelGGL(k_lin_multi, dim3(grid), dim3(LIN_BT), lds, s, B, V, P);
+    return hipGetLastError();
 }
 
 hipError_t cq_vm_launch_scan4(const cq_query_params* P, const cq_vm_program* V, int nregs, unsigned lds_group_bytes, int device,

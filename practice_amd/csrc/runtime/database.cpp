@@ -525,6 +525,38 @@ QueryResult DatabaseInstance::executeQueryRange(const std::string& sql, int64_t 
     return ctx_->result;
 }
 
+void DatabaseInstance::groupContexts(ExecutionContext& base, std::vector<std::unique_ptr<ExecutionContext>>& pool, size_t n) {
+    while (pool.size() < n) {
+        auto c = std::make_unique<ExecutionContext>();
+        c->device = base.device;
+        c->stream = base.stream;
+        c->owns_stream = false;
+        c->columns = base.columns;
+        pool.push_back(std::move(c));
+    }
+    for (size_t k = 0; k < n; ++k) {
+        ExecutionContext& c = *pool[k];
+        c.config = base.config;
+        c.loaded_device_bytes = base.loaded_device_bytes;
+        c.row_begin = 0;
+        c.row_end = -1;
+    }
+}
+
+std::vector<QueryResult> DatabaseInstance::executeGroup(const std::vector<std::string>& sqls, std::vector<std::string>& errs) {
+    if (device_ < 0) throw std::runtime_error("executeGroup needs a GPU device (host-only database)");
+    std::lock_guard<std::mutex> lock(mu_);
+    hipCheck(hipSetDevice(device_), "hipSetDevice");
+    groupContexts(*ctx_, group_ctx_, sqls.size());
+    std::vector<ExecutionContext*> cs;
+    for (size_t k = 0; k < sqls.size(); ++k) cs.push_back(group_ctx_[k].get());
+    errs = engine_->executeInterpretedGroup(sqls, cs);
+    std::vector<QueryResult> out(sqls.size());
+    for (size_t k = 0; k < sqls.size(); ++k)
+        if (errs[k].empty()) out[k] = std::move(cs[k]->result);
+    return out;
+}
+
 QueryProgram DatabaseInstance::prepare(const std::string& sql) const {
     if (ctx_->config.prepared_cache) return engine_->prepareCached(sql, ctx_->config.codegen);
     return engine_->prepare(sql, ctx_->config.codegen);

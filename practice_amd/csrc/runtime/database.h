@@ -99,6 +99,15 @@ public:
     // or dropped while such contexts run queries.
     std::unique_ptr<ExecutionContext> newContext();
     hipStream_t stream() const { return ctx_->stream; }
+    // Contexts on `base`'s stream over its columns, with its configuration
+    // (the members of an interpreted group); `pool` grows to n and keeps
+    // them.  A pool must be destroyed before its base context.
+    static void groupContexts(ExecutionContext& base, std::vector<std::unique_ptr<ExecutionContext>>& pool, size_t n);
+    // Statements run together on the database's stream
+    // (QueryEngine::executeInterpretedGroup): with the interpreted tier on,
+    // plans with a linear form share scans.  errs[k]: statement k's error
+    // ("" when it ran; its result is then out[k]).
+    std::vector<QueryResult> executeGroup(const std::vector<std::string>& sqls, std::vector<std::string>& errs);
     // Serializes use of the execution context (queries, batches, generation).
     std::mutex& contextMutex() const { return mu_; }
 
@@ -117,6 +126,7 @@ private:
     std::shared_ptr<HipModuleExecutor> executor_;
     std::unique_ptr<QueryEngine> engine_;
     std::unique_ptr<ExecutionContext> ctx_;
+    std::vector<std::unique_ptr<ExecutionContext>> group_ctx_;  // on ctx_'s stream (destroyed before it)
     mutable std::mutex mu_;
 
     void ensureTable(const std::string& table) const;

@@ -479,7 +479,8 @@ void HipModuleExecutor::scan(QueryRun& run, ExecutionContext* ctx) {
         if (rows <= 0) return;
         if (!ctx->config.vm_row_kernel && ctx->config.vm_linear && cq_lin_supported(p.vm.get()) &&
             run.P.row_begin % cq_lin_row_align() == 0) {
-            hipCheck(cq_lin_launch_scan(&run.P, run.vm, p.vm.get(), static_cast<unsigned>(p.vm_lds_bytes), ctx->device, ctx->stream),
+            hipCheck(cq_lin_launch_scan(&run.P, run.vm, p.vm.get(), static_cast<unsigned>(p.vm_lds_bytes), ctx->device,
+                                        ctx->config.vm_grid_percent, ctx->stream),
                      "interpreted scan (linear form)");
             return;
         }
@@ -882,6 +883,146 @@ void HipModuleExecutor::execute(const ModuleHandle& module, ExecutionContext* ct
     if (ke) (void)hipEventDestroy(ke);
 }
 
+bool HipModuleExecutor::linearGroupable(const QueryProgram& p) {
+    return p.vm && !p.projection && cq_lin_supported(p.vm.get());
+}
+
+void HipModuleExecutor::executeLinearGroup(const std::vector<ExecutionContext*>& ctxs) {
+    const size_t n = ctxs.size();
+    if (n == 0) return;
+    if (n > static_cast<size_t>(CQ_LIN_BATCH)) throw std::logic_error("linear group of more than CQ_LIN_BATCH plans");
+    ExecutionContext* c0 = ctxs.front();
+    hipStream_t s = c0->stream;
+    for (ExecutionContext* c : ctxs) {
+        if (c->stream != s || c->device != c0->device) throw std::logic_error("linear group: contexts on different streams");
+        if (!linearGroupable(c->program)) throw std::logic_error("linear group: a plan without a linear form");
+    }
+    hipCheck(hipSetDevice(c0->device), "hipSetDevice");
+    std::vector<QueryRun> runs(n);
+    // union of the plans' columns (by device address); every plan's linear
+    // form re-indexed to the union positions
+    std::vector<const int*> ucol;
+    std::vector<const unsigned int*> uvalid;
+    for (size_t k = 0; k < n; ++k) {
+        ExecutionContext* c = ctxs[k];
+        bind(nullptr, c->program, c, "", runs[k], nullptr);
+        const cq_query_params& P = runs[k].P;
+        if (P.row_begin != runs[0].P.row_begin || P.row_end != runs[0].P.row_end)
+            throw std::logic_error("linear group: plans over different rows");
+        if (P.row_begin % cq_lin_row_align() != 0) throw std::logic_error("linear group: unaligned first row");
+        auto prog = std::make_shared<cq_vm_program>(*c->program.vm);
+        cq_lin_program& L = prog->lin;
+        int map[CQ_LIN_COLS];
+        for (int c2 = 0; c2 < L.ncols; ++c2) {
+            const int* col = P.col[L.slot[c2]];
+            const unsigned int* vb = L.nullable[c2] ? P.valid[L.slot[c2]] : nullptr;
+            size_t u = 0;
+            while (u < ucol.size() && ucol[u] != col) ++u;
+            if (u == ucol.size()) {
+                if (ucol.size() == static_cast<size_t>(CQ_LIN_COLS)) throw std::logic_error("linear group: more than CQ_LIN_COLS columns");
+                ucol.push_back(col);
+                uvalid.push_back(vb);
+            } else if (uvalid[u] != vb) {
+                throw std::logic_error("linear group: a column read with and without its validity");
+            }
+            map[c2] = static_cast<int>(u);
+        }
+        for (int t = 0; t < L.nterms; ++t) L.term[t].col = map[L.term[t].col];
+        if (L.join >= 0) L.join_col = map[L.join_col];
+        for (int g = 0; g < L.ngroups; ++g) {
+            if (L.key[g].kind == CQ_LIN_COL) L.key[g].col = map[L.key[g].col];
+            if (L.key[g].valid_col >= 0) L.key[g].valid_col = map[L.key[g].valid_col];
+        }
+        for (int i = 0; i < L.naggs; ++i) {
+            cq_lin_agg& A = L.agg[i];
+            for (int v = 0; v < A.nvalid; ++v) A.valid_col[v] = map[A.valid_col[v]];
+            for (int st = 0; st < A.nsteps; ++st)
+                if (A.step[st].kind == CQ_LIN_COL) A.step[st].idx = map[A.step[st].idx];
+        }
+        c->program.vm = prog;  // setup() uploads this copy
+    }
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipCheck(hipEventCreate(&ev0), "hipEventCreate");
+    hipCheck(hipEventCreate(&ev1), "hipEventCreate");
+    hipCheck(hipEventRecord(ev0, s), "hipEventRecord");
+    for (size_t k = 0; k < n; ++k) setup(runs[k], ctxs[k], nullptr);
+
+    // LDS after the staged columns: group regions first (they replace
+    // global atomics), then join tables, within a budget that keeps two
+    // blocks per CU
+    const int nu = static_cast<int>(ucol.size());
+    const unsigned stage = cq_lin_batch_stage_bytes(nu);
+    const unsigned limit = 80u * 1024u > stage ? 80u * 1024u - stage : 0u;
+    cq_lin_batch hb;
+    std::memset(&hb, 0, sizeof hb);
+    hb.nprog = static_cast<int>(n);
+    hb.ncols = nu;
+    hb.row_begin = runs[0].P.row_begin;
+    hb.row_end = runs[0].P.row_end;
+    for (int u = 0; u < nu; ++u) {
+        hb.col[u] = ucol[static_cast<size_t>(u)];
+        hb.valid[u] = uvalid[static_cast<size_t>(u)];
+    }
+    unsigned rest = 0;
+    for (size_t k = 0; k < n; ++k) {
+        const QueryProgram& p = ctxs[k]->program;
+        hb.tab_off[k] = hb.grp_off[k] = CQ_LIN_NONE;
+        const unsigned gb = p.vm->ngroups > 0 ? static_cast<unsigned>((p.vm_lds_bytes + 7) & ~7ull) : 0u;
+        if (gb > 0 && rest + gb <= limit) {
+            hb.grp_off[k] = rest;
+            rest += gb;
+        }
+    }
+    for (size_t k = 0; k < n; ++k) {
+        const cq_lin_program& L = ctxs[k]->program.vm->lin;
+        const unsigned tb = L.join >= 0 && L.join_lds_words > 0 ? static_cast<unsigned>((L.join_lds_words + 1) & ~1) * 4u : 0u;
+        if (tb > 0 && rest + tb <= limit) {
+            hb.tab_off[k] = rest;
+            rest += tb;
+        }
+    }
+    // programs, parameter blocks and the descriptor -> device, as arrays
+    const size_t vbytes = n * sizeof(cq_vm_program), pbytes = n * sizeof(cq_query_params);
+    const size_t poff = (vbytes + 255) & ~static_cast<size_t>(255), boff = (poff + pbytes + 255) & ~static_cast<size_t>(255);
+    DeviceBuffer& db = c0->scratch("lin_group");
+    db.ensure(boff + sizeof(cq_lin_batch));
+    char* base = static_cast<char*>(db.data());
+    std::vector<char> host(boff + sizeof(cq_lin_batch));
+    for (size_t k = 0; k < n; ++k) {
+        std::memcpy(host.data() + k * sizeof(cq_vm_program), ctxs[k]->program.vm.get(), sizeof(cq_vm_program));
+        std::memcpy(host.data() + poff + k * sizeof(cq_query_params), &runs[k].P, sizeof(cq_query_params));
+    }
+    std::memcpy(host.data() + boff, &hb, sizeof hb);
+    hipCheck(hipMemcpyAsync(base, host.data(), host.size(), hipMemcpyHostToDevice, s), "upload linear group");
+    hipCheck(cq_lin_launch_batch(reinterpret_cast<const cq_lin_batch*>(base + boff), reinterpret_cast<const cq_vm_program*>(base),
+                                 reinterpret_cast<const cq_query_params*>(base + poff), nu, rest, hb.row_end - hb.row_begin,
+                                 c0->device, c0->config.vm_grid_percent, s),
+             "linear group scan");
+    for (size_t k = 0; k < n; ++k) shape(runs[k], ctxs[k], nullptr, nullptr);
+    // row counts of every plan in one copy each, one synchronization
+    auto* counts = static_cast<unsigned long long*>(c0->pinned().ensure(8 * n));
+    for (size_t k = 0; k < n; ++k) {
+        counts[k] = 0;
+        if (!runs[k].single_cell)
+            hipCheck(hipMemcpyAsync(&counts[k], runs[k].off + runs[k].cap_rows, 8, hipMemcpyDeviceToHost, s), "copy row count");
+    }
+    hipCheck(hipStreamSynchronize(s), "sync");
+    std::vector<unsigned long long> rows(counts, counts + n);
+    for (size_t k = 0; k < n; ++k) {
+        const auto f0 = std::chrono::steady_clock::now();
+        fetchPinned(runs[k], ctxs[k], rows[k]);
+        ctxs[k]->result = std::move(runs[k].result);
+        ctxs[k]->timing.host_fetch_ms = elapsedMs(f0, std::chrono::steady_clock::now());
+    }
+    hipCheck(hipEventRecord(ev1, s), "hipEventRecord");
+    hipCheck(hipEventSynchronize(ev1), "sync");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev0, ev1);
+    for (ExecutionContext* c : ctxs) c->timing.gpu_execute_ms = ms;
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+}
+
 void HipModuleExecutor::executeInto(const ModuleHandle& module, ExecutionContext* ctx, void* scratch, size_t bytes) {
     const QueryProgram& p = ctx->program;
     if (p.projection)
@@ -1072,7 +1213,7 @@ ModuleHandle QueryEngine::moduleForSource(const std::string& source, const std::
     return m;
 }
 
-void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
+void QueryEngine::planQuery(const std::string& sql, ExecutionContext* ctx, QueryTimingStats& timing) {
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t row_begin = ctx->row_begin, row_end = ctx->row_end;
     ctx->resetForQuery();
@@ -1085,28 +1226,33 @@ void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
         auto it = programs_.find(key);
         if (it != programs_.end()) known = it->second;
     }
-    QueryTimingStats timing;
     if (known) {
         ctx->program = *known;
-    } else {
-        auto tree = buildOptimizedTree(sql, catalog_.get());
-        const auto t1 = std::chrono::steady_clock::now();
-        try {
-            ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
-        } catch (const std::runtime_error& e) {
-            // plans the interpreter does not take (OR-key joins) run compiled
-            if (!ctx->config.codegen.vm || std::string(e.what()).rfind("interpreted tier:", 0) != 0) throw;
-            CodegenOptions co = ctx->config.codegen;
-            co.vm = false;
-            ctx->program = generateQueryProgram(tree.get(), *catalog_, co);
-        }
-        const auto t2 = std::chrono::steady_clock::now();
-        timing.parse_ms = elapsedMs(t0, t1);
-        timing.codegen_ms = elapsedMs(t1, t2);
-        std::lock_guard<std::mutex> lock(programs_mu_);
-        if (programs_.size() >= 1024) programs_.clear();
-        programs_[key] = std::make_shared<const QueryProgram>(ctx->program);
+        return;
     }
+    auto tree = buildOptimizedTree(sql, catalog_.get());
+    const auto t1 = std::chrono::steady_clock::now();
+    try {
+        ctx->program = generateQueryProgram(tree.get(), *catalog_, ctx->config.codegen);
+    } catch (const std::runtime_error& e) {
+        // plans the interpreter does not take (OR-key joins) run compiled
+        if (!ctx->config.codegen.vm || std::string(e.what()).rfind("interpreted tier:", 0) != 0) throw;
+        CodegenOptions co = ctx->config.codegen;
+        co.vm = false;
+        ctx->program = generateQueryProgram(tree.get(), *catalog_, co);
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    timing.parse_ms = elapsedMs(t0, t1);
+    timing.codegen_ms = elapsedMs(t1, t2);
+    std::lock_guard<std::mutex> lock(programs_mu_);
+    if (programs_.size() >= 1024) programs_.clear();
+    programs_[key] = std::make_shared<const QueryProgram>(ctx->program);
+}
+
+void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
+    const auto t0 = std::chrono::steady_clock::now();
+    QueryTimingStats timing;
+    planQuery(sql, ctx, timing);
     preflightDeviceMemoryOrThrow(*ctx, ctx->program);
     ModuleHandle m = ctx->program.vm ? nullptr : moduleFor(ctx->program, &timing);
     executor_->execute(m, ctx);
@@ -1116,6 +1262,109 @@ void QueryEngine::executeQuery(const std::string& sql, ExecutionContext* ctx) {
     timing.engine_ms = elapsedMs(t0, std::chrono::steady_clock::now());
     ctx->timing = timing;
     ctx->result.timing = timing;
+}
+
+std::vector<std::string> QueryEngine::executeInterpretedGroup(const std::vector<std::string>& sqls,
+                                                              const std::vector<ExecutionContext*>& ctxs) {
+    const size_t n = sqls.size();
+    if (ctxs.size() != n) throw std::invalid_argument("executeInterpretedGroup: one context per statement");
+    std::vector<std::string> errs(n);
+    std::vector<QueryTimingStats> timing(n);
+    std::vector<std::chrono::steady_clock::time_point> start(n);
+    HipModuleExecutor* hip = hipExecutor();
+    auto finishOne = [&](size_t k) {
+        ExecutionContext* c = ctxs[k];
+        timing[k].gpu_execute_ms = c->timing.gpu_execute_ms;
+        timing[k].scan_kernel_ms = c->timing.scan_kernel_ms;
+        timing[k].host_fetch_ms = c->timing.host_fetch_ms;
+        timing[k].engine_ms = elapsedMs(start[k], std::chrono::steady_clock::now());
+        c->timing = timing[k];
+        c->result.timing = timing[k];
+    };
+    auto runAlone = [&](size_t k) {
+        try {
+            ExecutionContext* c = ctxs[k];
+            ModuleHandle m = c->program.vm ? nullptr : moduleFor(c->program, &timing[k]);
+            executor_->execute(m, c);
+            finishOne(k);
+        } catch (const std::exception& e) {
+            errs[k] = e.what();
+        }
+    };
+    // plans; the ones with a linear form over the full fact range group by
+    // fact table and union columns, in statement order
+    std::vector<size_t> alone;
+    struct Group {
+        std::string table;
+        std::vector<size_t> members;
+        std::vector<std::string> cols;
+    };
+    std::vector<Group> groups;
+    for (size_t k = 0; k < n; ++k) {
+        start[k] = std::chrono::steady_clock::now();
+        ExecutionContext* c = ctxs[k];
+        try {
+            planQuery(sqls[k], c, timing[k]);
+            preflightDeviceMemoryOrThrow(*c, c->program);
+        } catch (const std::exception& e) {
+            errs[k] = e.what();
+            continue;
+        }
+        const QueryProgram& p = c->program;
+        if (!hip || !HipModuleExecutor::linearGroupable(p) || c->row_begin != 0 || c->row_end >= 0) {
+            alone.push_back(k);
+            continue;
+        }
+        std::vector<std::string> cols;
+        for (int i = 0; i < p.vm->lin.ncols; ++i) cols.push_back(p.fact_cols[static_cast<size_t>(p.vm->lin.slot[i])]);
+        bool placed = false;
+        for (Group& g : groups) {
+            if (g.table != p.fact_table || g.members.size() >= static_cast<size_t>(CQ_LIN_BATCH)) continue;
+            std::vector<std::string> u = g.cols;
+            for (const auto& col : cols)
+                if (std::find(u.begin(), u.end(), col) == u.end()) u.push_back(col);
+            if (u.size() > static_cast<size_t>(CQ_LIN_COLS)) continue;
+            g.cols = u;
+            g.members.push_back(k);
+            placed = true;
+            break;
+        }
+        if (!placed) {
+            Group g;
+            g.table = p.fact_table;
+            g.members.push_back(k);
+            for (const auto& col : cols)
+                if (std::find(g.cols.begin(), g.cols.end(), col) == g.cols.end()) g.cols.push_back(col);
+            groups.push_back(std::move(g));
+        }
+    }
+    for (const Group& g : groups) {
+        if (g.members.size() == 1) {
+            runAlone(g.members.front());
+            continue;
+        }
+        std::vector<ExecutionContext*> gc;
+        for (size_t k : g.members) gc.push_back(ctxs[k]);
+        try {
+            hip->executeLinearGroup(gc);
+            for (size_t k : g.members) finishOne(k);
+        } catch (const std::exception&) {
+            // the shared pass failed: each plan on its own, so one plan's
+            // error does not become every member's
+            for (size_t k : g.members) {
+                errs[k].clear();
+                try {
+                    planQuery(sqls[k], ctxs[k], timing[k]);  // the statement's own linear form again
+                } catch (const std::exception& e) {
+                    errs[k] = e.what();
+                    continue;
+                }
+                runAlone(k);
+            }
+        }
+    }
+    for (size_t k : alone) runAlone(k);
+    return errs;
 }
 
 }  // namespace db

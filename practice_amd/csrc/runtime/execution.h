@@ -122,6 +122,10 @@ struct EngineConfig {
     bool kernel_timing = false;          // record cq_scan duration with events
     bool vm_row_kernel = false;          // interpreted tier: always the one-row-per-thread scan
     bool vm_linear = true;               // interpreted tier: the linear-form kernel for plans that have one
+    // interpreted tier: the linear-form scans' grid as a percentage of a
+    // full-device persistent grid (below 100 the scan leaves CUs to
+    // concurrent work, such as a server's shared scans)
+    int vm_grid_percent = 100;
     long long host_sort_rows = 4096;     // ORDER BY results up to this size are sorted on the host
     bool prepared_cache = false;         // DatabaseInstance::prepare goes through QueryEngine::prepareCached
     CodegenOptions codegen;
@@ -359,6 +363,13 @@ public:
     // `scratch` (`bytes` long), the module must be loaded, no events.
     // Aggregate plans only (a projection's size is known after its scan).
     void executeInto(const ModuleHandle& module, ExecutionContext* ctx, void* scratch, size_t bytes);
+    // Interpreted plans in their linear form over the same fact rows, in one
+    // shared pass (k_lin_multi): each context's program is bound, set up,
+    // shaped and fetched as execute() does, one scan serves all of them.
+    // The contexts share one stream and device; at most CQ_LIN_BATCH plans
+    // reading at most CQ_LIN_COLS distinct fact columns (linearGroupable).
+    static bool linearGroupable(const QueryProgram& p);
+    void executeLinearGroup(const std::vector<ExecutionContext*>& ctxs);
 };
 
 // Persistent grid for a scan kernel: occupancy x CUs, at most one block per tile.
@@ -374,6 +385,12 @@ public:
                 std::shared_ptr<const Catalog> catalog);
     // SQL -> optimized plan -> program -> (cached) module -> execution
     void executeQuery(const std::string& sql, ExecutionContext* ctx);
+    // Interpreted statements run together: plans with a linear form over
+    // the same fact table share passes of at most CQ_LIN_BATCH plans and
+    // CQ_LIN_COLS distinct columns, the others run one by one.  The
+    // contexts share one stream; ctxs[k] receives statement k's result.
+    // Returns per statement its error message ("" when it ran).
+    std::vector<std::string> executeInterpretedGroup(const std::vector<std::string>& sqls, const std::vector<ExecutionContext*>& ctxs);
     // plan -> program without executing (codegen text / signature / layout)
     QueryProgram prepare(const std::string& sql, const CodegenOptions& options) const;
     // prepare() for statements that repeat with other integer constants:
@@ -406,6 +423,9 @@ public:
     IQueryCache& cache() { return *cache_; }
 
 private:
+    // the program of `sql` into ctx->program (statement cache, parse, code
+    // generation), with the parse / codegen times
+    void planQuery(const std::string& sql, ExecutionContext* ctx, QueryTimingStats& timing);
     std::shared_ptr<IQueryCache> cache_;
     std::shared_ptr<ICompiler> compiler_;
     std::shared_ptr<IExecutor> executor_;

@@ -40,6 +40,8 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "async_compile") o.async_compile = v != 0;
         else if (k == "interpret") o.interpret_cold = v != 0;
         else if (k == "cold_lanes") o.cold_lanes = static_cast<int>(v);
+        else if (k == "cold_group") o.cold_group = static_cast<int>(v);
+        else if (k == "cold_grid_percent") o.cold_grid_percent = static_cast<int>(v);
         else if (k == "per_template") o.per_template = v != 0;
         else if (k == "max_hot") o.max_hot = static_cast<int>(v);
         else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb") batch_opts += (batch_opts.empty() ? "" : ",") + item;
@@ -83,6 +85,7 @@ void QueryServer::start() {
     for (int i = 0; i < opt_.cold_lanes; ++i) {
         auto lane = std::make_unique<Lane>();
         lane->ctx = db_.newContext();
+        lane->ctx->config.vm_grid_percent = std::max(1, std::min(100, opt_.cold_grid_percent));
         cold_lanes_.push_back(std::move(lane));
     }
     for (auto& l : lanes_) {
@@ -147,6 +150,29 @@ long long QueryServer::submit(const std::string& sql) {
         done_cv_.notify_all();
         return ticket;
     }
+    // a cold request (a signature outside the shared scan, or a plan the
+    // shared scan cannot take) goes straight to the cold lanes when its
+    // footprint fits, instead of waiting for a hot lane between batches
+    if (opt_.interpret_cold && opt_.async_compile && (r.plain_only || !inScanSet(r.program->signature)) &&
+        in_flight_bytes_ + r.footprint <= opt_.budget_bytes) {
+        bool compile_work = false;
+        if (!r.plain_only) {
+            const std::string& sig = r.program->signature;
+            const int n = ++seen_[sig];
+            if (n >= opt_.promote_after && !promoting_.count(sig)) {
+                promoting_.insert(sig);
+                promote_q_.push_back(r.program);
+                compile_work = true;
+            }
+        }
+        in_flight_bytes_ += r.footprint;
+        r.interpreted = !r.plain_only;
+        r.cold = true;
+        cold_q_.push_back(std::move(r));
+        cold_cv_.notify_one();
+        if (compile_work) compile_cv_.notify_all();
+        return ticket;
+    }
     queue_.push_back(std::move(r));
     cv_.notify_one();
     return ticket;
@@ -159,7 +185,10 @@ void QueryServer::finish(Request& r, int rc, const std::string& err, QueryResult
     if (result) d.result = std::move(*result);
     d.latency_ms = elapsedMs(r.arrival, std::chrono::steady_clock::now());
     std::lock_guard<std::mutex> lock(mu_);
-    if (rc == 0) latencies_.push_back(d.latency_ms);
+    if (rc == 0) {
+        latencies_.push_back(d.latency_ms);
+        if (r.cold) cold_latencies_.push_back(d.latency_ms);
+    }
     else ++rejected_;
     done_[r.ticket] = std::move(d);
     done_cv_.notify_all();
@@ -299,8 +328,24 @@ void QueryServer::laneLoop(Lane* lane) {
             continue;
         }
 
-        size_t released = 0;
+        {
+            // cold requests (footprint already reserved) run on the cold
+            // lanes, concurrently with this lane's batch
+            std::lock_guard<std::mutex> lock(mu_);
+            for (auto& r : interp) {
+                r.interpreted = true;
+                r.cold = true;
+                cold_q_.push_back(std::move(r));
+            }
+            for (auto& r : plain) {
+                r.interpreted = false;
+                r.cold = true;
+                cold_q_.push_back(std::move(r));
+            }
+        }
+        if (!interp.empty() || !plain.empty()) cold_cv_.notify_all();
         if (!batch.empty()) {
+            size_t released = 0;
             std::vector<QueryProgram> programs;
             programs.reserve(batch.size());
             for (auto& r : batch) programs.push_back(*r.program);
@@ -308,6 +353,7 @@ void QueryServer::laneLoop(Lane* lane) {
             for (const auto& t : set) templates.push_back(t.get());
             std::vector<QueryResult> results;
             std::string err;
+            const auto tb = std::chrono::steady_clock::now();
             try {
                 if (opt_.per_template) {
                     // one launch per signature, each on its own (precompiled) module
@@ -338,30 +384,15 @@ void QueryServer::laneLoop(Lane* lane) {
             {
                 std::lock_guard<std::mutex> lock(mu_);
                 in_flight_bytes_ -= released;
+                batch_wall_ms_ += elapsedMs(tb, std::chrono::steady_clock::now());
                 ++batches_;
                 batched_queries_ += static_cast<long long>(batch.size());
             }
-            released = 0;
             for (size_t i = 0; i < batch.size(); ++i) {
                 if (err.empty()) finish(batch[i], 0, "", &results[i]);
                 else finish(batch[i], -3, err, nullptr);
             }
         }
-        {
-            // cold requests (footprint already reserved) run on the cold
-            // lanes, concurrently with this lane's batches
-            std::lock_guard<std::mutex> lock(mu_);
-            for (auto& r : interp) {
-                r.interpreted = true;
-                cold_q_.push_back(std::move(r));
-            }
-            for (auto& r : plain) {
-                r.interpreted = false;
-                cold_q_.push_back(std::move(r));
-            }
-            in_flight_bytes_ -= released;
-        }
-        if (!interp.empty() || !plain.empty()) cold_cv_.notify_all();
         cv_.notify_all();
     }
 }
@@ -376,17 +407,64 @@ void QueryServer::release(const Request& r) {
 
 void QueryServer::coldLoop(Lane* lane) {
     for (;;) {
-        Request r;
+        std::vector<Request> rs;
         {
             std::unique_lock<std::mutex> lock(mu_);
             cold_cv_.wait(lock, [&] { return stop_ || !cold_q_.empty(); });
             if (stop_) return;
-            r = std::move(cold_q_.front());
+            rs.push_back(std::move(cold_q_.front()));
             cold_q_.pop_front();
+            // interpreted requests queued behind it run with it
+            if (rs.front().interpreted)
+                for (auto it = cold_q_.begin(); it != cold_q_.end() && static_cast<int>(rs.size()) < opt_.cold_group;) {
+                    if (!it->interpreted) {
+                        ++it;
+                        continue;
+                    }
+                    rs.push_back(std::move(*it));
+                    it = cold_q_.erase(it);
+                }
+        }
+        ExecutionContext* ctx = lane->ctx.get();
+        if (rs.size() > 1) {
+            // interpreted requests together: linear forms share scans
+            std::vector<std::string> sqls;
+            for (const auto& r : rs) sqls.push_back(r.sql);
+            std::vector<std::string> errs;
+            std::vector<ExecutionContext*> cs;
+            try {
+                DatabaseInstance::groupContexts(*ctx, lane->group, rs.size());
+                for (size_t k = 0; k < rs.size(); ++k) {
+                    lane->group[k]->config.codegen.vm = true;
+                    cs.push_back(lane->group[k].get());
+                }
+                errs = db_.engine().executeInterpretedGroup(sqls, cs);
+            } catch (const std::exception& e) {
+                errs.assign(rs.size(), e.what());
+            }
+            {
+                std::lock_guard<std::mutex> lock(mu_);
+                ++cold_groups_;
+                cold_grouped_ += static_cast<long long>(rs.size());
+            }
+            for (size_t k = 0; k < rs.size(); ++k) {
+                QueryResult res;
+                if (errs[k].empty()) res = std::move(cs[k]->result);
+                release(rs[k]);
+                if (errs[k].empty()) {
+                    finish(rs[k], 0, "", &res);
+                } else {
+                    finish(rs[k], -3, errs[k], nullptr);
+                    std::lock_guard<std::mutex> lock(mu_);
+                    last_error_ = errs[k];
+                }
+            }
+            cv_.notify_all();
+            continue;
         }
         // a cold signature: its own scan on the interpreted tier (no
         // compilation) or on its plain module once that is compiled
-        ExecutionContext* ctx = lane->ctx.get();
+        Request& r = rs.front();
         bool returned = false;
         try {
             ctx->row_begin = 0;
@@ -544,19 +622,24 @@ int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::s
 void QueryServer::resetStats() {
     std::lock_guard<std::mutex> lock(mu_);
     latencies_.clear();
+    cold_latencies_.clear();
+    batch_wall_ms_ = 0;
+    cold_groups_ = cold_grouped_ = 0;
     batches_ = batched_queries_ = plain_queries_ = interpreted_queries_ = admission_waits_ = compiles_ = promotions_ = rejected_ = 0;
     stats_start_ = std::chrono::steady_clock::now();
 }
 
 std::string QueryServer::statsJson() {
     std::lock_guard<std::mutex> lock(mu_);
-    std::vector<double> l = latencies_;
+    std::vector<double> l = latencies_, lc = cold_latencies_;
     std::sort(l.begin(), l.end());
-    auto pct = [&](double q) {
-        if (l.empty()) return 0.0;
-        const size_t i = static_cast<size_t>(q * static_cast<double>(l.size() - 1) + 0.5);
-        return l[std::min(l.size() - 1, i)];
+    std::sort(lc.begin(), lc.end());
+    auto pctOf = [](const std::vector<double>& v, double q) {
+        if (v.empty()) return 0.0;
+        const size_t i = static_cast<size_t>(q * static_cast<double>(v.size() - 1) + 0.5);
+        return v[std::min(v.size() - 1, i)];
     };
+    auto pct = [&](double q) { return pctOf(l, q); };
     const double secs = elapsedMs(stats_start_, std::chrono::steady_clock::now()) / 1000.0;
     long long g_capt = 0, g_replay = 0, g_fail = 0;
     for (const auto& l : lanes_) {
@@ -567,6 +650,9 @@ std::string QueryServer::statsJson() {
     std::ostringstream o;
     o << "{\"completed\":" << l.size() << ",\"p50_ms\":" << pct(0.50) << ",\"p99_ms\":" << pct(0.99)
       << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
+      << ",\"cold_completed\":" << lc.size() << ",\"cold_p50_ms\":" << pctOf(lc, 0.50) << ",\"cold_p99_ms\":" << pctOf(lc, 0.99)
+      << ",\"cold_groups\":" << cold_groups_ << ",\"cold_grouped\":" << cold_grouped_
+      << ",\"batch_wall_ms\":" << (batches_ ? batch_wall_ms_ / static_cast<double>(batches_) : 0.0)
       << ",\"batches\":" << batches_ << ",\"mean_batch\":"
       << (batches_ ? static_cast<double>(batched_queries_) / static_cast<double>(batches_) : 0.0)
       << ",\"plain_queries\":" << plain_queries_ << ",\"interpreted_queries\":" << interpreted_queries_

@@ -53,7 +53,15 @@ struct ServerOptions {
     int window_us = 0;             // wait for more arrivals before taking a batch
     bool async_compile = true;     // compile off the lanes
     bool interpret_cold = true;    // run new signatures at once on the interpreted tier
-    int cold_lanes = 1;            // lanes (own stream each) running cold requests one at a time
+    int cold_lanes = 1;            // lanes (own stream each) running cold requests
+    // interpreted requests a cold lane takes at once: those with a linear
+    // form share scans (16 per scan) and read the fact columns once.  One
+    // at a time by default: at scale 100 the shared scan's per-plan
+    // arithmetic costs more than the column pass it saves beyond pairs
+    int cold_group = 1;
+    // cold lanes' interpreted scans on this percentage of a full-device
+    // grid, leaving CUs to the shared scans running meanwhile
+    int cold_grid_percent = 100;
     // hot signatures each get their own shared-scan module and launch (a
     // batch runs one launch per signature present); off: one module holds
     // the whole hot set (up to max_templates) and a batch is one launch
@@ -64,7 +72,7 @@ struct ServerOptions {
 
 // "key=value,..." with lanes, max_batch, max_templates, promote_after,
 // compile_threads, budget_mb, window_us, async_compile, interpret,
-// cold_lanes, per_template, max_hot, bt, vpt, graph, lds_kb
+// cold_lanes, cold_group, cold_grid_percent, per_template, max_hot, bt, vpt, graph, lds_kb
 ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
@@ -90,6 +98,7 @@ private:
         size_t footprint = 0;
         bool interpreted = false;                 // cold request: interpreted tier (else its plain module)
         bool plain_only = false;                  // the shared scan cannot take the plan (OR-key joins): cold lanes, compiled
+        bool cold = false;                        // ran on a cold lane (interpreted or plain), not in a shared scan
     };
     struct Done {
         int rc = 0;
@@ -101,6 +110,7 @@ private:
         std::unique_ptr<ExecutionContext> ctx;
         std::mutex mu;
         std::unique_ptr<BatchExecutor> exec;   // replaced under the server mutex
+        std::vector<std::unique_ptr<ExecutionContext>> group;  // cold lanes: contexts of a grouped run, on ctx's stream
         std::thread thread;
         long long graph_captures = 0, graph_replays = 0, graph_failures = 0;  // of replaced executors
     };
@@ -131,6 +141,9 @@ private:
 
     // statistics
     std::vector<double> latencies_;
+    std::vector<double> cold_latencies_;  // the cold lanes' share of latencies_
+    double batch_wall_ms_ = 0;            // shared-scan batches: host wall time, summed
+    long long cold_groups_ = 0, cold_grouped_ = 0;  // interpreted requests run together: runs, requests
     std::chrono::steady_clock::time_point stats_start_;
     std::string last_error_;            // message of the last failed batch or cold request
     long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, interpreted_queries_ = 0, admission_waits_ = 0,

@@ -245,3 +245,47 @@ def test_gpu_served_adhoc_shapes():
             assert_same(gdb.execute(sql), odb.execute(gdb.explain(sql)))
     finally:
         _set(gdb, MODES["compiled"])
+
+
+@pytest.mark.gpu
+def test_gpu_linear_group(gpu_pair):
+    """Statements run together: the linear forms share scans (16 plans and 8
+    columns at most per scan, so 26 statements take two), the bytecode-only
+    shapes run alone; every result equals the statement run alone and the
+    oracle."""
+    tables, db, odb = gpu_pair
+    _set(db, MODES["linear"])
+    try:
+        sqls = list(LINEAR.values()) + list(BYTECODE.values()) + [
+            f"select count(*), sum(lo_a + {k}), max(lo_b) from lineorder where lo_c < {10 + 3 * k} and lo_a > {k - 20}"
+            for k in range(13)]
+        print(f"[linear] group of {len(sqls)} statements", flush=True)
+        res = db.execute_group(sqls)
+        for sql, r in zip(sqls, res):
+            print(f"[linear] group member: {sql}", flush=True)
+            o = odb.execute(db.explain(sql))
+            assert_same(r, o)
+            assert_same(db.execute(sql), o)
+        for name in ("scalar", "avg", "chains", "count_col"):
+            assert res[list(LINEAR).index(name)].tuples() == closed_form(tables, name)
+    finally:
+        _set(db, MODES["compiled"])
+
+
+@pytest.mark.gpu
+def test_gpu_linear_group_served_adhoc():
+    """The served ad-hoc shapes on SSB data with nullable lo_revenue, run as
+    one group (two shared scans), against the oracle."""
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1c")
+    _set(gdb, MODES["linear"])
+    try:
+        rng = random.Random(12)
+        sqls = [Q.ADHOC_1, Q.ADHOC_2] + [Q.adhoc_query(rng) for _ in range(22)]
+        print(f"[linear] adhoc group of {len(sqls)}", flush=True)
+        res = gdb.execute_group(sqls)
+        for sql, r in zip(sqls, res):
+            print(f"[linear] adhoc group member: {sql}", flush=True)
+            assert_same(r, odb.execute(gdb.explain(sql)))
+    finally:
+        _set(gdb, MODES["compiled"])

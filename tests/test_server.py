@@ -183,3 +183,26 @@ def test_server_graph_launches_with_changing_batches():
     for q, r in done:
         print(f"[server-graph] sql={q}", flush=True)
         assert_same(r, odb.execute(gdb.explain(q)))
+
+
+@pytest.mark.gpu
+def test_cold_interpreted_requests_run_together():
+    """Ad-hoc requests queued together on a cold lane run as one group
+    (their linear forms share scans); every result matches the oracle and
+    the same statement run alone."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1c")
+    rng = random.Random(41)
+    qs = [Q.adhoc_query(rng) for _ in range(40)] + [Q.template_query(k, rng) for k in (1, 2, 3, 4)]
+    srv = pa.Server(gdb, {"lanes": 1, "cold_lanes": 1, "cold_group": 32})
+    try:
+        tickets = [(q, srv.submit(q)) for q in qs]
+        done = [(q, srv.wait(t, 120_000)) for q, t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st["rejected"] == 0 and st["cold_groups"] >= 1 and st["cold_grouped"] >= 2, st
+    for q, r in done:
+        print(f"[server-cold] sql={q}", flush=True)
+        assert_same(r, odb.execute(gdb.explain(q)))
