@@ -172,11 +172,13 @@ def bench_batched(pa, Q, args):
     # bytes of one shared pass: every union column of every launch, read once
     union_bytes = sum(4 * 120_000_000 * sc["union_columns"] for sc in rep["scans"])
     db.close()
-    return {"workload": f"{n} queries, 4 templates (1-4 dimension joins), random constants, scale 20 (120,000,000 fact rows)",
+    return {"workload": f"{n} queries, 4 templates (2-, 3-, 4- and 4-way joins: 1, 2, 3 and 3 dimensions), random constants, "
+                        "scale 20 (120,000,000 fact rows)",
             "qps": round(n / batched_s, 1), "serial_qps": round(n / serial_s, 1),
             "speedup": round(serial_s / batched_s, 2), "batch_ms": round(batched_s * 1000, 3),
             "serial_ms": round(serial_s * 1000, 3), "shared_scans": rep["shared_scans"],
             "block_threads": tuned["block_threads"], "vectors_per_thread": tuned["vectors_per_thread"],
+            "table_mask": tuned.get("table_mask"), "lds_table_bytes": tuned.get("lds_table_bytes"),
             "graph": rep["scans"][0]["graph"], "scan_ms_last": rep["scans"][0]["scan_ms"],
             "device_ms_last": rep["scans"][0]["device_ms"], "host_ms_last": rep["scans"][0]["host_ms"],
             "plan_ms_last": rep["plan_ms"], "union_GBps": round(union_bytes / batched_s / 1e9, 1),

@@ -43,7 +43,7 @@ std::string ldsRegion(const QueryProgram* p, size_t t, const std::string& ind) {
 
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt,
                                     std::vector<std::string>& union_cols, std::vector<bool>& union_nullable,
-                                    unsigned* table_lds_bytes) {
+                                    unsigned* table_lds_bytes, unsigned table_mask, int* lds_candidates) {
     if (templates.empty()) throw std::runtime_error("batch without templates");
     if (templates.size() > 16) throw std::runtime_error("too many query templates in one batch (max 16)");
     union_cols.clear();
@@ -123,10 +123,13 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     std::stable_sort(cands.begin(), cands.end(), [](const LdsTable& x, const LdsTable& y) {
         return x.users.size() != y.users.size() ? x.users.size() > y.users.size() : x.words < y.words;
     });
+    if (lds_candidates) *lds_candidates = static_cast<int>(cands.size());
     std::vector<LdsTable> tabs;
     uint64_t tab_bytes = 0;
     std::map<std::pair<size_t, size_t>, size_t> tab_of;
-    for (auto& c : cands) {
+    for (size_t k = 0; k < cands.size(); ++k) {
+        auto& c = cands[k];
+        if (k < 32 && !((table_mask >> k) & 1u)) continue;  // placed in HBM
         if (mask_bytes + tab_bytes + 4 * c.words > kBatchTableLdsBytes) continue;
         tab_bytes += 4 * c.words;
         for (const auto& u : c.users) tab_of[u] = tabs.size();

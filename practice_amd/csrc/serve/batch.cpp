@@ -44,6 +44,7 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "lds_agg") o.lds_aggregation = v != 0;
         else if (k == "lds_kb") o.lds_kb = static_cast<int>(v);
         else if (k == "masks") o.masks = v != 0;
+        else if (k == "table_mask") o.table_mask = static_cast<unsigned>(v);
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
@@ -81,7 +82,7 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
               [](const QueryProgram* a, const QueryProgram* b) { return a->signature < b->signature; });
     std::vector<std::string> cols;
     std::vector<bool> nullable;
-    return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable);
+    return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable, nullptr, opt.table_mask);
 }
 
 BatchExecutor::BatchExecutor(DatabaseInstance& db)
@@ -253,7 +254,8 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         runScan(scans[s], programs, opt, static_cast<int>(s), results, rep, module_templates);
         r << (s ? "," : "") << "{\"queries\":" << rep.queries << ",\"templates\":" << rep.templates
           << ",\"union_columns\":" << rep.union_columns << ",\"block_threads\":" << rep.block_threads
-          << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
+          << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"table_mask\":" << rep.table_mask
+          << ",\"lds_table_bytes\":" << rep.lds_table_bytes << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
           << ",\"graph\":\"" << rep.graph_mode << "\""
           << ",\"scan_ms\":" << rep.scan_ms << ",\"device_ms\":" << rep.device_ms << ",\"fact_bytes\":" << rep.fact_bytes
           << ",\"host_ms\":{\"modules\":" << rep.modules_ms << ",\"bind\":" << rep.bind_ms << ",\"enqueue\":" << rep.enqueue_ms
@@ -269,8 +271,9 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
 namespace {
 
 std::string batchSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt, std::vector<std::string>& cols,
-                        std::vector<bool>& nullable) {
-    return generateBatchScanSource(templates, bt, vpt, cols, nullable);
+                        std::vector<bool>& nullable, unsigned table_mask = ~0u, unsigned* table_lds_bytes = nullptr,
+                        int* lds_candidates = nullptr) {
+    return generateBatchScanSource(templates, bt, vpt, cols, nullable, table_lds_bytes, table_mask, lds_candidates);
 }
 
 }  // namespace
@@ -516,7 +519,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     hipCheck(hipMemcpyAsync(dqp, stage, upload_bytes, hipMemcpyHostToDevice, s), "upload parameter blocks");
 
     // launch shape: tuned per template set when asked
-    Tuned cfg{opt.block_threads, opt.vectors_per_thread};
+    Tuned cfg{opt.block_threads, opt.vectors_per_thread, opt.table_mask};
     auto tit = tuned_.find(set_key);
     if (tit != tuned_.end()) cfg = tit->second;
     else if (opt.autotune) {
@@ -525,7 +528,8 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     }
     std::vector<std::string> ucols;
     std::vector<bool> unull;
-    const std::string src = batchSource(templates, cfg.block_threads, cfg.vectors_per_thread, ucols, unull);
+    unsigned table_bytes = 0;
+    const std::string src = batchSource(templates, cfg.block_threads, cfg.vectors_per_thread, ucols, unull, cfg.table_mask, &table_bytes);
     ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
     hipFunction_t fn = bm->function("cq_batch_scan");
 
@@ -738,6 +742,8 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     rep.union_columns = static_cast<int>(ucols.size());
     rep.block_threads = cfg.block_threads;
     rep.vectors_per_thread = cfg.vectors_per_thread;
+    rep.table_mask = cfg.table_mask;
+    rep.lds_table_bytes = table_bytes;
     rep.graph_replayed = replayed;
     rep.scan_ms = sms;
     rep.device_ms = dms;
@@ -757,53 +763,78 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
                                              ExecutionContext* ctx) {
     QueryEngine& engine = db_.engine();
     hipStream_t s = ctx->stream;
+    // one variant: its module, then two timed launches (the first warms the
+    // module and the caches); -1 when the launch covers no rows
+    auto timeVariant = [&](const Tuned& v, int* candidates) -> float {
+        std::vector<std::string> ucols;
+        std::vector<bool> unull;
+        const std::string src = batchSource(templates, v.block_threads, v.vectors_per_thread, ucols, unull, v.table_mask, nullptr, candidates);
+        ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
+        hipFunction_t fn = bm->function("cq_batch_scan");
+        cq_batch_params B;
+        std::memset(&B, 0, sizeof B);
+        for (size_t u = 0; u < ucols.size(); ++u) {
+            const DeviceColumn& c = ctx->columns->at(ucols[u]);
+            B.col[u] = c.data;
+            B.valid[u] = reinterpret_cast<const unsigned int*>(c.validity);
+        }
+        B.qp = dqp;
+        int at = 0;
+        for (size_t t = 0; t < templates.size(); ++t) {
+            B.tq_begin[t] = at;
+            at += static_cast<int>(per_template[t].size());
+            B.tq_end[t] = at;
+        }
+        B.row_end = templates.front()->fact_rows;
+        const long long tile = static_cast<long long>(v.block_threads) * 4 * v.vectors_per_thread;
+        B.num_tiles = (B.row_end + tile - 1) / tile;
+        if (B.num_tiles == 0) return -1.0f;
+        const unsigned lds_bytes = layoutLds(templates, per_template, lds_limit, B);
+        const unsigned grid = scanGridShared(fn, v.block_threads, ctx->device, B.num_tiles, lds_bytes);
+        float ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+            prepareTables(templates, per_template, runs, dqp, init, ctx);
+            hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
+            void* args[] = {&B};
+            hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(v.block_threads), 1, 1, lds_bytes, s, args, nullptr),
+                     "launch batch scan");
+            hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
+            hipCheck(hipEventSynchronize(ev_[3]), "sync");
+            (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
+        }
+        return ms;
+    };
+    // launch shape with every candidate table in LDS
     Tuned best;
     float best_ms = -1;
+    int candidates = 0;
     const int bts[] = {64, 128, 256, 512, 1024};
     const int vpts[] = {1, 2, 4};
     for (int bt : bts) {
         for (int vpt : vpts) {
-            std::vector<std::string> ucols;
-            std::vector<bool> unull;
-            const std::string src = batchSource(templates, bt, vpt, ucols, unull);
-            ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
-            hipFunction_t fn = bm->function("cq_batch_scan");
-            cq_batch_params B;
-            std::memset(&B, 0, sizeof B);
-            for (size_t u = 0; u < ucols.size(); ++u) {
-                const DeviceColumn& c = ctx->columns->at(ucols[u]);
-                B.col[u] = c.data;
-                B.valid[u] = reinterpret_cast<const unsigned int*>(c.validity);
-            }
-            B.qp = dqp;
-                    int at = 0;
-            for (size_t t = 0; t < templates.size(); ++t) {
-                B.tq_begin[t] = at;
-                at += static_cast<int>(per_template[t].size());
-                B.tq_end[t] = at;
-            }
-            B.row_end = templates.front()->fact_rows;
-            const long long tile = static_cast<long long>(bt) * 4 * vpt;
-            B.num_tiles = (B.row_end + tile - 1) / tile;
-            if (B.num_tiles == 0) return best;
-            const unsigned lds_bytes = layoutLds(templates, per_template, lds_limit, B);
-            const unsigned grid = scanGridShared(fn, bt, ctx->device, B.num_tiles, lds_bytes);
-            float ms = 0;
-            for (int rep = 0; rep < 2; ++rep) {  // first run warms the module and caches
-                prepareTables(templates, per_template, runs, dqp, init, ctx);
-                hipCheck(hipEventRecord(ev_[2], s), "hipEventRecord");
-                void* args[] = {&B};
-                hipCheck(hipModuleLaunchKernel(fn, grid, 1, 1, static_cast<unsigned>(bt), 1, 1, lds_bytes, s, args, nullptr),
-                         "launch batch scan");
-                hipCheck(hipEventRecord(ev_[3], s), "hipEventRecord");
-                hipCheck(hipEventSynchronize(ev_[3]), "sync");
-                (void)hipEventElapsedTime(&ms, ev_[2], ev_[3]);
-            }
+            Tuned v;
+            v.block_threads = bt;
+            v.vectors_per_thread = vpt;
+            const float ms = timeVariant(v, &candidates);
+            if (ms < 0) return best;
             if (best_ms < 0 || ms < best_ms) {
                 best_ms = ms;
-                best.block_threads = bt;
-                best.vectors_per_thread = vpt;
+                best = v;
             }
+        }
+    }
+    // LDS-versus-HBM placement of the first three candidate tables at that
+    // shape (a table left in HBM frees LDS for the others and costs L2
+    // gathers instead of LDS bank conflicts)
+    const int m = std::min(candidates, 3);
+    const Tuned shape = best;
+    for (unsigned bits = 0; bits + 1 < (1u << m); ++bits) {
+        Tuned v = shape;
+        v.table_mask = (~0u << m) | bits;
+        const float ms = timeVariant(v, nullptr);
+        if (ms >= 0 && ms < best_ms) {
+            best_ms = ms;
+            best = v;
         }
     }
     return best;

@@ -36,15 +36,20 @@ struct BatchOptions {
     bool lds_aggregation = true;  // per-query LDS aggregation for small group domains
     int lds_kb = 160;             // dynamic LDS of one shared-scan block (KiB, 16..160)
     bool masks = true;            // query-mask form for templates whose filters allow it
+    // shared dimension tables staged into LDS: bit k for the k-th candidate
+    // (generateBatchScanSource); the autotuner sweeps the first three
+    unsigned table_mask = ~0u;
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
-// max_queries, shared, lds_agg, lds_kb, masks)
+// max_queries, shared, lds_agg, lds_kb, masks, table_mask)
 BatchOptions parseBatchOptions(const std::string& text);
 
 struct BatchScanReport {
     int queries = 0, templates = 0, union_columns = 0;
     int block_threads = 0, vectors_per_thread = 0;
+    unsigned lds_table_bytes = 0;  // static LDS of the staged shared tables and query-mask tables
+    unsigned table_mask = ~0u;     // LDS placement of the candidate tables
     bool graph_replayed = false;
     const char* graph_mode = "eager";  // eager / captured / replayed
     double scan_ms = 0;            // cq_batch_scan alone (events)
@@ -100,6 +105,7 @@ private:
     };
     struct Tuned {
         int block_threads = 1024, vectors_per_thread = 1;
+        unsigned table_mask = ~0u;
     };
     DatabaseInstance& db_;
     ExecutionContext* ctx_;

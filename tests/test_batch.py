@@ -106,6 +106,20 @@ def test_mask_form_for_template_queries(hdb):
     assert "// query masks" not in hdb.generate_batch_code(wide)
 
 
+def test_table_placement_mask(hdb):
+    """table_mask places each candidate shared table in LDS (bit set) or
+    HBM: every mask compiles, 0 stages no dimension table, the default
+    stages them all (the placements the autotuner times)."""
+    qs = [Q.C1, Q.C2, Q.C3]
+    full = hdb.generate_batch_code(qs)
+    none = hdb.generate_batch_code(qs, {"table_mask": 0})
+    one = hdb.generate_batch_code(qs, {"table_mask": 1})
+    n_full = full.count("__shared__ u32 cq_tab")
+    assert n_full >= 2 and none.count("__shared__ u32 cq_tab") == 0 and one.count("__shared__ u32 cq_tab") == 1
+    for src in (full, none, one):
+        pa.compile_source(src)
+
+
 @pytest.mark.parametrize("opts", [{"bt": 1024, "vpt": 4}, {"bt": 64, "vpt": 1}])
 def test_launch_shapes_compile(hdb, opts):
     pa.compile_source(hdb.generate_batch_code([Q.C3, Q.C1], opts))
@@ -200,7 +214,12 @@ def test_autotuned_batch_matches():
     res = gdb.batch_execute(qs, {"autotune": 1, "graph": 0})
     scan = gdb.batch_report()["scans"][0]
     assert scan["block_threads"] in (64, 128, 256, 512, 1024) and scan["vectors_per_thread"] in (1, 2, 4)
+    assert scan["table_mask"] | 7 == 0xFFFFFFFF  # the autotuner places the first three candidates
     _check_batch(gdb, odb, qs, res)
+    # every placement gives the same results
+    for mask in (0, 1, 2):
+        print(f"[batch] table_mask={mask}", flush=True)
+        _check_batch(gdb, odb, qs, gdb.batch_execute(qs, {"table_mask": mask, "graph": 0}))
 
 
 def _key_zero_tables(first_key=1):
