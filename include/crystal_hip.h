@@ -131,7 +131,8 @@ long long cq_db_table_rows(cq_db* db, const char* table);
  * "dense_tables" (key-indexed tables), "calendar_tables" (key-indexed tables
  * of yyyymmdd date keys indexed by year, month and day), "stream_tables",
  * "memory_guard_reserve_bytes", "prefetch" (load the next tile while the
- * current one is probed), "interpret" (1: run plans on the ahead-of-time
+ * current one is probed), "defer_probes" (batched mask form: probe tables
+ * too large for LDS only for items some query still accepts), "interpret" (1: run plans on the ahead-of-time
  * interpreter instead of hipRTC-compiled kernels; same results, no compile),
  * "vm_row_kernel", "vm_linear" (0: interpreted plans always run their bytecode,
  * also those with a linear form), "vm_grid_percent" (1-100: linear-form scans

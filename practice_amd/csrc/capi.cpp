@@ -300,6 +300,7 @@ int cq_db_set_option(cq_db* d, const char* name, long long v) {
         else if (n == "split_probes") c.codegen.split_probes = v != 0;
         else if (n == "buffer_tables") c.codegen.buffer_tables = v != 0;
         else if (n == "hoist_gathers") c.codegen.hoist_gathers = v != 0;
+        else if (n == "defer_probes") c.codegen.defer_probes = v != 0;
         else if (n == "interpret") c.codegen.vm = v != 0;
         else if (n == "vm_row_kernel") c.vm_row_kernel = v != 0;
         else if (n == "vm_linear") c.vm_linear = v != 0;

@@ -106,6 +106,13 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     }
     for (size_t t = 0; t < templates.size(); ++t)
         if (use_mask[t]) src << withTemplate(templates[t]->batch_mask_build, t);
+    // a launch of one mask-form template takes its deferred-probe body and
+    // makes those probes inside it
+    auto useDeferred = [&](size_t t) { return use_mask[t] && templates.size() == 1 && !templates[t]->batch_mask_body_deferred.empty(); };
+    auto deferred = [&](size_t t, size_t j) {
+        const auto& d = templates[t]->batch_mask_deferred;
+        return useDeferred(t) && std::find(d.begin(), d.end(), static_cast<int>(j)) != d.end();
+    };
     std::vector<LdsTable> cands;
     for (size_t t = 0; t < templates.size(); ++t) {
         const QueryProgram* p = templates[t];
@@ -165,7 +172,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         if (!p->shared_tables || p->batch_shared_parts.size() != p->joins.size()) continue;
         for (size_t j = 0; j < p->joins.size(); ++j) {
             const JoinProgram& jp = p->joins[j];
-            if (!jp.shared || jp.s_probe.empty()) continue;
+            if (!jp.shared || jp.s_probe.empty() || deferred(t, j)) continue;
             // Probe key 0 (the reference's empty-slot corner): a template
             // whose every dimension key is in the table resolves it by the
             // plain lookup (zmode 0), others mark it (z) and read the entry
@@ -311,6 +318,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
                     << ";\n";
             }
             for (size_t j = 0; j < p->joins.size(); ++j) {
+                if (deferred(t, j)) continue;  // the mask body probes it
                 const auto it = lp_of.find({t, j});
                 if (it == lp_of.end()) {
                     src << (p->batch_shared_parts.size() == p->joins.size() ? p->batch_shared_parts[j] : (j ? std::string() : p->batch_shared));
@@ -323,7 +331,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
             }
         }
         if (use_mask[t]) {
-            src << withTemplate(p->batch_mask_body, t) << "        }\n";
+            src << withTemplate(useDeferred(t) ? p->batch_mask_body_deferred : p->batch_mask_body, t) << "        }\n";
             continue;
         }
         src << "        for (int q = B.tq_begin[" << t << "]; q < B.tq_end[" << t << "]; ++q) {\n"

@@ -2518,6 +2518,8 @@ private:
                 prog_.batch_mask_tables.clear();
                 prog_.batch_mask_build.clear();
                 prog_.batch_mask_body.clear();
+                prog_.batch_mask_body_deferred.clear();
+                prog_.batch_mask_deferred.clear();
             }
             visible_mask_ = vis;
         }
@@ -2558,6 +2560,8 @@ private:
 
     void emitMaskForm(const std::vector<Op>& fact_f, const std::vector<Op>& late_f) {
         prog_.batch_mask = false;
+        prog_.batch_mask_deferred.clear();
+        prog_.batch_mask_body_deferred.clear();
         if (!opt_.batch_masks || !late_f.empty()) return;
         // aggregation kinds with a per-row insert for one query
         const bool scalar_lds = !prog_.grouped && prog_.batch_lds_stride > 0;
@@ -2678,29 +2682,91 @@ private:
                  << "            qm[i] &= v < " << f.domain << "u ? cq_m@T@_f" << k << "[v] : 0ull;\n"
                  << "        }\n";
         }
-        for (int j = 0; j < nj; ++j) {
-            const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
-            const std::string J = std::to_string(j);
-            const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
-            const bool filt = jbits[static_cast<size_t>(j)] >= 0;
-            const std::string idx = "((e >> " + std::to_string(jlo[static_cast<size_t>(j)]) + ") & " +
-                                    std::to_string((1u << std::max(0, jbits[static_cast<size_t>(j)])) - 1u) + "u)";
-            const std::string acc = filt ? "cq_m@T@_j" + J + "[" + idx + "]" : "~0ull";
-            body << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
-                 << "            const u32 e = e" << J << "[i];\n"
-                 << "            u64 jm = m" << J << "[i] ? " << acc << " : 0ull;\n";
-            if (jp.zmode == 1) body << "            if (z" << J << "[i]) jm = ~0ull;\n";
-            if (jp.zmode == 2) {
-                // probe key 0 matches the reference's empty slot: the queries
-                // that did not insert the key sharing that slot
-                const std::string ins = jp.s_present ? "((e & 1u) ? " + acc + " : 0ull)" : acc;
-                body << "            if (z" << J << "[i]) jm = ~" << ins << ";\n";
+        // Joins whose table can never sit in LDS are probed last, and only
+        // for the items some query still accepts after the fact filters and
+        // the other joins: each item skipped saves an L2 gather.  Worth it
+        // only when something before them can reject items.  The deferred
+        // probe waits on the masks, so its L2 round trip no longer overlaps
+        // them: a launch of several templates (one block per CU, each
+        // wave running every template per tile) keeps the early probe
+        // (batch_mask_body), a launch of one template takes the deferred
+        // body (batch_mask_body_deferred).
+        prog_.batch_mask_deferred.clear();
+        std::vector<int> jorder;
+        {
+            std::vector<int> late;
+            bool rejects = !fact_order.empty();
+            for (int j = 0; j < nj; ++j) {
+                const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+                const bool big = opt_.defer_probes && jp.shared && !jp.s_probe.empty() && 4 * jp.ht_words > kBatchTableLdsBytes;
+                if (big) late.push_back(j);
+                else {
+                    jorder.push_back(j);
+                    rejects = rejects || jbits[static_cast<size_t>(j)] >= 0;
+                }
             }
-            body << "            qm[i] &= jm;\n";
-            if (jp.zmode != 0) body << "            p" << J << "[i] = z" << J << "[i] ? 0 : " << pay << ";\n";
-            else body << "            p" << J << "[i] = " << pay << ";\n";
-            body << "        }\n";
+            for (int j : late) {
+                jorder.push_back(j);
+                if (rejects) prog_.batch_mask_deferred.push_back(j);
+            }
         }
+        auto joinSection = [&](bool defer) {
+            std::ostringstream js;
+            for (int j : jorder) {
+                const JoinProgram& jp = prog_.joins[static_cast<size_t>(j)];
+                const std::string J = std::to_string(j);
+                if (defer && std::find(prog_.batch_mask_deferred.begin(), prog_.batch_mask_deferred.end(), j) !=
+                                 prog_.batch_mask_deferred.end()) {
+                    // the shared probe text, live items only
+                    const int fs = factSlot(jp.fk_col);
+                    std::string t = jp.s_probe;
+                    const std::string live = "            if (!((tile_fl >> i) & 1u)) continue;\n";
+                    const size_t at = t.find(live);
+                    if (at == std::string::npos) throw std::logic_error("internal: shared probe without its live-item test");
+                    t.replace(at, live.size(), "            if (qm[i] == 0ull) continue;\n");
+                    auto subst = [](std::string text, const std::string& from, const std::string& to) {
+                        for (size_t k = text.find(from); k != std::string::npos; k = text.find(from, k + to.size()))
+                            text.replace(k, from.size(), to);
+                        return text;
+                    };
+                    const std::string fv = prog_.fact_nullable[static_cast<size_t>(fs)] ? "((v" + std::to_string(fs) + " >> i) & 1u)" : "";
+                    t = subst(t, "@FVTEST@", fv.empty() ? "" : "            if (!" + fv + ") continue;\n");
+                    t = subst(t, "@X@", "x" + J);
+                    t = subst(t, "@M@", "m" + J);
+                    t = subst(t, "@Z@", "z" + J);
+                    t = subst(t, "@E@", "e" + J);
+                    t = subst(t, "@FK@", "c" + std::to_string(fs) + "[i]");
+                    t = subst(t, "@TAB@", "cq_sht" + J);
+                    js << "        // " << jp.dim_table << " on " << jp.fk_col << ": probed for the items some query still accepts\n"
+                       << "        u32 e" << J << "[IPT]; bool m" << J << "[IPT], z" << J << "[IPT];\n" << t;
+                }
+                const std::string pay = jp.s_payload >= 0 ? sharedAttr(jp, static_cast<size_t>(jp.s_payload), "e") : "0";
+                const bool filt = jbits[static_cast<size_t>(j)] >= 0;
+                const std::string idx = "((e >> " + std::to_string(jlo[static_cast<size_t>(j)]) + ") & " +
+                                        std::to_string((1u << std::max(0, jbits[static_cast<size_t>(j)])) - 1u) + "u)";
+                const std::string acc = filt ? "cq_m@T@_j" + J + "[" + idx + "]" : "~0ull";
+                js << "        int p" << J << "[IPT];\n#pragma unroll\n        for (int i = 0; i < IPT; ++i) {\n"
+                   << "            const u32 e = e" << J << "[i];\n"
+                   << "            u64 jm = m" << J << "[i] ? " << acc << " : 0ull;\n";
+                if (jp.zmode == 1) js << "            if (z" << J << "[i]) jm = ~0ull;\n";
+                if (jp.zmode == 2) {
+                    // probe key 0 matches the reference's empty slot: the queries
+                    // that did not insert the key sharing that slot
+                    const std::string ins = jp.s_present ? "((e & 1u) ? " + acc + " : 0ull)" : acc;
+                    js << "            if (z" << J << "[i]) jm = ~" << ins << ";\n";
+                }
+                js << "            qm[i] &= jm;\n";
+                if (jp.zmode != 0) js << "            p" << J << "[i] = z" << J << "[i] ? 0 : " << pay << ";\n";
+                else js << "            p" << J << "[i] = " << pay << ";\n";
+                js << "        }\n";
+            }
+            return js.str();
+        };
+        const std::string head = body.str();
+        const std::string joins_early = joinSection(false);
+        const std::string joins_deferred = prog_.batch_mask_deferred.empty() ? std::string() : joinSection(true);
+        body.str("");
+        body.clear();
         visible_mask_ = (1u << nj) - 1u;
 
         // ---- aggregation over (row, query) pairs
@@ -2854,7 +2920,8 @@ private:
         }
         body << "        }\n";
         prog_.batch_mask_build = build.str();
-        prog_.batch_mask_body = body.str();
+        prog_.batch_mask_body = head + joins_early + body.str();
+        prog_.batch_mask_body_deferred = joins_deferred.empty() ? std::string() : head + joins_deferred + body.str();
         prog_.batch_mask = true;
     }
 

@@ -45,6 +45,9 @@ struct CodegenOptions {
     // ahead of LDS probes that remove no rows, so the L2 round trip overlaps
     // their work
     bool hoist_gathers = true;
+    // batched mask form: probe tables too large for LDS after the other
+    // joins' masks, only for items some query still accepts
+    bool defer_probes = true;
     // Also emit the per-tile pipeline body for the batched shared scan
     // (global tables, per-tile scalar reductions, global GROUP BY atomics).
     bool batch_body = false;
@@ -183,6 +186,13 @@ struct QueryProgram {
     bool batch_mask = false;
     std::vector<std::pair<std::string, int>> batch_mask_tables;  // LDS u64 tables: name, entries
     std::string batch_mask_build, batch_mask_body;
+    // joins whose probe batch_mask_body_deferred makes itself, after the
+    // other joins' masks and only for items some query still accepts
+    // (tables too large for LDS, so each skipped item saves an L2 gather);
+    // a launch of this template alone uses that body and emits no shared
+    // probe part for them (empty: no such join)
+    std::vector<int> batch_mask_deferred;
+    std::string batch_mask_body_deferred;
     std::string batch_shared;
     std::vector<std::string> batch_shared_parts;  // batch_shared per join (declarations + loops)
     // (scope of the batch kernel: cq_hbits = log2 slots of the query's LDS

@@ -1101,7 +1101,7 @@ bool splitLiterals(const std::string& sql, std::string& shape, std::vector<long 
 std::string optionsKey(const CodegenOptions& o) {
     std::ostringstream k;
     k << o.block_threads << ',' << o.vectors_per_thread << ',' << o.lds_budget << ',' << o.allow_lds_tables << o.allow_lds_aggregation
-      << o.allow_compact_tables << o.allow_dense_tables << o.calendar_tables << o.split_probes << o.buffer_tables << o.hoist_gathers << o.prefetch << o.stream_large_tables << o.batch_body << o.batch_shared_tables << o.batch_masks
+      << o.allow_compact_tables << o.allow_dense_tables << o.calendar_tables << o.split_probes << o.buffer_tables << o.hoist_gathers << o.defer_probes << o.prefetch << o.stream_large_tables << o.batch_body << o.batch_shared_tables << o.batch_masks
       << o.vm;
     return k.str();
 }

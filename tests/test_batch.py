@@ -97,6 +97,22 @@ def test_mask_form_for_template_queries(hdb):
     src = hdb.generate_batch_code(qs)
     assert src.count("// query masks") == 4
     assert "for (int q = B.tq_begin[" not in src.split("for (long long tile")[1].split("__syncthreads")[0]
+    # CUSTOMER and PART (too large for LDS): in a launch of several
+    # templates their probes are issued early (their L2 round trips overlap
+    # the masks); a launch of one template probes them after the other
+    # joins' masks, only for items some query still accepts
+    assert "if (qm[i] == 0ull) continue;" not in src
+    for k, dim, key in ((3, "PART", "lo_partkey"), (4, "CUSTOMER", "lo_custkey")):
+        assert f"// shared probe {dim}" in src
+        one = hdb.generate_batch_code([Q.template_query(k, rng) for _ in range(3)])
+        assert f"// {dim} on {key}: probed for the items some query still accepts" in one
+        assert f"// shared probe {dim}" not in one and one.count("if (qm[i] == 0ull) continue;") == 1
+        pa.compile_source(one)
+        hdb.set_option("defer_probes", 0)
+        try:
+            assert "if (qm[i] == 0ull)" not in hdb.generate_batch_code([Q.template_query(k, rng) for _ in range(3)])
+        finally:
+            hdb.set_option("defer_probes", 1)
     plain = hdb.generate_batch_code(qs, {"masks": 0})
     assert "// query masks" not in plain
     pa.compile_source(src)
@@ -172,6 +188,25 @@ def test_templates_with_many_queries(masks):
     assert rep["queries"] == len(qs)
     for sql, r in zip(qs, res):
         assert_same(r, odb.execute(gdb.explain(sql)))
+
+
+@pytest.mark.gpu
+def test_deferred_probes_with_few_queries():
+    """Three queries of each template, one launch per template: most items
+    are rejected before the CUSTOMER and PART probes, which then skip them;
+    results equal the oracle and each query alone."""
+    import random
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(37)
+    qs = [Q.template_query(k, rng) for k in (1, 2, 3, 4) for _ in range(3)]
+    res = gdb.batch_execute(qs, {"graph": 0, "masks": 1, "max_templates": 1})
+    assert gdb.batch_report()["shared_scans"] == 4
+    for sql, r in zip(qs, res):
+        print(f"[batch] deferred: {sql}", flush=True)
+        o = odb.execute(gdb.explain(sql))
+        assert_same(r, o)
+        assert_same(gdb.execute(sql), o)
 
 
 @pytest.mark.gpu
