@@ -246,10 +246,20 @@ public:
         prog_.plan_sexpr = sx.str();
         analyze(root);
         emit();
-        // a batched template is identified by its per-tile body as well:
-        // different pipelines can share identical build / flag kernels
-        prog_.signature = signatureOf(prog_.batch_body.empty() ? prog_.source
-                                                               : prog_.source + prog_.batch_body + prog_.batch_mask_build + prog_.batch_mask_body);
+        // a batched template is identified by its per-tile body and by the
+        // columns it binds as well: different pipelines can share identical
+        // build / flag kernels, and plans over different columns can share
+        // identical text (a launch binds one column set per template)
+        std::string binding;
+        if (!prog_.batch_body.empty()) {
+            binding = "|" + prog_.fact_table;
+            for (const auto& c : prog_.fact_cols) binding += "|f:" + c;
+            for (const auto& c : prog_.dim_cols) binding += "|d:" + c;
+        }
+        prog_.signature = signatureOf(prog_.batch_body.empty()
+                                          ? prog_.source
+                                          : prog_.source + prog_.batch_body + prog_.batch_mask_build + prog_.batch_mask_body +
+                                                prog_.batch_mask_body_deferred + binding);
         return std::move(prog_);
     }
 

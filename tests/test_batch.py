@@ -319,3 +319,21 @@ def test_probe_key_zero_against_reference_empty_slot(first_key):
             assert_same(r, o)
     finally:
         db.close()
+
+
+def test_templates_over_different_columns_stay_distinct(hdb):
+    """Plans whose generated text is identical but which bind different fact
+    columns (avg of lo_quantity and of lo_tax) are different templates: a
+    launch binds one column set per template."""
+    qs = ["select avg(lo_quantity) from lineorder", "select avg(lo_tax) from lineorder"]
+    assert hdb.generate_batch_code(qs).count("        // template ") == 2
+
+
+@pytest.mark.gpu
+def test_batch_with_same_text_different_columns():
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    qs = ["select avg(lo_quantity) from lineorder", "select avg(lo_tax) from lineorder",
+          "select sum(lo_quantity) from lineorder where lo_discount < 3", "select sum(lo_tax) from lineorder where lo_discount < 3"]
+    for opts in ({"graph": 0}, {"graph": 0, "max_templates": 1}):
+        _check_batch(gdb, odb, qs, gdb.batch_execute(qs, opts))
