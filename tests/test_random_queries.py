@@ -48,7 +48,9 @@ def _pred(rng, col, lo, hi, allow_not=True):
     return _cmp(rng, col, lo, hi)
 
 
-def random_statement(rng):
+def random_statement(rng, rich=False):
+    """rich: also NULL tests on the nullable lo_revenue, CASE and division
+    in aggregate inputs, HAVING, ORDER BY with DESC and LIMIT."""
     dims = rng.sample(sorted(DIMS), rng.randint(0, 3))
     where = [f"{DIMS[d][0]} = {DIMS[d][1]}" for d in dims]
     for _ in range(rng.randint(0, 2)):
@@ -57,6 +59,9 @@ def random_statement(rng):
     if rng.random() < 0.2:
         c = rng.choice(sorted(WIDE))
         where.append(_cmp(rng, c, *WIDE[c]))
+    if rich and rng.random() < 0.3:
+        where.append(rng.choice(["lo_revenue is null", "lo_revenue is not null",
+                                 f"(lo_revenue > {rng.randint(100_000, 10_000_000)} or lo_revenue is null)"]))
     keys = []
     for d in dims:
         attrs = DIMS[d][2]
@@ -73,25 +78,39 @@ def random_statement(rng):
     for _ in range(rng.randint(1, 3)):
         w = rng.choice(sorted(WIDE))
         s = rng.choice(sorted(SMALL))
-        aggs.append(rng.choice([f"sum({w})", "count(*)", f"count({w})", f"min({w})", f"max({s})", f"avg({s})",
-                                f"sum({w} * {s})", f"sum({w} - {s})", f"sum({s} + 7)"]))
+        forms = [f"sum({w})", "count(*)", f"count({w})", f"min({w})", f"max({s})", f"avg({s})",
+                 f"sum({w} * {s})", f"sum({w} - {s})", f"sum({s} + 7)"]
+        if rich:
+            forms += [f"sum(case when {s} > {rng.randint(*SMALL[s])} then {w} else 0 end)", f"sum({w} / {rng.randint(2, 9)})",
+                      f"sum(case when lo_revenue is null then 1 else 0 end)", f"max({w} - {s} * 3)"]
+        aggs.append(rng.choice(forms))
     sql = "select " + ", ".join(keys + aggs) + " from lineorder" + "".join(", " + d for d in dims)
     if where:
         sql += " where " + " and ".join(where)
     if keys:
         sql += " group by " + ", ".join(keys)
+        if rich and rng.random() < 0.25:
+            # the HAVING aggregate must be one of the SELECT list's (the
+            # reference's rule, which this planner shares)
+            sql += f" having {aggs[0]} > {rng.randint(0, 2000)}"
+        if rich and rng.random() < 0.4:
+            sql += " order by " + ", ".join(keys)
+            if rng.random() < 0.5:
+                n_out = len(keys) + len(aggs)
+                sql = sql[: -len(", ".join(keys))] + f"{rng.randint(len(keys) + 1, n_out)} desc, " + ", ".join(keys)
+                sql += f" limit {rng.randint(1, 10)}"
     return sql
 
 
-def statements(n=40, seed=2026):
+def statements(n=40, seed=2026, rich=False):
     rng = random.Random(seed)
-    return [random_statement(rng) for _ in range(n)]
+    return [random_statement(rng, rich) for _ in range(n)]
 
 
 def test_random_statements_plan():
     from tests.test_batch import _host_db
     db = _host_db()
-    for sql in statements():
+    for sql in statements() + statements(40, 2027, rich=True):
         d = db.describe(sql)
         assert d["signature"], sql
 
@@ -127,3 +146,27 @@ def test_random_statements_batched(sf1, opts):
     spec, data, odb, gdb = sf1
     sqls = statements()
     _against_oracle(gdb, odb, sqls, gdb.batch_execute(sqls, opts), f"batched {opts}")
+
+
+@pytest.mark.gpu
+def test_rich_random_statements_every_path(sf1):
+    """NULL tests, CASE, division, HAVING, ORDER BY ... DESC LIMIT: plain,
+    interpreted, batched and served (every statement submitted at once to a
+    server, so cold and hot tiers both run them)."""
+    spec, data, odb, gdb = sf1
+    sqls = statements(40, 2027, rich=True)
+    _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "plain")
+    gdb.set_option("interpret", 1)
+    try:
+        _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "interpreted")
+        _against_oracle(gdb, odb, sqls, gdb.execute_group(sqls), "interpreted group")
+    finally:
+        gdb.set_option("interpret", 0)
+    _against_oracle(gdb, odb, sqls, gdb.batch_execute(sqls, {"graph": 0}), "batched")
+    srv = pa.Server(gdb, {"lanes": 1, "cold_lanes": 1, "promote_after": 1})
+    try:
+        tickets = [srv.submit(s) for s in sqls + sqls]
+        done = [srv.wait(t, 120_000) for t in tickets]
+    finally:
+        srv.stop()
+    _against_oracle(gdb, odb, sqls + sqls, done, "served")
