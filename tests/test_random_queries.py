@@ -170,3 +170,16 @@ def test_rich_random_statements_every_path(sf1):
     finally:
         srv.stop()
     _against_oracle(gdb, odb, sqls + sqls, done, "served")
+
+
+@pytest.mark.gpu
+def test_random_statements_scale100_dimensions():
+    """Scale-100 dimension tables (3M customers, 1.4M parts: other table
+    layouts, deferred probes of the large ones in single-template launches)
+    under 2M fact rows: plain and batched paths against the oracle."""
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf100")
+    sqls = statements(30, 2028, rich=True)
+    _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "plain")
+    for opts in ({"graph": 0}, {"graph": 0, "max_templates": 1}):
+        _against_oracle(gdb, odb, sqls, gdb.batch_execute(sqls, opts), f"batched {opts}")
