@@ -183,3 +183,42 @@ def test_random_statements_scale100_dimensions():
     _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "plain")
     for opts in ({"graph": 0}, {"graph": 0, "max_templates": 1}):
         _against_oracle(gdb, odb, sqls, gdb.batch_execute(sqls, opts), f"batched {opts}")
+
+
+def random_projection(rng):
+    """A non-aggregate statement: fact and dimension columns of the rows
+    that pass, optionally ordered and limited."""
+    dims = rng.sample(sorted(DIMS), rng.randint(0, 2))
+    where = [f"{DIMS[d][0]} = {DIMS[d][1]}" for d in dims]
+    c = rng.choice(sorted(SMALL))
+    where.append(f"{c} = {rng.randint(*SMALL[c])}")
+    c2 = rng.choice(sorted(SMALL))
+    where.append(_pred(rng, c2, *SMALL[c2]))
+    cols = rng.sample(sorted(SMALL) + sorted(WIDE), rng.randint(1, 3))
+    for d in dims:
+        attrs = DIMS[d][2]
+        if rng.random() < 0.7:
+            cols.append(rng.choice(sorted(attrs)))
+        if rng.random() < 0.4:
+            a = rng.choice(sorted(attrs))
+            where.append(_pred(rng, a, *attrs[a], allow_not=False))
+    sql = "select " + ", ".join(cols) + " from lineorder" + "".join(", " + d for d in dims) + " where " + " and ".join(where)
+    if rng.random() < 0.5:
+        sql += f" order by {cols[0]}{' desc' if rng.random() < 0.5 else ''}, {cols[-1]} limit {rng.randint(1, 50)}"
+    return sql
+
+
+@pytest.mark.gpu
+def test_random_projections(sf1):
+    """Random projections (scan order, or ORDER BY ... LIMIT) against the
+    oracle on the compiled path and through the server's cold lanes."""
+    spec, data, odb, gdb = sf1
+    rng = random.Random(2029)
+    sqls = [random_projection(rng) for _ in range(20)]
+    _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "projection")
+    srv = pa.Server(gdb, {"lanes": 1, "cold_lanes": 2})
+    try:
+        done = [srv.wait(t, 120_000) for t in [srv.submit(s) for s in sqls]]
+    finally:
+        srv.stop()
+    _against_oracle(gdb, odb, sqls, done, "served projection")
