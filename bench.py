@@ -178,7 +178,7 @@ def bench_batched(pa, Q, args):
             "speedup": round(serial_s / batched_s, 2), "batch_ms": round(batched_s * 1000, 3),
             "serial_ms": round(serial_s * 1000, 3), "shared_scans": rep["shared_scans"],
             "block_threads": tuned["block_threads"], "vectors_per_thread": tuned["vectors_per_thread"],
-            "table_mask": tuned.get("table_mask"), "lds_table_bytes": tuned.get("lds_table_bytes"),
+            "table_mask": tuned.get("table_mask"), "lds_table_bytes": tuned.get("lds_table_bytes"), "table_kb": tuned.get("table_kb"),
             "graph": rep["scans"][0]["graph"], "scan_ms_last": rep["scans"][0]["scan_ms"],
             "device_ms_last": rep["scans"][0]["device_ms"], "host_ms_last": rep["scans"][0]["host_ms"],
             "plan_ms_last": rep["plan_ms"], "union_GBps": round(union_bytes / batched_s / 1e9, 1),

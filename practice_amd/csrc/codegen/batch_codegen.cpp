@@ -43,7 +43,9 @@ std::string ldsRegion(const QueryProgram* p, size_t t, const std::string& ind) {
 
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt,
                                     std::vector<std::string>& union_cols, std::vector<bool>& union_nullable,
-                                    unsigned* table_lds_bytes, unsigned table_mask, int* lds_candidates) {
+                                    unsigned* table_lds_bytes, unsigned table_mask, int* lds_candidates,
+                                    unsigned table_lds_limit) {
+    if (table_lds_limit > kBatchTableLdsMax) throw std::runtime_error("batch table LDS beyond its maximum");
     if (templates.empty()) throw std::runtime_error("batch without templates");
     if (templates.size() > 16) throw std::runtime_error("too many query templates in one batch (max 16)");
     union_cols.clear();
@@ -98,7 +100,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         if (!p->batch_mask) continue;
         uint64_t b = 0;
         for (const auto& tb : p->batch_mask_tables) b += 8 * static_cast<uint64_t>(tb.second);
-        if (mask_bytes + b > kBatchTableLdsBytes) continue;
+        if (mask_bytes + b > table_lds_limit) continue;
         mask_bytes += b;
         use_mask[t] = true;
         for (const auto& tb : p->batch_mask_tables)
@@ -119,7 +121,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
         if (!p->shared_tables) continue;
         for (size_t j = 0; j < p->joins.size(); ++j) {
             const JoinProgram& jp = p->joins[j];
-            if (!jp.shared || jp.db == 0 || jp.s_key.empty() || 4 * jp.ht_words > kBatchTableLdsBytes) continue;
+            if (!jp.shared || jp.db == 0 || jp.s_key.empty() || 4 * jp.ht_words > table_lds_limit) continue;
             size_t k = 0;
             while (k < cands.size() && cands[k].key != jp.s_key) ++k;
             if (k == cands.size()) cands.push_back(LdsTable{jp.s_key, jp.ht_words, {}});
@@ -137,7 +139,7 @@ std::string generateBatchScanSource(const std::vector<const QueryProgram*>& temp
     for (size_t k = 0; k < cands.size(); ++k) {
         auto& c = cands[k];
         if (k < 32 && !((table_mask >> k) & 1u)) continue;  // placed in HBM
-        if (mask_bytes + tab_bytes + 4 * c.words > kBatchTableLdsBytes) continue;
+        if (mask_bytes + tab_bytes + 4 * c.words > table_lds_limit) continue;
         tab_bytes += 4 * c.words;
         for (const auto& u : c.users) tab_of[u] = tabs.size();
         tabs.push_back(std::move(c));

@@ -215,16 +215,21 @@ struct QueryProgram {
 // been generated with CodegenOptions::batch_body.  `union_cols` receives the
 // union slot order and `union_nullable` which of them carry validity.
 // `table_lds_bytes` (optional) receives the static LDS of the shared
-// dimension tables staged into LDS (at most kBatchTableLdsBytes).
+// dimension tables staged into LDS (at most table_lds_limit).
 // Placement: the shared tables that fit LDS are candidates, ranked by the
 // templates that probe them, then by size; candidate k is staged into LDS
 // only when bit k of `table_mask` is set (and it still fits), else it is
 // probed in HBM.  `lds_candidates` (optional) receives the candidate count.
+// `table_lds_limit`: the static LDS the staged tables and query-mask
+// tables may take (default kBatchTableLdsBytes, at most
+// kBatchTableLdsMax); the per-query regions get what the block has left.
 constexpr unsigned kBatchTableLdsBytes = 32 * 1024;
+constexpr unsigned kBatchTableLdsMax = 96 * 1024;
 std::string generateBatchScanSource(const std::vector<const QueryProgram*>& templates, int block_threads,
                                     int vectors_per_thread, std::vector<std::string>& union_cols,
                                     std::vector<bool>& union_nullable, unsigned* table_lds_bytes = nullptr,
-                                    unsigned table_mask = ~0u, int* lds_candidates = nullptr);
+                                    unsigned table_mask = ~0u, int* lds_candidates = nullptr,
+                                    unsigned table_lds_limit = kBatchTableLdsBytes);
 
 // Visitor entry points as in the reference: AggregateNode / SortLimitNode
 // roots start generation (visitor.cpp:1589-1621); projections are rejected.

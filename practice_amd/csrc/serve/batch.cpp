@@ -18,10 +18,11 @@ constexpr int kMaskQueries = 64;                    // queries of a mask-form te
 constexpr int kMaxGraphs = 64;                      // captured batch shapes kept per executor
 // dynamic LDS for the per-query regions: what lds_kb allows, minus the room
 // kept for shared dimension tables staged into LDS
-unsigned ldsLimit(const BatchOptions& opt) {
+unsigned ldsLimitFor(const BatchOptions& opt, int table_kb) {
     // 1 KiB more for the kernel's other static LDS
-    return std::min(static_cast<unsigned>(opt.lds_kb) * 1024u, 160u * 1024u - kBatchTableLdsBytes - 1024u);
+    return std::min(static_cast<unsigned>(opt.lds_kb) * 1024u, 160u * 1024u - static_cast<unsigned>(table_kb) * 1024u - 1024u);
 }
+unsigned ldsLimit(const BatchOptions& opt) { return ldsLimitFor(opt, opt.table_kb); }
 }  // namespace
 
 BatchOptions parseBatchOptions(const std::string& text) {
@@ -45,12 +46,14 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "lds_kb") o.lds_kb = static_cast<int>(v);
         else if (k == "masks") o.masks = v != 0;
         else if (k == "table_mask") o.table_mask = static_cast<unsigned>(v);
+        else if (k == "table_kb") o.table_kb = static_cast<int>(v);
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
         throw std::runtime_error("batch block size must be a multiple of 64 in [64, 1024]");
     if (o.vectors_per_thread < 1 || o.vectors_per_thread > 4) throw std::runtime_error("batch vpt must be in [1, 4]");
     if (o.lds_kb < 16 || o.lds_kb > 160) throw std::runtime_error("batch lds_kb must be in [16, 160]");
+    if (o.table_kb < 16 || o.table_kb > static_cast<int>(kBatchTableLdsMax / 1024)) throw std::runtime_error("batch table_kb must be in [16, 96]");
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
     o.max_queries = std::max(1, o.max_queries);
     return o;
@@ -82,7 +85,8 @@ std::string generateBatchCode(const DatabaseInstance& db, const std::vector<std:
               [](const QueryProgram* a, const QueryProgram* b) { return a->signature < b->signature; });
     std::vector<std::string> cols;
     std::vector<bool> nullable;
-    return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable, nullptr, opt.table_mask);
+    return generateBatchScanSource(templates, opt.block_threads, opt.vectors_per_thread, cols, nullable, nullptr, opt.table_mask, nullptr,
+                                   static_cast<unsigned>(opt.table_kb) * 1024u);
 }
 
 BatchExecutor::BatchExecutor(DatabaseInstance& db)
@@ -255,7 +259,7 @@ std::vector<QueryResult> BatchExecutor::executePrograms(std::vector<QueryProgram
         r << (s ? "," : "") << "{\"queries\":" << rep.queries << ",\"templates\":" << rep.templates
           << ",\"union_columns\":" << rep.union_columns << ",\"block_threads\":" << rep.block_threads
           << ",\"vectors_per_thread\":" << rep.vectors_per_thread << ",\"table_mask\":" << rep.table_mask
-          << ",\"lds_table_bytes\":" << rep.lds_table_bytes << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
+          << ",\"lds_table_bytes\":" << rep.lds_table_bytes << ",\"table_kb\":" << rep.table_kb << ",\"graph_replayed\":" << (rep.graph_replayed ? 1 : 0)
           << ",\"graph\":\"" << rep.graph_mode << "\""
           << ",\"scan_ms\":" << rep.scan_ms << ",\"device_ms\":" << rep.device_ms << ",\"fact_bytes\":" << rep.fact_bytes
           << ",\"host_ms\":{\"modules\":" << rep.modules_ms << ",\"bind\":" << rep.bind_ms << ",\"enqueue\":" << rep.enqueue_ms
@@ -272,8 +276,9 @@ namespace {
 
 std::string batchSource(const std::vector<const QueryProgram*>& templates, int bt, int vpt, std::vector<std::string>& cols,
                         std::vector<bool>& nullable, unsigned table_mask = ~0u, unsigned* table_lds_bytes = nullptr,
-                        int* lds_candidates = nullptr) {
-    return generateBatchScanSource(templates, bt, vpt, cols, nullable, table_lds_bytes, table_mask, lds_candidates);
+                        int* lds_candidates = nullptr, int table_kb = 32) {
+    return generateBatchScanSource(templates, bt, vpt, cols, nullable, table_lds_bytes, table_mask, lds_candidates,
+                                   static_cast<unsigned>(table_kb) * 1024u);
 }
 
 }  // namespace
@@ -519,17 +524,30 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     hipCheck(hipMemcpyAsync(dqp, stage, upload_bytes, hipMemcpyHostToDevice, s), "upload parameter blocks");
 
     // launch shape: tuned per template set when asked
-    Tuned cfg{opt.block_threads, opt.vectors_per_thread, opt.table_mask};
+    Tuned cfg{opt.block_threads, opt.vectors_per_thread, opt.table_mask, opt.table_kb};
     auto tit = tuned_.find(set_key);
     if (tit != tuned_.end()) cfg = tit->second;
     else if (opt.autotune) {
-        cfg = autotune(templates, per_template, runs, dqp, init, ldsLimit(opt), ctx);
+        cfg = autotune(templates, per_template, runs, dqp, init, opt, ctx);
         tuned_[set_key] = cfg;
+    }
+    // a tuned table budget leaves less LDS to the per-query regions: a
+    // launch whose regions no longer fit takes the default budget
+    if (cfg.table_kb != opt.table_kb) {
+        cq_batch_params probe;
+        std::memset(&probe, 0, sizeof probe);
+        try {
+            (void)layoutLds(templates, per_template, ldsLimitFor(opt, cfg.table_kb), probe);
+        } catch (const std::runtime_error&) {
+            cfg.table_kb = opt.table_kb;
+            cfg.table_mask = opt.table_mask;
+        }
     }
     std::vector<std::string> ucols;
     std::vector<bool> unull;
     unsigned table_bytes = 0;
-    const std::string src = batchSource(templates, cfg.block_threads, cfg.vectors_per_thread, ucols, unull, cfg.table_mask, &table_bytes);
+    const std::string src = batchSource(templates, cfg.block_threads, cfg.vectors_per_thread, ucols, unull, cfg.table_mask, &table_bytes,
+                                        nullptr, cfg.table_kb);
     ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
     hipFunction_t fn = bm->function("cq_batch_scan");
 
@@ -554,7 +572,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     B.row_end = first.fact_rows;
     const long long tile = static_cast<long long>(cfg.block_threads) * 4 * cfg.vectors_per_thread;
     B.num_tiles = (B.row_end - B.row_begin + tile - 1) / tile;
-    const unsigned lds_bytes = layoutLds(templates, per_template, ldsLimit(opt), B);
+    const unsigned lds_bytes = layoutLds(templates, per_template, ldsLimitFor(opt, cfg.table_kb), B);
     const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes);
 
     auto enqueue = [&](bool timed) {
@@ -743,6 +761,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     rep.block_threads = cfg.block_threads;
     rep.vectors_per_thread = cfg.vectors_per_thread;
     rep.table_mask = cfg.table_mask;
+    rep.table_kb = cfg.table_kb;
     rep.lds_table_bytes = table_bytes;
     rep.graph_replayed = replayed;
     rep.scan_ms = sms;
@@ -759,7 +778,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
 
 BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgram*>& templates,
                                              const std::vector<std::vector<int>>& per_template, std::vector<QueryRun>& runs,
-                                             const cq_query_params* dqp, const BatchInit& init, unsigned lds_limit,
+                                             const cq_query_params* dqp, const BatchInit& init, const BatchOptions& opt,
                                              ExecutionContext* ctx) {
     QueryEngine& engine = db_.engine();
     hipStream_t s = ctx->stream;
@@ -768,7 +787,8 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
     auto timeVariant = [&](const Tuned& v, int* candidates) -> float {
         std::vector<std::string> ucols;
         std::vector<bool> unull;
-        const std::string src = batchSource(templates, v.block_threads, v.vectors_per_thread, ucols, unull, v.table_mask, nullptr, candidates);
+        const std::string src =
+            batchSource(templates, v.block_threads, v.vectors_per_thread, ucols, unull, v.table_mask, nullptr, candidates, v.table_kb);
         ModuleHandle bm = engine.moduleForSource(src, signatureOf(src), nullptr);
         hipFunction_t fn = bm->function("cq_batch_scan");
         cq_batch_params B;
@@ -789,7 +809,12 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
         const long long tile = static_cast<long long>(v.block_threads) * 4 * v.vectors_per_thread;
         B.num_tiles = (B.row_end + tile - 1) / tile;
         if (B.num_tiles == 0) return -1.0f;
-        const unsigned lds_bytes = layoutLds(templates, per_template, lds_limit, B);
+        unsigned lds_bytes = 0;
+        try {
+            lds_bytes = layoutLds(templates, per_template, ldsLimitFor(opt, v.table_kb), B);
+        } catch (const std::runtime_error&) {
+            return -2.0f;  // the per-query regions do not fit beside the tables
+        }
         const unsigned grid = scanGridShared(fn, v.block_threads, ctx->device, B.num_tiles, lds_bytes);
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {
@@ -806,6 +831,7 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
     };
     // launch shape with every candidate table in LDS
     Tuned best;
+    best.table_kb = opt.table_kb;
     float best_ms = -1;
     int candidates = 0;
     const int bts[] = {64, 128, 256, 512, 1024};
@@ -815,12 +841,33 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
             Tuned v;
             v.block_threads = bt;
             v.vectors_per_thread = vpt;
+            v.table_kb = opt.table_kb;
             const float ms = timeVariant(v, &candidates);
-            if (ms < 0) return best;
+            if (ms == -1.0f) return best;
+            if (ms < 0) continue;
             if (best_ms < 0 || ms < best_ms) {
                 best_ms = ms;
                 best = v;
             }
+        }
+    }
+    // a larger LDS budget for the tables, when it admits more of them (a
+    // table staged in LDS takes its probe off the L2; the per-query
+    // regions get less LDS)
+    for (int tk : {64, 96}) {
+        if (tk <= opt.table_kb) continue;
+        Tuned v = best;
+        v.table_kb = tk;
+        int more = 0;
+        std::vector<std::string> uc;
+        std::vector<bool> un;
+        (void)batchSource(templates, v.block_threads, v.vectors_per_thread, uc, un, v.table_mask, nullptr, &more, tk);
+        if (more <= candidates) continue;
+        const float ms = timeVariant(v, nullptr);
+        if (ms >= 0 && ms < best_ms) {
+            best_ms = ms;
+            best = v;
+            candidates = more;
         }
     }
     // LDS-versus-HBM placement of the first three candidate tables at that

@@ -39,10 +39,13 @@ struct BatchOptions {
     // shared dimension tables staged into LDS: bit k for the k-th candidate
     // (generateBatchScanSource); the autotuner sweeps the first three
     unsigned table_mask = ~0u;
+    // static LDS for staged dimension tables and query-mask tables (KiB,
+    // 16..96); the per-query regions get what the block has left
+    int table_kb = 32;
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
-// max_queries, shared, lds_agg, lds_kb, masks, table_mask)
+// max_queries, shared, lds_agg, lds_kb, masks, table_mask, table_kb)
 BatchOptions parseBatchOptions(const std::string& text);
 
 struct BatchScanReport {
@@ -50,6 +53,7 @@ struct BatchScanReport {
     int block_threads = 0, vectors_per_thread = 0;
     unsigned lds_table_bytes = 0;  // static LDS of the staged shared tables and query-mask tables
     unsigned table_mask = ~0u;     // LDS placement of the candidate tables
+    int table_kb = 32;             // LDS budget of the staged tables
     bool graph_replayed = false;
     const char* graph_mode = "eager";  // eager / captured / replayed
     double scan_ms = 0;            // cq_batch_scan alone (events)
@@ -106,6 +110,7 @@ private:
     struct Tuned {
         int block_threads = 1024, vectors_per_thread = 1;
         unsigned table_mask = ~0u;
+        int table_kb = 32;
     };
     DatabaseInstance& db_;
     ExecutionContext* ctx_;
@@ -136,7 +141,7 @@ private:
     void prepareTables(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
                        std::vector<QueryRun>& runs, const cq_query_params* dqp, const BatchInit& init, ExecutionContext* ctx);
     Tuned autotune(const std::vector<const QueryProgram*>& templates, const std::vector<std::vector<int>>& per_template,
-                   std::vector<QueryRun>& runs, const cq_query_params* dqp, const BatchInit& init, unsigned lds_limit,
+                   std::vector<QueryRun>& runs, const cq_query_params* dqp, const BatchInit& init, const BatchOptions& opt,
                    ExecutionContext* ctx);
     void dropGraph(Graph& g);
 };

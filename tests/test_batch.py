@@ -136,6 +136,29 @@ def test_table_placement_mask(hdb):
         pa.compile_source(src)
 
 
+def test_table_lds_budget(hdb):
+    """table_kb raises the static LDS the staged tables may take: a shared
+    table between 32 and 64 KB is staged only with the larger budget."""
+    from tests.fixtures import build_pair
+    import numpy as np
+    n = 40_000  # 40,000 suppliers x 8-bit entries (region + nation) = 40 KB
+    sup = {"s_suppkey": (np.arange(1, n + 1, dtype=np.int32), None),
+           "s_region": (np.arange(n, dtype=np.int32) % 5, None), "s_nation": (np.arange(n, dtype=np.int32) % 25, None)}
+    rng = np.random.default_rng(3)
+    lo = {"lo_suppkey": (rng.integers(1, n + 1, 5000).astype(np.int32), None),
+          "lo_revenue": (rng.integers(1, 1000, 5000).astype(np.int32), None)}
+    db, _ = build_pair({"LINEORDER": (5000, True, lo), "SUPPLIER": (n, False, sup)}, pks=[("SUPPLIER", "s_suppkey")])
+    qs = ["select s_nation, sum(lo_revenue) from lineorder, supplier where lo_suppkey = s_suppkey and s_region = 1 group by s_nation",
+          "select count(*) from lineorder, supplier where lo_suppkey = s_suppkey and s_nation = 3"]
+    small = db.generate_batch_code(qs)
+    large = db.generate_batch_code(qs, {"table_kb": 64})
+    assert "__shared__ u32 cq_tab" not in small and "__shared__ u32 cq_tab0[" in large
+    pa.compile_source(large)
+    with pytest.raises(pa.CrystalError, match="table_kb"):
+        db.generate_batch_code(qs, {"table_kb": 200})
+    db.close()
+
+
 @pytest.mark.parametrize("opts", [{"bt": 1024, "vpt": 4}, {"bt": 64, "vpt": 1}])
 def test_launch_shapes_compile(hdb, opts):
     pa.compile_source(hdb.generate_batch_code([Q.C3, Q.C1], opts))
@@ -251,10 +274,10 @@ def test_autotuned_batch_matches():
     assert scan["block_threads"] in (64, 128, 256, 512, 1024) and scan["vectors_per_thread"] in (1, 2, 4)
     assert scan["table_mask"] | 7 == 0xFFFFFFFF  # the autotuner places the first three candidates
     _check_batch(gdb, odb, qs, res)
-    # every placement gives the same results
-    for mask in (0, 1, 2):
-        print(f"[batch] table_mask={mask}", flush=True)
-        _check_batch(gdb, odb, qs, gdb.batch_execute(qs, {"table_mask": mask, "graph": 0}))
+    # every placement and table budget gives the same results
+    for opts in ({"table_mask": 0}, {"table_mask": 1}, {"table_mask": 2}, {"table_kb": 64}, {"table_kb": 96}):
+        print(f"[batch] {opts}", flush=True)
+        _check_batch(gdb, odb, qs, gdb.batch_execute(qs, dict(opts, graph=0)))
 
 
 def _key_zero_tables(first_key=1):
