@@ -222,3 +222,34 @@ def test_random_projections(sf1):
     finally:
         srv.stop()
     _against_oracle(gdb, odb, sqls, done, "served projection")
+
+
+def random_orkey(rng):
+    """OR-key joins: a fact row counts once per distinct date key (order /
+    commit) that names a DDATE row."""
+    key = rng.choice(["d_year", "d_month", None])
+    aggs = rng.sample(["count(*)", "sum(lo_revenue)", "max(lo_quantity)", "min(lo_extendedprice)"], rng.randint(1, 2))
+    where = ["(lo_orderdate = d_datekey or lo_commitdate = d_datekey)"]
+    if rng.random() < 0.6:
+        where.append(_pred(rng, "d_year", 1992, 1998, allow_not=False))
+    if rng.random() < 0.5:
+        where.append(_cmp(rng, "lo_quantity", 1, 50))
+    sql = "select " + ", ".join(([key] if key else []) + aggs) + " from lineorder, ddate where " + " and ".join(where)
+    return sql + (f" group by {key}" if key else "")
+
+
+@pytest.mark.gpu
+def test_random_orkey_joins():
+    """Random OR-key joins on nullable commit dates: compiled alone and
+    through the server's cold lanes, against the oracle."""
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1c")
+    rng = random.Random(2030)
+    sqls = [random_orkey(rng) for _ in range(10)]
+    _against_oracle(gdb, odb, sqls, [gdb.execute(s) for s in sqls], "or-key")
+    srv = pa.Server(gdb, {"lanes": 1, "cold_lanes": 1})
+    try:
+        done = [srv.wait(t, 120_000) for t in [srv.submit(s) for s in sqls]]
+    finally:
+        srv.stop()
+    _against_oracle(gdb, odb, sqls, done, "served or-key")

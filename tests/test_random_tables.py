@@ -126,3 +126,91 @@ def test_random_tables_every_path(seed):
         check(db.execute_group(sqls), "interpreted group")
     finally:
         db.close()
+
+
+def random_mht_tables(seed):
+    """CUSTOMER without a primary key and with repeated keys (a multi-value
+    join: every matching dimension row joins)."""
+    rng = np.random.default_rng(seed + 100)
+    r = random.Random(seed + 100)
+    nc = r.randint(20, 400)
+    # distinct keys with distinct slots k % nc (the reference's table has
+    # one directory key per slot; colliding distinct keys are undefined
+    # there), each repeated over several rows
+    kmin = r.choice([0, 1, 3])
+    distinct = rng.choice(np.arange(nc), r.randint(max(2, nc // 8), max(3, nc // 2)), replace=False)
+    distinct = distinct + nc * rng.integers(0, 2, len(distinct))
+    distinct = distinct[distinct >= kmin] if (distinct >= kmin).sum() >= 2 else distinct + kmin * nc
+    span = int(distinct.max()) - kmin + 1
+    cus = {"c_custkey": (rng.choice(distinct, nc).astype(np.int32), None),
+           "c_region": (rng.permutation(np.arange(nc) % 5).astype(np.int32), None),
+           "c_nation": (rng.permutation(np.arange(nc) % 25).astype(np.int32), None)}
+    n = r.randint(2000, 12000)
+    lo = {"lo_custkey": (rng.integers(min(kmin, 0) - 1, kmin + span + 2, n).astype(np.int32), None),
+          "lo_revenue": (rng.integers(1, 100_000, n).astype(np.int32), None),
+          "lo_quantity": (rng.integers(1, 51, n).astype(np.int32), None)}
+    return {"LINEORDER": (n, True, lo), "CUSTOMER": (nc, False, cus)}
+
+
+def random_mht_statements(seed, n=8):
+    r = random.Random(seed + 200)
+    out = []
+    for _ in range(n):
+        where = ["lo_custkey = c_custkey"]
+        if r.random() < 0.5:
+            where.append(f"c_region {r.choice(['=', '<>', '<'])} {r.randint(0, 4)}")
+        if r.random() < 0.5:
+            where.append(f"lo_quantity < {r.randint(1, 50)}")
+        if r.random() < 0.25:
+            where.append(f"lo_quantity = {r.randint(1, 50)}")
+            out.append("select lo_revenue, c_nation from lineorder, customer where " + " and ".join(where))
+            continue
+        key = r.choice(["c_region", "c_nation", None])
+        aggs = r.sample(["count(*)", "sum(lo_revenue)", "max(lo_quantity)", "min(lo_revenue)"], r.randint(1, 3))
+        sql = "select " + ", ".join(([key] if key else []) + aggs) + " from lineorder, customer where " + " and ".join(where)
+        if key:
+            sql += f" group by {key}"
+        out.append(sql)
+    return out
+
+
+MHT_SEEDS = [1, 2, 3]
+
+
+@pytest.mark.parametrize("seed", MHT_SEEDS)
+def test_random_mht_plan(seed):
+    db, _ = build_pair(random_mht_tables(seed), pks=[])
+    for sql in random_mht_statements(seed):
+        d = db.describe(sql)
+        assert d["joins"][0]["layout"] == "multi-value", sql
+    db.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", MHT_SEEDS)
+def test_random_mht_every_path(seed):
+    """Multi-value joins run compiled on every path: alone, with the
+    interpreted tier asked for (they fall back to compiled modules) and
+    through the server (cold lanes)."""
+    db, odb = build_pair(random_mht_tables(seed), pks=[], device=0)
+    try:
+        sqls = random_mht_statements(seed)
+        want = [odb.execute(db.explain(s)) for s in sqls]
+
+        def check(results, path):
+            for s, r, o in zip(sqls, results, want):
+                print(f"[random-mht] seed={seed} {path}: {s}", flush=True)
+                assert_same(r, o)
+
+        check([db.execute(s) for s in sqls], "plain")
+        db.set_option("interpret", 1)
+        check([db.execute(s) for s in sqls], "interpreted option")
+        db.set_option("interpret", 0)
+        srv = pa.Server(db, {"lanes": 1, "cold_lanes": 1})
+        try:
+            done = [srv.wait(t, 120_000) for t in [srv.submit(s) for s in sqls]]
+        finally:
+            srv.stop()
+        check(done, "served")
+    finally:
+        db.close()
