@@ -59,7 +59,9 @@
  *   - a probe or build key below the hash-table minimum (negative slot,
  *     an out-of-bounds access in the reference) never matches / is skipped;
  *   - every column reference reads that column's value (the reference's
- *     register bookkeeping can alias two columns onto `items`).
+ *     register bookkeeping can alias two columns onto `items`: a fast-path
+ *     column stays mapped to a register that a later load reuses,
+ *     src/core/visitor.cpp:1765-1776).
  */
 #include "ssb_oracle.h"
 
