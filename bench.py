@@ -304,7 +304,18 @@ def bench_served(pa, Q, args):
                         f"(600,000,000 fact rows)",
             "server_options": args.serve_options, "budget_mb": round(srv_budget / 2**20, 1),
             "serial_qps": round(serial_qps, 1), "rates": rates, "burst": burst,
-            "saturated_speedup": round(burst["qps"] / serial_qps, 2)}
+            "saturated_speedup": round(burst["qps"] / serial_qps, 2),
+            "within_50ms": within_slo(rates, serial_qps, 50.0)}
+
+
+def within_slo(rates, serial_qps, p99_ms):
+    """The highest achieved rate whose p99 latency stays within `p99_ms`,
+    and its ratio to serial execution (None when no rate does)."""
+    ok = [r for r in rates if r["p99_ms"] <= p99_ms and not r.get("errors")]
+    if not ok:
+        return None
+    best = max(ok, key=lambda r: r["qps"])
+    return {"rate": best["rate"], "qps": best["qps"], "p99_ms": best["p99_ms"], "speedup": round(best["qps"] / serial_qps, 2)}
 
 
 def copy_calibration(nbytes, steps):

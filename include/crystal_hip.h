@@ -227,7 +227,10 @@ void cq_result_free(cq_result* r);
  * keep the fastest), max_templates (<= 16), max_queries (per scan), shared
  * (0/1: one probe per template and dimension), lds_agg (0/1: per-query LDS
  * aggregation), lds_kb (16..160, default 160: LDS per scan block for the
- * per-query aggregation regions). */
+ * per-query aggregation regions), masks (0/1: query-mask form), table_mask
+ * (bit k: stage the k-th candidate shared table in LDS), table_kb (16..96,
+ * default 32: LDS of the staged tables), grid_percent (1..100, default 100:
+ * scan grid as a percentage of the blocks that fit on the device at once). */
 int cq_batch_execute(cq_db* db, const char* const* sqls, int n, const char* options, cq_result** outs);
 /* JSON of the last batch (per shared scan: queries, templates, columns,
  * launch shape, graph replay, scan / device milliseconds, fact bytes). */
@@ -243,15 +246,18 @@ int cq_db_generate_batch_code(cq_db* db, const char* const* sqls, int n, const c
  * request they can admit, up to max_batch, and run them as one shared scan;
  * admission reserves each request's device footprint against the budget;
  * cache misses compile on a background thread.  options: NULL or
- * "key=value,..." with lanes (1..8, default 2), max_batch (64),
+ * "key=value,..." with lanes (1..8, default 1), max_batch (128),
  * max_templates (16), promote_after (2: requests of a new signature before
  * it is compiled into the shared scan), compile_threads (2), budget_mb
  * (default: a quarter of free memory), window_us (0), async_compile (1),
  * interpret (1: new signatures run at once on the interpreted tier),
- * cold_lanes (1: lanes for not-yet-compiled signatures), per_template (1:
- * one shared-scan launch per hot signature in a batch; 0: one module for the
- * whole hot set), max_hot (64), bt, vpt, graph, lds_kb (as for
- * cq_batch_execute). */
+ * cold_lanes (1: lanes for not-yet-compiled signatures), cold_group (1:
+ * interpreted requests a cold lane runs in one shared pass), cold_grid_percent
+ * (100: cold scans' grid), per_template (1: one shared-scan launch per hot
+ * signature in a batch; 0: one module for the whole hot set; 2: one launch
+ * of the hot-set module while fewer than cold_backlog (2) cold requests are
+ * queued, one launch per signature otherwise), max_hot (64),
+ * bt, vpt, graph, lds_kb, grid_percent (as for cq_batch_execute). */
 int cq_server_start(cq_db* db, const char* options, cq_server** out);
 /* Enqueue a request; returns a ticket >= 0 (negative on error). */
 long long cq_server_submit(cq_server* s, const char* sql);

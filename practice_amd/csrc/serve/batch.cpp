@@ -47,6 +47,7 @@ BatchOptions parseBatchOptions(const std::string& text) {
         else if (k == "masks") o.masks = v != 0;
         else if (k == "table_mask") o.table_mask = static_cast<unsigned>(v);
         else if (k == "table_kb") o.table_kb = static_cast<int>(v);
+        else if (k == "grid_percent") o.grid_percent = static_cast<int>(v);
         else throw std::runtime_error("unknown batch option: " + k);
     }
     if (o.block_threads < 64 || o.block_threads > 1024 || o.block_threads % 64 != 0)
@@ -54,6 +55,7 @@ BatchOptions parseBatchOptions(const std::string& text) {
     if (o.vectors_per_thread < 1 || o.vectors_per_thread > 4) throw std::runtime_error("batch vpt must be in [1, 4]");
     if (o.lds_kb < 16 || o.lds_kb > 160) throw std::runtime_error("batch lds_kb must be in [16, 160]");
     if (o.table_kb < 16 || o.table_kb > static_cast<int>(kBatchTableLdsMax / 1024)) throw std::runtime_error("batch table_kb must be in [16, 96]");
+    if (o.grid_percent < 1 || o.grid_percent > 100) throw std::runtime_error("batch grid_percent must be in [1, 100]");
     o.max_templates = std::max(1, std::min(o.max_templates, CQ_MAX_TEMPLATES));
     o.max_queries = std::max(1, o.max_queries);
     return o;
@@ -321,14 +323,16 @@ unsigned layoutLds(const std::vector<const QueryProgram*>& templates, const std:
     return off;
 }
 
-// persistent grid for a launch with dynamic shared memory
-unsigned scanGridShared(hipFunction_t f, int block_threads, int device, long long num_tiles, unsigned lds_bytes) {
+// persistent grid for a launch with dynamic shared memory, `percent` of the
+// blocks that fit on the device at once (at least one)
+unsigned scanGridShared(hipFunction_t f, int block_threads, int device, long long num_tiles, unsigned lds_bytes,
+                        int percent) {
     int per_cu = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, block_threads, lds_bytes) != hipSuccess || per_cu < 1)
         per_cu = 1;
     hipDeviceProp_t prop;
     hipCheck(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    const long long max_blocks = static_cast<long long>(per_cu) * prop.multiProcessorCount;
+    const long long max_blocks = static_cast<long long>(per_cu) * prop.multiProcessorCount * percent / 100;
     return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(num_tiles, max_blocks)));
 }
 
@@ -573,7 +577,7 @@ void BatchExecutor::runScan(const std::vector<int>& members, std::vector<QueryPr
     const long long tile = static_cast<long long>(cfg.block_threads) * 4 * cfg.vectors_per_thread;
     B.num_tiles = (B.row_end - B.row_begin + tile - 1) / tile;
     const unsigned lds_bytes = layoutLds(templates, per_template, ldsLimitFor(opt, cfg.table_kb), B);
-    const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes);
+    const unsigned grid = scanGridShared(fn, cfg.block_threads, ctx->device, B.num_tiles, lds_bytes, opt.grid_percent);
 
     auto enqueue = [&](bool timed) {
         prepareTables(templates, per_template, runs, dqp, init, ctx);
@@ -815,7 +819,7 @@ BatchExecutor::Tuned BatchExecutor::autotune(const std::vector<const QueryProgra
         } catch (const std::runtime_error&) {
             return -2.0f;  // the per-query regions do not fit beside the tables
         }
-        const unsigned grid = scanGridShared(fn, v.block_threads, ctx->device, B.num_tiles, lds_bytes);
+        const unsigned grid = scanGridShared(fn, v.block_threads, ctx->device, B.num_tiles, lds_bytes, opt.grid_percent);
         float ms = 0;
         for (int rep = 0; rep < 2; ++rep) {
             prepareTables(templates, per_template, runs, dqp, init, ctx);

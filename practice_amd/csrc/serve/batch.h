@@ -42,10 +42,13 @@ struct BatchOptions {
     // static LDS for staged dimension tables and query-mask tables (KiB,
     // 16..96); the per-query regions get what the block has left
     int table_kb = 32;
+    // shared-scan grid as a percentage of the persistent grid (blocks that
+    // fit at once on every CU): below 100 leaves CUs to other streams
+    int grid_percent = 100;
 };
 
 // "key=value,key=value" (keys: bt, vpt, graph, autotune, max_templates,
-// max_queries, shared, lds_agg, lds_kb, masks, table_mask, table_kb)
+// max_queries, shared, lds_agg, lds_kb, masks, table_mask, table_kb, grid_percent)
 BatchOptions parseBatchOptions(const std::string& text);
 
 struct BatchScanReport {

@@ -42,9 +42,10 @@ ServerOptions parseServerOptions(const std::string& text) {
         else if (k == "cold_lanes") o.cold_lanes = static_cast<int>(v);
         else if (k == "cold_group") o.cold_group = static_cast<int>(v);
         else if (k == "cold_grid_percent") o.cold_grid_percent = static_cast<int>(v);
-        else if (k == "per_template") o.per_template = v != 0;
+        else if (k == "per_template") o.per_template = static_cast<int>(v);
+        else if (k == "cold_backlog") o.cold_backlog = static_cast<int>(v);
         else if (k == "max_hot") o.max_hot = static_cast<int>(v);
-        else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb") batch_opts += (batch_opts.empty() ? "" : ",") + item;
+        else if (k == "bt" || k == "vpt" || k == "graph" || k == "lds_kb" || k == "grid_percent") batch_opts += (batch_opts.empty() ? "" : ",") + item;
         else throw std::runtime_error("unknown server option: " + k);
     }
     // served batches vary in composition and buffer placement, so captured
@@ -57,6 +58,7 @@ ServerOptions parseServerOptions(const std::string& text) {
     o.promote_after = std::max(1, o.promote_after);
     o.compile_threads = std::max(1, std::min(o.compile_threads, 16));
     if (o.cold_lanes < 1 || o.cold_lanes > 8) throw std::runtime_error("server cold_lanes must be in [1, 8]");
+    if (o.per_template < 0 || o.per_template > 2) throw std::runtime_error("server per_template must be 0, 1 or 2");
     o.max_hot = std::max(1, o.max_hot);
     return o;
 }
@@ -354,8 +356,28 @@ void QueryServer::laneLoop(Lane* lane) {
             std::vector<QueryResult> results;
             std::string err;
             const auto tb = std::chrono::steady_clock::now();
+            // per_template = 2: one launch for the batch when no cold backlog
+            // builds and the compiled hot-set module covers every request
+            std::vector<std::shared_ptr<QueryProgram>> combined_hold;
+            std::vector<const QueryProgram*> combined;
+            if (opt_.per_template == 2) {
+                std::lock_guard<std::mutex> lock(mu_);
+                bool covers = static_cast<int>(cold_q_.size()) < opt_.cold_backlog && !combined_set_.empty();
+                for (size_t i = 0; covers && i < programs.size(); ++i) {
+                    bool have = false;
+                    for (const auto& t : combined_set_) have = have || t->signature == programs[i].signature;
+                    covers = have;
+                }
+                if (covers) {
+                    combined_hold = combined_set_;
+                    for (const auto& t : combined_hold) combined.push_back(t.get());
+                    ++combined_batches_;
+                }
+            }
             try {
-                if (opt_.per_template) {
+                if (!combined.empty()) {
+                    results = lane->exec->executePrograms(programs, opt_.batch, &combined);
+                } else if (opt_.per_template) {
                     // one launch per signature, each on its own (precompiled) module
                     BatchOptions bo = opt_.batch;
                     bo.max_templates = 1;
@@ -550,6 +572,7 @@ bool QueryServer::compileOne() {
             std::lock_guard<std::mutex> lock(mu_);
             promoting_.erase(promote->signature);  // stays on the cold tier
         }
+        if (opt_.per_template == 2) compileCombined();
         cv_.notify_all();
         return true;
     }
@@ -619,12 +642,41 @@ int QueryServer::wait(long long ticket, int timeout_ms, QueryResult& out, std::s
     return 0;
 }
 
+// per_template = 2: the one-launch module of the current hot set (one fact
+// table, at most max_templates signatures); it takes over once compiled,
+// unless the set changed meanwhile (the later promotion compiles its own)
+void QueryServer::compileCombined() {
+    std::vector<std::shared_ptr<QueryProgram>> set;
+    {
+        std::lock_guard<std::mutex> lock(mu_);
+        set = scan_set_;
+    }
+    if (set.empty() || static_cast<int>(set.size()) > opt_.max_templates) return;
+    for (const auto& t : set)
+        if (t->fact_table != set.front()->fact_table) return;
+    try {
+        std::vector<const QueryProgram*> tp;
+        for (const auto& t : set) tp.push_back(t.get());
+        std::vector<std::string> cols;
+        std::vector<bool> nullable;
+        const std::string src = generateBatchScanSource(tp, opt_.batch.block_threads, opt_.batch.vectors_per_thread, cols, nullable);
+        db_.engine().moduleForSource(src, signatureOf(src), nullptr);
+    } catch (const std::exception&) {
+        return;  // batches keep one launch per signature
+    }
+    std::lock_guard<std::mutex> lock(mu_);
+    bool same = set.size() == scan_set_.size();
+    for (size_t i = 0; same && i < set.size(); ++i) same = set[i]->signature == scan_set_[i]->signature;
+    if (same) combined_set_ = std::move(set);
+}
+
 void QueryServer::resetStats() {
     std::lock_guard<std::mutex> lock(mu_);
     latencies_.clear();
     cold_latencies_.clear();
     batch_wall_ms_ = 0;
     cold_groups_ = cold_grouped_ = 0;
+    combined_batches_ = 0;
     batches_ = batched_queries_ = plain_queries_ = interpreted_queries_ = admission_waits_ = compiles_ = promotions_ = rejected_ = 0;
     stats_start_ = std::chrono::steady_clock::now();
 }
@@ -652,6 +704,7 @@ std::string QueryServer::statsJson() {
       << ",\"max_ms\":" << (l.empty() ? 0.0 : l.back()) << ",\"qps\":" << (secs > 0 ? static_cast<double>(l.size()) / secs : 0.0)
       << ",\"cold_completed\":" << lc.size() << ",\"cold_p50_ms\":" << pctOf(lc, 0.50) << ",\"cold_p99_ms\":" << pctOf(lc, 0.99)
       << ",\"cold_groups\":" << cold_groups_ << ",\"cold_grouped\":" << cold_grouped_
+      << ",\"combined_batches\":" << combined_batches_
       << ",\"batch_wall_ms\":" << (batches_ ? batch_wall_ms_ / static_cast<double>(batches_) : 0.0)
       << ",\"batches\":" << batches_ << ",\"mean_batch\":"
       << (batches_ ? static_cast<double>(batched_queries_) / static_cast<double>(batches_) : 0.0)

@@ -62,17 +62,23 @@ struct ServerOptions {
     // cold lanes' interpreted scans on this percentage of a full-device
     // grid, leaving CUs to the shared scans running meanwhile
     int cold_grid_percent = 100;
-    // hot signatures each get their own shared-scan module and launch (a
-    // batch runs one launch per signature present); off: one module holds
-    // the whole hot set (up to max_templates) and a batch is one launch
-    bool per_template = true;
+    // 1: hot signatures each get their own shared-scan module and launch (a
+    // batch runs one launch per signature present); 0: one module holds the
+    // whole hot set (up to max_templates) and a batch is one launch; 2: both
+    // modules, and a batch is one launch while fewer than cold_backlog cold
+    // requests are queued, one launch per signature otherwise (a single
+    // launch holds every CU for its whole scan, so cold scans wait behind
+    // it and a backlog would grow)
+    int per_template = 1;
+    int cold_backlog = 2;
     int max_hot = 64;              // hot signatures kept (least recently promoted dropped)
     BatchOptions batch;
 };
 
 // "key=value,..." with lanes, max_batch, max_templates, promote_after,
 // compile_threads, budget_mb, window_us, async_compile, interpret,
-// cold_lanes, cold_group, cold_grid_percent, per_template, max_hot, bt, vpt, graph, lds_kb
+// cold_lanes, cold_group, cold_grid_percent, per_template, cold_backlog, max_hot, bt, vpt,
+// graph, lds_kb, grid_percent
 ServerOptions parseServerOptions(const std::string& text);
 
 class QueryServer {
@@ -134,6 +140,8 @@ private:
 
     // templates of the compiled shared-scan set (order = kernel order)
     std::vector<std::shared_ptr<QueryProgram>> scan_set_;
+    // per_template = 2: the hot set whose one-launch module is compiled
+    std::vector<std::shared_ptr<QueryProgram>> combined_set_;
 
     std::vector<std::unique_ptr<Lane>> lanes_;
     std::vector<std::unique_ptr<Lane>> cold_lanes_;
@@ -144,6 +152,7 @@ private:
     std::vector<double> cold_latencies_;  // the cold lanes' share of latencies_
     double batch_wall_ms_ = 0;            // shared-scan batches: host wall time, summed
     long long cold_groups_ = 0, cold_grouped_ = 0;  // interpreted requests run together: runs, requests
+    long long combined_batches_ = 0;    // per_template = 2: batches run as one launch
     std::chrono::steady_clock::time_point stats_start_;
     std::string last_error_;            // message of the last failed batch or cold request
     long long batches_ = 0, batched_queries_ = 0, plain_queries_ = 0, interpreted_queries_ = 0, admission_waits_ = 0,
@@ -155,6 +164,7 @@ private:
     void release(const Request& r);
     void compileLoop();
     bool compileOne();  // one queued compilation job; false when there is none
+    void compileCombined();
     void finish(Request& r, int rc, const std::string& err, QueryResult* result);
     bool prepare(Request& r);
     bool inScanSet(const std::string& signature) const;

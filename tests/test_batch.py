@@ -165,7 +165,7 @@ def test_launch_shapes_compile(hdb, opts):
 
 
 @pytest.mark.parametrize("opts,msg", [({"bt": 100}, "multiple of 64"), ({"vpt": 8}, "vpt"), ({"bogus": 1}, "unknown"),
-                                      ({"lds_kb": 200}, "lds_kb")])
+                                      ({"lds_kb": 200}, "lds_kb"), ({"grid_percent": 0}, "grid_percent")])
 def test_bad_options_rejected(hdb, opts, msg):
     with pytest.raises(pa.CrystalError, match=msg):
         hdb.generate_batch_code([Q.C1], opts)

@@ -22,6 +22,14 @@ def test_bad_server_options_rejected_without_gpu():
 
 
 @pytest.mark.gpu
+def test_bad_per_template_mode_rejected():
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    with pytest.raises(pa.CrystalError, match="per_template"):
+        pa.Server(gdb, {"per_template": 3})
+
+
+@pytest.mark.gpu
 def test_mixed_requests_match_alone_and_oracle():
     from tests.test_batch import mixed_queries
     from tests.test_gpu_parity import dataset
@@ -205,4 +213,40 @@ def test_cold_interpreted_requests_run_together():
     assert st["rejected"] == 0 and st["cold_groups"] >= 1 and st["cold_grouped"] >= 2, st
     for q, r in done:
         print(f"[server-cold] sql={q}", flush=True)
+        assert_same(r, odb.execute(gdb.explain(q)))
+
+
+@pytest.mark.gpu
+def test_adaptive_launch_form_matches_oracle():
+    """per_template=2: a batch runs as one launch of the hot-set module while
+    fewer than cold_backlog cold requests are queued, one launch per
+    signature otherwise; waves with and without ad-hoc requests exercise
+    both forms, and every result matches the oracle."""
+    import random
+    import time
+    from tests.test_gpu_parity import dataset
+    spec, data, odb, gdb = dataset("sf1")
+    rng = random.Random(57)
+    srv = pa.Server(gdb, {"lanes": 1, "max_batch": 32, "cold_lanes": 1, "per_template": 2})
+    try:
+        done = []
+        # promote the four templates (each seen twice), then let the
+        # hot-set module compile
+        warm = [Q.template_query(k, rng) for k in (1, 2, 3, 4) for _ in range(2)]
+        done += [(q, srv.wait(t, 120_000)) for q, t in [(q, srv.submit(q)) for q in warm]]
+        for wave in range(12):
+            if wave < 6:
+                time.sleep(0.5)
+            qs = [Q.template_query(rng.randint(1, 4), rng) for _ in range(rng.randint(4, 16))]
+            if wave % 2:
+                qs += [Q.adhoc_query(rng) for _ in range(2)]
+            tickets = [(q, srv.submit(q)) for q in qs]
+            done += [(q, srv.wait(t, 120_000)) for q, t in tickets]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st["rejected"] == 0 and st["combined_batches"] >= 1, st
+    assert st["combined_batches"] < st["batches"], st
+    for q, r in done:
+        print(f"[server-adaptive] sql={q}", flush=True)
         assert_same(r, odb.execute(gdb.explain(q)))
